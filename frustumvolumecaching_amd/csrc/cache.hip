@@ -1,0 +1,715 @@
+// cache.hip -- frustum brick cache: blockwise fill (CacheRenderer::
+// initCacheBlockwise, src/cache_renderer.cu:18-569, 931-1051) and the
+// reprojecting cache renderer (CachePerformanceRenderer::render,
+// src/cache_performance_renderer.cu:22-313, 670-935, 1059-1263).
+//
+// Storage: a sparse brick pool of 8^3-voxel bricks (32-byte voxel records:
+// 8 fp16 latents, f32 density, f32 isset) addressed through an int32
+// indirection table over the frustum's brick grid (-1 = unset).  The
+// reference's dense 1024x1024xD 3D arrays and texture objects are replaced
+// by this table + in-kernel point fetches; trilinear filtering is done in
+// software exactly as its live path does (cache_performance_renderer.cu:678-787).
+//
+// Fill: one wave64 per 8x8 pixel tile (the reference's 64-thread cg tile).
+// Per sub-iteration every alive tile processes its next non-empty 8-step
+// slab.  Brick slots are handed out by a scan over the alive tiles in tile
+// order, which makes the indirection table independent of scheduling (the
+// reference's atomicAdd at cache_renderer.cu:420 is run-dependent).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mlp_mfma.h"
+#include "render_common.h"
+
+namespace fvc {
+
+struct FillRay {
+  float ox, oy, oz, dx, dy, dz, far_, T;
+  int alive, cnt, off, pad;
+};
+struct FillTile {
+  int slab, alive, n_bricks, processed;
+  int visible, slot, overflow, pad;
+};
+
+// ------------------------------------------------------------------ fill
+// initPayloads_cache_init_kernel (cache_renderer.cu:18-81)
+__global__ __launch_bounds__(64) void k_fill_init(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, int bx,
+                                                  FillRay* __restrict__ rays, FillTile* __restrict__ tiles) {
+  const int t = blockIdx.x, lane = threadIdx.x;
+  const int px = (t % bx) * BRICK + (lane & 7), py = (t / bx) * BRICK + (lane >> 3);
+  const fvc_stepsize_info& s = rm.stepsize_info;
+  const Ray r = gen_ray(rm.cam_info, (float)px + 0.5f, (float)py + 0.5f);
+  float a0, a1;
+  const bool hit = ray_aabb(r, aabb_from, aabb_to, a0, a1);
+  FillRay fr;
+  fr.ox = r.o.x; fr.oy = r.o.y; fr.oz = r.o.z;
+  fr.dx = r.d.x; fr.dy = r.d.y; fr.dz = r.d.z;
+  fr.far_ = hit ? fminf_(a1, s.far_) : s.far_;
+  fr.T = 1.0f;
+  fr.alive = hit;
+  fr.cnt = 0; fr.off = 0; fr.pad = 0;
+  rays[(int64_t)t * 64 + lane] = fr;
+  int start = (int)(step_from_t(s, hit ? a0 : s.near_) - s.near_i);
+  start = start > 0 ? start : 0;
+  const int mn = wave_min(start);
+  const int n_alive = wave_sum(hit ? 1 : 0);
+  if (lane == 0) {
+    FillTile ft{};
+    ft.slab = mn / BRICK;
+    ft.alive = n_alive > 0;
+    tiles[t] = ft;
+  }
+}
+
+// alive tiles in tile order (deterministic compaction), single block
+__global__ __launch_bounds__(1024) void k_fill_list(int n_tiles, const FillTile* __restrict__ tiles,
+                                                    int* __restrict__ list, int* __restrict__ count) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int per = (n_tiles + 1023) / 1024;
+  const int b = tid * per, e = min(n_tiles, b + per);
+  int c = 0;
+  for (int i = b; i < e; i++) c += tiles[i].alive != 0;
+  part[tid] = c;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int v = tid >= d ? part[tid - d] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int o = part[tid] - c;
+  for (int i = b; i < e; i++)
+    if (tiles[i].alive) list[o++] = i;
+  if (tid == 1023) *count = part[1023];
+}
+
+// sample_cache_init_first_kernel + sample_cache_init_second_kernel
+// (cache_renderer.cu:83-287): skip empty slabs, emit the occupied samples of
+// the first non-empty one as a compacted stream.
+__global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list, fvc_stepsize_info st, OccParams occ,
+                                                    ContractParams con, const uint8_t* __restrict__ bf, int bz,
+                                                    FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
+                                                    float* __restrict__ f_pos, float* __restrict__ f_t0,
+                                                    int* __restrict__ f_meta, int* __restrict__ counter) {
+  const int t = list[blockIdx.x], lane = threadIdx.x;
+  FillRay r = rays[(int64_t)t * 64 + lane];
+  FillTile tile = tiles[t];
+  const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
+  int slab = tile.slab;
+  unsigned mask = 0;
+  int total = 0;
+  while (true) {
+    if (wave_sum(r.alive) <= 0) break;
+    if (slab >= bz) {  // beyond the frustum: the column ends
+      r.alive = 0;
+      break;
+    }
+    float t1 = t_from_step(st, (float)(slab * BRICK) + st.near_i);
+    mask = 0;
+    for (int j = 0; j < BRICK; j++) {
+      const float t0 = t1;
+      const float dt = step_dt(st, t0);
+      t1 = t0 + dt;
+      const float tm = (t0 + t1) * 0.5f;
+      if (tm > r.far_) { r.alive = 0; continue; }
+      if (occupied(o + tm * d, occ, bf)) mask |= 1u << j;
+    }
+    total = wave_sum(__popc(mask));
+    if (total > 0) break;
+    slab++;
+  }
+  const bool tile_alive = wave_sum(r.alive) > 0;
+  const bool processed = total > 0 && tile_alive;
+  r.cnt = 0;
+  if (processed) {
+    const int n = __popc(mask);
+    const int off = wave_reserve(n, counter);
+    float t1 = t_from_step(st, (float)(slab * BRICK) + st.near_i);
+    int k = 0;
+    for (int j = 0; j < BRICK; j++) {
+      const float t0 = t1;
+      const float dt = step_dt(st, t0);
+      t1 = t0 + dt;
+      if (mask & (1u << j)) {
+        const float tm = (t0 + t1) * 0.5f;
+        const V3 p = contract(o + tm * d, con);
+        const int64_t q = (int64_t)off + k;
+        f_pos[3 * q] = p.x; f_pos[3 * q + 1] = p.y; f_pos[3 * q + 2] = p.z;
+        f_t0[q] = t0;
+        f_meta[q] = j;
+        k++;
+      }
+    }
+    r.cnt = n;
+    r.off = off;
+  }
+  rays[(int64_t)t * 64 + lane] = r;
+  if (lane == 0) {
+    tile.slab = slab;
+    tile.alive = tile_alive;
+    tile.processed = processed;
+    tile.visible = 0;
+    tiles[t] = tile;
+  }
+}
+
+// first loop of accumulateNew_cache_init_second_kernel (cache_renderer.cu:358-405):
+// transmittance over the slab's samples and the count of visible samples.
+__global__ __launch_bounds__(64) void k_fill_accum(const int* __restrict__ list, fvc_stepsize_info st, float alpha_thre,
+                                                   const uint2* __restrict__ out16, const float* __restrict__ f_t0,
+                                                   FillRay* __restrict__ rays, FillTile* __restrict__ tiles) {
+  const int t = list[blockIdx.x], lane = threadIdx.x;
+  if (!tiles[t].processed) return;
+  FillRay r = rays[(int64_t)t * 64 + lane];
+  int vis = 0;
+  for (int k = 0; k < r.cnt; k++) {
+    const int64_t q = (int64_t)r.off + k;
+    const float ls = h2f(out16[q * 4].x & 0xffffu);
+    const float t0 = f_t0[q];
+    const float dt = step_dt(st, t0);
+    const float density = expf(ls);
+    float alpha = 1.0f - expf(-density * dt);
+    alpha = fminf_(fmaxf_(alpha, 0.0f), 1.0f);
+    if (alpha > alpha_thre) {
+      r.T = r.T * (1.0f - alpha);
+      vis++;
+    }
+    if (r.T < 1e-4f) r.alive = 0;
+  }
+  rays[(int64_t)t * 64 + lane].T = r.T;
+  rays[(int64_t)t * 64 + lane].alive = r.alive;
+  const int v = wave_sum(vis);
+  if (lane == 0) tiles[t].visible = v;
+}
+
+// brick slot allocation in tile order; single block
+__global__ __launch_bounds__(1024) void k_fill_alloc(const int* __restrict__ list, const int* __restrict__ count,
+                                                     int capacity, FillTile* __restrict__ tiles,
+                                                     int* __restrict__ n_used, int* __restrict__ overflow_flag) {
+  __shared__ int part[1024];
+  const int n = *count;
+  const int tid = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = tid * per, e = min(n, b + per);
+  int c = 0;
+  for (int i = b; i < e; i++) {
+    const FillTile& ft = tiles[list[i]];
+    c += (ft.processed && ft.visible > 0);
+  }
+  part[tid] = c;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int v = tid >= d ? part[tid - d] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  const int base = *n_used;
+  int o = base + part[tid] - c;
+  for (int i = b; i < e; i++) {
+    FillTile& ft = tiles[list[i]];
+    if (ft.processed && ft.visible > 0) {
+      ft.slot = o;
+      ft.overflow = o >= capacity;
+      if (ft.overflow) *overflow_flag = 1;
+      o++;
+    } else {
+      ft.overflow = 0;
+    }
+  }
+  __syncthreads();
+  if (tid == 1023) *n_used = base + part[1023];
+}
+
+// brick write + slab advance (accumulateNew_cache_init_second_kernel,
+// cache_renderer.cu:405-528).  Each lane owns one (x,y) column of the brick
+// and writes its 8 voxels; voxels without a sample are zero.
+__global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list, int bx, int by, int latent_offset,
+                                                   const uint16_t* __restrict__ out16, const int* __restrict__ f_meta,
+                                                   const FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
+                                                   int* __restrict__ table, Voxel* __restrict__ pool) {
+  const int t = list[blockIdx.x], lane = threadIdx.x;
+  FillTile tile = tiles[t];
+  if (!tile.processed) return;
+  const FillRay r = rays[(int64_t)t * 64 + lane];
+  bool overflow = false;
+  if (tile.visible > 0) {
+    overflow = tile.overflow != 0;
+    if (lane == 0) table[((int64_t)tile.slab * by + t / bx) * bx + t % bx] = overflow ? -1 : tile.slot;
+    if (!overflow) {
+      Voxel* brick = pool + (int64_t)tile.slot * BRICK_VOXELS;
+      int k = 0;
+      for (int z = 0; z < BRICK; z++) {
+        Voxel v;
+        const bool has = k < r.cnt && f_meta[r.off + k] == z;
+        if (has) {
+          const int64_t q = (int64_t)r.off + k;
+          const uint16_t* o16 = out16 + q * 16;
+          for (int i = 0; i < 8; i++) v.lat[i] = o16[latent_offset + i];
+          v.density = expf(h2f(o16[0]));
+          v.isset = 1.0f;
+          k++;
+        } else {
+          for (int i = 0; i < 8; i++) v.lat[i] = 0;
+          v.density = 0.0f;
+          v.isset = 0.0f;
+        }
+        v.pad[0] = v.pad[1] = 0.0f;
+        brick[z * 64 + lane] = v;
+      }
+      tile.n_bricks++;
+    }
+  }
+  const int n_alive = wave_sum(r.alive);
+  if (lane == 0) {
+    if (!overflow) tile.slab++;
+    tile.alive = !overflow && n_alive > 0;
+    tiles[t] = tile;
+  }
+}
+
+// writeImageBuffer_cache_init_kernel2 (cache_renderer.cu:530-569): known range
+__global__ __launch_bounds__(64) void k_fill_known(int W, int H, int bx, fvc_stepsize_info st,
+                                                   const FillTile* __restrict__ tiles, float* __restrict__ known) {
+  const int t = blockIdx.x, lane = threadIdx.x;
+  const int px = (t % bx) * BRICK + (lane & 7), py = (t / bx) * BRICK + (lane >> 3);
+  if (px < W && py < H) known[(int64_t)py * W + px] = t_from_step(st, (float)(tiles[t].slab * BRICK) + st.near_i);
+}
+
+// ------------------------------------------------------------------ geometry
+// Frustum depth in steps: the reference's max_step_in_scene
+// (raymarch_common.h:298-303, cache_renderer.cu:841-843) padded to 4,
+// extended to the farthest AABB corner seen from the fill camera (plus one
+// slab) so a camera outside the AABB keeps every brick inside the frustum.
+static int frustum_depth(const fvc_raymarch_info& rm, const fvc_scene_info& sc) {
+  const fvc_stepsize_info& st = rm.stepsize_info;
+  const V3 from = cvt(sc.aabb_from), to = cvt(sc.aabb_to);
+  const V3 dg = to - from;
+  const float diag = std::sqrt(dot3(dg, dg));
+  int steps = (int)std::ceil(step_from_t(st, diag + st.near_));
+  const V3 o = translation(rm.cam_info.cam2world);
+  float tf = 0.0f;
+  for (int c = 0; c < 8; c++) {
+    const V3 corner = {(c & 1) ? to.x : from.x, (c & 2) ? to.y : from.y, (c & 4) ? to.z : from.z};
+    const V3 v = corner - o;
+    tf = std::max(tf, std::sqrt(dot3(v, v)));
+  }
+  tf = std::min(tf, st.far_);
+  const int steps_cam = (int)std::ceil(step_from_t(st, tf)) + BRICK;
+  steps = std::max(steps, steps_cam);
+  return (steps + 3) / 4 * 4;
+}
+
+// CacheRenderer::resizeCache estimate (cache_renderer.cu:832-846 + cache.cu:143-146)
+static int default_capacity(int n_rays) {
+  const int est_samples = n_rays * 32;
+  const int per_brick = (int)(0.5f * BRICK * BRICK * BRICK);
+  const int est_bricks = est_samples / per_brick;
+  const int per_layer = 128 * 128;
+  return (est_bricks + per_layer - 1) / per_layer * per_layer;
+}
+
+void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats) {
+  const Model& m = *r.model;
+  CacheStore& C = r.caches[r.fill_idx];
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+  hipStream_t s = r.stream();
+  if (!r.counters.p || r.width != W || r.height != H) resize_renderbuffers(r, W, H);
+
+  // resize (Cache::resize, cache.cu:44-84), sparse pool sized by the estimate
+  C.rm = rm;
+  C.fx = W; C.fy = H; C.fz = frustum_depth(rm, m.scene);
+  C.bx = (W + BRICK - 1) / BRICK; C.by = (H + BRICK - 1) / BRICK; C.bz = (C.fz + BRICK - 1) / BRICK;
+  C.capacity = r.capacity_override > 0 ? r.capacity_override : default_capacity(W * H);
+  const int64_t n_table = (int64_t)C.bx * C.by * C.bz;
+  C.table.ensure(n_table * sizeof(int32_t));
+  C.known.ensure((int64_t)W * H * sizeof(float));
+  C.pool.ensure(std::max<int64_t>(1, (int64_t)C.capacity) * BRICK_VOXELS * sizeof(Voxel));
+  FVC_HIP(hipMemsetAsync(C.table.p, 0xff, n_table * sizeof(int32_t), s));
+
+  const int n_tiles = C.bx * C.by;
+  const int64_t f_cap = (int64_t)n_tiles * 512;
+  r.f_rays.ensure((int64_t)n_tiles * 64 * sizeof(FillRay));
+  r.f_tiles.ensure((int64_t)n_tiles * sizeof(FillTile));
+  r.f_list.ensure((int64_t)n_tiles * sizeof(int));
+  r.f_pos.ensure(f_cap * 3 * sizeof(float));
+  r.f_t0.ensure(f_cap * sizeof(float));
+  r.f_meta.ensure(f_cap * sizeof(int));
+  r.f_out16.ensure(f_cap * 16 * sizeof(uint16_t));
+
+  zero_counters(r, s);
+  hipLaunchKernelGGL(k_fill_init, dim3(n_tiles), dim3(64), 0, s, rm, cvt(m.scene.aabb_from), cvt(m.scene.aabb_to),
+                     C.bx, r.f_rays.as<FillRay>(), r.f_tiles.as<FillTile>());
+  FVC_HIP(hipGetLastError());
+  int* cnt = r.counters.as<int>();
+  int subiter = 0;
+  int64_t n_eval = 0;
+  for (; subiter < 1000; subiter++) {  // CacheRenderer::MAX_SUBITER
+    hipLaunchKernelGGL(k_fill_list, dim3(1), dim3(1024), 0, s, n_tiles, r.f_tiles.as<FillTile>(), r.f_list.as<int>(),
+                       cnt + CNT_TILES);
+    FVC_HIP(hipGetLastError());
+    const int n_alive = read_counter(r, CNT_TILES, s);
+    if (n_alive == 0) break;
+    FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_fill_sample, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), rm.stepsize_info, m.occ,
+                       m.con, m.bitfield.as<uint8_t>(), C.bz, r.f_rays.as<FillRay>(), r.f_tiles.as<FillTile>(),
+                       r.f_pos.as<float>(), r.f_t0.as<float>(), r.f_meta.as<int>(), cnt + CNT_SAMPLES);
+    FVC_HIP(hipGetLastError());
+    const int n_samples = read_counter(r, CNT_SAMPLES, s);
+    if (n_samples <= 0) break;
+    n_eval += n_samples;
+    launch_network_latents(m, r.f_pos.as<float>(), n_samples, r.f_out16.as<uint16_t>(), s);
+    hipLaunchKernelGGL(k_fill_accum, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), rm.stepsize_info,
+                       m.scene.alpha_thre, r.f_out16.as<uint2>(), r.f_t0.as<float>(), r.f_rays.as<FillRay>(),
+                       r.f_tiles.as<FillTile>());
+    FVC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fill_alloc, dim3(1), dim3(1024), 0, s, r.f_list.as<int>(), cnt + CNT_TILES, C.capacity,
+                       r.f_tiles.as<FillTile>(), cnt + CNT_SLOTS, cnt + CNT_OVERFLOW);
+    FVC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fill_write, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), C.bx, C.by,
+                       m.net.latent_offset, r.f_out16.as<uint16_t>(), r.f_meta.as<int>(), r.f_rays.as<FillRay>(),
+                       r.f_tiles.as<FillTile>(), C.table.as<int>(), C.pool.as<Voxel>());
+    FVC_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_fill_known, dim3(n_tiles), dim3(64), 0, s, W, H, C.bx, rm.stepsize_info,
+                     r.f_tiles.as<FillTile>(), C.known.as<float>());
+  FVC_HIP(hipGetLastError());
+  const int used = read_counter(r, CNT_SLOTS, s);
+  const int overflow = read_counter(r, CNT_OVERFLOW, s);
+  C.n_used = std::min(used, C.capacity);
+  C.initialized = true;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->n_bricks_set = C.n_used;
+    stats->n_bricks_reserved = C.capacity;
+    stats->overflow = overflow;
+    stats->n_subiters = subiter;
+    stats->n_samples_evaluated = n_eval;
+    stats->samples_ppx = (float)n_eval / (float)((int64_t)W * H);
+  }
+}
+
+// ------------------------------------------------------------------ render
+struct CacheView {
+  fvc_camera_info cam;
+  fvc_stepsize_info st;
+  int fx, fy, fz, bx, by;
+  const int* table;
+  const float* known;
+  const Voxel* pool;
+};
+
+// sample_cache_performance_kernel (cache_performance_renderer.cu:22-177):
+// march, reproject every occupied sample into the cache frustum and
+// classify it as resample (outside / beyond the known range), cache hit
+// (set brick, nearest-voxel density > 0) or skip.
+template <int MAXS>
+__global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, int step_cutoff, fvc_stepsize_info st,
+                                                     OccParams occ, ContractParams con, const uint8_t* __restrict__ bf,
+                                                     CacheView cv, RayState* __restrict__ rays, int4* __restrict__ entry,
+                                                     int* __restrict__ alive_flag, float* __restrict__ s_pos,
+                                                     float* __restrict__ s_dir, float* __restrict__ s_t0,
+                                                     float* __restrict__ c_frox, float* __restrict__ c_dir,
+                                                     float* __restrict__ c_t0, int* __restrict__ cnt_r,
+                                                     int* __restrict__ cnt_c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n_alive;
+  float tr[MAXS], tc[MAXS];
+  int nr = 0, nc = 0;
+  bool alive = false;
+  RayState r{};
+  V3 oc = {0, 0, 0}, dc = {0, 0, 0};
+  if (valid) {
+    r = rays[i];
+    alive = true;
+    const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
+    oc = transform(cv.cam.world2cam, o.x, o.y, o.z, 1.0f);
+    dc = transform(cv.cam.world2cam, d.x, d.y, d.z, 0.0f);
+    float t1 = r.t;
+    int taken = 0;
+    while (nr < n_steps && nc < n_steps && taken < step_cutoff) {
+      taken++;
+      const float t0 = t1;
+      const float dt = step_dt(st, t0);
+      t1 = t0 + dt;
+      const float tm = (t0 + t1) * 0.5f;
+      if (tm > r.far_) { alive = false; break; }
+      if (!occupied(o + tm * d, occ, bf)) continue;
+      float tf;
+      const V3 f = cam2froxel(oc + tm * dc, cv.cam, cv.st, tf);
+      const bool in_fov = f.x >= 0.0f && f.x < (float)cv.fx && f.y >= 0.0f && f.y < (float)cv.fy;
+      const bool inside = in_fov && f.z >= 0.0f && f.z < (float)cv.fz;
+      bool unsampled = true;
+      if (in_fov) {
+        const float known = cv.known[(int64_t)(int)f.y * cv.fx + (int)f.x];
+        unsampled = tf < cv.st.near_ || tf > known;
+      }
+      if (!inside || unsampled) {
+#pragma unroll
+        for (int k = 0; k < MAXS; k++)
+          if (k == nr) tr[k] = t0;
+        nr++;
+      } else {
+        const int vx = (int)floorf(f.x), vy = (int)floorf(f.y), vz = (int)floorf(f.z);
+        const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
+        if (slot < 0) continue;
+        const float dens = cv.pool[(int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))].density;
+        if (!(dens > 0.0f)) continue;
+#pragma unroll
+        for (int k = 0; k < MAXS; k++)
+          if (k == nc) tc[k] = t0;
+        nc++;
+      }
+    }
+    r.t = t1;
+  }
+  const int off_r = wave_reserve(nr, cnt_r);
+  const int off_c = wave_reserve(nc, cnt_c);
+  if (valid) {
+    const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
+    const V3 d01 = unit_to_01(d);
+    // generateNetworkInput_resample_perf_kernel (198-240): t_mid = t0 + dt/2
+#pragma unroll
+    for (int k = 0; k < MAXS; k++) {
+      if (k < nr) {
+        const float t0 = tr[k];
+        const float tm2 = t0 + step_dt(st, t0) * 0.5f;
+        const V3 p = contract(o + tm2 * d, con);
+        const int64_t q = (int64_t)off_r + k;
+        s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
+        s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
+        s_t0[q] = t0;
+      }
+    }
+    // generateNetworkInput_preparation_perf_kernel (242-313)
+#pragma unroll
+    for (int k = 0; k < MAXS; k++) {
+      if (k < nc) {
+        const float t0 = tc[k];
+        const float tm2 = t0 + step_dt(st, t0) * 0.5f;
+        float tf;
+        const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
+        const int64_t q = (int64_t)off_c + k;
+        c_frox[3 * q] = f.x; c_frox[3 * q + 1] = f.y; c_frox[3 * q + 2] = f.z;
+        c_dir[3 * q] = d01.x; c_dir[3 * q + 1] = d01.y; c_dir[3 * q + 2] = d01.z;
+        c_t0[q] = t0;
+      }
+    }
+    rays[i].t = r.t;
+    entry[i] = make_int4(off_r, nr, off_c, nc);
+    alive_flag[i] = alive;
+  }
+}
+
+// Trilinear lookup (generateNetworkInput_cache_shfl_perf_kernel_cg,
+// cache_performance_renderer.cu:678-787) fused with the colour MLP.  Lane
+// (sample l%16, group g) fetches corners 2g and 2g+1 (corner bits x=4,y=2,z=1);
+// the two xor-shuffles over the 4 lanes of a sample reproduce the reference's
+// ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) summation order.
+__global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_frox, const float* __restrict__ c_dir,
+                                                    int64_t n, CacheView cv, const h8* __restrict__ frags,
+                                                    uint2* __restrict__ c_out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  HeadWeights hw;
+  load_head(frags, lane, hw);
+  const int64_t n_groups = (n + 15) / 16;
+  for (int64_t grp = (int64_t)blockIdx.x * 4 + wave; grp < n_groups; grp += (int64_t)gridDim.x * 4) {
+    const int64_t s = grp * 16 + col;
+    const bool valid = s < n;
+    const int64_t sl = valid ? s : n - 1;
+    const float fcx = c_frox[3 * sl] - 0.5f, fcy = c_frox[3 * sl + 1] - 0.5f, fcz = c_frox[3 * sl + 2] - 0.5f;
+    const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
+    const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
+    float wd = 0.0f, dd = 0.0f, ld[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) ld[i] = 0.0f;
+#pragma unroll
+    for (int cc = 0; cc < 2; cc++) {
+      const int c = 2 * g + cc;
+      const float ox = (float)((c >> 2) & 1), oy = (float)((c >> 1) & 1), oz = (float)(c & 1);
+      const float px = ix + ox, py = iy + oy, pz = iz + oz;
+      const float wx = (1.0f - ox) * (1.0f - 2.0f * lx) + lx;
+      const float wy = (1.0f - oy) * (1.0f - 2.0f * ly) + ly;
+      const float wz = (1.0f - oz) * (1.0f - 2.0f * lz) + lz;
+      const float w = wx * wy * wz;
+      float cw = 0.0f, cd = 0.0f, cl[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) cl[i] = 0.0f;
+      const bool inside = px >= 0.0f && px < (float)cv.fx && py >= 0.0f && py < (float)cv.fy && pz >= 0.0f &&
+                          pz < (float)cv.fz;
+      if (inside) {
+        const int vx = (int)px, vy = (int)py, vz = (int)pz;
+        const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
+        if (slot >= 0) {
+          const Voxel v = cv.pool[(int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))];
+          cw = w * v.isset;
+          cd = w * v.density;
+#pragma unroll
+          for (int i = 0; i < 8; i++) cl[i] = w * h2f(v.lat[i]);
+        }
+      }
+      if (cc == 0) {
+        wd = cw; dd = cd;
+#pragma unroll
+        for (int i = 0; i < 8; i++) ld[i] = cl[i];
+      } else {
+        wd = wd + cw; dd = dd + cd;
+#pragma unroll
+        for (int i = 0; i < 8; i++) ld[i] = ld[i] + cl[i];
+      }
+    }
+    wd = wd + __shfl_xor(wd, 16, 64);
+    wd = wd + __shfl_xor(wd, 32, 64);
+    dd = dd + __shfl_xor(dd, 16, 64);
+    dd = dd + __shfl_xor(dd, 32, 64);
+    _Float16 lat[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      float v = ld[i] + __shfl_xor(ld[i], 16, 64);
+      v = v + __shfl_xor(v, 32, 64);
+      lat[i] = (_Float16)(v / wd);
+    }
+    const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
+    _Float16 rows[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int rr = 4 * g + e;
+      _Float16 v = (_Float16)1.0f;
+#pragma unroll
+      for (int i = 0; i < 8; i++)
+        if (rr == i + 1) v = lat[i];
+      rows[e] = v;
+    }
+    const h8 x = head_input(rows, g, c_dir[3 * sl], c_dir[3 * sl + 1], c_dir[3 * sl + 2]);
+    const f4 o = head_mlp(hw, x);
+    if (g == 0 && valid) {
+      uint2 v;
+      v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
+            ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[1]) << 16);
+      v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[2]) | ((uint32_t)__builtin_bit_cast(uint16_t, logsig) << 16);
+      c_out[s] = v;
+    }
+  }
+}
+
+// accumulate_cache_perf_kernel (cache_performance_renderer.cu:789-908): merge
+// the resample and cache streams of a ray by t0 (resample first iff t0_r <
+// t0_c) and composite; then compact the alive rays.
+__global__ __launch_bounds__(256) void k_cache_composite(
+    int n_alive, fvc_stepsize_info st, float alpha_thre, const RayState* __restrict__ rays_in,
+    const int* __restrict__ pix_in, const float4* __restrict__ res_in, const int4* __restrict__ entry,
+    const int* __restrict__ alive_flag, const uint2* __restrict__ s_out, const float* __restrict__ s_t0,
+    const uint2* __restrict__ c_out, const float* __restrict__ c_t0, RayState* __restrict__ rays_out,
+    int* __restrict__ pix_out, float4* __restrict__ res_out, float4* __restrict__ results, int* __restrict__ counter) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n_alive;
+  bool keep = false;
+  float4 res = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  if (valid) {
+    res = res_in[i];
+    const int4 e = entry[i];
+    bool alive = alive_flag[i] != 0;
+    int kr = 0, kc = 0;
+    while (kr < e.y || kc < e.w) {
+      const float t0r = kr < e.y ? s_t0[e.x + kr] : 1e10f;
+      const float t0c = kc < e.w ? c_t0[e.z + kc] : 1e10f;
+      uint2 v;
+      float t0;
+      if (kc >= e.w || t0r < t0c) { v = s_out[e.x + kr]; t0 = t0r; kr++; }
+      else { v = c_out[e.z + kc]; t0 = t0c; kc++; }
+      if (!composite_sample(res, v, t0, st, alpha_thre)) { alive = false; break; }
+    }
+    keep = alive;
+    if (!keep) results[pix_in[i]] = res;
+  }
+  const int slot = wave_reserve(keep ? 1 : 0, counter);
+  if (keep) {
+    rays_out[slot] = rays_in[i];
+    pix_out[slot] = pix_in[i];
+    res_out[slot] = res;
+  }
+}
+
+void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
+  const Model& m = *r.model;
+  const CacheStore& C = r.caches[r.render_idx];
+  if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+  if (W != r.width || H != r.height) resize_renderbuffers(r, W, H);
+  const int64_t cap = r.sample_capacity;
+  r.c_frox.ensure(cap * 3 * sizeof(float));
+  r.c_dir.ensure(cap * 3 * sizeof(float));
+  r.c_t0.ensure(cap * sizeof(float));
+  r.c_out.ensure(cap * sizeof(uint2));
+  hipStream_t s = r.stream();
+  CacheView cv;
+  cv.cam = C.rm.cam_info;
+  cv.st = C.rm.stepsize_info;
+  cv.fx = C.fx; cv.fy = C.fy; cv.fz = C.fz; cv.bx = C.bx; cv.by = C.by;
+  cv.table = C.table.as<int>();
+  cv.known = C.known.as<float>();
+  cv.pool = C.pool.as<Voxel>();
+
+  int n_alive = launch_init_rays(r, rm, s);
+  int* cnt = r.counters.as<int>();
+  int cur = 0, subiter = 0;
+  int64_t n_res = 0, n_hit = 0;
+  const int64_t target = std::min<int64_t>(1 << 22, cap - 64);
+  while (n_alive > 0 && subiter < 1000) {
+    const int n_steps = steps_for(n_alive, target, 16);
+    FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 2 * sizeof(int), s));
+    const dim3 grid((n_alive + 255) / 256);
+    const int cutoff = 4096;
+#define FVC_CACHE_MARCH(MS)                                                                                        \
+  hipLaunchKernelGGL(k_cache_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, cutoff, rm.stepsize_info, m.occ, \
+                     m.con, m.bitfield.as<uint8_t>(), cv, r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),     \
+                     r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),          \
+                     r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(), cnt + CNT_SAMPLES, cnt + CNT_CACHE)
+    if (n_steps <= 8) FVC_CACHE_MARCH(8);
+    else FVC_CACHE_MARCH(16);
+#undef FVC_CACHE_MARCH
+    FVC_HIP(hipGetLastError());
+    FVC_HIP(hipMemcpyAsync(r.pinned + CNT_SAMPLES, cnt + CNT_SAMPLES, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    FVC_HIP(hipStreamSynchronize(s));
+    const int nr = r.pinned[CNT_SAMPLES], nc = r.pinned[CNT_CACHE];
+    n_res += nr;
+    n_hit += nc;
+    launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), nr, r.s_out.as<uint32_t>(), s);
+    if (nc > 0) {
+      const int64_t groups = ((int64_t)nc + 15) / 16;
+      const int blocks = (int)std::min<int64_t>((groups + 3) / 4, 256 * 16);
+      hipLaunchKernelGGL(k_cache_head, dim3(blocks), dim3(256), 0, s, r.c_frox.as<float>(), r.c_dir.as<float>(),
+                         (int64_t)nc, cv, m.frags.as<h8>(), r.c_out.as<uint2>());
+      FVC_HIP(hipGetLastError());
+    }
+    FVC_HIP(hipMemsetAsync(cnt + CNT_ALIVE, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_cache_composite, grid, dim3(256), 0, s, n_alive, rm.stepsize_info, m.scene.alpha_thre,
+                       r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
+                       r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.s_out.as<uint2>(), r.s_t0.as<float>(),
+                       r.c_out.as<uint2>(), r.c_t0.as<float>(), r.rays[cur ^ 1].as<RayState>(),
+                       r.ray_pix[cur ^ 1].as<int>(), r.ray_res[cur ^ 1].as<float4>(), r.results.as<float4>(),
+                       cnt + CNT_ALIVE);
+    FVC_HIP(hipGetLastError());
+    n_alive = read_counter(r, CNT_ALIVE, s);
+    cur ^= 1;
+    subiter++;
+  }
+  launch_write_image(r, rm, rgba, s);
+  FVC_HIP(hipStreamSynchronize(s));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    const float np = (float)((int64_t)W * H);
+    stats->samples_ppx = (float)(n_res + n_hit) / np;
+    stats->resample_samples_evaluated = (float)n_res / np;
+    stats->cache_samples_evaluated = (float)n_hit / np;
+    stats->n_resamples = (uint64_t)n_res;
+    stats->n_cache_hits = (uint64_t)n_hit;
+    stats->n_subiters = subiter;
+  }
+}
+
+}  // namespace fvc

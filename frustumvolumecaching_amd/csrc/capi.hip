@@ -1,0 +1,336 @@
+// capi.hip -- extern "C" surface of libfvc.so (include/fvc.h).  Every entry
+// point catches the C++ exceptions of the implementation and turns them into
+// a status code + thread-local message; nothing throws across the ABI.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "fvc_internal.h"
+
+namespace fvc {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+void pack_fragments(const uint16_t* base, const uint16_t* head, std::vector<uint16_t>& out);
+}  // namespace fvc
+
+using namespace fvc;
+
+struct fvc_model { Model m; };
+struct fvc_renderer { Renderer r; };
+
+template <class F> static int guard(F&& f) {
+  try {
+    f();
+    return FVC_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return FVC_E_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return FVC_E_INVALID;
+  }
+}
+
+#define REQUIRE(cond, msg) \
+  do {                     \
+    if (!(cond)) throw Error(FVC_E_INVALID, msg); \
+  } while (0)
+
+extern "C" {
+
+const char* fvc_last_error_message(void) { return g_last_error.c_str(); }
+int fvc_version(void) { return 1; }
+
+// StepsizeInfo::setAdditionalInfo (common.h:109-128)
+int fvc_stepsize_setup(fvc_stepsize_info* s, const fvc_scene_info* sc) {
+  return guard([&] {
+    REQUIRE(s && sc, "null argument");
+    s->dt_min = s->stepsize;
+    const float res = (float)sc->grid_resolution;
+    s->dt_max = std::fmax(std::fmax((sc->aabb_to.x - sc->aabb_from.x) / res * 1.732f,
+                                    (sc->aabb_to.y - sc->aabb_from.y) / res * 1.732f),
+                          (sc->aabb_to.z - sc->aabb_from.z) / res * 1.732f);
+    s->t_min = (float)(((double)s->dt_min) / s->cone_angle);
+    const double d_i_min = 1.0 / s->cone_angle;
+    s->i_min = (float)d_i_min;
+    const double d_log1p_c = std::log(1.0 + s->cone_angle);
+    s->log1p_c = (float)d_log1p_c;
+    const double d_i_max = std::log(s->dt_max / (double)s->dt_min) / d_log1p_c + d_i_min;
+    s->i_max = (float)d_i_max;
+    s->t_max = (float)(s->t_min * std::pow(1.0 + s->cone_angle, d_i_max - d_i_min));
+    s->near_i = step_from_t(*s, s->near_);
+  });
+}
+
+int fvc_net_param_counts(const fvc_net_desc* net, int64_t* a, int64_t* b, int64_t* c) {
+  return guard([&] {
+    REQUIRE(net && a && b && c, "null argument");
+    GridParams g;
+    if (!net_layout(*net, g, *a, *b, *c)) throw Error(FVC_E_UNSUPPORTED, "unsupported network description");
+  });
+}
+
+int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const uint8_t* occ,
+                     const uint16_t* p_pos, int64_t n_pos, const uint16_t* p_base, int64_t n_base,
+                     const uint16_t* p_head, int64_t n_head, fvc_model** out) {
+  return guard([&] {
+    REQUIRE(scene && net && occ && p_pos && p_base && p_head && out, "null argument");
+    *out = nullptr;
+    // PerformanceRenderer only supports the fixed 128^3 grid (performance_renderer.cu:162-163)
+    if (scene->grid_resolution != 128) throw Error(FVC_E_UNSUPPORTED, "only grid_resolution 128 is supported");
+    REQUIRE(scene->grid_nlvl >= 1 && scene->grid_nlvl <= 16, "grid_nlvl out of range");
+    if (net->precision != FVC_PREC_FP16) throw Error(FVC_E_UNSUPPORTED, "only FP16 MLP arithmetic is built");
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) throw Error(FVC_E_NO_DEVICE, "no HIP device");
+    auto* h = new fvc_model();
+    Model& m = h->m;
+    try {
+      FVC_HIP(hipGetDevice(&m.device));
+      m.scene = *scene;
+      m.net = *net;
+      if (!net_layout(*net, m.grid, m.n_pos_enc, m.n_base, m.n_head))
+        throw Error(FVC_E_UNSUPPORTED, "unsupported network description");
+      if (n_pos != m.n_pos_enc || n_base != m.n_base || n_head != m.n_head)
+        throw Error(FVC_E_INVALID, "parameter counts do not match the network description");
+      // Morton bitfield (occupancy_grid_to_morton_bitfield, loader.cu:20-55, 120-144)
+      const int64_t vals = 128 * 128 * 128;
+      m.bitfield_host.assign((size_t)(vals * scene->grid_nlvl / 8), 0);
+      for (int l = 0; l < scene->grid_nlvl; l++)
+        for (int x = 0; x < 128; x++)
+          for (int y = 0; y < 128; y++)
+            for (int z = 0; z < 128; z++) {
+              if (!occ[l * vals + ((int64_t)x * 128 + y) * 128 + z]) continue;
+              const int64_t bit = l * vals + morton3d(x, y, z);
+              m.bitfield_host[bit >> 3] |= (uint8_t)(1u << (bit & 7));
+            }
+      m.bitfield.ensure(m.bitfield_host.size());
+      FVC_HIP(hipMemcpy(m.bitfield.p, m.bitfield_host.data(), m.bitfield_host.size(), hipMemcpyHostToDevice));
+      m.table.ensure(n_pos * sizeof(uint16_t));
+      FVC_HIP(hipMemcpy(m.table.p, p_pos, n_pos * sizeof(uint16_t), hipMemcpyHostToDevice));
+      std::vector<uint16_t> frags;
+      pack_fragments(p_base, p_head, frags);
+      m.frags.ensure(frags.size() * sizeof(uint16_t));
+      FVC_HIP(hipMemcpy(m.frags.p, frags.data(), frags.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+      // grid_occupied_at constants (raymarch_common.h:660-666)
+      const float S = (float)(1 << (scene->grid_nlvl - 1));
+      const V3 from = cvt(scene->aabb_from), to = cvt(scene->aabb_to);
+      m.occ.unit_off = from / S;
+      m.occ.unit_ext = (to - from) / S;
+      m.occ.nlvl = scene->grid_nlvl;
+      m.occ.normalized = scene->normalized;
+      // apply_contraction constants (raymarch_common.h:610-653)
+      m.con.type = scene->contraction_type;
+      if (scene->contraction_type == FVC_CONTRACTION_AABB) {
+        m.con.from = from;
+        m.con.ext = to - from;
+      } else {
+        m.con.from = cvt(scene->contraction_aabb_from);
+        m.con.ext = cvt(scene->contraction_aabb_to) - m.con.from;
+      }
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+void fvc_model_destroy(fvc_model* m) { delete m; }
+
+int fvc_model_get_bitfield(fvc_model* h, uint8_t* out, int64_t n) {
+  return guard([&] {
+    REQUIRE(h && out, "null argument");
+    REQUIRE(n == (int64_t)h->m.bitfield_host.size(), "size mismatch");
+    std::memcpy(out, h->m.bitfield_host.data(), (size_t)n);
+  });
+}
+
+int fvc_network_latents(fvc_model* h, const float* pos, int64_t n, uint16_t* out, void* stream) {
+  return guard([&] {
+    REQUIRE(h && pos && out && n >= 0, "invalid argument");
+    launch_network_latents(h->m, pos, n, out, (hipStream_t)stream);
+  });
+}
+int fvc_network_inference(fvc_model* h, const float* pos, const float* dir, int64_t n, uint16_t* out, void* stream) {
+  return guard([&] {
+    REQUIRE(h && pos && dir && out && n >= 0, "invalid argument");
+    launch_network_full(h->m, pos, dir, n, (uint32_t*)out, (hipStream_t)stream);
+  });
+}
+int fvc_network_head(fvc_model* h, const uint16_t* hin, const float* dir, int64_t n, uint16_t* out, void* stream) {
+  return guard([&] {
+    REQUIRE(h && hin && dir && out && n >= 0, "invalid argument");
+    launch_network_head(h->m, hin, dir, n, (uint32_t*)out, (hipStream_t)stream);
+  });
+}
+int fvc_network_encode(fvc_model* h, const float* pos, int64_t n, uint16_t* out, void* stream) {
+  return guard([&] {
+    REQUIRE(h && pos && out && n >= 0, "invalid argument");
+    launch_network_encode(h->m, pos, n, out, (hipStream_t)stream);
+  });
+}
+
+int fvc_renderer_create(fvc_model* model, int double_buffered, fvc_renderer** out) {
+  return guard([&] {
+    REQUIRE(model && out, "null argument");
+    *out = nullptr;
+    auto* h = new fvc_renderer();
+    h->r.model = &model->m;
+    h->r.double_buffered = double_buffered != 0;
+    h->r.fill_idx = h->r.double_buffered ? 1 : 0;
+    // default priority on purpose: non-default stream priorities are avoided
+    hipError_t e = hipStreamCreateWithFlags(&h->r.own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete h;
+      throw Error(FVC_E_HIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+    }
+    *out = h;
+  });
+}
+void fvc_renderer_destroy(fvc_renderer* r) { delete r; }
+
+int fvc_renderer_set_stream(fvc_renderer* h, void* stream) {
+  return guard([&] {
+    REQUIRE(h, "null argument");
+    h->r.user_stream = (hipStream_t)stream;
+  });
+}
+
+int fvc_resize_renderbuffers(fvc_renderer* h, int w, int hh) {
+  return guard([&] {
+    REQUIRE(h, "null argument");
+    resize_renderbuffers(h->r, w, hh);
+  });
+}
+
+static void check_rm(const fvc_raymarch_info* rm) {
+  REQUIRE(rm, "null raymarch info");
+  REQUIRE(rm->cam_info.res_x > 0 && rm->cam_info.res_y > 0, "resolution must be positive");
+  if (!rm->deterministic || rm->motion_blur)
+    throw Error(FVC_E_UNSUPPORTED, "only deterministic rendering without motion blur is supported");
+}
+
+int fvc_render_full(fvc_renderer* h, const fvc_raymarch_info* rm, uint8_t* rgba, fvc_render_stats* stats) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    check_rm(rm);
+    render_full(h->r, *rm, rgba, stats);
+  });
+}
+
+int fvc_cache_fill(fvc_renderer* h, const fvc_raymarch_info* rm, fvc_init_stats* stats) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    check_rm(rm);
+    std::scoped_lock lk(h->r.write_mutex);
+    cache_fill(h->r, *rm, stats);
+  });
+}
+
+int fvc_render_cached(fvc_renderer* h, const fvc_raymarch_info* rm, uint8_t* rgba, fvc_render_stats* stats) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    check_rm(rm);
+    std::scoped_lock lk(h->r.read_mutex);
+    render_cached(h->r, *rm, rgba, stats);
+  });
+}
+
+int fvc_cache_swap(fvc_renderer* h) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    std::scoped_lock lk(h->r.read_mutex, h->r.write_mutex);
+    if (h->r.double_buffered) std::swap(h->r.render_idx, h->r.fill_idx);
+  });
+}
+
+int fvc_cache_set_settings(fvc_renderer* h, const fvc_cache_settings* s) {
+  return guard([&] {
+    REQUIRE(h && s, "null argument");
+    if (s->interpol_variant != FVC_INTERPOL_DENSITY || s->interpol_function != FVC_FN_LINEAR ||
+        !s->use_inter_brick_interpolation)
+      throw Error(FVC_E_UNSUPPORTED, "only the default cache settings (Density, Linear, inter-brick) are supported");
+    h->r.settings = *s;
+  });
+}
+
+int fvc_cache_set_capacity(fvc_renderer* h, int32_t cap) {
+  return guard([&] {
+    REQUIRE(h && cap >= 0, "invalid argument");
+    h->r.capacity_override = cap;
+  });
+}
+
+int fvc_get_last_result(fvc_renderer* h, float* out, int64_t n) {
+  return guard([&] {
+    REQUIRE(h && out, "null argument");
+    const int64_t need = (int64_t)h->r.width * h->r.height * 4;
+    REQUIRE(n >= need && need > 0, "buffer too small or nothing rendered");
+    FVC_HIP(hipMemcpyAsync(out, h->r.results.p, need * sizeof(float), hipMemcpyDeviceToDevice, h->r.stream()));
+    FVC_HIP(hipStreamSynchronize(h->r.stream()));
+  });
+}
+
+static CacheStore& pick(fvc_renderer* h, int which) {
+  REQUIRE(which == 0 || which == 1, "which must be 0 or 1");
+  return h->r.caches[which == 0 ? h->r.render_idx : h->r.fill_idx];
+}
+
+int fvc_cache_get_info(fvc_renderer* h, int which, fvc_cache_info* info) {
+  return guard([&] {
+    REQUIRE(h && info, "null argument");
+    const CacheStore& C = pick(h, which);
+    info->frustum_x = C.fx; info->frustum_y = C.fy; info->frustum_z = C.fz;
+    info->bricks_x = C.bx; info->bricks_y = C.by; info->bricks_z = C.bz;
+    info->capacity_bricks = C.capacity;
+    info->n_bricks_used = C.n_used;
+    info->initialized = C.initialized;
+    info->brick_size = BRICK;
+  });
+}
+
+int fvc_cache_get_brick_table(fvc_renderer* h, int which, int32_t* out, int64_t n) {
+  return guard([&] {
+    REQUIRE(h && out, "null argument");
+    const CacheStore& C = pick(h, which);
+    if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
+    const int64_t need = (int64_t)C.bx * C.by * C.bz;
+    REQUIRE(n >= need, "buffer too small");
+    FVC_HIP(hipMemcpy(out, C.table.p, need * sizeof(int32_t), hipMemcpyDeviceToHost));
+  });
+}
+
+int fvc_cache_get_known_ranges(fvc_renderer* h, int which, float* out, int64_t n) {
+  return guard([&] {
+    REQUIRE(h && out, "null argument");
+    const CacheStore& C = pick(h, which);
+    if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
+    const int64_t need = (int64_t)C.fx * C.fy;
+    REQUIRE(n >= need, "buffer too small");
+    FVC_HIP(hipMemcpy(out, C.known.p, need * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int fvc_cache_get_pool(fvc_renderer* h, int which, void* out, int64_t n_bytes) {
+  return guard([&] {
+    REQUIRE(h && out, "null argument");
+    const CacheStore& C = pick(h, which);
+    if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
+    const int64_t have = (int64_t)C.n_used * BRICK_VOXELS * (int64_t)sizeof(Voxel);
+    REQUIRE(n_bytes <= have, "request exceeds the used part of the pool");
+    FVC_HIP(hipMemcpy(out, C.pool.p, (size_t)n_bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int fvc_debug_march(fvc_renderer* h, const fvc_raymarch_info* rm, int32_t max_per_ray, int32_t* counts, float* t0s) {
+  return guard([&] {
+    REQUIRE(h && counts && max_per_ray >= 0, "invalid argument");
+    check_rm(rm);
+    debug_march(h->r, *rm, max_per_ray, counts, t0s);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+// fvc_internal.h -- host-side objects behind the C ABI and the kernel
+// parameter blocks they hand to the device code.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fvc_math.h"
+
+namespace fvc {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+void set_last_error(const std::string& m);
+
+#define FVC_HIP(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      throw ::fvc::Error(e_ == hipErrorOutOfMemory ? FVC_E_OUT_OF_MEMORY : FVC_E_HIP,          \
+                         std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+  } while (0)
+
+// ---------------------------------------------------------------- memory
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // grow-only allocation; contents are not preserved
+  void ensure(size_t n) {
+    if (n <= bytes) return;
+    release();
+    if (n == 0) return;
+    FVC_HIP(hipMalloc(&p, n));
+    bytes = n;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------- network
+static const int MAX_LEVELS = 16;
+// Hash-grid level table (tcnn GridEncoding conventions).
+struct GridLevel {
+  float scale;
+  uint32_t res;
+  uint32_t size;     // entries in the level
+  uint32_t offset;   // first entry of the level
+};
+struct GridParams {
+  GridLevel lv[MAX_LEVELS];
+  uint32_t dense_mask;  // bit l: level l indexed densely
+  int n_levels;
+};
+
+// Weight fragments: 20 MFMA A-operands of 64 lanes x 8 halves, in the order
+// W1[nb=0..3], W2[nb=0..3][kb=0..1], W3[kb=0..1], H1[nb=0..3], H2[kb=0..1].
+static const int N_FRAGS = 20;
+static const int FRAG_W1 = 0, FRAG_W2 = 4, FRAG_W3 = 12, FRAG_H1 = 14, FRAG_H2 = 18;
+
+struct Model {
+  int device = 0;
+  fvc_scene_info scene{};
+  fvc_net_desc net{};
+  GridParams grid{};
+  OccParams occ{};
+  ContractParams con{};
+  int64_t n_pos_enc = 0, n_base = 0, n_head = 0;
+  DevBuf bitfield;   // Morton occupancy bitfield
+  DevBuf table;      // hash grid, uint32 per entry (2 x fp16)
+  DevBuf frags;      // packed weight fragments (fp16)
+  std::vector<uint8_t> bitfield_host;
+};
+
+// layout of the parameter blocks (nerfacc export) for a description
+bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n_base, int64_t& n_head);
+
+// network launches (network.hip)
+void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
+                         hipStream_t s);
+void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s);
+void launch_network_head(const Model& m, const uint16_t* head_in16, const float* dir, int64_t n, uint32_t* out_h4,
+                         hipStream_t s);
+void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t* out32, hipStream_t s);
+
+// ---------------------------------------------------------------- cache
+static const int BRICK = 8;               // Cache::BRICK_SIZE (cache.h:17)
+static const int BRICK_VOXELS = 512;
+
+// voxel record of the brick pool (32 bytes)
+struct alignas(16) Voxel {
+  uint16_t lat[8];   // fp16 latents
+  float density;
+  float isset;
+  float pad[2];
+};
+
+struct CacheStore {
+  bool initialized = false;
+  fvc_raymarch_info rm{};
+  int fx = 0, fy = 0, fz = 0;
+  int bx = 0, by = 0, bz = 0;
+  int capacity = 0;
+  int n_used = 0;
+  DevBuf table;     // int32 [bz][by][bx]
+  DevBuf known;     // float [fy][fx]
+  DevBuf pool;      // Voxel [capacity][512]
+};
+
+// ---------------------------------------------------------------- renderer
+struct Renderer {
+  Model* model = nullptr;
+  hipStream_t own_stream = nullptr;
+  hipStream_t fill_stream = nullptr;
+  hipStream_t user_stream = nullptr;
+  bool double_buffered = false;
+  CacheStore caches[2];
+  int render_idx = 0, fill_idx = 0;
+  fvc_cache_settings settings{0, 0, 1, 1, 1, 1};
+  int capacity_override = 0;
+  std::mutex read_mutex, write_mutex;
+
+  int width = 0, height = 0;
+  int64_t sample_capacity = 0;
+  // ray state (double buffered for compaction)
+  DevBuf rays[2];        // RayState
+  DevBuf ray_pix[2];     // int32 pixel index
+  DevBuf ray_res[2];     // float4 rgb + T
+  DevBuf results;        // float4 [H*W] final
+  DevBuf ray_entry;      // int4 per ray: offset/count per stream
+  DevBuf ray_alive;      // int32 per ray
+  DevBuf counters;       // int32 device counters
+  int* pinned = nullptr; // pinned host mirror of the counters
+  // sample streams
+  DevBuf s_pos, s_dir, s_t0, s_out;            // resample / full stream
+  DevBuf c_frox, c_dir, c_t0, c_out;            // cache-hit stream
+  // fill scratch
+  DevBuf f_rays, f_tiles, f_list, f_pos, f_t0, f_meta, f_out16;
+  int fill_tiles = 0;
+  int64_t fill_sample_capacity = 0;
+  ~Renderer() {
+    if (pinned) (void)hipHostFree(pinned);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+  }
+
+  hipStream_t stream() const { return user_stream ? user_stream : own_stream; }
+};
+
+// render entry points (render.hip / cache.hip)
+void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats);
+void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats);
+void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats);
+void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts, float* t0s);
+void resize_renderbuffers(Renderer& r, int w, int h);
+
+}  // namespace fvc

@@ -1,0 +1,280 @@
+// network.hip -- multiresolution hash-grid encoding fused with the density /
+// latent MLP and the view-dependent colour MLP (NerfNetwork<half>::inference,
+// inferenceLatents, inferenceHead; nerf_network.h:160-266).
+//
+// Hash-grid conventions follow the tcnn GridEncoding the reference calls
+// (nerf_network.h:90, 181): per level l, scale = exp2(l*log2(b))*N_min - 1,
+// resolution = ceil(scale)+1, entries = min(next_multiple(res^3,8), T);
+// dense index x + y*res + z*res^2 while res^3 <= T, otherwise the hash
+// x ^ y*2654435761 ^ z*805459861; index mod entries; trilinear weights over
+// the 8 corners, accumulated in FP32 and rounded to fp16.
+//
+// Work split: one wave64 evaluates 16 samples per step.  Lane l computes
+// levels 4g..4g+3 (g = l/16) of sample l%16 -- exactly the 8 k-values of the
+// first layer's B fragment -- so the gathered features feed the MFMA chain in
+// mlp_mfma.h directly from registers.
+#include <cmath>
+#include <cstring>
+
+#include "mlp_mfma.h"
+
+namespace fvc {
+
+// ------------------------------------------------------------------ layout
+bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n_base, int64_t& n_head) {
+  if (d.n_levels != 16 || d.n_features_per_level != 2 || d.log2_hashmap_size < 10 || d.log2_hashmap_size > 24)
+    return false;
+  if (d.base_width != 64 || d.base_n_hidden_layers != 2 || d.base_n_output < 9 || d.base_n_output > 16) return false;
+  if (d.head_width != 64 || d.head_n_hidden_layers != 1 || d.n_latents != 8 || d.latent_offset != 1 ||
+      d.sh_degree != 4)
+    return false;
+  std::memset(&g, 0, sizeof(g));
+  g.n_levels = d.n_levels;
+  const uint32_t T = 1u << d.log2_hashmap_size;
+  uint32_t off = 0;
+  for (int l = 0; l < d.n_levels; l++) {
+    const double s = std::exp2(l * std::log2((double)d.per_level_scale)) * d.base_resolution - 1.0;
+    const float sf = (float)s;
+    const uint32_t r = (uint32_t)std::ceil((double)sf) + 1u;
+    const uint64_t dense = (uint64_t)r * r * r;
+    uint64_t p = (dense + 7) / 8 * 8;
+    if (p > T) p = T;
+    g.lv[l].scale = sf;
+    g.lv[l].res = r;
+    g.lv[l].size = (uint32_t)p;
+    g.lv[l].offset = off;
+    if (dense <= p) g.dense_mask |= 1u << l;
+    off += (uint32_t)p;
+  }
+  n_pos = (int64_t)off * 2;
+  n_base = 32 * 64 + 64 * 64 + 64 * 16;
+  n_head = 32 * 64 + 64 * 16;
+  return true;
+}
+
+// ------------------------------------------------------------------ encode
+// Features of levels l0..l0+3 at p (clamped to [0,1]^3) -> f[8].
+__device__ __forceinline__ void encode4(const GridLevel* __restrict__ lv, uint32_t dense_mask,
+                                        const uint32_t* __restrict__ table, V3 p, int l0, float* f) {
+  const float px = fminf_(fmaxf_(p.x, 0.0f), 1.0f);
+  const float py = fminf_(fmaxf_(p.y, 0.0f), 1.0f);
+  const float pz = fminf_(fmaxf_(p.z, 0.0f), 1.0f);
+  uint32_t ent[4][8];
+  float wx[4], wy[4], wz[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int l = l0 + j;
+    const GridLevel L = lv[l];
+    const float ax = px * L.scale + 0.5f, ay = py * L.scale + 0.5f, az = pz * L.scale + 0.5f;
+    const float fx = floorf(ax), fy = floorf(ay), fz = floorf(az);
+    const uint32_t gx = (uint32_t)(int)fx, gy = (uint32_t)(int)fy, gz = (uint32_t)(int)fz;
+    wx[j] = ax - fx; wy[j] = ay - fy; wz[j] = az - fz;
+    const bool dense = (dense_mask >> l) & 1u;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const uint32_t qx = gx + (c & 1), qy = gy + ((c >> 1) & 1), qz = gz + ((c >> 2) & 1);
+      uint32_t i;
+      if (dense) {
+        i = qx + qy * L.res + qz * L.res * L.res;
+        i = i >= L.size ? i - L.size : i;  // i < 2*size for res >= 2
+      } else {
+        i = (qx ^ (qy * 2654435761u) ^ (qz * 805459861u)) & (L.size - 1u);
+      }
+      ent[j][c] = table[L.offset + i];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      float w = 1.0f;
+      w *= (c & 1) ? wx[j] : 1.0f - wx[j];
+      w *= (c & 2) ? wy[j] : 1.0f - wy[j];
+      w *= (c & 4) ? wz[j] : 1.0f - wz[j];
+      const uint32_t e = ent[j][c];
+      const float v0 = (float)__builtin_bit_cast(_Float16, (uint16_t)(e & 0xffffu));
+      const float v1 = (float)__builtin_bit_cast(_Float16, (uint16_t)(e >> 16));
+      a0 = a0 + w * v0;
+      a1 = a1 + w * v1;
+    }
+    f[2 * j] = a0;
+    f[2 * j + 1] = a1;
+  }
+}
+
+__device__ __forceinline__ void load_levels(const GridParams& gp, GridLevel* lv) {
+  if (threadIdx.x < MAX_LEVELS) lv[threadIdx.x] = gp.lv[threadIdx.x];
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ kernels
+static const int NET_BLOCK = 256;  // 4 waves
+
+// HEAD: full inference -> half4 (r, g, b, log sigma) per sample
+// !HEAD: inferenceLatents -> 16 halves per sample
+template <bool HEAD>
+__global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__ pos, const float* __restrict__ dir,
+                                                       int64_t n, const uint32_t* __restrict__ table,
+                                                       const h8* __restrict__ frags, GridParams gp,
+                                                       uint2* __restrict__ out_h4, uint2* __restrict__ out16) {
+  __shared__ GridLevel lv[MAX_LEVELS];
+  load_levels(gp, lv);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  BaseWeights bw;
+  load_base(frags, lane, bw);
+  HeadWeights hw;
+  if (HEAD) load_head(frags, lane, hw);
+  const int64_t n_groups = (n + 15) / 16;
+  for (int64_t grp = (int64_t)blockIdx.x * (NET_BLOCK / 64) + wave; grp < n_groups;
+       grp += (int64_t)gridDim.x * (NET_BLOCK / 64)) {
+    const int64_t s = grp * 16 + col;
+    const bool valid = s < n;
+    const int64_t sl = valid ? s : n - 1;
+    const V3 p = {pos[3 * sl], pos[3 * sl + 1], pos[3 * sl + 2]};
+    float f[8];
+    encode4(lv, gp.dense_mask, table, p, 4 * g, f);
+    h8 x;
+#pragma unroll
+    for (int e = 0; e < 8; e++) x[e] = (_Float16)f[e];
+    const f4 d3 = base_mlp(bw, x);
+    _Float16 rows[4] = {(_Float16)d3[0], (_Float16)d3[1], (_Float16)d3[2], (_Float16)d3[3]};
+    if (!HEAD) {
+      if (valid) {
+        uint2 v;
+        v.x = (uint32_t)__builtin_bit_cast(uint16_t, rows[0]) | ((uint32_t)__builtin_bit_cast(uint16_t, rows[1]) << 16);
+        v.y = (uint32_t)__builtin_bit_cast(uint16_t, rows[2]) | ((uint32_t)__builtin_bit_cast(uint16_t, rows[3]) << 16);
+        out16[s * 4 + g] = v;
+      }
+    } else {
+      const h8 hin = head_input(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2]);
+      const f4 o = head_mlp(hw, hin);
+      if (g == 0 && valid) {
+        uint2 v;
+        v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
+              ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[1]) << 16);
+        v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[2]) |
+              ((uint32_t)__builtin_bit_cast(uint16_t, rows[0]) << 16);
+        out_h4[s] = v;
+      }
+    }
+  }
+}
+
+// inferenceHead on given head inputs (16 halves per sample: row 0 log sigma,
+// rows 1..8 latents)
+__global__ __launch_bounds__(NET_BLOCK) void k_head(const uint16_t* __restrict__ hin16, const float* __restrict__ dir,
+                                                    int64_t n, const h8* __restrict__ frags, uint2* __restrict__ out_h4) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  HeadWeights hw;
+  load_head(frags, lane, hw);
+  const int64_t n_groups = (n + 15) / 16;
+  for (int64_t grp = (int64_t)blockIdx.x * (NET_BLOCK / 64) + wave; grp < n_groups;
+       grp += (int64_t)gridDim.x * (NET_BLOCK / 64)) {
+    const int64_t s = grp * 16 + col;
+    const bool valid = s < n;
+    const int64_t sl = valid ? s : n - 1;
+    _Float16 rows[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) rows[e] = __builtin_bit_cast(_Float16, hin16[sl * 16 + 4 * g + e]);
+    const h8 x = head_input(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2]);
+    const f4 o = head_mlp(hw, x);
+    if (g == 0 && valid) {
+      uint2 v;
+      v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
+            ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[1]) << 16);
+      v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[2]) | ((uint32_t)hin16[sl * 16] << 16);
+      out_h4[s] = v;
+    }
+  }
+}
+
+// encoding alone: 32 halves per sample (thread per sample)
+__global__ void k_encode(const float* __restrict__ pos, int64_t n, const uint32_t* __restrict__ table, GridParams gp,
+                         uint16_t* __restrict__ out32) {
+  __shared__ GridLevel lv[MAX_LEVELS];
+  load_levels(gp, lv);
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const V3 p = {pos[3 * s], pos[3 * s + 1], pos[3 * s + 2]};
+  for (int q = 0; q < 4; q++) {
+    float f[8];
+    encode4(lv, gp.dense_mask, table, p, 4 * q, f);
+    for (int e = 0; e < 8; e++) out32[s * 32 + 8 * q + e] = __builtin_bit_cast(uint16_t, (_Float16)f[e]);
+  }
+}
+
+static int grid_for(int64_t n_groups) {
+  const int64_t waves = n_groups;
+  int64_t blocks = (waves + NET_BLOCK / 64 - 1) / (NET_BLOCK / 64);
+  const int64_t cap = 256 * 16;  // 256 CUs x 16 blocks: grid-stride beyond
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  return (int)blocks;
+}
+
+void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
+                         hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_network<true>, dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, pos, dir, n,
+                     m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, (uint2*)out_h4, (uint2*)nullptr);
+  FVC_HIP(hipGetLastError());
+}
+void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_network<false>, dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, pos,
+                     (const float*)nullptr, n, m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, (uint2*)nullptr,
+                     (uint2*)out16);
+  FVC_HIP(hipGetLastError());
+}
+void launch_network_head(const Model& m, const uint16_t* hin16, const float* dir, int64_t n, uint32_t* out_h4,
+                         hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_head, dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, hin16, dir, n, m.frags.as<h8>(),
+                     (uint2*)out_h4);
+  FVC_HIP(hipGetLastError());
+}
+void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t* out32, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_encode, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos, n, m.table.as<uint32_t>(),
+                     m.grid, out32);
+  FVC_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ packing
+// Column of the reference weight matrix that lane-group g, slot e of a
+// fragment multiplies (see mlp_mfma.h).
+static int col_first(int g, int e) { return 8 * g + e; }                                       // W1
+static int col_chain(int kb, int g, int e) { return e < 4 ? 32 * kb + 4 * g + e : 32 * kb + 16 + 4 * g + e - 4; }
+static int col_head_in(int g, int e) {                                                         // H1
+  if (e >= 4) return 8 + 4 * g + (e - 4);  // SH coefficient 4g + e - 4
+  const int r = 4 * g + e;
+  if (r >= 1 && r <= 8) return r - 1;      // latent r - 1
+  return r == 0 ? 24 : r + 16;             // padding columns 24..31
+}
+
+void pack_fragments(const uint16_t* base, const uint16_t* head, std::vector<uint16_t>& out) {
+  out.assign((size_t)N_FRAGS * 64 * 8, 0);
+  auto put = [&](int frag, const uint16_t* W, int n_in, int row0, auto colf) {
+    for (int lane = 0; lane < 64; lane++) {
+      const int i = lane % 16, g = lane / 16;
+      for (int e = 0; e < 8; e++) out[((size_t)frag * 64 + lane) * 8 + e] = W[(size_t)(row0 + i) * n_in + colf(g, e)];
+    }
+  };
+  const uint16_t* W1 = base;
+  const uint16_t* W2 = base + 32 * 64;
+  const uint16_t* W3 = base + 32 * 64 + 64 * 64;
+  const uint16_t* H1 = head;
+  const uint16_t* H2 = head + 32 * 64;
+  for (int nb = 0; nb < 4; nb++) put(FRAG_W1 + nb, W1, 32, nb * 16, col_first);
+  for (int nb = 0; nb < 4; nb++)
+    for (int kb = 0; kb < 2; kb++)
+      put(FRAG_W2 + nb * 2 + kb, W2, 64, nb * 16, [kb](int g, int e) { return col_chain(kb, g, e); });
+  for (int kb = 0; kb < 2; kb++) put(FRAG_W3 + kb, W3, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
+  for (int nb = 0; nb < 4; nb++) put(FRAG_H1 + nb, H1, 32, nb * 16, col_head_in);
+  for (int kb = 0; kb < 2; kb++) put(FRAG_H2 + kb, H2, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
+}
+
+}  // namespace fvc

@@ -1,0 +1,314 @@
+// render.hip -- full render (PerformanceRenderer::render,
+// src/performance_renderer.cu:214-286) as a sub-iteration loop of
+//   march (compacted sample stream) -> fused network -> composite + ray compaction.
+//
+// Rays live in a compacted SoA list that shrinks every sub-iteration; dead
+// rays scatter their result to the per-pixel result buffer (as
+// compactPayloads_split_kernel does, renderer.cu:158-189).  Samples of one ray
+// are contiguous in the stream (offsets from a wave prefix sum + one atomic
+// per wave), so compositing walks them in order.
+#include <algorithm>
+#include <cstring>
+
+#include "render_common.h"
+
+namespace fvc {
+
+// initRayPayloads_split_kernel (renderer.cu:86-134), deterministic branch:
+// per pixel ray, AABB clip, start t snapped by valid_t_from_t; rays that hit
+// the AABB are appended to the alive list, every pixel result starts as
+// (0,0,0, T=1).
+__global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to,
+                                                   RayState* __restrict__ rays, int* __restrict__ pix,
+                                                   float4* __restrict__ rres, float4* __restrict__ results,
+                                                   int* __restrict__ counter) {
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const bool valid = x < W && y < H;
+  bool alive = false;
+  RayState st{};
+  if (valid) {
+    const Ray r = gen_ray(rm.cam_info, (float)x + 0.5f, (float)y + 0.5f);
+    float t0, t1;
+    const bool hit = ray_aabb(r, aabb_from, aabb_to, t0, t1);
+    const fvc_stepsize_info& s = rm.stepsize_info;
+    st.ox = r.o.x; st.oy = r.o.y; st.oz = r.o.z;
+    st.dx = r.d.x; st.dy = r.d.y; st.dz = r.d.z;
+    st.far_ = hit ? fminf_(t1, s.far_) : s.far_;
+    st.t = hit ? valid_t_from_t(s, t0) : s.near_;
+    alive = hit;
+    results[(int64_t)y * W + x] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  }
+  const int slot = wave_reserve(alive ? 1 : 0, counter);
+  if (alive) {
+    rays[slot] = st;
+    pix[slot] = y * W + x;
+    rres[slot] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  }
+}
+
+// sample_performance_kernel (performance_renderer.cu:15-69): march until
+// n_steps occupied bin midpoints or t_mid > far; emit contracted position,
+// unit_to_01(dir) and t0 of every occupied sample into the compacted stream.
+template <int MAXS>
+__global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, fvc_stepsize_info st, OccParams occ,
+                                               ContractParams con, const uint8_t* __restrict__ bf,
+                                               RayState* __restrict__ rays, int4* __restrict__ entry,
+                                               int* __restrict__ alive_flag, float* __restrict__ s_pos,
+                                               float* __restrict__ s_dir, float* __restrict__ s_t0,
+                                               int* __restrict__ s_counter) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n_alive;
+  float ts[MAXS];
+  int cnt = 0;
+  bool alive = false;
+  RayState r{};
+  if (valid) {
+    r = rays[i];
+    alive = true;
+    const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
+    float t1 = r.t;
+    while (cnt < n_steps) {
+      const float t0 = t1;
+      const float dt = step_dt(st, t0);
+      t1 = t0 + dt;
+      const float tm = (t0 + t1) * 0.5f;
+      if (tm > r.far_) { alive = false; break; }
+      if (occupied(o + tm * d, occ, bf)) {
+#pragma unroll
+        for (int k = 0; k < MAXS; k++)
+          if (k == cnt) ts[k] = t0;
+        cnt++;
+      }
+    }
+    r.t = t1;
+  }
+  const int off = wave_reserve(cnt, s_counter);
+  if (valid) {
+    const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
+    const V3 d01 = unit_to_01(d);
+#pragma unroll
+    for (int k = 0; k < MAXS; k++) {
+      if (k < cnt) {
+        const float t0 = ts[k];
+        const float t1 = t0 + step_dt(st, t0);
+        const float tm = (t0 + t1) * 0.5f;
+        const V3 p = contract(o + tm * d, con);
+        const int64_t q = (int64_t)off + k;
+        s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
+        s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
+        s_t0[q] = t0;
+      }
+    }
+    rays[i].t = r.t;
+    entry[i] = make_int4(off, cnt, 0, 0);
+    alive_flag[i] = alive;
+  }
+}
+
+// accumulate_performance_kernel (performance_renderer.cu:71-125) + the ray
+// compaction of the next sub-iteration (renderer.cu:158-189).
+__global__ __launch_bounds__(256) void k_composite(int n_alive, fvc_stepsize_info st, float alpha_thre,
+                                                   const RayState* __restrict__ rays_in, const int* __restrict__ pix_in,
+                                                   const float4* __restrict__ res_in, const int4* __restrict__ entry,
+                                                   const int* __restrict__ alive_flag, const uint2* __restrict__ s_out,
+                                                   const float* __restrict__ s_t0, RayState* __restrict__ rays_out,
+                                                   int* __restrict__ pix_out, float4* __restrict__ res_out,
+                                                   float4* __restrict__ results, int* __restrict__ counter) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n_alive;
+  bool keep = false;
+  float4 res = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  if (valid) {
+    res = res_in[i];
+    const int4 e = entry[i];
+    bool alive = alive_flag[i] != 0;
+    for (int k = 0; k < e.y; k++) {
+      if (!composite_sample(res, s_out[e.x + k], s_t0[e.x + k], st, alpha_thre)) { alive = false; break; }
+    }
+    keep = alive;
+    if (!keep) results[pix_in[i]] = res;
+  }
+  const int slot = wave_reserve(keep ? 1 : 0, counter);
+  if (keep) {
+    rays_out[slot] = rays_in[i];
+    pix_out[slot] = pix_in[i];
+    res_out[slot] = res;
+  }
+}
+
+// writeImageBuffer_performance_kernel (performance_renderer.cu:127-149)
+__global__ void k_write_image(int W, int H, const float4* __restrict__ results, uchar4* __restrict__ rgba,
+                              int sample_index) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= W || y >= H) return;
+  const int64_t p = (int64_t)y * W + x;
+  const float4 r = results[p];
+  const uchar4 prev = rgba[p];
+  const float si = (float)sample_index, den = (float)(sample_index + 1);
+  const float cr = (r.x + r.w + (float)prev.x / 255.0f * si) / den;
+  const float cg = (r.y + r.w + (float)prev.y / 255.0f * si) / den;
+  const float cb = (r.z + r.w + (float)prev.z / 255.0f * si) / den;
+  rgba[p] = make_uchar4((unsigned char)to_8bit(cr), (unsigned char)to_8bit(cg), (unsigned char)to_8bit(cb), 255);
+}
+
+// march only: counts of occupied samples + first max_per_ray t0 per pixel
+__global__ void k_debug_march(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, OccParams occ,
+                              const uint8_t* __restrict__ bf, int max_per_ray, int* __restrict__ counts,
+                              float* __restrict__ t0s) {
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= W || y >= H) return;
+  const int64_t p = (int64_t)y * W + x;
+  const fvc_stepsize_info& s = rm.stepsize_info;
+  const Ray r = gen_ray(rm.cam_info, (float)x + 0.5f, (float)y + 0.5f);
+  float a0, a1;
+  const bool hit = ray_aabb(r, aabb_from, aabb_to, a0, a1);
+  int cnt = 0;
+  if (hit) {
+    const float far_ = fminf_(a1, s.far_);
+    float t1 = valid_t_from_t(s, a0);
+    while (true) {
+      const float t0 = t1;
+      const float dt = step_dt(s, t0);
+      t1 = t0 + dt;
+      const float tm = (t0 + t1) * 0.5f;
+      if (tm > far_) break;
+      if (!occupied(r.o + tm * r.d, occ, bf)) continue;
+      if (cnt < max_per_ray) t0s[p * max_per_ray + cnt] = t0;
+      cnt++;
+    }
+  }
+  counts[p] = cnt;
+}
+
+// ---------------------------------------------------------------- host
+static const int MAX_SUBITER = 1000;       // PerformanceRenderer::MAX_SUBITER
+static const int64_t TARGET_BATCH = 1 << 22;
+
+void resize_renderbuffers(Renderer& r, int w, int h) {
+  if (w <= 0 || h <= 0) throw Error(FVC_E_INVALID, "resolution must be positive");
+  const int64_t n = (int64_t)w * h;
+  const int64_t cap = std::max<int64_t>(TARGET_BATCH, n) + 64;
+  for (int b = 0; b < 2; b++) {
+    r.rays[b].ensure(n * sizeof(RayState));
+    r.ray_pix[b].ensure(n * sizeof(int));
+    r.ray_res[b].ensure(n * sizeof(float4));
+  }
+  r.results.ensure(n * sizeof(float4));
+  r.ray_entry.ensure(n * sizeof(int4));
+  r.ray_alive.ensure(n * sizeof(int));
+  r.counters.ensure(64 * sizeof(int));
+  r.s_pos.ensure(cap * 3 * sizeof(float));
+  r.s_dir.ensure(cap * 3 * sizeof(float));
+  r.s_t0.ensure(cap * sizeof(float));
+  r.s_out.ensure(cap * sizeof(uint2));
+  r.width = w;
+  r.height = h;
+  r.sample_capacity = cap;
+  if (!r.pinned) FVC_HIP(hipHostMalloc((void**)&r.pinned, 64 * sizeof(int), hipHostMallocDefault));
+}
+
+int read_counter(Renderer& r, int idx, hipStream_t s) {
+  FVC_HIP(hipMemcpyAsync(r.pinned + idx, r.counters.as<int>() + idx, sizeof(int), hipMemcpyDeviceToHost, s));
+  FVC_HIP(hipStreamSynchronize(s));
+  return r.pinned[idx];
+}
+
+void zero_counters(Renderer& r, hipStream_t s) {
+  FVC_HIP(hipMemsetAsync(r.counters.p, 0, 64 * sizeof(int), s));
+}
+
+int launch_init_rays(Renderer& r, const fvc_raymarch_info& rm, hipStream_t s) {
+  const Model& m = *r.model;
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+  zero_counters(r, s);
+  hipLaunchKernelGGL(k_init_rays, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, cvt(m.scene.aabb_from),
+                     cvt(m.scene.aabb_to), r.rays[0].as<RayState>(), r.ray_pix[0].as<int>(), r.ray_res[0].as<float4>(),
+                     r.results.as<float4>(), r.counters.as<int>() + CNT_ALIVE);
+  FVC_HIP(hipGetLastError());
+  return read_counter(r, CNT_ALIVE, s);
+}
+
+void launch_write_image(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, hipStream_t s) {
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+  if (!rgba) return;
+  hipLaunchKernelGGL(k_write_image, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, W, H,
+                     r.results.as<float4>(), (uchar4*)rgba, rm.sample_index);
+  FVC_HIP(hipGetLastError());
+}
+
+int steps_for(int n_alive, int64_t target, int max_steps) {
+  int64_t k = target / std::max(n_alive, 1);
+  if (k < 1) k = 1;
+  if (k > max_steps) k = max_steps;
+  return (int)k;
+}
+
+void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
+  const Model& m = *r.model;
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+  if (W != r.width || H != r.height) resize_renderbuffers(r, W, H);
+  hipStream_t s = r.stream();
+  int n_alive = launch_init_rays(r, rm, s);
+  int cur = 0, subiter = 0;
+  int64_t n_eval = 0;
+  const int64_t target = std::min<int64_t>(TARGET_BATCH, r.sample_capacity - 64);
+  while (n_alive > 0 && subiter < MAX_SUBITER) {
+    const int n_steps = steps_for(n_alive, target, 16);
+    FVC_HIP(hipMemsetAsync(r.counters.as<int>() + CNT_SAMPLES, 0, sizeof(int), s));
+    const dim3 grid((n_alive + 255) / 256);
+    if (n_steps <= 8)
+      hipLaunchKernelGGL(k_march<8>, grid, dim3(256), 0, s, n_alive, n_steps, rm.stepsize_info, m.occ, m.con,
+                         m.bitfield.as<uint8_t>(), r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),
+                         r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
+                         r.counters.as<int>() + CNT_SAMPLES);
+    else
+      hipLaunchKernelGGL(k_march<16>, grid, dim3(256), 0, s, n_alive, n_steps, rm.stepsize_info, m.occ, m.con,
+                         m.bitfield.as<uint8_t>(), r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),
+                         r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
+                         r.counters.as<int>() + CNT_SAMPLES);
+    FVC_HIP(hipGetLastError());
+    const int n_samples = read_counter(r, CNT_SAMPLES, s);
+    n_eval += n_samples;
+    launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), n_samples, r.s_out.as<uint32_t>(), s);
+    FVC_HIP(hipMemsetAsync(r.counters.as<int>() + CNT_ALIVE, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_composite, grid, dim3(256), 0, s, n_alive, rm.stepsize_info, m.scene.alpha_thre,
+                       r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
+                       r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.s_out.as<uint2>(), r.s_t0.as<float>(),
+                       r.rays[cur ^ 1].as<RayState>(), r.ray_pix[cur ^ 1].as<int>(), r.ray_res[cur ^ 1].as<float4>(),
+                       r.results.as<float4>(), r.counters.as<int>() + CNT_ALIVE);
+    FVC_HIP(hipGetLastError());
+    n_alive = read_counter(r, CNT_ALIVE, s);
+    cur ^= 1;
+    subiter++;
+  }
+  launch_write_image(r, rm, rgba, s);
+  FVC_HIP(hipStreamSynchronize(s));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->samples_ppx = (float)n_eval / (float)((int64_t)W * H);
+    stats->resample_samples_evaluated = stats->samples_ppx;
+    stats->n_resamples = (uint64_t)n_eval;
+    stats->n_subiters = subiter;
+  }
+}
+
+void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts_host, float* t0_host) {
+  const Model& m = *r.model;
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+  const int64_t n = (int64_t)W * H;
+  hipStream_t s = r.stream();
+  DevBuf c, t;
+  c.ensure(n * sizeof(int));
+  t.ensure(std::max<int64_t>(1, n * max_per_ray) * sizeof(float));
+  hipLaunchKernelGGL(k_debug_march, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, cvt(m.scene.aabb_from),
+                     cvt(m.scene.aabb_to), m.occ, m.bitfield.as<uint8_t>(), max_per_ray, c.as<int>(), t.as<float>());
+  FVC_HIP(hipGetLastError());
+  FVC_HIP(hipMemcpyAsync(counts_host, c.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (max_per_ray > 0 && t0_host)
+    FVC_HIP(hipMemcpyAsync(t0_host, t.p, n * max_per_ray * sizeof(float), hipMemcpyDeviceToHost, s));
+  FVC_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace fvc

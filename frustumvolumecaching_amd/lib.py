@@ -1,0 +1,261 @@
+"""ctypes binding of libfvc.so and a thin mirror of the reference's renderer
+classes (PerformanceRenderer, CachePerformanceRenderer / CacheRenderer,
+NerfNetwork) over the C ABI of include/fvc.h.
+
+Device buffers are passed as raw device addresses: anything with a
+``data_ptr()`` (a torch tensor on the GPU) or a plain int.  The library is
+the only compute path: if libfvc.so is missing this module raises.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfvc.so")
+
+_lib = None
+
+
+class FvcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fvc error {code}: {msg}")
+        self.code = code
+
+
+OK, E_INVALID, E_UNSUPPORTED, E_NOT_INITIALIZED, E_HIP, E_OUT_OF_MEMORY, E_NO_DEVICE = 0, -1, -2, -3, -4, -5, -6
+
+_P = C.c_void_p
+_SIGS = {
+    "fvc_last_error_message": (C.c_char_p, []),
+    "fvc_version": (C.c_int, []),
+    "fvc_stepsize_setup": (C.c_int, [C.POINTER(abi.StepsizeInfo), C.POINTER(abi.SceneInfo)]),
+    "fvc_net_param_counts": (C.c_int, [C.POINTER(abi.NetDesc), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64)]),
+    "fvc_model_create": (C.c_int, [C.POINTER(abi.SceneInfo), C.POINTER(abi.NetDesc), _P, _P, C.c_int64, _P,
+                                   C.c_int64, _P, C.c_int64, C.POINTER(_P)]),
+    "fvc_model_destroy": (None, [_P]),
+    "fvc_model_get_bitfield": (C.c_int, [_P, _P, C.c_int64]),
+    "fvc_network_latents": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    "fvc_network_inference": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P]),
+    "fvc_network_head": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P]),
+    "fvc_network_encode": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    "fvc_renderer_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "fvc_renderer_destroy": (None, [_P]),
+    "fvc_renderer_set_stream": (C.c_int, [_P, _P]),
+    "fvc_resize_renderbuffers": (C.c_int, [_P, C.c_int, C.c_int]),
+    "fvc_render_full": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.POINTER(abi.RenderStats)]),
+    "fvc_cache_fill": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.POINTER(abi.InitStats)]),
+    "fvc_render_cached": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.POINTER(abi.RenderStats)]),
+    "fvc_cache_swap": (C.c_int, [_P]),
+    "fvc_cache_set_settings": (C.c_int, [_P, C.POINTER(abi.CacheSettings)]),
+    "fvc_cache_set_capacity": (C.c_int, [_P, C.c_int32]),
+    "fvc_get_last_result": (C.c_int, [_P, _P, C.c_int64]),
+    "fvc_cache_get_info": (C.c_int, [_P, C.c_int, C.POINTER(abi.CacheInfo)]),
+    "fvc_cache_get_brick_table": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
+    "fvc_cache_get_known_ranges": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
+    "fvc_cache_get_pool": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
+    "fvc_debug_march": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, _P, _P]),
+}
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def load():
+    """Load libfvc.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: run `python -m frustumvolumecaching_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        raise FvcError(rc, load().fvc_last_error_message().decode(errors="replace"))
+
+
+def _addr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return C.c_void_p(x)
+    if hasattr(x, "data_ptr"):
+        return C.c_void_p(x.data_ptr())
+    if hasattr(x, "ctypes"):
+        return C.c_void_p(x.ctypes.data)
+    raise TypeError(f"cannot take the address of {type(x)}")
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if hasattr(s, "cuda_stream"):
+        return C.c_void_p(s.cuda_stream)
+    return C.c_void_p(int(s))
+
+
+def stepsize_setup(step, scene):
+    """StepsizeInfo::setAdditionalInfo (common.h:109-128)."""
+    check(load().fvc_stepsize_setup(C.byref(step), C.byref(scene)))
+    return step
+
+
+def net_param_counts(net):
+    a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+    check(load().fvc_net_param_counts(C.byref(net), C.byref(a), C.byref(b), C.byref(c)))
+    return a.value, b.value, c.value
+
+
+class NerfNetwork:
+    """Model handle: NerfNetwork<half> + Morton occupancy bitfield
+    (nerf_network.h:84-369, loader.cu:20-55)."""
+
+    def __init__(self, scene, net, occupancy_linear, params):
+        import numpy as np
+        pos, base, head = (np.ascontiguousarray(p, dtype=np.uint16) for p in params)
+        occ = np.ascontiguousarray(occupancy_linear, dtype=np.uint8)
+        self.scene, self.net = scene, net
+        h = C.c_void_p()
+        check(load().fvc_model_create(C.byref(scene), C.byref(net), _addr(occ), _addr(pos), pos.size, _addr(base),
+                                      base.size, _addr(head), head.size, C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_workload(cls, wl):
+        return cls(wl.scene, wl.net, wl.occupancy, wl.params)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fvc_model_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def bitfield(self):
+        import numpy as np
+        out = np.zeros(self.scene.grid_nlvl * 128 ** 3 // 8, np.uint8)
+        check(load().fvc_model_get_bitfield(self.h, _addr(out), out.size))
+        return out
+
+    # NerfNetwork::inferenceLatents / inference / inferenceHead
+    def inferenceLatents(self, pos, n, out16, stream=None):
+        check(load().fvc_network_latents(self.h, _addr(pos), n, _addr(out16), _stream(stream)))
+
+    def inference(self, pos, dir01, n, out4, stream=None):
+        check(load().fvc_network_inference(self.h, _addr(pos), _addr(dir01), n, _addr(out4), _stream(stream)))
+
+    def inferenceHead(self, head_in16, dir01, n, out4, stream=None):
+        check(load().fvc_network_head(self.h, _addr(head_in16), _addr(dir01), n, _addr(out4), _stream(stream)))
+
+    def encode(self, pos, n, out32, stream=None):
+        check(load().fvc_network_encode(self.h, _addr(pos), n, _addr(out32), _stream(stream)))
+
+
+class Renderer:
+    """One handle = PerformanceRenderer + CacheRenderer + CachePerformanceRenderer
+    sharing the model (eval.cu:208-227).  double_buffered=False mirrors
+    CachePerformanceRenderer(&cache, &cache)."""
+
+    def __init__(self, model, double_buffered=False):
+        self.model = model
+        h = C.c_void_p()
+        check(load().fvc_renderer_create(model.h, int(bool(double_buffered)), C.byref(h)))
+        self.h = h
+        self.last_render_stats = None
+        self.init_stats = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fvc_renderer_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_stream(self, stream):
+        check(load().fvc_renderer_set_stream(self.h, _stream(stream)))
+
+    def resizeRenderbuffers(self, width, height):
+        check(load().fvc_resize_renderbuffers(self.h, int(width), int(height)))
+
+    # PerformanceRenderer::render
+    def render_full(self, rm, rgba=None):
+        st = abi.RenderStats()
+        check(load().fvc_render_full(self.h, C.byref(rm), _addr(rgba), C.byref(st)))
+        self.last_render_stats = st
+        return st
+
+    # CacheRenderer::initCacheBlockwise
+    def initCacheBlockwise(self, rm):
+        st = abi.InitStats()
+        check(load().fvc_cache_fill(self.h, C.byref(rm), C.byref(st)))
+        self.init_stats = st
+        return st
+
+    # CachePerformanceRenderer::render
+    def render_cached(self, rm, rgba=None):
+        st = abi.RenderStats()
+        check(load().fvc_render_cached(self.h, C.byref(rm), _addr(rgba), C.byref(st)))
+        self.last_render_stats = st
+        return st
+
+    def swapCache(self):
+        check(load().fvc_cache_swap(self.h))
+
+    def setCacheSettings(self, settings):
+        check(load().fvc_cache_set_settings(self.h, C.byref(settings)))
+
+    def set_cache_capacity(self, bricks):
+        check(load().fvc_cache_set_capacity(self.h, int(bricks)))
+
+    def last_result(self, out_dev, n_floats):
+        check(load().fvc_get_last_result(self.h, _addr(out_dev), int(n_floats)))
+
+    def cache_info(self, which=0):
+        info = abi.CacheInfo()
+        check(load().fvc_cache_get_info(self.h, which, C.byref(info)))
+        return info
+
+    def brick_table(self, which=0):
+        import numpy as np
+        i = self.cache_info(which)
+        out = np.zeros((i.bricks_z, i.bricks_y, i.bricks_x), np.int32)
+        check(load().fvc_cache_get_brick_table(self.h, which, _addr(out), out.size))
+        return out
+
+    def known_ranges(self, which=0):
+        import numpy as np
+        i = self.cache_info(which)
+        out = np.zeros((i.frustum_y, i.frustum_x), np.float32)
+        check(load().fvc_cache_get_known_ranges(self.h, which, _addr(out), out.size))
+        return out
+
+    def pool(self, which=0, n_bricks=None):
+        import numpy as np
+        i = self.cache_info(which)
+        n = i.n_bricks_used if n_bricks is None else n_bricks
+        out = np.zeros((n, 8, 8, 8), abi.VOXEL_DTYPE)
+        if n:
+            check(load().fvc_cache_get_pool(self.h, which, _addr(out), out.nbytes))
+        return out
+
+    def debug_march(self, rm, max_per_ray=64):
+        import numpy as np
+        n = rm.cam_info.res_x * rm.cam_info.res_y
+        counts = np.zeros(n, np.int32)
+        t0s = np.zeros((n, max_per_ray), np.float32)
+        check(load().fvc_debug_march(self.h, C.byref(rm), max_per_ray, _addr(counts), _addr(t0s)))
+        return counts, t0s
+
+
+# reference class names
+PerformanceRenderer = Renderer
+CachePerformanceRenderer = Renderer

@@ -1,0 +1,255 @@
+/*
+ * fvc.h -- C ABI of the MI355X-native FrustumVolumeCaching render path.
+ *
+ * One shared library (libfvc.so) exposes the renderer / cache-renderer of the
+ * reference behind plain C: opaque handles, POD structs, int status codes.
+ * No C++ exception and no torch type crosses this boundary.
+ *
+ * Reference interface each entry point replaces (paths relative to the
+ * reference repository root):
+ *   SceneInfo / StepsizeInfo / CameraInfo / RaymarchInfo   include/common.h:74-218
+ *   NerfNetwork<half>::{init, set_params, inference,
+ *                       inferenceLatents, inferenceHead}   include/nerf_network.h:88-308
+ *   PerformanceRenderer::{render, resizeRenderbuffers}     src/performance_renderer.cu:186-286
+ *   CacheRenderer::{resizeCache, initCacheBlockwise,
+ *                   swapCache, setCacheSettings}           src/cache_renderer.cu:832-1057,
+ *                                                           include/cache_renderer.h:243-267
+ *   CachePerformanceRenderer::render                       src/cache_performance_renderer.cu:1059-1263
+ *   Cache (brick indirection, pool, known_ranges_to)       include/cache.h:12-74
+ *
+ * All device pointers are HIP device pointers on the renderer's device; all
+ * other pointers are host pointers.  Every function returns FVC_OK (0) or a
+ * negative status; fvc_last_error_message() gives the text (thread-local).
+ */
+#ifndef FVC_H
+#define FVC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference throws C++ exceptions instead) ---- */
+#define FVC_OK                   0
+#define FVC_E_INVALID          (-1)  /* bad argument / null pointer */
+#define FVC_E_UNSUPPORTED      (-2)  /* std::invalid_argument in my_fully_fused_mlp.cu:1386-1411 */
+#define FVC_E_NOT_INITIALIZED  (-3)  /* "Cache is not initialized!" cache_performance_renderer.cu:1063-1064 */
+#define FVC_E_HIP              (-4)  /* MY_CUDA_CHECK_THROW cuda_helper.h:20-27 */
+#define FVC_E_OUT_OF_MEMORY    (-5)
+#define FVC_E_NO_DEVICE        (-6)
+
+/* ---- POD mirrors of the reference types ---- */
+typedef struct { float x, y; } fvc_float2;
+typedef struct { float x, y, z; } fvc_float3;
+typedef struct { float x, y, z, w; } fvc_float4;
+
+/* CameraMatrix (common.h:22-62): three rows of a 3x4 rigid transform. */
+typedef struct { fvc_float4 m0, m1, m2; } fvc_camera_matrix;
+
+/* ContractionType (common.h:66-72) */
+enum {
+  FVC_CONTRACTION_NONE = 0,
+  FVC_CONTRACTION_AABB = 1,
+  FVC_CONTRACTION_WARP_AABB_L2 = 2,
+  FVC_CONTRACTION_WARP_AABB_LINF = 3
+};
+
+/* SceneInfo (common.h:74-91).  bools are int32 here. */
+typedef struct {
+  int32_t grid_resolution;        /* must be 128 (performance_renderer.cu:162-163) */
+  int32_t grid_nlvl;              /* occupancy cascades */
+  int32_t vals_per_lvl;           /* grid_resolution^3 */
+  int32_t normalized;             /* positions already in [0,1] of the inner cascade */
+  fvc_float3 aabb_from, aabb_to;  /* outer (ray-bounding) AABB */
+  fvc_float3 contraction_aabb_from, contraction_aabb_to;
+  int32_t contraction_type;
+  int32_t is_open_gl;
+  float alpha_thre;
+} fvc_scene_info;
+
+/* StepsizeInfo (common.h:93-164).  Fill the first six fields, then call
+ * fvc_stepsize_setup() (setAdditionalInfo, common.h:109-128). */
+typedef struct {
+  float cone_angle, log1p_c, stepsize, near_, near_i, far_;
+  float dt_min, dt_max, t_min, t_max, i_min, i_max;
+} fvc_stepsize_info;
+
+/* CameraInfo (common.h:166-195); depth-of-field fields are not supported. */
+typedef struct {
+  int32_t res_x, res_y;
+  int32_t is_open_gl;
+  fvc_float2 focal, principal;
+  fvc_camera_matrix cam2world, world2cam;
+} fvc_camera_info;
+
+/* RaymarchInfo (common.h:210-218).  Only deterministic = 1 and
+ * motion_blur = 0 are supported (the eval and cache-fill settings,
+ * eval.cu:157,182,190). */
+typedef struct {
+  fvc_camera_info cam_info;
+  fvc_stepsize_info stepsize_info;
+  int32_t sample_index;           /* running-mean index of the uchar4 write */
+  int32_t motion_blur;
+  int32_t deterministic;
+} fvc_raymarch_info;
+
+/* Network description: the subset of the tcnn JSON config the render path
+ * needs (nerf_network.h:88-118).  Supported: HashGrid L=16,F=2; density MLP
+ * 32->64->64->16 (ReLU, no bias); head (n_latents + SH4(16) + pad 1.0 = 32)
+ * ->64->3 (padded 16).  Anything else returns FVC_E_UNSUPPORTED. */
+enum { FVC_PREC_FP16 = 0, FVC_PREC_BF16 = 1, FVC_PREC_FP8 = 2 };
+typedef struct {
+  int32_t n_levels;               /* 16 */
+  int32_t n_features_per_level;   /* 2 */
+  int32_t log2_hashmap_size;      /* 19 .. 22 */
+  int32_t base_resolution;        /* 16 */
+  float   per_level_scale;        /* e.g. (2048/16)^(1/15) */
+  int32_t base_width;             /* 64 */
+  int32_t base_n_hidden_layers;   /* 2 */
+  int32_t base_n_output;          /* 1 (log sigma) + n_latents, <= 16 */
+  int32_t head_width;             /* 64 */
+  int32_t head_n_hidden_layers;   /* 1 */
+  int32_t n_latents;              /* 8 */
+  int32_t latent_offset;          /* 1: head input starts after log sigma (nerf_network.h:334-337) */
+  int32_t sh_degree;              /* 4 */
+  int32_t precision;              /* FVC_PREC_* (MLP arithmetic) */
+} fvc_net_desc;
+
+/* DensityInterpolVariant (cache_renderer.h:15-22) */
+enum {
+  FVC_INTERPOL_DENSITY = 0, FVC_INTERPOL_ALPHA = 1, FVC_INTERPOL_INTERMEDIATES = 2,
+  FVC_INTERPOL_DENSITY_INTERMEDIATES = 3, FVC_INTERPOL_ALPHA_INTERMEDIATES = 4
+};
+/* InterpolFunction (common.h:221-227) */
+enum { FVC_FN_LINEAR = 0, FVC_FN_SMOOTHSTEP = 1, FVC_FN_SMOOTHERSTEP = 2, FVC_FN_NEAREST = 3 };
+
+/* CacheSettings (cache_renderer.h:24-32).  Supported: interpol_variant
+ * DENSITY, interpol_function LINEAR, use_inter_brick_interpolation = 1 (the
+ * default_cache_settings, cache_renderer.h:55-63). */
+typedef struct {
+  int32_t interpol_variant;
+  int32_t interpol_function;
+  int32_t use_hw_interpol;        /* ignored: no texture hardware is used */
+  int32_t reweight_intermediates;
+  int32_t disable_reweighting_first_intermediate;
+  int32_t use_inter_brick_interpolation;
+} fvc_cache_settings;
+
+/* CacheRenderer::InitStats (cache_renderer.h:104-109) + fill diagnostics */
+typedef struct {
+  float   samples_ppx;
+  int32_t n_bricks_set;
+  int32_t n_bricks_reserved;      /* pool capacity in bricks */
+  int32_t overflow;               /* 1 if a brick request exceeded capacity */
+  int32_t n_subiters;
+  int64_t n_samples_evaluated;
+} fvc_init_stats;
+
+/* RenderStats (performance_renderer.h:45-49, cache_performance_renderer.h:123-131) */
+typedef struct {
+  float    samples_ppx;
+  float    resample_samples_evaluated;  /* per pixel */
+  float    cache_samples_evaluated;     /* per pixel */
+  uint64_t n_resamples;
+  uint64_t n_cache_hits;
+  int32_t  n_subiters;
+  int32_t  pad_;
+} fvc_render_stats;
+
+/* Frustum cache geometry (Cache::frustumDims / brickArrayDims, cache.h:70-73) */
+typedef struct {
+  int32_t frustum_x, frustum_y, frustum_z;
+  int32_t bricks_x, bricks_y, bricks_z;
+  int32_t capacity_bricks;
+  int32_t n_bricks_used;
+  int32_t initialized;
+  int32_t brick_size;             /* 8 */
+} fvc_cache_info;
+
+typedef struct fvc_model fvc_model;
+typedef struct fvc_renderer fvc_renderer;
+
+/* ---- library ---- */
+const char* fvc_last_error_message(void);
+int fvc_version(void);
+/* StepsizeInfo::setAdditionalInfo (common.h:109-128); host only. */
+int fvc_stepsize_setup(fvc_stepsize_info* info, const fvc_scene_info* scene);
+/* Number of fp16 parameters of each block of the nerfacc export layout
+ * (pos_enc.dat, mlp_base.dat, mlp_head.dat; loader.cu:232-285). Host only. */
+int fvc_net_param_counts(const fvc_net_desc* net, int64_t* n_pos_enc, int64_t* n_mlp_base, int64_t* n_mlp_head);
+
+/* ---- model: NerfNetwork<half> + occupancy bitfield (loader.cu:20-55, 287-301) ----
+ * occupancy_linear_u8: host [nlvl][res][res][res] (x slowest), >0 = occupied;
+ * converted to the Morton bitfield the renderers read. params_*: host fp16 bits. */
+int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net,
+                     const uint8_t* occupancy_linear_u8,
+                     const uint16_t* params_pos_enc, int64_t n_pos_enc,
+                     const uint16_t* params_mlp_base, int64_t n_mlp_base,
+                     const uint16_t* params_mlp_head, int64_t n_mlp_head,
+                     fvc_model** out_model);
+void fvc_model_destroy(fvc_model* model);
+/* Copy the Morton bitfield (nlvl*res^3/8 bytes) to host. */
+int fvc_model_get_bitfield(fvc_model* model, uint8_t* out_host, int64_t n_bytes);
+
+/* NerfNetwork::inferenceLatents (nerf_network.h:173-202): pos_dev float[n][3]
+ * in [0,1]^3 -> out_dev half[n][16] (row 0 log sigma, rows 1.. latents). */
+int fvc_network_latents(fvc_model* model, const float* pos_dev, int64_t n, uint16_t* out_dev, void* stream);
+/* NerfNetwork::inference (nerf_network.h:160-171): pos, dir01 float[n][3] ->
+ * out half[n][4] = (r, g, b raw, log sigma) (output rows 0-3 of the reference). */
+int fvc_network_inference(fvc_model* model, const float* pos_dev, const float* dir_dev, int64_t n, uint16_t* out_dev, void* stream);
+/* NerfNetwork::inferenceHead (nerf_network.h:204-266): head_in half[n][16]
+ * (row 0 log sigma, rows 1..8 latents), dir01 float[n][3] -> out half[n][4]. */
+int fvc_network_head(fvc_model* model, const uint16_t* head_in_dev, const float* dir_dev, int64_t n, uint16_t* out_dev, void* stream);
+/* Multiresolution hash-grid encoding alone: pos float[n][3] -> half[n][32]. */
+int fvc_network_encode(fvc_model* model, const float* pos_dev, int64_t n, uint16_t* out_dev, void* stream);
+
+/* ---- renderer: PerformanceRenderer + CacheRenderer + CachePerformanceRenderer ----
+ * double_buffered = 0 mirrors CachePerformanceRenderer(&cache, &cache)
+ * (eval.cu:224); 1 gives a separate fill target swapped by fvc_cache_swap. */
+int fvc_renderer_create(fvc_model* model, int double_buffered, fvc_renderer** out_renderer);
+void fvc_renderer_destroy(fvc_renderer* renderer);
+/* Use the caller's HIP stream for render calls (NULL = renderer's own stream). */
+int fvc_renderer_set_stream(fvc_renderer* renderer, void* stream);
+/* Renderer::resizeRenderbuffers (renderer.h:112) */
+int fvc_resize_renderbuffers(fvc_renderer* renderer, int width, int height);
+/* PerformanceRenderer::render (performance_renderer.cu:214-286):
+ * rgba_dev uchar4[H][W] (running mean over sample_index, as the reference). */
+int fvc_render_full(fvc_renderer* renderer, const fvc_raymarch_info* rm, uint8_t* rgba_dev, fvc_render_stats* stats);
+/* CacheRenderer::initCacheBlockwise (cache_renderer.cu:931-1051) into the fill target. */
+int fvc_cache_fill(fvc_renderer* renderer, const fvc_raymarch_info* rm, fvc_init_stats* stats);
+/* CachePerformanceRenderer::render (cache_performance_renderer.cu:1059-1263). */
+int fvc_render_cached(fvc_renderer* renderer, const fvc_raymarch_info* rm, uint8_t* rgba_dev, fvc_render_stats* stats);
+/* CacheRenderer::swapCache (cache_renderer.cu:1053-1057) */
+int fvc_cache_swap(fvc_renderer* renderer);
+/* CacheRenderer::setCacheSettings (cache_renderer.h:266-267) */
+int fvc_cache_set_settings(fvc_renderer* renderer, const fvc_cache_settings* settings);
+/* Pool capacity override in bricks (0 = reference estimate n_rays/8 rounded
+ * up to 16384, cache_renderer.cu:832-846 + cache.cu:143-146). */
+int fvc_cache_set_capacity(fvc_renderer* renderer, int32_t capacity_bricks);
+
+/* ---- inspection (bit-exact parity tests) ---- */
+/* Float result of the last render before the 8-bit write: rgb + transmittance
+ * per pixel, float[H][W][4] (BaseRayResult, renderer_common.h:74-78). */
+int fvc_get_last_result(fvc_renderer* renderer, float* out_dev, int64_t n_floats);
+int fvc_cache_get_info(fvc_renderer* renderer, int which /*0 = render cache, 1 = fill target*/, fvc_cache_info* info);
+/* Indirection table int32[bz][by][bx]: pool slot or -1 (cache.h:31-33). */
+int fvc_cache_get_brick_table(fvc_renderer* renderer, int which, int32_t* out_host, int64_t n);
+/* known_ranges_to float[H][W] (cache.h:36) */
+int fvc_cache_get_known_ranges(fvc_renderer* renderer, int which, float* out_host, int64_t n);
+/* Voxel records of the first n_bricks pool slots: per voxel
+ * {half latents[8]; float density; float isset; float pad[2]} = 32 bytes,
+ * voxel order z,y,x inside a brick. */
+int fvc_cache_get_pool(fvc_renderer* renderer, int which, void* out_host, int64_t n_bytes);
+
+/* Ray marcher alone (sample_performance_kernel, performance_renderer.cu:15-69,
+ * without the network): per pixel the number of occupied samples and the
+ * first max_per_ray sample start times t0; host outputs. */
+int fvc_debug_march(fvc_renderer* renderer, const fvc_raymarch_info* rm, int32_t max_per_ray,
+                    int32_t* counts_host, float* t0_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVC_H */

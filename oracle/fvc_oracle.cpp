@@ -1,0 +1,1218 @@
+// fvc_oracle.cpp -- CPU oracle of the FrustumVolumeCaching render path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness checker and the
+// CPU baseline: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg load it; the product library never does.
+//
+// It restates the reference's algorithm in plain C++/OpenMP, templated on
+// the arithmetic type:
+//   mode 0  F64    geometry + network in double, no half rounding
+//   mode 1  F32    geometry + network in float, no half rounding
+//   mode 2  EMU16  float geometry/accumulation, IEEE binary16 rounding at
+//                  every point where the reference stores half
+//   mode 3  EMU_BF16 as EMU16, MLP activations/weights rounded to bfloat16
+//   mode 4  EMU_FP8  as EMU16, MLP weights + activations in FP8-E4M3 with a
+//                    per-layer absmax scale (FP32 accumulate)
+// Every function cites the reference file:line it follows (paths relative
+// to the reference repository root).
+//
+// Conventions the reference leaves to tiny-cuda-nn (absent from the
+// reference: an empty submodule) follow the published iNGP / tcnn
+// definitions: HashGrid level scale exp2(l*log2(b))*N_min - 1, resolution
+// ceil(scale)+1, dense indexing while res^3 <= T else the xor-of-primes hash
+// (1, 2654435761, 805459861) mod T, trilinear weights over the 8 corners;
+// FullyFusedMLP layers stored row-major [n_out][n_in], no bias, ReLU hidden.
+//
+// Deterministic transcendental functions: the cone-angle step schedule uses
+// det_logf / det_exp2f below (basic IEEE operations only, fixed evaluation
+// order) so that the FP32 marcher is bit-reproducible across CPU and GPU.
+#include "../include/fvc.h"
+#include "half_emu.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+enum Mode { M_F64 = 0, M_F32 = 1, M_EMU16 = 2, M_EMU_BF16 = 3, M_EMU_FP8 = 4 };
+
+// ---------------------------------------------------------------------------
+// Deterministic float log / exp2 (range reduction + series, no FMA).
+// ---------------------------------------------------------------------------
+static const float LN2_HI = 0.693145751953125f;          // 15 significant bits
+static const float LN2_LO = 1.428606765330187045e-06f;
+static const float LOG2E_F = 1.44269504088896340736f;
+static const float LN2_F = 0.693147180559945309417f;
+
+float det_logf(float x) {
+  // x = m * 2^e, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh((m-1)/(m+1))
+  int e;
+  float m = std::frexp(x, &e);
+  if (m < 0.70710678118654752f) { m = m * 2.0f; e = e - 1; }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  float p = 0.111111111111111111f;
+  p = p * s2 + 0.142857142857142857f;
+  p = p * s2 + 0.2f;
+  p = p * s2 + 0.333333333333333333f;
+  p = p * s2 + 1.0f;
+  const float lm = (2.0f * s) * p;
+  const float ef = (float)e;
+  return ef * LN2_HI + (ef * LN2_LO + lm);
+}
+
+float det_exp2f(float y) {
+  const float n = std::floor(y + 0.5f);
+  const float f = y - n;
+  const float u = f * LN2_F;
+  float p = 1.98412698412698413e-4f;   // 1/5040
+  p = p * u + 1.38888888888888889e-3f; // 1/720
+  p = p * u + 8.33333333333333333e-3f; // 1/120
+  p = p * u + 4.16666666666666667e-2f; // 1/24
+  p = p * u + 0.166666666666666667f;   // 1/6
+  p = p * u + 0.5f;
+  p = p * u + 1.0f;
+  p = p * u + 1.0f;
+  return std::ldexp(p, (int)n);
+}
+
+float det_powf(float b, float y) { return det_exp2f(y * (det_logf(b) * LOG2E_F)); }
+
+// ---------------------------------------------------------------------------
+// Small vector type
+// ---------------------------------------------------------------------------
+template <class R> struct V3 { R x, y, z; };
+template <class R> inline V3<R> mk(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, V3<R> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class R> inline V3<R> operator/(V3<R> a, V3<R> b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+template <class R> inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> inline V3<R> operator+(V3<R> a, R s) { return {a.x + s, a.y + s, a.z + s}; }
+template <class R> inline V3<R> operator-(V3<R> a, R s) { return {a.x - s, a.y - s, a.z - s}; }
+template <class R> inline R dot3(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline V3<R> cvt(fvc_float3 f) { return {(R)f.x, (R)f.y, (R)f.z}; }
+
+// helper_math normalize (helper_math.h:1326-1330) with invLen = 1/sqrt(.)
+template <class R> inline V3<R> normalize(V3<R> v) {
+  R inv = R(1) / std::sqrt(dot3(v, v));
+  return v * inv;
+}
+
+// CameraMatrix (common.h:22-62); transform = three float4 dot products,
+// evaluated left to right (helper_math.h:1269-1272).
+template <class R> struct Mat34 {
+  R m[3][4];
+  explicit Mat34(const fvc_camera_matrix& c) {
+    const fvc_float4* rows[3] = {&c.m0, &c.m1, &c.m2};
+    for (int i = 0; i < 3; i++) {
+      m[i][0] = rows[i]->x; m[i][1] = rows[i]->y; m[i][2] = rows[i]->z; m[i][3] = rows[i]->w;
+    }
+  }
+  V3<R> transform(R x, R y, R z, R w) const {
+    R r[3];
+    for (int i = 0; i < 3; i++) r[i] = x * m[i][0] + y * m[i][1] + z * m[i][2] + w * m[i][3];
+    return {r[0], r[1], r[2]};
+  }
+  V3<R> translation() const { return {m[0][3], m[1][3], m[2][3]}; }
+};
+
+// ---------------------------------------------------------------------------
+// StepsizeInfo (common.h:93-164)
+// ---------------------------------------------------------------------------
+template <class R> struct Step {
+  R cone_angle, log1p_c, stepsize, near_, near_i, far_, dt_min, dt_max, t_min, t_max, i_min, i_max;
+  explicit Step(const fvc_stepsize_info& s)
+      : cone_angle(s.cone_angle), log1p_c(s.log1p_c), stepsize(s.stepsize), near_(s.near_), near_i(s.near_i),
+        far_(s.far_), dt_min(s.dt_min), dt_max(s.dt_max), t_min(s.t_min), t_max(s.t_max), i_min(s.i_min),
+        i_max(s.i_max) {}
+  static R pw(R b, R y);
+  static R lg(R x);
+  // t_from_step (common.h:137-149)
+  R t_from_step(R step) const {
+    if (cone_angle <= R(0)) return step * dt_min;
+    if (step >= i_max) return t_max + (step - i_max) * dt_max;
+    else if (step < i_min) return step * dt_min;
+    else return t_min * pw(R(1) + cone_angle, step - i_min);
+  }
+  // step_from_t (common.h:151-163)
+  R step_from_t(R t) const {
+    if (cone_angle <= R(0)) return t / dt_min;
+    if (t >= t_max) return (t - t_max) / dt_max + i_max;
+    else if (t < t_min) return t / dt_min;
+    else return lg(t / t_min) / log1p_c + i_min;
+  }
+  // calculate_stepsize (raymarch_common.h:274-284): clamp = fmin(fmax(x, a), b)
+  R dt(R t) const { return std::min(std::max(t * cone_angle, dt_min), dt_max); }
+  // valid_t_from_t (raymarch_common.h:286-296)
+  R valid_t_from_t(R t) const { return t_from_step(std::round(step_from_t(std::max(t, near_)) - near_i) + near_i); }
+};
+template <> float Step<float>::pw(float b, float y) { return det_powf(b, y); }
+template <> float Step<float>::lg(float x) { return det_logf(x); }
+template <> double Step<double>::pw(double b, double y) { return std::pow(b, y); }
+template <> double Step<double>::lg(double x) { return std::log(x); }
+
+// StepsizeInfo::setAdditionalInfo (common.h:109-128), double intermediates.
+void stepsize_setup(fvc_stepsize_info* s, const fvc_scene_info* sc) {
+  s->dt_min = s->stepsize;
+  const float res = (float)sc->grid_resolution;
+  s->dt_max = std::fmax(std::fmax((sc->aabb_to.x - sc->aabb_from.x) / res * 1.732f,
+                                  (sc->aabb_to.y - sc->aabb_from.y) / res * 1.732f),
+                        (sc->aabb_to.z - sc->aabb_from.z) / res * 1.732f);
+  s->t_min = (float)(((double)s->dt_min) / s->cone_angle);
+  const double d_i_min = 1.0 / s->cone_angle;
+  s->i_min = (float)d_i_min;
+  const double d_log1p_c = std::log(1.0 + s->cone_angle);
+  s->log1p_c = (float)d_log1p_c;
+  const double d_i_max = std::log(s->dt_max / (double)s->dt_min) / d_log1p_c + d_i_min;
+  s->i_max = (float)d_i_max;
+  s->t_max = (float)(s->t_min * std::pow(1.0 + s->cone_angle, d_i_max - d_i_min));
+  s->near_i = Step<float>(*s).step_from_t(s->near_);
+}
+
+// ---------------------------------------------------------------------------
+// Morton order + occupancy (raymarch_common.h:98-110, 660-710; loader.cu:20-55)
+// ---------------------------------------------------------------------------
+inline uint32_t expand_bits_3d(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+inline uint32_t morton3d(uint32_t x, uint32_t y, uint32_t z) {
+  return expand_bits_3d(x) | (expand_bits_3d(y) << 1) | (expand_bits_3d(z) << 2);
+}
+
+void morton_bitfield(const uint8_t* linear, int nlvl, int res, uint8_t* out) {
+  const int64_t vals = (int64_t)res * res * res;
+  std::memset(out, 0, (size_t)(vals * nlvl / 8));
+  for (int l = 0; l < nlvl; l++)
+    for (int x = 0; x < res; x++)
+      for (int y = 0; y < res; y++)
+        for (int z = 0; z < res; z++) {
+          if (!linear[l * vals + ((int64_t)x * res + y) * res + z]) continue;
+          const int64_t bit = l * vals + morton3d(x, y, z);
+          out[bit >> 3] |= (uint8_t)(1u << (bit & 7));
+        }
+}
+
+// grid_occupied_at<128, true> (raymarch_common.h:680-710).  The integer cell
+// index is clamped to [0,127]: (p-0.5)*2^-mip+0.5 can round to exactly 1.0,
+// which would address one cell past the level.
+template <class R>
+bool occupied(V3<R> p, const fvc_scene_info& sc, const uint8_t* bf) {
+  V3<R> u = p;
+  if (!sc.normalized) {
+    // world_to_aabb_unit (raymarch_common.h:660-666)
+    const R scale = (R)(1 << (sc.grid_nlvl - 1));
+    const V3<R> from = cvt<R>(sc.aabb_from), to = cvt<R>(sc.aabb_to);
+    u = (p - from / scale) / ((to - from) / scale);
+  }
+  // mip_from_unit (raymarch_common.h:668-677)
+  const R ax = std::fabs(u.x - R(0.5)), ay = std::fabs(u.y - R(0.5)), az = std::fabs(u.z - R(0.5));
+  const R mx = std::max(std::max(ax, ay), az);
+  int e;
+  std::frexp(mx, &e);
+  const int mip = std::max(0, e + 1);
+  if (mip >= sc.grid_nlvl) return false;
+  const R sc_m = std::ldexp(R(1), -mip);
+  const V3<R> pm = (u - R(0.5)) * sc_m + R(0.5);
+  int ix = (int)(pm.x * R(128)), iy = (int)(pm.y * R(128)), iz = (int)(pm.z * R(128));
+  ix = std::min(std::max(ix, 0), 127);
+  iy = std::min(std::max(iy, 0), 127);
+  iz = std::min(std::max(iz, 0), 127);
+  const uint32_t idx = morton3d((uint32_t)ix, (uint32_t)iy, (uint32_t)iz);
+  const int64_t byte = (int64_t)(idx >> 3) + ((int64_t)128 * 128 * 128 * mip) / 8;
+  return (bf[byte] >> (idx & 7)) & 1;
+}
+
+// apply_contraction (raymarch_common.h:610-653)
+template <class R> V3<R> contract(V3<R> p, const fvc_scene_info& sc) {
+  switch (sc.contraction_type) {
+    case FVC_CONTRACTION_AABB:
+      return (p - cvt<R>(sc.aabb_from)) / (cvt<R>(sc.aabb_to) - cvt<R>(sc.aabb_from));
+    case FVC_CONTRACTION_WARP_AABB_LINF:
+    case FVC_CONTRACTION_WARP_AABB_L2: {
+      const V3<R> from = cvt<R>(sc.contraction_aabb_from), to = cvt<R>(sc.contraction_aabb_to);
+      const V3<R> u = ((p - from) / (to - from)) * R(2) - R(1);
+      R n;
+      if (sc.contraction_type == FVC_CONTRACTION_WARP_AABB_LINF)
+        n = std::max(std::max(std::fabs(u.x), std::fabs(u.y)), std::fabs(u.z));
+      else
+        n = std::sqrt(dot3(u, u));
+      const R f = std::max(n, R(1));
+      const V3<R> c = (R(2) - R(1) / f) * (u / f);
+      return c / R(4) + R(0.5);
+    }
+    default:
+      return p;
+  }
+}
+
+// generateRayDir + generateRay, deterministic branch (raymarch_common.h:190-251)
+template <class R> struct Ray { V3<R> o, d; };
+template <class R> Ray<R> gen_ray(const fvc_camera_info& cam, R sx, R sy) {
+  const Mat34<R> c2w(cam.cam2world);
+  const R gl = cam.is_open_gl ? R(-1) : R(1);
+  const R dx = (sx - (R)cam.principal.x) / (R)cam.focal.x;
+  const R dy = (sy - (R)cam.principal.y) / (R)cam.focal.y * gl;
+  Ray<R> r;
+  r.o = c2w.translation();
+  r.d = normalize(c2w.transform(dx, dy, gl, R(0)));
+  return r;
+}
+
+// ray_aabb_intersect (raymarch_common.h:482-532)
+template <class R> bool ray_aabb(const Ray<R>& ray, V3<R> from, V3<R> to, R& nr, R& fr) {
+  R tmin = (from.x - ray.o.x) / ray.d.x, tmax = (to.x - ray.o.x) / ray.d.x;
+  if (tmin > tmax) std::swap(tmin, tmax);
+  R tymin = (from.y - ray.o.y) / ray.d.y, tymax = (to.y - ray.o.y) / ray.d.y;
+  if (tymin > tymax) std::swap(tymin, tymax);
+  if (tmin > tymax || tymin > tmax) { nr = fr = R(1e10); return false; }
+  if (tymin > tmin) tmin = tymin;
+  if (tymax < tmax) tmax = tymax;
+  R tzmin = (from.z - ray.o.z) / ray.d.z, tzmax = (to.z - ray.o.z) / ray.d.z;
+  if (tzmin > tzmax) std::swap(tzmin, tzmax);
+  if (tmin > tzmax || tzmin > tmax) { nr = fr = R(1e10); return false; }
+  if (tzmin > tmin) tmin = tzmin;
+  if (tzmax < tmax) tmax = tzmax;
+  nr = tmin; fr = tmax;
+  return true;
+}
+
+// initRayPayloads_split_kernel (renderer.cu:86-134): far, start t, alive
+template <class R> struct RayInit { Ray<R> ray; R far_; R start; bool alive; };
+template <class R> RayInit<R> init_ray(const fvc_raymarch_info& rm, const fvc_scene_info& sc, int px, int py) {
+  const Step<R> st(rm.stepsize_info);
+  RayInit<R> ri;
+  ri.ray = gen_ray<R>(rm.cam_info, (R)px + R(0.5), (R)py + R(0.5));
+  R t0, t1;
+  const bool hit = ray_aabb(ri.ray, cvt<R>(sc.aabb_from), cvt<R>(sc.aabb_to), t0, t1);
+  ri.far_ = hit ? std::min(t1, st.far_) : st.far_;
+  ri.start = hit ? st.valid_t_from_t(t0) : st.near_;
+  ri.alive = hit;
+  return ri;
+}
+
+// cam2froxel (raymarch_common.h:316-329)
+template <class R> V3<R> cam2froxel(V3<R> c, const fvc_camera_info& cam, const Step<R>& st, R& t) {
+  const R gl = cam.is_open_gl ? R(-1) : R(1);
+  const R z = c.z * gl;
+  t = std::sqrt(dot3(c, c));
+  return {c.x / z * (R)cam.focal.x + (R)cam.principal.x, c.y * gl / z * (R)cam.focal.y + (R)cam.principal.y,
+          (st.step_from_t(t) - st.near_i) * (c.z < R(0) ? R(-1) : R(1))};
+}
+
+// ---------------------------------------------------------------------------
+// Network: hash grid + density MLP + head (nerf_network.h:160-266)
+// ---------------------------------------------------------------------------
+struct Net {
+  fvc_net_desc d;
+  int L, F;
+  uint32_t T;
+  std::vector<double> scale_d;
+  std::vector<float> scale_f;
+  std::vector<uint32_t> res, size, offset;  // per level: resolution, entries, entry offset
+  std::vector<float> table;                 // fp16 values
+  std::vector<float> W1, W2, W3, H1, H2;    // fp16 values, row-major [out][in]
+  int n_pos_enc = 0, n_base = 0, n_head = 0;
+  int head_in = 32;
+  // per-layer absmax (FP8 mode)
+  float amax_w[5] = {0, 0, 0, 0, 0};
+};
+
+// HashGrid geometry shared with fvc_net_param_counts (tcnn conventions).
+bool net_layout(const fvc_net_desc& d, Net& n) {
+  if (d.n_levels <= 0 || d.n_levels > 32 || d.n_features_per_level != 2 || d.log2_hashmap_size < 8 ||
+      d.log2_hashmap_size > 24)
+    return false;
+  n.d = d;
+  n.L = d.n_levels;
+  n.F = d.n_features_per_level;
+  n.T = 1u << d.log2_hashmap_size;
+  n.scale_d.resize(n.L); n.scale_f.resize(n.L); n.res.resize(n.L); n.size.resize(n.L); n.offset.resize(n.L);
+  uint32_t off = 0;
+  for (int l = 0; l < n.L; l++) {
+    const double s = std::exp2(l * std::log2((double)d.per_level_scale)) * d.base_resolution - 1.0;
+    n.scale_d[l] = s;
+    n.scale_f[l] = (float)s;
+    const uint32_t r = (uint32_t)std::ceil((double)n.scale_f[l]) + 1u;
+    n.res[l] = r;
+    const uint64_t dense = (uint64_t)r * r * r;
+    uint64_t p = (dense + 7) / 8 * 8;
+    if (p > n.T) p = n.T;
+    n.size[l] = (uint32_t)p;
+    n.offset[l] = off;
+    off += (uint32_t)p;
+  }
+  n.n_pos_enc = (int)off * n.F;
+  n.n_base = 32 * d.base_width + d.base_width * d.base_width + d.base_width * 16;
+  n.n_head = 32 * d.head_width + d.head_width * 16;
+  return true;
+}
+
+inline uint32_t grid_index(const Net& n, int l, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t r = n.res[l];
+  const uint64_t dense = (uint64_t)r * r * r;
+  uint32_t idx;
+  if (dense <= n.size[l]) idx = x + y * r + z * r * r;
+  else idx = (x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u);
+  return idx % n.size[l];
+}
+
+template <class R> inline R q16(R v, Mode m) { return m >= M_EMU16 ? (R)round_f16((float)v) : v; }
+
+// storage rounding of MLP inputs/activations for each precision mode
+template <class R> inline R qact(R v, Mode m, float scale) {
+  switch (m) {
+    case M_EMU16: return (R)round_f16((float)v);
+    case M_EMU_BF16: return (R)round_bf16((float)v);
+    case M_EMU_FP8: return (R)(round_e4m3((float)v / scale) * scale);
+    default: return v;
+  }
+}
+
+// multiresolution hash-grid encoding (tcnn GridEncoding, called at
+// nerf_network.h:181); positions are clamped to [0,1]
+template <class R> void encode(const Net& n, V3<R> p, Mode m, R* out) {
+  const R pc[3] = {std::min(std::max(p.x, R(0)), R(1)), std::min(std::max(p.y, R(0)), R(1)),
+                   std::min(std::max(p.z, R(0)), R(1))};
+  for (int l = 0; l < n.L; l++) {
+    const R scale = m == M_F64 ? (R)n.scale_d[l] : (R)n.scale_f[l];
+    R frac[3];
+    uint32_t g[3];
+    for (int k = 0; k < 3; k++) {
+      const R pos = pc[k] * scale + R(0.5);
+      const R fl = std::floor(pos);
+      g[k] = (uint32_t)(int)fl;
+      frac[k] = pos - fl;
+    }
+    R acc[2] = {0, 0};
+    for (int c = 0; c < 8; c++) {
+      R w = 1;
+      uint32_t q[3];
+      for (int k = 0; k < 3; k++) {
+        if (c & (1 << k)) { w *= frac[k]; q[k] = g[k] + 1; }
+        else { w *= R(1) - frac[k]; q[k] = g[k]; }
+      }
+      const uint32_t idx = grid_index(n, l, q[0], q[1], q[2]);
+      const float* e = &n.table[((size_t)n.offset[l] + idx) * 2];
+      acc[0] += w * (R)e[0];
+      acc[1] += w * (R)e[1];
+    }
+    out[2 * l + 0] = q16(acc[0], m);
+    out[2 * l + 1] = q16(acc[1], m);
+  }
+}
+
+// per-layer FP8 scales: activations use a fixed absmax of 64 (the layer
+// inputs are bounded by the fp16 range of the reference), weights their own
+// absmax; both mapped to the E4M3 max of 448.
+inline float fp8_wscale(const Net& n, int layer) { return n.amax_w[layer] > 0 ? n.amax_w[layer] / 448.0f : 1.0f; }
+static const float FP8_ASCALE = 64.0f / 448.0f;
+
+template <class R>
+void layer(const Net& n, int li, const std::vector<float>& W, int n_out, int n_in, const R* x, R* y, bool relu,
+           Mode m, bool round_out_f16) {
+  const float ws = fp8_wscale(n, li);
+  for (int j = 0; j < n_out; j++) {
+    R acc = 0;
+    const float* w = &W[(size_t)j * n_in];
+    for (int k = 0; k < n_in; k++) {
+      R wv = (R)w[k];
+      if (m == M_EMU_BF16) wv = (R)round_bf16((float)wv);
+      if (m == M_EMU_FP8) wv = (R)(round_e4m3((float)wv / ws) * ws);
+      acc += wv * x[k];
+    }
+    if (relu) acc = std::max(acc, R(0));
+    y[j] = round_out_f16 ? q16(acc, m) : qact(acc, m, FP8_ASCALE);
+  }
+}
+
+// mlp_base (density / latent MLP): 32 -> W -> W -> 16 (nerf_network.h:199-200)
+template <class R> void latents(const Net& n, V3<R> p, Mode m, R* out16) {
+  R enc[32], h1[128], h2[128];
+  encode(n, p, m, enc);
+  R xin[32];
+  for (int k = 0; k < 32; k++) xin[k] = (m == M_EMU_BF16 || m == M_EMU_FP8) ? qact(enc[k], m, FP8_ASCALE) : enc[k];
+  const int W = n.d.base_width;
+  layer(n, 0, n.W1, W, 32, xin, h1, true, m, false);
+  layer(n, 1, n.W2, W, W, h1, h2, true, m, false);
+  layer(n, 2, n.W3, 16, W, h2, out16, false, m, true);
+}
+
+// SH degree 4 of dir01*2-1 (my_fully_fused_mlp.cu:206-234)
+template <class R> void sh4(V3<R> dir01, R* o) {
+  const R x = dir01.x * R(2) - R(1), y = dir01.y * R(2) - R(1), z = dir01.z * R(2) - R(1);
+  const R xy = x * y, xz = x * z, yz = y * z, x2 = x * x, y2 = y * y, z2 = z * z;
+  o[0] = R(0.28209479177387814);
+  o[1] = R(-0.48860251190291987) * y;
+  o[2] = R(0.48860251190291987) * z;
+  o[3] = R(-0.48860251190291987) * x;
+  o[4] = R(1.0925484305920792) * xy;
+  o[5] = R(-1.0925484305920792) * yz;
+  o[6] = R(0.94617469575755997) * z2 - R(0.31539156525251999);
+  o[7] = R(-1.0925484305920792) * xz;
+  o[8] = R(0.54627421529603959) * x2 - R(0.54627421529603959) * y2;
+  o[9] = R(0.59004358992664352) * y * (R(-3) * x2 + y2);
+  o[10] = R(2.8906114426405538) * xy * z;
+  o[11] = R(0.45704579946446572) * y * (R(1) - R(5) * z2);
+  o[12] = R(0.3731763325901154) * z * (R(5) * z2 - R(3));
+  o[13] = R(0.45704579946446572) * x * (R(1) - R(5) * z2);
+  o[14] = R(1.4453057213202769) * z * (x2 - y2);
+  o[15] = R(0.59004358992664352) * x * (-x2 + R(3) * y2);
+}
+
+// head: [latents, SH4(dir), 1.0 padding] -> W -> 3 (nerf_network.h:221-257)
+template <class R> void head(const Net& n, const R* lat, V3<R> dir01, Mode m, R* rgb) {
+  R xin[32], h[128], o[16];
+  const int NL = n.d.n_latents;
+  for (int k = 0; k < NL; k++) xin[k] = lat[k];
+  R sh[16];
+  sh4(dir01, sh);
+  for (int k = 0; k < 16; k++) xin[NL + k] = q16(sh[k], m);
+  for (int k = NL + 16; k < 32; k++) xin[k] = R(1);
+  if (m == M_EMU_BF16 || m == M_EMU_FP8)
+    for (int k = 0; k < 32; k++) xin[k] = qact(xin[k], m, FP8_ASCALE);
+  layer(n, 3, n.H1, n.d.head_width, 32, xin, h, true, m, false);
+  layer(n, 4, n.H2, 16, n.d.head_width, h, o, false, m, true);
+  rgb[0] = o[0]; rgb[1] = o[1]; rgb[2] = o[2];
+}
+
+// inference = latents + head + extract_density (nerf_network.h:160-171, 259-265)
+template <class R> void inference(const Net& n, V3<R> p, V3<R> dir01, Mode m, R* out4) {
+  R l16[16];
+  latents(n, p, m, l16);
+  R rgb[3];
+  head(n, l16 + n.d.latent_offset, dir01, m, rgb);
+  out4[0] = rgb[0]; out4[1] = rgb[1]; out4[2] = rgb[2]; out4[3] = l16[0];
+}
+
+// ---------------------------------------------------------------------------
+// Compositing (accumulate_performance_kernel, performance_renderer.cu:71-125)
+// ---------------------------------------------------------------------------
+template <class R> struct Accum {
+  R rgb[3] = {0, 0, 0};
+  R T = 1;
+  // returns false once the ray terminates (T < 1e-4)
+  bool add(const R* out4, R dt, R alpha_thre) {
+    const R density = std::exp(out4[3]);
+    R alpha = R(1) - std::exp(-density * dt);
+    alpha = std::min(std::max(alpha, R(0)), R(1));
+    if (alpha > alpha_thre) {
+      const R w = alpha * T;
+      for (int c = 0; c < 3; c++) rgb[c] = rgb[c] + w * (R(1) / (R(1) + std::exp(-out4[c])));
+      T = T * (R(1) - alpha);
+    }
+    return !(T < R(1e-4));
+  }
+};
+
+struct Model {
+  fvc_scene_info scene;
+  std::vector<uint8_t> bitfield;
+  Net net;
+};
+
+inline V3<float> unit_to_01(V3<float> d) { return (d + 1.0f) / 2.0f; }
+inline V3<double> unit_to_01(V3<double> d) { return (d + 1.0) / 2.0; }
+
+// Full render of a pixel crop (PerformanceRenderer::render,
+// performance_renderer.cu:214-286; per-ray semantics of sample_performance_kernel
+// + accumulate_performance_kernel).  out: float[h][w][4] (rgb, T).
+template <class R>
+void render_full(const Model& M, const fvc_raymarch_info& rm, Mode mode, int x0, int y0, int w, int h, float* out,
+                 int64_t* n_samples) {
+  const Step<R> st(rm.stepsize_info);
+  int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+  for (int64_t i = 0; i < (int64_t)w * h; i++) {
+    const int px = x0 + (int)(i % w), py = y0 + (int)(i / w);
+    const RayInit<R> ri = init_ray<R>(rm, M.scene, px, py);
+    Accum<R> acc;
+    if (ri.alive) {
+      const V3<R> dir01 = unit_to_01(ri.ray.d);
+      R t1 = ri.start;
+      while (true) {
+        const R t0 = t1;
+        const R dt = st.dt(t0);
+        t1 = t0 + dt;
+        const R tm = (t0 + t1) * R(0.5);
+        if (tm > ri.far_) break;
+        const V3<R> wp = ri.ray.o + tm * ri.ray.d;
+        if (!occupied(wp, M.scene, M.bitfield.data())) continue;
+        R o4[4];
+        inference(M.net, contract(wp, M.scene), dir01, mode, o4);
+        total++;
+        if (!acc.add(o4, dt, (R)M.scene.alpha_thre)) break;
+      }
+    }
+    float* o = out + i * 4;
+    o[0] = (float)acc.rgb[0]; o[1] = (float)acc.rgb[1]; o[2] = (float)acc.rgb[2]; o[3] = (float)acc.T;
+  }
+  if (n_samples) *n_samples = total;
+}
+
+// March only, FP32: per pixel the number of occupied samples and the first
+// max_per_ray t0 values (sample_performance_kernel without the network).
+void march_f32(const Model& M, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts, float* t0s) {
+  const Step<float> st(rm.stepsize_info);
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < (int64_t)W * H; i++) {
+    const RayInit<float> ri = init_ray<float>(rm, M.scene, (int)(i % W), (int)(i / W));
+    int cnt = 0;
+    if (ri.alive) {
+      float t1 = ri.start;
+      while (true) {
+        const float t0 = t1;
+        const float dt = st.dt(t0);
+        t1 = t0 + dt;
+        const float tm = (t0 + t1) * 0.5f;
+        if (tm > ri.far_) break;
+        if (!occupied(ri.ray.o + tm * ri.ray.d, M.scene, M.bitfield.data())) continue;
+        if (cnt < max_per_ray) t0s[i * max_per_ray + cnt] = t0;
+        cnt++;
+      }
+    }
+    counts[i] = cnt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Frustum brick cache (cache.h:12-74) and its fill (cache_renderer.cu:18-569,
+// 832-846, 931-1051) with deterministic slot order (sub-iteration, tile).
+// ---------------------------------------------------------------------------
+static const int BS = 8;  // Cache::BRICK_SIZE (cache.h:17)
+
+template <class R> struct Cache {
+  fvc_raymarch_info rm;
+  int fx = 0, fy = 0, fz = 0;  // frustum dims
+  int bx = 0, by = 0, bz = 0;  // brick array dims
+  int capacity = 0;
+  int n_used = 0;  // slots handed out (including overflowing requests)
+  std::vector<int32_t> table;
+  std::vector<R> known_to;
+  std::vector<R> lat;      // [slot][512][8]
+  std::vector<R> density;  // [slot][512]
+  std::vector<R> isset;    // [slot][512]
+  fvc_init_stats stats{};
+};
+
+// Frustum depth in steps: the reference's max_step_in_scene
+// (raymarch_common.h:298-303) padded to 4 (cache_renderer.cu:841-843),
+// extended to reach the farthest AABB corner seen from the fill camera so
+// that a camera outside the AABB keeps every brick inside the frustum.
+int frustum_depth(const fvc_raymarch_info& rm, const fvc_scene_info& sc) {
+  const Step<float> st(rm.stepsize_info);
+  const V3<float> from = cvt<float>(sc.aabb_from), to = cvt<float>(sc.aabb_to);
+  const V3<float> dg = to - from;
+  const float diag = std::sqrt(dot3(dg, dg));
+  int steps = (int)std::ceil(st.step_from_t(diag + st.near_));
+  const Mat34<float> c2w(rm.cam_info.cam2world);
+  const V3<float> o = c2w.translation();
+  float tf = 0.0f;
+  for (int c = 0; c < 8; c++) {
+    const V3<float> corner = {(c & 1) ? to.x : from.x, (c & 2) ? to.y : from.y, (c & 4) ? to.z : from.z};
+    const V3<float> v = corner - o;
+    tf = std::max(tf, std::sqrt(dot3(v, v)));
+  }
+  tf = std::min(tf, st.far_);
+  const int steps_cam = (int)std::ceil(st.step_from_t(tf)) + BS;
+  steps = std::max(steps, steps_cam);
+  return (steps + 3) / 4 * 4;
+}
+
+int default_capacity(int n_rays) {
+  const int est_samples = n_rays * 32;                 // INIT_ESTIMATED_N_SAMPLES_PER_RAY
+  const int per_brick = (int)(0.5f * BS * BS * BS);    // INIT_ESTIMATED_BRICK_SPARSITY * 8^3
+  const int est_bricks = est_samples / per_brick;
+  const int per_layer = 128 * 128;                     // DATA_ARRAY_BRICKS_XY
+  return (est_bricks + per_layer - 1) / per_layer * per_layer;
+}
+
+template <class R> struct FillRay { Ray<R> ray; R far_; R T; bool alive; };
+template <class R> struct FillSample { int lane; int step; R t0; R out[16]; };
+template <class R> struct FillTile {
+  int slab = 0;
+  bool alive = false;
+  int n_bricks = 0;
+  FillRay<R> rays[64];
+  std::vector<FillSample<R>> samples;
+  int processed = 0;  // this sub-iteration: 1 = has samples and survived the first pass
+  int visible = 0;
+};
+
+template <class R>
+Cache<R>* cache_fill(const Model& M, const fvc_raymarch_info& rm, Mode mode, int capacity) {
+  auto* C = new Cache<R>();
+  C->rm = rm;
+  const fvc_scene_info& sc = M.scene;
+  const Step<R> st(rm.stepsize_info);
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+  C->fx = W; C->fy = H; C->fz = frustum_depth(rm, sc);
+  C->bx = (W + BS - 1) / BS; C->by = (H + BS - 1) / BS; C->bz = (C->fz + BS - 1) / BS;
+  C->capacity = capacity > 0 ? capacity : default_capacity(W * H);
+  C->table.assign((size_t)C->bx * C->by * C->bz, -1);
+  C->known_to.assign((size_t)W * H, R(0));
+  const int n_tiles = C->bx * C->by;
+  std::vector<FillTile<R>> tiles(n_tiles);
+
+  // initPayloads_cache_init_kernel (cache_renderer.cu:18-81)
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int t = 0; t < n_tiles; t++) {
+    FillTile<R>& T = tiles[t];
+    const int tx = t % C->bx, ty = t / C->bx;
+    int n_alive = 0, min_start = 1 << 30;
+    for (int l = 0; l < 64; l++) {
+      const int px = tx * BS + (l % BS), py = ty * BS + (l / BS);
+      FillRay<R>& r = T.rays[l];
+      r.ray = gen_ray<R>(rm.cam_info, (R)px + R(0.5), (R)py + R(0.5));
+      R a0, a1;
+      const bool hit = ray_aabb(r.ray, cvt<R>(sc.aabb_from), cvt<R>(sc.aabb_to), a0, a1);
+      r.far_ = hit ? std::min(a1, st.far_) : st.far_;
+      r.alive = hit;
+      r.T = 1;
+      n_alive += hit;
+      const int s = std::max(0, (int)(st.step_from_t(hit ? a0 : st.near_) - st.near_i));
+      min_start = std::min(min_start, s);
+    }
+    T.slab = min_start / BS;
+    T.alive = n_alive > 0;
+  }
+
+  int64_t n_eval = 0;
+  int subiter = 0;
+  bool overflow_any = false;
+  for (; subiter < 1000; subiter++) {
+    std::vector<int> alive;
+    for (int t = 0; t < n_tiles; t++)
+      if (tiles[t].alive) alive.push_back(t);
+    if (alive.empty()) break;
+
+    // sample_cache_init_first_kernel + second kernel + inferenceLatents +
+    // first loop of accumulateNew_cache_init_second_kernel
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : n_eval)
+    for (size_t ai = 0; ai < alive.size(); ai++) {
+      FillTile<R>& T = tiles[alive[ai]];
+      T.samples.clear();
+      T.processed = 0;
+      T.visible = 0;
+      int total = 0;
+      while (true) {
+        int n_alive = 0;
+        for (int l = 0; l < 64; l++) n_alive += T.rays[l].alive;
+        if (n_alive <= 0) break;
+        if (T.slab >= C->bz) {  // beyond the frustum: the column ends here
+          for (int l = 0; l < 64; l++) T.rays[l].alive = false;
+          break;
+        }
+        for (int l = 0; l < 64; l++) {
+          FillRay<R>& r = T.rays[l];
+          R t1 = st.t_from_step((R)(T.slab * BS) + st.near_i);
+          for (int j = 0; j < BS; j++) {
+            const R t0 = t1;
+            const R dt = st.dt(t0);
+            t1 = t0 + dt;
+            const R tm = (t0 + t1) * R(0.5);
+            if (tm > r.far_) { r.alive = false; continue; }
+            if (occupied(r.ray.o + tm * r.ray.d, sc, M.bitfield.data())) {
+              FillSample<R> s;
+              s.lane = l; s.step = j; s.t0 = t0;
+              T.samples.push_back(s);
+            }
+          }
+        }
+        total = (int)T.samples.size();
+        if (total > 0) break;
+        T.slab++;
+      }
+      int n_alive = 0;
+      for (int l = 0; l < 64; l++) n_alive += T.rays[l].alive;
+      T.alive = n_alive > 0;
+      if (total == 0 || !T.alive) continue;  // accumulateNew returns early
+      T.processed = 1;
+      n_eval += total;
+      for (auto& s : T.samples) {
+        const FillRay<R>& r = T.rays[s.lane];
+        const R dt = st.dt(s.t0);
+        const R tm = (s.t0 + (s.t0 + dt)) * R(0.5);
+        latents(M.net, contract(r.ray.o + tm * r.ray.d, sc), mode, s.out);
+      }
+      // transmittance walk in step order per ray (samples are stored per ray in step order)
+      for (auto& s : T.samples) {
+        FillRay<R>& r = T.rays[s.lane];
+        const R dt = st.dt(s.t0);
+        const R density = std::exp(s.out[0]);
+        R alpha = R(1) - std::exp(-density * dt);
+        alpha = std::min(std::max(alpha, R(0)), R(1));
+        if (alpha > (R)sc.alpha_thre) { r.T = r.T * (R(1) - alpha); T.visible++; }
+        if (r.T < R(1e-4)) r.alive = false;
+      }
+    }
+
+    // no sample in any alive tile: the loop ends (cache_renderer.cu:1017-1018)
+    int64_t round_samples = 0;
+    for (size_t ai = 0; ai < alive.size(); ai++) round_samples += (int64_t)tiles[alive[ai]].samples.size();
+    if (round_samples == 0) break;
+
+    // brick allocation in tile order (replaces the atomicAdd at cache_renderer.cu:420)
+    std::vector<int> slot_of(alive.size(), -1);
+    for (size_t ai = 0; ai < alive.size(); ai++) {
+      FillTile<R>& T = tiles[alive[ai]];
+      if (!T.processed || T.visible <= 0) continue;
+      slot_of[ai] = C->n_used++;
+    }
+    const int new_slots = C->n_used;
+    const size_t need = (size_t)std::min(new_slots, C->capacity);
+    if (C->density.size() < need * 512) {
+      C->lat.resize(need * 512 * 8, R(0));
+      C->density.resize(need * 512, R(0));
+      C->isset.resize(need * 512, R(0));
+    }
+
+#pragma omp parallel for schedule(dynamic, 4)
+    for (size_t ai = 0; ai < alive.size(); ai++) {
+      FillTile<R>& T = tiles[alive[ai]];
+      if (!T.processed) continue;
+      bool overflow = false;
+      if (T.visible > 0) {
+        const int slot = slot_of[ai];
+        overflow = slot >= C->capacity;
+        const int tx = alive[ai] % C->bx, ty = alive[ai] / C->bx;
+        C->table[((size_t)T.slab * C->by + ty) * C->bx + tx] = overflow ? -1 : slot;
+        if (!overflow) {
+          T.n_bricks++;
+          for (auto& s : T.samples) {
+            const int vx = s.lane % BS, vy = s.lane / BS, vz = s.step;
+            const size_t v = (size_t)slot * 512 + (size_t)(vz * 64 + vy * 8 + vx);
+            C->density[v] = std::exp(s.out[0]);
+            C->isset[v] = R(1);
+            for (int i = 0; i < 8; i++) C->lat[v * 8 + i] = s.out[M.net.d.latent_offset + i];
+          }
+        }
+      }
+      int n_alive = 0;
+      for (int l = 0; l < 64; l++) n_alive += T.rays[l].alive;
+      if (!overflow) T.slab++;
+      T.alive = !overflow && n_alive > 0;
+      if (overflow) {
+#pragma omp critical
+        overflow_any = true;
+      }
+    }
+  }
+
+  // writeImageBuffer_cache_init_kernel2 (cache_renderer.cu:530-569)
+  for (int t = 0; t < n_tiles; t++) {
+    const int tx = t % C->bx, ty = t / C->bx;
+    const R to = st.t_from_step((R)(tiles[t].slab * BS) + st.near_i);
+    for (int l = 0; l < 64; l++) {
+      const int px = tx * BS + l % BS, py = ty * BS + l / BS;
+      if (px < W && py < H) C->known_to[(size_t)py * W + px] = to;
+    }
+  }
+  int n_set = 0;
+  for (int32_t v : C->table) n_set += v >= 0;
+  C->stats.n_bricks_set = n_set;
+  C->stats.n_bricks_reserved = C->capacity;
+  C->stats.overflow = overflow_any;
+  C->stats.n_subiters = subiter;
+  C->stats.n_samples_evaluated = n_eval;
+  C->stats.samples_ppx = (float)n_eval / (float)(W * H);
+  return C;
+}
+
+// Trilinear cache lookup (generateNetworkInput_cache_shfl_perf_kernel_cg,
+// cache_performance_renderer.cu:678-787, FIRST_LATENT_IS_DENSITY = false):
+// head input row 0 = log(max(sum w*sigma, 1e-6)), rows 1..8 = sum w*l / sum w*isset.
+template <class R> void cache_lookup(const Cache<R>& C, V3<R> f, Mode mode, R* head_in9) {
+  const R fc[3] = {f.x - R(0.5), f.y - R(0.5), f.z - R(0.5)};
+  R i0[3], lt[3];
+  for (int k = 0; k < 3; k++) { i0[k] = std::floor(fc[k]); lt[k] = fc[k] - i0[k]; }
+  R wd[8], dd[8], ld[8][8];
+  for (int c = 0; c < 8; c++) {
+    const int off[3] = {(c >> 2) & 1, (c >> 1) & 1, c & 1};
+    R p[3], wk[3];
+    for (int k = 0; k < 3; k++) {
+      p[k] = i0[k] + (R)off[k];
+      wk[k] = (R(1) - (R)off[k]) * (R(1) - R(2) * lt[k]) + lt[k];
+    }
+    const R w = wk[0] * wk[1] * wk[2];
+    wd[c] = dd[c] = 0;
+    for (int i = 0; i < 8; i++) ld[c][i] = 0;
+    const bool inside = p[0] >= R(0) && p[0] < (R)C.fx && p[1] >= R(0) && p[1] < (R)C.fy && p[2] >= R(0) &&
+                        p[2] < (R)C.fz;
+    if (!inside) continue;
+    const int vx = (int)p[0], vy = (int)p[1], vz = (int)p[2];
+    const int slot = C.table[((size_t)(vz / BS) * C.by + vy / BS) * C.bx + vx / BS];
+    if (slot < 0) continue;
+    const size_t v = (size_t)slot * 512 + (size_t)((vz % BS) * 64 + (vy % BS) * 8 + vx % BS);
+    wd[c] = w * C.isset[v];
+    dd[c] = w * C.density[v];
+    for (int i = 0; i < 8; i++) ld[c][i] = w * C.lat[v * 8 + i];
+  }
+  auto tree = [](const R* a) { return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])); };
+  const R wsum = tree(wd);
+  const R dsum = tree(dd);
+  head_in9[0] = q16(std::log(std::max(dsum, R(1e-6))), mode);
+  for (int i = 0; i < 8; i++) {
+    R a[8];
+    for (int c = 0; c < 8; c++) a[c] = ld[c][i];
+    head_in9[1 + i] = q16(tree(a) / wsum, mode);
+  }
+}
+
+// Cached render of a pixel crop (CachePerformanceRenderer::render,
+// cache_performance_renderer.cu:22-177 classification, 198-313 network
+// inputs, 789-908 merged compositing).
+template <class R>
+void render_cached(const Model& M, const Cache<R>& C, const fvc_raymarch_info& rm, Mode mode, int x0, int y0, int w,
+                   int h, float* out, int64_t* n_res, int64_t* n_cache) {
+  const Step<R> st(rm.stepsize_info);
+  const Step<R> cst(C.rm.stepsize_info);
+  const Mat34<R> w2c(C.rm.cam_info.world2cam);
+  const fvc_scene_info& sc = M.scene;
+  int64_t tr = 0, tc = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : tr, tc)
+  for (int64_t i = 0; i < (int64_t)w * h; i++) {
+    const int px = x0 + (int)(i % w), py = y0 + (int)(i / w);
+    const RayInit<R> ri = init_ray<R>(rm, sc, px, py);
+    Accum<R> acc;
+    if (ri.alive) {
+      const V3<R> dir01 = unit_to_01(ri.ray.d);
+      // transformRay (raymarch_common.h:253-262)
+      const V3<R> oc = w2c.transform(ri.ray.o.x, ri.ray.o.y, ri.ray.o.z, R(1));
+      const V3<R> dc = w2c.transform(ri.ray.d.x, ri.ray.d.y, ri.ray.d.z, R(0));
+      R t1 = ri.start;
+      while (true) {
+        const R t0 = t1;
+        const R dt = st.dt(t0);
+        t1 = t0 + dt;
+        const R tm = (t0 + t1) * R(0.5);
+        if (tm > ri.far_) break;
+        const V3<R> wp = ri.ray.o + tm * ri.ray.d;
+        if (!occupied(wp, sc, M.bitfield.data())) continue;
+        R tf;
+        const V3<R> f = cam2froxel(oc + tm * dc, C.rm.cam_info, cst, tf);
+        const bool in_fov = f.x >= R(0) && f.x < (R)C.fx && f.y >= R(0) && f.y < (R)C.fy;
+        const bool inside = in_fov && f.z >= R(0) && f.z < (R)C.fz;
+        bool unsampled = true;
+        if (in_fov) {
+          const R known = C.known_to[(size_t)(int)f.y * C.fx + (int)f.x];
+          unsampled = tf < cst.near_ || tf > known;
+        }
+        const R tm2 = t0 + dt * R(0.5);
+        R o4[4];
+        if (!inside || unsampled) {
+          // resample: full network (generateNetworkInput_resample_perf_kernel, 198-240)
+          inference(M.net, contract(ri.ray.o + tm2 * ri.ray.d, sc), dir01, mode, o4);
+          tr++;
+        } else {
+          const int vx = (int)std::floor(f.x), vy = (int)std::floor(f.y), vz = (int)std::floor(f.z);
+          const int slot = C.table[((size_t)(vz / BS) * C.by + vy / BS) * C.bx + vx / BS];
+          if (slot < 0) continue;
+          const size_t v = (size_t)slot * 512 + (size_t)((vz % BS) * 64 + (vy % BS) * 8 + vx % BS);
+          if (!(C.density[v] > R(0))) continue;
+          R tf2;
+          const V3<R> f2 = cam2froxel(oc + tm2 * dc, C.rm.cam_info, cst, tf2);
+          R hin[9];
+          cache_lookup(C, f2, mode, hin);
+          R rgb[3];
+          head(M.net, hin + 1, dir01, mode, rgb);
+          o4[0] = rgb[0]; o4[1] = rgb[1]; o4[2] = rgb[2]; o4[3] = hin[0];
+          tc++;
+        }
+        if (!acc.add(o4, dt, (R)sc.alpha_thre)) break;
+      }
+    }
+    float* o = out + i * 4;
+    o[0] = (float)acc.rgb[0]; o[1] = (float)acc.rgb[1]; o[2] = (float)acc.rgb[2]; o[3] = (float)acc.T;
+  }
+  if (n_res) *n_res = tr;
+  if (n_cache) *n_cache = tc;
+}
+
+template <class R> static void fill_info(const Cache<R>& C, fvc_cache_info* info) {
+  info->frustum_x = C.fx; info->frustum_y = C.fy; info->frustum_z = C.fz;
+  info->bricks_x = C.bx; info->bricks_y = C.by; info->bricks_z = C.bz;
+  info->capacity_bricks = C.capacity;
+  info->n_bricks_used = std::min(C.n_used, C.capacity);
+  info->initialized = 1;
+  info->brick_size = BS;
+}
+
+struct CacheHandle {
+  int mode;
+  std::unique_ptr<Cache<double>> d;
+  std::unique_ptr<Cache<float>> f;
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+// ---------------------------------------------------------------------------
+// extern "C" surface for ctypes
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+void orc_stepsize_setup(fvc_stepsize_info* s, const fvc_scene_info* sc) { stepsize_setup(s, sc); }
+float orc_det_logf(float x) { return det_logf(x); }
+float orc_det_exp2f(float x) { return det_exp2f(x); }
+float orc_t_from_step_f32(const fvc_stepsize_info* s, float step) { return Step<float>(*s).t_from_step(step); }
+float orc_step_from_t_f32(const fvc_stepsize_info* s, float t) { return Step<float>(*s).step_from_t(t); }
+double orc_t_from_step_f64(const fvc_stepsize_info* s, double step) { return Step<double>(*s).t_from_step(step); }
+double orc_step_from_t_f64(const fvc_stepsize_info* s, double t) { return Step<double>(*s).step_from_t(t); }
+float orc_valid_t_from_t_f32(const fvc_stepsize_info* s, float t) { return Step<float>(*s).valid_t_from_t(t); }
+uint32_t orc_morton3d(uint32_t x, uint32_t y, uint32_t z) { return morton3d(x, y, z); }
+uint16_t orc_f32_to_f16(float f) { return f32_to_f16(f); }
+float orc_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
+uint16_t orc_f32_to_bf16(float f) { return f32_to_bf16(f); }
+uint8_t orc_f32_to_e4m3(float f) { return f32_to_e4m3(f); }
+float orc_e4m3_to_f32(uint8_t b) { return e4m3_to_f32(b); }
+
+void orc_morton_bitfield(const uint8_t* linear, int nlvl, int res, uint8_t* out) { morton_bitfield(linear, nlvl, res, out); }
+
+void orc_occupied(const fvc_scene_info* sc, const uint8_t* bf, const float* pts, int64_t n, int f64, uint8_t* out) {
+  for (int64_t i = 0; i < n; i++) {
+    if (f64) out[i] = occupied(V3<double>{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, *sc, bf);
+    else out[i] = occupied(V3<float>{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, *sc, bf);
+  }
+}
+
+void orc_contract(const fvc_scene_info* sc, const float* pts, int64_t n, int f64, double* out) {
+  for (int64_t i = 0; i < n; i++) {
+    if (f64) {
+      V3<double> c = contract(V3<double>{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, *sc);
+      out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    } else {
+      V3<float> c = contract(V3<float>{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, *sc);
+      out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+    }
+  }
+}
+
+void orc_sh4(const double* dir01, int64_t n, double* out) {
+  for (int64_t i = 0; i < n; i++) sh4(V3<double>{dir01[3 * i], dir01[3 * i + 1], dir01[3 * i + 2]}, out + 16 * i);
+}
+
+// rays of a pixel crop: origin, dir (float), far, start, alive
+void orc_init_rays(const fvc_scene_info* sc, const fvc_raymarch_info* rm, int x0, int y0, int w, int h, float* o6,
+                   float* far_start, uint8_t* alive) {
+  for (int64_t i = 0; i < (int64_t)w * h; i++) {
+    RayInit<float> ri = init_ray<float>(*rm, *sc, x0 + (int)(i % w), y0 + (int)(i / w));
+    o6[6 * i + 0] = ri.ray.o.x; o6[6 * i + 1] = ri.ray.o.y; o6[6 * i + 2] = ri.ray.o.z;
+    o6[6 * i + 3] = ri.ray.d.x; o6[6 * i + 4] = ri.ray.d.y; o6[6 * i + 5] = ri.ray.d.z;
+    far_start[2 * i] = ri.far_; far_start[2 * i + 1] = ri.start;
+    alive[i] = ri.alive;
+  }
+}
+
+int orc_net_param_counts(const fvc_net_desc* d, int64_t* a, int64_t* b, int64_t* c) {
+  Net n;
+  if (!net_layout(*d, n)) return -1;
+  *a = n.n_pos_enc; *b = n.n_base; *c = n.n_head;
+  return 0;
+}
+
+Model* orc_model_create(const fvc_scene_info* sc, const fvc_net_desc* d, const uint8_t* occ_linear,
+                        const uint16_t* p_pos, int64_t n_pos, const uint16_t* p_base, int64_t n_base,
+                        const uint16_t* p_head, int64_t n_head) {
+  auto* M = new Model();
+  M->scene = *sc;
+  if (!net_layout(*d, M->net) || n_pos != M->net.n_pos_enc || n_base != M->net.n_base || n_head != M->net.n_head ||
+      sc->grid_resolution != 128) {
+    delete M;
+    return nullptr;
+  }
+  const int64_t vals = (int64_t)128 * 128 * 128;
+  M->bitfield.resize((size_t)(vals * sc->grid_nlvl / 8));
+  morton_bitfield(occ_linear, sc->grid_nlvl, 128, M->bitfield.data());
+  Net& n = M->net;
+  n.table.resize(n_pos);
+  for (int64_t i = 0; i < n_pos; i++) n.table[i] = f16_to_f32(p_pos[i]);
+  const int W = d->base_width, HW = d->head_width;
+  auto take = [](const uint16_t* p, int64_t off, int64_t cnt, std::vector<float>& dst) {
+    dst.resize(cnt);
+    for (int64_t i = 0; i < cnt; i++) dst[i] = f16_to_f32(p[off + i]);
+  };
+  take(p_base, 0, 32 * W, n.W1);
+  take(p_base, 32 * W, (int64_t)W * W, n.W2);
+  take(p_base, 32 * W + (int64_t)W * W, 16 * W, n.W3);
+  take(p_head, 0, 32 * HW, n.H1);
+  take(p_head, 32 * HW, 16 * HW, n.H2);
+  const std::vector<float>* Ws[5] = {&n.W1, &n.W2, &n.W3, &n.H1, &n.H2};
+  for (int l = 0; l < 5; l++) {
+    float a = 0;
+    for (float v : *Ws[l]) a = std::max(a, std::fabs(v));
+    n.amax_w[l] = a;
+  }
+  return M;
+}
+void orc_model_destroy(Model* M) { delete M; }
+void orc_model_bitfield(const Model* M, uint8_t* out) { std::memcpy(out, M->bitfield.data(), M->bitfield.size()); }
+
+void orc_encode(const Model* M, const float* pos, int64_t n, int mode, double* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    if (mode == M_F64) {
+      double o[32];
+      encode(M->net, V3<double>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, (Mode)mode, o);
+      for (int k = 0; k < 32; k++) out[32 * i + k] = o[k];
+    } else {
+      float o[32];
+      encode(M->net, V3<float>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, (Mode)mode, o);
+      for (int k = 0; k < 32; k++) out[32 * i + k] = o[k];
+    }
+  }
+}
+
+void orc_latents(const Model* M, const float* pos, int64_t n, int mode, double* out16) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    if (mode == M_F64) {
+      double o[16];
+      latents(M->net, V3<double>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, (Mode)mode, o);
+      for (int k = 0; k < 16; k++) out16[16 * i + k] = o[k];
+    } else {
+      float o[16];
+      latents(M->net, V3<float>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, (Mode)mode, o);
+      for (int k = 0; k < 16; k++) out16[16 * i + k] = o[k];
+    }
+  }
+}
+
+void orc_inference(const Model* M, const float* pos, const float* dir, int64_t n, int mode, double* out4) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    if (mode == M_F64) {
+      double o[4];
+      inference(M->net, V3<double>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]},
+                V3<double>{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]}, (Mode)mode, o);
+      for (int k = 0; k < 4; k++) out4[4 * i + k] = o[k];
+    } else {
+      float o[4];
+      inference(M->net, V3<float>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]},
+                V3<float>{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]}, (Mode)mode, o);
+      for (int k = 0; k < 4; k++) out4[4 * i + k] = o[k];
+    }
+  }
+}
+
+// head on given head inputs: head_in16 rows [0]=log sigma, [1..8] latents
+void orc_head(const Model* M, const float* head_in16, const float* dir, int64_t n, int mode, double* out4) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    if (mode == M_F64) {
+      double lat[8], rgb[3];
+      for (int k = 0; k < 8; k++) lat[k] = head_in16[16 * i + 1 + k];
+      head(M->net, lat, V3<double>{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]}, (Mode)mode, rgb);
+      for (int k = 0; k < 3; k++) out4[4 * i + k] = rgb[k];
+      out4[4 * i + 3] = head_in16[16 * i];
+    } else {
+      float lat[8], rgb[3];
+      for (int k = 0; k < 8; k++) lat[k] = head_in16[16 * i + 1 + k];
+      head(M->net, lat, V3<float>{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]}, (Mode)mode, rgb);
+      for (int k = 0; k < 3; k++) out4[4 * i + k] = rgb[k];
+      out4[4 * i + 3] = head_in16[16 * i];
+    }
+  }
+}
+
+void orc_march(const Model* M, const fvc_raymarch_info* rm, int max_per_ray, int32_t* counts, float* t0s) {
+  march_f32(*M, *rm, max_per_ray, counts, t0s);
+}
+
+void orc_render_full(const Model* M, const fvc_raymarch_info* rm, int mode, int x0, int y0, int w, int h, float* out,
+                     int64_t* n_samples) {
+  if (mode == M_F64) render_full<double>(*M, *rm, (Mode)mode, x0, y0, w, h, out, n_samples);
+  else render_full<float>(*M, *rm, (Mode)mode, x0, y0, w, h, out, n_samples);
+}
+
+CacheHandle* orc_cache_fill(const Model* M, const fvc_raymarch_info* rm, int mode, int capacity, fvc_init_stats* stats) {
+  auto* h = new CacheHandle();
+  h->mode = mode;
+  if (mode == M_F64) {
+    h->d.reset(cache_fill<double>(*M, *rm, (Mode)mode, capacity));
+    if (stats) *stats = h->d->stats;
+  } else {
+    h->f.reset(cache_fill<float>(*M, *rm, (Mode)mode, capacity));
+    if (stats) *stats = h->f->stats;
+  }
+  return h;
+}
+void orc_cache_destroy(CacheHandle* h) { delete h; }
+
+void orc_cache_info(const CacheHandle* h, fvc_cache_info* info) {
+  if (h->d) fill_info(*h->d, info); else fill_info(*h->f, info);
+}
+void orc_cache_table(const CacheHandle* h, int32_t* out) {
+  const auto& t = h->d ? h->d->table : h->f->table;
+  std::memcpy(out, t.data(), t.size() * 4);
+}
+void orc_cache_known(const CacheHandle* h, float* out) {
+  if (h->d) for (size_t i = 0; i < h->d->known_to.size(); i++) out[i] = (float)h->d->known_to[i];
+  else std::memcpy(out, h->f->known_to.data(), h->f->known_to.size() * 4);
+}
+// pool of the first n slots as float records: [slot][512][10] = 8 latents, density, isset
+void orc_cache_pool(const CacheHandle* h, int64_t n_slots, float* out) {
+  for (int64_t v = 0; v < n_slots * 512; v++) {
+    if (h->d) {
+      for (int i = 0; i < 8; i++) out[v * 10 + i] = (float)h->d->lat[v * 8 + i];
+      out[v * 10 + 8] = (float)h->d->density[v]; out[v * 10 + 9] = (float)h->d->isset[v];
+    } else {
+      for (int i = 0; i < 8; i++) out[v * 10 + i] = h->f->lat[v * 8 + i];
+      out[v * 10 + 8] = h->f->density[v]; out[v * 10 + 9] = h->f->isset[v];
+    }
+  }
+}
+
+void orc_render_cached(const Model* M, const CacheHandle* h, const fvc_raymarch_info* rm, int x0, int y0, int w, int hh,
+                       float* out, int64_t* n_res, int64_t* n_cache) {
+  if (h->d) render_cached<double>(*M, *h->d, *rm, (Mode)h->mode, x0, y0, w, hh, out, n_res, n_cache);
+  else render_cached<float>(*M, *h->f, *rm, (Mode)h->mode, x0, y0, w, hh, out, n_res, n_cache);
+}
+
+// trilinear lookup alone at given froxel points (head inputs rows 0..8)
+void orc_cache_lookup(const CacheHandle* h, const float* froxel, int64_t n, double* out9) {
+  for (int64_t i = 0; i < n; i++) {
+    if (h->d) {
+      double o[9];
+      cache_lookup(*h->d, V3<double>{froxel[3 * i], froxel[3 * i + 1], froxel[3 * i + 2]}, (Mode)h->mode, o);
+      for (int k = 0; k < 9; k++) out9[9 * i + k] = o[k];
+    } else {
+      float o[9];
+      cache_lookup(*h->f, V3<float>{froxel[3 * i], froxel[3 * i + 1], froxel[3 * i + 2]}, (Mode)h->mode, o);
+      for (int k = 0; k < 9; k++) out9[9 * i + k] = o[k];
+    }
+  }
+}
+
+}  // extern "C"
