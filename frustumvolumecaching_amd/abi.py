@@ -117,6 +117,11 @@ class CacheInfo(C.Structure):
         "capacity_bricks", "n_bricks_used", "initialized", "brick_size")]
 
 
+class KernelStats(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_int64), ("total_ms", C.c_double),
+                ("units", C.c_double), ("alg_bytes", C.c_double), ("alg_flops", C.c_double)]
+
+
 # voxel record of the brick pool: 8 fp16 latents, density, isset, 2 pad floats
 VOXEL_DTYPE = np.dtype([("lat", np.uint16, 8), ("density", np.float32), ("isset", np.float32),
                         ("pad", np.float32, 2)])

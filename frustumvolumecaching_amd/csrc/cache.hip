@@ -341,46 +341,66 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats)
   r.f_out16.ensure(f_cap * 16 * sizeof(uint16_t));
 
   zero_counters(r, s);
+  Profiler& P = r.prof;
+  hipEvent_t ev = P.begin(s);
   hipLaunchKernelGGL(k_fill_init, dim3(n_tiles), dim3(64), 0, s, rm, cvt(m.scene.aabb_from), cvt(m.scene.aabb_to),
                      C.bx, r.f_rays.as<FillRay>(), r.f_tiles.as<FillTile>());
   FVC_HIP(hipGetLastError());
+  P.end(ev, s, "fill_init", n_tiles * 64.0, n_tiles * (64.0 * sizeof(FillRay) + sizeof(FillTile)), 0);
   int* cnt = r.counters.as<int>();
   int subiter = 0;
   int64_t n_eval = 0;
   for (; subiter < 1000; subiter++) {  // CacheRenderer::MAX_SUBITER
+    ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_list, dim3(1), dim3(1024), 0, s, n_tiles, r.f_tiles.as<FillTile>(), r.f_list.as<int>(),
                        cnt + CNT_TILES);
     FVC_HIP(hipGetLastError());
+    P.end(ev, s, "fill_list", n_tiles, n_tiles * 36.0, 0);
     const int n_alive = read_counter(r, CNT_TILES, s);
     if (n_alive == 0) break;
     FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, sizeof(int), s));
+    ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_sample, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), rm.stepsize_info, m.occ,
                        m.con, m.bitfield.as<uint8_t>(), C.bz, r.f_rays.as<FillRay>(), r.f_tiles.as<FillTile>(),
                        r.f_pos.as<float>(), r.f_t0.as<float>(), r.f_meta.as<int>(), cnt + CNT_SAMPLES);
     FVC_HIP(hipGetLastError());
+    const int hs = P.end(ev, s, "fill_sample");
     const int n_samples = read_counter(r, CNT_SAMPLES, s);
+    P.work(hs, n_samples, n_alive * 64.0 * (2 * sizeof(FillRay) + 8) + 20.0 * n_samples, 0);
     if (n_samples <= 0) break;
     n_eval += n_samples;
+    ev = P.begin(s);
     launch_network_latents(m, r.f_pos.as<float>(), n_samples, r.f_out16.as<uint16_t>(), s);
+    P.end(ev, s, "network_latents", n_samples, n_samples * (ALG_GATHER_B + 12.0 + 32.0),
+          n_samples * ALG_DENSITY_FLOP);
+    ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_accum, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), rm.stepsize_info,
                        m.scene.alpha_thre, r.f_out16.as<uint2>(), r.f_t0.as<float>(), r.f_rays.as<FillRay>(),
                        r.f_tiles.as<FillTile>());
     FVC_HIP(hipGetLastError());
+    P.end(ev, s, "fill_accum", n_samples, n_samples * 12.0 + n_alive * 64.0 * 56, 0);
+    ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_alloc, dim3(1), dim3(1024), 0, s, r.f_list.as<int>(), cnt + CNT_TILES, C.capacity,
                        r.f_tiles.as<FillTile>(), cnt + CNT_SLOTS, cnt + CNT_OVERFLOW);
     FVC_HIP(hipGetLastError());
+    P.end(ev, s, "fill_alloc", n_alive, n_alive * 40.0, 0);
+    ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_write, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), C.bx, C.by,
                        m.net.latent_offset, r.f_out16.as<uint16_t>(), r.f_meta.as<int>(), r.f_rays.as<FillRay>(),
                        r.f_tiles.as<FillTile>(), C.table.as<int>(), C.pool.as<Voxel>());
     FVC_HIP(hipGetLastError());
+    P.end(ev, s, "fill_write", n_samples, n_samples * 36.0 + n_alive * 64.0 * 48);
   }
+  ev = P.begin(s);
   hipLaunchKernelGGL(k_fill_known, dim3(n_tiles), dim3(64), 0, s, W, H, C.bx, rm.stepsize_info,
                      r.f_tiles.as<FillTile>(), C.known.as<float>());
   FVC_HIP(hipGetLastError());
+  P.end(ev, s, "fill_known", (double)W * H, (double)W * H * 4 + n_tiles * 32.0, 0);
   const int used = read_counter(r, CNT_SLOTS, s);
   const int overflow = read_counter(r, CNT_OVERFLOW, s);
   C.n_used = std::min(used, C.capacity);
   C.initialized = true;
+  P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->n_bricks_set = C.n_used;
@@ -414,11 +434,11 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
                                                      float* __restrict__ s_dir, float* __restrict__ s_t0,
                                                      float* __restrict__ c_frox, float* __restrict__ c_dir,
                                                      float* __restrict__ c_t0, int* __restrict__ cnt_r,
-                                                     int* __restrict__ cnt_c) {
+                                                     int* __restrict__ cnt_c, int* __restrict__ step_counter) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   float tr[MAXS], tc[MAXS];
-  int nr = 0, nc = 0;
+  int nr = 0, nc = 0, taken = 0;
   bool alive = false;
   RayState r{};
   V3 oc = {0, 0, 0}, dc = {0, 0, 0};
@@ -429,7 +449,6 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
     oc = transform(cv.cam.world2cam, o.x, o.y, o.z, 1.0f);
     dc = transform(cv.cam.world2cam, d.x, d.y, d.z, 0.0f);
     float t1 = r.t;
-    int taken = 0;
     while (nr < n_steps && nc < n_steps && taken < step_cutoff) {
       taken++;
       const float t0 = t1;
@@ -468,6 +487,8 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
   }
   const int off_r = wave_reserve(nr, cnt_r);
   const int off_c = wave_reserve(nc, cnt_c);
+  const int wave_steps = wave_sum(taken);
+  if (lane_id() == 0 && wave_steps > 0) atomicAdd(step_counter, wave_steps);
   if (valid) {
     const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
     const V3 d01 = unit_to_01(d);
@@ -654,39 +675,54 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
   cv.known = C.known.as<float>();
   cv.pool = C.pool.as<Voxel>();
 
+  Profiler& P = r.prof;
+  hipEvent_t ev = P.begin(s);
   int n_alive = launch_init_rays(r, rm, s);
+  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n_alive * 52.0, 0);
   int* cnt = r.counters.as<int>();
   int cur = 0, subiter = 0;
   int64_t n_res = 0, n_hit = 0;
   const int64_t target = std::min<int64_t>(1 << 22, cap - 64);
   while (n_alive > 0 && subiter < 1000) {
     const int n_steps = steps_for(n_alive, target, 16);
-    FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 2 * sizeof(int), s));
+    FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 3 * sizeof(int), s));
     const dim3 grid((n_alive + 255) / 256);
     const int cutoff = 4096;
+    ev = P.begin(s);
 #define FVC_CACHE_MARCH(MS)                                                                                        \
   hipLaunchKernelGGL(k_cache_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, cutoff, rm.stepsize_info, m.occ, \
                      m.con, m.bitfield.as<uint8_t>(), cv, r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),     \
                      r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),          \
-                     r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(), cnt + CNT_SAMPLES, cnt + CNT_CACHE)
+                     r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(), cnt + CNT_SAMPLES, cnt + CNT_CACHE, \
+                     cnt + CNT_STEPS)
     if (n_steps <= 8) FVC_CACHE_MARCH(8);
     else FVC_CACHE_MARCH(16);
 #undef FVC_CACHE_MARCH
     FVC_HIP(hipGetLastError());
-    FVC_HIP(hipMemcpyAsync(r.pinned + CNT_SAMPLES, cnt + CNT_SAMPLES, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    const int hm = P.end(ev, s, "cache_march");
+    FVC_HIP(hipMemcpyAsync(r.pinned + CNT_SAMPLES, cnt + CNT_SAMPLES, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     FVC_HIP(hipStreamSynchronize(s));
     const int nr = r.pinned[CNT_SAMPLES], nc = r.pinned[CNT_CACHE];
+    const double n_step = (double)(uint32_t)r.pinned[CNT_STEPS];
+    // per step: occupancy byte; per classified sample: index + density probe
+    // (8 B) + 28 B stream record; ray state + entry per ray
+    P.work(hm, n_step, n_step + 36.0 * (nr + nc) + 84.0 * n_alive, 0);
     n_res += nr;
     n_hit += nc;
+    ev = P.begin(s);
     launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), nr, r.s_out.as<uint32_t>(), s);
+    P.end(ev, s, "network_resample", nr, nr * (ALG_GATHER_B + 24.0 + 8.0), nr * (ALG_DENSITY_FLOP + ALG_COLOUR_FLOP));
     if (nc > 0) {
       const int64_t groups = ((int64_t)nc + 15) / 16;
       const int blocks = (int)std::min<int64_t>((groups + 3) / 4, 256 * 16);
+      ev = P.begin(s);
       hipLaunchKernelGGL(k_cache_head, dim3(blocks), dim3(256), 0, s, r.c_frox.as<float>(), r.c_dir.as<float>(),
                          (int64_t)nc, cv, m.frags.as<h8>(), r.c_out.as<uint2>());
       FVC_HIP(hipGetLastError());
+      P.end(ev, s, "cache_head", nc, nc * (ALG_LOOKUP_B + 24.0 + 8.0), nc * ALG_COLOUR_FLOP);
     }
     FVC_HIP(hipMemsetAsync(cnt + CNT_ALIVE, 0, sizeof(int), s));
+    ev = P.begin(s);
     hipLaunchKernelGGL(k_cache_composite, grid, dim3(256), 0, s, n_alive, rm.stepsize_info, m.scene.alpha_thre,
                        r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
                        r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.s_out.as<uint2>(), r.s_t0.as<float>(),
@@ -694,12 +730,16 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
                        r.ray_pix[cur ^ 1].as<int>(), r.ray_res[cur ^ 1].as<float4>(), r.results.as<float4>(),
                        cnt + CNT_ALIVE);
     FVC_HIP(hipGetLastError());
+    P.end(ev, s, "cache_composite", nr + nc, 12.0 * (nr + nc) + 124.0 * n_alive, 0);
     n_alive = read_counter(r, CNT_ALIVE, s);
     cur ^= 1;
     subiter++;
   }
+  ev = P.begin(s);
   launch_write_image(r, rm, rgba, s);
+  P.end(ev, s, "write_image", (double)W * H, (double)W * H * 24, 0);
   FVC_HIP(hipStreamSynchronize(s));
+  P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     const float np = (float)((int64_t)W * H);

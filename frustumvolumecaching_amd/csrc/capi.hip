@@ -325,6 +325,35 @@ int fvc_cache_get_pool(fvc_renderer* h, int which, void* out, int64_t n_bytes) {
   });
 }
 
+int fvc_profile_enable(fvc_renderer* h, int enable) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    h->r.prof.table.clear();
+    h->r.prof.enabled = enable != 0;
+  });
+}
+
+int fvc_profile_get(fvc_renderer* h, fvc_kernel_stats* out, int32_t max_entries, int32_t* n_entries) {
+  return guard([&] {
+    REQUIRE(h && n_entries && (out || max_entries == 0), "null argument");
+    int i = 0;
+    for (const auto& kv : h->r.prof.table) {
+      if (i < max_entries) {
+        fvc_kernel_stats& o = out[i];
+        std::memset(&o, 0, sizeof(o));
+        std::strncpy(o.name, kv.first.c_str(), sizeof(o.name) - 1);
+        o.launches = kv.second.launches;
+        o.total_ms = kv.second.ms;
+        o.units = kv.second.units;
+        o.alg_bytes = kv.second.bytes;
+        o.alg_flops = kv.second.flops;
+      }
+      i++;
+    }
+    *n_entries = i;
+  });
+}
+
 int fvc_debug_march(fvc_renderer* h, const fvc_raymarch_info* rm, int32_t max_per_ray, int32_t* counts, float* t0s) {
   return guard([&] {
     REQUIRE(h && counts && max_per_ray >= 0, "invalid argument");

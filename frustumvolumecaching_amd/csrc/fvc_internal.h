@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -121,6 +122,90 @@ struct CacheStore {
   DevBuf pool;      // Voxel [capacity][512]
 };
 
+// ---------------------------------------------------------------- profiling
+// Per-kernel HIP-event timing (the reference's CudaTimer per TimingState,
+// renderer.h:79-105, cuda_timer.h:59-161) plus the algorithmic work each
+// launch did, for roofline reporting.
+struct ProfEntry {
+  int64_t launches = 0;
+  double ms = 0, units = 0, bytes = 0, flops = 0;
+};
+struct Profiler {
+  bool enabled = false;
+  std::map<std::string, ProfEntry> table;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+    double units, bytes, flops;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> free_events;
+  hipEvent_t get_event() {
+    if (!free_events.empty()) {
+      hipEvent_t e = free_events.back();
+      free_events.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    FVC_HIP(hipEventCreate(&e));
+    return e;
+  }
+  // returns an opaque start token (nullptr when disabled)
+  hipEvent_t begin(hipStream_t s) {
+    if (!enabled) return nullptr;
+    hipEvent_t a = get_event();
+    FVC_HIP(hipEventRecord(a, s));
+    return a;
+  }
+  // records the stop event; returns a handle for work() (-1 when disabled)
+  int end(hipEvent_t a, hipStream_t s, const char* name, double units = 0, double bytes = 0, double flops = 0) {
+    if (!enabled || !a) return -1;
+    hipEvent_t b = get_event();
+    FVC_HIP(hipEventRecord(b, s));
+    pending.push_back({name, a, b, units, bytes, flops});
+    return (int)pending.size() - 1;
+  }
+  // attach work known only after the launch (counters read back later)
+  void work(int h, double units, double bytes, double flops) {
+    if (h < 0) return;
+    pending[h].units = units;
+    pending[h].bytes = bytes;
+    pending[h].flops = flops;
+  }
+  // after the stream is synchronised
+  void collect() {
+    for (auto& p : pending) {
+      float ms = 0;
+      FVC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+      ProfEntry& e = table[p.name];
+      e.launches++;
+      e.ms += ms;
+      e.units += p.units;
+      e.bytes += p.bytes;
+      e.flops += p.flops;
+      free_events.push_back(p.a);
+      free_events.push_back(p.b);
+    }
+    pending.clear();
+  }
+  ~Profiler() {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : free_events) (void)hipEventDestroy(e);
+  }
+};
+
+// Algorithmic work per unit (one evaluated sample unless noted):
+//   hash-grid gather: 16 levels x 8 corners x 2 features x 2 B = 512 B
+//   density MLP: 2*(32*64 + 64*64 + 64*9) = 13440 FLOP
+//   colour MLP:  2*(24*64 + 64*3)        =  3456 FLOP
+//   cache trilinear lookup: 8 corners x (4 B index + 4 B isset + 4 B density
+//                           + 16 B latents) = 224 B
+static const double ALG_GATHER_B = 512, ALG_LOOKUP_B = 224;
+static const double ALG_DENSITY_FLOP = 13440, ALG_COLOUR_FLOP = 3456;
+
 // ---------------------------------------------------------------- renderer
 struct Renderer {
   Model* model = nullptr;
@@ -133,6 +218,7 @@ struct Renderer {
   fvc_cache_settings settings{0, 0, 1, 1, 1, 1};
   int capacity_override = 0;
   std::mutex read_mutex, write_mutex;
+  Profiler prof;
 
   int width = 0, height = 0;
   int64_t sample_capacity = 0;

@@ -56,11 +56,11 @@ __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, fvc_ste
                                                RayState* __restrict__ rays, int4* __restrict__ entry,
                                                int* __restrict__ alive_flag, float* __restrict__ s_pos,
                                                float* __restrict__ s_dir, float* __restrict__ s_t0,
-                                               int* __restrict__ s_counter) {
+                                               int* __restrict__ s_counter, int* __restrict__ step_counter) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   float ts[MAXS];
-  int cnt = 0;
+  int cnt = 0, taken = 0;
   bool alive = false;
   RayState r{};
   if (valid) {
@@ -69,6 +69,7 @@ __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, fvc_ste
     const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
     float t1 = r.t;
     while (cnt < n_steps) {
+      taken++;
       const float t0 = t1;
       const float dt = step_dt(st, t0);
       t1 = t0 + dt;
@@ -84,6 +85,8 @@ __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, fvc_ste
     r.t = t1;
   }
   const int off = wave_reserve(cnt, s_counter);
+  const int wave_steps = wave_sum(taken);
+  if (lane_id() == 0 && wave_steps > 0) atomicAdd(step_counter, wave_steps);
   if (valid) {
     const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
     const V3 d01 = unit_to_01(d);
@@ -250,41 +253,58 @@ void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_re
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   if (W != r.width || H != r.height) resize_renderbuffers(r, W, H);
   hipStream_t s = r.stream();
+  Profiler& P = r.prof;
+  hipEvent_t ev = P.begin(s);
   int n_alive = launch_init_rays(r, rm, s);
+  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n_alive * 52.0, 0);
+  int* cnt = r.counters.as<int>();
   int cur = 0, subiter = 0;
   int64_t n_eval = 0;
   const int64_t target = std::min<int64_t>(TARGET_BATCH, r.sample_capacity - 64);
   while (n_alive > 0 && subiter < MAX_SUBITER) {
     const int n_steps = steps_for(n_alive, target, 16);
-    FVC_HIP(hipMemsetAsync(r.counters.as<int>() + CNT_SAMPLES, 0, sizeof(int), s));
+    FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 3 * sizeof(int), s));
     const dim3 grid((n_alive + 255) / 256);
-    if (n_steps <= 8)
-      hipLaunchKernelGGL(k_march<8>, grid, dim3(256), 0, s, n_alive, n_steps, rm.stepsize_info, m.occ, m.con,
-                         m.bitfield.as<uint8_t>(), r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),
-                         r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
-                         r.counters.as<int>() + CNT_SAMPLES);
-    else
-      hipLaunchKernelGGL(k_march<16>, grid, dim3(256), 0, s, n_alive, n_steps, rm.stepsize_info, m.occ, m.con,
-                         m.bitfield.as<uint8_t>(), r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),
-                         r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
-                         r.counters.as<int>() + CNT_SAMPLES);
+    ev = P.begin(s);
+#define FVC_MARCH(MS)                                                                                          \
+  hipLaunchKernelGGL(k_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, rm.stepsize_info, m.occ, m.con,    \
+                     m.bitfield.as<uint8_t>(), r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),            \
+                     r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),     \
+                     cnt + CNT_SAMPLES, cnt + CNT_STEPS)
+    if (n_steps <= 8) FVC_MARCH(8);
+    else FVC_MARCH(16);
+#undef FVC_MARCH
     FVC_HIP(hipGetLastError());
-    const int n_samples = read_counter(r, CNT_SAMPLES, s);
+    const int hm = P.end(ev, s, "march");
+    FVC_HIP(hipMemcpyAsync(r.pinned + CNT_SAMPLES, cnt + CNT_SAMPLES, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+    FVC_HIP(hipStreamSynchronize(s));
+    const int n_samples = r.pinned[CNT_SAMPLES];
+    const double n_step = (double)(uint32_t)r.pinned[CNT_STEPS];
+    // 1 B occupancy per step, 28 B per emitted sample, ray state + entry per ray
+    P.work(hm, n_step, n_step + 28.0 * n_samples + 84.0 * n_alive, 0);
     n_eval += n_samples;
+    ev = P.begin(s);
     launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), n_samples, r.s_out.as<uint32_t>(), s);
-    FVC_HIP(hipMemsetAsync(r.counters.as<int>() + CNT_ALIVE, 0, sizeof(int), s));
+    P.end(ev, s, "network_full", n_samples, n_samples * (ALG_GATHER_B + 24.0 + 8.0),
+          n_samples * (ALG_DENSITY_FLOP + ALG_COLOUR_FLOP));
+    FVC_HIP(hipMemsetAsync(cnt + CNT_ALIVE, 0, sizeof(int), s));
+    ev = P.begin(s);
     hipLaunchKernelGGL(k_composite, grid, dim3(256), 0, s, n_alive, rm.stepsize_info, m.scene.alpha_thre,
                        r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
                        r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.s_out.as<uint2>(), r.s_t0.as<float>(),
                        r.rays[cur ^ 1].as<RayState>(), r.ray_pix[cur ^ 1].as<int>(), r.ray_res[cur ^ 1].as<float4>(),
-                       r.results.as<float4>(), r.counters.as<int>() + CNT_ALIVE);
+                       r.results.as<float4>(), cnt + CNT_ALIVE);
     FVC_HIP(hipGetLastError());
+    P.end(ev, s, "composite", n_samples, 12.0 * n_samples + 124.0 * n_alive, 0);
     n_alive = read_counter(r, CNT_ALIVE, s);
     cur ^= 1;
     subiter++;
   }
+  ev = P.begin(s);
   launch_write_image(r, rm, rgba, s);
+  P.end(ev, s, "write_image", (double)W * H, (double)W * H * 24, 0);
   FVC_HIP(hipStreamSynchronize(s));
+  P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->samples_ppx = (float)n_eval / (float)((int64_t)W * H);

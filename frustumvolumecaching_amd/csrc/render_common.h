@@ -12,7 +12,7 @@ namespace fvc {
 struct RayState { float ox, oy, oz, dx, dy, dz, far_, t; };
 
 // slots of Renderer::counters
-enum { CNT_ALIVE = 0, CNT_SAMPLES = 1, CNT_CACHE = 2, CNT_TILES = 3, CNT_SLOTS = 4, CNT_OVERFLOW = 5 };
+enum { CNT_ALIVE = 0, CNT_SAMPLES = 1, CNT_CACHE = 2, CNT_STEPS = 3, CNT_TILES = 4, CNT_SLOTS = 5, CNT_OVERFLOW = 6 };
 
 __device__ __forceinline__ float h2f(uint32_t bits16) { return (float)__builtin_bit_cast(_Float16, (uint16_t)bits16); }
 

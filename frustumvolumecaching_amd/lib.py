@@ -56,6 +56,8 @@ _SIGS = {
     "fvc_cache_get_known_ranges": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "fvc_cache_get_pool": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "fvc_debug_march": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, _P, _P]),
+    "fvc_profile_enable": (C.c_int, [_P, C.c_int]),
+    "fvc_profile_get": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
 }
 
 
@@ -246,6 +248,18 @@ class Renderer:
         if n:
             check(load().fvc_cache_get_pool(self.h, which, _addr(out), out.nbytes))
         return out
+
+    def profile_enable(self, enable=True):
+        check(load().fvc_profile_enable(self.h, int(bool(enable))))
+
+    def profile(self):
+        """{kernel: dict(launches, ms, units, bytes, flops)} since profile_enable."""
+        n = C.c_int32()
+        check(load().fvc_profile_get(self.h, None, 0, C.byref(n)))
+        arr = (abi.KernelStats * max(1, n.value))()
+        check(load().fvc_profile_get(self.h, C.cast(arr, C.c_void_p), n.value, C.byref(n)))
+        return {a.name.decode(): dict(launches=a.launches, ms=a.total_ms, units=a.units, bytes=a.alg_bytes,
+                                      flops=a.alg_flops) for a in arr[:n.value]}
 
     def debug_march(self, rm, max_per_ray=64):
         import numpy as np

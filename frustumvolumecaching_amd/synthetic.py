@@ -136,13 +136,13 @@ def blobs(seed, n, center_radius, sigma_range, amp=1.0):
     return c, s, np.full(n, amp)
 
 
-def blobs_spread(seed, n, radius, sigma_frac=(0.08, 0.2), min_sigma=0.08):
-    """Blobs whose centres fill a ball of `radius` (radius ~ U^(1/3)) and whose
-    width grows with the distance from the origin."""
+def blobs_spread(seed, n, radius, sigma_frac=(0.04, 0.1), min_sigma=0.08):
+    """Blobs with centres uniform in direction and in distance from the origin
+    (so every cascade holds some) and widths growing with that distance."""
     u = uniform(seed, n * 6).reshape(n, 6)
     d = u[:, :3] * 2.0 - 1.0
     d /= np.linalg.norm(d, axis=1, keepdims=True) + 1e-12
-    r = radius * u[:, 3] ** (1.0 / 3.0)
+    r = radius * u[:, 3]
     c = d * r[:, None]
     s = np.maximum(min_sigma, (sigma_frac[0] + u[:, 4] * (sigma_frac[1] - sigma_frac[0])) * r)
     return c, s, np.full(n, 1.0)

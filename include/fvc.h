@@ -168,6 +168,18 @@ typedef struct {
   int32_t brick_size;             /* 8 */
 } fvc_cache_info;
 
+/* Per-kernel timing of the renderer's launches (HIP events; the reference's
+ * CudaTimer per TimingState, renderer.h:79-105) with the algorithmic work
+ * they did, accumulated while profiling is enabled. */
+typedef struct {
+  char    name[32];
+  int64_t launches;
+  double  total_ms;
+  double  units;        /* samples, steps or pixels, per kernel */
+  double  alg_bytes;    /* algorithmic bytes (no cache effects) */
+  double  alg_flops;
+} fvc_kernel_stats;
+
 typedef struct fvc_model fvc_model;
 typedef struct fvc_renderer fvc_renderer;
 
@@ -228,6 +240,11 @@ int fvc_cache_set_settings(fvc_renderer* renderer, const fvc_cache_settings* set
 /* Pool capacity override in bricks (0 = reference estimate n_rays/8 rounded
  * up to 16384, cache_renderer.cu:832-846 + cache.cu:143-146). */
 int fvc_cache_set_capacity(fvc_renderer* renderer, int32_t capacity_bricks);
+
+/* ---- profiling ---- */
+/* Enable (1) or disable (0) per-kernel timing; enabling clears the table. */
+int fvc_profile_enable(fvc_renderer* renderer, int enable);
+int fvc_profile_get(fvc_renderer* renderer, fvc_kernel_stats* out, int32_t max_entries, int32_t* n_entries);
 
 /* ---- inspection (bit-exact parity tests) ---- */
 /* Float result of the last render before the 8-bit write: rgb + transmittance

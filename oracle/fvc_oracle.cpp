@@ -655,7 +655,7 @@ template <class R> struct FillTile {
 };
 
 template <class R>
-Cache<R>* cache_fill(const Model& M, const fvc_raymarch_info& rm, Mode mode, int capacity) {
+Cache<R>* cache_fill(const Model& M, const fvc_raymarch_info& rm, Mode mode, int capacity, const int* crop = nullptr) {
   auto* C = new Cache<R>();
   C->rm = rm;
   const fvc_scene_info& sc = M.scene;
@@ -690,6 +690,10 @@ Cache<R>* cache_fill(const Model& M, const fvc_raymarch_info& rm, Mode mode, int
     }
     T.slab = min_start / BS;
     T.alive = n_alive > 0;
+    // optional tile crop (pixel rectangle x0, y0, w, h): tiles outside never fill
+    if (crop && (tx * BS + BS <= crop[0] || tx * BS >= crop[0] + crop[2] || ty * BS + BS <= crop[1] ||
+                 ty * BS >= crop[1] + crop[3]))
+      T.alive = false;
   }
 
   int64_t n_eval = 0;
@@ -1156,14 +1160,16 @@ void orc_render_full(const Model* M, const fvc_raymarch_info* rm, int mode, int 
   else render_full<float>(*M, *rm, (Mode)mode, x0, y0, w, h, out, n_samples);
 }
 
-CacheHandle* orc_cache_fill(const Model* M, const fvc_raymarch_info* rm, int mode, int capacity, fvc_init_stats* stats) {
+// crop (x0, y0, w, h) or null: only tiles overlapping the crop are filled
+CacheHandle* orc_cache_fill(const Model* M, const fvc_raymarch_info* rm, int mode, int capacity, const int* crop,
+                            fvc_init_stats* stats) {
   auto* h = new CacheHandle();
   h->mode = mode;
   if (mode == M_F64) {
-    h->d.reset(cache_fill<double>(*M, *rm, (Mode)mode, capacity));
+    h->d.reset(cache_fill<double>(*M, *rm, (Mode)mode, capacity, crop));
     if (stats) *stats = h->d->stats;
   } else {
-    h->f.reset(cache_fill<float>(*M, *rm, (Mode)mode, capacity));
+    h->f.reset(cache_fill<float>(*M, *rm, (Mode)mode, capacity, crop));
     if (stats) *stats = h->f->stats;
   }
   return h;

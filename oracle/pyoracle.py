@@ -66,7 +66,7 @@ def lib():
             "orc_march": (None, [P, C.POINTER(abi.RaymarchInfo), C.c_int, P, P]),
             "orc_render_full": (None, [P, C.POINTER(abi.RaymarchInfo), C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int, P, C.POINTER(C.c_int64)]),
-            "orc_cache_fill": (P, [P, C.POINTER(abi.RaymarchInfo), C.c_int, C.c_int, C.POINTER(abi.InitStats)]),
+            "orc_cache_fill": (P, [P, C.POINTER(abi.RaymarchInfo), C.c_int, C.c_int, P, C.POINTER(abi.InitStats)]),
             "orc_cache_destroy": (None, [P]),
             "orc_cache_info": (None, [P, C.POINTER(abi.CacheInfo)]),
             "orc_cache_table": (None, [P, P]),
@@ -192,15 +192,19 @@ class Model:
         lib().orc_render_full(self.h, C.byref(rm), mode, x0, y0, w, h, _p(out), C.byref(ns))
         return out, ns.value
 
-    def cache_fill(self, rm, mode=EMU16, capacity=0):
-        return Cache(self, rm, mode, capacity)
+    def cache_fill(self, rm, mode=EMU16, capacity=0, crop=None):
+        """CacheRenderer::initCacheBlockwise; crop = (x0, y0, w, h) fills only the
+        tiles overlapping that pixel rectangle."""
+        return Cache(self, rm, mode, capacity, crop)
 
 
 class Cache:
-    def __init__(self, model, rm, mode, capacity):
+    def __init__(self, model, rm, mode, capacity, crop=None):
         self.model = model
         self.stats = abi.InitStats()
-        self.h = lib().orc_cache_fill(model.h, C.byref(rm), mode, capacity, C.byref(self.stats))
+        cr = None if crop is None else np.asarray(crop, np.int32)
+        self.h = lib().orc_cache_fill(model.h, C.byref(rm), mode, capacity, None if cr is None else _p(cr),
+                                      C.byref(self.stats))
         self.info = abi.CacheInfo()
         lib().orc_cache_info(self.h, C.byref(self.info))
 
