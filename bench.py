@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-psnr", action="store_true", help="skip the oracle PSNR checks")
     ap.add_argument("--cpu-rows", type=int, default=16, help="rows of the CPU baseline band")
+    ap.add_argument("--batch", type=int, default=0, help="target samples per launch (0 = library default)")
+    ap.add_argument("--max-steps", type=int, default=0, help="cap on samples per ray per launch")
     return ap.parse_args()
 
 
@@ -60,6 +62,8 @@ def main():
     log(f"[bench] workload built in {time.time() - t0:.1f}s")
     model = lib.NerfNetwork.from_workload(wl)
     r = lib.Renderer(model)
+    if a.batch or a.max_steps:
+        r.set_batching(a.batch or (1 << 22), a.max_steps or 16)
     W, H = a.width, a.height
     n_px = W * H
     img = torch.zeros(n_px * 4, dtype=torch.uint8, device="cuda")

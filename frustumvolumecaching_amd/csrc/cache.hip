@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "march.h"
 #include "mlp_mfma.h"
 #include "render_common.h"
 
@@ -90,14 +91,19 @@ __global__ __launch_bounds__(1024) void k_fill_list(int n_tiles, const FillTile*
 // (cache_renderer.cu:83-287): skip empty slabs, emit the occupied samples of
 // the first non-empty one as a compacted stream.
 __global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list, fvc_stepsize_info st, OccParams occ,
-                                                    ContractParams con, const uint8_t* __restrict__ bf, int bz,
+                                                    ContractParams con, const uint8_t* __restrict__ bf,
+                                                    const uint32_t* __restrict__ coarse_g, int bz,
                                                     FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
                                                     float* __restrict__ f_pos, float* __restrict__ f_t0,
                                                     int* __restrict__ f_meta, int* __restrict__ counter) {
+  __shared__ uint32_t coarse[16 * COARSE_WORDS];
+  load_coarse(coarse_g, occ.nlvl, coarse);
   const int t = list[blockIdx.x], lane = threadIdx.x;
   FillRay r = rays[(int64_t)t * 64 + lane];
   FillTile tile = tiles[t];
   const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
+  const V3 inv_d = recip(d);
+  Skip sk = no_skip();
   int slab = tile.slab;
   unsigned mask = 0;
   int total = 0;
@@ -115,7 +121,8 @@ __global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list
       t1 = t0 + dt;
       const float tm = (t0 + t1) * 0.5f;
       if (tm > r.far_) { r.alive = 0; continue; }
-      if (occupied(o + tm * d, occ, bf)) mask |= 1u << j;
+      if (in_skip(sk, tm)) continue;
+      if (occ_query(o + tm * d, occ, bf, coarse, o, inv_d, sk)) mask |= 1u << j;
     }
     total = wave_sum(__popc(mask));
     if (total > 0) break;
@@ -361,7 +368,8 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats)
     FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, sizeof(int), s));
     ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_sample, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), rm.stepsize_info, m.occ,
-                       m.con, m.bitfield.as<uint8_t>(), C.bz, r.f_rays.as<FillRay>(), r.f_tiles.as<FillTile>(),
+                       m.con, m.bitfield.as<uint8_t>(), m.coarse.as<uint32_t>(), C.bz, r.f_rays.as<FillRay>(),
+                       r.f_tiles.as<FillTile>(),
                        r.f_pos.as<float>(), r.f_t0.as<float>(), r.f_meta.as<int>(), cnt + CNT_SAMPLES);
     FVC_HIP(hipGetLastError());
     const int hs = P.end(ev, s, "fill_sample");
@@ -429,12 +437,15 @@ struct CacheView {
 template <int MAXS>
 __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, int step_cutoff, fvc_stepsize_info st,
                                                      OccParams occ, ContractParams con, const uint8_t* __restrict__ bf,
+                                                     const uint32_t* __restrict__ coarse_g, StepTableView tab,
                                                      CacheView cv, RayState* __restrict__ rays, int4* __restrict__ entry,
                                                      int* __restrict__ alive_flag, float* __restrict__ s_pos,
                                                      float* __restrict__ s_dir, float* __restrict__ s_t0,
                                                      float* __restrict__ c_frox, float* __restrict__ c_dir,
                                                      float* __restrict__ c_t0, int* __restrict__ cnt_r,
                                                      int* __restrict__ cnt_c, int* __restrict__ step_counter) {
+  __shared__ uint32_t coarse[16 * COARSE_WORDS];
+  load_coarse(coarse_g, occ.nlvl, coarse);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   float tr[MAXS], tc[MAXS];
@@ -448,15 +459,24 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
     const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
     oc = transform(cv.cam.world2cam, o.x, o.y, o.z, 1.0f);
     dc = transform(cv.cam.world2cam, d.x, d.y, d.z, 0.0f);
+    const V3 inv_d = recip(d);
+    Skip sk = no_skip();
     float t1 = r.t;
-    while (nr < n_steps && nc < n_steps && taken < step_cutoff) {
+    int j = r.j, iters = 0;
+    while (nr < n_steps && nc < n_steps && iters < step_cutoff) {
+      iters++;
       taken++;
       const float t0 = t1;
       const float dt = step_dt(st, t0);
       t1 = t0 + dt;
+      j++;
       const float tm = (t0 + t1) * 0.5f;
       if (tm > r.far_) { alive = false; break; }
-      if (!occupied(o + tm * d, occ, bf)) continue;
+      if (in_skip(sk, tm)) {
+        taken += table_jump(tab, st, r.m, j, t1, sk.hi);
+        continue;
+      }
+      if (!occ_query(o + tm * d, occ, bf, coarse, o, inv_d, sk)) continue;
       float tf;
       const V3 f = cam2froxel(oc + tm * dc, cv.cam, cv.st, tf);
       const bool in_fov = f.x >= 0.0f && f.x < (float)cv.fx && f.y >= 0.0f && f.y < (float)cv.fy;
@@ -484,6 +504,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
       }
     }
     r.t = t1;
+    r.j = j;
   }
   const int off_r = wave_reserve(nr, cnt_r);
   const int off_c = wave_reserve(nc, cnt_c);
@@ -520,6 +541,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
       }
     }
     rays[i].t = r.t;
+    rays[i].j = r.j;
     entry[i] = make_int4(off_r, nr, off_c, nc);
     alive_flag[i] = alive;
   }
@@ -674,29 +696,32 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
   cv.table = C.table.as<int>();
   cv.known = C.known.as<float>();
   cv.pool = C.pool.as<Voxel>();
+  const StepTableView tab = step_table(r, rm);
 
   Profiler& P = r.prof;
   hipEvent_t ev = P.begin(s);
   int n_alive = launch_init_rays(r, rm, s);
-  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n_alive * 52.0, 0);
+  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n_alive * 60.0, 0);
   int* cnt = r.counters.as<int>();
   int cur = 0, subiter = 0;
   int64_t n_res = 0, n_hit = 0;
-  const int64_t target = std::min<int64_t>(1 << 22, cap - 64);
+  const int64_t target = std::min<int64_t>(r.target_batch, cap - 64);
   while (n_alive > 0 && subiter < 1000) {
-    const int n_steps = steps_for(n_alive, target, 16);
+    const int n_steps = steps_for(n_alive, target, r.max_steps);
     FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 3 * sizeof(int), s));
     const dim3 grid((n_alive + 255) / 256);
     const int cutoff = 4096;
     ev = P.begin(s);
 #define FVC_CACHE_MARCH(MS)                                                                                        \
   hipLaunchKernelGGL(k_cache_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, cutoff, rm.stepsize_info, m.occ, \
-                     m.con, m.bitfield.as<uint8_t>(), cv, r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),     \
+                     m.con, m.bitfield.as<uint8_t>(), m.coarse.as<uint32_t>(), tab, cv, r.rays[cur].as<RayState>(), \
+                     r.ray_entry.as<int4>(),                                                                        \
                      r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),          \
                      r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(), cnt + CNT_SAMPLES, cnt + CNT_CACHE, \
                      cnt + CNT_STEPS)
     if (n_steps <= 8) FVC_CACHE_MARCH(8);
-    else FVC_CACHE_MARCH(16);
+    else if (n_steps <= 16) FVC_CACHE_MARCH(16);
+    else FVC_CACHE_MARCH(32);
 #undef FVC_CACHE_MARCH
     FVC_HIP(hipGetLastError());
     const int hm = P.end(ev, s, "cache_march");

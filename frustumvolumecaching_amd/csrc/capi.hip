@@ -108,6 +108,19 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
             }
       m.bitfield.ensure(m.bitfield_host.size());
       FVC_HIP(hipMemcpy(m.bitfield.p, m.bitfield_host.data(), m.bitfield_host.size(), hipMemcpyHostToDevice));
+      // coarse empty-space table: bit (c x 16 + cy) x 16 + cz of cascade l is set
+      // when any cell of the 8^3 block is occupied
+      std::vector<uint32_t> coarse((size_t)scene->grid_nlvl * 128, 0u);
+      for (int l = 0; l < scene->grid_nlvl; l++)
+        for (int x = 0; x < 128; x++)
+          for (int y = 0; y < 128; y++)
+            for (int z = 0; z < 128; z++) {
+              if (!occ[l * vals + ((int64_t)x * 128 + y) * 128 + z]) continue;
+              const int cb = ((x >> 3) * 16 + (y >> 3)) * 16 + (z >> 3);
+              coarse[(size_t)l * 128 + (cb >> 5)] |= 1u << (cb & 31);
+            }
+      m.coarse.ensure(coarse.size() * sizeof(uint32_t));
+      FVC_HIP(hipMemcpy(m.coarse.p, coarse.data(), coarse.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
       m.table.ensure(n_pos * sizeof(uint16_t));
       FVC_HIP(hipMemcpy(m.table.p, p_pos, n_pos * sizeof(uint16_t), hipMemcpyHostToDevice));
       std::vector<uint16_t> frags;
@@ -119,6 +132,8 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
       const V3 from = cvt(scene->aabb_from), to = cvt(scene->aabb_to);
       m.occ.unit_off = from / S;
       m.occ.unit_ext = (to - from) / S;
+      m.occ.unit_inv = V3{1.0f / m.occ.unit_ext.x, 1.0f / m.occ.unit_ext.y, 1.0f / m.occ.unit_ext.z};
+      m.occ.pow2 = is_pow2f(m.occ.unit_ext.x) && is_pow2f(m.occ.unit_ext.y) && is_pow2f(m.occ.unit_ext.z);
       m.occ.nlvl = scene->grid_nlvl;
       m.occ.normalized = scene->normalized;
       // apply_contraction constants (raymarch_common.h:610-653)
@@ -130,6 +145,8 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
         m.con.from = cvt(scene->contraction_aabb_from);
         m.con.ext = cvt(scene->contraction_aabb_to) - m.con.from;
       }
+      m.con.inv = V3{1.0f / m.con.ext.x, 1.0f / m.con.ext.y, 1.0f / m.con.ext.z};
+      m.con.pow2 = is_pow2f(m.con.ext.x) && is_pow2f(m.con.ext.y) && is_pow2f(m.con.ext.z);
     } catch (...) {
       delete h;
       throw;
@@ -196,6 +213,17 @@ int fvc_renderer_set_stream(fvc_renderer* h, void* stream) {
   return guard([&] {
     REQUIRE(h, "null argument");
     h->r.user_stream = (hipStream_t)stream;
+  });
+}
+
+int fvc_renderer_set_batching(fvc_renderer* h, int64_t target_samples, int32_t max_samples_per_ray) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    REQUIRE(target_samples >= 4096 && target_samples <= (int64_t)1 << 28, "target_samples out of range");
+    REQUIRE(max_samples_per_ray >= 1 && max_samples_per_ray <= 32, "max_samples_per_ray must be in [1, 32]");
+    h->r.target_batch = target_samples;
+    h->r.max_steps = max_samples_per_ray;
+    if (h->r.width > 0) resize_renderbuffers(h->r, h->r.width, h->r.height);
   });
 }
 

@@ -82,6 +82,7 @@ struct Model {
   ContractParams con{};
   int64_t n_pos_enc = 0, n_base = 0, n_head = 0;
   DevBuf bitfield;   // Morton occupancy bitfield
+  DevBuf coarse;     // per cascade 16^3 "any of 8^3 cells occupied" bits
   DevBuf table;      // hash grid, uint32 per entry (2 x fp16)
   DevBuf frags;      // packed weight fragments (fp16)
   std::vector<uint8_t> bitfield_host;
@@ -222,6 +223,16 @@ struct Renderer {
 
   int width = 0, height = 0;
   int64_t sample_capacity = 0;
+  // sub-iteration batching: samples per launch targeted, and the cap on
+  // samples per ray per launch (the reference's TARGET_BATCH_SIZE and
+  // MAX_N_STEPS_PER_SUBITER, performance_renderer.h:54-58)
+  int64_t target_batch = 1 << 22;
+  int max_steps = 16;
+  // step table of the marchers (render.hip: step_table)
+  DevBuf tab_t, tab_off;
+  fvc_stepsize_info tab_key{};
+  float tab_limit = 0.0f;
+  int tab_rows = -1;   // -1: not built; 0: disabled for this stepsize
   // ray state (double buffered for compaction)
   DevBuf rays[2];        // RayState
   DevBuf ray_pix[2];     // int32 pixel index

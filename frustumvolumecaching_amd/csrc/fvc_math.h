@@ -111,15 +111,17 @@ FVC_HD uint32_t morton3d(uint32_t x, uint32_t y, uint32_t z) {
 struct OccParams {
   V3 unit_off;    // aabb_from / 2^(nlvl-1)
   V3 unit_ext;    // (aabb_to - aabb_from) / 2^(nlvl-1)
+  V3 unit_inv;    // 1 / unit_ext, exact when pow2
   int nlvl;
   int normalized;
+  int pow2;       // every unit_ext component is a power of two
 };
 
 // Byte index (<0: outside every cascade) and bit of the occupancy cell of p.
 // The cell index is clamped to [0,127] (the float expression can round to
 // exactly 1.0 at the upper face of a cascade).
 FVC_HD int occ_cell(V3 p, const OccParams& o, int& bit) {
-  const V3 u = o.normalized ? p : (p - o.unit_off) / o.unit_ext;
+  const V3 u = o.normalized ? p : (o.pow2 ? (p - o.unit_off) * o.unit_inv : (p - o.unit_off) / o.unit_ext);
   const float ax = fabsf(u.x - 0.5f), ay = fabsf(u.y - 0.5f), az = fabsf(u.z - 0.5f);
   const float mx = fmaxf_(fmaxf_(ax, ay), az);
   int e;
@@ -146,11 +148,14 @@ FVC_HD bool occupied(V3 p, const OccParams& o, const uint8_t* bf) {
 struct ContractParams {
   int type;
   V3 from, ext;    // AABB contraction: scene aabb; warps: contraction aabb
+  V3 inv;          // 1 / ext, exact when pow2
+  int pow2;
 };
 FVC_HD V3 contract(V3 p, const ContractParams& c) {
-  if (c.type == FVC_CONTRACTION_AABB) return (p - c.from) / c.ext;
+  const V3 q = c.pow2 ? (p - c.from) * c.inv : (p - c.from) / c.ext;
+  if (c.type == FVC_CONTRACTION_AABB) return q;
   if (c.type == FVC_CONTRACTION_WARP_AABB_LINF || c.type == FVC_CONTRACTION_WARP_AABB_L2) {
-    const V3 u = ((p - c.from) / c.ext) * 2.0f - 1.0f;
+    const V3 u = q * 2.0f - 1.0f;
     float n;
     if (c.type == FVC_CONTRACTION_WARP_AABB_LINF) n = fmaxf_(fmaxf_(fabsf(u.x), fabsf(u.y)), fabsf(u.z));
     else n = sqrtf(dot3(u, u));
@@ -200,5 +205,11 @@ FVC_HD V3 cam2froxel(V3 c, const fvc_camera_info& cam, const fvc_stepsize_info& 
 }
 
 FVC_HD V3 unit_to_01(V3 d) { return (d + 1.0f) / 2.0f; }
+
+// x is a positive power of two (then a / x == a * (1/x) exactly)
+FVC_HD bool is_pow2f(float x) {
+  int e;
+  return x > 0.0f && frexpf(x, &e) == 0.5f;
+}
 
 }  // namespace fvc

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "march.h"
 #include "render_common.h"
 
 namespace fvc {
@@ -35,7 +36,11 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, V3 aabb
     st.ox = r.o.x; st.oy = r.o.y; st.oz = r.o.z;
     st.dx = r.d.x; st.dy = r.d.y; st.dz = r.d.z;
     st.far_ = hit ? fminf_(t1, s.far_) : s.far_;
-    st.t = hit ? valid_t_from_t(s, t0) : s.near_;
+    // valid_t_from_t with its bin index kept: t = t_from_step(m + near_i)
+    const float mf = hit ? roundf(step_from_t(s, fmaxf_(t0, s.near_)) - s.near_i) : 0.0f;
+    st.t = hit ? t_from_step(s, mf + s.near_i) : s.near_;
+    st.m = (int)mf;
+    st.j = 0;
     alive = hit;
     results[(int64_t)y * W + x] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   }
@@ -53,10 +58,13 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, V3 aabb
 template <int MAXS>
 __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, fvc_stepsize_info st, OccParams occ,
                                                ContractParams con, const uint8_t* __restrict__ bf,
+                                               const uint32_t* __restrict__ coarse_g, StepTableView tab,
                                                RayState* __restrict__ rays, int4* __restrict__ entry,
                                                int* __restrict__ alive_flag, float* __restrict__ s_pos,
                                                float* __restrict__ s_dir, float* __restrict__ s_t0,
                                                int* __restrict__ s_counter, int* __restrict__ step_counter) {
+  __shared__ uint32_t coarse[16 * COARSE_WORDS];
+  load_coarse(coarse_g, occ.nlvl, coarse);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   float ts[MAXS];
@@ -67,15 +75,23 @@ __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, fvc_ste
     r = rays[i];
     alive = true;
     const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
+    const V3 inv_d = recip(d);
+    Skip sk = no_skip();
     float t1 = r.t;
+    int j = r.j;
     while (cnt < n_steps) {
       taken++;
       const float t0 = t1;
       const float dt = step_dt(st, t0);
       t1 = t0 + dt;
+      j++;
       const float tm = (t0 + t1) * 0.5f;
       if (tm > r.far_) { alive = false; break; }
-      if (occupied(o + tm * d, occ, bf)) {
+      if (in_skip(sk, tm)) {
+        taken += table_jump(tab, st, r.m, j, t1, sk.hi);
+        continue;
+      }
+      if (occ_query(o + tm * d, occ, bf, coarse, o, inv_d, sk)) {
 #pragma unroll
         for (int k = 0; k < MAXS; k++)
           if (k == cnt) ts[k] = t0;
@@ -83,6 +99,7 @@ __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, fvc_ste
       }
     }
     r.t = t1;
+    r.j = j;
   }
   const int off = wave_reserve(cnt, s_counter);
   const int wave_steps = wave_sum(taken);
@@ -104,6 +121,7 @@ __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, fvc_ste
       }
     }
     rays[i].t = r.t;
+    rays[i].j = r.j;
     entry[i] = make_int4(off, cnt, 0, 0);
     alive_flag[i] = alive;
   }
@@ -157,8 +175,11 @@ __global__ void k_write_image(int W, int H, const float4* __restrict__ results, 
 
 // march only: counts of occupied samples + first max_per_ray t0 per pixel
 __global__ void k_debug_march(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, OccParams occ,
-                              const uint8_t* __restrict__ bf, int max_per_ray, int* __restrict__ counts,
+                              const uint8_t* __restrict__ bf, const uint32_t* __restrict__ coarse_g,
+                              StepTableView tab, int max_per_ray, int* __restrict__ counts,
                               float* __restrict__ t0s) {
+  __shared__ uint32_t coarse[16 * COARSE_WORDS];
+  load_coarse(coarse_g, occ.nlvl, coarse);
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x >= W || y >= H) return;
@@ -170,14 +191,24 @@ __global__ void k_debug_march(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, Oc
   int cnt = 0;
   if (hit) {
     const float far_ = fminf_(a1, s.far_);
-    float t1 = valid_t_from_t(s, a0);
+    const V3 inv_d = recip(r.d);
+    Skip sk = no_skip();
+    const float mf = roundf(step_from_t(s, fmaxf_(a0, s.near_)) - s.near_i);
+    const int m = (int)mf;
+    float t1 = t_from_step(s, mf + s.near_i);
+    int j = 0;
     while (true) {
       const float t0 = t1;
       const float dt = step_dt(s, t0);
       t1 = t0 + dt;
+      j++;
       const float tm = (t0 + t1) * 0.5f;
       if (tm > far_) break;
-      if (!occupied(r.o + tm * r.d, occ, bf)) continue;
+      if (in_skip(sk, tm)) {
+        table_jump(tab, s, m, j, t1, sk.hi);
+        continue;
+      }
+      if (!occ_query(r.o + tm * r.d, occ, bf, coarse, r.o, inv_d, sk)) continue;
       if (cnt < max_per_ray) t0s[p * max_per_ray + cnt] = t0;
       cnt++;
     }
@@ -187,12 +218,11 @@ __global__ void k_debug_march(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, Oc
 
 // ---------------------------------------------------------------- host
 static const int MAX_SUBITER = 1000;       // PerformanceRenderer::MAX_SUBITER
-static const int64_t TARGET_BATCH = 1 << 22;
 
 void resize_renderbuffers(Renderer& r, int w, int h) {
   if (w <= 0 || h <= 0) throw Error(FVC_E_INVALID, "resolution must be positive");
   const int64_t n = (int64_t)w * h;
-  const int64_t cap = std::max<int64_t>(TARGET_BATCH, n) + 64;
+  const int64_t cap = std::max<int64_t>(r.target_batch, n) + 64;
   for (int b = 0; b < 2; b++) {
     r.rays[b].ensure(n * sizeof(RayState));
     r.ray_pix[b].ensure(n * sizeof(int));
@@ -241,6 +271,63 @@ void launch_write_image(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba,
   FVC_HIP(hipGetLastError());
 }
 
+// Largest t any ray of this view can march to: the AABB exit is at most the
+// distance to the farthest AABB corner, and never beyond `far`.
+static float max_ray_t(const fvc_raymarch_info& rm, const fvc_scene_info& sc) {
+  const V3 o = translation(rm.cam_info.cam2world);
+  const V3 from = cvt(sc.aabb_from), to = cvt(sc.aabb_to);
+  float tf = 0.0f;
+  for (int c = 0; c < 8; c++) {
+    const V3 corner = {(c & 1) ? to.x : from.x, (c & 2) ? to.y : from.y, (c & 4) ? to.z : from.z};
+    const V3 v = corner - o;
+    tf = std::max(tf, std::sqrt(dot3(v, v)));
+  }
+  return std::min(tf, rm.stepsize_info.far_);
+}
+
+// Build (or reuse) the step table for this stepsize and view.  Row m starts
+// at t_from_step(m + near_i) and runs the device recurrence until the bin
+// midpoint passes the largest reachable t.  Rows are skipped when the table
+// would exceed 64 Mi entries; the marchers then simply step.
+StepTableView step_table(Renderer& r, const fvc_raymarch_info& rm) {
+  const fvc_stepsize_info& st = rm.stepsize_info;
+  const float need = max_ray_t(rm, r.model->scene);
+  if (r.tab_rows >= 0 && std::memcmp(&r.tab_key, &st, sizeof(st)) == 0 && need <= r.tab_limit)
+    return StepTableView{r.tab_t.as<float>(), r.tab_off.as<int>(), r.tab_rows};
+  const float lim = need * 1.02f + 4.0f * st.dt_max;
+  const double rows_d = std::ceil((double)step_from_t(st, lim) - (double)st.near_i) + 2.0;
+  std::vector<float> t;
+  std::vector<int> off;
+  int rows = 0;
+  if (rows_d > 0 && rows_d < 200000) {
+    rows = (int)rows_d;
+    off.reserve(rows + 1);
+    for (int m = 0; m < rows && rows > 0; m++) {
+      off.push_back((int)t.size());
+      float t1 = t_from_step(st, (float)m + st.near_i);
+      t.push_back(t1);
+      for (int k = 0; k < 4 * rows + 4096; k++) {
+        const float t0 = t1;
+        t1 = t0 + step_dt(st, t0);
+        t.push_back(t1);
+        if ((t0 + t1) * 0.5f > lim) break;
+      }
+      if (t.size() > ((size_t)64 << 20)) rows = 0;
+    }
+    off.push_back((int)t.size());
+  }
+  if (rows > 0) {
+    r.tab_t.ensure(t.size() * sizeof(float));
+    r.tab_off.ensure(off.size() * sizeof(int));
+    FVC_HIP(hipMemcpy(r.tab_t.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    FVC_HIP(hipMemcpy(r.tab_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  r.tab_rows = rows;
+  r.tab_key = st;
+  r.tab_limit = rows > 0 ? lim : 3.0e38f;
+  return StepTableView{rows > 0 ? r.tab_t.as<float>() : nullptr, rows > 0 ? r.tab_off.as<int>() : nullptr, rows};
+}
+
 int steps_for(int n_alive, int64_t target, int max_steps) {
   int64_t k = target / std::max(n_alive, 1);
   if (k < 1) k = 1;
@@ -253,26 +340,29 @@ void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_re
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   if (W != r.width || H != r.height) resize_renderbuffers(r, W, H);
   hipStream_t s = r.stream();
+  const StepTableView tab = step_table(r, rm);
   Profiler& P = r.prof;
   hipEvent_t ev = P.begin(s);
   int n_alive = launch_init_rays(r, rm, s);
-  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n_alive * 52.0, 0);
+  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n_alive * 60.0, 0);
   int* cnt = r.counters.as<int>();
   int cur = 0, subiter = 0;
   int64_t n_eval = 0;
-  const int64_t target = std::min<int64_t>(TARGET_BATCH, r.sample_capacity - 64);
+  const int64_t target = std::min<int64_t>(r.target_batch, r.sample_capacity - 64);
   while (n_alive > 0 && subiter < MAX_SUBITER) {
-    const int n_steps = steps_for(n_alive, target, 16);
+    const int n_steps = steps_for(n_alive, target, r.max_steps);
     FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 3 * sizeof(int), s));
     const dim3 grid((n_alive + 255) / 256);
     ev = P.begin(s);
 #define FVC_MARCH(MS)                                                                                          \
   hipLaunchKernelGGL(k_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, rm.stepsize_info, m.occ, m.con,    \
-                     m.bitfield.as<uint8_t>(), r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),            \
+                     m.bitfield.as<uint8_t>(), m.coarse.as<uint32_t>(), tab, r.rays[cur].as<RayState>(),      \
+                     r.ray_entry.as<int4>(),                                                                  \
                      r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),     \
                      cnt + CNT_SAMPLES, cnt + CNT_STEPS)
     if (n_steps <= 8) FVC_MARCH(8);
-    else FVC_MARCH(16);
+    else if (n_steps <= 16) FVC_MARCH(16);
+    else FVC_MARCH(32);
 #undef FVC_MARCH
     FVC_HIP(hipGetLastError());
     const int hm = P.end(ev, s, "march");
@@ -319,11 +409,13 @@ void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int3
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   const int64_t n = (int64_t)W * H;
   hipStream_t s = r.stream();
+  const StepTableView tab = step_table(r, rm);
   DevBuf c, t;
   c.ensure(n * sizeof(int));
   t.ensure(std::max<int64_t>(1, n * max_per_ray) * sizeof(float));
   hipLaunchKernelGGL(k_debug_march, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, cvt(m.scene.aabb_from),
-                     cvt(m.scene.aabb_to), m.occ, m.bitfield.as<uint8_t>(), max_per_ray, c.as<int>(), t.as<float>());
+                     cvt(m.scene.aabb_to), m.occ, m.bitfield.as<uint8_t>(), m.coarse.as<uint32_t>(), tab, max_per_ray,
+                     c.as<int>(), t.as<float>());
   FVC_HIP(hipGetLastError());
   FVC_HIP(hipMemcpyAsync(counts_host, c.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
   if (max_per_ray > 0 && t0_host)

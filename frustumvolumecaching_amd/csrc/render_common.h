@@ -2,14 +2,19 @@
 // full renderer (render.hip) and the cache renderers (cache.hip).
 #pragma once
 #include "fvc_internal.h"
+#include "march.h"
 #include "wave.h"
 
 namespace fvc {
 
 // BaseRayInitInfo + SegmentedRayPayload of the reference
-// (renderer_common.h:65-88) in one 32-byte record: origin, direction, far,
-// next bin start t.
-struct RayState { float ox, oy, oz, dx, dy, dz, far_, t; };
+// (renderer_common.h:65-88) in one 48-byte record: origin, direction, far,
+// next bin start t, and the bin position in the step table (start index m,
+// bins advanced j, so t == T[m][j]).
+struct RayState {
+  float ox, oy, oz, dx, dy, dz, far_, t;
+  int m, j, pad0, pad1;
+};
 
 // slots of Renderer::counters
 enum { CNT_ALIVE = 0, CNT_SAMPLES = 1, CNT_CACHE = 2, CNT_STEPS = 3, CNT_TILES = 4, CNT_SLOTS = 5, CNT_OVERFLOW = 6 };
@@ -43,6 +48,7 @@ FVC_HD int to_8bit(float f) {
 }
 
 // host helpers (render.hip)
+StepTableView step_table(Renderer& r, const fvc_raymarch_info& rm);
 int read_counter(Renderer& r, int idx, hipStream_t s);
 void zero_counters(Renderer& r, hipStream_t s);
 int launch_init_rays(Renderer& r, const fvc_raymarch_info& rm, hipStream_t s);

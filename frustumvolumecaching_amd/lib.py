@@ -44,6 +44,7 @@ _SIGS = {
     "fvc_renderer_destroy": (None, [_P]),
     "fvc_renderer_set_stream": (C.c_int, [_P, _P]),
     "fvc_resize_renderbuffers": (C.c_int, [_P, C.c_int, C.c_int]),
+    "fvc_renderer_set_batching": (C.c_int, [_P, C.c_int64, C.c_int32]),
     "fvc_render_full": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.POINTER(abi.RenderStats)]),
     "fvc_cache_fill": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.POINTER(abi.InitStats)]),
     "fvc_render_cached": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.POINTER(abi.RenderStats)]),
@@ -184,6 +185,9 @@ class Renderer:
 
     def set_stream(self, stream):
         check(load().fvc_renderer_set_stream(self.h, _stream(stream)))
+
+    def set_batching(self, target_samples, max_samples_per_ray):
+        check(load().fvc_renderer_set_batching(self.h, int(target_samples), int(max_samples_per_ray)))
 
     def resizeRenderbuffers(self, width, height):
         check(load().fvc_resize_renderbuffers(self.h, int(width), int(height)))
