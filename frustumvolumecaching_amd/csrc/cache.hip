@@ -91,7 +91,7 @@ __global__ __launch_bounds__(1024) void k_fill_list(int n_tiles, const FillTile*
 // (cache_renderer.cu:83-287): skip empty slabs, emit the occupied samples of
 // the first non-empty one as a compacted stream.
 __global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list, fvc_stepsize_info st, OccParams occ,
-                                                    ContractParams con, const uint8_t* __restrict__ bf,
+                                                    ContractParams con, const uint64_t* __restrict__ bf,
                                                     const uint32_t* __restrict__ coarse_g, int bz,
                                                     FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
                                                     float* __restrict__ f_pos, float* __restrict__ f_t0,
@@ -368,7 +368,7 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats)
     FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, sizeof(int), s));
     ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_sample, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), rm.stepsize_info, m.occ,
-                       m.con, m.bitfield.as<uint8_t>(), m.coarse.as<uint32_t>(), C.bz, r.f_rays.as<FillRay>(),
+                       m.con, m.bitfield.as<uint64_t>(), m.coarse.as<uint32_t>(), C.bz, r.f_rays.as<FillRay>(),
                        r.f_tiles.as<FillTile>(),
                        r.f_pos.as<float>(), r.f_t0.as<float>(), r.f_meta.as<int>(), cnt + CNT_SAMPLES);
     FVC_HIP(hipGetLastError());
@@ -435,48 +435,39 @@ struct CacheView {
 // classify it as resample (outside / beyond the known range), cache hit
 // (set brick, nearest-voxel density > 0) or skip.
 template <int MAXS>
-__global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, int step_cutoff, fvc_stepsize_info st,
-                                                     OccParams occ, ContractParams con, const uint8_t* __restrict__ bf,
-                                                     const uint32_t* __restrict__ coarse_g, StepTableView tab,
+__global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, int trip_cutoff, MarchGrid mg,
+                                                     ContractParams con, const uint32_t* __restrict__ coarse_g,
                                                      CacheView cv, RayState* __restrict__ rays, int4* __restrict__ entry,
                                                      int* __restrict__ alive_flag, float* __restrict__ s_pos,
                                                      float* __restrict__ s_dir, float* __restrict__ s_t0,
                                                      float* __restrict__ c_frox, float* __restrict__ c_dir,
-                                                     float* __restrict__ c_t0, int* __restrict__ cnt_r,
-                                                     int* __restrict__ cnt_c, int* __restrict__ step_counter) {
+                                                     float* __restrict__ c_t0, uint64_t* __restrict__ order_out,
+                                                     int* __restrict__ cnt_r, int* __restrict__ cnt_c,
+                                                     int* __restrict__ step_counter) {
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
-  load_coarse(coarse_g, occ.nlvl, coarse);
+  load_coarse(coarse_g, mg.occ.nlvl, coarse);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   float tr[MAXS], tc[MAXS];
-  int nr = 0, nc = 0, taken = 0;
+  int nr = 0, nc = 0, bins = 0;
+  uint64_t order = 0;  // bit k set: the k-th sample of the ray went to the cache stream
   bool alive = false;
   RayState r{};
   V3 oc = {0, 0, 0}, dc = {0, 0, 0};
   if (valid) {
     r = rays[i];
     alive = true;
-    const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
-    oc = transform(cv.cam.world2cam, o.x, o.y, o.z, 1.0f);
-    dc = transform(cv.cam.world2cam, d.x, d.y, d.z, 0.0f);
-    const V3 inv_d = recip(d);
-    Skip sk = no_skip();
-    float t1 = r.t;
-    int j = r.j, iters = 0;
-    while (nr < n_steps && nc < n_steps && iters < step_cutoff) {
-      iters++;
-      taken++;
-      const float t0 = t1;
-      const float dt = step_dt(st, t0);
-      t1 = t0 + dt;
-      j++;
-      const float tm = (t0 + t1) * 0.5f;
-      if (tm > r.far_) { alive = false; break; }
-      if (in_skip(sk, tm)) {
-        taken += table_jump(tab, st, r.m, j, t1, sk.hi);
-        continue;
-      }
-      if (!occ_query(o + tm * d, occ, bf, coarse, o, inv_d, sk)) continue;
+    MarchRay mr = march_ray(r);
+    oc = transform(cv.cam.world2cam, mr.o.x, mr.o.y, mr.o.z, 1.0f);
+    dc = transform(cv.cam.world2cam, mr.d.x, mr.d.y, mr.d.z, 0.0f);
+    int trips = 0;
+    while (nr < n_steps && nc < n_steps && trips < trip_cutoff) {
+      trips++;
+      float t0;
+      const int res = march_trip(mg, coarse, mr, t0, bins);
+      if (res == TRIP_END) { alive = false; break; }
+      if (res != TRIP_SAMPLE) continue;
+      const float tm = (t0 + mr.t1) * 0.5f;
       float tf;
       const V3 f = cam2froxel(oc + tm * dc, cv.cam, cv.st, tf);
       const bool in_fov = f.x >= 0.0f && f.x < (float)cv.fx && f.y >= 0.0f && f.y < (float)cv.fy;
@@ -500,16 +491,16 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
 #pragma unroll
         for (int k = 0; k < MAXS; k++)
           if (k == nc) tc[k] = t0;
+        order |= 1ull << (nr + nc);
         nc++;
       }
     }
-    r.t = t1;
-    r.j = j;
+    r.t = mr.t1;
+    r.j = mr.j;
   }
-  const int off_r = wave_reserve(nr, cnt_r);
-  const int off_c = wave_reserve(nc, cnt_c);
-  const int wave_steps = wave_sum(taken);
-  if (lane_id() == 0 && wave_steps > 0) atomicAdd(step_counter, wave_steps);
+  int slot[3];
+  block_reserve<3>({nr, nc, bins}, {cnt_r, cnt_c, step_counter}, slot);
+  const int off_r = slot[0], off_c = slot[1];
   if (valid) {
     const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
     const V3 d01 = unit_to_01(d);
@@ -518,7 +509,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
     for (int k = 0; k < MAXS; k++) {
       if (k < nr) {
         const float t0 = tr[k];
-        const float tm2 = t0 + step_dt(st, t0) * 0.5f;
+        const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;
         const V3 p = contract(o + tm2 * d, con);
         const int64_t q = (int64_t)off_r + k;
         s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
@@ -531,7 +522,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
     for (int k = 0; k < MAXS; k++) {
       if (k < nc) {
         const float t0 = tc[k];
-        const float tm2 = t0 + step_dt(st, t0) * 0.5f;
+        const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;
         float tf;
         const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
         const int64_t q = (int64_t)off_c + k;
@@ -543,6 +534,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
     rays[i].t = r.t;
     rays[i].j = r.j;
     entry[i] = make_int4(off_r, nr, off_c, nc);
+    order_out[i] = order;
     alive_flag[i] = alive;
   }
 }
@@ -645,9 +637,10 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
 __global__ __launch_bounds__(256) void k_cache_composite(
     int n_alive, fvc_stepsize_info st, float alpha_thre, const RayState* __restrict__ rays_in,
     const int* __restrict__ pix_in, const float4* __restrict__ res_in, const int4* __restrict__ entry,
-    const int* __restrict__ alive_flag, const uint2* __restrict__ s_out, const float* __restrict__ s_t0,
-    const uint2* __restrict__ c_out, const float* __restrict__ c_t0, RayState* __restrict__ rays_out,
-    int* __restrict__ pix_out, float4* __restrict__ res_out, float4* __restrict__ results, int* __restrict__ counter) {
+    const uint64_t* __restrict__ order, const int* __restrict__ alive_flag, const uint2* __restrict__ s_out,
+    const float* __restrict__ s_t0, const uint2* __restrict__ c_out, const float* __restrict__ c_t0,
+    RayState* __restrict__ rays_out, int* __restrict__ pix_out, float4* __restrict__ res_out,
+    float4* __restrict__ results, int* __restrict__ counter) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   bool keep = false;
@@ -655,25 +648,38 @@ __global__ __launch_bounds__(256) void k_cache_composite(
   if (valid) {
     res = res_in[i];
     const int4 e = entry[i];
-    bool alive = alive_flag[i] != 0;
+    const uint64_t ord = order[i];
+    const int n = e.y + e.w;
+    bool visible = true;  // transmittance still above the early-stop threshold
     int kr = 0, kc = 0;
-    while (kr < e.y || kc < e.w) {
-      const float t0r = kr < e.y ? s_t0[e.x + kr] : 1e10f;
-      const float t0c = kc < e.w ? c_t0[e.z + kc] : 1e10f;
-      uint2 v;
-      float t0;
-      if (kc >= e.w || t0r < t0c) { v = s_out[e.x + kr]; t0 = t0r; kr++; }
-      else { v = c_out[e.z + kc]; t0 = t0c; kc++; }
-      if (!composite_sample(res, v, t0, st, alpha_thre)) { alive = false; break; }
+    // the two streams interleaved in march (= t0) order, batch by batch
+    for (int k0 = 0; k0 < n && visible; k0 += COMPOSITE_BATCH) {
+      uint2 v[COMPOSITE_BATCH];
+      float t[COMPOSITE_BATCH];
+#pragma unroll
+      for (int u = 0; u < COMPOSITE_BATCH; u++) {
+        if (k0 + u < n) {
+          const bool from_cache = (ord >> (k0 + u)) & 1u;
+          const int q = from_cache ? e.z + kc : e.x + kr;
+          kc += from_cache ? 1 : 0;
+          kr += from_cache ? 0 : 1;
+          v[u] = (from_cache ? c_out : s_out)[q];
+          t[u] = (from_cache ? c_t0 : s_t0)[q];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < COMPOSITE_BATCH; u++)
+        if (visible && k0 + u < n) visible = composite_sample(res, v[u], t[u], st, alpha_thre);
     }
-    keep = alive;
+    keep = visible && alive_flag[i] != 0;
     if (!keep) results[pix_in[i]] = res;
   }
-  const int slot = wave_reserve(keep ? 1 : 0, counter);
+  int slot[1];
+  block_reserve<1>({keep ? 1 : 0}, {counter}, slot);
   if (keep) {
-    rays_out[slot] = rays_in[i];
-    pix_out[slot] = pix_in[i];
-    res_out[slot] = res;
+    rays_out[slot[0]] = rays_in[i];
+    pix_out[slot[0]] = pix_in[i];
+    res_out[slot[0]] = res;
   }
 }
 
@@ -696,7 +702,7 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
   cv.table = C.table.as<int>();
   cv.known = C.known.as<float>();
   cv.pool = C.pool.as<Voxel>();
-  const StepTableView tab = step_table(r, rm);
+  const MarchGrid mg = march_grid(r, rm);
 
   Profiler& P = r.prof;
   hipEvent_t ev = P.begin(s);
@@ -710,15 +716,17 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
     const int n_steps = steps_for(n_alive, target, r.max_steps);
     FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 3 * sizeof(int), s));
     const dim3 grid((n_alive + 255) / 256);
+    // loop trips per ray and launch; a performance knob only (a ray resumes
+    // where it stopped), like the reference's step_cutoff
+    // (cache_performance_renderer.cu:72)
     const int cutoff = 4096;
     ev = P.begin(s);
 #define FVC_CACHE_MARCH(MS)                                                                                        \
-  hipLaunchKernelGGL(k_cache_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, cutoff, rm.stepsize_info, m.occ, \
-                     m.con, m.bitfield.as<uint8_t>(), m.coarse.as<uint32_t>(), tab, cv, r.rays[cur].as<RayState>(), \
-                     r.ray_entry.as<int4>(),                                                                        \
+  hipLaunchKernelGGL(k_cache_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, cutoff, mg, m.con,               \
+                     m.coarse.as<uint32_t>(), cv, r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),              \
                      r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),          \
-                     r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(), cnt + CNT_SAMPLES, cnt + CNT_CACHE, \
-                     cnt + CNT_STEPS)
+                     r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(), r.ray_order.as<uint64_t>(),   \
+                     cnt + CNT_SAMPLES, cnt + CNT_CACHE, cnt + CNT_STEPS)
     if (n_steps <= 8) FVC_CACHE_MARCH(8);
     else if (n_steps <= 16) FVC_CACHE_MARCH(16);
     else FVC_CACHE_MARCH(32);
@@ -750,12 +758,13 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
     ev = P.begin(s);
     hipLaunchKernelGGL(k_cache_composite, grid, dim3(256), 0, s, n_alive, rm.stepsize_info, m.scene.alpha_thre,
                        r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
-                       r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.s_out.as<uint2>(), r.s_t0.as<float>(),
-                       r.c_out.as<uint2>(), r.c_t0.as<float>(), r.rays[cur ^ 1].as<RayState>(),
+                       r.ray_entry.as<int4>(), r.ray_order.as<uint64_t>(), r.ray_alive.as<int>(),
+                       r.s_out.as<uint2>(), r.s_t0.as<float>(), r.c_out.as<uint2>(), r.c_t0.as<float>(),
+                       r.rays[cur ^ 1].as<RayState>(),
                        r.ray_pix[cur ^ 1].as<int>(), r.ray_res[cur ^ 1].as<float4>(), r.results.as<float4>(),
                        cnt + CNT_ALIVE);
     FVC_HIP(hipGetLastError());
-    P.end(ev, s, "cache_composite", nr + nc, 12.0 * (nr + nc) + 124.0 * n_alive, 0);
+    P.end(ev, s, "cache_composite", nr + nc, 12.0 * (nr + nc) + 132.0 * n_alive, 0);
     n_alive = read_counter(r, CNT_ALIVE, s);
     cur ^= 1;
     subiter++;

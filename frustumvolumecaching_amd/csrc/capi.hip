@@ -6,6 +6,7 @@
 #include <string>
 
 #include "fvc_internal.h"
+#include "march.h"
 
 namespace fvc {
 static thread_local std::string g_last_error;
@@ -108,17 +109,8 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
             }
       m.bitfield.ensure(m.bitfield_host.size());
       FVC_HIP(hipMemcpy(m.bitfield.p, m.bitfield_host.data(), m.bitfield_host.size(), hipMemcpyHostToDevice));
-      // coarse empty-space table: bit (c x 16 + cy) x 16 + cz of cascade l is set
-      // when any cell of the 8^3 block is occupied
-      std::vector<uint32_t> coarse((size_t)scene->grid_nlvl * 128, 0u);
-      for (int l = 0; l < scene->grid_nlvl; l++)
-        for (int x = 0; x < 128; x++)
-          for (int y = 0; y < 128; y++)
-            for (int z = 0; z < 128; z++) {
-              if (!occ[l * vals + ((int64_t)x * 128 + y) * 128 + z]) continue;
-              const int cb = ((x >> 3) * 16 + (y >> 3)) * 16 + (z >> 3);
-              coarse[(size_t)l * 128 + (cb >> 5)] |= 1u << (cb & 31);
-            }
+      // coarse empty-space table of the marchers
+      const std::vector<uint32_t> coarse = build_coarse(occ, scene->grid_nlvl);
       m.coarse.ensure(coarse.size() * sizeof(uint32_t));
       FVC_HIP(hipMemcpy(m.coarse.p, coarse.data(), coarse.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
       m.table.ensure(n_pos * sizeof(uint16_t));
@@ -127,15 +119,8 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
       pack_fragments(p_base, p_head, frags);
       m.frags.ensure(frags.size() * sizeof(uint16_t));
       FVC_HIP(hipMemcpy(m.frags.p, frags.data(), frags.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
-      // grid_occupied_at constants (raymarch_common.h:660-666)
-      const float S = (float)(1 << (scene->grid_nlvl - 1));
+      m.occ = occ_params(*scene);
       const V3 from = cvt(scene->aabb_from), to = cvt(scene->aabb_to);
-      m.occ.unit_off = from / S;
-      m.occ.unit_ext = (to - from) / S;
-      m.occ.unit_inv = V3{1.0f / m.occ.unit_ext.x, 1.0f / m.occ.unit_ext.y, 1.0f / m.occ.unit_ext.z};
-      m.occ.pow2 = is_pow2f(m.occ.unit_ext.x) && is_pow2f(m.occ.unit_ext.y) && is_pow2f(m.occ.unit_ext.z);
-      m.occ.nlvl = scene->grid_nlvl;
-      m.occ.normalized = scene->normalized;
       // apply_contraction constants (raymarch_common.h:610-653)
       m.con.type = scene->contraction_type;
       if (scene->contraction_type == FVC_CONTRACTION_AABB) {

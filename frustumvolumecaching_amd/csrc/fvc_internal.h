@@ -239,6 +239,7 @@ struct Renderer {
   DevBuf ray_res[2];     // float4 rgb + T
   DevBuf results;        // float4 [H*W] final
   DevBuf ray_entry;      // int4 per ray: offset/count per stream
+  DevBuf ray_order;      // uint64 per ray: stream of each sample, in march order
   DevBuf ray_alive;      // int32 per ray
   DevBuf counters;       // int32 device counters
   int* pinned = nullptr; // pinned host mirror of the counters

@@ -112,6 +112,8 @@ struct OccParams {
   V3 unit_off;    // aabb_from / 2^(nlvl-1)
   V3 unit_ext;    // (aabb_to - aabb_from) / 2^(nlvl-1)
   V3 unit_inv;    // 1 / unit_ext, exact when pow2
+  V3 box_c;       // world centre of the cascades (unit coordinate 0.5)
+  V3 box_e;       // world extent of cascade 0 (unit_ext, or 1 when normalized)
   int nlvl;
   int normalized;
   int pow2;       // every unit_ext component is a power of two
@@ -210,6 +212,22 @@ FVC_HD V3 unit_to_01(V3 d) { return (d + 1.0f) / 2.0f; }
 FVC_HD bool is_pow2f(float x) {
   int e;
   return x > 0.0f && frexpf(x, &e) == 0.5f;
+}
+
+// grid_occupied_at constants of a scene (raymarch_common.h:660-666)
+inline OccParams occ_params(const fvc_scene_info& sc) {
+  OccParams o{};
+  const float S = (float)(1 << (sc.grid_nlvl - 1));
+  const V3 from = cvt(sc.aabb_from), to = cvt(sc.aabb_to);
+  o.unit_off = from / S;
+  o.unit_ext = (to - from) / S;
+  o.unit_inv = V3{1.0f / o.unit_ext.x, 1.0f / o.unit_ext.y, 1.0f / o.unit_ext.z};
+  o.pow2 = is_pow2f(o.unit_ext.x) && is_pow2f(o.unit_ext.y) && is_pow2f(o.unit_ext.z);
+  o.nlvl = sc.grid_nlvl;
+  o.normalized = sc.normalized;
+  o.box_e = sc.normalized ? V3{1.0f, 1.0f, 1.0f} : o.unit_ext;
+  o.box_c = sc.normalized ? V3{0.5f, 0.5f, 0.5f} : o.unit_off + 0.5f * o.unit_ext;
+  return o;
 }
 
 }  // namespace fvc

@@ -26,29 +26,15 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, V3 aabb
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
   const bool valid = x < W && y < H;
-  bool alive = false;
   RayState st{};
-  if (valid) {
-    const Ray r = gen_ray(rm.cam_info, (float)x + 0.5f, (float)y + 0.5f);
-    float t0, t1;
-    const bool hit = ray_aabb(r, aabb_from, aabb_to, t0, t1);
-    const fvc_stepsize_info& s = rm.stepsize_info;
-    st.ox = r.o.x; st.oy = r.o.y; st.oz = r.o.z;
-    st.dx = r.d.x; st.dy = r.d.y; st.dz = r.d.z;
-    st.far_ = hit ? fminf_(t1, s.far_) : s.far_;
-    // valid_t_from_t with its bin index kept: t = t_from_step(m + near_i)
-    const float mf = hit ? roundf(step_from_t(s, fmaxf_(t0, s.near_)) - s.near_i) : 0.0f;
-    st.t = hit ? t_from_step(s, mf + s.near_i) : s.near_;
-    st.m = (int)mf;
-    st.j = 0;
-    alive = hit;
-    results[(int64_t)y * W + x] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-  }
-  const int slot = wave_reserve(alive ? 1 : 0, counter);
+  const bool alive = valid && init_ray_state(rm, aabb_from, aabb_to, x, y, st);
+  if (valid) results[(int64_t)y * W + x] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  int slot[1];
+  block_reserve<1>({alive ? 1 : 0}, {counter}, slot);
   if (alive) {
-    rays[slot] = st;
-    pix[slot] = y * W + x;
-    rres[slot] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    rays[slot[0]] = st;
+    pix[slot[0]] = y * W + x;
+    rres[slot[0]] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   }
 }
 
@@ -56,62 +42,49 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, V3 aabb
 // n_steps occupied bin midpoints or t_mid > far; emit contracted position,
 // unit_to_01(dir) and t0 of every occupied sample into the compacted stream.
 template <int MAXS>
-__global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, fvc_stepsize_info st, OccParams occ,
-                                               ContractParams con, const uint8_t* __restrict__ bf,
-                                               const uint32_t* __restrict__ coarse_g, StepTableView tab,
-                                               RayState* __restrict__ rays, int4* __restrict__ entry,
-                                               int* __restrict__ alive_flag, float* __restrict__ s_pos,
-                                               float* __restrict__ s_dir, float* __restrict__ s_t0,
-                                               int* __restrict__ s_counter, int* __restrict__ step_counter) {
+__global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, MarchGrid mg, ContractParams con,
+                                               const uint32_t* __restrict__ coarse_g, RayState* __restrict__ rays,
+                                               int4* __restrict__ entry, int* __restrict__ alive_flag,
+                                               float* __restrict__ s_pos, float* __restrict__ s_dir,
+                                               float* __restrict__ s_t0, int* __restrict__ s_counter,
+                                               int* __restrict__ step_counter) {
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
-  load_coarse(coarse_g, occ.nlvl, coarse);
+  load_coarse(coarse_g, mg.occ.nlvl, coarse);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   float ts[MAXS];
-  int cnt = 0, taken = 0;
+  int cnt = 0, bins = 0;
   bool alive = false;
-  RayState r{};
+  RayState rs{};
   if (valid) {
-    r = rays[i];
+    rs = rays[i];
     alive = true;
-    const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
-    const V3 inv_d = recip(d);
-    Skip sk = no_skip();
-    float t1 = r.t;
-    int j = r.j;
+    MarchRay r = march_ray(rs);
     while (cnt < n_steps) {
-      taken++;
-      const float t0 = t1;
-      const float dt = step_dt(st, t0);
-      t1 = t0 + dt;
-      j++;
-      const float tm = (t0 + t1) * 0.5f;
-      if (tm > r.far_) { alive = false; break; }
-      if (in_skip(sk, tm)) {
-        taken += table_jump(tab, st, r.m, j, t1, sk.hi);
-        continue;
-      }
-      if (occ_query(o + tm * d, occ, bf, coarse, o, inv_d, sk)) {
+      float t0;
+      const int res = march_trip(mg, coarse, r, t0, bins);
+      if (res == TRIP_END) { alive = false; break; }
+      if (res == TRIP_SAMPLE) {
 #pragma unroll
         for (int k = 0; k < MAXS; k++)
           if (k == cnt) ts[k] = t0;
         cnt++;
       }
     }
-    r.t = t1;
-    r.j = j;
+    rs.t = r.t1;
+    rs.j = r.j;
   }
-  const int off = wave_reserve(cnt, s_counter);
-  const int wave_steps = wave_sum(taken);
-  if (lane_id() == 0 && wave_steps > 0) atomicAdd(step_counter, wave_steps);
+  int slot[2];
+  block_reserve<2>({cnt, bins}, {s_counter, step_counter}, slot);
+  const int off = slot[0];
   if (valid) {
-    const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
+    const V3 o = {rs.ox, rs.oy, rs.oz}, d = {rs.dx, rs.dy, rs.dz};
     const V3 d01 = unit_to_01(d);
 #pragma unroll
     for (int k = 0; k < MAXS; k++) {
       if (k < cnt) {
         const float t0 = ts[k];
-        const float t1 = t0 + step_dt(st, t0);
+        const float t1 = t0 + step_dt(mg.st, t0);
         const float tm = (t0 + t1) * 0.5f;
         const V3 p = contract(o + tm * d, con);
         const int64_t q = (int64_t)off + k;
@@ -120,15 +93,17 @@ __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, fvc_ste
         s_t0[q] = t0;
       }
     }
-    rays[i].t = r.t;
-    rays[i].j = r.j;
+    rays[i].t = rs.t;
+    rays[i].j = rs.j;
     entry[i] = make_int4(off, cnt, 0, 0);
     alive_flag[i] = alive;
   }
 }
 
 // accumulate_performance_kernel (performance_renderer.cu:71-125) + the ray
-// compaction of the next sub-iteration (renderer.cu:158-189).
+// compaction of the next sub-iteration (renderer.cu:158-189).  The samples
+// of a ray are read in batches of COMPOSITE_BATCH independent loads ahead of
+// the sequential compositing.
 __global__ __launch_bounds__(256) void k_composite(int n_alive, fvc_stepsize_info st, float alpha_thre,
                                                    const RayState* __restrict__ rays_in, const int* __restrict__ pix_in,
                                                    const float4* __restrict__ res_in, const int4* __restrict__ entry,
@@ -143,18 +118,30 @@ __global__ __launch_bounds__(256) void k_composite(int n_alive, fvc_stepsize_inf
   if (valid) {
     res = res_in[i];
     const int4 e = entry[i];
-    bool alive = alive_flag[i] != 0;
-    for (int k = 0; k < e.y; k++) {
-      if (!composite_sample(res, s_out[e.x + k], s_t0[e.x + k], st, alpha_thre)) { alive = false; break; }
+    bool visible = true;  // transmittance still above the early-stop threshold
+    for (int k0 = 0; k0 < e.y && visible; k0 += COMPOSITE_BATCH) {
+      uint2 v[COMPOSITE_BATCH];
+      float t[COMPOSITE_BATCH];
+#pragma unroll
+      for (int u = 0; u < COMPOSITE_BATCH; u++) {
+        if (k0 + u < e.y) {
+          v[u] = s_out[e.x + k0 + u];
+          t[u] = s_t0[e.x + k0 + u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < COMPOSITE_BATCH; u++)
+        if (visible && k0 + u < e.y) visible = composite_sample(res, v[u], t[u], st, alpha_thre);
     }
-    keep = alive;
+    keep = visible && alive_flag[i] != 0;
     if (!keep) results[pix_in[i]] = res;
   }
-  const int slot = wave_reserve(keep ? 1 : 0, counter);
+  int slot[1];
+  block_reserve<1>({keep ? 1 : 0}, {counter}, slot);
   if (keep) {
-    rays_out[slot] = rays_in[i];
-    pix_out[slot] = pix_in[i];
-    res_out[slot] = res;
+    rays_out[slot[0]] = rays_in[i];
+    pix_out[slot[0]] = pix_in[i];
+    res_out[slot[0]] = res;
   }
 }
 
@@ -174,41 +161,25 @@ __global__ void k_write_image(int W, int H, const float4* __restrict__ results, 
 }
 
 // march only: counts of occupied samples + first max_per_ray t0 per pixel
-__global__ void k_debug_march(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, OccParams occ,
-                              const uint8_t* __restrict__ bf, const uint32_t* __restrict__ coarse_g,
-                              StepTableView tab, int max_per_ray, int* __restrict__ counts,
+__global__ void k_debug_march(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, MarchGrid mg,
+                              const uint32_t* __restrict__ coarse_g, int max_per_ray, int* __restrict__ counts,
                               float* __restrict__ t0s) {
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
-  load_coarse(coarse_g, occ.nlvl, coarse);
+  load_coarse(coarse_g, mg.occ.nlvl, coarse);
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x >= W || y >= H) return;
   const int64_t p = (int64_t)y * W + x;
-  const fvc_stepsize_info& s = rm.stepsize_info;
-  const Ray r = gen_ray(rm.cam_info, (float)x + 0.5f, (float)y + 0.5f);
-  float a0, a1;
-  const bool hit = ray_aabb(r, aabb_from, aabb_to, a0, a1);
+  RayState rs;
   int cnt = 0;
-  if (hit) {
-    const float far_ = fminf_(a1, s.far_);
-    const V3 inv_d = recip(r.d);
-    Skip sk = no_skip();
-    const float mf = roundf(step_from_t(s, fmaxf_(a0, s.near_)) - s.near_i);
-    const int m = (int)mf;
-    float t1 = t_from_step(s, mf + s.near_i);
-    int j = 0;
+  if (init_ray_state(rm, aabb_from, aabb_to, x, y, rs)) {
+    MarchRay r = march_ray(rs);
+    int bins = 0;
     while (true) {
-      const float t0 = t1;
-      const float dt = step_dt(s, t0);
-      t1 = t0 + dt;
-      j++;
-      const float tm = (t0 + t1) * 0.5f;
-      if (tm > far_) break;
-      if (in_skip(sk, tm)) {
-        table_jump(tab, s, m, j, t1, sk.hi);
-        continue;
-      }
-      if (!occ_query(r.o + tm * r.d, occ, bf, coarse, r.o, inv_d, sk)) continue;
+      float t0;
+      const int res = march_trip(mg, coarse, r, t0, bins);
+      if (res == TRIP_END) break;
+      if (res != TRIP_SAMPLE) continue;
       if (cnt < max_per_ray) t0s[p * max_per_ray + cnt] = t0;
       cnt++;
     }
@@ -230,6 +201,7 @@ void resize_renderbuffers(Renderer& r, int w, int h) {
   }
   r.results.ensure(n * sizeof(float4));
   r.ray_entry.ensure(n * sizeof(int4));
+  r.ray_order.ensure(n * sizeof(uint64_t));
   r.ray_alive.ensure(n * sizeof(int));
   r.counters.ensure(64 * sizeof(int));
   r.s_pos.ensure(cap * 3 * sizeof(float));
@@ -285,47 +257,40 @@ static float max_ray_t(const fvc_raymarch_info& rm, const fvc_scene_info& sc) {
   return std::min(tf, rm.stepsize_info.far_);
 }
 
-// Build (or reuse) the step table for this stepsize and view.  Row m starts
-// at t_from_step(m + near_i) and runs the device recurrence until the bin
-// midpoint passes the largest reachable t.  Rows are skipped when the table
-// would exceed 64 Mi entries; the marchers then simply step.
+// Build (or reuse) the step table for this stepsize and view (see
+// build_step_rows); when it is disabled the marchers simply step.
 StepTableView step_table(Renderer& r, const fvc_raymarch_info& rm) {
   const fvc_stepsize_info& st = rm.stepsize_info;
   const float need = max_ray_t(rm, r.model->scene);
-  if (r.tab_rows >= 0 && std::memcmp(&r.tab_key, &st, sizeof(st)) == 0 && need <= r.tab_limit)
-    return StepTableView{r.tab_t.as<float>(), r.tab_off.as<int>(), r.tab_rows};
-  const float lim = need * 1.02f + 4.0f * st.dt_max;
-  const double rows_d = std::ceil((double)step_from_t(st, lim) - (double)st.near_i) + 2.0;
-  std::vector<float> t;
-  std::vector<int> off;
-  int rows = 0;
-  if (rows_d > 0 && rows_d < 200000) {
-    rows = (int)rows_d;
-    off.reserve(rows + 1);
-    for (int m = 0; m < rows && rows > 0; m++) {
-      off.push_back((int)t.size());
-      float t1 = t_from_step(st, (float)m + st.near_i);
-      t.push_back(t1);
-      for (int k = 0; k < 4 * rows + 4096; k++) {
-        const float t0 = t1;
-        t1 = t0 + step_dt(st, t0);
-        t.push_back(t1);
-        if ((t0 + t1) * 0.5f > lim) break;
-      }
-      if (t.size() > ((size_t)64 << 20)) rows = 0;
+  StepTableView v = step_view(st);
+  if (!(r.tab_rows >= 0 && std::memcmp(&r.tab_key, &st, sizeof(st)) == 0 && need <= r.tab_limit)) {
+    const float lim = need * 1.02f + 4.0f * st.dt_max;
+    std::vector<float> t;
+    std::vector<int> off;
+    const int rows = build_step_rows(st, lim, t, off);
+    if (rows > 0) {
+      r.tab_t.ensure(t.size() * sizeof(float));
+      r.tab_off.ensure(off.size() * sizeof(int));
+      FVC_HIP(hipMemcpy(r.tab_t.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+      FVC_HIP(hipMemcpy(r.tab_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
     }
-    off.push_back((int)t.size());
+    r.tab_rows = rows;
+    r.tab_key = st;
+    r.tab_limit = rows > 0 ? lim : 3.0e38f;
   }
-  if (rows > 0) {
-    r.tab_t.ensure(t.size() * sizeof(float));
-    r.tab_off.ensure(off.size() * sizeof(int));
-    FVC_HIP(hipMemcpy(r.tab_t.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
-    FVC_HIP(hipMemcpy(r.tab_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
-  }
-  r.tab_rows = rows;
-  r.tab_key = st;
-  r.tab_limit = rows > 0 ? lim : 3.0e38f;
-  return StepTableView{rows > 0 ? r.tab_t.as<float>() : nullptr, rows > 0 ? r.tab_off.as<int>() : nullptr, rows};
+  v.t = r.tab_rows > 0 ? r.tab_t.as<float>() : nullptr;
+  v.off = r.tab_rows > 0 ? r.tab_off.as<int>() : nullptr;
+  v.n_rows = r.tab_rows;
+  return v;
+}
+
+MarchGrid march_grid(Renderer& r, const fvc_raymarch_info& rm) {
+  MarchGrid g;
+  g.occ = r.model->occ;
+  g.st = rm.stepsize_info;
+  g.tab = step_table(r, rm);
+  g.bf64 = r.model->bitfield.as<uint64_t>();
+  return g;
 }
 
 int steps_for(int n_alive, int64_t target, int max_steps) {
@@ -340,7 +305,7 @@ void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_re
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   if (W != r.width || H != r.height) resize_renderbuffers(r, W, H);
   hipStream_t s = r.stream();
-  const StepTableView tab = step_table(r, rm);
+  const MarchGrid mg = march_grid(r, rm);
   Profiler& P = r.prof;
   hipEvent_t ev = P.begin(s);
   int n_alive = launch_init_rays(r, rm, s);
@@ -354,12 +319,11 @@ void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_re
     FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 3 * sizeof(int), s));
     const dim3 grid((n_alive + 255) / 256);
     ev = P.begin(s);
-#define FVC_MARCH(MS)                                                                                          \
-  hipLaunchKernelGGL(k_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, rm.stepsize_info, m.occ, m.con,    \
-                     m.bitfield.as<uint8_t>(), m.coarse.as<uint32_t>(), tab, r.rays[cur].as<RayState>(),      \
-                     r.ray_entry.as<int4>(),                                                                  \
-                     r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),     \
-                     cnt + CNT_SAMPLES, cnt + CNT_STEPS)
+#define FVC_MARCH(MS)                                                                                        \
+  hipLaunchKernelGGL(k_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, mg, m.con, m.coarse.as<uint32_t>(), \
+                     r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),                 \
+                     r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(), cnt + CNT_SAMPLES,          \
+                     cnt + CNT_STEPS)
     if (n_steps <= 8) FVC_MARCH(8);
     else if (n_steps <= 16) FVC_MARCH(16);
     else FVC_MARCH(32);
@@ -409,13 +373,12 @@ void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int3
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   const int64_t n = (int64_t)W * H;
   hipStream_t s = r.stream();
-  const StepTableView tab = step_table(r, rm);
+  const MarchGrid mg = march_grid(r, rm);
   DevBuf c, t;
   c.ensure(n * sizeof(int));
   t.ensure(std::max<int64_t>(1, n * max_per_ray) * sizeof(float));
   hipLaunchKernelGGL(k_debug_march, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, cvt(m.scene.aabb_from),
-                     cvt(m.scene.aabb_to), m.occ, m.bitfield.as<uint8_t>(), m.coarse.as<uint32_t>(), tab, max_per_ray,
-                     c.as<int>(), t.as<float>());
+                     cvt(m.scene.aabb_to), mg, m.coarse.as<uint32_t>(), max_per_ray, c.as<int>(), t.as<float>());
   FVC_HIP(hipGetLastError());
   FVC_HIP(hipMemcpyAsync(counts_host, c.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
   if (max_per_ray > 0 && t0_host)

@@ -16,6 +16,43 @@ struct RayState {
   int m, j, pad0, pad1;
 };
 
+// Ray of pixel (x, y) clipped to the scene AABB, its start snapped by
+// valid_t_from_t with the bin index kept: t = t_from_step(m + near_i)
+// (initRayPayloads_split_kernel, renderer.cu:86-134, deterministic branch).
+// Returns false when the ray misses the AABB.
+__device__ __forceinline__ bool init_ray_state(const fvc_raymarch_info& rm, V3 aabb_from, V3 aabb_to, int x, int y,
+                                               RayState& st) {
+  const Ray r = gen_ray(rm.cam_info, (float)x + 0.5f, (float)y + 0.5f);
+  float t0, t1;
+  const bool hit = ray_aabb(r, aabb_from, aabb_to, t0, t1);
+  const fvc_stepsize_info& s = rm.stepsize_info;
+  st.ox = r.o.x; st.oy = r.o.y; st.oz = r.o.z;
+  st.dx = r.d.x; st.dy = r.d.y; st.dz = r.d.z;
+  st.far_ = hit ? fminf_(t1, s.far_) : s.far_;
+  const float mf = hit ? roundf(step_from_t(s, fmaxf_(t0, s.near_)) - s.near_i) : 0.0f;
+  st.t = hit ? t_from_step(s, mf + s.near_i) : s.near_;
+  st.m = (int)mf;
+  st.j = 0;
+  st.pad0 = st.pad1 = 0;
+  return hit;
+}
+
+__device__ __forceinline__ MarchRay march_ray(const RayState& s) {
+  MarchRay r;
+  r.o = V3{s.ox, s.oy, s.oz};
+  r.d = V3{s.dx, s.dy, s.dz};
+  r.inv_d = recip(r.d);
+  r.far_ = s.far_;
+  r.m = s.m;
+  r.j = s.j;
+  r.t1 = s.t;
+  r.sk = no_skip();
+  return r;
+}
+
+// samples a compositing lane loads ahead
+static const int COMPOSITE_BATCH = 8;
+
 // slots of Renderer::counters
 enum { CNT_ALIVE = 0, CNT_SAMPLES = 1, CNT_CACHE = 2, CNT_STEPS = 3, CNT_TILES = 4, CNT_SLOTS = 5, CNT_OVERFLOW = 6 };
 
@@ -49,6 +86,7 @@ FVC_HD int to_8bit(float f) {
 
 // host helpers (render.hip)
 StepTableView step_table(Renderer& r, const fvc_raymarch_info& rm);
+MarchGrid march_grid(Renderer& r, const fvc_raymarch_info& rm);
 int read_counter(Renderer& r, int idx, hipStream_t s);
 void zero_counters(Renderer& r, hipStream_t s);
 int launch_init_rays(Renderer& r, const fvc_raymarch_info& rm, hipStream_t s);

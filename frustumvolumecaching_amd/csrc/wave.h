@@ -32,6 +32,39 @@ __device__ __forceinline__ int wave_reserve(int v, int* counter) {
   return base + prefix;
 }
 
+// Reserve v[c] consecutive slots of *ctr[c] per thread, for each of the N
+// counters, with one atomic per block and counter; slot[c] receives the
+// thread's first slot.  Slots are consecutive within the block in thread
+// order.  Every thread of the block must call it, at most once per kernel;
+// blockDim.x must be a multiple of 64, at most 1024.
+template <int N>
+__device__ __forceinline__ void block_reserve(const int (&v)[N], int* const (&ctr)[N], int (&slot)[N]) {
+  __shared__ int s_off[N][16];
+  __shared__ int s_base[N];
+  const int wave = (int)(threadIdx.x >> 6), n_waves = (int)(blockDim.x >> 6);
+  int pre[N];
+#pragma unroll
+  for (int c = 0; c < N; c++) {
+    int tot;
+    pre[c] = wave_excl_scan(v[c], tot);
+    if (lane_id() == 0) s_off[c][wave] = tot;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < N) {
+    const int c = (int)threadIdx.x;
+    int run = 0;
+    for (int w = 0; w < n_waves; w++) {
+      const int t = s_off[c][w];
+      s_off[c][w] = run;
+      run += t;
+    }
+    s_base[c] = run > 0 ? atomicAdd(ctr[c], run) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < N; c++) slot[c] = s_base[c] + s_off[c][wave] + pre[c];
+}
+
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
