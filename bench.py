@@ -63,7 +63,7 @@ def main():
     model = lib.NerfNetwork.from_workload(wl)
     r = lib.Renderer(model)
     if a.batch or a.max_steps:
-        r.set_batching(a.batch or (1 << 22), a.max_steps or 16)
+        r.set_batching(a.batch or (1 << 25), a.max_steps or 32)
     W, H = a.width, a.height
     n_px = W * H
     img = torch.zeros(n_px * 4, dtype=torch.uint8, device="cuda")

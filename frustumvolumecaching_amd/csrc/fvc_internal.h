@@ -226,8 +226,8 @@ struct Renderer {
   // sub-iteration batching: samples per launch targeted, and the cap on
   // samples per ray per launch (the reference's TARGET_BATCH_SIZE and
   // MAX_N_STEPS_PER_SUBITER, performance_renderer.h:54-58)
-  int64_t target_batch = 1 << 22;
-  int max_steps = 16;
+  int64_t target_batch = 1 << 25;
+  int max_steps = 32;
   // step table of the marchers (render.hip: step_table)
   DevBuf tab_t, tab_off;
   fvc_stepsize_info tab_key{};

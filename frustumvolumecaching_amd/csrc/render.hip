@@ -193,7 +193,9 @@ static const int MAX_SUBITER = 1000;       // PerformanceRenderer::MAX_SUBITER
 void resize_renderbuffers(Renderer& r, int w, int h) {
   if (w <= 0 || h <= 0) throw Error(FVC_E_INVALID, "resolution must be positive");
   const int64_t n = (int64_t)w * h;
-  const int64_t cap = std::max<int64_t>(r.target_batch, n) + 64;
+  // a launch emits at most n_alive * n_steps <= max(target, n) samples per
+  // stream, and never more than n * max_steps
+  const int64_t cap = std::min<int64_t>(std::max<int64_t>(r.target_batch, n), n * r.max_steps) + 64;
   for (int b = 0; b < 2; b++) {
     r.rays[b].ensure(n * sizeof(RayState));
     r.ray_pix[b].ensure(n * sizeof(int));

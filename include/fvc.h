@@ -225,7 +225,7 @@ void fvc_renderer_destroy(fvc_renderer* renderer);
 /* Use the caller's HIP stream for render calls (NULL = renderer's own stream). */
 int fvc_renderer_set_stream(fvc_renderer* renderer, void* stream);
 /* Sub-iteration batching (the reference's TARGET_BATCH_SIZE = 2^21 and
- * MAX_N_STEPS_PER_SUBITER = 8, performance_renderer.h:54-58; here 2^22 and 16
+ * MAX_N_STEPS_PER_SUBITER = 8, performance_renderer.h:54-58; here 2^25 and 32
  * by default): samples targeted per launch, and the cap on samples per ray
  * per launch (1..32).  Changes launch sizes only, never the image. */
 int fvc_renderer_set_batching(fvc_renderer* renderer, int64_t target_samples, int32_t max_samples_per_ray);
