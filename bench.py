@@ -38,7 +38,23 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=16, help="rows of the CPU baseline band")
     ap.add_argument("--batch", type=int, default=0, help="target samples per launch (0 = library default)")
     ap.add_argument("--max-steps", type=int, default=0, help="cap on samples per ray per launch")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (reference point for rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """One plain device-to-device copy of a known size: the HBM counters of
+    its dispatch calibrate FETCH_SIZE / WRITE_SIZE of the render kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "ok": bool(dst[-1].item() == n - 1)}), flush=True)
 
 
 def log(*a):
@@ -51,6 +67,9 @@ def main():
         print(f"bench.py: this render path runs on one GPU; --gpus {a.gpus} is not supported "
               f"(multi-GPU sharding is a design note only)", flush=True)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     import numpy as np
     import torch
 
@@ -181,8 +200,17 @@ def main():
     dom = max(cached_kernels, key=lambda k: cached_kernels[k]["ms"])
     d = cached_kernels[dom]
     gbs = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    # HBM bytes per launch of that kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes (calibrated against a plain copy)
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "cache_march_1080p_traffic.json")
+    if os.path.exists(tpath) and (W, H) == (1920, 1080):
+        with open(tpath) as fh:
+            tj = json.load(fh)
+        if tj.get("kernel") == dom:
+            traffic, traffic_src = round(tj["traffic_bytes_per_launch"], 1), "profiles/cache_march_1080p.md"
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "avg_launch_ms": round(d["ms"] / max(1, d["launches"]), 4),
                 "alg_bytes_per_launch": round(d["bytes"] / max(1, d["launches"]), 1),
                 "units_per_launch": round(d["units"] / max(1, d["launches"]), 1)}
