@@ -102,7 +102,7 @@ __global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list
   FillRay r = rays[(int64_t)t * 64 + lane];
   FillTile tile = tiles[t];
   const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
-  const V3 inv_d = recip(d);
+  const RaySlab rslab = ray_slab(o, d);
   Skip sk = no_skip();
   int slab = tile.slab;
   unsigned mask = 0;
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list
       const float tm = (t0 + t1) * 0.5f;
       if (tm > r.far_) { r.alive = 0; continue; }
       if (in_skip(sk, tm)) continue;
-      if (occ_query(o + tm * d, occ, bf, coarse, o, inv_d, sk)) mask |= 1u << j;
+      if (occ_query(o + tm * d, occ, bf, coarse, rslab, sk)) mask |= 1u << j;
     }
     total = wave_sum(__popc(mask));
     if (total > 0) break;
@@ -434,21 +434,23 @@ struct CacheView {
 // march, reproject every occupied sample into the cache frustum and
 // classify it as resample (outside / beyond the known range), cache hit
 // (set brick, nearest-voxel density > 0) or skip.
-template <int MAXS>
+// The t0 of a ray's k-th resample / cache sample of the launch go to
+// t_scr[k * n_alive + i] / t_scr[(MAX_STEPS_PER_LAUNCH + k) * n_alive + i].
 __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, int trip_cutoff, MarchGrid mg,
                                                      ContractParams con, const uint32_t* __restrict__ coarse_g,
                                                      CacheView cv, RayState* __restrict__ rays, int4* __restrict__ entry,
-                                                     int* __restrict__ alive_flag, float* __restrict__ s_pos,
-                                                     float* __restrict__ s_dir, float* __restrict__ s_t0,
-                                                     float* __restrict__ c_frox, float* __restrict__ c_dir,
-                                                     float* __restrict__ c_t0, uint64_t* __restrict__ order_out,
-                                                     int* __restrict__ cnt_r, int* __restrict__ cnt_c,
-                                                     int* __restrict__ step_counter) {
+                                                     int* __restrict__ alive_flag, float* __restrict__ t_scr,
+                                                     float* __restrict__ s_pos, float* __restrict__ s_dir,
+                                                     float* __restrict__ s_t0, float* __restrict__ c_frox,
+                                                     float* __restrict__ c_dir, float* __restrict__ c_t0,
+                                                     uint64_t* __restrict__ order_out, int* __restrict__ cnt_r,
+                                                     int* __restrict__ cnt_c, int* __restrict__ step_counter) {
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
   load_coarse(coarse_g, mg.occ.nlvl, coarse);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
-  float tr[MAXS], tc[MAXS];
+  float* tr_scr = t_scr;
+  float* tc_scr = t_scr + (int64_t)MAX_STEPS_PER_LAUNCH * n_alive;
   int nr = 0, nc = 0, bins = 0;
   uint64_t order = 0;  // bit k set: the k-th sample of the ray went to the cache stream
   bool alive = false;
@@ -478,9 +480,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
         unsampled = tf < cv.st.near_ || tf > known;
       }
       if (!inside || unsampled) {
-#pragma unroll
-        for (int k = 0; k < MAXS; k++)
-          if (k == nr) tr[k] = t0;
+        tr_scr[(int64_t)nr * n_alive + i] = t0;
         nr++;
       } else {
         const int vx = (int)floorf(f.x), vy = (int)floorf(f.y), vz = (int)floorf(f.z);
@@ -488,9 +488,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
         if (slot < 0) continue;
         const float dens = cv.pool[(int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))].density;
         if (!(dens > 0.0f)) continue;
-#pragma unroll
-        for (int k = 0; k < MAXS; k++)
-          if (k == nc) tc[k] = t0;
+        tc_scr[(int64_t)nc * n_alive + i] = t0;
         order |= 1ull << (nr + nc);
         nc++;
       }
@@ -505,31 +503,25 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
     const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
     const V3 d01 = unit_to_01(d);
     // generateNetworkInput_resample_perf_kernel (198-240): t_mid = t0 + dt/2
-#pragma unroll
-    for (int k = 0; k < MAXS; k++) {
-      if (k < nr) {
-        const float t0 = tr[k];
-        const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;
-        const V3 p = contract(o + tm2 * d, con);
-        const int64_t q = (int64_t)off_r + k;
-        s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
-        s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
-        s_t0[q] = t0;
-      }
+    for (int k = 0; k < nr; k++) {
+      const float t0 = tr_scr[(int64_t)k * n_alive + i];
+      const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;
+      const V3 p = contract(o + tm2 * d, con);
+      const int64_t q = (int64_t)off_r + k;
+      s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
+      s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
+      s_t0[q] = t0;
     }
     // generateNetworkInput_preparation_perf_kernel (242-313)
-#pragma unroll
-    for (int k = 0; k < MAXS; k++) {
-      if (k < nc) {
-        const float t0 = tc[k];
-        const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;
-        float tf;
-        const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
-        const int64_t q = (int64_t)off_c + k;
-        c_frox[3 * q] = f.x; c_frox[3 * q + 1] = f.y; c_frox[3 * q + 2] = f.z;
-        c_dir[3 * q] = d01.x; c_dir[3 * q + 1] = d01.y; c_dir[3 * q + 2] = d01.z;
-        c_t0[q] = t0;
-      }
+    for (int k = 0; k < nc; k++) {
+      const float t0 = tc_scr[(int64_t)k * n_alive + i];
+      const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;
+      float tf;
+      const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
+      const int64_t q = (int64_t)off_c + k;
+      c_frox[3 * q] = f.x; c_frox[3 * q + 1] = f.y; c_frox[3 * q + 2] = f.z;
+      c_dir[3 * q] = d01.x; c_dir[3 * q + 1] = d01.y; c_dir[3 * q + 2] = d01.z;
+      c_t0[q] = t0;
     }
     rays[i].t = r.t;
     rays[i].j = r.j;
@@ -721,16 +713,11 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
     // (cache_performance_renderer.cu:72)
     const int cutoff = 4096;
     ev = P.begin(s);
-#define FVC_CACHE_MARCH(MS)                                                                                        \
-  hipLaunchKernelGGL(k_cache_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, cutoff, mg, m.con,               \
-                     m.coarse.as<uint32_t>(), cv, r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),              \
-                     r.ray_alive.as<int>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),          \
-                     r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(), r.ray_order.as<uint64_t>(),   \
-                     cnt + CNT_SAMPLES, cnt + CNT_CACHE, cnt + CNT_STEPS)
-    if (n_steps <= 8) FVC_CACHE_MARCH(8);
-    else if (n_steps <= 16) FVC_CACHE_MARCH(16);
-    else FVC_CACHE_MARCH(32);
-#undef FVC_CACHE_MARCH
+    hipLaunchKernelGGL(k_cache_march, grid, dim3(256), 0, s, n_alive, n_steps, cutoff, mg, m.con,
+                       m.coarse.as<uint32_t>(), cv, r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),
+                       r.ray_alive.as<int>(), r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(),
+                       r.s_t0.as<float>(), r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(),
+                       r.ray_order.as<uint64_t>(), cnt + CNT_SAMPLES, cnt + CNT_CACHE, cnt + CNT_STEPS);
     FVC_HIP(hipGetLastError());
     const int hm = P.end(ev, s, "cache_march");
     FVC_HIP(hipMemcpyAsync(r.pinned + CNT_SAMPLES, cnt + CNT_SAMPLES, 3 * sizeof(int), hipMemcpyDeviceToHost, s));

@@ -208,6 +208,7 @@ static const double ALG_GATHER_B = 512, ALG_LOOKUP_B = 224;
 static const double ALG_DENSITY_FLOP = 13440, ALG_COLOUR_FLOP = 3456;
 
 // ---------------------------------------------------------------- renderer
+static const int MAX_STEPS_PER_LAUNCH = 32;   // cap of fvc_renderer_set_batching
 struct Renderer {
   Model* model = nullptr;
   hipStream_t own_stream = nullptr;
@@ -240,6 +241,7 @@ struct Renderer {
   DevBuf results;        // float4 [H*W] final
   DevBuf ray_entry;      // int4 per ray: offset/count per stream
   DevBuf ray_order;      // uint64 per ray: stream of each sample, in march order
+  DevBuf t_scratch;      // float [2 * MAX_STEPS_PER_LAUNCH][n]: sample t0 per launch
   DevBuf ray_alive;      // int32 per ray
   DevBuf counters;       // int32 device counters
   int* pinned = nullptr; // pinned host mirror of the counters

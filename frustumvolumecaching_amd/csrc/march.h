@@ -79,41 +79,71 @@ FVC_HD V3 occ_unit(V3 p, const OccParams& o) {
   return (p - o.unit_off) / o.unit_ext;
 }
 
+// Per-ray terms of box_skip: 1/d with +-1e30 for (near) axis-parallel
+// directions (the slab test then needs no special case), o/d, |1/d|, the
+// sign of d and the largest |o| component.
+struct RaySlab {
+  V3 inv, oinv, ainv;
+  float omax;
+  int neg;   // bit a set: d_a < 0
+};
+FVC_HD RaySlab ray_slab(V3 o, V3 d) {
+  RaySlab r;
+  const float dd[3] = {d.x, d.y, d.z}, oo[3] = {o.x, o.y, o.z};
+  float iv[3];
+  r.neg = 0;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    iv[a] = fabsf(dd[a]) < 1e-30f ? (dd[a] < 0.0f ? -1e30f : 1e30f) : 1.0f / dd[a];
+    r.neg |= (iv[a] < 0.0f ? 1 : 0) << a;
+  }
+  r.inv = V3{iv[0], iv[1], iv[2]};
+  r.oinv = V3{oo[0] * iv[0], oo[1] * iv[1], oo[2] * iv[2]};
+  r.ainv = V3{fabsf(iv[0]), fabsf(iv[1]), fabsf(iv[2])};
+  r.omax = fmaxf_(fmaxf_(fabsf(oo[0]), fabsf(oo[1])), fabsf(oo[2]));
+  return r;
+}
+
 // Parameter interval of the ray inside the world box of cells [c, c+g) of
 // cascade mip, each face pulled in by a margin.  The lower face of cell i
-// lies at box_c + (i - 64) * box_e * 2^(mip-7).
+// lies at box_c + (i - 64) * box_e * 2^(mip-7).  The interval is only ever
+// used to skip bins, so it is computed with fused multiply-adds; its margin
+// absorbs that rounding.
 //
 // Margin: the lookup of a point p = o + tm*d rounds at most about ten times
 // (the product and sum of p, the unit transform, the cascade rescale), each
 // by 2^-24 relative to a magnitude bounded by |o| + |t| + the cascade's
 // largest coordinate, so its cell boundaries are uncertain by well under
-// 1e-6 of that magnitude.  The margin is 4e-5 of it (over 60x the error);
-// a box thinner than four margins gives no skip.
-FVC_HD Skip box_skip(int cx, int cy, int cz, int g, int mip, const OccParams& o, V3 ro, V3 inv_d) {
+// 1e-6 of that magnitude.  The margin is 4e-5 of it (over 60x the error).
+// A face is pulled in by margin/|d_a| in t; a box thinner than two margins
+// along an axis leaves an empty interval (no skip).
+FVC_HD Skip box_skip(int cx, int cy, int cz, int g, int mip, const OccParams& o, const RaySlab& rs) {
   const float s = ldexpf(1.0f, mip - 7);
-  const float span_cells = (float)g;
-  float tin = -3.0e38f, tout = 3.0e38f;
+  const float gs = (float)g;
   const int ci[3] = {cx, cy, cz};
   const float bc[3] = {o.box_c.x, o.box_c.y, o.box_c.z}, be[3] = {o.box_e.x, o.box_e.y, o.box_e.z};
-  const float oo[3] = {ro.x, ro.y, ro.z}, id[3] = {inv_d.x, inv_d.y, inv_d.z};
+  const float iv[3] = {rs.inv.x, rs.inv.y, rs.inv.z}, oi[3] = {rs.oinv.x, rs.oinv.y, rs.oinv.z};
+  const float ai[3] = {rs.ainv.x, rs.ainv.y, rs.ainv.z};
+  float tn[3], tf[3];
+  float cmag = 0.0f;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     const float w = be[a] * s;
-    const float lo = bc[a] + (float)(ci[a] - 64) * w;
-    const float hi = lo + span_cells * w;
-    const float cmag = fabsf(bc[a]) + 64.0f * w;  // largest coordinate of the cascade
-    if (id[a] == 0.0f || fabsf(id[a]) > 1e30f) {  // axis-parallel ray: coordinate is constant
-      const float m = 4e-5f * (fabsf(oo[a]) + cmag);
-      if (!(oo[a] >= lo + m && oo[a] <= hi - m)) return no_skip();
-      continue;
-    }
-    const float far_t = fmaxf_(fabsf((lo - oo[a]) * id[a]), fabsf((hi - oo[a]) * id[a]));
-    const float m = 4e-5f * (fabsf(oo[a]) + far_t + cmag);
-    if (4.0f * m > hi - lo) return no_skip();
-    float ta = ((lo + m) - oo[a]) * id[a], tb = ((hi - m) - oo[a]) * id[a];
-    if (ta > tb) { const float t = ta; ta = tb; tb = t; }
-    tin = ta > tin ? ta : tin;
-    tout = tb < tout ? tb : tout;
+    const float lo = fmaf((float)(ci[a] - 64), w, bc[a]);
+    const float hi = fmaf(gs, w, lo);
+    const float ta = fmaf(lo, iv[a], -oi[a]), tb = fmaf(hi, iv[a], -oi[a]);
+    const bool neg = (rs.neg >> a) & 1;
+    tn[a] = neg ? tb : ta;
+    tf[a] = neg ? ta : tb;
+    cmag = fmaxf_(cmag, fmaf(64.0f, w, fabsf(bc[a])));
+  }
+  const float tout0 = fminf_(fminf_(tf[0], tf[1]), tf[2]);
+  const float m = 4e-5f * (rs.omax + fabsf(tout0) + cmag);
+  float tin = -3.0e38f, tout = 3.0e38f;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    tin = fmaxf_(tin, fmaf(m, ai[a], tn[a]));
+    tout = fminf_(tout, fmaf(-m, ai[a], tf[a]));
   }
   return Skip{tin, tout};
 }
@@ -121,8 +151,8 @@ FVC_HD Skip box_skip(int cx, int cy, int cz, int g, int mip, const OccParams& o,
 // grid_occupied_at<128, true> (raymarch_common.h:680-710) for p = o + tm*d.
 // When the answer is "empty" the skip interval of the largest empty box
 // around the cell is stored in sk (left unchanged outside every cascade).
-FVC_HD bool occ_query(V3 p, const OccParams& o, const uint64_t* __restrict__ bf64, const uint32_t* coarse, V3 ro,
-                      V3 inv_d, Skip& sk) {
+FVC_HD bool occ_query(V3 p, const OccParams& o, const uint64_t* __restrict__ bf64, const uint32_t* coarse,
+                      const RaySlab& rs, Skip& sk) {
   const V3 u = occ_unit(p, o);
   const float ax = fabsf(u.x - 0.5f), ay = fabsf(u.y - 0.5f), az = fabsf(u.z - 0.5f);
   const float mx = fmaxf_(fmaxf_(ax, ay), az);
@@ -152,7 +182,7 @@ FVC_HD bool occ_query(V3 p, const OccParams& o, const uint64_t* __restrict__ bf6
     const bool e32 = !((cw[COARSE8_WORDS + COARSE16_WORDS + (b32 >> 5)] >> (b32 & 31)) & 1u);
     g = e32 ? 32 : (e16 ? 16 : 8);
   }
-  sk = box_skip(ix & -g, iy & -g, iz & -g, g, mip, o, ro, inv_d);
+  sk = box_skip(ix & -g, iy & -g, iz & -g, g, mip, o, rs);
   return false;
 }
 
@@ -190,18 +220,22 @@ FVC_HD float step_estimate(const StepTableView& v, float t) {
 // bin k >= j whose midpoint exceeds hi.  Every bin in between has its
 // midpoint in [that midpoint, hi], inside the same empty box.  t1 and j keep
 // the invariant t1 == T[m][j].  Returns the number of bins skipped.
+//
+// The continuous schedule puts the answer within a bin of its estimate (the
+// float recurrence drifts by a small fraction of a bin over a row), so one
+// round of four independent loads around the estimate decides it; a
+// bisection or a forward walk covers the rare misses.
 FVC_HD int table_jump(const StepTableView& tab, int m, int& j, float& t1, float hi) {
   if (m < 0 || m >= tab.n_rows) return 0;
   const int base = tab.off[m];
   const int last = tab.off[m + 1] - base - 2;  // largest k whose bin end is in the row
   if (j >= last) return 0;
   const float* row = tab.t + base;
-  // guess two bins early from the continuous schedule, then verify
-  int k = (int)(step_estimate(tab, hi) - tab.near_i) - m - 2;
-  k = k > last ? last : k;
-  if (k <= j) {
-    k = j;
-  } else if (!((row[k - 1] + row[k]) * 0.5f <= hi)) {
+  // first k with midpoint(k) > hi: k + m + near_i + 1/2 > step(hi)
+  int k = (int)floorf(step_estimate(tab, hi) - tab.near_i - (float)m - 0.5f) + 1;
+  k = k < j ? j : (k > last ? last : k);
+  const float r0 = row[k > 0 ? k - 1 : 0], r1 = row[k], r2 = row[k + 1], r3 = row[k + 1 < last ? k + 2 : last + 1];
+  if (k > j && !((r0 + r1) * 0.5f <= hi)) {
     // overshoot: the answer lies in [j, k-1]; bisect
     int a = j, b = k - 1;
     while (a < b) {
@@ -210,8 +244,13 @@ FVC_HD int table_jump(const StepTableView& tab, int m, int& j, float& t1, float 
       else b = c;
     }
     k = a;
+  } else if ((r1 + r2) * 0.5f <= hi) {
+    if (k < last) {
+      k++;
+      if ((r2 + r3) * 0.5f <= hi)
+        while (k < last && (row[k] + row[k + 1]) * 0.5f <= hi) k++;
+    }
   }
-  while (k < last && (row[k] + row[k + 1]) * 0.5f <= hi) k++;
   const int skipped = k - j;
   j = k;
   t1 = row[k];
@@ -269,10 +308,6 @@ inline int build_step_rows(const fvc_stepsize_info& st, float lim, std::vector<f
   return rows;
 }
 
-FVC_HD V3 recip(V3 d) {
-  return V3{d.x != 0.0f ? 1.0f / d.x : 0.0f, d.y != 0.0f ? 1.0f / d.y : 0.0f, d.z != 0.0f ? 1.0f / d.z : 0.0f};
-}
-
 // Read-only inputs of the marcher.
 struct MarchGrid {
   OccParams occ;
@@ -283,7 +318,8 @@ struct MarchGrid {
 
 // A ray between marcher trips: the next bin starts at t1 == T[m][j].
 struct MarchRay {
-  V3 o, d, inv_d;
+  V3 o, d;
+  RaySlab slab;
   float far_;
   int m, j;
   float t1;
@@ -306,7 +342,7 @@ FVC_HD int march_trip(const MarchGrid& g, const uint32_t* coarse, MarchRay& r, f
   const float tm = (t0 + r.t1) * 0.5f;
   if (tm > r.far_) return TRIP_END;
   if (!in_skip(r.sk, tm)) {
-    if (occ_query(r.o + tm * r.d, g.occ, g.bf64, coarse, r.o, r.inv_d, r.sk)) return TRIP_SAMPLE;
+    if (occ_query(r.o + tm * r.d, g.occ, g.bf64, coarse, r.slab, r.sk)) return TRIP_SAMPLE;
     if (!in_skip(r.sk, tm)) return TRIP_EMPTY;
   }
   bins += table_jump(g.tab, r.m, r.j, r.t1, r.sk.hi);

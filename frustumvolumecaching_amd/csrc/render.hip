@@ -41,18 +41,18 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, V3 aabb
 // sample_performance_kernel (performance_renderer.cu:15-69): march until
 // n_steps occupied bin midpoints or t_mid > far; emit contracted position,
 // unit_to_01(dir) and t0 of every occupied sample into the compacted stream.
-template <int MAXS>
+// The t0 of a ray's k-th sample of the launch goes to t_scr[k * n_alive + i]
+// (coalesced across the wave when read back for the stream).
 __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, MarchGrid mg, ContractParams con,
                                                const uint32_t* __restrict__ coarse_g, RayState* __restrict__ rays,
                                                int4* __restrict__ entry, int* __restrict__ alive_flag,
-                                               float* __restrict__ s_pos, float* __restrict__ s_dir,
-                                               float* __restrict__ s_t0, int* __restrict__ s_counter,
-                                               int* __restrict__ step_counter) {
+                                               float* __restrict__ t_scr, float* __restrict__ s_pos,
+                                               float* __restrict__ s_dir, float* __restrict__ s_t0,
+                                               int* __restrict__ s_counter, int* __restrict__ step_counter) {
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
   load_coarse(coarse_g, mg.occ.nlvl, coarse);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
-  float ts[MAXS];
   int cnt = 0, bins = 0;
   bool alive = false;
   RayState rs{};
@@ -65,9 +65,7 @@ __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, MarchGr
       const int res = march_trip(mg, coarse, r, t0, bins);
       if (res == TRIP_END) { alive = false; break; }
       if (res == TRIP_SAMPLE) {
-#pragma unroll
-        for (int k = 0; k < MAXS; k++)
-          if (k == cnt) ts[k] = t0;
+        t_scr[(int64_t)cnt * n_alive + i] = t0;
         cnt++;
       }
     }
@@ -80,18 +78,15 @@ __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, MarchGr
   if (valid) {
     const V3 o = {rs.ox, rs.oy, rs.oz}, d = {rs.dx, rs.dy, rs.dz};
     const V3 d01 = unit_to_01(d);
-#pragma unroll
-    for (int k = 0; k < MAXS; k++) {
-      if (k < cnt) {
-        const float t0 = ts[k];
-        const float t1 = t0 + step_dt(mg.st, t0);
-        const float tm = (t0 + t1) * 0.5f;
-        const V3 p = contract(o + tm * d, con);
-        const int64_t q = (int64_t)off + k;
-        s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
-        s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
-        s_t0[q] = t0;
-      }
+    for (int k = 0; k < cnt; k++) {
+      const float t0 = t_scr[(int64_t)k * n_alive + i];
+      const float t1 = t0 + step_dt(mg.st, t0);
+      const float tm = (t0 + t1) * 0.5f;
+      const V3 p = contract(o + tm * d, con);
+      const int64_t q = (int64_t)off + k;
+      s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
+      s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
+      s_t0[q] = t0;
     }
     rays[i].t = rs.t;
     rays[i].j = rs.j;
@@ -204,6 +199,7 @@ void resize_renderbuffers(Renderer& r, int w, int h) {
   r.results.ensure(n * sizeof(float4));
   r.ray_entry.ensure(n * sizeof(int4));
   r.ray_order.ensure(n * sizeof(uint64_t));
+  r.t_scratch.ensure(n * 2 * MAX_STEPS_PER_LAUNCH * sizeof(float));
   r.ray_alive.ensure(n * sizeof(int));
   r.counters.ensure(64 * sizeof(int));
   r.s_pos.ensure(cap * 3 * sizeof(float));
@@ -321,15 +317,10 @@ void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_re
     FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 3 * sizeof(int), s));
     const dim3 grid((n_alive + 255) / 256);
     ev = P.begin(s);
-#define FVC_MARCH(MS)                                                                                        \
-  hipLaunchKernelGGL(k_march<MS>, grid, dim3(256), 0, s, n_alive, n_steps, mg, m.con, m.coarse.as<uint32_t>(), \
-                     r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),                 \
-                     r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(), cnt + CNT_SAMPLES,          \
-                     cnt + CNT_STEPS)
-    if (n_steps <= 8) FVC_MARCH(8);
-    else if (n_steps <= 16) FVC_MARCH(16);
-    else FVC_MARCH(32);
-#undef FVC_MARCH
+    hipLaunchKernelGGL(k_march, grid, dim3(256), 0, s, n_alive, n_steps, mg, m.con, m.coarse.as<uint32_t>(),
+                       r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
+                       r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
+                       cnt + CNT_SAMPLES, cnt + CNT_STEPS);
     FVC_HIP(hipGetLastError());
     const int hm = P.end(ev, s, "march");
     FVC_HIP(hipMemcpyAsync(r.pinned + CNT_SAMPLES, cnt + CNT_SAMPLES, 3 * sizeof(int), hipMemcpyDeviceToHost, s));

@@ -41,7 +41,7 @@ __device__ __forceinline__ MarchRay march_ray(const RayState& s) {
   MarchRay r;
   r.o = V3{s.ox, s.oy, s.oz};
   r.d = V3{s.dx, s.dy, s.dz};
-  r.inv_d = recip(r.d);
+  r.slab = ray_slab(r.o, r.d);
   r.far_ = s.far_;
   r.m = s.m;
   r.j = s.j;
