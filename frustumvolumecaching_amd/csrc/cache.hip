@@ -589,27 +589,21 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
         for (int i = 0; i < 8; i++) ld[i] = ld[i] + cl[i];
       }
     }
-    wd = wd + __shfl_xor(wd, 16, 64);
-    wd = wd + __shfl_xor(wd, 32, 64);
-    dd = dd + __shfl_xor(dd, 16, 64);
-    dd = dd + __shfl_xor(dd, 32, 64);
-    _Float16 lat[8];
+    wd = xor32_sum(xor16_sum(wd));
+    dd = xor32_sum(xor16_sum(dd));
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      float v = ld[i] + __shfl_xor(ld[i], 16, 64);
-      v = v + __shfl_xor(v, 32, 64);
-      lat[i] = (_Float16)(v / wd);
-    }
+    for (int i = 0; i < 8; i++) ld[i] = xor32_sum(xor16_sum(ld[i]));
     const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
+    // rows 4g..4g+3 of the base output: rows 1..8 are the latents
+    // sum w*l / sum w*isset, the others the 1.0 padding
     _Float16 rows[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      const int rr = 4 * g + e;
-      _Float16 v = (_Float16)1.0f;
+      const int li = 4 * g + e - 1;
+      float num = ld[0];
 #pragma unroll
-      for (int i = 0; i < 8; i++)
-        if (rr == i + 1) v = lat[i];
-      rows[e] = v;
+      for (int i = 1; i < 8; i++) num = li == i ? ld[i] : num;
+      rows[e] = (li >= 0 && li < 8) ? (_Float16)(num / wd) : (_Float16)1.0f;
     }
     const h8 x = head_input(rows, g, c_dir[3 * sl], c_dir[3 * sl + 1], c_dir[3 * sl + 2]);
     const f4 o = head_mlp(hw, x);

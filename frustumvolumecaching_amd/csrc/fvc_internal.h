@@ -59,12 +59,15 @@ static const int MAX_LEVELS = 16;
 struct GridLevel {
   float scale;
   uint32_t res;
+  uint32_t res2;     // res * res
   uint32_t size;     // entries in the level
   uint32_t offset;   // first entry of the level
 };
 struct GridParams {
   GridLevel lv[MAX_LEVELS];
   uint32_t dense_mask;  // bit l: level l indexed densely
+  uint32_t quad_dense;  // bit j: levels 4j..4j+3 all dense
+  uint32_t quad_hash;   // bit j: levels 4j..4j+3 all hashed
   int n_levels;
 };
 

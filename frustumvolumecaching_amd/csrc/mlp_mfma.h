@@ -75,6 +75,39 @@ __device__ __forceinline__ f4 head_mlp(const HeadWeights& w, h8 x) {
   return mfma(w.h2[1], b_1, mfma(w.h2[0], b_0, f4zero()));
 }
 
+// The same MLPs with the weight fragments read from LDS at each use (w =
+// the N_FRAGS x 64 fragments staged by the block); li is the lane index made
+// opaque per loop iteration by the caller so the reads are not hoisted into
+// registers.  Frees 80 VGPRs for gathers in flight.
+__device__ __forceinline__ f4 base_mlp_lds(const h8* w, int li, h8 x) {
+  f4 d1[4];
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) d1[nb] = mfma(w[(FRAG_W1 + nb) * 64 + li], x, f4zero());
+  const h8 b2_0 = relu_pack(d1[0], d1[1]), b2_1 = relu_pack(d1[2], d1[3]);
+  f4 d2[4];
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++)
+    d2[nb] = mfma(w[(FRAG_W2 + nb * 2 + 1) * 64 + li], b2_1, mfma(w[(FRAG_W2 + nb * 2) * 64 + li], b2_0, f4zero()));
+  const h8 b3_0 = relu_pack(d2[0], d2[1]), b3_1 = relu_pack(d2[2], d2[3]);
+  return mfma(w[(FRAG_W3 + 1) * 64 + li], b3_1, mfma(w[FRAG_W3 * 64 + li], b3_0, f4zero()));
+}
+__device__ __forceinline__ f4 head_mlp_lds(const h8* w, int li, h8 x) {
+  f4 d1[4];
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) d1[nb] = mfma(w[(FRAG_H1 + nb) * 64 + li], x, f4zero());
+  const h8 b_0 = relu_pack(d1[0], d1[1]), b_1 = relu_pack(d1[2], d1[3]);
+  return mfma(w[(FRAG_H2 + 1) * 64 + li], b_1, mfma(w[FRAG_H2 * 64 + li], b_0, f4zero()));
+}
+// stage all weight fragments of the model into LDS (every thread of the block)
+__device__ __forceinline__ void stage_frags(const h8* __restrict__ frags, h8* lds) {
+  for (int i = threadIdx.x; i < N_FRAGS * 64; i += blockDim.x) lds[i] = frags[i];
+  __syncthreads();
+}
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // SH degree 4 coefficients 4g..4g+3 of dir01*2-1 (my_fully_fused_mlp.cu:206-234),
 // rounded to fp16.
 __device__ __forceinline__ void sh4_quarter(float dx01, float dy01, float dz01, int g, _Float16* o) {

@@ -10,9 +10,10 @@
 // the 8 corners, accumulated in FP32 and rounded to fp16.
 //
 // Work split: one wave64 evaluates 16 samples per step.  Lane l computes
-// levels 4g..4g+3 (g = l/16) of sample l%16 -- exactly the 8 k-values of the
-// first layer's B fragment -- so the gathered features feed the MFMA chain in
-// mlp_mfma.h directly from registers.
+// levels g, g+4, g+8, g+12 (g = l/16) of sample l%16 -- the 8 k-values of the
+// first layer's B fragment, whose weight columns are permuted to match on the
+// host -- so the gathered features feed the MFMA chain in mlp_mfma.h
+// directly from registers.
 #include <cmath>
 #include <cstring>
 
@@ -41,10 +42,16 @@ bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n
     if (p > T) p = T;
     g.lv[l].scale = sf;
     g.lv[l].res = r;
+    g.lv[l].res2 = r * r;
     g.lv[l].size = (uint32_t)p;
     g.lv[l].offset = off;
     if (dense <= p) g.dense_mask |= 1u << l;
     off += (uint32_t)p;
+  }
+  for (int j = 0; j < 4; j++) {
+    const uint32_t q = (g.dense_mask >> (4 * j)) & 15u;
+    if (q == 15u) g.quad_dense |= 1u << j;
+    if (q == 0u) g.quad_hash |= 1u << j;
   }
   n_pos = (int64_t)off * 2;
   n_base = 32 * 64 + 64 * 64 + 64 * 16;
@@ -53,9 +60,15 @@ bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n
 }
 
 // ------------------------------------------------------------------ encode
-// Features of levels l0..l0+3 at p (clamped to [0,1]^3) -> f[8].
-__device__ __forceinline__ void encode4(const GridLevel* __restrict__ lv, uint32_t dense_mask,
-                                        const uint32_t* __restrict__ table, V3 p, int l0, float* f) {
+// Features of levels g, g+4, g+8, g+12 at p (clamped to [0,1]^3) -> f[8]
+// (f[2j], f[2j+1] = level 4j+g).  With this assignment the four lane groups
+// of a wave work on levels 4j..4j+3 together, which are either all dense,
+// all hashed, or (one quad at most) mixed: the first two take wave-uniform
+// branches.  Per level the y / z index terms of both corners are formed once
+// (24-bit products for dense levels, which stay below 2^24); each corner then
+// costs one 3-input add or xor and the reduction to the level's entries.
+__device__ __forceinline__ void encode_quads(const GridLevel* __restrict__ lv, const GridParams& gp,
+                                            const uint32_t* __restrict__ table, V3 p, int g, float* f) {
   const float px = fminf_(fmaxf_(p.x, 0.0f), 1.0f);
   const float py = fminf_(fmaxf_(p.y, 0.0f), 1.0f);
   const float pz = fminf_(fmaxf_(p.z, 0.0f), 1.0f);
@@ -63,35 +76,51 @@ __device__ __forceinline__ void encode4(const GridLevel* __restrict__ lv, uint32
   float wx[4], wy[4], wz[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int l = l0 + j;
+    const int l = 4 * j + g;
     const GridLevel L = lv[l];
     const float ax = px * L.scale + 0.5f, ay = py * L.scale + 0.5f, az = pz * L.scale + 0.5f;
     const float fx = floorf(ax), fy = floorf(ay), fz = floorf(az);
     const uint32_t gx = (uint32_t)(int)fx, gy = (uint32_t)(int)fy, gz = (uint32_t)(int)fz;
     wx[j] = ax - fx; wy[j] = ay - fy; wz[j] = az - fz;
-    const bool dense = (dense_mask >> l) & 1u;
+    const uint32_t* tl = table + L.offset;
+    uint32_t idx[8];
+    if ((gp.quad_dense >> j) & 1u) {
+      const uint32_t y0 = __umul24(gy, L.res), z0 = __umul24(gz, L.res2);
+      const uint32_t yy[2] = {y0, y0 + L.res}, zz[2] = {z0, z0 + L.res2};
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const uint32_t qx = gx + (c & 1), qy = gy + ((c >> 1) & 1), qz = gz + ((c >> 2) & 1);
-      uint32_t i;
-      if (dense) {
-        i = qx + qy * L.res + qz * L.res * L.res;
-        i = i >= L.size ? i - L.size : i;  // i < 2*size for res >= 2
-      } else {
-        i = (qx ^ (qy * 2654435761u) ^ (qz * 805459861u)) & (L.size - 1u);
+      for (int c = 0; c < 8; c++) {
+        const uint32_t t = gx + (c & 1) + yy[(c >> 1) & 1] + zz[c >> 2];
+        idx[c] = min(t, t - L.size);  // t < 2 * size
       }
-      ent[j][c] = table[L.offset + i];
+    } else if ((gp.quad_hash >> j) & 1u) {
+      const uint32_t y0 = gy * 2654435761u, z0 = gz * 805459861u;
+      const uint32_t yy[2] = {y0, y0 + 2654435761u}, zz[2] = {z0, z0 + 805459861u};
+#pragma unroll
+      for (int c = 0; c < 8; c++) idx[c] = ((gx + (c & 1)) ^ yy[(c >> 1) & 1] ^ zz[c >> 2]) & (L.size - 1u);
+    } else {
+      const bool dense = (gp.dense_mask >> l) & 1u;
+      const uint32_t y0 = dense ? __umul24(gy, L.res) : gy * 2654435761u;
+      const uint32_t z0 = dense ? __umul24(gz, L.res2) : gz * 805459861u;
+      const uint32_t yy[2] = {y0, y0 + (dense ? L.res : 2654435761u)};
+      const uint32_t zz[2] = {z0, z0 + (dense ? L.res2 : 805459861u)};
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const uint32_t x = gx + (c & 1), y = yy[(c >> 1) & 1], z = zz[c >> 2];
+        const uint32_t ta = x + y + z, th = (x ^ y ^ z) & (L.size - 1u);
+        idx[c] = dense ? min(ta, ta - L.size) : th;
+      }
     }
+#pragma unroll
+    for (int c = 0; c < 8; c++) ent[j][c] = tl[idx[c]];
   }
 #pragma unroll
   for (int j = 0; j < 4; j++) {
+    const float ux = 1.0f - wx[j], uy = 1.0f - wy[j], uz = 1.0f - wz[j];
+    const float wxy[4] = {ux * uy, wx[j] * uy, ux * wy[j], wx[j] * wy[j]};
     float a0 = 0.0f, a1 = 0.0f;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
-      float w = 1.0f;
-      w *= (c & 1) ? wx[j] : 1.0f - wx[j];
-      w *= (c & 2) ? wy[j] : 1.0f - wy[j];
-      w *= (c & 4) ? wz[j] : 1.0f - wz[j];
+      const float w = wxy[c & 3] * ((c & 4) ? wz[j] : uz);
       const uint32_t e = ent[j][c];
       const float v0 = (float)__builtin_bit_cast(_Float16, (uint16_t)(e & 0xffffu));
       const float v1 = (float)__builtin_bit_cast(_Float16, (uint16_t)(e >> 16));
@@ -119,26 +148,25 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
                                                        const h8* __restrict__ frags, GridParams gp,
                                                        uint2* __restrict__ out_h4, uint2* __restrict__ out16) {
   __shared__ GridLevel lv[MAX_LEVELS];
+  __shared__ h8 wl[N_FRAGS * 64];
   load_levels(gp, lv);
+  stage_frags(frags, wl);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
-  BaseWeights bw;
-  load_base(frags, lane, bw);
-  HeadWeights hw;
-  if (HEAD) load_head(frags, lane, hw);
   const int64_t n_groups = (n + 15) / 16;
   for (int64_t grp = (int64_t)blockIdx.x * (NET_BLOCK / 64) + wave; grp < n_groups;
        grp += (int64_t)gridDim.x * (NET_BLOCK / 64)) {
+    const int li = opaque(lane);
     const int64_t s = grp * 16 + col;
     const bool valid = s < n;
     const int64_t sl = valid ? s : n - 1;
     const V3 p = {pos[3 * sl], pos[3 * sl + 1], pos[3 * sl + 2]};
     float f[8];
-    encode4(lv, gp.dense_mask, table, p, 4 * g, f);
+    encode_quads(lv, gp, table, p, g, f);
     h8 x;
 #pragma unroll
     for (int e = 0; e < 8; e++) x[e] = (_Float16)f[e];
-    const f4 d3 = base_mlp(bw, x);
+    const f4 d3 = base_mlp_lds(wl, li, x);
     _Float16 rows[4] = {(_Float16)d3[0], (_Float16)d3[1], (_Float16)d3[2], (_Float16)d3[3]};
     if (!HEAD) {
       if (valid) {
@@ -149,7 +177,7 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
       }
     } else {
       const h8 hin = head_input(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2]);
-      const f4 o = head_mlp(hw, hin);
+      const f4 o = head_mlp_lds(wl, li, hin);
       if (g == 0 && valid) {
         uint2 v;
         v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
@@ -201,8 +229,10 @@ __global__ void k_encode(const float* __restrict__ pos, int64_t n, const uint32_
   const V3 p = {pos[3 * s], pos[3 * s + 1], pos[3 * s + 2]};
   for (int q = 0; q < 4; q++) {
     float f[8];
-    encode4(lv, gp.dense_mask, table, p, 4 * q, f);
-    for (int e = 0; e < 8; e++) out32[s * 32 + 8 * q + e] = __builtin_bit_cast(uint16_t, (_Float16)f[e]);
+    encode_quads(lv, gp, table, p, q, f);
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 2; k++)
+        out32[s * 32 + 2 * (4 * j + q) + k] = __builtin_bit_cast(uint16_t, (_Float16)f[2 * j + k]);
   }
 }
 
@@ -246,7 +276,8 @@ void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t
 // ------------------------------------------------------------------ packing
 // Column of the reference weight matrix that lane-group g, slot e of a
 // fragment multiplies (see mlp_mfma.h).
-static int col_first(int g, int e) { return 8 * g + e; }                                       // W1
+// W1: slot pair j of group g holds level 4j+g (encode_quads)
+static int col_first(int g, int e) { return 2 * (4 * (e >> 1) + g) + (e & 1); }
 static int col_chain(int kb, int g, int e) { return e < 4 ? 32 * kb + 4 * g + e : 32 * kb + 16 + 4 * g + e - 4; }
 static int col_head_in(int g, int e) {                                                         // H1
   if (e >= 4) return 8 + 4 * g + (e - 4);  // SH coefficient 4g + e - 4

@@ -65,6 +65,22 @@ __device__ __forceinline__ void block_reserve(const int (&v)[N], int* const (&ct
   for (int c = 0; c < N; c++) slot[c] = s_base[c] + s_off[c][wave] + pre[c];
 }
 
+// v + v(lane ^ 16) and v + v(lane ^ 32) with the gfx950 row swaps: for a
+// value passed as both operands, v_permlane16_swap returns rows {0,0,2,2}
+// and {1,1,3,3}, v_permlane32_swap halves {0,0} and {1,1}, so the sum of the
+// two results is the xor-partner sum on every lane (same two operands, so
+// the same float result as v + partner).
+__device__ __forceinline__ float xor16_sum(float v) {
+  const unsigned u = __builtin_bit_cast(unsigned, v);
+  const auto p = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+  return __builtin_bit_cast(float, (unsigned)p[0]) + __builtin_bit_cast(float, (unsigned)p[1]);
+}
+__device__ __forceinline__ float xor32_sum(float v) {
+  const unsigned u = __builtin_bit_cast(unsigned, v);
+  const auto p = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+  return __builtin_bit_cast(float, (unsigned)p[0]) + __builtin_bit_cast(float, (unsigned)p[1]);
+}
+
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
