@@ -434,8 +434,12 @@ struct CacheView {
 // march, reproject every occupied sample into the cache frustum and
 // classify it as resample (outside / beyond the known range), cache hit
 // (set brick, nearest-voxel density > 0) or skip.
-// The t0 of a ray's k-th resample / cache sample of the launch go to
-// t_scr[k * n_alive + i] / t_scr[(MAX_STEPS_PER_LAUNCH + k) * n_alive + i].
+// The march collects up to n_steps occupied samples per ray (their t0 in
+// t_scr[k * n_alive + i]); a second pass classifies them, so the reprojection
+// and its three dependent loads stay out of the divergent march loop.  The
+// reference stops a launch when either stream reaches n_steps samples; here
+// the launch holds n_steps samples in total.  That only moves sub-iteration
+// boundaries: every ray still composites the same samples in the same order.
 __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, int trip_cutoff, MarchGrid mg,
                                                      ContractParams con, const uint32_t* __restrict__ coarse_g,
                                                      CacheView cv, RayState* __restrict__ rays, int4* __restrict__ entry,
@@ -449,10 +453,8 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
   load_coarse(coarse_g, mg.occ.nlvl, coarse);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
-  float* tr_scr = t_scr;
-  float* tc_scr = t_scr + (int64_t)MAX_STEPS_PER_LAUNCH * n_alive;
-  int nr = 0, nc = 0, bins = 0;
-  uint64_t order = 0;  // bit k set: the k-th sample of the ray went to the cache stream
+  int cnt = 0, bins = 0;
+  uint32_t res_mask = 0, cache_mask = 0;  // class of march sample k
   bool alive = false;
   RayState r{};
   V3 oc = {0, 0, 0}, dc = {0, 0, 0};
@@ -460,16 +462,26 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
     r = rays[i];
     alive = true;
     MarchRay mr = march_ray(r);
-    oc = transform(cv.cam.world2cam, mr.o.x, mr.o.y, mr.o.z, 1.0f);
-    dc = transform(cv.cam.world2cam, mr.d.x, mr.d.y, mr.d.z, 0.0f);
     int trips = 0;
-    while (nr < n_steps && nc < n_steps && trips < trip_cutoff) {
+    while (cnt < n_steps && trips < trip_cutoff) {
       trips++;
       float t0;
       const int res = march_trip(mg, coarse, mr, t0, bins);
       if (res == TRIP_END) { alive = false; break; }
-      if (res != TRIP_SAMPLE) continue;
-      const float tm = (t0 + mr.t1) * 0.5f;
+      if (res == TRIP_SAMPLE) {
+        t_scr[(int64_t)cnt * n_alive + i] = t0;
+        cnt++;
+      }
+    }
+    r.t = mr.t1;
+    r.j = mr.j;
+    // classification (sample_cache_performance_kernel, cache_performance_renderer.cu:96-160)
+    oc = transform(cv.cam.world2cam, mr.o.x, mr.o.y, mr.o.z, 1.0f);
+    dc = transform(cv.cam.world2cam, mr.d.x, mr.d.y, mr.d.z, 0.0f);
+    for (int k = 0; k < cnt; k++) {
+      const float t0 = t_scr[(int64_t)k * n_alive + i];
+      const float t1 = t0 + step_dt(mg.st, t0);
+      const float tm = (t0 + t1) * 0.5f;
       float tf;
       const V3 f = cam2froxel(oc + tm * dc, cv.cam, cv.st, tf);
       const bool in_fov = f.x >= 0.0f && f.x < (float)cv.fx && f.y >= 0.0f && f.y < (float)cv.fy;
@@ -480,48 +492,49 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
         unsampled = tf < cv.st.near_ || tf > known;
       }
       if (!inside || unsampled) {
-        tr_scr[(int64_t)nr * n_alive + i] = t0;
-        nr++;
+        res_mask |= 1u << k;
       } else {
         const int vx = (int)floorf(f.x), vy = (int)floorf(f.y), vz = (int)floorf(f.z);
         const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
-        if (slot < 0) continue;
-        const float dens = cv.pool[(int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))].density;
-        if (!(dens > 0.0f)) continue;
-        tc_scr[(int64_t)nc * n_alive + i] = t0;
-        order |= 1ull << (nr + nc);
-        nc++;
+        if (slot >= 0 &&
+            cv.pool[(int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))].density > 0.0f)
+          cache_mask |= 1u << k;
       }
     }
-    r.t = mr.t1;
-    r.j = mr.j;
   }
+  const int nr = __popc(res_mask), nc = __popc(cache_mask);
   int slot[3];
   block_reserve<3>({nr, nc, bins}, {cnt_r, cnt_c, step_counter}, slot);
   const int off_r = slot[0], off_c = slot[1];
   if (valid) {
     const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
     const V3 d01 = unit_to_01(d);
-    // generateNetworkInput_resample_perf_kernel (198-240): t_mid = t0 + dt/2
-    for (int k = 0; k < nr; k++) {
-      const float t0 = tr_scr[(int64_t)k * n_alive + i];
+    uint64_t order = 0;  // bit q set: the q-th kept sample is a cache sample
+    int kr = 0, kc = 0;
+    for (int k = 0; k < cnt; k++) {
+      const bool is_r = (res_mask >> k) & 1u, is_c = (cache_mask >> k) & 1u;
+      if (!is_r && !is_c) continue;
+      const float t0 = t_scr[(int64_t)k * n_alive + i];
       const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;
-      const V3 p = contract(o + tm2 * d, con);
-      const int64_t q = (int64_t)off_r + k;
-      s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
-      s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
-      s_t0[q] = t0;
-    }
-    // generateNetworkInput_preparation_perf_kernel (242-313)
-    for (int k = 0; k < nc; k++) {
-      const float t0 = tc_scr[(int64_t)k * n_alive + i];
-      const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;
-      float tf;
-      const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
-      const int64_t q = (int64_t)off_c + k;
-      c_frox[3 * q] = f.x; c_frox[3 * q + 1] = f.y; c_frox[3 * q + 2] = f.z;
-      c_dir[3 * q] = d01.x; c_dir[3 * q + 1] = d01.y; c_dir[3 * q + 2] = d01.z;
-      c_t0[q] = t0;
+      if (is_r) {
+        // generateNetworkInput_resample_perf_kernel (198-240): t_mid = t0 + dt/2
+        const V3 p = contract(o + tm2 * d, con);
+        const int64_t q = (int64_t)off_r + kr;
+        s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
+        s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
+        s_t0[q] = t0;
+      } else {
+        // generateNetworkInput_preparation_perf_kernel (242-313)
+        float tf;
+        const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
+        const int64_t q = (int64_t)off_c + kc;
+        c_frox[3 * q] = f.x; c_frox[3 * q + 1] = f.y; c_frox[3 * q + 2] = f.z;
+        c_dir[3 * q] = d01.x; c_dir[3 * q + 1] = d01.y; c_dir[3 * q + 2] = d01.z;
+        c_t0[q] = t0;
+        order |= 1ull << (kr + kc);
+      }
+      kr += is_r ? 1 : 0;
+      kc += is_c ? 1 : 0;
     }
     rays[i].t = r.t;
     rays[i].j = r.j;
