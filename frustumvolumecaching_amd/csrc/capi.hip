@@ -142,6 +142,50 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
 
 void fvc_model_destroy(fvc_model* m) { delete m; }
 
+int fvc_nerfacc_read_config(const char* dir, fvc_scene_info* scene, fvc_stepsize_info* stepsize, fvc_net_desc* net,
+                            int64_t* counts) {
+  return guard([&] {
+    REQUIRE(dir && scene && stepsize && net && counts, "null argument");
+    NerfaccExport ex;
+    nerfacc_read_config(dir, ex);
+    *scene = ex.scene;
+    *stepsize = ex.step;
+    *net = ex.net;
+    counts[0] = ex.n_pos;
+    counts[1] = ex.n_base;
+    counts[2] = ex.n_head;
+    counts[3] = ex.n_occ;
+  });
+}
+
+int fvc_nerfacc_read_data(const char* dir, uint16_t* pos, uint16_t* base, uint16_t* head, uint8_t* occ) {
+  return guard([&] {
+    REQUIRE(dir && pos && base && head && occ, "null argument");
+    NerfaccExport ex;
+    nerfacc_read_config(dir, ex);
+    nerfacc_read_data(dir, ex, pos, base, head, occ);
+  });
+}
+
+int fvc_model_load_nerfacc(const char* dir, fvc_scene_info* scene, fvc_stepsize_info* stepsize, fvc_net_desc* net,
+                           fvc_model** out) {
+  return guard([&] {
+    REQUIRE(dir && out, "null argument");
+    *out = nullptr;
+    NerfaccExport ex;
+    nerfacc_read_config(dir, ex);
+    std::vector<uint16_t> pos((size_t)ex.n_pos), base((size_t)ex.n_base), head((size_t)ex.n_head);
+    std::vector<uint8_t> occ((size_t)ex.n_occ);
+    nerfacc_read_data(dir, ex, pos.data(), base.data(), head.data(), occ.data());
+    const int rc = fvc_model_create(&ex.scene, &ex.net, occ.data(), pos.data(), ex.n_pos, base.data(), ex.n_base,
+                                    head.data(), ex.n_head, out);
+    if (rc != FVC_OK) throw Error(rc, fvc_last_error_message());
+    if (scene) *scene = ex.scene;
+    if (stepsize) *stepsize = ex.step;
+    if (net) *net = ex.net;
+  });
+}
+
 int fvc_model_get_bitfield(fvc_model* h, uint8_t* out, int64_t n) {
   return guard([&] {
     REQUIRE(h && out, "null argument");

@@ -94,6 +94,18 @@ struct Model {
 // layout of the parameter blocks (nerfacc export) for a description
 bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n_base, int64_t& n_head);
 
+// model export of the reference (loader.hip)
+struct NerfaccExport {
+  fvc_scene_info scene{};
+  fvc_stepsize_info step{};
+  fvc_net_desc net{};
+  bool pos_separate = true;
+  int64_t n_pos = 0, n_base = 0, n_head = 0, n_occ = 0;
+};
+void nerfacc_read_config(const std::string& dir, NerfaccExport& ex);
+void nerfacc_read_data(const std::string& dir, const NerfaccExport& ex, uint16_t* pos, uint16_t* base, uint16_t* head,
+                       uint8_t* occ);
+
 // network launches (network.hip)
 void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s);

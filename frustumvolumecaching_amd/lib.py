@@ -35,6 +35,11 @@ _SIGS = {
     "fvc_model_create": (C.c_int, [C.POINTER(abi.SceneInfo), C.POINTER(abi.NetDesc), _P, _P, C.c_int64, _P,
                                    C.c_int64, _P, C.c_int64, C.POINTER(_P)]),
     "fvc_model_destroy": (None, [_P]),
+    "fvc_nerfacc_read_config": (C.c_int, [C.c_char_p, C.POINTER(abi.SceneInfo), C.POINTER(abi.StepsizeInfo),
+                                          C.POINTER(abi.NetDesc), C.POINTER(C.c_int64)]),
+    "fvc_nerfacc_read_data": (C.c_int, [C.c_char_p, _P, _P, _P, _P]),
+    "fvc_model_load_nerfacc": (C.c_int, [C.c_char_p, C.POINTER(abi.SceneInfo), C.POINTER(abi.StepsizeInfo),
+                                         C.POINTER(abi.NetDesc), C.POINTER(_P)]),
     "fvc_model_get_bitfield": (C.c_int, [_P, _P, C.c_int64]),
     "fvc_network_latents": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
     "fvc_network_inference": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P]),
@@ -112,6 +117,20 @@ def stepsize_setup(step, scene):
     return step
 
 
+def nerfacc_read(data_dir):
+    """load_nerfacc_export without the GPU part (loader.cu:232-285): returns
+    (scene, stepsize, net, (pos_enc, mlp_base, mlp_head) fp16 bits, occupancy)."""
+    import numpy as np
+    scene, step, net = abi.SceneInfo(), abi.StepsizeInfo(), abi.NetDesc()
+    counts = (C.c_int64 * 4)()
+    d = os.fsencode(str(data_dir))
+    check(load().fvc_nerfacc_read_config(d, C.byref(scene), C.byref(step), C.byref(net), counts))
+    pos, base, head = (np.zeros(int(counts[k]), np.uint16) for k in range(3))
+    occ = np.zeros(int(counts[3]), np.uint8)
+    check(load().fvc_nerfacc_read_data(d, _addr(pos), _addr(base), _addr(head), _addr(occ)))
+    return scene, step, net, (pos, base, head), occ
+
+
 def net_param_counts(net):
     a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
     check(load().fvc_net_param_counts(C.byref(net), C.byref(a), C.byref(b), C.byref(c)))
@@ -135,6 +154,18 @@ class NerfNetwork:
     @classmethod
     def from_workload(cls, wl):
         return cls(wl.scene, wl.net, wl.occupancy, wl.params)
+
+    @classmethod
+    def from_nerfacc(cls, data_dir):
+        """load_data for a nerfacc export directory (loader.cu:232-301);
+        returns (model, scene, stepsize)."""
+        self = cls.__new__(cls)
+        scene, step, net = abi.SceneInfo(), abi.StepsizeInfo(), abi.NetDesc()
+        h = C.c_void_p()
+        check(load().fvc_model_load_nerfacc(os.fsencode(str(data_dir)), C.byref(scene), C.byref(step),
+                                            C.byref(net), C.byref(h)))
+        self.scene, self.net, self.h = scene, net, h
+        return self, scene, step
 
     def close(self):
         if getattr(self, "h", None):

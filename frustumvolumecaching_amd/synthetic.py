@@ -327,3 +327,57 @@ def _default_setup(step, scene):
     """StepsizeInfo::setAdditionalInfo through the product library (host code)."""
     from . import lib
     lib.stepsize_setup(step, scene)
+
+
+# ---------------------------------------------------------------------------
+# model export (the reference's nerfacc layout, loader.cu:232-285)
+# ---------------------------------------------------------------------------
+def nerfacc_config(wl):
+    """config.json content of a workload in the nerfacc export layout."""
+    sc, st, net = wl.scene, wl.step, wl.net
+    scene = {
+        "grid_resolution": int(sc.grid_resolution), "grid_nlvl": int(sc.grid_nlvl),
+        "aabb": [sc.aabb_from.x, sc.aabb_from.y, sc.aabb_from.z, sc.aabb_to.x, sc.aabb_to.y, sc.aabb_to.z],
+        "is_open_gl": bool(sc.is_open_gl), "alpha_thre": float(sc.alpha_thre),
+        "cone_angle": float(st.cone_angle), "stepsize": float(st.stepsize), "near": float(st.near_),
+        "far": float(st.far_),
+    }
+    if sc.contraction_type == abi.CONTRACTION_WARP_AABB_LINF:
+        c0, c1 = sc.contraction_aabb_from, sc.contraction_aabb_to
+        scene["contraction"] = {"aabb": [c0.x, c0.y, c0.z, c1.x, c1.y, c1.z]}
+    mlp = {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "None"}
+    return {
+        "scene": scene,
+        "mlp_base": {
+            "encoding": {"otype": "HashGrid", "n_levels": net.n_levels, "n_features_per_level": net.n_features_per_level,
+                         "log2_hashmap_size": net.log2_hashmap_size, "base_resolution": net.base_resolution,
+                         "per_level_scale": float(np.float32(net.per_level_scale))},
+            "network": dict(mlp, n_neurons=net.base_width, n_hidden_layers=net.base_n_hidden_layers),
+            "n_output": net.base_n_output,
+        },
+        "mlp_head": {
+            "encoding": {"otype": "Composite", "nested": [
+                {"n_dims_to_encode": 3, "otype": "SphericalHarmonics", "degree": net.sh_degree},
+                {"otype": "Identity"}]},
+            "network": dict(mlp, n_neurons=net.head_width, n_hidden_layers=net.head_n_hidden_layers),
+        },
+    }
+
+
+def write_nerfacc_export(wl, out_dir, pos_separate=True):
+    """Write a workload as config.json + raw fp16 / uint8 .dat files; with
+    pos_separate=False the hash table is appended to mlp_base.dat."""
+    import json
+    import os
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, "config.json"), "w") as f:
+        json.dump(nerfacc_config(wl), f, indent=1)
+    pos, base, head = (np.ascontiguousarray(p, np.uint16) for p in wl.params)
+    if pos_separate:
+        pos.tofile(os.path.join(out_dir, "pos_enc.dat"))
+        base.tofile(os.path.join(out_dir, "mlp_base.dat"))
+    else:
+        np.concatenate([base, pos]).tofile(os.path.join(out_dir, "mlp_base.dat"))
+    head.tofile(os.path.join(out_dir, "mlp_head.dat"))
+    np.ascontiguousarray(wl.occupancy, np.uint8).tofile(os.path.join(out_dir, "occupancy_grid.dat"))
+    return out_dir

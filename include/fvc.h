@@ -202,6 +202,21 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net,
                      const uint16_t* params_mlp_head, int64_t n_mlp_head,
                      fvc_model** out_model);
 void fvc_model_destroy(fvc_model* model);
+/* ---- model export: load_nerfacc_export (loader.cu:232-285) ----
+ * <dir>/config.json ("scene", "mlp_base", "mlp_head"; parse_helper.h:25-63)
+ * and raw .dat files: pos_enc.dat (fp16 table; optional, else appended to
+ * mlp_base.dat), mlp_base.dat, mlp_head.dat (fp16), occupancy_grid.dat
+ * (uint8 [nlvl][res][res][res]).  read_config parses config.json, applies
+ * StepsizeInfo::setAdditionalInfo and checks the file sizes; counts[4] =
+ * element counts of pos_enc, mlp_base, mlp_head, occupancy.  Host only. */
+int fvc_nerfacc_read_config(const char* dir, fvc_scene_info* scene_out, fvc_stepsize_info* stepsize_out,
+                            fvc_net_desc* net_out, int64_t* counts_out);
+/* Read the four data blocks into caller buffers sized by read_config.  Host only. */
+int fvc_nerfacc_read_data(const char* dir, uint16_t* pos_enc, uint16_t* mlp_base, uint16_t* mlp_head,
+                          uint8_t* occupancy_linear_u8);
+/* read_config + read_data + fvc_model_create; scene / stepsize / net may be NULL. */
+int fvc_model_load_nerfacc(const char* dir, fvc_scene_info* scene_out, fvc_stepsize_info* stepsize_out,
+                           fvc_net_desc* net_out, fvc_model** out_model);
 /* Copy the Morton bitfield (nlvl*res^3/8 bytes) to host. */
 int fvc_model_get_bitfield(fvc_model* model, uint8_t* out_host, int64_t n_bytes);
 
