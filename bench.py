@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=16, help="rows of the CPU baseline band")
     ap.add_argument("--batch", type=int, default=0, help="target samples per launch (0 = library default)")
     ap.add_argument("--max-steps", type=int, default=0, help="cap on samples per ray per launch")
+    ap.add_argument("--precision", choices=["fp16", "bf16", "fp8"], default="fp16",
+                    help="MLP arithmetic (operands; FP32 accumulation)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference point for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -79,6 +81,8 @@ def main():
     wl = S.unbounded_1080p(width=a.width, height=a.height, stepsize_setup=lib.stepsize_setup)
     path = S.camera_path_1080p(wl, 30)
     log(f"[bench] workload built in {time.time() - t0:.1f}s")
+    from frustumvolumecaching_amd import abi
+    wl.net.precision = {"fp16": abi.PREC_FP16, "bf16": abi.PREC_BF16, "fp8": abi.PREC_FP8}[a.precision]
     model = lib.NerfNetwork.from_workload(wl)
     r = lib.Renderer(model)
     if a.batch or a.max_steps:
@@ -235,7 +239,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16",
+        "dtype": a.precision,
         "data": "synthetic (seeded scene and random-init weights; no dataset)",
         "config": {"workload": f"unbounded_{W}x{H}: hash grid L16 F2 T2^21, 5x128^3 occupancy cascades, 64 blobs "
                                f"to r=16, cone 1/256; 30-frame path yaw 1deg + 0.02 fwd/frame, refill every 10",

@@ -549,15 +549,18 @@ __global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, i
 // (sample l%16, group g) fetches corners 2g and 2g+1 (corner bits x=4,y=2,z=1);
 // the two xor-shuffles over the 4 lanes of a sample reproduce the reference's
 // ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) summation order.
+template <int P>
 __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_frox, const float* __restrict__ c_dir,
-                                                    int64_t n, CacheView cv, const h8* __restrict__ frags,
+                                                    int64_t n, CacheView cv,
+                                                    const typename Mx<P>::T* __restrict__ frags, MlpScales sc,
                                                     uint2* __restrict__ c_out) {
+  __shared__ typename Mx<P>::T wl[N_FRAGS * 64];
+  stage_frags(frags, wl);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
-  HeadWeights hw;
-  load_head(frags, lane, hw);
   const int64_t n_groups = (n + 15) / 16;
   for (int64_t grp = (int64_t)blockIdx.x * 4 + wave; grp < n_groups; grp += (int64_t)gridDim.x * 4) {
+    const int li = opaque(lane);
     const int64_t s = grp * 16 + col;
     const bool valid = s < n;
     const int64_t sl = valid ? s : n - 1;
@@ -618,8 +621,8 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
       for (int i = 1; i < 8; i++) num = li == i ? ld[i] : num;
       rows[e] = (li >= 0 && li < 8) ? (_Float16)(num / wd) : (_Float16)1.0f;
     }
-    const h8 x = head_input(rows, g, c_dir[3 * sl], c_dir[3 * sl + 1], c_dir[3 * sl + 2]);
-    const f4 o = head_mlp(hw, x);
+    const typename Mx<P>::T x = head_input<P>(rows, g, c_dir[3 * sl], c_dir[3 * sl + 1], c_dir[3 * sl + 2], sc.inv_act);
+    const f4 o = head_mlp<P>(wl, li, x, sc);
     if (g == 0 && valid) {
       uint2 v;
       v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
@@ -743,8 +746,15 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
       const int64_t groups = ((int64_t)nc + 15) / 16;
       const int blocks = (int)std::min<int64_t>((groups + 3) / 4, 256 * 16);
       ev = P.begin(s);
-      hipLaunchKernelGGL(k_cache_head, dim3(blocks), dim3(256), 0, s, r.c_frox.as<float>(), r.c_dir.as<float>(),
-                         (int64_t)nc, cv, m.frags.as<h8>(), r.c_out.as<uint2>());
+#define L(P)                                                                                                  \
+  hipLaunchKernelGGL((k_cache_head<P>), dim3(blocks), dim3(256), 0, s, r.c_frox.as<float>(), r.c_dir.as<float>(), \
+                     (int64_t)nc, cv, m.frags.as<typename Mx<P>::T>(), m.mlp_scales, r.c_out.as<uint2>())
+      switch (m.net.precision) {
+        case FVC_PREC_BF16: L(FVC_PREC_BF16); break;
+        case FVC_PREC_FP8: L(FVC_PREC_FP8); break;
+        default: L(FVC_PREC_FP16); break;
+      }
+#undef L
       FVC_HIP(hipGetLastError());
       P.end(ev, s, "cache_head", nc, nc * (ALG_LOOKUP_B + 24.0 + 8.0), nc * ALG_COLOUR_FLOP);
     }

@@ -11,7 +11,8 @@
 namespace fvc {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& m) { g_last_error = m; }
-void pack_fragments(const uint16_t* base, const uint16_t* head, std::vector<uint16_t>& out);
+void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, std::vector<uint8_t>& out,
+                    MlpScales& sc);
 }  // namespace fvc
 
 using namespace fvc;
@@ -83,7 +84,8 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
     // PerformanceRenderer only supports the fixed 128^3 grid (performance_renderer.cu:162-163)
     if (scene->grid_resolution != 128) throw Error(FVC_E_UNSUPPORTED, "only grid_resolution 128 is supported");
     REQUIRE(scene->grid_nlvl >= 1 && scene->grid_nlvl <= 16, "grid_nlvl out of range");
-    if (net->precision != FVC_PREC_FP16) throw Error(FVC_E_UNSUPPORTED, "only FP16 MLP arithmetic is built");
+    if (net->precision != FVC_PREC_FP16 && net->precision != FVC_PREC_BF16 && net->precision != FVC_PREC_FP8)
+      throw Error(FVC_E_UNSUPPORTED, "unknown MLP precision");
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) throw Error(FVC_E_NO_DEVICE, "no HIP device");
     auto* h = new fvc_model();
@@ -115,10 +117,10 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
       FVC_HIP(hipMemcpy(m.coarse.p, coarse.data(), coarse.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
       m.table.ensure(n_pos * sizeof(uint16_t));
       FVC_HIP(hipMemcpy(m.table.p, p_pos, n_pos * sizeof(uint16_t), hipMemcpyHostToDevice));
-      std::vector<uint16_t> frags;
-      pack_fragments(p_base, p_head, frags);
-      m.frags.ensure(frags.size() * sizeof(uint16_t));
-      FVC_HIP(hipMemcpy(m.frags.p, frags.data(), frags.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+      std::vector<uint8_t> frags;
+      pack_fragments(p_base, p_head, net->precision, frags, m.mlp_scales);
+      m.frags.ensure(frags.size());
+      FVC_HIP(hipMemcpy(m.frags.p, frags.data(), frags.size(), hipMemcpyHostToDevice));
       m.occ = occ_params(*scene);
       const V3 from = cvt(scene->aabb_from), to = cvt(scene->aabb_to);
       // apply_contraction constants (raymarch_common.h:610-653)

@@ -76,6 +76,13 @@ struct GridParams {
 static const int N_FRAGS = 20;
 static const int FRAG_W1 = 0, FRAG_W2 = 4, FRAG_W3 = 12, FRAG_H1 = 14, FRAG_H2 = 18;
 
+// per-layer accumulator scales of the MLPs (FP8: s_w[l] * s_a; otherwise 1)
+// and 1 / s_a (mlp_mfma.h)
+struct MlpScales {
+  float layer[5];   // W1, W2, W3, H1, H2
+  float inv_act;
+};
+
 struct Model {
   int device = 0;
   fvc_scene_info scene{};
@@ -87,7 +94,8 @@ struct Model {
   DevBuf bitfield;   // Morton occupancy bitfield
   DevBuf coarse;     // per cascade 16^3 "any of 8^3 cells occupied" bits
   DevBuf table;      // hash grid, uint32 per entry (2 x fp16)
-  DevBuf frags;      // packed weight fragments (fp16)
+  DevBuf frags;      // packed weight fragments in the MLP precision
+  MlpScales mlp_scales{};
   std::vector<uint8_t> bitfield_host;
 };
 

@@ -1,15 +1,23 @@
-// mlp_mfma.h -- register-resident MLP tiles on v_mfma_f32_16x16x32_f16.
+// mlp_mfma.h -- MLP tiles on the 16x16x32 MFMA instructions of gfx950, in
+// FP16, BF16 or FP8-E4M3 operands with FP32 accumulation.
 //
 // One wave64 evaluates 16 samples per step.  Operands use the transposed
 // formulation D[neuron][sample] = W[neuron][k] * X[k][sample]:
-//   A (16x32, weights):  lane l -> neuron row l%16, k-group g = l/16 (8 halves)
-//   B (32x16, inputs):   lane l -> sample   l%16, k-group g       (8 halves)
+//   A (16x32, weights):  lane l -> neuron row l%16, k-group g = l/16 (8 values)
+//   B (32x16, inputs):   lane l -> sample   l%16, k-group g       (8 values)
 //   D (16x16, f32):      lane l -> sample   l%16, neurons 4g..4g+3
 // so the D fragments of one layer become the B fragment of the next without
-// any data movement: B[kb] = {relu(D[2kb]) , relu(D[2kb+1])} as halves.  The
-// k-index order inside a lane is a free permutation, applied to the weights
-// once on the host (pack_fragments in network.hip).  Accumulation is FP32;
-// activations are rounded to fp16 between layers, as the reference stores them.
+// any data movement: B[kb] = {act(D[2kb]), act(D[2kb+1])}.  The k-index
+// order inside a lane is a free permutation, applied to the weights once on
+// the host (pack_fragments in network.hip).
+//
+// Storage rounding follows the reference where it stores half: encodings,
+// the density/latent outputs and the colour outputs are fp16.  In BF16 mode
+// every MLP operand (weights, inputs, hidden activations) is rounded to
+// bfloat16 (RNE).  In FP8 mode weights are E4M3 codes of w / s_w with the
+// per-layer scale s_w = absmax(W) / 448, activations codes of a / s_a with
+// the fixed s_a = 64 / 448 (saturating at +-448); each layer's accumulator is
+// rescaled by s_w * s_a.  The weight fragments are staged in LDS per block.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -18,88 +26,103 @@
 namespace fvc {
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ f4 mfma(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ f4 f4zero() { return f4{0.0f, 0.0f, 0.0f, 0.0f}; }
 
-// B fragment for K block kb from four 16-neuron D blocks (ReLU + fp16)
-__device__ __forceinline__ h8 relu_pack(f4 lo, f4 hi) {
-  h8 b;
-  b[0] = (_Float16)fmaxf(lo[0], 0.0f); b[1] = (_Float16)fmaxf(lo[1], 0.0f);
-  b[2] = (_Float16)fmaxf(lo[2], 0.0f); b[3] = (_Float16)fmaxf(lo[3], 0.0f);
-  b[4] = (_Float16)fmaxf(hi[0], 0.0f); b[5] = (_Float16)fmaxf(hi[1], 0.0f);
-  b[6] = (_Float16)fmaxf(hi[2], 0.0f); b[7] = (_Float16)fmaxf(hi[3], 0.0f);
-  return b;
+static const float FP8_ACT_SCALE = 64.0f / 448.0f;
+
+// operand traits per precision
+template <int P> struct Mx;
+template <> struct Mx<FVC_PREC_FP16> {
+  typedef h8 T;
+  static __device__ __forceinline__ T pack(const float* v, float) {
+    T r;
+#pragma unroll
+    for (int e = 0; e < 8; e++) r[e] = (_Float16)v[e];
+    return r;
+  }
+  static __device__ __forceinline__ f4 mma(T a, T b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mx<FVC_PREC_BF16> {
+  typedef b8 T;
+  static __device__ __forceinline__ T pack(const float* v, float) {
+    T r;
+#pragma unroll
+    for (int e = 0; e < 8; e++) r[e] = (__bf16)v[e];
+    return r;
+  }
+  static __device__ __forceinline__ f4 mma(T a, T b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mx<FVC_PREC_FP8> {
+  typedef long T;
+  // E4M3 codes of v * inv_scale, clamped to the finite range (the hardware
+  // conversion rounds to nearest even but does not saturate)
+  static __device__ __forceinline__ T pack(const float* v, float inv_scale) {
+    float q[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) q[e] = fminf(fmaxf(v[e] * inv_scale, -448.0f), 448.0f);
+    int lo = __builtin_amdgcn_cvt_pk_fp8_f32(q[0], q[1], 0, false);
+    lo = __builtin_amdgcn_cvt_pk_fp8_f32(q[2], q[3], lo, true);
+    int hi = __builtin_amdgcn_cvt_pk_fp8_f32(q[4], q[5], 0, false);
+    hi = __builtin_amdgcn_cvt_pk_fp8_f32(q[6], q[7], hi, true);
+    return (long)(((unsigned long)(unsigned)hi << 32) | (unsigned)lo);
+  }
+  static __device__ __forceinline__ f4 mma(T a, T b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
+  }
+};
+
+template <int P> __device__ __forceinline__ f4 rescale(f4 d, float s) {
+  if (P == FVC_PREC_FP8) return d * s;
+  return d;
 }
 
-struct BaseWeights { h8 w1[4], w2[4][2], w3[2]; };
-struct HeadWeights { h8 h1[4], h2[2]; };
-
-__device__ __forceinline__ void load_base(const h8* __restrict__ frags, int lane, BaseWeights& w) {
-#pragma unroll
-  for (int nb = 0; nb < 4; nb++) w.w1[nb] = frags[(FRAG_W1 + nb) * 64 + lane];
-#pragma unroll
-  for (int nb = 0; nb < 4; nb++)
-#pragma unroll
-    for (int kb = 0; kb < 2; kb++) w.w2[nb][kb] = frags[(FRAG_W2 + nb * 2 + kb) * 64 + lane];
-#pragma unroll
-  for (int kb = 0; kb < 2; kb++) w.w3[kb] = frags[(FRAG_W3 + kb) * 64 + lane];
-}
-__device__ __forceinline__ void load_head(const h8* __restrict__ frags, int lane, HeadWeights& w) {
-#pragma unroll
-  for (int nb = 0; nb < 4; nb++) w.h1[nb] = frags[(FRAG_H1 + nb) * 64 + lane];
-#pragma unroll
-  for (int kb = 0; kb < 2; kb++) w.h2[kb] = frags[(FRAG_H2 + kb) * 64 + lane];
+// B fragment from two 16-neuron D blocks: ReLU, then operand rounding
+template <int P> __device__ __forceinline__ typename Mx<P>::T act_pack(f4 lo, f4 hi, float inv_act) {
+  const float v[8] = {fmaxf(lo[0], 0.0f), fmaxf(lo[1], 0.0f), fmaxf(lo[2], 0.0f), fmaxf(lo[3], 0.0f),
+                      fmaxf(hi[0], 0.0f), fmaxf(hi[1], 0.0f), fmaxf(hi[2], 0.0f), fmaxf(hi[3], 0.0f)};
+  return Mx<P>::pack(v, inv_act);
 }
 
-// density / latent MLP 32 -> 64 -> 64 -> 16; returns output rows 4g..4g+3 (f32)
-__device__ __forceinline__ f4 base_mlp(const BaseWeights& w, h8 x) {
+// density / latent MLP 32 -> 64 -> 64 -> 16 with the fragments in LDS (w);
+// li is the lane index made opaque per loop iteration by the caller so the
+// reads are not hoisted into registers.  Returns output rows 4g..4g+3.
+template <int P>
+__device__ __forceinline__ f4 base_mlp(const typename Mx<P>::T* w, int li, typename Mx<P>::T x, const MlpScales& sc) {
   f4 d1[4];
 #pragma unroll
-  for (int nb = 0; nb < 4; nb++) d1[nb] = mfma(w.w1[nb], x, f4zero());
-  const h8 b2_0 = relu_pack(d1[0], d1[1]), b2_1 = relu_pack(d1[2], d1[3]);
+  for (int nb = 0; nb < 4; nb++) d1[nb] = rescale<P>(Mx<P>::mma(w[(FRAG_W1 + nb) * 64 + li], x, f4zero()), sc.layer[0]);
+  const typename Mx<P>::T b2_0 = act_pack<P>(d1[0], d1[1], sc.inv_act), b2_1 = act_pack<P>(d1[2], d1[3], sc.inv_act);
   f4 d2[4];
 #pragma unroll
-  for (int nb = 0; nb < 4; nb++) d2[nb] = mfma(w.w2[nb][1], b2_1, mfma(w.w2[nb][0], b2_0, f4zero()));
-  const h8 b3_0 = relu_pack(d2[0], d2[1]), b3_1 = relu_pack(d2[2], d2[3]);
-  return mfma(w.w3[1], b3_1, mfma(w.w3[0], b3_0, f4zero()));
-}
-
-// head MLP 32 -> 64 -> 16; returns output rows 4g..4g+3 (rows 0..2 = rgb)
-__device__ __forceinline__ f4 head_mlp(const HeadWeights& w, h8 x) {
-  f4 d1[4];
-#pragma unroll
-  for (int nb = 0; nb < 4; nb++) d1[nb] = mfma(w.h1[nb], x, f4zero());
-  const h8 b_0 = relu_pack(d1[0], d1[1]), b_1 = relu_pack(d1[2], d1[3]);
-  return mfma(w.h2[1], b_1, mfma(w.h2[0], b_0, f4zero()));
-}
-
-// The same MLPs with the weight fragments read from LDS at each use (w =
-// the N_FRAGS x 64 fragments staged by the block); li is the lane index made
-// opaque per loop iteration by the caller so the reads are not hoisted into
-// registers.  Frees 80 VGPRs for gathers in flight.
-__device__ __forceinline__ f4 base_mlp_lds(const h8* w, int li, h8 x) {
-  f4 d1[4];
-#pragma unroll
-  for (int nb = 0; nb < 4; nb++) d1[nb] = mfma(w[(FRAG_W1 + nb) * 64 + li], x, f4zero());
-  const h8 b2_0 = relu_pack(d1[0], d1[1]), b2_1 = relu_pack(d1[2], d1[3]);
-  f4 d2[4];
-#pragma unroll
   for (int nb = 0; nb < 4; nb++)
-    d2[nb] = mfma(w[(FRAG_W2 + nb * 2 + 1) * 64 + li], b2_1, mfma(w[(FRAG_W2 + nb * 2) * 64 + li], b2_0, f4zero()));
-  const h8 b3_0 = relu_pack(d2[0], d2[1]), b3_1 = relu_pack(d2[2], d2[3]);
-  return mfma(w[(FRAG_W3 + 1) * 64 + li], b3_1, mfma(w[FRAG_W3 * 64 + li], b3_0, f4zero()));
+    d2[nb] = rescale<P>(Mx<P>::mma(w[(FRAG_W2 + nb * 2 + 1) * 64 + li], b2_1,
+                                   Mx<P>::mma(w[(FRAG_W2 + nb * 2) * 64 + li], b2_0, f4zero())),
+                        sc.layer[1]);
+  const typename Mx<P>::T b3_0 = act_pack<P>(d2[0], d2[1], sc.inv_act), b3_1 = act_pack<P>(d2[2], d2[3], sc.inv_act);
+  return rescale<P>(Mx<P>::mma(w[(FRAG_W3 + 1) * 64 + li], b3_1, Mx<P>::mma(w[FRAG_W3 * 64 + li], b3_0, f4zero())),
+                    sc.layer[2]);
 }
-__device__ __forceinline__ f4 head_mlp_lds(const h8* w, int li, h8 x) {
+
+// colour MLP 32 -> 64 -> 16; returns output rows 4g..4g+3 (rows 0..2 = rgb)
+template <int P>
+__device__ __forceinline__ f4 head_mlp(const typename Mx<P>::T* w, int li, typename Mx<P>::T x, const MlpScales& sc) {
   f4 d1[4];
 #pragma unroll
-  for (int nb = 0; nb < 4; nb++) d1[nb] = mfma(w[(FRAG_H1 + nb) * 64 + li], x, f4zero());
-  const h8 b_0 = relu_pack(d1[0], d1[1]), b_1 = relu_pack(d1[2], d1[3]);
-  return mfma(w[(FRAG_H2 + 1) * 64 + li], b_1, mfma(w[FRAG_H2 * 64 + li], b_0, f4zero()));
+  for (int nb = 0; nb < 4; nb++) d1[nb] = rescale<P>(Mx<P>::mma(w[(FRAG_H1 + nb) * 64 + li], x, f4zero()), sc.layer[3]);
+  const typename Mx<P>::T b_0 = act_pack<P>(d1[0], d1[1], sc.inv_act), b_1 = act_pack<P>(d1[2], d1[3], sc.inv_act);
+  return rescale<P>(Mx<P>::mma(w[(FRAG_H2 + 1) * 64 + li], b_1, Mx<P>::mma(w[FRAG_H2 * 64 + li], b_0, f4zero())),
+                    sc.layer[4]);
 }
+
 // stage all weight fragments of the model into LDS (every thread of the block)
-__device__ __forceinline__ void stage_frags(const h8* __restrict__ frags, h8* lds) {
+template <class T> __device__ __forceinline__ void stage_frags(const T* __restrict__ frags, T* lds) {
   for (int i = threadIdx.x; i < N_FRAGS * 64; i += blockDim.x) lds[i] = frags[i];
   __syncthreads();
 }
@@ -141,18 +164,22 @@ __device__ __forceinline__ void sh4_quarter(float dx01, float dy01, float dz01, 
 // Head input fragment.  Slots e=0..3 carry base-output rows 4g+e: rows 1..8
 // are the latents, rows 0 and 9..15 are replaced by the constant 1.0 padding
 // of the reference's head input (nerf_network.h:248-253); slots e=4..7 carry
-// SH coefficients 4g..4g+3.
-__device__ __forceinline__ h8 head_input(const _Float16* rows4, int g, float dx01, float dy01, float dz01) {
-  h8 b;
+// SH coefficients 4g..4g+3.  Values are the fp16 ones the reference stores,
+// then rounded to the MLP's operand precision.
+template <int P>
+__device__ __forceinline__ typename Mx<P>::T head_input(const _Float16* rows4, int g, float dx01, float dy01,
+                                                         float dz01, float inv_act) {
+  float v[8];
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     const int r = 4 * g + e;
-    b[e] = (r >= 1 && r <= 8) ? rows4[e] : (_Float16)1.0f;
+    v[e] = (r >= 1 && r <= 8) ? (float)rows4[e] : 1.0f;
   }
   _Float16 sh[4];
   sh4_quarter(dx01, dy01, dz01, g, sh);
-  b[4] = sh[0]; b[5] = sh[1]; b[6] = sh[2]; b[7] = sh[3];
-  return b;
+#pragma unroll
+  for (int e = 0; e < 4; e++) v[4 + e] = (float)sh[e];
+  return Mx<P>::pack(v, inv_act);
 }
 
 }  // namespace fvc

@@ -14,6 +14,7 @@
 // first layer's B fragment, whose weight columns are permuted to match on the
 // host -- so the gathered features feed the MFMA chain in mlp_mfma.h
 // directly from registers.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -140,15 +141,23 @@ __device__ __forceinline__ void load_levels(const GridParams& gp, GridLevel* lv)
 // ------------------------------------------------------------------ kernels
 static const int NET_BLOCK = 256;  // 4 waves
 
+__device__ __forceinline__ uint2 pack_h4(_Float16 a, _Float16 b, _Float16 c, _Float16 d) {
+  uint2 v;
+  v.x = (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+  v.y = (uint32_t)__builtin_bit_cast(uint16_t, c) | ((uint32_t)__builtin_bit_cast(uint16_t, d) << 16);
+  return v;
+}
+
 // HEAD: full inference -> half4 (r, g, b, log sigma) per sample
 // !HEAD: inferenceLatents -> 16 halves per sample
-template <bool HEAD>
+template <bool HEAD, int P>
 __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__ pos, const float* __restrict__ dir,
                                                        int64_t n, const uint32_t* __restrict__ table,
-                                                       const h8* __restrict__ frags, GridParams gp,
-                                                       uint2* __restrict__ out_h4, uint2* __restrict__ out16) {
+                                                       const typename Mx<P>::T* __restrict__ frags, GridParams gp,
+                                                       MlpScales sc, uint2* __restrict__ out_h4,
+                                                       uint2* __restrict__ out16) {
   __shared__ GridLevel lv[MAX_LEVELS];
-  __shared__ h8 wl[N_FRAGS * 64];
+  __shared__ typename Mx<P>::T wl[N_FRAGS * 64];
   load_levels(gp, lv);
   stage_frags(frags, wl);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -163,59 +172,46 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
     const V3 p = {pos[3 * sl], pos[3 * sl + 1], pos[3 * sl + 2]};
     float f[8];
     encode_quads(lv, gp, table, p, g, f);
-    h8 x;
 #pragma unroll
-    for (int e = 0; e < 8; e++) x[e] = (_Float16)f[e];
-    const f4 d3 = base_mlp_lds(wl, li, x);
-    _Float16 rows[4] = {(_Float16)d3[0], (_Float16)d3[1], (_Float16)d3[2], (_Float16)d3[3]};
+    for (int e = 0; e < 8; e++) f[e] = (float)(_Float16)f[e];  // the encoding is stored as half
+    const typename Mx<P>::T x = Mx<P>::pack(f, sc.inv_act);
+    const f4 d3 = base_mlp<P>(wl, li, x, sc);
+    const _Float16 rows[4] = {(_Float16)d3[0], (_Float16)d3[1], (_Float16)d3[2], (_Float16)d3[3]};
     if (!HEAD) {
-      if (valid) {
-        uint2 v;
-        v.x = (uint32_t)__builtin_bit_cast(uint16_t, rows[0]) | ((uint32_t)__builtin_bit_cast(uint16_t, rows[1]) << 16);
-        v.y = (uint32_t)__builtin_bit_cast(uint16_t, rows[2]) | ((uint32_t)__builtin_bit_cast(uint16_t, rows[3]) << 16);
-        out16[s * 4 + g] = v;
-      }
+      if (valid) out16[s * 4 + g] = pack_h4(rows[0], rows[1], rows[2], rows[3]);
     } else {
-      const h8 hin = head_input(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2]);
-      const f4 o = head_mlp_lds(wl, li, hin);
-      if (g == 0 && valid) {
-        uint2 v;
-        v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
-              ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[1]) << 16);
-        v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[2]) |
-              ((uint32_t)__builtin_bit_cast(uint16_t, rows[0]) << 16);
-        out_h4[s] = v;
-      }
+      const typename Mx<P>::T hin = head_input<P>(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2], sc.inv_act);
+      const f4 o = head_mlp<P>(wl, li, hin, sc);
+      if (g == 0 && valid) out_h4[s] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2], rows[0]);
     }
   }
 }
 
 // inferenceHead on given head inputs (16 halves per sample: row 0 log sigma,
 // rows 1..8 latents)
+template <int P>
 __global__ __launch_bounds__(NET_BLOCK) void k_head(const uint16_t* __restrict__ hin16, const float* __restrict__ dir,
-                                                    int64_t n, const h8* __restrict__ frags, uint2* __restrict__ out_h4) {
+                                                    int64_t n, const typename Mx<P>::T* __restrict__ frags,
+                                                    MlpScales sc, uint2* __restrict__ out_h4) {
+  __shared__ typename Mx<P>::T wl[N_FRAGS * 64];
+  stage_frags(frags, wl);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
-  HeadWeights hw;
-  load_head(frags, lane, hw);
   const int64_t n_groups = (n + 15) / 16;
   for (int64_t grp = (int64_t)blockIdx.x * (NET_BLOCK / 64) + wave; grp < n_groups;
        grp += (int64_t)gridDim.x * (NET_BLOCK / 64)) {
+    const int li = opaque(lane);
     const int64_t s = grp * 16 + col;
     const bool valid = s < n;
     const int64_t sl = valid ? s : n - 1;
     _Float16 rows[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) rows[e] = __builtin_bit_cast(_Float16, hin16[sl * 16 + 4 * g + e]);
-    const h8 x = head_input(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2]);
-    const f4 o = head_mlp(hw, x);
-    if (g == 0 && valid) {
-      uint2 v;
-      v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
-            ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[1]) << 16);
-      v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[2]) | ((uint32_t)hin16[sl * 16] << 16);
-      out_h4[s] = v;
-    }
+    const typename Mx<P>::T x = head_input<P>(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2], sc.inv_act);
+    const f4 o = head_mlp<P>(wl, li, x, sc);
+    if (g == 0 && valid)
+      out_h4[s] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2],
+                          __builtin_bit_cast(_Float16, hin16[sl * 16]));
   }
 }
 
@@ -245,25 +241,43 @@ static int grid_for(int64_t n_groups) {
   return (int)blocks;
 }
 
+// one launch per MLP precision of the model
+#define FVC_BY_PREC(m, LAUNCH)                                               \
+  switch ((m).net.precision) {                                               \
+    case FVC_PREC_BF16: LAUNCH(FVC_PREC_BF16); break;                        \
+    case FVC_PREC_FP8: LAUNCH(FVC_PREC_FP8); break;                          \
+    default: LAUNCH(FVC_PREC_FP16); break;                                   \
+  }
+
 void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_network<true>, dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, pos, dir, n,
-                     m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, (uint2*)out_h4, (uint2*)nullptr);
+#define L(P)                                                                                                 \
+  hipLaunchKernelGGL((k_network<true, P>), dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, pos, dir, n, \
+                     m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, m.mlp_scales, (uint2*)out_h4, \
+                     (uint2*)nullptr)
+  FVC_BY_PREC(m, L)
+#undef L
   FVC_HIP(hipGetLastError());
 }
 void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_network<false>, dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, pos,
-                     (const float*)nullptr, n, m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, (uint2*)nullptr,
-                     (uint2*)out16);
+#define L(P)                                                                                                    \
+  hipLaunchKernelGGL((k_network<false, P>), dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, pos,           \
+                     (const float*)nullptr, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid,  \
+                     m.mlp_scales, (uint2*)nullptr, (uint2*)out16)
+  FVC_BY_PREC(m, L)
+#undef L
   FVC_HIP(hipGetLastError());
 }
 void launch_network_head(const Model& m, const uint16_t* hin16, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_head, dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, hin16, dir, n, m.frags.as<h8>(),
-                     (uint2*)out_h4);
+#define L(P)                                                                                                    \
+  hipLaunchKernelGGL((k_head<P>), dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, hin16, dir, n,            \
+                     m.frags.as<typename Mx<P>::T>(), m.mlp_scales, (uint2*)out_h4)
+  FVC_BY_PREC(m, L)
+#undef L
   FVC_HIP(hipGetLastError());
 }
 void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t* out32, hipStream_t s) {
@@ -286,26 +300,86 @@ static int col_head_in(int g, int e) {                                          
   return r == 0 ? 24 : r + 16;             // padding columns 24..31
 }
 
-void pack_fragments(const uint16_t* base, const uint16_t* head, std::vector<uint16_t>& out) {
-  out.assign((size_t)N_FRAGS * 64 * 8, 0);
-  auto put = [&](int frag, const uint16_t* W, int n_in, int row0, auto colf) {
-    for (int lane = 0; lane < 64; lane++) {
-      const int i = lane % 16, g = lane / 16;
-      for (int e = 0; e < 8; e++) out[((size_t)frag * 64 + lane) * 8 + e] = W[(size_t)(row0 + i) * n_in + colf(g, e)];
-    }
-  };
+// host conversions for the weight fragments
+static float f16_value(uint16_t h) {
+  const uint32_t s = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 31u, m = h & 1023u;
+  float v;
+  if (e == 0) v = std::ldexp((float)m, -24);
+  else if (e == 31) v = m ? NAN : INFINITY;
+  else v = std::ldexp((float)(m | 1024u), (int)e - 25);
+  return s ? -v : v;
+}
+static uint16_t bf16_bits(float f) {  // round to nearest even
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+// OCP E4M3 (bias 7, max 448), round to nearest even, saturating
+static uint8_t e4m3_code(float f) {
+  const uint8_t sign = std::signbit(f) ? 0x80 : 0x00;
+  const float a = std::fabs(f);
+  if (!(a < 448.0f)) return (uint8_t)(sign | 0x7e);
+  int e;
+  std::frexp(a, &e);
+  int ue = e - 1;                               // exponent of the leading bit
+  if (ue < -6) ue = -6;                         // subnormals share exponent -6
+  const float q = std::ldexp(1.0f, ue - 3);     // quantum of 3 fraction bits
+  const float r = std::nearbyint(a / q) * q;    // ties to even
+  if (r >= 448.0f) return (uint8_t)(sign | 0x7e);
+  if (r == 0.0f) return sign;
+  int e2;
+  std::frexp(r, &e2);
+  if (e2 - 1 < -6) return (uint8_t)(sign | (int)std::lround(r / std::ldexp(1.0f, -9)));
+  const int ef = e2 - 1 + 7;
+  const int mant = (int)std::lround(r / std::ldexp(1.0f, e2 - 1 - 3)) - 8;
+  return (uint8_t)(sign | (ef << 3) | mant);
+}
+
+// Fragments in the operand format of the precision: 2 bytes per value for
+// FP16 / BF16, 1 byte for FP8 (E4M3 codes of w / s_w, per-layer
+// s_w = absmax / 448 over the whole fp16 layer).  Fills the scales.
+void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, std::vector<uint8_t>& out,
+                    MlpScales& sc) {
+  const int bytes = precision == FVC_PREC_FP8 ? 1 : 2;
+  out.assign((size_t)N_FRAGS * 64 * 8 * bytes, 0);
   const uint16_t* W1 = base;
   const uint16_t* W2 = base + 32 * 64;
   const uint16_t* W3 = base + 32 * 64 + 64 * 64;
   const uint16_t* H1 = head;
   const uint16_t* H2 = head + 32 * 64;
-  for (int nb = 0; nb < 4; nb++) put(FRAG_W1 + nb, W1, 32, nb * 16, col_first);
+  const uint16_t* Ws[5] = {W1, W2, W3, H1, H2};
+  const int counts[5] = {64 * 32, 64 * 64, 16 * 64, 64 * 32, 16 * 64};
+  float wscale[5];
+  for (int l = 0; l < 5; l++) {
+    float amax = 0.0f;
+    for (int i = 0; i < counts[l]; i++) amax = std::max(amax, std::fabs(f16_value(Ws[l][i])));
+    wscale[l] = amax > 0.0f ? amax / 448.0f : 1.0f;
+    sc.layer[l] = precision == FVC_PREC_FP8 ? wscale[l] * FP8_ACT_SCALE : 1.0f;
+  }
+  sc.inv_act = 1.0f / FP8_ACT_SCALE;
+  auto put = [&](int frag, int layer, const uint16_t* W, int n_in, int row0, auto colf) {
+    for (int lane = 0; lane < 64; lane++) {
+      const int i = lane % 16, g = lane / 16;
+      for (int e = 0; e < 8; e++) {
+        const uint16_t h = W[(size_t)(row0 + i) * n_in + colf(g, e)];
+        const size_t at = ((size_t)frag * 64 + lane) * 8 + e;
+        if (precision == FVC_PREC_FP8) {
+          out[at] = e4m3_code(f16_value(h) / wscale[layer]);
+        } else {
+          const uint16_t v = precision == FVC_PREC_BF16 ? bf16_bits(f16_value(h)) : h;
+          std::memcpy(&out[at * 2], &v, 2);
+        }
+      }
+    }
+  };
+  for (int nb = 0; nb < 4; nb++) put(FRAG_W1 + nb, 0, W1, 32, nb * 16, col_first);
   for (int nb = 0; nb < 4; nb++)
     for (int kb = 0; kb < 2; kb++)
-      put(FRAG_W2 + nb * 2 + kb, W2, 64, nb * 16, [kb](int g, int e) { return col_chain(kb, g, e); });
-  for (int kb = 0; kb < 2; kb++) put(FRAG_W3 + kb, W3, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
-  for (int nb = 0; nb < 4; nb++) put(FRAG_H1 + nb, H1, 32, nb * 16, col_head_in);
-  for (int kb = 0; kb < 2; kb++) put(FRAG_H2 + kb, H2, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
+      put(FRAG_W2 + nb * 2 + kb, 1, W2, 64, nb * 16, [kb](int g, int e) { return col_chain(kb, g, e); });
+  for (int kb = 0; kb < 2; kb++) put(FRAG_W3 + kb, 2, W3, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
+  for (int nb = 0; nb < 4; nb++) put(FRAG_H1 + nb, 3, H1, 32, nb * 16, col_head_in);
+  for (int kb = 0; kb < 2; kb++) put(FRAG_H2 + kb, 4, H2, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
 }
 
 }  // namespace fvc
