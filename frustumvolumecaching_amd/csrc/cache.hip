@@ -440,15 +440,20 @@ struct CacheView {
 // reference stops a launch when either stream reaches n_steps samples; here
 // the launch holds n_steps samples in total.  That only moves sub-iteration
 // boundaries: every ray still composites the same samples in the same order.
-__global__ __launch_bounds__(256) void k_cache_march(int n_alive, int n_steps, int trip_cutoff, MarchGrid mg,
+__global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int par, int trip_cutoff, MarchGrid mg,
                                                      ContractParams con, const uint32_t* __restrict__ coarse_g,
                                                      CacheView cv, RayState* __restrict__ rays, int4* __restrict__ entry,
                                                      int* __restrict__ alive_flag, float* __restrict__ t_scr,
                                                      float* __restrict__ s_pos, float* __restrict__ s_dir,
                                                      float* __restrict__ s_t0, float* __restrict__ c_frox,
                                                      float* __restrict__ c_dir, float* __restrict__ c_t0,
-                                                     uint64_t* __restrict__ order_out, int* __restrict__ cnt_r,
-                                                     int* __restrict__ cnt_c, int* __restrict__ step_counter) {
+                                                     uint64_t* __restrict__ order_out) {
+  const int n_alive = ctl[CTL_ALIVE + par];
+  if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;  // whole block past the alive rays
+  const int n_steps = ctl[CTL_NSTEPS];
+  int* cnt_r = ctl + CTL_R;
+  int* cnt_c = ctl + CTL_C;
+  int* step_counter = ctl + CTL_BINS;
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
   load_coarse(coarse_g, mg.occ.nlvl, coarse);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -553,7 +558,10 @@ template <int P>
 __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_frox, const float* __restrict__ c_dir,
                                                     int64_t n, CacheView cv,
                                                     const typename Mx<P>::T* __restrict__ frags, MlpScales sc,
-                                                    uint2* __restrict__ c_out) {
+                                                    uint2* __restrict__ c_out, const int* __restrict__ n_dev) {
+  // n_dev: sample count written by the marcher (n is then its bound)
+  if (n_dev) n = *n_dev;
+  if ((int64_t)blockIdx.x * 4 * 16 >= n) return;
   __shared__ typename Mx<P>::T wl[N_FRAGS * 64];
   stage_frags(frags, wl);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -637,12 +645,15 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
 // the resample and cache streams of a ray by t0 (resample first iff t0_r <
 // t0_c) and composite; then compact the alive rays.
 __global__ __launch_bounds__(256) void k_cache_composite(
-    int n_alive, fvc_stepsize_info st, float alpha_thre, const RayState* __restrict__ rays_in,
+    int* __restrict__ ctl, int par, fvc_stepsize_info st, float alpha_thre, const RayState* __restrict__ rays_in,
     const int* __restrict__ pix_in, const float4* __restrict__ res_in, const int4* __restrict__ entry,
     const uint64_t* __restrict__ order, const int* __restrict__ alive_flag, const uint2* __restrict__ s_out,
     const float* __restrict__ s_t0, const uint2* __restrict__ c_out, const float* __restrict__ c_t0,
     RayState* __restrict__ rays_out, int* __restrict__ pix_out, float4* __restrict__ res_out,
-    float4* __restrict__ results, int* __restrict__ counter) {
+    float4* __restrict__ results) {
+  const int n_alive = ctl[CTL_ALIVE + par];
+  if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;
+  int* counter = ctl + CTL_ALIVE + (par ^ 1);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   bool keep = false;
@@ -708,47 +719,42 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
 
   Profiler& P = r.prof;
   hipEvent_t ev = P.begin(s);
-  int n_alive = launch_init_rays(r, rm, s);
-  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n_alive * 60.0, 0);
-  int* cnt = r.counters.as<int>();
-  int cur = 0, subiter = 0;
-  int64_t n_res = 0, n_hit = 0;
+  const int n0 = launch_init_rays(r, rm, s);
+  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n0 * 60.0, 0);
+  int* ctl = r.counters.as<int>();
   const int64_t target = std::min<int64_t>(r.target_batch, cap - 64);
-  while (n_alive > 0 && subiter < 1000) {
-    const int n_steps = steps_for(n_alive, target, r.max_steps);
-    FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 3 * sizeof(int), s));
-    const dim3 grid((n_alive + 255) / 256);
-    // loop trips per ray and launch; a performance knob only (a ray resumes
-    // where it stopped), like the reference's step_cutoff
-    // (cache_performance_renderer.cu:72)
-    const int cutoff = 4096;
-    ev = P.begin(s);
-    hipLaunchKernelGGL(k_cache_march, grid, dim3(256), 0, s, n_alive, n_steps, cutoff, mg, m.con,
-                       m.coarse.as<uint32_t>(), cv, r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(),
-                       r.ray_alive.as<int>(), r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(),
-                       r.s_t0.as<float>(), r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(),
-                       r.ray_order.as<uint64_t>(), cnt + CNT_SAMPLES, cnt + CNT_CACHE, cnt + CNT_STEPS);
-    FVC_HIP(hipGetLastError());
-    const int hm = P.end(ev, s, "cache_march");
-    FVC_HIP(hipMemcpyAsync(r.pinned + CNT_SAMPLES, cnt + CNT_SAMPLES, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
-    FVC_HIP(hipStreamSynchronize(s));
-    const int nr = r.pinned[CNT_SAMPLES], nc = r.pinned[CNT_CACHE];
-    const double n_step = (double)(uint32_t)r.pinned[CNT_STEPS];
-    // per step: occupancy byte; per classified sample: index + density probe
-    // (8 B) + 28 B stream record; ray state + entry per ray
-    P.work(hm, n_step, n_step + 36.0 * (nr + nc) + 84.0 * n_alive, 0);
-    n_res += nr;
-    n_hit += nc;
-    ev = P.begin(s);
-    launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), nr, r.s_out.as<uint32_t>(), s);
-    P.end(ev, s, "network_resample", nr, nr * (ALG_GATHER_B + 24.0 + 8.0), nr * (ALG_DENSITY_FLOP + ALG_COLOUR_FLOP));
-    if (nc > 0) {
-      const int64_t groups = ((int64_t)nc + 15) / 16;
-      const int blocks = (int)std::min<int64_t>((groups + 3) / 4, 256 * 16);
+  const int64_t n_bound = std::min<int64_t>(cap - 64, (int64_t)n0 * r.max_steps);  // samples per launch
+  const dim3 grid((n0 + 255) / 256);
+  const int head_blocks = (int)std::min<int64_t>((n_bound + 255) / 256, 256 * 16);
+  // loop trips per ray and launch; a performance knob only (a ray resumes
+  // where it stopped), like the reference's step_cutoff
+  // (cache_performance_renderer.cu:72)
+  const int cutoff = 4096;
+  struct Handles { int march, net, head, comp; };
+  std::vector<Handles> hs;
+  int it = 0;
+  while (n0 > 0 && it < MAX_SUBITER) {
+    const int burst = next_burst(r, it);
+    for (int b = 0; b < burst && it < MAX_SUBITER; b++, it++) {
+      const int cur = it & 1;
+      hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap - 64, 0);
+      Handles h;
       ev = P.begin(s);
-#define L(P)                                                                                                  \
-  hipLaunchKernelGGL((k_cache_head<P>), dim3(blocks), dim3(256), 0, s, r.c_frox.as<float>(), r.c_dir.as<float>(), \
-                     (int64_t)nc, cv, m.frags.as<typename Mx<P>::T>(), m.mlp_scales, r.c_out.as<uint2>())
+      hipLaunchKernelGGL(k_cache_march, grid, dim3(256), 0, s, ctl, cur, cutoff, mg, m.con, m.coarse.as<uint32_t>(), cv,
+                         r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
+                         r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
+                         r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(), r.ray_order.as<uint64_t>());
+      FVC_HIP(hipGetLastError());
+      h.march = P.end(ev, s, "cache_march");
+      ev = P.begin(s);
+      launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), n_bound, r.s_out.as<uint32_t>(), s,
+                          ctl + CTL_R);
+      h.net = P.end(ev, s, "network_resample");
+      ev = P.begin(s);
+#define L(P)                                                                                                       \
+  hipLaunchKernelGGL((k_cache_head<P>), dim3(head_blocks), dim3(256), 0, s, r.c_frox.as<float>(), r.c_dir.as<float>(), \
+                     (int64_t)n_bound, cv, m.frags.as<typename Mx<P>::T>(), m.mlp_scales, r.c_out.as<uint2>(),      \
+                     ctl + CTL_C)
       switch (m.net.precision) {
         case FVC_PREC_BF16: L(FVC_PREC_BF16); break;
         case FVC_PREC_FP8: L(FVC_PREC_FP8); break;
@@ -756,27 +762,41 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
       }
 #undef L
       FVC_HIP(hipGetLastError());
-      P.end(ev, s, "cache_head", nc, nc * (ALG_LOOKUP_B + 24.0 + 8.0), nc * ALG_COLOUR_FLOP);
+      h.head = P.end(ev, s, "cache_head");
+      ev = P.begin(s);
+      hipLaunchKernelGGL(k_cache_composite, grid, dim3(256), 0, s, ctl, cur, rm.stepsize_info, m.scene.alpha_thre,
+                         r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
+                         r.ray_entry.as<int4>(), r.ray_order.as<uint64_t>(), r.ray_alive.as<int>(),
+                         r.s_out.as<uint2>(), r.s_t0.as<float>(), r.c_out.as<uint2>(), r.c_t0.as<float>(),
+                         r.rays[cur ^ 1].as<RayState>(), r.ray_pix[cur ^ 1].as<int>(),
+                         r.ray_res[cur ^ 1].as<float4>(), r.results.as<float4>());
+      FVC_HIP(hipGetLastError());
+      h.comp = P.end(ev, s, "cache_composite");
+      hs.push_back(h);
     }
-    FVC_HIP(hipMemsetAsync(cnt + CNT_ALIVE, 0, sizeof(int), s));
-    ev = P.begin(s);
-    hipLaunchKernelGGL(k_cache_composite, grid, dim3(256), 0, s, n_alive, rm.stepsize_info, m.scene.alpha_thre,
-                       r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
-                       r.ray_entry.as<int4>(), r.ray_order.as<uint64_t>(), r.ray_alive.as<int>(),
-                       r.s_out.as<uint2>(), r.s_t0.as<float>(), r.c_out.as<uint2>(), r.c_t0.as<float>(),
-                       r.rays[cur ^ 1].as<RayState>(),
-                       r.ray_pix[cur ^ 1].as<int>(), r.ray_res[cur ^ 1].as<float4>(), r.results.as<float4>(),
-                       cnt + CNT_ALIVE);
-    FVC_HIP(hipGetLastError());
-    P.end(ev, s, "cache_composite", nr + nc, 12.0 * (nr + nc) + 132.0 * n_alive, 0);
-    n_alive = read_counter(r, CNT_ALIVE, s);
-    cur ^= 1;
-    subiter++;
+    if (read_counter(r, CTL_ALIVE + (it & 1), s) == 0) break;
   }
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap - 64, 1);
   ev = P.begin(s);
   launch_write_image(r, rm, rgba, s);
   P.end(ev, s, "write_image", (double)W * H, (double)W * H * 24, 0);
-  FVC_HIP(hipStreamSynchronize(s));
+  const std::vector<IterLog> lg = read_iter_log(r, it, s);
+  int64_t n_res = 0, n_hit = 0;
+  int n_real = 0;
+  for (int i = 0; i < it; i++) {
+    const IterLog& l = lg[i];
+    if (l.alive > 0) n_real = i + 1;
+    const double bins = (double)(uint32_t)l.bins;
+    // per bin: occupancy byte; per classified sample: index + density probe
+    // (8 B) + 28 B stream record; ray state + entry per ray
+    P.work(hs[i].march, bins, bins + 36.0 * (l.n_r + l.n_c) + 84.0 * l.alive, 0);
+    P.work(hs[i].net, l.n_r, l.n_r * (ALG_GATHER_B + 24.0 + 8.0), l.n_r * (ALG_DENSITY_FLOP + ALG_COLOUR_FLOP));
+    P.work(hs[i].head, l.n_c, l.n_c * (ALG_LOOKUP_B + 24.0 + 8.0), l.n_c * ALG_COLOUR_FLOP);
+    P.work(hs[i].comp, l.n_r + l.n_c, 12.0 * (l.n_r + l.n_c) + 132.0 * l.alive, 0);
+    n_res += l.n_r;
+    n_hit += l.n_c;
+  }
+  r.last_iters = n_real;
   P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
@@ -786,7 +806,7 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
     stats->cache_samples_evaluated = (float)n_hit / np;
     stats->n_resamples = (uint64_t)n_res;
     stats->n_cache_hits = (uint64_t)n_hit;
-    stats->n_subiters = subiter;
+    stats->n_subiters = n_real;
   }
 }
 

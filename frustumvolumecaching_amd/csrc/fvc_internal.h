@@ -115,8 +115,9 @@ void nerfacc_read_data(const std::string& dir, const NerfaccExport& ex, uint16_t
                        uint8_t* occ);
 
 // network launches (network.hip)
+// n_dev (optional): the sample count in device memory, n its upper bound
 void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
-                         hipStream_t s);
+                         hipStream_t s, const int* n_dev = nullptr);
 void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s);
 void launch_network_head(const Model& m, const uint16_t* head_in16, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s);
@@ -252,6 +253,7 @@ struct Renderer {
   // MAX_N_STEPS_PER_SUBITER, performance_renderer.h:54-58)
   int64_t target_batch = 1 << 25;
   int max_steps = 32;
+  int last_iters = 0;    // sub-iterations of the previous frame (first burst length)
   // step table of the marchers (render.hip: step_table)
   DevBuf tab_t, tab_off;
   fvc_stepsize_info tab_key{};

@@ -155,7 +155,10 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
                                                        int64_t n, const uint32_t* __restrict__ table,
                                                        const typename Mx<P>::T* __restrict__ frags, GridParams gp,
                                                        MlpScales sc, uint2* __restrict__ out_h4,
-                                                       uint2* __restrict__ out16) {
+                                                       uint2* __restrict__ out16, const int* __restrict__ n_dev) {
+  // n_dev: sample count written by an earlier kernel (n is then its bound)
+  if (n_dev) n = *n_dev;
+  if ((int64_t)blockIdx.x * (NET_BLOCK / 64) * 16 >= n) return;
   __shared__ GridLevel lv[MAX_LEVELS];
   __shared__ typename Mx<P>::T wl[N_FRAGS * 64];
   load_levels(gp, lv);
@@ -250,12 +253,12 @@ static int grid_for(int64_t n_groups) {
   }
 
 void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
-                         hipStream_t s) {
+                         hipStream_t s, const int* n_dev) {
   if (n <= 0) return;
 #define L(P)                                                                                                 \
   hipLaunchKernelGGL((k_network<true, P>), dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, pos, dir, n, \
                      m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, m.mlp_scales, (uint2*)out_h4, \
-                     (uint2*)nullptr)
+                     (uint2*)nullptr, n_dev)
   FVC_BY_PREC(m, L)
 #undef L
   FVC_HIP(hipGetLastError());
@@ -265,7 +268,7 @@ void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_
 #define L(P)                                                                                                    \
   hipLaunchKernelGGL((k_network<false, P>), dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, pos,           \
                      (const float*)nullptr, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid,  \
-                     m.mlp_scales, (uint2*)nullptr, (uint2*)out16)
+                     m.mlp_scales, (uint2*)nullptr, (uint2*)out16, (const int*)nullptr)
   FVC_BY_PREC(m, L)
 #undef L
   FVC_HIP(hipGetLastError());

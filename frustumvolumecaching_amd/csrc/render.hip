@@ -43,12 +43,16 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, V3 aabb
 // unit_to_01(dir) and t0 of every occupied sample into the compacted stream.
 // The t0 of a ray's k-th sample of the launch goes to t_scr[k * n_alive + i]
 // (coalesced across the wave when read back for the stream).
-__global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, MarchGrid mg, ContractParams con,
+__global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, MarchGrid mg, ContractParams con,
                                                const uint32_t* __restrict__ coarse_g, RayState* __restrict__ rays,
                                                int4* __restrict__ entry, int* __restrict__ alive_flag,
                                                float* __restrict__ t_scr, float* __restrict__ s_pos,
-                                               float* __restrict__ s_dir, float* __restrict__ s_t0,
-                                               int* __restrict__ s_counter, int* __restrict__ step_counter) {
+                                               float* __restrict__ s_dir, float* __restrict__ s_t0) {
+  const int n_alive = ctl[CTL_ALIVE + par];
+  if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;  // whole block past the alive rays
+  const int n_steps = ctl[CTL_NSTEPS];
+  int* s_counter = ctl + CTL_R;
+  int* step_counter = ctl + CTL_BINS;
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
   load_coarse(coarse_g, mg.occ.nlvl, coarse);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -99,13 +103,16 @@ __global__ __launch_bounds__(256) void k_march(int n_alive, int n_steps, MarchGr
 // compaction of the next sub-iteration (renderer.cu:158-189).  The samples
 // of a ray are read in batches of COMPOSITE_BATCH independent loads ahead of
 // the sequential compositing.
-__global__ __launch_bounds__(256) void k_composite(int n_alive, fvc_stepsize_info st, float alpha_thre,
-                                                   const RayState* __restrict__ rays_in, const int* __restrict__ pix_in,
-                                                   const float4* __restrict__ res_in, const int4* __restrict__ entry,
-                                                   const int* __restrict__ alive_flag, const uint2* __restrict__ s_out,
-                                                   const float* __restrict__ s_t0, RayState* __restrict__ rays_out,
-                                                   int* __restrict__ pix_out, float4* __restrict__ res_out,
-                                                   float4* __restrict__ results, int* __restrict__ counter) {
+__global__ __launch_bounds__(256) void k_composite(int* __restrict__ ctl, int par, fvc_stepsize_info st,
+                                                   float alpha_thre, const RayState* __restrict__ rays_in,
+                                                   const int* __restrict__ pix_in, const float4* __restrict__ res_in,
+                                                   const int4* __restrict__ entry, const int* __restrict__ alive_flag,
+                                                   const uint2* __restrict__ s_out, const float* __restrict__ s_t0,
+                                                   RayState* __restrict__ rays_out, int* __restrict__ pix_out,
+                                                   float4* __restrict__ res_out, float4* __restrict__ results) {
+  const int n_alive = ctl[CTL_ALIVE + par];
+  if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;
+  int* counter = ctl + CTL_ALIVE + (par ^ 1);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   bool keep = false;
@@ -183,7 +190,6 @@ __global__ void k_debug_march(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, Ma
 }
 
 // ---------------------------------------------------------------- host
-static const int MAX_SUBITER = 1000;       // PerformanceRenderer::MAX_SUBITER
 
 void resize_renderbuffers(Renderer& r, int w, int h) {
   if (w <= 0 || h <= 0) throw Error(FVC_E_INVALID, "resolution must be positive");
@@ -201,7 +207,7 @@ void resize_renderbuffers(Renderer& r, int w, int h) {
   r.ray_order.ensure(n * sizeof(uint64_t));
   r.t_scratch.ensure(n * 2 * MAX_STEPS_PER_LAUNCH * sizeof(float));
   r.ray_alive.ensure(n * sizeof(int));
-  r.counters.ensure(64 * sizeof(int));
+  r.counters.ensure(CTL_WORDS * sizeof(int));
   r.s_pos.ensure(cap * 3 * sizeof(float));
   r.s_dir.ensure(cap * 3 * sizeof(float));
   r.s_t0.ensure(cap * sizeof(float));
@@ -212,6 +218,38 @@ void resize_renderbuffers(Renderer& r, int w, int h) {
   if (!r.pinned) FVC_HIP(hipHostMalloc((void**)&r.pinned, 64 * sizeof(int), hipHostMallocDefault));
 }
 
+__global__ void k_plan(int* __restrict__ ctl, int it, int64_t target, int max_steps, int64_t cap, int log_only) {
+  if (threadIdx.x != 0) return;
+  if (it > 0 && it - 1 <= MAX_SUBITER) {
+    int* lg = ctl + CTL_LOG + CTL_LOG_FIELDS * (it - 1);
+    lg[0] = ctl[CTL_ALIVE + ((it - 1) & 1)];
+    lg[1] = ctl[CTL_NSTEPS];
+    lg[2] = ctl[CTL_R];
+    lg[3] = ctl[CTL_C];
+    lg[4] = ctl[CTL_BINS];
+  }
+  if (log_only) return;
+  const int64_t n = ctl[CTL_ALIVE + (it & 1)];
+  int64_t k = target / (n > 1 ? n : 1);
+  k = k < 1 ? 1 : (k > max_steps ? max_steps : k);
+  if (n > 0 && n * k > cap) k = cap / n > 1 ? cap / n : 1;
+  ctl[CTL_NSTEPS] = (int)k;
+  ctl[CTL_R] = 0;
+  ctl[CTL_C] = 0;
+  ctl[CTL_BINS] = 0;
+  ctl[CTL_ALIVE + ((it + 1) & 1)] = 0;
+}
+
+std::vector<IterLog> read_iter_log(Renderer& r, int n_it, hipStream_t s) {
+  std::vector<IterLog> out((size_t)n_it);
+  if (n_it > 0) {
+    FVC_HIP(hipMemcpyAsync(out.data(), r.counters.as<int>() + CTL_LOG, sizeof(IterLog) * n_it, hipMemcpyDeviceToHost,
+                           s));
+    FVC_HIP(hipStreamSynchronize(s));
+  }
+  return out;
+}
+
 int read_counter(Renderer& r, int idx, hipStream_t s) {
   FVC_HIP(hipMemcpyAsync(r.pinned + idx, r.counters.as<int>() + idx, sizeof(int), hipMemcpyDeviceToHost, s));
   FVC_HIP(hipStreamSynchronize(s));
@@ -219,7 +257,7 @@ int read_counter(Renderer& r, int idx, hipStream_t s) {
 }
 
 void zero_counters(Renderer& r, hipStream_t s) {
-  FVC_HIP(hipMemsetAsync(r.counters.p, 0, 64 * sizeof(int), s));
+  FVC_HIP(hipMemsetAsync(r.counters.p, 0, CTL_LOG * sizeof(int), s));
 }
 
 int launch_init_rays(Renderer& r, const fvc_raymarch_info& rm, hipStream_t s) {
@@ -228,9 +266,9 @@ int launch_init_rays(Renderer& r, const fvc_raymarch_info& rm, hipStream_t s) {
   zero_counters(r, s);
   hipLaunchKernelGGL(k_init_rays, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, cvt(m.scene.aabb_from),
                      cvt(m.scene.aabb_to), r.rays[0].as<RayState>(), r.ray_pix[0].as<int>(), r.ray_res[0].as<float4>(),
-                     r.results.as<float4>(), r.counters.as<int>() + CNT_ALIVE);
+                     r.results.as<float4>(), r.counters.as<int>() + CTL_ALIVE);
   FVC_HIP(hipGetLastError());
-  return read_counter(r, CNT_ALIVE, s);
+  return read_counter(r, CTL_ALIVE, s);
 }
 
 void launch_write_image(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, hipStream_t s) {
@@ -298,6 +336,11 @@ int steps_for(int n_alive, int64_t target, int max_steps) {
   return (int)k;
 }
 
+// Sub-iterations are launched in bursts with their counts kept on the
+// device (k_plan); the host reads the alive count once per burst.  The
+// first burst is as long as the previous frame's loop.
+int next_burst(const Renderer& r, int it) { return it == 0 ? std::max(2, r.last_iters) : 2; }
+
 void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
   const Model& m = *r.model;
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
@@ -306,58 +349,69 @@ void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_re
   const MarchGrid mg = march_grid(r, rm);
   Profiler& P = r.prof;
   hipEvent_t ev = P.begin(s);
-  int n_alive = launch_init_rays(r, rm, s);
-  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n_alive * 60.0, 0);
-  int* cnt = r.counters.as<int>();
-  int cur = 0, subiter = 0;
-  int64_t n_eval = 0;
-  const int64_t target = std::min<int64_t>(r.target_batch, r.sample_capacity - 64);
-  while (n_alive > 0 && subiter < MAX_SUBITER) {
-    const int n_steps = steps_for(n_alive, target, r.max_steps);
-    FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, 3 * sizeof(int), s));
-    const dim3 grid((n_alive + 255) / 256);
-    ev = P.begin(s);
-    hipLaunchKernelGGL(k_march, grid, dim3(256), 0, s, n_alive, n_steps, mg, m.con, m.coarse.as<uint32_t>(),
-                       r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
-                       r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
-                       cnt + CNT_SAMPLES, cnt + CNT_STEPS);
-    FVC_HIP(hipGetLastError());
-    const int hm = P.end(ev, s, "march");
-    FVC_HIP(hipMemcpyAsync(r.pinned + CNT_SAMPLES, cnt + CNT_SAMPLES, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
-    FVC_HIP(hipStreamSynchronize(s));
-    const int n_samples = r.pinned[CNT_SAMPLES];
-    const double n_step = (double)(uint32_t)r.pinned[CNT_STEPS];
-    // 1 B occupancy per step, 28 B per emitted sample, ray state + entry per ray
-    P.work(hm, n_step, n_step + 28.0 * n_samples + 84.0 * n_alive, 0);
-    n_eval += n_samples;
-    ev = P.begin(s);
-    launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), n_samples, r.s_out.as<uint32_t>(), s);
-    P.end(ev, s, "network_full", n_samples, n_samples * (ALG_GATHER_B + 24.0 + 8.0),
-          n_samples * (ALG_DENSITY_FLOP + ALG_COLOUR_FLOP));
-    FVC_HIP(hipMemsetAsync(cnt + CNT_ALIVE, 0, sizeof(int), s));
-    ev = P.begin(s);
-    hipLaunchKernelGGL(k_composite, grid, dim3(256), 0, s, n_alive, rm.stepsize_info, m.scene.alpha_thre,
-                       r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
-                       r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.s_out.as<uint2>(), r.s_t0.as<float>(),
-                       r.rays[cur ^ 1].as<RayState>(), r.ray_pix[cur ^ 1].as<int>(), r.ray_res[cur ^ 1].as<float4>(),
-                       r.results.as<float4>(), cnt + CNT_ALIVE);
-    FVC_HIP(hipGetLastError());
-    P.end(ev, s, "composite", n_samples, 12.0 * n_samples + 124.0 * n_alive, 0);
-    n_alive = read_counter(r, CNT_ALIVE, s);
-    cur ^= 1;
-    subiter++;
+  const int n0 = launch_init_rays(r, rm, s);
+  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n0 * 60.0, 0);
+  int* ctl = r.counters.as<int>();
+  const int64_t cap = r.sample_capacity - 64;
+  const int64_t target = std::min<int64_t>(r.target_batch, cap);
+  const int64_t n_bound = std::min<int64_t>(cap, (int64_t)n0 * r.max_steps);  // samples per launch
+  const dim3 grid((n0 + 255) / 256);
+  struct Handles { int march, net, comp; };
+  std::vector<Handles> hs;
+  int it = 0;
+  while (n0 > 0 && it < MAX_SUBITER) {
+    const int burst = next_burst(r, it);
+    for (int b = 0; b < burst && it < MAX_SUBITER; b++, it++) {
+      const int cur = it & 1;
+      hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap, 0);
+      Handles h;
+      ev = P.begin(s);
+      hipLaunchKernelGGL(k_march, grid, dim3(256), 0, s, ctl, cur, mg, m.con, m.coarse.as<uint32_t>(),
+                         r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
+                         r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>());
+      FVC_HIP(hipGetLastError());
+      h.march = P.end(ev, s, "march");
+      ev = P.begin(s);
+      launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), n_bound, r.s_out.as<uint32_t>(), s,
+                          ctl + CTL_R);
+      h.net = P.end(ev, s, "network_full");
+      ev = P.begin(s);
+      hipLaunchKernelGGL(k_composite, grid, dim3(256), 0, s, ctl, cur, rm.stepsize_info, m.scene.alpha_thre,
+                         r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
+                         r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.s_out.as<uint2>(), r.s_t0.as<float>(),
+                         r.rays[cur ^ 1].as<RayState>(), r.ray_pix[cur ^ 1].as<int>(),
+                         r.ray_res[cur ^ 1].as<float4>(), r.results.as<float4>());
+      FVC_HIP(hipGetLastError());
+      h.comp = P.end(ev, s, "composite");
+      hs.push_back(h);
+    }
+    if (read_counter(r, CTL_ALIVE + (it & 1), s) == 0) break;
   }
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap, 1);
   ev = P.begin(s);
   launch_write_image(r, rm, rgba, s);
   P.end(ev, s, "write_image", (double)W * H, (double)W * H * 24, 0);
-  FVC_HIP(hipStreamSynchronize(s));
+  const std::vector<IterLog> lg = read_iter_log(r, it, s);
+  int64_t n_eval = 0;
+  int n_real = 0;
+  for (int i = 0; i < it; i++) {
+    const IterLog& l = lg[i];
+    if (l.alive > 0) n_real = i + 1;
+    const double bins = (double)(uint32_t)l.bins;
+    // 1 B occupancy per bin, 28 B per emitted sample, ray state + entry per ray
+    P.work(hs[i].march, bins, bins + 28.0 * l.n_r + 84.0 * l.alive, 0);
+    P.work(hs[i].net, l.n_r, l.n_r * (ALG_GATHER_B + 24.0 + 8.0), l.n_r * (ALG_DENSITY_FLOP + ALG_COLOUR_FLOP));
+    P.work(hs[i].comp, l.n_r, 12.0 * l.n_r + 124.0 * l.alive, 0);
+    n_eval += l.n_r;
+  }
+  r.last_iters = n_real;
   P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->samples_ppx = (float)n_eval / (float)((int64_t)W * H);
     stats->resample_samples_evaluated = stats->samples_ppx;
     stats->n_resamples = (uint64_t)n_eval;
-    stats->n_subiters = subiter;
+    stats->n_subiters = n_real;
   }
 }
 

@@ -56,6 +56,21 @@ static const int COMPOSITE_BATCH = 8;
 // slots of Renderer::counters
 enum { CNT_ALIVE = 0, CNT_SAMPLES = 1, CNT_CACHE = 2, CNT_STEPS = 3, CNT_TILES = 4, CNT_SLOTS = 5, CNT_OVERFLOW = 6 };
 
+// Device control block of a frame's sub-iteration loop (also in
+// Renderer::counters): alive counts (ping-pong by iteration parity), the
+// samples per ray of the current iteration, the stream and bin counters,
+// and a per-iteration log {alive, n_steps, resample, cache, bins}.  The host
+// launches iterations in bursts and reads the alive count once per burst.
+enum { CTL_ALIVE = 16, CTL_NSTEPS = 18, CTL_R = 19, CTL_C = 20, CTL_BINS = 21, CTL_LOG = 32 };
+static const int CTL_LOG_FIELDS = 5;
+static const int MAX_SUBITER = 1000;        // PerformanceRenderer::MAX_SUBITER
+static const int CTL_WORDS = CTL_LOG + CTL_LOG_FIELDS * (MAX_SUBITER + 1);
+
+// Plan iteration it: log iteration it-1, set the samples per ray
+// (TARGET_BATCH_SIZE / n_alive clamped to [1, max_steps] and to the stream
+// capacity, performance_renderer.cu:249-250) and zero the counters.
+__global__ void k_plan(int* __restrict__ ctl, int it, int64_t target, int max_steps, int64_t cap, int log_only);
+
 __device__ __forceinline__ float h2f(uint32_t bits16) { return (float)__builtin_bit_cast(_Float16, (uint16_t)bits16); }
 
 // One step of front-to-back compositing (accumulate_performance_kernel,
@@ -90,7 +105,11 @@ MarchGrid march_grid(Renderer& r, const fvc_raymarch_info& rm);
 int read_counter(Renderer& r, int idx, hipStream_t s);
 void zero_counters(Renderer& r, hipStream_t s);
 int launch_init_rays(Renderer& r, const fvc_raymarch_info& rm, hipStream_t s);
+// per-iteration log entries {alive, n_steps, resample, cache, bins} of a frame
+struct IterLog { int alive, n_steps, n_r, n_c, bins; };
+std::vector<IterLog> read_iter_log(Renderer& r, int n_it, hipStream_t s);
 void launch_write_image(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, hipStream_t s);
 int steps_for(int n_alive, int64_t target, int max_steps);
+int next_burst(const Renderer& r, int it);
 
 }  // namespace fvc
