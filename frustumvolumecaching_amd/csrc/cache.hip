@@ -446,8 +446,8 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
                                                      int* __restrict__ alive_flag, float* __restrict__ t_scr,
                                                      float* __restrict__ s_pos, float* __restrict__ s_dir,
                                                      float* __restrict__ s_t0, float* __restrict__ c_frox,
-                                                     float* __restrict__ c_dir, float* __restrict__ c_t0,
-                                                     uint64_t* __restrict__ order_out) {
+                                                     int* __restrict__ c_ray, float* __restrict__ c_t0,
+                                                     uint4* __restrict__ ray_sh, uint64_t* __restrict__ order_out) {
   const int n_alive = ctl[CTL_ALIVE + par];
   if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;  // whole block past the alive rays
   const int n_steps = ctl[CTL_NSTEPS];
@@ -534,12 +534,19 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
         const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
         const int64_t q = (int64_t)off_c + kc;
         c_frox[3 * q] = f.x; c_frox[3 * q + 1] = f.y; c_frox[3 * q + 2] = f.z;
-        c_dir[3 * q] = d01.x; c_dir[3 * q + 1] = d01.y; c_dir[3 * q + 2] = d01.z;
+        c_ray[q] = i;
         c_t0[q] = t0;
         order |= 1ull << (kr + kc);
       }
       kr += is_r ? 1 : 0;
       kc += is_c ? 1 : 0;
+    }
+    if (nc > 0) {
+      // the colour MLP's SH input depends on the ray only
+      uint4 lo, hi;
+      sh4_all(d01.x, d01.y, d01.z, lo, hi);
+      ray_sh[2 * i] = lo;
+      ray_sh[2 * i + 1] = hi;
     }
     rays[i].t = r.t;
     rays[i].j = r.j;
@@ -551,87 +558,90 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
 
 // Trilinear lookup (generateNetworkInput_cache_shfl_perf_kernel_cg,
 // cache_performance_renderer.cu:678-787) fused with the colour MLP.  Lane
-// (sample l%16, group g) fetches corners 2g and 2g+1 (corner bits x=4,y=2,z=1);
-// the two xor-shuffles over the 4 lanes of a sample reproduce the reference's
-// ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) summation order.
+// (sample l%16, group g) fetches the corners (x, y) = (g >> 1, g & 1), z = 0
+// and 1 (corners 2g and 2g+1 with corner bits x=4, y=2, z=1).  Corners
+// outside the frustum or in unset bricks read a zero voxel.  The latent sums
+// are reduce-scattered over the 4 lanes of a sample (row swaps over lane^16,
+// then lane^32), which reproduces the reference's
+// ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) summation order and leaves lane g with
+// latents (g&1) + 4(g>>1) and that + 2, the inputs it feeds to the colour
+// MLP (fragments FRAG_H1C).
+__device__ Voxel k_zero_voxel;  // zero-initialised, never written
+
 template <int P>
-__global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_frox, const float* __restrict__ c_dir,
-                                                    int64_t n, CacheView cv,
+__global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_frox, const int* __restrict__ c_ray,
+                                                    const uint2* __restrict__ ray_sh, int64_t n, CacheView cv,
                                                     const typename Mx<P>::T* __restrict__ frags, MlpScales sc,
                                                     uint2* __restrict__ c_out, const int* __restrict__ n_dev) {
   // n_dev: sample count written by the marcher (n is then its bound)
   if (n_dev) n = *n_dev;
   if ((int64_t)blockIdx.x * 4 * 16 >= n) return;
-  __shared__ typename Mx<P>::T wl[N_FRAGS * 64];
-  stage_frags(frags, wl);
+  typedef typename Mx<P>::T T;
+  __shared__ T wl[6 * 64];  // FRAG_H1C (4), FRAG_H2 (2)
+  stage_frags(frags + FRAG_H1C * 64, wl, 4);
+  stage_frags(frags + FRAG_H2 * 64, wl + 4 * 64, 2);
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
-  const int64_t n_groups = (n + 15) / 16;
-  for (int64_t grp = (int64_t)blockIdx.x * 4 + wave; grp < n_groups; grp += (int64_t)gridDim.x * 4) {
+  const int n32 = (int)n;  // the stream holds fewer than 2^31 / 3 samples
+  const int n_groups = (n32 + 15) / 16;
+  const float ox = (float)(g >> 1), oy = (float)(g & 1);
+  const float fx = (float)cv.fx, fy = (float)cv.fy, fz = (float)cv.fz;
+  for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
     const int li = opaque(lane);
-    const int64_t s = grp * 16 + col;
-    const bool valid = s < n;
-    const int64_t sl = valid ? s : n - 1;
+    const int s = grp * 16 + col;
+    const bool valid = s < n32;
+    const int sl = valid ? s : n32 - 1;
     const float fcx = c_frox[3 * sl] - 0.5f, fcy = c_frox[3 * sl + 1] - 0.5f, fcz = c_frox[3 * sl + 2] - 0.5f;
+    const uint2 sh = ray_sh[c_ray[sl] * 4 + g];
     const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
     const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
+    const float px = ix + ox, py = iy + oy;
+    const float wx = (1.0f - ox) * (1.0f - 2.0f * lx) + lx;
+    const float wy = (1.0f - oy) * (1.0f - 2.0f * ly) + ly;
+    const float wxy = wx * wy;
+    const bool in_xy = px >= 0.0f && px < fx && py >= 0.0f && py < fy;
+    const int vx = in_xy ? (int)px : 0, vy = in_xy ? (int)py : 0;
+    const int col_xy = (vy >> 3) * cv.bx + (vx >> 3), loc_xy = (vy & 7) * 8 + (vx & 7);
     float wd = 0.0f, dd = 0.0f, ld[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) ld[i] = 0.0f;
-#pragma unroll
     for (int cc = 0; cc < 2; cc++) {
-      const int c = 2 * g + cc;
-      const float ox = (float)((c >> 2) & 1), oy = (float)((c >> 1) & 1), oz = (float)(c & 1);
-      const float px = ix + ox, py = iy + oy, pz = iz + oz;
-      const float wx = (1.0f - ox) * (1.0f - 2.0f * lx) + lx;
-      const float wy = (1.0f - oy) * (1.0f - 2.0f * ly) + ly;
-      const float wz = (1.0f - oz) * (1.0f - 2.0f * lz) + lz;
-      const float w = wx * wy * wz;
-      float cw = 0.0f, cd = 0.0f, cl[8];
-#pragma unroll
-      for (int i = 0; i < 8; i++) cl[i] = 0.0f;
-      const bool inside = px >= 0.0f && px < (float)cv.fx && py >= 0.0f && py < (float)cv.fy && pz >= 0.0f &&
-                          pz < (float)cv.fz;
-      if (inside) {
-        const int vx = (int)px, vy = (int)py, vz = (int)pz;
-        const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
-        if (slot >= 0) {
-          const Voxel v = cv.pool[(int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))];
-          cw = w * v.isset;
-          cd = w * v.density;
-#pragma unroll
-          for (int i = 0; i < 8; i++) cl[i] = w * h2f(v.lat[i]);
-        }
-      }
+      const float pz = iz + (float)cc;
+      const float wz = cc == 0 ? (1.0f - 2.0f * lz) + lz : lz;
+      const float w = wxy * wz;
+      const bool inside = in_xy && pz >= 0.0f && pz < fz;
+      const int vz = inside ? (int)pz : 0;
+      const int slot = cv.table[(vz >> 3) * cv.by * cv.bx + col_xy];
+      const Voxel* vp = (inside && slot >= 0)
+                            ? cv.pool + ((size_t)slot * BRICK_VOXELS + (uint32_t)((vz & 7) * 64 + loc_xy))
+                            : &k_zero_voxel;
+      const uint4 lat = *reinterpret_cast<const uint4*>(vp->lat);
+      const float2 di = *reinterpret_cast<const float2*>(&vp->density);
+      const uint32_t lw[4] = {lat.x, lat.y, lat.z, lat.w};
+      const float cw = w * di.y, cd = w * di.x;
       if (cc == 0) {
-        wd = cw; dd = cd;
+        wd = cw;
+        dd = cd;
 #pragma unroll
-        for (int i = 0; i < 8; i++) ld[i] = cl[i];
+        for (int k = 0; k < 8; k++) ld[k] = w * h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
       } else {
-        wd = wd + cw; dd = dd + cd;
+        wd = wd + cw;
+        dd = dd + cd;
 #pragma unroll
-        for (int i = 0; i < 8; i++) ld[i] = ld[i] + cl[i];
+        for (int k = 0; k < 8; k++) ld[k] = ld[k] + w * h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
       }
     }
     wd = xor32_sum(xor16_sum(wd));
     dd = xor32_sum(xor16_sum(dd));
-#pragma unroll
-    for (int i = 0; i < 8; i++) ld[i] = xor32_sum(xor16_sum(ld[i]));
-    const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
-    // rows 4g..4g+3 of the base output: rows 1..8 are the latents
-    // sum w*l / sum w*isset, the others the 1.0 padding
-    _Float16 rows[4];
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int li = 4 * g + e - 1;
-      float num = ld[0];
-#pragma unroll
-      for (int i = 1; i < 8; i++) num = li == i ? ld[i] : num;
-      rows[e] = (li >= 0 && li < 8) ? (_Float16)(num / wd) : (_Float16)1.0f;
-    }
-    const typename Mx<P>::T x = head_input<P>(rows, g, c_dir[3 * sl], c_dir[3 * sl + 1], c_dir[3 * sl + 2], sc.inv_act);
-    const f4 o = head_mlp<P>(wl, li, x, sc);
+    // reduce-scatter: lane^16 pairs (0,1) (2,3) (4,5) (6,7), then lane^32
+    const float m0 = swap16_sum(ld[0], ld[1]), m1 = swap16_sum(ld[2], ld[3]);
+    const float m2 = swap16_sum(ld[4], ld[5]), m3 = swap16_sum(ld[6], ld[7]);
+    const float n0 = swap32_sum(m0, m2), n1 = swap32_sum(m1, m3);
+    // latents sum w*l / sum w*isset (re-weighted)
+    const T x = head_input_cache<P>((_Float16)(n0 / wd), (_Float16)(n1 / wd), sh, sc.inv_act);
+    const f4 o = head_mlp<P>(wl, wl + 4 * 64, li, x, sc);
     if (g == 0 && valid) {
+      const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
       uint2 v;
       v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
             ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[1]) << 16);
@@ -704,7 +714,8 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
   if (W != r.width || H != r.height) resize_renderbuffers(r, W, H);
   const int64_t cap = r.sample_capacity;
   r.c_frox.ensure(cap * 3 * sizeof(float));
-  r.c_dir.ensure(cap * 3 * sizeof(float));
+  r.c_ray.ensure(cap * sizeof(int));
+  r.ray_sh.ensure((int64_t)W * H * 32);
   r.c_t0.ensure(cap * sizeof(float));
   r.c_out.ensure(cap * sizeof(uint2));
   hipStream_t s = r.stream();
@@ -743,7 +754,8 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
       hipLaunchKernelGGL(k_cache_march, grid, dim3(256), 0, s, ctl, cur, cutoff, mg, m.con, m.coarse.as<uint32_t>(), cv,
                          r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
                          r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
-                         r.c_frox.as<float>(), r.c_dir.as<float>(), r.c_t0.as<float>(), r.ray_order.as<uint64_t>());
+                         r.c_frox.as<float>(), r.c_ray.as<int>(), r.c_t0.as<float>(), r.ray_sh.as<uint4>(),
+                         r.ray_order.as<uint64_t>());
       FVC_HIP(hipGetLastError());
       h.march = P.end(ev, s, "cache_march");
       ev = P.begin(s);
@@ -752,7 +764,8 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
       h.net = P.end(ev, s, "network_resample");
       ev = P.begin(s);
 #define L(P)                                                                                                       \
-  hipLaunchKernelGGL((k_cache_head<P>), dim3(head_blocks), dim3(256), 0, s, r.c_frox.as<float>(), r.c_dir.as<float>(), \
+  hipLaunchKernelGGL((k_cache_head<P>), dim3(head_blocks), dim3(256), 0, s, r.c_frox.as<float>(), r.c_ray.as<int>(), \
+                     r.ray_sh.as<uint2>(),                                                                          \
                      (int64_t)n_bound, cv, m.frags.as<typename Mx<P>::T>(), m.mlp_scales, r.c_out.as<uint2>(),      \
                      ctl + CTL_C)
       switch (m.net.precision) {

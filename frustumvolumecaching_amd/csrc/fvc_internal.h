@@ -73,8 +73,12 @@ struct GridParams {
 
 // Weight fragments: 20 MFMA A-operands of 64 lanes x 8 halves, in the order
 // W1[nb=0..3], W2[nb=0..3][kb=0..1], W3[kb=0..1], H1[nb=0..3], H2[kb=0..1].
-static const int N_FRAGS = 20;
-static const int FRAG_W1 = 0, FRAG_W2 = 4, FRAG_W3 = 12, FRAG_H1 = 14, FRAG_H2 = 18;
+static const int N_FRAGS = 24;
+// FRAG_H1C: the colour MLP's first layer again, in the input order of the
+// cache lookup (head_input_cache); the network kernels stage the first
+// N_NET_FRAGS fragments
+static const int FRAG_W1 = 0, FRAG_W2 = 4, FRAG_W3 = 12, FRAG_H1 = 14, FRAG_H2 = 18, FRAG_H1C = 20;
+static const int N_NET_FRAGS = 20;
 
 // per-layer accumulator scales of the MLPs (FP8: s_w[l] * s_a; otherwise 1)
 // and 1 / s_a (mlp_mfma.h)
@@ -272,7 +276,8 @@ struct Renderer {
   int* pinned = nullptr; // pinned host mirror of the counters
   // sample streams
   DevBuf s_pos, s_dir, s_t0, s_out;            // resample / full stream
-  DevBuf c_frox, c_dir, c_t0, c_out;            // cache-hit stream
+  DevBuf c_frox, c_ray, c_t0, c_out;            // cache-hit stream (c_ray: ray index)
+  DevBuf ray_sh;                                // 16 fp16 SH coefficients per ray and launch
   // fill scratch
   DevBuf f_rays, f_tiles, f_list, f_pos, f_t0, f_meta, f_out16;
   int fill_tiles = 0;

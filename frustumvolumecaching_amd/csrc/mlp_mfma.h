@@ -38,10 +38,9 @@ template <int P> struct Mx;
 template <> struct Mx<FVC_PREC_FP16> {
   typedef h8 T;
   static __device__ __forceinline__ T pack(const float* v, float) {
-    T r;
-#pragma unroll
-    for (int e = 0; e < 8; e++) r[e] = (_Float16)v[e];
-    return r;
+    typedef float f8v __attribute__((ext_vector_type(8)));
+    const f8v w = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+    return __builtin_convertvector(w, T);
   }
   static __device__ __forceinline__ f4 mma(T a, T b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
@@ -50,10 +49,9 @@ template <> struct Mx<FVC_PREC_FP16> {
 template <> struct Mx<FVC_PREC_BF16> {
   typedef b8 T;
   static __device__ __forceinline__ T pack(const float* v, float) {
-    T r;
-#pragma unroll
-    for (int e = 0; e < 8; e++) r[e] = (__bf16)v[e];
-    return r;
+    typedef float f8v __attribute__((ext_vector_type(8)));
+    const f8v w = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+    return __builtin_convertvector(w, T);
   }
   static __device__ __forceinline__ f4 mma(T a, T b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -83,11 +81,28 @@ template <int P> __device__ __forceinline__ f4 rescale(f4 d, float s) {
   return d;
 }
 
+// ReLU as a signed-integer max on the bit pattern: every value with the
+// sign bit set becomes +0, the others pass unchanged (one instruction, no
+// NaN canonicalisation).  On packed 16-bit operands the same max of the
+// halves (or bfloat16s) commutes with the rounding, since rounding keeps the
+// sign and maps 0 to 0.
+__device__ __forceinline__ float relu_bits(float x) {
+  const int b = __builtin_bit_cast(int, x);
+  return __builtin_bit_cast(float, b > 0 ? b : 0);
+}
+typedef short s8v __attribute__((ext_vector_type(8)));
+
 // B fragment from two 16-neuron D blocks: ReLU, then operand rounding
 template <int P> __device__ __forceinline__ typename Mx<P>::T act_pack(f4 lo, f4 hi, float inv_act) {
-  const float v[8] = {fmaxf(lo[0], 0.0f), fmaxf(lo[1], 0.0f), fmaxf(lo[2], 0.0f), fmaxf(lo[3], 0.0f),
-                      fmaxf(hi[0], 0.0f), fmaxf(hi[1], 0.0f), fmaxf(hi[2], 0.0f), fmaxf(hi[3], 0.0f)};
-  return Mx<P>::pack(v, inv_act);
+  if constexpr (P == FVC_PREC_FP8) {
+    const float v[8] = {relu_bits(lo[0]), relu_bits(lo[1]), relu_bits(lo[2]), relu_bits(lo[3]),
+                        relu_bits(hi[0]), relu_bits(hi[1]), relu_bits(hi[2]), relu_bits(hi[3])};
+    return Mx<P>::pack(v, inv_act);
+  } else {
+    const float v[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    const s8v b = __builtin_bit_cast(s8v, Mx<P>::pack(v, inv_act));
+    return __builtin_bit_cast(typename Mx<P>::T, __builtin_elementwise_max(b, (s8v)(short)0));
+  }
 }
 
 // density / latent MLP 32 -> 64 -> 64 -> 16 with the fragments in LDS (w);
@@ -110,21 +125,22 @@ __device__ __forceinline__ f4 base_mlp(const typename Mx<P>::T* w, int li, typen
                     sc.layer[2]);
 }
 
-// colour MLP 32 -> 64 -> 16; returns output rows 4g..4g+3 (rows 0..2 = rgb)
+// colour MLP 32 -> 64 -> 16 with the first-layer fragments at w1 (4) and the
+// output-layer fragments at w2 (2); returns output rows 4g..4g+3 (rows 0..2 =
+// rgb)
 template <int P>
-__device__ __forceinline__ f4 head_mlp(const typename Mx<P>::T* w, int li, typename Mx<P>::T x, const MlpScales& sc) {
+__device__ __forceinline__ f4 head_mlp(const typename Mx<P>::T* w1, const typename Mx<P>::T* w2, int li,
+                                       typename Mx<P>::T x, const MlpScales& sc) {
   f4 d1[4];
 #pragma unroll
-  for (int nb = 0; nb < 4; nb++) d1[nb] = rescale<P>(Mx<P>::mma(w[(FRAG_H1 + nb) * 64 + li], x, f4zero()), sc.layer[3]);
+  for (int nb = 0; nb < 4; nb++) d1[nb] = rescale<P>(Mx<P>::mma(w1[nb * 64 + li], x, f4zero()), sc.layer[3]);
   const typename Mx<P>::T b_0 = act_pack<P>(d1[0], d1[1], sc.inv_act), b_1 = act_pack<P>(d1[2], d1[3], sc.inv_act);
-  return rescale<P>(Mx<P>::mma(w[(FRAG_H2 + 1) * 64 + li], b_1, Mx<P>::mma(w[FRAG_H2 * 64 + li], b_0, f4zero())),
-                    sc.layer[4]);
+  return rescale<P>(Mx<P>::mma(w2[64 + li], b_1, Mx<P>::mma(w2[li], b_0, f4zero())), sc.layer[4]);
 }
 
-// stage all weight fragments of the model into LDS (every thread of the block)
-template <class T> __device__ __forceinline__ void stage_frags(const T* __restrict__ frags, T* lds) {
-  for (int i = threadIdx.x; i < N_FRAGS * 64; i += blockDim.x) lds[i] = frags[i];
-  __syncthreads();
+// stage n weight fragments into LDS (every thread of the block)
+template <class T> __device__ __forceinline__ void stage_frags(const T* __restrict__ frags, T* lds, int n) {
+  for (int i = threadIdx.x; i < n * 64; i += blockDim.x) lds[i] = frags[i];
 }
 __device__ __forceinline__ int opaque(int v) {
   asm volatile("" : "+v"(v));
@@ -161,6 +177,19 @@ __device__ __forceinline__ void sh4_quarter(float dx01, float dy01, float dz01, 
   o[0] = (_Float16)c0; o[1] = (_Float16)c1; o[2] = (_Float16)c2; o[3] = (_Float16)c3;
 }
 
+// all 16 coefficients as 4 dwords of packed halves per 8 coefficients
+__device__ __forceinline__ void sh4_all(float dx01, float dy01, float dz01, uint4& lo, uint4& hi) {
+  _Float16 c[16];
+#pragma unroll
+  for (int g = 0; g < 4; g++) sh4_quarter(dx01, dy01, dz01, g, c + 4 * g);
+  uint32_t w[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    w[k] = (uint32_t)__builtin_bit_cast(uint16_t, c[2 * k]) | ((uint32_t)__builtin_bit_cast(uint16_t, c[2 * k + 1]) << 16);
+  lo = make_uint4(w[0], w[1], w[2], w[3]);
+  hi = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
 // Head input fragment.  Slots e=0..3 carry base-output rows 4g+e: rows 1..8
 // are the latents, rows 0 and 9..15 are replaced by the constant 1.0 padding
 // of the reference's head input (nerf_network.h:248-253); slots e=4..7 carry
@@ -180,6 +209,30 @@ __device__ __forceinline__ typename Mx<P>::T head_input(const _Float16* rows4, i
 #pragma unroll
   for (int e = 0; e < 4; e++) v[4 + e] = (float)sh[e];
   return Mx<P>::pack(v, inv_act);
+}
+
+// Head input fragment of the cache lookup (fragments FRAG_H1C): slots 0, 1
+// carry latents (g & 1) + 4 (g >> 1) and that + 2, slots 2, 3 two of the
+// constant 1.0 padding inputs, slots 4..7 SH coefficients 4g..4g+3 (packed
+// halves in sh).
+template <int P>
+__device__ __forceinline__ typename Mx<P>::T head_input_cache(_Float16 l0, _Float16 l1, uint2 sh, float inv_act) {
+  if constexpr (P == FVC_PREC_FP16) {
+    const uint32_t w0 = (uint32_t)__builtin_bit_cast(uint16_t, l0) | ((uint32_t)__builtin_bit_cast(uint16_t, l1) << 16);
+    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+    const u4v w = {w0, 0x3c003c00u, sh.x, sh.y};
+    return __builtin_bit_cast(h8, w);
+  } else {
+    const float v[8] = {(float)l0,
+                        (float)l1,
+                        1.0f,
+                        1.0f,
+                        (float)__builtin_bit_cast(_Float16, (uint16_t)(sh.x & 0xffffu)),
+                        (float)__builtin_bit_cast(_Float16, (uint16_t)(sh.x >> 16)),
+                        (float)__builtin_bit_cast(_Float16, (uint16_t)(sh.y & 0xffffu)),
+                        (float)__builtin_bit_cast(_Float16, (uint16_t)(sh.y >> 16))};
+    return Mx<P>::pack(v, inv_act);
+  }
 }
 
 }  // namespace fvc

@@ -160,9 +160,10 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
   if (n_dev) n = *n_dev;
   if ((int64_t)blockIdx.x * (NET_BLOCK / 64) * 16 >= n) return;
   __shared__ GridLevel lv[MAX_LEVELS];
-  __shared__ typename Mx<P>::T wl[N_FRAGS * 64];
+  __shared__ typename Mx<P>::T wl[N_NET_FRAGS * 64];
   load_levels(gp, lv);
-  stage_frags(frags, wl);
+  stage_frags(frags, wl, N_NET_FRAGS);
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
   const int64_t n_groups = (n + 15) / 16;
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
       if (valid) out16[s * 4 + g] = pack_h4(rows[0], rows[1], rows[2], rows[3]);
     } else {
       const typename Mx<P>::T hin = head_input<P>(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2], sc.inv_act);
-      const f4 o = head_mlp<P>(wl, li, hin, sc);
+      const f4 o = head_mlp<P>(wl + FRAG_H1 * 64, wl + FRAG_H2 * 64, li, hin, sc);
       if (g == 0 && valid) out_h4[s] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2], rows[0]);
     }
   }
@@ -196,8 +197,9 @@ template <int P>
 __global__ __launch_bounds__(NET_BLOCK) void k_head(const uint16_t* __restrict__ hin16, const float* __restrict__ dir,
                                                     int64_t n, const typename Mx<P>::T* __restrict__ frags,
                                                     MlpScales sc, uint2* __restrict__ out_h4) {
-  __shared__ typename Mx<P>::T wl[N_FRAGS * 64];
-  stage_frags(frags, wl);
+  __shared__ typename Mx<P>::T wl[N_NET_FRAGS * 64];
+  stage_frags(frags, wl, N_NET_FRAGS);
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
   const int64_t n_groups = (n + 15) / 16;
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(NET_BLOCK) void k_head(const uint16_t* __restrict__
 #pragma unroll
     for (int e = 0; e < 4; e++) rows[e] = __builtin_bit_cast(_Float16, hin16[sl * 16 + 4 * g + e]);
     const typename Mx<P>::T x = head_input<P>(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2], sc.inv_act);
-    const f4 o = head_mlp<P>(wl, li, x, sc);
+    const f4 o = head_mlp<P>(wl + FRAG_H1 * 64, wl + FRAG_H2 * 64, li, x, sc);
     if (g == 0 && valid)
       out_h4[s] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2],
                           __builtin_bit_cast(_Float16, hin16[sl * 16]));
@@ -302,6 +304,11 @@ static int col_head_in(int g, int e) {                                          
   if (r >= 1 && r <= 8) return r - 1;      // latent r - 1
   return r == 0 ? 24 : r + 16;             // padding columns 24..31
 }
+static int col_head_cache(int g, int e) {                                                      // H1C
+  if (e >= 4) return 8 + 4 * g + (e - 4);         // SH coefficient 4g + e - 4
+  if (e < 2) return (g & 1) + 4 * (g >> 1) + 2 * e;  // latents of the reduce-scatter
+  return 24 + 2 * g + (e - 2);                    // padding columns 24..31
+}
 
 // host conversions for the weight fragments
 static float f16_value(uint16_t h) {
@@ -383,6 +390,7 @@ void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, s
   for (int kb = 0; kb < 2; kb++) put(FRAG_W3 + kb, 2, W3, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
   for (int nb = 0; nb < 4; nb++) put(FRAG_H1 + nb, 3, H1, 32, nb * 16, col_head_in);
   for (int kb = 0; kb < 2; kb++) put(FRAG_H2 + kb, 4, H2, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
+  for (int nb = 0; nb < 4; nb++) put(FRAG_H1C + nb, 3, H1, 32, nb * 16, col_head_cache);
 }
 
 }  // namespace fvc
