@@ -104,6 +104,11 @@ class InitStats(C.Structure):
                 ("overflow", C.c_int32), ("n_subiters", C.c_int32), ("n_samples_evaluated", C.c_int64)]
 
 
+class DiffStats(C.Structure):
+    """fvc_diff_stats (differentiable render)."""
+    _fields_ = [("n_rays", C.c_int64), ("n_samples", C.c_int64)]
+
+
 class RenderStats(C.Structure):
     """RenderStats (cache_performance_renderer.h:123-131)."""
     _fields_ = [("samples_ppx", C.c_float), ("resample_samples_evaluated", C.c_float),

@@ -237,8 +237,22 @@ static const double ALG_DENSITY_FLOP = 13440, ALG_COLOUR_FLOP = 3456;
 
 // ---------------------------------------------------------------- renderer
 static const int MAX_STEPS_PER_LAUNCH = 32;   // cap of fvc_renderer_set_batching
+// state of the differentiable render between forward and backward (diff.hip)
+struct DiffState {
+  DevBuf frag_index, frags;        // FP32 MFMA fragments of the caller's weights
+  DevBuf rays, counts, offsets, ray_out, n_walk, scan_tmp;
+  DevBuf s_pos, s_dt, s_ray, out4, d_out4;
+  int n = 0;
+  int64_t n_samples = 0;
+  const float* pos = nullptr;
+  const float* base = nullptr;
+  const float* head = nullptr;
+  fvc_raymarch_info rm{};
+};
+
 struct Renderer {
   Model* model = nullptr;
+  DiffState diff;
   hipStream_t own_stream = nullptr;
   hipStream_t fill_stream = nullptr;
   hipStream_t user_stream = nullptr;
@@ -294,6 +308,10 @@ struct Renderer {
 void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats);
 void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats);
 void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats);
+// differentiable full render (diff.hip)
+void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int n, const float* pos,
+                  const float* base, const float* head, float* rgb, fvc_diff_stats* stats);
+void diff_backward(Renderer& r, const float* d_rgb, float* g_pos, float* g_base, float* g_head);
 void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts, float* t0s);
 void resize_renderbuffers(Renderer& r, int w, int h);
 

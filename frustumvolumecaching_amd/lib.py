@@ -62,6 +62,9 @@ _SIGS = {
     "fvc_cache_get_known_ranges": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "fvc_cache_get_pool": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "fvc_debug_march": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, _P, _P]),
+    "fvc_diff_forward": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.c_int32, _P, _P, _P, _P,
+                                   C.POINTER(abi.DiffStats)]),
+    "fvc_diff_backward": (C.c_int, [_P, _P, _P, _P, _P]),
     "fvc_profile_enable": (C.c_int, [_P, C.c_int]),
     "fvc_profile_get": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
 }
@@ -295,6 +298,21 @@ class Renderer:
         check(load().fvc_profile_get(self.h, C.cast(arr, C.c_void_p), n.value, C.byref(n)))
         return {a.name.decode(): dict(launches=a.launches, ms=a.total_ms, units=a.units, bytes=a.alg_bytes,
                                       flops=a.alg_flops) for a in arr[:n.value]}
+
+    # differentiable full render (training mode)
+    def diff_forward(self, rm, pix, params, rgb):
+        """rgb [n, 3] (device) = C + T of the pixels pix (device int32, y * W + x)
+        with FP32 parameter blocks params = (pos_enc, mlp_base, mlp_head)."""
+        st = abi.DiffStats()
+        pos, base, head = params
+        check(load().fvc_diff_forward(self.h, C.byref(rm), _addr(pix), int(pix.numel()), _addr(pos), _addr(base),
+                                      _addr(head), _addr(rgb), C.byref(st)))
+        return st
+
+    def diff_backward(self, d_rgb, grads):
+        """Adds d(sum(d_rgb * rgb))/d(params) of the last diff_forward to grads."""
+        g_pos, g_base, g_head = grads
+        check(load().fvc_diff_backward(self.h, _addr(d_rgb), _addr(g_pos), _addr(g_base), _addr(g_head)))
 
     def debug_march(self, rm, max_per_ray=64):
         import numpy as np

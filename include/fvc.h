@@ -158,6 +158,12 @@ typedef struct {
   int32_t  pad_;
 } fvc_render_stats;
 
+/* Differentiable render statistics */
+typedef struct {
+  int64_t n_rays;
+  int64_t n_samples;              /* occupied samples marched (all evaluated forward) */
+} fvc_diff_stats;
+
 /* Frustum cache geometry (Cache::frustumDims / brickArrayDims, cache.h:70-73) */
 typedef struct {
   int32_t frustum_x, frustum_y, frustum_z;
@@ -260,6 +266,23 @@ int fvc_cache_set_settings(fvc_renderer* renderer, const fvc_cache_settings* set
 /* Pool capacity override in bricks (0 = reference estimate n_rays/8 rounded
  * up to 16384, cache_renderer.cu:832-846 + cache.cu:143-146). */
 int fvc_cache_set_capacity(fvc_renderer* renderer, int32_t capacity_bricks);
+
+/* ---- differentiable full render (training mode) ----
+ * No counterpart entry point in the reference, whose renderers are
+ * inference-only; the forward is PerformanceRenderer::render's per-ray
+ * semantics (performance_renderer.cu:15-125) for a pixel list, with the
+ * caller's FP32 parameter blocks (nerfacc layout: pos_enc [entries][2],
+ * mlp_base W1|W2|W3, mlp_head H1|H2, row-major [out][in]) and FP32
+ * arithmetic.  All pointers are device pointers.  pix: pixel indices y*W+x
+ * of rm's camera; rgb [n][3] = composited colour + T (white background).
+ * The parameter blocks must stay unchanged until the backward call.
+ * backward: adds d(sum(d_rgb * rgb))/d(params) of the last forward to the
+ * three gradient blocks (same layouts; the caller zeroes them). */
+int fvc_diff_forward(fvc_renderer* renderer, const fvc_raymarch_info* rm, const int32_t* pix, int32_t n_rays,
+                     const float* pos_enc, const float* mlp_base, const float* mlp_head, float* rgb,
+                     fvc_diff_stats* stats);
+int fvc_diff_backward(fvc_renderer* renderer, const float* d_rgb, float* g_pos_enc, float* g_mlp_base,
+                      float* g_mlp_head);
 
 /* ---- profiling ---- */
 /* Enable (1) or disable (0) per-kernel timing; enabling clears the table. */

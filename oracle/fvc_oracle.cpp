@@ -963,6 +963,286 @@ struct CacheHandle {
   std::unique_ptr<Cache<float>> f;
 };
 
+// ---------------------------------------------------------------------------
+// Differentiable full render (training mode).
+//
+// Forward: the full render of a pixel list with FP32 parameter blocks (nerfacc
+// layout) in place of the model's fp16 ones, the network and compositing in R
+// without half rounding.  The sample geometry is float, as in the FP32
+// pipeline: the march (bit-exact with the GPU marcher), the contracted sample
+// positions and the hash-grid cell coordinates; the rest is R.  (In double,
+// a float position's grid fraction at the finest levels moves by up to ~1e-4,
+// which changes small corner weights, and so the table gradient, by far more
+// than FP32 arithmetic elsewhere does.)
+// Output per ray: composited colour plus the white background, C + T.
+// Backward: the gradient of sum(d_rgb * rgb) with respect to the three
+// parameter blocks, by the chain rule through
+//   compositing  C = sum_i w_i c_i + T_N, w_i = a_i T_i, T_{i+1} = T_i (1 - a_i),
+//                a_i = 1 - exp(-sigma_i dt_i), sigma = exp(raw), c = sigmoid(o):
+//                dC/dc_i = w_i, dC/dsigma_i = dt_i (T_{i+1} c_i - S_i) with the
+//                suffix S_i = C - sum_{j<=i} w_j c_j (background included);
+//   the colour MLP, the density MLP (ReLU derivative 1 where the
+//   pre-activation is > 0) and the trilinear hash-grid interpolation
+//   (scatter of w_corner * dL/dfeature into the corner's entry).
+// Samples skipped by alpha_thre and samples after termination (T < 1e-4)
+// carry no gradient: the branch decisions are constants of the forward.
+// ---------------------------------------------------------------------------
+template <class R> struct Acts {
+  R enc[32], a1[128], a2[128], out[16], xh[32], z1[128], o[16];
+};
+
+template <class R> void dense(const std::vector<float>& W, int n_out, int n_in, const R* x, R* y) {
+  for (int j = 0; j < n_out; j++) {
+    R acc = 0;
+    for (int k = 0; k < n_in; k++) acc += (R)W[(size_t)j * n_in + k] * x[k];
+    y[j] = acc;
+  }
+}
+template <class R> inline R relu(R v) { return v > R(0) ? v : R(0); }
+
+template <class R> Mode diff_mode() { return sizeof(R) == 8 ? M_F64 : M_F32; }
+
+// Trilinear corners of level l at the clamped position pc.  The grid
+// coordinate pc * scale + 0.5, its cell and fraction are sample geometry:
+// float, as the FP32 pipeline computes them; the weights are products in R.
+template <class R> void corners(const Net& n, int l, const float* pc, uint32_t* idx, R* w) {
+  const float scale = n.scale_f[l];
+  R frac[3];
+  uint32_t g[3];
+  for (int k = 0; k < 3; k++) {
+    const float pos = pc[k] * scale + 0.5f;
+    const float fl = std::floor(pos);
+    g[k] = (uint32_t)(int)fl;
+    frac[k] = (R)(pos - fl);
+  }
+  for (int c = 0; c < 8; c++) {
+    R ww = 1;
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+      if (c & (1 << k)) { ww *= frac[k]; q[k] = g[k] + 1; }
+      else { ww *= R(1) - frac[k]; q[k] = g[k]; }
+    }
+    idx[c] = n.offset[l] + grid_index(n, l, q[0], q[1], q[2]);
+    w[c] = ww;
+  }
+}
+
+// hash-grid encoding with the float sample geometry of corners()
+template <class R> void encode_fgeo(const Net& n, V3<float> p, R* out) {
+  const float pc[3] = {std::min(std::max(p.x, 0.0f), 1.0f), std::min(std::max(p.y, 0.0f), 1.0f),
+                       std::min(std::max(p.z, 0.0f), 1.0f)};
+  for (int l = 0; l < n.L; l++) {
+    uint32_t idx[8];
+    R w[8];
+    corners(n, l, pc, idx, w);
+    R acc[2] = {0, 0};
+    for (int c = 0; c < 8; c++) {
+      acc[0] += w[c] * (R)n.table[(size_t)idx[c] * 2];
+      acc[1] += w[c] * (R)n.table[(size_t)idx[c] * 2 + 1];
+    }
+    out[2 * l] = acc[0];
+    out[2 * l + 1] = acc[1];
+  }
+}
+
+template <class R> void net_fwd(const Net& n, V3<float> p, V3<R> dir01, Acts<R>& a) {
+  const int Wb = n.d.base_width, Wh = n.d.head_width;
+  encode_fgeo(n, p, a.enc);
+  R h[128];
+  dense(n.W1, Wb, 32, a.enc, a.a1);
+  for (int k = 0; k < Wb; k++) h[k] = relu(a.a1[k]);
+  dense(n.W2, Wb, Wb, h, a.a2);
+  for (int k = 0; k < Wb; k++) h[k] = relu(a.a2[k]);
+  dense(n.W3, 16, Wb, h, a.out);
+  const int NL = n.d.n_latents;
+  for (int k = 0; k < NL; k++) a.xh[k] = a.out[n.d.latent_offset + k];
+  R sh[16];
+  sh4(dir01, sh);
+  for (int k = 0; k < 16; k++) a.xh[NL + k] = sh[k];
+  for (int k = NL + 16; k < 32; k++) a.xh[k] = R(1);
+  dense(n.H1, Wh, 32, a.xh, a.z1);
+  for (int k = 0; k < Wh; k++) h[k] = relu(a.z1[k]);
+  dense(n.H2, 16, Wh, h, a.o);
+}
+
+struct DiffGrads {
+  std::vector<double> base, head;  // per thread
+};
+
+template <class R>
+void net_bwd(const Net& n, V3<float> p, const Acts<R>& a, const R* d_rgb3, R d_sraw, double* gb, double* gh,
+             double* gpos) {
+  const int Wb = n.d.base_width, Wh = n.d.head_width, NL = n.d.n_latents;
+  // colour MLP
+  R hh[128], d_z1[128], d_xh[32];
+  for (int k = 0; k < Wh; k++) hh[k] = relu(a.z1[k]);
+  double* gH1 = gh;
+  double* gH2 = gh + 32 * Wh;
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < Wh; k++) gH2[j * Wh + k] += (double)(d_rgb3[j] * hh[k]);
+  for (int k = 0; k < Wh; k++) {
+    R s = 0;
+    for (int j = 0; j < 3; j++) s += (R)n.H2[(size_t)j * Wh + k] * d_rgb3[j];
+    d_z1[k] = a.z1[k] > R(0) ? s : R(0);
+  }
+  for (int j = 0; j < Wh; j++)
+    for (int k = 0; k < 32; k++) gH1[j * 32 + k] += (double)(d_z1[j] * a.xh[k]);
+  for (int k = 0; k < 32; k++) {
+    R s = 0;
+    for (int j = 0; j < Wh; j++) s += (R)n.H1[(size_t)j * 32 + k] * d_z1[j];
+    d_xh[k] = s;
+  }
+  // density MLP
+  R d_out[16] = {};
+  d_out[0] = d_sraw;
+  for (int k = 0; k < NL; k++) d_out[n.d.latent_offset + k] += d_xh[k];
+  R h1[128], h2[128], d_a2[128], d_a1[128], d_enc[32];
+  for (int k = 0; k < Wb; k++) { h1[k] = relu(a.a1[k]); h2[k] = relu(a.a2[k]); }
+  double* gW1 = gb;
+  double* gW2 = gb + 32 * Wb;
+  double* gW3 = gb + 32 * Wb + Wb * Wb;
+  for (int j = 0; j < 16; j++)
+    for (int k = 0; k < Wb; k++) gW3[j * Wb + k] += (double)(d_out[j] * h2[k]);
+  for (int k = 0; k < Wb; k++) {
+    R s = 0;
+    for (int j = 0; j < 16; j++) s += (R)n.W3[(size_t)j * Wb + k] * d_out[j];
+    d_a2[k] = a.a2[k] > R(0) ? s : R(0);
+  }
+  for (int j = 0; j < Wb; j++)
+    for (int k = 0; k < Wb; k++) gW2[j * Wb + k] += (double)(d_a2[j] * h1[k]);
+  for (int k = 0; k < Wb; k++) {
+    R s = 0;
+    for (int j = 0; j < Wb; j++) s += (R)n.W2[(size_t)j * Wb + k] * d_a2[j];
+    d_a1[k] = a.a1[k] > R(0) ? s : R(0);
+  }
+  for (int j = 0; j < Wb; j++)
+    for (int k = 0; k < 32; k++) gW1[j * 32 + k] += (double)(d_a1[j] * a.enc[k]);
+  for (int k = 0; k < 32; k++) {
+    R s = 0;
+    for (int j = 0; j < Wb; j++) s += (R)n.W1[(size_t)j * 32 + k] * d_a1[j];
+    d_enc[k] = s;
+  }
+  // hash grid
+  const float pc[3] = {std::min(std::max(p.x, 0.0f), 1.0f), std::min(std::max(p.y, 0.0f), 1.0f),
+                       std::min(std::max(p.z, 0.0f), 1.0f)};
+  for (int l = 0; l < n.L; l++) {
+    uint32_t idx[8];
+    R w[8];
+    corners(n, l, pc, idx, w);
+    for (int c = 0; c < 8; c++)
+      for (int f = 0; f < 2; f++) {
+        const double v = (double)(w[c] * d_enc[2 * l + f]);
+#pragma omp atomic
+        gpos[(size_t)idx[c] * 2 + f] += v;
+      }
+  }
+}
+
+// one ray: forward (rgb) and, with d_rgb, backward
+template <class R>
+int64_t diff_ray(const Model& M, const Net& n, const fvc_raymarch_info& rm, int px, int py, R* rgb, const R* d_rgb,
+                 double* gb, double* gh, double* gpos) {
+  const Step<float> st(rm.stepsize_info);
+  const RayInit<float> ri = init_ray<float>(rm, M.scene, px, py);
+  struct Rec { float t0, dt; R w, T1, c[3], sigma; };
+  std::vector<Rec> used;
+  R C[3] = {0, 0, 0}, T = 1;
+  int64_t n_eval = 0;
+  const V3<R> d = {(R)ri.ray.d.x, (R)ri.ray.d.y, (R)ri.ray.d.z};
+  const V3<R> dir01 = unit_to_01(d);
+  if (ri.alive) {
+    float t1 = ri.start;
+    while (true) {
+      const float t0 = t1;
+      const float dt = st.dt(t0);
+      t1 = t0 + dt;
+      const float tm = (t0 + t1) * 0.5f;
+      if (tm > ri.far_) break;
+      if (!occupied(ri.ray.o + tm * ri.ray.d, M.scene, M.bitfield.data())) continue;
+      Acts<R> a;
+      net_fwd(n, contract(ri.ray.o + tm * ri.ray.d, M.scene), dir01, a);
+      n_eval++;
+      const R sigma = std::exp(a.out[0]);
+      R alpha = R(1) - std::exp(-sigma * (R)dt);
+      alpha = std::min(std::max(alpha, R(0)), R(1));
+      if (alpha > (R)M.scene.alpha_thre) {
+        Rec r;
+        r.t0 = t0; r.dt = dt; r.sigma = sigma;
+        r.w = alpha * T;
+        for (int c = 0; c < 3; c++) {
+          r.c[c] = R(1) / (R(1) + std::exp(-a.o[c]));
+          C[c] = C[c] + r.w * r.c[c];
+        }
+        T = T * (R(1) - alpha);
+        r.T1 = T;
+        used.push_back(r);
+      }
+      if (T < R(1e-4)) break;
+    }
+  }
+  for (int c = 0; c < 3; c++) rgb[c] = C[c] + T;
+  if (!d_rgb) return n_eval;
+  R prefix[3] = {0, 0, 0};
+  for (const Rec& r : used) {
+    R dsig = 0, dcr[3];
+    for (int c = 0; c < 3; c++) {
+      prefix[c] = prefix[c] + r.w * r.c[c];
+      const R S = rgb[c] - prefix[c];
+      dsig += d_rgb[c] * (r.T1 * r.c[c] - S);
+      dcr[c] = d_rgb[c] * r.w * r.c[c] * (R(1) - r.c[c]);
+    }
+    dsig *= (R)r.dt;
+    const float tm = (r.t0 + (r.t0 + r.dt)) * 0.5f;
+    const V3<float> p = contract(ri.ray.o + tm * ri.ray.d, M.scene);
+    Acts<R> a;
+    net_fwd(n, p, dir01, a);
+    net_bwd(n, p, a, dcr, dsig * r.sigma, gb, gh, gpos);
+  }
+  return n_eval;
+}
+
+// Network with FP32 parameter blocks in the nerfacc layout
+inline Net net_with_params(const Net& base, const float* pos, const float* mb, const float* mh) {
+  Net n = base;
+  const int Wb = n.d.base_width, Wh = n.d.head_width;
+  n.table.assign(pos, pos + n.n_pos_enc);
+  n.W1.assign(mb, mb + 32 * Wb);
+  n.W2.assign(mb + 32 * Wb, mb + 32 * Wb + Wb * Wb);
+  n.W3.assign(mb + 32 * Wb + Wb * Wb, mb + 32 * Wb + Wb * Wb + 16 * Wb);
+  n.H1.assign(mh, mh + 32 * Wh);
+  n.H2.assign(mh + 32 * Wh, mh + 32 * Wh + 16 * Wh);
+  return n;
+}
+
+template <class R>
+int64_t diff_render(const Model& M, const fvc_raymarch_info& rm, const int32_t* pix, int64_t n_rays, const float* pos,
+                    const float* mb, const float* mh, double* rgb_out, const double* d_rgb, double* g_pos,
+                    double* g_base, double* g_head) {
+  const Net n = net_with_params(M.net, pos, mb, mh);
+  const int W = rm.cam_info.res_x;
+  int64_t total = 0;
+  const size_t nb = (size_t)n.n_base, nh = (size_t)n.n_head;
+#pragma omp parallel reduction(+ : total)
+  {
+    std::vector<double> gb(d_rgb ? nb : 0, 0.0), gh(d_rgb ? nh : 0, 0.0);
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t i = 0; i < n_rays; i++) {
+      R rgb[3], dr[3];
+      if (d_rgb) for (int c = 0; c < 3; c++) dr[c] = (R)d_rgb[3 * i + c];
+      total += diff_ray<R>(M, n, rm, pix[i] % W, pix[i] / W, rgb, d_rgb ? dr : nullptr, gb.data(), gh.data(), g_pos);
+      for (int c = 0; c < 3; c++) rgb_out[3 * i + c] = (double)rgb[c];
+    }
+    if (d_rgb) {
+#pragma omp critical
+      {
+        for (size_t k = 0; k < nb; k++) g_base[k] += gb[k];
+        for (size_t k = 0; k < nh; k++) g_head[k] += gh[k];
+      }
+    }
+  }
+  return total;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1219,6 +1499,17 @@ void orc_cache_lookup(const CacheHandle* h, const float* froxel, int64_t n, doub
       for (int k = 0; k < 9; k++) out9[9 * i + k] = o[k];
     }
   }
+}
+
+// Differentiable full render of the pixels pix[n] (y * W + x), see
+// diff_render; f64 selects double arithmetic (else float).  rgb[n][3];
+// with d_rgb[n][3] the parameter gradients are added to g_pos, g_base,
+// g_head.  Returns the number of network evaluations.
+int64_t orc_diff_render(const Model* M, const fvc_raymarch_info* rm, const int32_t* pix, int64_t n, const float* pos,
+                        const float* base, const float* head, int f64, double* rgb, const double* d_rgb,
+                        double* g_pos, double* g_base, double* g_head) {
+  if (f64) return diff_render<double>(*M, *rm, pix, n, pos, base, head, rgb, d_rgb, g_pos, g_base, g_head);
+  return diff_render<float>(*M, *rm, pix, n, pos, base, head, rgb, d_rgb, g_pos, g_base, g_head);
 }
 
 }  // extern "C"

@@ -75,6 +75,8 @@ def lib():
             "orc_render_cached": (None, [P, P, C.POINTER(abi.RaymarchInfo), C.c_int, C.c_int, C.c_int, C.c_int,
                                          P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
             "orc_cache_lookup": (None, [P, P, C.c_int64, P]),
+            "orc_diff_render": (C.c_int64, [P, C.POINTER(abi.RaymarchInfo), P, C.c_int64, P, P, P, C.c_int, P, P,
+                                            P, P, P]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -191,6 +193,23 @@ class Model:
         ns = C.c_int64()
         lib().orc_render_full(self.h, C.byref(rm), mode, x0, y0, w, h, _p(out), C.byref(ns))
         return out, ns.value
+
+    def diff_render(self, rm, pix, params32, f64=True, d_rgb=None):
+        """Differentiable full render of pixel indices pix (y * W + x) with FP32
+        parameter blocks params32 = (pos_enc, mlp_base, mlp_head).  Returns
+        (rgb [n, 3] float64, grads or None, n_evaluations); with d_rgb [n, 3]
+        grads = (g_pos, g_base, g_head) of sum(d_rgb * rgb)."""
+        pix = np.ascontiguousarray(pix, dtype=np.int32)
+        pos, base, head = (np.ascontiguousarray(p, dtype=np.float32) for p in params32)
+        rgb = np.zeros((len(pix), 3), np.float64)
+        grads, dr = None, None
+        if d_rgb is not None:
+            dr = np.ascontiguousarray(d_rgb, dtype=np.float64).reshape(len(pix), 3)
+            grads = tuple(np.zeros(p.size, np.float64) for p in (pos, base, head))
+        n = lib().orc_diff_render(self.h, C.byref(rm), _p(pix), len(pix), _p(pos), _p(base), _p(head), int(f64),
+                                  _p(rgb), None if dr is None else _p(dr),
+                                  *(None if grads is None else _p(g) for g in (grads or (None,) * 3)))
+        return rgb, grads, n
 
     def cache_fill(self, rm, mode=EMU16, capacity=0, crop=None):
         """CacheRenderer::initCacheBlockwise; crop = (x0, y0, w, h) fills only the
