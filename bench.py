@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--max-steps", type=int, default=0, help="cap on samples per ray per launch")
     ap.add_argument("--precision", choices=["fp16", "bf16", "fp8"], default="fp16",
                     help="MLP arithmetic (operands; FP32 accumulation)")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render leg")
+    ap.add_argument("--diff-rays", type=int, default=1 << 18, help="rays per differentiable step (800x800 view)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference point for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -262,8 +264,56 @@ def main():
         "kernels_cached": kernel_table(prof_cached),
         "kernels_full": kernel_table(prof_full),
         "cpu_baseline": cpu,
+        "diff": None if a.no_diff else diff_leg(a),
     }
     print(json.dumps(out), flush=True)
+
+
+def diff_leg(a, steps=3):
+    """Training-mode step of the differentiable full render: forward +
+    backward of a.diff_rays random pixels of an 800x800 view of the small
+    plain scene, FP32 parameters = fp16 weights + N(0, 0.01), L2 loss."""
+    import numpy as np
+    import torch
+    from frustumvolumecaching_amd import autograd as AG
+    from frustumvolumecaching_amd import lib
+    from frustumvolumecaching_amd import synthetic as S
+    wl = S.small_plain(800, 800)
+    model = lib.NerfNetwork.from_workload(wl)
+    r = lib.Renderer(model)
+    rm = wl.rm()
+    rng = np.random.default_rng(21)
+    params = []
+    for p in wl.params:
+        v = np.asarray(p, np.uint16).view(np.float16).astype(np.float32)
+        params.append(torch.from_numpy(v + rng.normal(0, 0.01, v.shape).astype(np.float32)).cuda().requires_grad_(True))
+    fwd, bwd, samples = [], [], []
+    for k in range(steps + 1):
+        pix = torch.from_numpy(rng.choice(800 * 800, a.diff_rays, replace=False).astype(np.int32)).cuda()
+        target = torch.rand(a.diff_rays, 3, device="cuda")
+        for p in params:
+            p.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rgb = AG.render_rays(r, rm, pix, *params)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ((rgb - target) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if k > 0:
+            fwd.append((t1 - t0) * 1e3)
+            bwd.append((t2 - t1) * 1e3)
+    rgb = torch.empty(a.diff_rays, 3, device="cuda")
+    st = r.diff_forward(rm, pix, tuple(p.detach() for p in params), rgb)
+    r.close()
+    model.close()
+    fm, bm = sum(fwd) / len(fwd), sum(bwd) / len(bwd)
+    log(f"[bench] differentiable step: forward {fm:.2f} ms, backward {bm:.2f} ms, {st.n_samples} samples")
+    return {"workload": "small_plain_800x800, 2^18 random rays per step, L2 loss",
+            "rays": a.diff_rays, "samples_per_step": int(st.n_samples),
+            "forward_ms": round(fm, 3), "backward_ms": round(bm, 3), "step_ms": round(fm + bm, 3),
+            "msamples_per_s": round(st.n_samples / (fm + bm) / 1e3, 1)}
 
 
 if __name__ == "__main__":

@@ -106,7 +106,7 @@ class InitStats(C.Structure):
 
 class DiffStats(C.Structure):
     """fvc_diff_stats (differentiable render)."""
-    _fields_ = [("n_rays", C.c_int64), ("n_samples", C.c_int64)]
+    _fields_ = [("n_rays", C.c_int64), ("n_samples", C.c_int64), ("n_cache_samples", C.c_int64)]
 
 
 class RenderStats(C.Structure):

@@ -421,15 +421,6 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats)
 }
 
 // ------------------------------------------------------------------ render
-struct CacheView {
-  fvc_camera_info cam;
-  fvc_stepsize_info st;
-  int fx, fy, fz, bx, by;
-  const int* table;
-  const float* known;
-  const Voxel* pool;
-};
-
 // sample_cache_performance_kernel (cache_performance_renderer.cu:22-177):
 // march, reproject every occupied sample into the cache frustum and
 // classify it as resample (outside / beyond the known range), cache hit
@@ -719,13 +710,7 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
   r.c_t0.ensure(cap * sizeof(float));
   r.c_out.ensure(cap * sizeof(uint2));
   hipStream_t s = r.stream();
-  CacheView cv;
-  cv.cam = C.rm.cam_info;
-  cv.st = C.rm.stepsize_info;
-  cv.fx = C.fx; cv.fy = C.fy; cv.fz = C.fz; cv.bx = C.bx; cv.by = C.by;
-  cv.table = C.table.as<int>();
-  cv.known = C.known.as<float>();
-  cv.pool = C.pool.as<Voxel>();
+  const CacheView cv = cache_view(C);
   const MarchGrid mg = march_grid(r, rm);
 
   Profiler& P = r.prof;

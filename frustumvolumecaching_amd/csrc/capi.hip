@@ -308,16 +308,41 @@ int fvc_diff_forward(fvc_renderer* h, const fvc_raymarch_info* rm, const int32_t
     REQUIRE(n_rays >= 0, "negative ray count");
     REQUIRE(n_rays == 0 || (pix && pos_enc && mlp_base && mlp_head && rgb), "null argument");
     std::scoped_lock lk(h->r.read_mutex);
-    diff_forward(h->r, *rm, pix, n_rays, pos_enc, mlp_base, mlp_head, rgb, stats);
+    diff_forward(h->r, *rm, pix, n_rays, pos_enc, mlp_base, mlp_head, rgb, stats, false);
   });
 }
 
 int fvc_diff_backward(fvc_renderer* h, const float* d_rgb, float* g_pos_enc, float* g_mlp_base, float* g_mlp_head) {
   return guard([&] {
     REQUIRE(h, "null renderer");
+    REQUIRE(!h->r.diff.cached, "the last forward was a cached one: use fvc_diff_cached_backward");
     REQUIRE(h->r.diff.n == 0 || (d_rgb && g_pos_enc && g_mlp_base && g_mlp_head), "null argument");
     std::scoped_lock lk(h->r.read_mutex);
-    diff_backward(h->r, d_rgb, g_pos_enc, g_mlp_base, g_mlp_head);
+    diff_backward(h->r, d_rgb, g_pos_enc, g_mlp_base, g_mlp_head, nullptr);
+  });
+}
+
+int fvc_diff_cached_forward(fvc_renderer* h, const fvc_raymarch_info* rm, const int32_t* pix, int32_t n_rays,
+                            const float* pos_enc, const float* mlp_base, const float* mlp_head, float* rgb,
+                            fvc_diff_stats* stats) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    check_rm(rm);
+    REQUIRE(n_rays >= 0, "negative ray count");
+    REQUIRE(n_rays == 0 || (pix && pos_enc && mlp_base && mlp_head && rgb), "null argument");
+    std::scoped_lock lk(h->r.read_mutex);
+    diff_forward(h->r, *rm, pix, n_rays, pos_enc, mlp_base, mlp_head, rgb, stats, true);
+  });
+}
+
+int fvc_diff_cached_backward(fvc_renderer* h, const float* d_rgb, float* g_pos_enc, float* g_mlp_base,
+                             float* g_mlp_head, float* g_pool) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    REQUIRE(h->r.diff.cached, "the last forward was not a cached one: use fvc_diff_backward");
+    REQUIRE(h->r.diff.n == 0 || (d_rgb && g_pos_enc && g_mlp_base && g_mlp_head && g_pool), "null argument");
+    std::scoped_lock lk(h->r.read_mutex);
+    diff_backward(h->r, d_rgb, g_pos_enc, g_mlp_base, g_mlp_head, g_pool);
   });
 }
 

@@ -217,6 +217,23 @@ struct FwdActs {
   float xh[8];
 };
 
+// colour MLP on the base-output rows 4g..4g+3 in `rows` (rows 1..8 are the
+// latents; row 0 and rows 9..15 give way to the 1.0 padding inputs)
+__device__ __forceinline__ void head_forward(const float* __restrict__ fr, int lane, int g, f4 rows, V3 d01,
+                                             FwdActs& a) {
+  float sh[4], h[16];
+  sh4_quarter_f32(d01, g, sh);
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int r = 4 * g + m;
+    a.xh[m] = (r >= 1 && r <= 8) ? rows[m] : 1.0f;
+    a.xh[4 + m] = sh[m];
+  }
+  dense4<4, 8>(fr + F_H1 * 64, lane, a.xh, a.z1);
+  relu_slots<4>(a.z1, h);
+  dense4<1, 16>(fr + F_H2 * 64, lane, h, &a.o);
+}
+
 __device__ __forceinline__ void net_forward(const float* __restrict__ fr, const float* __restrict__ table,
                                             const GridLevel* __restrict__ lv, const GridParams& gp, V3 p, V3 d01,
                                             int lane, int g, FwdActs& a) {
@@ -242,17 +259,7 @@ __device__ __forceinline__ void net_forward(const float* __restrict__ fr, const 
   dense4<4, 16>(fr + F_W2 * 64, lane, h, a.a2);
   relu_slots<4>(a.a2, h);
   dense4<1, 16>(fr + F_W3 * 64, lane, h, &a.out);
-  float sh[4];
-  sh4_quarter_f32(d01, g, sh);
-#pragma unroll
-  for (int m = 0; m < 4; m++) {
-    const int r = 4 * g + m;
-    a.xh[m] = (r >= 1 && r <= 8) ? a.out[m] : 1.0f;
-    a.xh[4 + m] = sh[m];
-  }
-  dense4<4, 8>(fr + F_H1 * 64, lane, a.xh, a.z1);
-  relu_slots<4>(a.z1, h);
-  dense4<1, 16>(fr + F_H2 * 64, lane, h, &a.o);
+  head_forward(fr, lane, g, a.out, d01, a);
 }
 
 __device__ __forceinline__ V3 ray_dir01(const RayState& r) { return unit_to_01(V3{r.dx, r.dy, r.dz}); }
@@ -315,7 +322,8 @@ __global__ __launch_bounds__(256) void k_diff_emit(int n, MarchGrid mg, Contract
 }
 
 // ------------------------------------------------------------ network
-__global__ __launch_bounds__(256) void k_diff_net_fwd(int n, const float* __restrict__ s_pos,
+__global__ __launch_bounds__(256) void k_diff_net_fwd(int n, const int* __restrict__ list,
+                                                      const float* __restrict__ s_pos,
                                                       const int* __restrict__ s_ray,
                                                       const RayState* __restrict__ rays,
                                                       const float* __restrict__ table, GridParams gp,
@@ -327,14 +335,14 @@ __global__ __launch_bounds__(256) void k_diff_net_fwd(int n, const float* __rest
   const int col = lane & 15, g = lane >> 4;
   const int n_groups = (n + 15) / 16;
   for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
-    const int s = grp * 16 + col;
-    const bool valid = s < n;
-    const int sl = valid ? s : n - 1;
+    const int j = grp * 16 + col;
+    const bool valid = j < n;
+    const int sl = list ? list[valid ? j : n - 1] : (valid ? j : n - 1);
     const V3 p = {s_pos[3 * sl], s_pos[3 * sl + 1], s_pos[3 * sl + 2]};
     const V3 d01 = ray_dir01(rays[s_ray[sl]]);
     FwdActs a;
     net_forward(frags, table, lv, gp, p, d01, lane, g, a);
-    if (g == 0 && valid) out4[s] = make_float4(a.o[0], a.o[1], a.o[2], a.out[0]);
+    if (g == 0 && valid) out4[sl] = make_float4(a.o[0], a.o[1], a.o[2], a.out[0]);
   }
 }
 
@@ -475,7 +483,50 @@ struct KHead {
   }
 };
 
-__global__ __launch_bounds__(256) void k_diff_net_bwd(int n, const float* __restrict__ s_pos,
+// colour MLP backward of a tile: d_o (output rows 4g..4g+3), d_z1
+// (hidden pre-activations) and d_xh (head inputs in the rows of the
+// base output, row r = latent r - 1 for r in 1..8)
+__device__ __forceinline__ void head_backward(const float* __restrict__ frags, int lane, int g, const FwdActs& a,
+                                              float4 dq, f4& d_o, f4* d_z1, f4& d_xh) {
+  d_o = g == 0 ? f4{dq.x, dq.y, dq.z, 0.0f} : f4zero();
+  float x[16];
+#pragma unroll
+  for (int e = 0; e < 4; e++) x[e] = d_o[e];
+  dense4<4, 4>(frags + B_H2T * 64, lane, x, d_z1);
+  relu_grad<4>(a.z1, d_z1);
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) x[4 * nb + e] = d_z1[nb][e];
+  dense4<1, 16>(frags + B_H1T * 64, lane, x, &d_xh);
+}
+
+__device__ __forceinline__ void head_wgrad(float* ys, float* xs, const FwdActs& a, const f4& d_o, const f4* d_z1,
+                                           int col, int g, float* acc) {
+  float h[16];
+  relu_slots<4>(a.z1, h);
+  wgrad<1, 16>(ys, xs, &d_o, h, KChain(), col, g, acc + G_H2, 64);
+  wgrad<4, 8>(ys, xs, d_z1, a.xh, KHead(), col, g, acc + G_H1, 32);
+}
+
+// block accumulators into the caller's gradient blocks
+__device__ __forceinline__ void flush_grads(const float* acc, float* g_base, float* g_head) {
+  for (int k = threadIdx.x; k < G_WORDS; k += blockDim.x) {
+    const float v = acc[k];
+    if (v == 0.0f) continue;
+    if (k < G_H1) unsafeAtomicAdd(&g_base[k], v);
+    else unsafeAtomicAdd(&g_head[k - G_H1], v);
+  }
+}
+__device__ __forceinline__ void flush_head_grads(const float* acc, float* g_head) {
+  for (int k = G_H1 + threadIdx.x; k < G_WORDS; k += blockDim.x) {
+    const float v = acc[k];
+    if (v != 0.0f) unsafeAtomicAdd(&g_head[k - G_H1], v);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_diff_net_bwd(int n, const int* __restrict__ list,
+                                                      const float* __restrict__ s_pos,
                                                       const int* __restrict__ s_ray,
                                                       const RayState* __restrict__ rays,
                                                       const float* __restrict__ table, GridParams gp,
@@ -494,27 +545,18 @@ __global__ __launch_bounds__(256) void k_diff_net_bwd(int n, const float* __rest
   float* xs = stage[wave][1];
   const int n_groups = (n + 15) / 16;
   for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
-    const int s = grp * 16 + col;
-    const bool valid = s < n;
-    const int sl = valid ? s : n - 1;
-    const float4 dq = valid ? d_out4[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int j = grp * 16 + col;
+    const bool valid = j < n;
+    const int sl = list ? list[valid ? j : n - 1] : (valid ? j : n - 1);
+    const float4 dq = valid ? d_out4[sl] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const V3 p = {s_pos[3 * sl], s_pos[3 * sl + 1], s_pos[3 * sl + 2]};
     const V3 d01 = ray_dir01(rays[s_ray[sl]]);
     FwdActs a;
     net_forward(frags, table, lv, gp, p, d01, lane, g, a);
     // colour MLP
-    const f4 d_o = g == 0 ? f4{dq.x, dq.y, dq.z, 0.0f} : f4zero();
+    f4 d_o, d_z1[4], d_xh;
+    head_backward(frags, lane, g, a, dq, d_o, d_z1, d_xh);
     float x[16];
-    f4 d_z1[4];
-    for (int e = 0; e < 4; e++) x[e] = d_o[e];
-    dense4<4, 4>(frags + B_H2T * 64, lane, x, d_z1);
-    relu_grad<4>(a.z1, d_z1);
-    f4 d_xh;
-#pragma unroll
-    for (int nb = 0; nb < 4; nb++)
-#pragma unroll
-      for (int e = 0; e < 4; e++) x[4 * nb + e] = d_z1[nb][e];
-    dense4<1, 16>(frags + B_H1T * 64, lane, x, &d_xh);
     // density MLP: output rows 4g + e (0: log density, 1..8: latents)
     f4 d_out;
 #pragma unroll
@@ -555,10 +597,8 @@ __global__ __launch_bounds__(256) void k_diff_net_bwd(int n, const float* __rest
       }
     }
     // weight gradients (activations as the forward's B slots)
+    head_wgrad(ys, xs, a, d_o, d_z1, col, g, acc);
     float h[16];
-    relu_slots<4>(a.z1, h);
-    wgrad<1, 16>(ys, xs, &d_o, h, KChain(), col, g, acc + G_H2, 64);
-    wgrad<4, 8>(ys, xs, d_z1, a.xh, KHead(), col, g, acc + G_H1, 32);
     relu_slots<4>(a.a2, h);
     wgrad<1, 16>(ys, xs, &d_out, h, KChain(), col, g, acc + G_W3, 64);
     relu_slots<4>(a.a1, h);
@@ -566,12 +606,247 @@ __global__ __launch_bounds__(256) void k_diff_net_bwd(int n, const float* __rest
     wgrad<4, 8>(ys, xs, d_a1, a.enc, KEnc(), col, g, acc + G_W1, 32);
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < G_WORDS; k += blockDim.x) {
-    const float v = acc[k];
-    if (v == 0.0f) continue;
-    if (k < G_H1) unsafeAtomicAdd(&g_base[k], v);
-    else unsafeAtomicAdd(&g_head[k - G_H1], v);
+  flush_grads(acc, g_base, g_head);
+}
+
+// ------------------------------------------------------------ cached mode
+// Differentiable cached render: classification of every occupied sample as
+// in the cache marcher (classify_sample); resample samples take the network
+// (k_diff_net_fwd / _bwd over list_r), cache samples the trilinear lookup of
+// the brick pool and the colour MLP (k_diffc_head_fwd / _bwd over list_c).
+// The adjoint of the lookup (generateNetworkInput_cache_shfl_perf_kernel_cg,
+// cache_performance_renderer.cu:678-787): with s = log(max(sum_c w_c
+// sigma_c, 1e-6)) and l_i = sum_c w_c l_c,i / sum_c w_c isset_c,
+//   dL/dsigma_c = w_c dL/ds / sum (0 when the clamp is active),
+//   dL/dl_c,i   = w_c dL/dl_i / sum_c w_c isset_c;
+// isset is a mask.  Voxel gradients go to g_pool [slot][512][10] (8 latents,
+// density, isset = 0) with FP32 atomics.
+__global__ __launch_bounds__(256) void k_diffc_count(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to,
+                                                     const int* __restrict__ pix, int n, MarchGrid mg, CacheView cv,
+                                                     const uint32_t* __restrict__ coarse_g,
+                                                     RayState* __restrict__ rays, int* __restrict__ counts_r,
+                                                     int* __restrict__ counts_c) {
+  __shared__ uint32_t coarse[16 * COARSE_WORDS];
+  load_coarse(coarse_g, mg.occ.nlvl, coarse);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int W = rm.cam_info.res_x;
+  RayState r;
+  const bool hit = init_ray_state(rm, aabb_from, aabb_to, pix[i] % W, pix[i] / W, r);
+  rays[i] = r;
+  int nr = 0, nc = 0;
+  if (hit) {
+    MarchRay mr = march_ray(r);
+    const V3 oc = transform(cv.cam.world2cam, mr.o.x, mr.o.y, mr.o.z, 1.0f);
+    const V3 dc = transform(cv.cam.world2cam, mr.d.x, mr.d.y, mr.d.z, 0.0f);
+    int bins = 0;
+    while (true) {
+      float t0;
+      const int res = march_trip(mg, coarse, mr, t0, bins);
+      if (res == TRIP_END) break;
+      if (res != TRIP_SAMPLE) continue;
+      const int k = classify_sample(cv, oc, dc, t0, step_dt(mg.st, t0));
+      nr += k == SAMPLE_RESAMPLE;
+      nc += k == SAMPLE_CACHE;
+    }
   }
+  counts_r[i] = nr;
+  counts_c[i] = nc;
+}
+
+__global__ __launch_bounds__(256) void k_diffc_emit(int n, MarchGrid mg, ContractParams con, CacheView cv,
+                                                    const uint32_t* __restrict__ coarse_g,
+                                                    const RayState* __restrict__ rays,
+                                                    const int* __restrict__ counts_r, const int* __restrict__ counts_c,
+                                                    const int64_t* __restrict__ off_r,
+                                                    const int64_t* __restrict__ off_c, int* __restrict__ counts,
+                                                    int64_t* __restrict__ offsets, float* __restrict__ s_pos,
+                                                    float* __restrict__ s_dt, int* __restrict__ s_ray,
+                                                    int* __restrict__ list_r, int* __restrict__ list_c) {
+  __shared__ uint32_t coarse[16 * COARSE_WORDS];
+  load_coarse(coarse_g, mg.occ.nlvl, coarse);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nr = counts_r[i], nc = counts_c[i];
+  int q = (int)(off_r[i] + off_c[i]);  // the batch holds fewer than 2^31 samples
+  counts[i] = nr + nc;
+  offsets[i] = q;
+  if (nr + nc == 0) return;
+  const RayState r = rays[i];
+  MarchRay mr = march_ray(r);
+  const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
+  const V3 oc = transform(cv.cam.world2cam, o.x, o.y, o.z, 1.0f);
+  const V3 dc = transform(cv.cam.world2cam, d.x, d.y, d.z, 0.0f);
+  int kr = (int)off_r[i], kc = (int)off_c[i];
+  const int end = q + nr + nc;
+  int bins = 0;
+  while (q < end) {
+    float t0;
+    const int res = march_trip(mg, coarse, mr, t0, bins);
+    if (res == TRIP_END) break;
+    if (res != TRIP_SAMPLE) continue;
+    const float dt = step_dt(mg.st, t0);
+    const int k = classify_sample(cv, oc, dc, t0, dt);
+    if (k == SAMPLE_SKIP) continue;
+    // network inputs at t0 + dt / 2 (cache_performance_renderer.cu:198-313)
+    const float tm2 = t0 + dt * 0.5f;
+    V3 p;
+    if (k == SAMPLE_RESAMPLE) {
+      p = contract(o + tm2 * d, con);
+      list_r[kr++] = q;
+    } else {
+      float tf;
+      p = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
+      list_c[kc++] = q;
+    }
+    s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
+    s_dt[q] = dt;
+    s_ray[q] = i;
+    q++;
+  }
+}
+
+// trilinear lookup of one sample by its 4 lanes: lane group g fetches the
+// corners (x, y) = (g >> 1, g & 1), z = 0 / 1; every lane ends with the sums
+struct LookupLane {
+  float w[2];
+  int64_t v[2];  // voxel index, -1: outside or unset brick
+  float wsum, dsum, lat[8];
+};
+__device__ __forceinline__ void lookup_f32(const CacheView& cv, V3 f, int g, LookupLane& L) {
+  const float fcx = f.x - 0.5f, fcy = f.y - 0.5f, fcz = f.z - 0.5f;
+  const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
+  const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
+  const float ox = (float)(g >> 1), oy = (float)(g & 1);
+  const float px = ix + ox, py = iy + oy;
+  const float wx = (1.0f - ox) * (1.0f - 2.0f * lx) + lx;
+  const float wy = (1.0f - oy) * (1.0f - 2.0f * ly) + ly;
+  const float wxy = wx * wy;
+  const bool in_xy = px >= 0.0f && px < (float)cv.fx && py >= 0.0f && py < (float)cv.fy;
+  float wd = 0.0f, dd = 0.0f, ld[8];
+#pragma unroll
+  for (int cc = 0; cc < 2; cc++) {
+    const float pz = iz + (float)cc;
+    const float wz = cc == 0 ? (1.0f - 2.0f * lz) + lz : lz;
+    const float w = wxy * wz;
+    L.w[cc] = w;
+    L.v[cc] = -1;
+    float cw = 0.0f, cd = 0.0f, cl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (in_xy && pz >= 0.0f && pz < (float)cv.fz) {
+      const int vx = (int)px, vy = (int)py, vz = (int)pz;
+      const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
+      if (slot >= 0) {
+        const int64_t v = (int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7));
+        L.v[cc] = v;
+        const Voxel vox = cv.pool[v];
+        cw = w * vox.isset;
+        cd = w * vox.density;
+#pragma unroll
+        for (int k = 0; k < 8; k++) cl[k] = w * h2f(vox.lat[k]);
+      }
+    }
+    if (cc == 0) {
+      wd = cw; dd = cd;
+#pragma unroll
+      for (int k = 0; k < 8; k++) ld[k] = cl[k];
+    } else {
+      wd = wd + cw; dd = dd + cd;
+#pragma unroll
+      for (int k = 0; k < 8; k++) ld[k] = ld[k] + cl[k];
+    }
+  }
+  L.wsum = xor32_sum(xor16_sum(wd));
+  L.dsum = xor32_sum(xor16_sum(dd));
+#pragma unroll
+  for (int k = 0; k < 8; k++) L.lat[k] = xor32_sum(xor16_sum(ld[k])) / L.wsum;
+}
+
+// base-output rows 4g..4g+3 of a cache sample: latents in rows 1..8
+__device__ __forceinline__ f4 lookup_rows(const LookupLane& L, int g) {
+  f4 r;
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const int k = 4 * g + e - 1;
+    float v = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v = k == i ? L.lat[i] : v;
+    r[e] = v;
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_diffc_head_fwd(int n, const int* __restrict__ list,
+                                                        const float* __restrict__ s_pos,
+                                                        const int* __restrict__ s_ray,
+                                                        const RayState* __restrict__ rays, CacheView cv,
+                                                        const float* __restrict__ frags,
+                                                        float4* __restrict__ out4) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  const int n_groups = (n + 15) / 16;
+  for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
+    const int j = grp * 16 + col;
+    const bool valid = j < n;
+    const int sl = list[valid ? j : n - 1];
+    const V3 f = {s_pos[3 * sl], s_pos[3 * sl + 1], s_pos[3 * sl + 2]};
+    const V3 d01 = ray_dir01(rays[s_ray[sl]]);
+    LookupLane L;
+    lookup_f32(cv, f, g, L);
+    FwdActs a;
+    head_forward(frags, lane, g, lookup_rows(L, g), d01, a);
+    if (g == 0 && valid) out4[sl] = make_float4(a.o[0], a.o[1], a.o[2], logf(fmaxf_(L.dsum, 1e-6f)));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_diffc_head_bwd(int n, const int* __restrict__ list,
+                                                        const float* __restrict__ s_pos,
+                                                        const int* __restrict__ s_ray,
+                                                        const RayState* __restrict__ rays, CacheView cv,
+                                                        const float* __restrict__ frags,
+                                                        const float4* __restrict__ d_out4,
+                                                        float* __restrict__ g_head, float* __restrict__ g_pool) {
+  __shared__ float acc[G_WORDS];
+  __shared__ __attribute__((aligned(16))) float stage[4][2][64 * STG];
+  for (int k = G_H1 + threadIdx.x; k < G_WORDS; k += blockDim.x) acc[k] = 0.0f;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  float* ys = stage[wave][0];
+  float* xs = stage[wave][1];
+  const int n_groups = (n + 15) / 16;
+  for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
+    const int j = grp * 16 + col;
+    const bool valid = j < n;
+    const int sl = list[valid ? j : n - 1];
+    const float4 dq = valid ? d_out4[sl] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const V3 f = {s_pos[3 * sl], s_pos[3 * sl + 1], s_pos[3 * sl + 2]};
+    const V3 d01 = ray_dir01(rays[s_ray[sl]]);
+    LookupLane L;
+    lookup_f32(cv, f, g, L);
+    FwdActs a;
+    head_forward(frags, lane, g, lookup_rows(L, g), d01, a);
+    f4 d_o, d_z1[4], d_xh;
+    head_backward(frags, lane, g, a, dq, d_o, d_z1, d_xh);
+    head_wgrad(ys, xs, a, d_o, d_z1, col, g, acc);
+    // dL/dl_i of latent i (row i + 1: lane group (i + 1) / 4, element (i + 1) % 4)
+    float d_lat[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) d_lat[i] = __shfl(d_xh[(i + 1) & 3], col + 16 * ((i + 1) >> 2), 64);
+    const float d_dsum = L.dsum > 1e-6f ? dq.w / L.dsum : 0.0f;
+    if (valid) {
+#pragma unroll
+      for (int cc = 0; cc < 2; cc++) {
+        if (L.v[cc] < 0) continue;
+        float* gv = g_pool + L.v[cc] * 10;
+        const float wl = L.w[cc] / L.wsum;
+#pragma unroll
+        for (int i = 0; i < 8; i++) unsafeAtomicAdd(&gv[i], wl * d_lat[i]);
+        unsafeAtomicAdd(&gv[8], L.w[cc] * d_dsum);
+      }
+    }
+  }
+  __syncthreads();
+  flush_head_grads(acc, g_head);
 }
 
 // ------------------------------------------------------------ host
@@ -580,25 +855,40 @@ static int persistent_blocks(int64_t n_groups) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, 256 * 8));
 }
 
+// exclusive scan of counts[0..n) into offs[0..n] (offs[n] = total)
+static int64_t scan_counts(DiffState& D, int* counts, int64_t* offs, int n, hipStream_t s) {
+  FVC_HIP(hipMemsetAsync(counts + n, 0, sizeof(int), s));
+  size_t tmp = 0;
+  FVC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, counts, offs, n + 1, s));
+  D.scan_tmp.ensure(std::max<size_t>(tmp, 16));
+  FVC_HIP(hipcub::DeviceScan::ExclusiveSum(D.scan_tmp.p, tmp, counts, offs, n + 1, s));
+  int64_t total = 0;
+  FVC_HIP(hipMemcpyAsync(&total, offs + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  FVC_HIP(hipStreamSynchronize(s));
+  return total;
+}
+
 void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int n, const float* pos,
-                  const float* base, const float* head, float* rgb, fvc_diff_stats* stats) {
+                  const float* base, const float* head, float* rgb, fvc_diff_stats* stats, bool cached) {
   const Model& m = *r.model;
   DiffState& D = r.diff;
   hipStream_t s = r.stream();
+  if (cached && !r.caches[r.render_idx].initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
   if (D.frag_index.p == nullptr) {
     const std::vector<int> t = diff_frag_index();
     D.frag_index.ensure(t.size() * sizeof(int));
     FVC_HIP(hipMemcpy(D.frag_index.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice));
     D.frags.ensure((size_t)N_DIFF_FRAGS * 64 * sizeof(float));
   }
+  D.pos = nullptr;  // set once the forward pass has completed
   D.n = n;
-  D.pos = pos;
-  D.base = base;
-  D.head = head;
-  D.rm = rm;
-  D.n_samples = 0;
+  D.cached = cached;
+  D.n_samples = D.n_r = D.n_c = 0;
   if (stats) std::memset(stats, 0, sizeof(*stats));
-  if (n == 0) return;
+  if (n == 0) {
+    D.pos = pos; D.base = base; D.head = head;
+    return;
+  }
   D.rays.ensure((size_t)n * sizeof(RayState));
   D.counts.ensure(((size_t)n + 1) * sizeof(int));
   D.offsets.ensure(((size_t)n + 1) * sizeof(int64_t));
@@ -610,21 +900,27 @@ void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int 
   const V3 af = {m.scene.aabb_from.x, m.scene.aabb_from.y, m.scene.aabb_from.z};
   const V3 at = {m.scene.aabb_to.x, m.scene.aabb_to.y, m.scene.aabb_to.z};
   const dim3 grid((n + 255) / 256);
-  FVC_HIP(hipMemsetAsync(D.counts.as<int>() + n, 0, sizeof(int), s));
-  hipLaunchKernelGGL(k_diff_count, grid, dim3(256), 0, s, rm, af, at, pix, n, mg, m.coarse.as<uint32_t>(),
-                     D.rays.as<RayState>(), D.counts.as<int>());
-  FVC_HIP(hipGetLastError());
-  size_t tmp = 0;
-  FVC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, D.counts.as<int>(), D.offsets.as<int64_t>(), n + 1, s));
-  D.scan_tmp.ensure(std::max<size_t>(tmp, 16));
-  FVC_HIP(hipcub::DeviceScan::ExclusiveSum(D.scan_tmp.p, tmp, D.counts.as<int>(), D.offsets.as<int64_t>(), n + 1, s));
   int64_t total64 = 0;
-  FVC_HIP(hipMemcpyAsync(&total64, D.offsets.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  FVC_HIP(hipStreamSynchronize(s));
-  if (total64 >= ((int64_t)1 << 31) / 3) {
-    D.pos = nullptr;
-    throw Error(FVC_E_INVALID, "more than 2^31 / 3 samples in one differentiable batch: pass fewer rays");
+  if (!cached) {
+    hipLaunchKernelGGL(k_diff_count, grid, dim3(256), 0, s, rm, af, at, pix, n, mg, m.coarse.as<uint32_t>(),
+                       D.rays.as<RayState>(), D.counts.as<int>());
+    FVC_HIP(hipGetLastError());
+    total64 = scan_counts(D, D.counts.as<int>(), D.offsets.as<int64_t>(), n, s);
+    D.n_r = total64;
+  } else {
+    D.cv = cache_view(r.caches[r.render_idx]);
+    D.counts_c.ensure(((size_t)n + 1) * sizeof(int));
+    D.off_r.ensure(((size_t)n + 1) * sizeof(int64_t));
+    D.off_c.ensure(((size_t)n + 1) * sizeof(int64_t));
+    hipLaunchKernelGGL(k_diffc_count, grid, dim3(256), 0, s, rm, af, at, pix, n, mg, D.cv, m.coarse.as<uint32_t>(),
+                       D.rays.as<RayState>(), D.counts.as<int>(), D.counts_c.as<int>());
+    FVC_HIP(hipGetLastError());
+    D.n_r = scan_counts(D, D.counts.as<int>(), D.off_r.as<int64_t>(), n, s);
+    D.n_c = scan_counts(D, D.counts_c.as<int>(), D.off_c.as<int64_t>(), n, s);
+    total64 = D.n_r + D.n_c;
   }
+  if (total64 >= ((int64_t)1 << 31) / 3)
+    throw Error(FVC_E_INVALID, "more than 2^31 / 3 samples in one differentiable batch: pass fewer rays");
   const int total = (int)total64;
   D.n_samples = total;
   const size_t ns = (size_t)std::max(total, 1);
@@ -633,43 +929,72 @@ void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int 
   D.s_ray.ensure(ns * sizeof(int));
   D.out4.ensure(ns * sizeof(float4));
   D.d_out4.ensure(ns * sizeof(float4));
-  hipLaunchKernelGGL(k_diff_emit, grid, dim3(256), 0, s, n, mg, m.con, m.coarse.as<uint32_t>(), D.rays.as<RayState>(),
-                     D.counts.as<int>(), D.offsets.as<int64_t>(), D.s_pos.as<float>(), D.s_dt.as<float>(),
-                     D.s_ray.as<int>());
+  if (!cached) {
+    hipLaunchKernelGGL(k_diff_emit, grid, dim3(256), 0, s, n, mg, m.con, m.coarse.as<uint32_t>(),
+                       D.rays.as<RayState>(), D.counts.as<int>(), D.offsets.as<int64_t>(), D.s_pos.as<float>(),
+                       D.s_dt.as<float>(), D.s_ray.as<int>());
+  } else {
+    D.list_r.ensure((size_t)std::max<int64_t>(D.n_r, 1) * sizeof(int));
+    D.list_c.ensure((size_t)std::max<int64_t>(D.n_c, 1) * sizeof(int));
+    D.counts_all.ensure((size_t)n * sizeof(int));
+    hipLaunchKernelGGL(k_diffc_emit, grid, dim3(256), 0, s, n, mg, m.con, D.cv, m.coarse.as<uint32_t>(),
+                       D.rays.as<RayState>(), D.counts.as<int>(), D.counts_c.as<int>(), D.off_r.as<int64_t>(),
+                       D.off_c.as<int64_t>(), D.counts_all.as<int>(), D.offsets.as<int64_t>(), D.s_pos.as<float>(),
+                       D.s_dt.as<float>(), D.s_ray.as<int>(), D.list_r.as<int>(), D.list_c.as<int>());
+  }
   FVC_HIP(hipGetLastError());
-  if (total > 0) {
-    hipLaunchKernelGGL(k_diff_net_fwd, dim3(persistent_blocks((total + 15) / 16)), dim3(256), 0, s, total,
-                       D.s_pos.as<float>(), D.s_ray.as<int>(), D.rays.as<RayState>(), pos, m.grid, D.frags.as<float>(),
-                       D.out4.as<float4>());
+  const int* list_r = cached ? D.list_r.as<int>() : nullptr;
+  if (D.n_r > 0) {
+    hipLaunchKernelGGL(k_diff_net_fwd, dim3(persistent_blocks((D.n_r + 15) / 16)), dim3(256), 0, s, (int)D.n_r,
+                       list_r, D.s_pos.as<float>(), D.s_ray.as<int>(), D.rays.as<RayState>(), pos, m.grid,
+                       D.frags.as<float>(), D.out4.as<float4>());
     FVC_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_diff_composite, grid, dim3(256), 0, s, n, m.scene.alpha_thre, D.counts.as<int>(),
-                     D.offsets.as<int64_t>(), D.out4.as<float4>(), D.s_dt.as<float>(), rgb, D.ray_out.as<float4>(),
-                     D.n_walk.as<int>());
+  if (D.n_c > 0) {
+    hipLaunchKernelGGL(k_diffc_head_fwd, dim3(persistent_blocks((D.n_c + 15) / 16)), dim3(256), 0, s, (int)D.n_c,
+                       D.list_c.as<int>(), D.s_pos.as<float>(), D.s_ray.as<int>(), D.rays.as<RayState>(), D.cv,
+                       D.frags.as<float>(), D.out4.as<float4>());
+    FVC_HIP(hipGetLastError());
+  }
+  const int* counts = cached ? D.counts_all.as<int>() : D.counts.as<int>();
+  hipLaunchKernelGGL(k_diff_composite, grid, dim3(256), 0, s, n, m.scene.alpha_thre, counts, D.offsets.as<int64_t>(),
+                     D.out4.as<float4>(), D.s_dt.as<float>(), rgb, D.ray_out.as<float4>(), D.n_walk.as<int>());
   FVC_HIP(hipGetLastError());
   FVC_HIP(hipStreamSynchronize(s));
+  D.pos = pos;
+  D.base = base;
+  D.head = head;
   if (stats) {
     stats->n_rays = n;
     stats->n_samples = total;
+    stats->n_cache_samples = D.n_c;
   }
 }
 
-void diff_backward(Renderer& r, const float* d_rgb, float* g_pos, float* g_base, float* g_head) {
+void diff_backward(Renderer& r, const float* d_rgb, float* g_pos, float* g_base, float* g_head, float* g_pool) {
   const Model& m = *r.model;
   DiffState& D = r.diff;
   if (D.pos == nullptr) throw Error(FVC_E_NOT_INITIALIZED, "no differentiable forward pass to differentiate");
+  if (D.cached && D.n_c > 0 && g_pool == nullptr) throw Error(FVC_E_INVALID, "the cached backward needs g_pool");
   hipStream_t s = r.stream();
   const int n = D.n;
   if (n == 0) return;
-  hipLaunchKernelGGL(k_diff_composite_bwd, dim3((n + 255) / 256), dim3(256), 0, s, n, m.scene.alpha_thre,
-                     D.counts.as<int>(), D.offsets.as<int64_t>(), D.n_walk.as<int>(), D.out4.as<float4>(),
-                     D.s_dt.as<float>(), D.ray_out.as<float4>(), d_rgb, D.d_out4.as<float4>());
+  const int* counts = D.cached ? D.counts_all.as<int>() : D.counts.as<int>();
+  hipLaunchKernelGGL(k_diff_composite_bwd, dim3((n + 255) / 256), dim3(256), 0, s, n, m.scene.alpha_thre, counts,
+                     D.offsets.as<int64_t>(), D.n_walk.as<int>(), D.out4.as<float4>(), D.s_dt.as<float>(),
+                     D.ray_out.as<float4>(), d_rgb, D.d_out4.as<float4>());
   FVC_HIP(hipGetLastError());
-  const int total = (int)D.n_samples;
-  if (total > 0) {
-    hipLaunchKernelGGL(k_diff_net_bwd, dim3(persistent_blocks((total + 15) / 16)), dim3(256), 0, s, total,
-                       D.s_pos.as<float>(), D.s_ray.as<int>(), D.rays.as<RayState>(), D.pos, m.grid,
-                       D.frags.as<float>(), D.d_out4.as<float4>(), g_pos, g_base, g_head);
+  if (D.n_r > 0) {
+    hipLaunchKernelGGL(k_diff_net_bwd, dim3(persistent_blocks((D.n_r + 15) / 16)), dim3(256), 0, s, (int)D.n_r,
+                       D.cached ? D.list_r.as<int>() : nullptr, D.s_pos.as<float>(), D.s_ray.as<int>(),
+                       D.rays.as<RayState>(), D.pos, m.grid, D.frags.as<float>(), D.d_out4.as<float4>(), g_pos,
+                       g_base, g_head);
+    FVC_HIP(hipGetLastError());
+  }
+  if (D.n_c > 0) {
+    hipLaunchKernelGGL(k_diffc_head_bwd, dim3(persistent_blocks((D.n_c + 15) / 16)), dim3(256), 0, s, (int)D.n_c,
+                       D.list_c.as<int>(), D.s_pos.as<float>(), D.s_ray.as<int>(), D.rays.as<RayState>(), D.cv,
+                       D.frags.as<float>(), D.d_out4.as<float4>(), g_head, g_pool);
     FVC_HIP(hipGetLastError());
   }
   FVC_HIP(hipStreamSynchronize(s));

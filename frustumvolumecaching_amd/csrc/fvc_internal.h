@@ -237,17 +237,30 @@ static const double ALG_DENSITY_FLOP = 13440, ALG_COLOUR_FLOP = 3456;
 
 // ---------------------------------------------------------------- renderer
 static const int MAX_STEPS_PER_LAUNCH = 32;   // cap of fvc_renderer_set_batching
+
+// the render cache as the kernels see it
+struct CacheView {
+  fvc_camera_info cam;
+  fvc_stepsize_info st;
+  int fx, fy, fz, bx, by;
+  const int* table;
+  const float* known;
+  const Voxel* pool;
+};
+
 // state of the differentiable render between forward and backward (diff.hip)
 struct DiffState {
   DevBuf frag_index, frags;        // FP32 MFMA fragments of the caller's weights
   DevBuf rays, counts, offsets, ray_out, n_walk, scan_tmp;
   DevBuf s_pos, s_dt, s_ray, out4, d_out4;
+  DevBuf counts_c, off_r, off_c, counts_all, list_r, list_c;  // cached mode
   int n = 0;
-  int64_t n_samples = 0;
-  const float* pos = nullptr;
+  bool cached = false;
+  int64_t n_samples = 0, n_r = 0, n_c = 0;
+  CacheView cv{};
+  const float* pos = nullptr;  // parameters of the last completed forward
   const float* base = nullptr;
   const float* head = nullptr;
-  fvc_raymarch_info rm{};
 };
 
 struct Renderer {
@@ -308,10 +321,10 @@ struct Renderer {
 void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats);
 void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats);
 void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats);
-// differentiable full render (diff.hip)
+// differentiable full / cached render (diff.hip)
 void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int n, const float* pos,
-                  const float* base, const float* head, float* rgb, fvc_diff_stats* stats);
-void diff_backward(Renderer& r, const float* d_rgb, float* g_pos, float* g_base, float* g_head);
+                  const float* base, const float* head, float* rgb, fvc_diff_stats* stats, bool cached);
+void diff_backward(Renderer& r, const float* d_rgb, float* g_pos, float* g_base, float* g_head, float* g_pool);
 void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts, float* t0s);
 void resize_renderbuffers(Renderer& r, int w, int h);
 

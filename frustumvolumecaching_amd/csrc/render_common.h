@@ -71,6 +71,43 @@ static const int CTL_WORDS = CTL_LOG + CTL_LOG_FIELDS * (MAX_SUBITER + 1);
 // capacity, performance_renderer.cu:249-250) and zero the counters.
 __global__ void k_plan(int* __restrict__ ctl, int it, int64_t target, int max_steps, int64_t cap, int log_only);
 
+inline CacheView cache_view(const CacheStore& C) {
+  CacheView cv;
+  cv.cam = C.rm.cam_info;
+  cv.st = C.rm.stepsize_info;
+  cv.fx = C.fx; cv.fy = C.fy; cv.fz = C.fz; cv.bx = C.bx; cv.by = C.by;
+  cv.table = C.table.as<int>();
+  cv.known = C.known.as<float>();
+  cv.pool = C.pool.as<Voxel>();
+  return cv;
+}
+
+// Class of an occupied sample [t0, t0 + dt) of a ray whose origin / direction
+// in the cache camera's frame are oc / dc (sample_cache_performance_kernel,
+// cache_performance_renderer.cu:96-160): SAMPLE_RESAMPLE outside the cache
+// frustum or outside [near, known_to] of its pixel, SAMPLE_CACHE when the
+// nearest voxel's brick is set and its density is > 0, else SAMPLE_SKIP.
+enum { SAMPLE_SKIP = 0, SAMPLE_RESAMPLE = 1, SAMPLE_CACHE = 2 };
+__device__ __forceinline__ int classify_sample(const CacheView& cv, V3 oc, V3 dc, float t0, float dt) {
+  const float t1 = t0 + dt;
+  const float tm = (t0 + t1) * 0.5f;
+  float tf;
+  const V3 f = cam2froxel(oc + tm * dc, cv.cam, cv.st, tf);
+  const bool in_fov = f.x >= 0.0f && f.x < (float)cv.fx && f.y >= 0.0f && f.y < (float)cv.fy;
+  const bool inside = in_fov && f.z >= 0.0f && f.z < (float)cv.fz;
+  bool unsampled = true;
+  if (in_fov) {
+    const float known = cv.known[(int64_t)(int)f.y * cv.fx + (int)f.x];
+    unsampled = tf < cv.st.near_ || tf > known;
+  }
+  if (!inside || unsampled) return SAMPLE_RESAMPLE;
+  const int vx = (int)floorf(f.x), vy = (int)floorf(f.y), vz = (int)floorf(f.z);
+  const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
+  if (slot >= 0 && cv.pool[(int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))].density > 0.0f)
+    return SAMPLE_CACHE;
+  return SAMPLE_SKIP;
+}
+
 __device__ __forceinline__ float h2f(uint32_t bits16) { return (float)__builtin_bit_cast(_Float16, (uint16_t)bits16); }
 
 // One step of front-to-back compositing (accumulate_performance_kernel,

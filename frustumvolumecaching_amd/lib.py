@@ -65,6 +65,9 @@ _SIGS = {
     "fvc_diff_forward": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.c_int32, _P, _P, _P, _P,
                                    C.POINTER(abi.DiffStats)]),
     "fvc_diff_backward": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fvc_diff_cached_forward": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.c_int32, _P, _P, _P, _P,
+                                          C.POINTER(abi.DiffStats)]),
+    "fvc_diff_cached_backward": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "fvc_profile_enable": (C.c_int, [_P, C.c_int]),
     "fvc_profile_get": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
 }
@@ -313,6 +316,21 @@ class Renderer:
         """Adds d(sum(d_rgb * rgb))/d(params) of the last diff_forward to grads."""
         g_pos, g_base, g_head = grads
         check(load().fvc_diff_backward(self.h, _addr(d_rgb), _addr(g_pos), _addr(g_base), _addr(g_head)))
+
+    def diff_cached_forward(self, rm, pix, params, rgb):
+        """diff_forward against the render cache (resample / cache samples)."""
+        st = abi.DiffStats()
+        pos, base, head = params
+        check(load().fvc_diff_cached_forward(self.h, C.byref(rm), _addr(pix), int(pix.numel()), _addr(pos),
+                                             _addr(base), _addr(head), _addr(rgb), C.byref(st)))
+        return st
+
+    def diff_cached_backward(self, d_rgb, grads, g_pool):
+        """diff_backward of the last cached forward; g_pool [bricks, 512, 10]
+        receives the voxel latent / density gradients."""
+        g_pos, g_base, g_head = grads
+        check(load().fvc_diff_cached_backward(self.h, _addr(d_rgb), _addr(g_pos), _addr(g_base), _addr(g_head),
+                                              _addr(g_pool)))
 
     def debug_march(self, rm, max_per_ray=64):
         import numpy as np

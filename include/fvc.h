@@ -161,7 +161,8 @@ typedef struct {
 /* Differentiable render statistics */
 typedef struct {
   int64_t n_rays;
-  int64_t n_samples;              /* occupied samples marched (all evaluated forward) */
+  int64_t n_samples;              /* samples evaluated forward (all occupied samples marched) */
+  int64_t n_cache_samples;        /* of which from the cache (cached render) */
 } fvc_diff_stats;
 
 /* Frustum cache geometry (Cache::frustumDims / brickArrayDims, cache.h:70-73) */
@@ -283,6 +284,19 @@ int fvc_diff_forward(fvc_renderer* renderer, const fvc_raymarch_info* rm, const 
                      fvc_diff_stats* stats);
 int fvc_diff_backward(fvc_renderer* renderer, const float* d_rgb, float* g_pos_enc, float* g_mlp_base,
                       float* g_mlp_head);
+/* The same for CachePerformanceRenderer::render's per-ray semantics
+ * (cache_performance_renderer.cu:22-313, 789-908) against the renderer's
+ * render cache: resampled samples take the network, cache samples the
+ * trilinear lookup of the brick pool and the colour MLP.  The backward also
+ * adds, through the adjoint of the lookup, the gradients of each voxel's 8
+ * latents and density to g_pool, float [n_bricks_used][512][10] (per voxel
+ * in the pool's order: 8 latents, density, isset; isset, a mask, gets
+ * none). */
+int fvc_diff_cached_forward(fvc_renderer* renderer, const fvc_raymarch_info* rm, const int32_t* pix, int32_t n_rays,
+                            const float* pos_enc, const float* mlp_base, const float* mlp_head, float* rgb,
+                            fvc_diff_stats* stats);
+int fvc_diff_cached_backward(fvc_renderer* renderer, const float* d_rgb, float* g_pos_enc, float* g_mlp_base,
+                             float* g_mlp_head, float* g_pool);
 
 /* ---- profiling ---- */
 /* Enable (1) or disable (0) per-kernel timing; enabling clears the table. */

@@ -1045,18 +1045,11 @@ template <class R> void encode_fgeo(const Net& n, V3<float> p, R* out) {
   }
 }
 
-template <class R> void net_fwd(const Net& n, V3<float> p, V3<R> dir01, Acts<R>& a) {
-  const int Wb = n.d.base_width, Wh = n.d.head_width;
-  encode_fgeo(n, p, a.enc);
-  R h[128];
-  dense(n.W1, Wb, 32, a.enc, a.a1);
-  for (int k = 0; k < Wb; k++) h[k] = relu(a.a1[k]);
-  dense(n.W2, Wb, Wb, h, a.a2);
-  for (int k = 0; k < Wb; k++) h[k] = relu(a.a2[k]);
-  dense(n.W3, 16, Wb, h, a.out);
-  const int NL = n.d.n_latents;
-  for (int k = 0; k < NL; k++) a.xh[k] = a.out[n.d.latent_offset + k];
-  R sh[16];
+// colour MLP on latents lat[n_latents]: a.xh, a.z1, a.o
+template <class R> void head_fwd(const Net& n, const R* lat, V3<R> dir01, Acts<R>& a) {
+  const int Wh = n.d.head_width, NL = n.d.n_latents;
+  for (int k = 0; k < NL; k++) a.xh[k] = lat[k];
+  R sh[16], h[128];
   sh4(dir01, sh);
   for (int k = 0; k < 16; k++) a.xh[NL + k] = sh[k];
   for (int k = NL + 16; k < 32; k++) a.xh[k] = R(1);
@@ -1065,16 +1058,10 @@ template <class R> void net_fwd(const Net& n, V3<float> p, V3<R> dir01, Acts<R>&
   dense(n.H2, 16, Wh, h, a.o);
 }
 
-struct DiffGrads {
-  std::vector<double> base, head;  // per thread
-};
-
-template <class R>
-void net_bwd(const Net& n, V3<float> p, const Acts<R>& a, const R* d_rgb3, R d_sraw, double* gb, double* gh,
-             double* gpos) {
-  const int Wb = n.d.base_width, Wh = n.d.head_width, NL = n.d.n_latents;
-  // colour MLP
-  R hh[128], d_z1[128], d_xh[32];
+// colour MLP backward: gradients of H1 | H2 into gh, dL/dlatents into d_lat
+template <class R> void head_bwd(const Net& n, const Acts<R>& a, const R* d_rgb3, double* gh, R* d_lat) {
+  const int Wh = n.d.head_width, NL = n.d.n_latents;
+  R hh[128], d_z1[128];
   for (int k = 0; k < Wh; k++) hh[k] = relu(a.z1[k]);
   double* gH1 = gh;
   double* gH2 = gh + 32 * Wh;
@@ -1087,11 +1074,36 @@ void net_bwd(const Net& n, V3<float> p, const Acts<R>& a, const R* d_rgb3, R d_s
   }
   for (int j = 0; j < Wh; j++)
     for (int k = 0; k < 32; k++) gH1[j * 32 + k] += (double)(d_z1[j] * a.xh[k]);
-  for (int k = 0; k < 32; k++) {
+  for (int k = 0; k < NL; k++) {
     R s = 0;
     for (int j = 0; j < Wh; j++) s += (R)n.H1[(size_t)j * 32 + k] * d_z1[j];
-    d_xh[k] = s;
+    d_lat[k] = s;
   }
+}
+
+template <class R> void net_fwd(const Net& n, V3<float> p, V3<R> dir01, Acts<R>& a) {
+  const int Wb = n.d.base_width;
+  encode_fgeo(n, p, a.enc);
+  R h[128];
+  dense(n.W1, Wb, 32, a.enc, a.a1);
+  for (int k = 0; k < Wb; k++) h[k] = relu(a.a1[k]);
+  dense(n.W2, Wb, Wb, h, a.a2);
+  for (int k = 0; k < Wb; k++) h[k] = relu(a.a2[k]);
+  dense(n.W3, 16, Wb, h, a.out);
+  head_fwd(n, a.out + n.d.latent_offset, dir01, a);
+}
+
+struct DiffGrads {
+  std::vector<double> base, head;  // per thread
+};
+
+template <class R>
+void net_bwd(const Net& n, V3<float> p, const Acts<R>& a, const R* d_rgb3, R d_sraw, double* gb, double* gh,
+             double* gpos) {
+  const int Wb = n.d.base_width, NL = n.d.n_latents;
+  // colour MLP
+  R d_xh[32];
+  head_bwd(n, a, d_rgb3, gh, d_xh);
   // density MLP
   R d_out[16] = {};
   d_out[0] = d_sraw;
@@ -1201,6 +1213,182 @@ int64_t diff_ray(const Model& M, const Net& n, const fvc_raymarch_info& rm, int 
   return n_eval;
 }
 
+// Differentiable cached render (CachePerformanceRenderer::render per ray:
+// classification cache_performance_renderer.cu:22-177, network inputs
+// 198-313, merged compositing 789-908) of a float cache.  Resampled samples
+// take the full network as in diff_ray; cache samples the trilinear lookup
+// of cache_lookup and the colour MLP.  The backward reaches the network
+// parameters and, through the lookup, each voxel's latents and density:
+//   log-density input  s = log(max(sum_c w_c sigma_c, 1e-6)):
+//                      ds/dsigma_c = w_c / sum (0 when the clamp is active);
+//   latent inputs      l_i = sum_c w_c l_c,i / sum_c w_c isset_c:
+//                      dl_i/dl_c,i = w_c / sum_c w_c isset_c.
+// isset is a mask (no gradient).  Sample geometry (march, froxels, lookup
+// fractions) is float; the rest R.
+template <class R> struct LookupFwd {
+  size_t v[8];    // voxel of corner c, or SIZE_MAX
+  R w[8];
+  R wsum, dsum, lat[8], sraw;
+};
+
+template <class R> void lookup_fwd(const Cache<float>& C, V3<float> f, LookupFwd<R>& L) {
+  const float fc[3] = {f.x - 0.5f, f.y - 0.5f, f.z - 0.5f};
+  float i0[3], lt[3];
+  for (int k = 0; k < 3; k++) { i0[k] = std::floor(fc[k]); lt[k] = fc[k] - i0[k]; }
+  R wd[8], dd[8], ld[8][8];
+  for (int c = 0; c < 8; c++) {
+    const int off[3] = {(c >> 2) & 1, (c >> 1) & 1, c & 1};
+    float p[3];
+    R wk[3];
+    for (int k = 0; k < 3; k++) {
+      p[k] = i0[k] + (float)off[k];
+      wk[k] = (R(1) - (R)off[k]) * (R(1) - R(2) * (R)lt[k]) + (R)lt[k];
+    }
+    const R w = wk[0] * wk[1] * wk[2];
+    L.w[c] = w;
+    L.v[c] = SIZE_MAX;
+    wd[c] = dd[c] = 0;
+    for (int i = 0; i < 8; i++) ld[c][i] = 0;
+    const bool inside = p[0] >= 0.0f && p[0] < (float)C.fx && p[1] >= 0.0f && p[1] < (float)C.fy && p[2] >= 0.0f &&
+                        p[2] < (float)C.fz;
+    if (!inside) continue;
+    const int vx = (int)p[0], vy = (int)p[1], vz = (int)p[2];
+    const int slot = C.table[((size_t)(vz / BS) * C.by + vy / BS) * C.bx + vx / BS];
+    if (slot < 0) continue;
+    const size_t v = (size_t)slot * 512 + (size_t)((vz % BS) * 64 + (vy % BS) * 8 + vx % BS);
+    L.v[c] = v;
+    wd[c] = w * (R)C.isset[v];
+    dd[c] = w * (R)C.density[v];
+    for (int i = 0; i < 8; i++) ld[c][i] = w * (R)C.lat[v * 8 + i];
+  }
+  auto tree = [](const R* a) { return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])); };
+  L.wsum = tree(wd);
+  L.dsum = tree(dd);
+  L.sraw = std::log(std::max(L.dsum, R(1e-6)));
+  for (int i = 0; i < 8; i++) {
+    R a[8];
+    for (int c = 0; c < 8; c++) a[c] = ld[c][i];
+    L.lat[i] = tree(a) / L.wsum;
+  }
+}
+
+template <class R>
+int64_t diff_ray_cached(const Model& M, const Net& n, const Cache<float>& C, const fvc_raymarch_info& rm, int px,
+                        int py, R* rgb, const R* d_rgb, double* gb, double* gh, double* gpos, double* gpool,
+                        int64_t* n_cache) {
+  const Step<float> st(rm.stepsize_info);
+  const Step<float> cst(C.rm.stepsize_info);
+  const Mat34<float> w2c(C.rm.cam_info.world2cam);
+  const fvc_scene_info& sc = M.scene;
+  const RayInit<float> ri = init_ray<float>(rm, sc, px, py);
+  struct Rec { float dt; R w, T1, c[3], sigma; bool cache; V3<float> q; };
+  std::vector<Rec> used;
+  R Cc[3] = {0, 0, 0}, T = 1;
+  int64_t n_eval = 0;
+  const V3<R> dir01 = unit_to_01(V3<R>{(R)ri.ray.d.x, (R)ri.ray.d.y, (R)ri.ray.d.z});
+  if (ri.alive) {
+    const V3<float> oc = w2c.transform(ri.ray.o.x, ri.ray.o.y, ri.ray.o.z, 1.0f);
+    const V3<float> dc = w2c.transform(ri.ray.d.x, ri.ray.d.y, ri.ray.d.z, 0.0f);
+    float t1 = ri.start;
+    while (true) {
+      const float t0 = t1;
+      const float dt = st.dt(t0);
+      t1 = t0 + dt;
+      const float tm = (t0 + t1) * 0.5f;
+      if (tm > ri.far_) break;
+      if (!occupied(ri.ray.o + tm * ri.ray.d, sc, M.bitfield.data())) continue;
+      float tf;
+      const V3<float> f = cam2froxel(oc + tm * dc, C.rm.cam_info, cst, tf);
+      const bool in_fov = f.x >= 0.0f && f.x < (float)C.fx && f.y >= 0.0f && f.y < (float)C.fy;
+      const bool inside = in_fov && f.z >= 0.0f && f.z < (float)C.fz;
+      bool unsampled = true;
+      if (in_fov) {
+        const float known = C.known_to[(size_t)(int)f.y * C.fx + (int)f.x];
+        unsampled = tf < cst.near_ || tf > known;
+      }
+      const float tm2 = t0 + dt * 0.5f;
+      Acts<R> a;
+      Rec r;
+      R sraw;
+      if (!inside || unsampled) {
+        r.cache = false;
+        r.q = contract(ri.ray.o + tm2 * ri.ray.d, sc);
+        net_fwd(n, r.q, dir01, a);
+        sraw = a.out[0];
+      } else {
+        const int vx = (int)std::floor(f.x), vy = (int)std::floor(f.y), vz = (int)std::floor(f.z);
+        const int slot = C.table[((size_t)(vz / BS) * C.by + vy / BS) * C.bx + vx / BS];
+        if (slot < 0) continue;
+        const size_t v = (size_t)slot * 512 + (size_t)((vz % BS) * 64 + (vy % BS) * 8 + vx % BS);
+        if (!(C.density[v] > 0.0f)) continue;
+        float tf2;
+        r.cache = true;
+        r.q = cam2froxel(oc + tm2 * dc, C.rm.cam_info, cst, tf2);
+        LookupFwd<R> L;
+        lookup_fwd(C, r.q, L);
+        head_fwd(n, L.lat, dir01, a);
+        sraw = L.sraw;
+        if (n_cache) (*n_cache)++;
+      }
+      n_eval++;
+      const R sigma = std::exp(sraw);
+      R alpha = R(1) - std::exp(-sigma * (R)dt);
+      alpha = std::min(std::max(alpha, R(0)), R(1));
+      if (alpha > (R)sc.alpha_thre) {
+        r.dt = dt;
+        r.sigma = sigma;
+        r.w = alpha * T;
+        for (int c = 0; c < 3; c++) {
+          r.c[c] = R(1) / (R(1) + std::exp(-a.o[c]));
+          Cc[c] = Cc[c] + r.w * r.c[c];
+        }
+        T = T * (R(1) - alpha);
+        r.T1 = T;
+        used.push_back(r);
+      }
+      if (T < R(1e-4)) break;
+    }
+  }
+  for (int c = 0; c < 3; c++) rgb[c] = Cc[c] + T;
+  if (!d_rgb) return n_eval;
+  R prefix[3] = {0, 0, 0};
+  for (const Rec& r : used) {
+    R dsig = 0, dcr[3];
+    for (int c = 0; c < 3; c++) {
+      prefix[c] = prefix[c] + r.w * r.c[c];
+      const R S = rgb[c] - prefix[c];
+      dsig += d_rgb[c] * (r.T1 * r.c[c] - S);
+      dcr[c] = d_rgb[c] * r.w * r.c[c] * (R(1) - r.c[c]);
+    }
+    const R d_sraw = dsig * (R)r.dt * r.sigma;
+    Acts<R> a;
+    if (!r.cache) {
+      net_fwd(n, r.q, dir01, a);
+      net_bwd(n, r.q, a, dcr, d_sraw, gb, gh, gpos);
+      continue;
+    }
+    LookupFwd<R> L;
+    lookup_fwd(C, r.q, L);
+    head_fwd(n, L.lat, dir01, a);
+    R d_lat[8];
+    head_bwd(n, a, dcr, gh, d_lat);
+    const R d_dsum = L.dsum > R(1e-6) ? d_sraw / L.dsum : R(0);
+    for (int c = 0; c < 8; c++) {
+      if (L.v[c] == SIZE_MAX) continue;
+      double* g = gpool + L.v[c] * 10;
+      for (int i = 0; i < 8; i++) {
+        const double dv = (double)(L.w[c] * d_lat[i] / L.wsum);
+#pragma omp atomic
+        g[i] += dv;
+      }
+      const double dd = (double)(L.w[c] * d_dsum);
+#pragma omp atomic
+      g[8] += dd;
+    }
+  }
+  return n_eval;
+}
+
 // Network with FP32 parameter blocks in the nerfacc layout
 inline Net net_with_params(const Net& base, const float* pos, const float* mb, const float* mh) {
   Net n = base;
@@ -1212,6 +1400,40 @@ inline Net net_with_params(const Net& base, const float* pos, const float* mb, c
   n.H1.assign(mh, mh + 32 * Wh);
   n.H2.assign(mh + 32 * Wh, mh + 32 * Wh + 16 * Wh);
   return n;
+}
+
+template <class R>
+int64_t diff_render_cached(const Model& M, const Cache<float>& C, const fvc_raymarch_info& rm, const int32_t* pix,
+                           int64_t n_rays, const float* pos, const float* mb, const float* mh, double* rgb_out,
+                           const double* d_rgb, double* g_pos, double* g_base, double* g_head, double* g_pool,
+                           int64_t* n_cache) {
+  const Net n = net_with_params(M.net, pos, mb, mh);
+  const int W = rm.cam_info.res_x;
+  int64_t total = 0, tc = 0;
+  const size_t nb = (size_t)n.n_base, nh = (size_t)n.n_head;
+#pragma omp parallel reduction(+ : total, tc)
+  {
+    std::vector<double> gb(d_rgb ? nb : 0, 0.0), gh(d_rgb ? nh : 0, 0.0);
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t i = 0; i < n_rays; i++) {
+      R rgb[3], dr[3];
+      if (d_rgb) for (int c = 0; c < 3; c++) dr[c] = (R)d_rgb[3 * i + c];
+      int64_t nc = 0;
+      total += diff_ray_cached<R>(M, n, C, rm, pix[i] % W, pix[i] / W, rgb, d_rgb ? dr : nullptr, gb.data(),
+                                  gh.data(), g_pos, g_pool, &nc);
+      tc += nc;
+      for (int c = 0; c < 3; c++) rgb_out[3 * i + c] = (double)rgb[c];
+    }
+    if (d_rgb) {
+#pragma omp critical
+      {
+        for (size_t k = 0; k < nb; k++) g_base[k] += gb[k];
+        for (size_t k = 0; k < nh; k++) g_head[k] += gh[k];
+      }
+    }
+  }
+  if (n_cache) *n_cache = tc;
+  return total;
 }
 
 template <class R>
@@ -1510,6 +1732,46 @@ int64_t orc_diff_render(const Model* M, const fvc_raymarch_info* rm, const int32
                         double* g_pos, double* g_base, double* g_head) {
   if (f64) return diff_render<double>(*M, *rm, pix, n, pos, base, head, rgb, d_rgb, g_pos, g_base, g_head);
   return diff_render<float>(*M, *rm, pix, n, pos, base, head, rgb, d_rgb, g_pos, g_base, g_head);
+}
+
+// Float cache from exported arrays: table [bz][by][bx], known [fy][fx],
+// pool [n_slots][512][10] (8 latents, density, isset)
+CacheHandle* orc_cache_from_arrays(const fvc_raymarch_info* rm, const fvc_cache_info* info, const int32_t* table,
+                                   const float* known, const float* pool, int64_t n_slots) {
+  auto* h = new CacheHandle();
+  h->mode = M_F32;
+  auto* C = new Cache<float>();
+  C->rm = *rm;
+  C->fx = info->frustum_x; C->fy = info->frustum_y; C->fz = info->frustum_z;
+  C->bx = info->bricks_x; C->by = info->bricks_y; C->bz = info->bricks_z;
+  C->capacity = (int)n_slots;
+  C->n_used = (int)n_slots;
+  C->table.assign(table, table + (size_t)C->bx * C->by * C->bz);
+  C->known_to.assign(known, known + (size_t)C->fx * C->fy);
+  C->lat.resize((size_t)n_slots * 512 * 8);
+  C->density.resize((size_t)n_slots * 512);
+  C->isset.resize((size_t)n_slots * 512);
+  for (int64_t v = 0; v < n_slots * 512; v++) {
+    for (int i = 0; i < 8; i++) C->lat[v * 8 + i] = pool[v * 10 + i];
+    C->density[v] = pool[v * 10 + 8];
+    C->isset[v] = pool[v * 10 + 9];
+  }
+  h->f.reset(C);
+  return h;
+}
+
+// Differentiable cached render with a float cache (diff_ray_cached);
+// g_pool [n_slots][512][10] receives latent (0..7) and density (8) gradients.
+int64_t orc_diff_render_cached(const Model* M, const CacheHandle* h, const fvc_raymarch_info* rm, const int32_t* pix,
+                               int64_t n, const float* pos, const float* base, const float* head, int f64, double* rgb,
+                               const double* d_rgb, double* g_pos, double* g_base, double* g_head, double* g_pool,
+                               int64_t* n_cache) {
+  if (!h->f) return -1;
+  if (f64)
+    return diff_render_cached<double>(*M, *h->f, *rm, pix, n, pos, base, head, rgb, d_rgb, g_pos, g_base, g_head,
+                                      g_pool, n_cache);
+  return diff_render_cached<float>(*M, *h->f, *rm, pix, n, pos, base, head, rgb, d_rgb, g_pos, g_base, g_head,
+                                   g_pool, n_cache);
 }
 
 }  // extern "C"

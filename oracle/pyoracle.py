@@ -77,6 +77,10 @@ def lib():
             "orc_cache_lookup": (None, [P, P, C.c_int64, P]),
             "orc_diff_render": (C.c_int64, [P, C.POINTER(abi.RaymarchInfo), P, C.c_int64, P, P, P, C.c_int, P, P,
                                             P, P, P]),
+            "orc_cache_from_arrays": (P, [C.POINTER(abi.RaymarchInfo), C.POINTER(abi.CacheInfo), P, P, P,
+                                          C.c_int64]),
+            "orc_diff_render_cached": (C.c_int64, [P, P, C.POINTER(abi.RaymarchInfo), P, C.c_int64, P, P, P, C.c_int,
+                                                   P, P, P, P, P, P, C.POINTER(C.c_int64)]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -218,14 +222,50 @@ class Model:
 
 
 class Cache:
-    def __init__(self, model, rm, mode, capacity, crop=None):
+    def __init__(self, model, rm, mode, capacity, crop=None, _handle=None):
         self.model = model
         self.stats = abi.InitStats()
-        cr = None if crop is None else np.asarray(crop, np.int32)
-        self.h = lib().orc_cache_fill(model.h, C.byref(rm), mode, capacity, None if cr is None else _p(cr),
-                                      C.byref(self.stats))
+        if _handle is not None:
+            self.h = _handle
+        else:
+            cr = None if crop is None else np.asarray(crop, np.int32)
+            self.h = lib().orc_cache_fill(model.h, C.byref(rm), mode, capacity, None if cr is None else _p(cr),
+                                          C.byref(self.stats))
         self.info = abi.CacheInfo()
         lib().orc_cache_info(self.h, C.byref(self.info))
+
+    @classmethod
+    def from_arrays(cls, model, rm, info, table, known, pool):
+        """Float cache with the given contents (the export layouts of
+        brick_table, known_ranges and pool)."""
+        table = np.ascontiguousarray(table, np.int32)
+        known = np.ascontiguousarray(known, np.float32)
+        pool = np.ascontiguousarray(pool, np.float32)
+        n_slots = pool.size // (512 * 10)
+        h = lib().orc_cache_from_arrays(C.byref(rm), C.byref(info), _p(table), _p(known), _p(pool), n_slots)
+        return cls(model, rm, F32, 0, _handle=h)
+
+    def diff_render(self, rm, pix, params32, f64=True, d_rgb=None):
+        """Differentiable cached render (float cache).  Returns (rgb [n, 3],
+        grads or None, n_evaluations, n_cache_samples); grads = (g_pos, g_base,
+        g_head, g_pool [slots, 512, 10]) of sum(d_rgb * rgb)."""
+        pix = np.ascontiguousarray(pix, dtype=np.int32)
+        pos, base, head = (np.ascontiguousarray(p, dtype=np.float32) for p in params32)
+        rgb = np.zeros((len(pix), 3), np.float64)
+        grads, dr = None, None
+        if d_rgb is not None:
+            dr = np.ascontiguousarray(d_rgb, dtype=np.float64).reshape(len(pix), 3)
+            n_slots = max(self.info.n_bricks_used, 1)
+            grads = tuple(np.zeros(p.size, np.float64) for p in (pos, base, head)) + (
+                np.zeros((n_slots, 512, 10), np.float64),)
+        nc = C.c_int64()
+        n = lib().orc_diff_render_cached(self.model.h, self.h, C.byref(rm), _p(pix), len(pix), _p(pos), _p(base),
+                                         _p(head), int(f64), _p(rgb), None if dr is None else _p(dr),
+                                         *(None if grads is None else _p(g) for g in (grads or (None,) * 4)),
+                                         C.byref(nc))
+        if n < 0:
+            raise ValueError("differentiable cached render needs a float cache")
+        return rgb, grads, n, nc.value
 
     def __del__(self):
         if getattr(self, "h", None):
