@@ -483,6 +483,34 @@ struct KHead {
   }
 };
 
+// Adds (v0, v1) to entry `key` of the hash-table gradient for the 16
+// samples of a lane row (lane col = sample).  Consecutive samples of a tile
+// lie along one ray, so at the coarse levels they share corners: a
+// segmented inclusive scan over runs of equal keys sums each run, and only
+// the run's last lane issues the two FP32 atomics.
+__device__ __forceinline__ void scatter_run(float* __restrict__ g_pos, uint32_t key, float v0, float v1, int col,
+                                            bool valid) {
+  const uint32_t kprev = __shfl_up(key, 1, 16);
+  const bool vprev = __shfl_up((int)valid, 1, 16) != 0;
+  int f = (col == 0 || kprev != key || !vprev || !valid) ? 1 : 0;  // run head
+#pragma unroll
+  for (int d = 1; d < 16; d <<= 1) {
+    const float u0 = __shfl_up(v0, d, 16), u1 = __shfl_up(v1, d, 16);
+    const int fu = __shfl_up(f, d, 16);
+    if (col >= d) {
+      if (!f) { v0 += u0; v1 += u1; }
+      f |= fu;
+    }
+  }
+  const uint32_t knext = __shfl_down(key, 1, 16);
+  const bool vnext = __shfl_down((int)valid, 1, 16) != 0;
+  const bool tail = col == 15 || knext != key || !vnext;
+  if (valid && tail && (v0 != 0.0f || v1 != 0.0f)) {
+    unsafeAtomicAdd(&g_pos[2 * (size_t)key], v0);
+    unsafeAtomicAdd(&g_pos[2 * (size_t)key + 1], v1);
+  }
+}
+
 // colour MLP backward of a tile: d_o (output rows 4g..4g+3), d_z1
 // (hidden pre-activations) and d_xh (head inputs in the rows of the
 // base output, row r = latent r - 1 for r in 1..8)
@@ -580,20 +608,16 @@ __global__ __launch_bounds__(256) void k_diff_net_bwd(int n, const int* __restri
       for (int e = 0; e < 4; e++) x[4 * nb + e] = d_a1[nb][e];
     dense4<2, 16>(frags + B_W1T * 64, lane, x, d_enc);
     // hash-grid scatter: feature slot 2j + k of level 4j + g is d_enc[j / 2][2 (j % 2) + k]
-    if (valid) {
+    {
       uint32_t idx[4][8];
       float w[4][8];
       quad_corners(lv, gp, p, g, idx, w);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const float d0 = d_enc[j >> 1][2 * (j & 1)], d1 = d_enc[j >> 1][2 * (j & 1) + 1];
-        if (d0 != 0.0f || d1 != 0.0f) {
+        const float d0 = valid ? d_enc[j >> 1][2 * (j & 1)] : 0.0f;
+        const float d1 = valid ? d_enc[j >> 1][2 * (j & 1) + 1] : 0.0f;
 #pragma unroll
-          for (int c = 0; c < 8; c++) {
-            unsafeAtomicAdd(&g_pos[2 * (size_t)idx[j][c]], w[j][c] * d0);
-            unsafeAtomicAdd(&g_pos[2 * (size_t)idx[j][c] + 1], w[j][c] * d1);
-          }
-        }
+        for (int c = 0; c < 8; c++) scatter_run(g_pos, idx[j][c], w[j][c] * d0, w[j][c] * d1, col, valid);
       }
     }
     // weight gradients (activations as the forward's B slots)

@@ -424,6 +424,8 @@ void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int3
   DevBuf c, t;
   c.ensure(n * sizeof(int));
   t.ensure(std::max<int64_t>(1, n * max_per_ray) * sizeof(float));
+  // slots past a ray's count read as 0
+  FVC_HIP(hipMemsetAsync(t.p, 0, std::max<int64_t>(1, n * max_per_ray) * sizeof(float), s));
   hipLaunchKernelGGL(k_debug_march, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, cvt(m.scene.aabb_from),
                      cvt(m.scene.aabb_to), mg, m.coarse.as<uint32_t>(), max_per_ray, c.as<int>(), t.as<float>());
   FVC_HIP(hipGetLastError());
