@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--precision", choices=["fp16", "bf16", "fp8"], default="fp16",
                     help="MLP arithmetic (operands; FP32 accumulation)")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the library's default launch shapes")
     ap.add_argument("--diff-rays", type=int, default=1 << 18, help="rays per differentiable step (800x800 view)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference point for rocprofv3 "
@@ -111,6 +112,14 @@ def main():
         r.render_full(rm_of(1), img)
     torch.cuda.synchronize()
     log(f"[bench] warm-up done ({a.warmup} rounds)")
+    tune = None
+    if not a.no_tune and not (a.batch or a.max_steps):
+        # autotuner: MLP rows per wavefront + chunk sizes, on a reprojected frame
+        r.initCacheBlockwise(rm_of(0))
+        t = r.autotune(rm_of(1), cached=True)
+        tune = {"mlp_rows": t.mlp_rows, "target_batch": int(t.target_batch), "max_steps_per_ray": t.max_steps,
+                "ms_best": round(t.ms_best, 3), "ms_default": round(t.ms_default, 3)}
+        log(f"[bench] autotune: {tune}")
 
     # ---- timed region: cached path (profiled with HIP events)
     fill_ms, cached_ms, fill_stats, cached_stats = [], [], [], []
@@ -264,6 +273,7 @@ def main():
         "kernels_cached": kernel_table(prof_cached),
         "kernels_full": kernel_table(prof_full),
         "cpu_baseline": cpu,
+        "autotune": tune,
         "diff": None if a.no_diff else diff_leg(a),
     }
     print(json.dumps(out), flush=True)

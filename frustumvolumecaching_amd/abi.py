@@ -104,6 +104,17 @@ class InitStats(C.Structure):
                 ("overflow", C.c_int32), ("n_subiters", C.c_int32), ("n_samples_evaluated", C.c_int64)]
 
 
+TUNE_MAX = 16
+
+
+class TuneResult(C.Structure):
+    """fvc_tune_result (autotuner)."""
+    _fields_ = [("mlp_rows", C.c_int32), ("max_steps", C.c_int32), ("target_batch", C.c_int64),
+                ("ms_best", C.c_float), ("ms_default", C.c_float), ("n_tried", C.c_int32), ("pad_", C.c_int32),
+                ("ms", C.c_float * TUNE_MAX), ("rows", C.c_int32 * TUNE_MAX), ("steps", C.c_int32 * TUNE_MAX),
+                ("targets", C.c_int64 * TUNE_MAX)]
+
+
 class DiffStats(C.Structure):
     """fvc_diff_stats (differentiable render)."""
     _fields_ = [("n_rays", C.c_int64), ("n_samples", C.c_int64), ("n_cache_samples", C.c_int64)]

@@ -26,7 +26,7 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["network.hip", "render.hip", "cache.hip", "diff.hip", "loader.hip", "capi.hip"]
+SOURCES = ["network.hip", "render.hip", "cache.hip", "diff.hip", "tune.hip", "loader.hip", "capi.hip"]
 
 
 def _compile(src):

@@ -378,7 +378,7 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats)
     if (n_samples <= 0) break;
     n_eval += n_samples;
     ev = P.begin(s);
-    launch_network_latents(m, r.f_pos.as<float>(), n_samples, r.f_out16.as<uint16_t>(), s);
+    launch_network_latents(m, r.f_pos.as<float>(), n_samples, r.f_out16.as<uint16_t>(), s, r.mlp_rows);
     P.end(ev, s, "network_latents", n_samples, n_samples * (ALG_GATHER_B + 12.0 + 32.0),
           n_samples * ALG_DENSITY_FLOP);
     ev = P.begin(s);
@@ -559,14 +559,16 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
 // MLP (fragments FRAG_H1C).
 __device__ Voxel k_zero_voxel;  // zero-initialised, never written
 
-template <int P>
+// TILES: 16-sample tiles per wave and loop iteration (MLP rows per
+// wavefront = 16 TILES); all tiles' corner loads are issued before the math.
+template <int P, int TILES>
 __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_frox, const int* __restrict__ c_ray,
                                                     const uint2* __restrict__ ray_sh, int64_t n, CacheView cv,
                                                     const typename Mx<P>::T* __restrict__ frags, MlpScales sc,
                                                     uint2* __restrict__ c_out, const int* __restrict__ n_dev) {
   // n_dev: sample count written by the marcher (n is then its bound)
   if (n_dev) n = *n_dev;
-  if ((int64_t)blockIdx.x * 4 * 16 >= n) return;
+  if ((int64_t)blockIdx.x * 4 * 16 * TILES >= n) return;
   typedef typename Mx<P>::T T;
   __shared__ T wl[6 * 64];  // FRAG_H1C (4), FRAG_H2 (2)
   stage_frags(frags + FRAG_H1C * 64, wl, 4);
@@ -575,69 +577,86 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
   const int n32 = (int)n;  // the stream holds fewer than 2^31 / 3 samples
-  const int n_groups = (n32 + 15) / 16;
+  const int n_groups = (n32 + 16 * TILES - 1) / (16 * TILES);
   const float ox = (float)(g >> 1), oy = (float)(g & 1);
   const float fx = (float)cv.fx, fy = (float)cv.fy, fz = (float)cv.fz;
   for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
     const int li = opaque(lane);
-    const int s = grp * 16 + col;
-    const bool valid = s < n32;
-    const int sl = valid ? s : n32 - 1;
-    const float fcx = c_frox[3 * sl] - 0.5f, fcy = c_frox[3 * sl + 1] - 0.5f, fcz = c_frox[3 * sl + 2] - 0.5f;
-    const uint2 sh = ray_sh[c_ray[sl] * 4 + g];
-    const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
-    const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
-    const float px = ix + ox, py = iy + oy;
-    const float wx = (1.0f - ox) * (1.0f - 2.0f * lx) + lx;
-    const float wy = (1.0f - oy) * (1.0f - 2.0f * ly) + ly;
-    const float wxy = wx * wy;
-    const bool in_xy = px >= 0.0f && px < fx && py >= 0.0f && py < fy;
-    const int vx = in_xy ? (int)px : 0, vy = in_xy ? (int)py : 0;
-    const int col_xy = (vy >> 3) * cv.bx + (vx >> 3), loc_xy = (vy & 7) * 8 + (vx & 7);
-    float wd = 0.0f, dd = 0.0f, ld[8];
+    int s[TILES];
+    bool valid[TILES];
+    float w[TILES][2];
+    uint4 lat[TILES][2];
+    float2 di[TILES][2];
+    uint2 sh[TILES];
 #pragma unroll
-    for (int cc = 0; cc < 2; cc++) {
-      const float pz = iz + (float)cc;
-      const float wz = cc == 0 ? (1.0f - 2.0f * lz) + lz : lz;
-      const float w = wxy * wz;
-      const bool inside = in_xy && pz >= 0.0f && pz < fz;
-      const int vz = inside ? (int)pz : 0;
-      const int slot = cv.table[(vz >> 3) * cv.by * cv.bx + col_xy];
-      const Voxel* vp = (inside && slot >= 0)
-                            ? cv.pool + ((size_t)slot * BRICK_VOXELS + (uint32_t)((vz & 7) * 64 + loc_xy))
-                            : &k_zero_voxel;
-      const uint4 lat = *reinterpret_cast<const uint4*>(vp->lat);
-      const float2 di = *reinterpret_cast<const float2*>(&vp->density);
-      const uint32_t lw[4] = {lat.x, lat.y, lat.z, lat.w};
-      const float cw = w * di.y, cd = w * di.x;
-      if (cc == 0) {
-        wd = cw;
-        dd = cd;
+    for (int t = 0; t < TILES; t++) {
+      s[t] = (grp * TILES + t) * 16 + col;
+      valid[t] = s[t] < n32;
+      const int sl = valid[t] ? s[t] : n32 - 1;
+      const float fcx = c_frox[3 * sl] - 0.5f, fcy = c_frox[3 * sl + 1] - 0.5f, fcz = c_frox[3 * sl + 2] - 0.5f;
+      sh[t] = ray_sh[c_ray[sl] * 4 + g];
+      const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
+      const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
+      const float px = ix + ox, py = iy + oy;
+      const float wx = (1.0f - ox) * (1.0f - 2.0f * lx) + lx;
+      const float wy = (1.0f - oy) * (1.0f - 2.0f * ly) + ly;
+      const float wxy = wx * wy;
+      const bool in_xy = px >= 0.0f && px < fx && py >= 0.0f && py < fy;
+      const int vx = in_xy ? (int)px : 0, vy = in_xy ? (int)py : 0;
+      const int col_xy = (vy >> 3) * cv.bx + (vx >> 3), loc_xy = (vy & 7) * 8 + (vx & 7);
 #pragma unroll
-        for (int k = 0; k < 8; k++) ld[k] = w * h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
-      } else {
-        wd = wd + cw;
-        dd = dd + cd;
-#pragma unroll
-        for (int k = 0; k < 8; k++) ld[k] = ld[k] + w * h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
+      for (int cc = 0; cc < 2; cc++) {
+        const float pz = iz + (float)cc;
+        const float wz = cc == 0 ? (1.0f - 2.0f * lz) + lz : lz;
+        w[t][cc] = wxy * wz;
+        const bool inside = in_xy && pz >= 0.0f && pz < fz;
+        const int vz = inside ? (int)pz : 0;
+        const int slot = cv.table[(vz >> 3) * cv.by * cv.bx + col_xy];
+        const Voxel* vp = (inside && slot >= 0)
+                              ? cv.pool + ((size_t)slot * BRICK_VOXELS + (uint32_t)((vz & 7) * 64 + loc_xy))
+                              : &k_zero_voxel;
+        lat[t][cc] = *reinterpret_cast<const uint4*>(vp->lat);
+        di[t][cc] = *reinterpret_cast<const float2*>(&vp->density);
       }
     }
-    wd = xor32_sum(xor16_sum(wd));
-    dd = xor32_sum(xor16_sum(dd));
-    // reduce-scatter: lane^16 pairs (0,1) (2,3) (4,5) (6,7), then lane^32
-    const float m0 = swap16_sum(ld[0], ld[1]), m1 = swap16_sum(ld[2], ld[3]);
-    const float m2 = swap16_sum(ld[4], ld[5]), m3 = swap16_sum(ld[6], ld[7]);
-    const float n0 = swap32_sum(m0, m2), n1 = swap32_sum(m1, m3);
-    // latents sum w*l / sum w*isset (re-weighted)
-    const T x = head_input_cache<P>((_Float16)(n0 / wd), (_Float16)(n1 / wd), sh, sc.inv_act);
-    const f4 o = head_mlp<P>(wl, wl + 4 * 64, li, x, sc);
-    if (g == 0 && valid) {
-      const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
-      uint2 v;
-      v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
-            ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[1]) << 16);
-      v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[2]) | ((uint32_t)__builtin_bit_cast(uint16_t, logsig) << 16);
-      c_out[s] = v;
+#pragma unroll
+    for (int t = 0; t < TILES; t++) {
+      float wd = 0.0f, dd = 0.0f, ld[8];
+#pragma unroll
+      for (int cc = 0; cc < 2; cc++) {
+        const uint32_t lw[4] = {lat[t][cc].x, lat[t][cc].y, lat[t][cc].z, lat[t][cc].w};
+        const float ww = w[t][cc];
+        const float cw = ww * di[t][cc].y, cd = ww * di[t][cc].x;
+        if (cc == 0) {
+          wd = cw;
+          dd = cd;
+#pragma unroll
+          for (int k = 0; k < 8; k++) ld[k] = ww * h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
+        } else {
+          wd = wd + cw;
+          dd = dd + cd;
+#pragma unroll
+          for (int k = 0; k < 8; k++) ld[k] = ld[k] + ww * h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
+        }
+      }
+      wd = xor32_sum(xor16_sum(wd));
+      dd = xor32_sum(xor16_sum(dd));
+      // reduce-scatter: lane^16 pairs (0,1) (2,3) (4,5) (6,7), then lane^32
+      const float m0 = swap16_sum(ld[0], ld[1]), m1 = swap16_sum(ld[2], ld[3]);
+      const float m2 = swap16_sum(ld[4], ld[5]), m3 = swap16_sum(ld[6], ld[7]);
+      const float n0 = swap32_sum(m0, m2), n1 = swap32_sum(m1, m3);
+      // latents sum w*l / sum w*isset (re-weighted)
+      const T x = head_input_cache<P>((_Float16)(n0 / wd), (_Float16)(n1 / wd), sh[t], sc.inv_act);
+      const f4 o = head_mlp<P>(wl, wl + 4 * 64, li, x, sc);
+      if (g == 0 && valid[t]) {
+        const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
+        uint2 v;
+        v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
+              ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[1]) << 16);
+        v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[2]) |
+              ((uint32_t)__builtin_bit_cast(uint16_t, logsig) << 16);
+        c_out[s[t]] = v;
+      }
     }
   }
 }
@@ -697,26 +716,27 @@ __global__ __launch_bounds__(256) void k_cache_composite(
   }
 }
 
-void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
+void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   const Model& m = *r.model;
   const CacheStore& C = r.caches[r.render_idx];
   if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
-  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
-  if (W != r.width || H != r.height) resize_renderbuffers(r, W, H);
+  prepare_views(r, vs);
+  const fvc_raymarch_info& rm = vs.rms[0];
+  const double n_px = (double)vs.n_px;
   const int64_t cap = r.sample_capacity;
   r.c_frox.ensure(cap * 3 * sizeof(float));
   r.c_ray.ensure(cap * sizeof(int));
-  r.ray_sh.ensure((int64_t)W * H * 32);
+  r.ray_sh.ensure(vs.n_px * 32);
   r.c_t0.ensure(cap * sizeof(float));
   r.c_out.ensure(cap * sizeof(uint2));
   hipStream_t s = r.stream();
   const CacheView cv = cache_view(C);
-  const MarchGrid mg = march_grid(r, rm);
+  const MarchGrid mg = march_grid(r, vs.rms, vs.n);
 
   Profiler& P = r.prof;
   hipEvent_t ev = P.begin(s);
-  const int n0 = launch_init_rays(r, rm, s);
-  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n0 * 60.0, 0);
+  const int n0 = launch_init_rays(r, vs, s);
+  P.work(P.end(ev, s, "init_rays"), n_px, n_px * 16 + n0 * 60.0, 0);
   int* ctl = r.counters.as<int>();
   const int64_t target = std::min<int64_t>(r.target_batch, cap - 64);
   const int64_t n_bound = std::min<int64_t>(cap - 64, (int64_t)n0 * r.max_steps);  // samples per launch
@@ -745,20 +765,23 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
       h.march = P.end(ev, s, "cache_march");
       ev = P.begin(s);
       launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), n_bound, r.s_out.as<uint32_t>(), s,
-                          ctl + CTL_R);
+                          ctl + CTL_R, r.mlp_rows);
       h.net = P.end(ev, s, "network_resample");
       ev = P.begin(s);
-#define L(P)                                                                                                       \
-  hipLaunchKernelGGL((k_cache_head<P>), dim3(head_blocks), dim3(256), 0, s, r.c_frox.as<float>(), r.c_ray.as<int>(), \
-                     r.ray_sh.as<uint2>(),                                                                          \
-                     (int64_t)n_bound, cv, m.frags.as<typename Mx<P>::T>(), m.mlp_scales, r.c_out.as<uint2>(),      \
-                     ctl + CTL_C)
+#define L2(P, TL)                                                                                                    \
+  hipLaunchKernelGGL((k_cache_head<P, TL>), dim3(head_blocks), dim3(256), 0, s, r.c_frox.as<float>(),                 \
+                     r.c_ray.as<int>(), r.ray_sh.as<uint2>(), (int64_t)n_bound, cv, m.frags.as<typename Mx<P>::T>(),  \
+                     m.mlp_scales, r.c_out.as<uint2>(), ctl + CTL_C)
+#define L(P)                        \
+  if (r.mlp_rows == 32) { L2(P, 2); } \
+  else { L2(P, 1); }
       switch (m.net.precision) {
         case FVC_PREC_BF16: L(FVC_PREC_BF16); break;
         case FVC_PREC_FP8: L(FVC_PREC_FP8); break;
         default: L(FVC_PREC_FP16); break;
       }
 #undef L
+#undef L2
       FVC_HIP(hipGetLastError());
       h.head = P.end(ev, s, "cache_head");
       ev = P.begin(s);
@@ -776,8 +799,8 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
   }
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap - 64, 1);
   ev = P.begin(s);
-  launch_write_image(r, rm, rgba, s);
-  P.end(ev, s, "write_image", (double)W * H, (double)W * H * 24, 0);
+  launch_write_image(r, vs, s);
+  P.end(ev, s, "write_image", n_px, n_px * 24, 0);
   const std::vector<IterLog> lg = read_iter_log(r, it, s);
   int64_t n_res = 0, n_hit = 0;
   int n_real = 0;
@@ -798,7 +821,7 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
   P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
-    const float np = (float)((int64_t)W * H);
+    const float np = (float)n_px;
     stats->samples_ppx = (float)(n_res + n_hit) / np;
     stats->resample_samples_evaluated = (float)n_res / np;
     stats->cache_samples_evaluated = (float)n_hit / np;
@@ -806,6 +829,10 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
     stats->n_cache_hits = (uint64_t)n_hit;
     stats->n_subiters = n_real;
   }
+}
+
+void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
+  render_cached(r, Views(&rm, 1, &rgba), stats);
 }
 
 }  // namespace fvc

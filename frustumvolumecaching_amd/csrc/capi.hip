@@ -259,6 +259,25 @@ int fvc_renderer_set_batching(fvc_renderer* h, int64_t target_samples, int32_t m
   });
 }
 
+static void check_rm(const fvc_raymarch_info* rm);
+
+int fvc_renderer_autotune(fvc_renderer* h, const fvc_raymarch_info* rm, int cached, fvc_tune_result* result) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    check_rm(rm);
+    std::scoped_lock lk(h->r.read_mutex);
+    autotune(h->r, *rm, cached != 0, result);
+  });
+}
+
+int fvc_renderer_set_mlp_rows(fvc_renderer* h, int32_t rows) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    REQUIRE(rows == 16 || rows == 32, "rows must be 16 or 32");
+    h->r.mlp_rows = rows;
+  });
+}
+
 int fvc_resize_renderbuffers(fvc_renderer* h, int w, int hh) {
   return guard([&] {
     REQUIRE(h, "null argument");
@@ -296,6 +315,26 @@ int fvc_render_cached(fvc_renderer* h, const fvc_raymarch_info* rm, uint8_t* rgb
     check_rm(rm);
     std::scoped_lock lk(h->r.read_mutex);
     render_cached(h->r, *rm, rgba, stats);
+  });
+}
+
+int fvc_render_full_batch(fvc_renderer* h, const fvc_raymarch_info* rms, int32_t n_views, uint8_t* const* rgba,
+                          fvc_render_stats* stats) {
+  return guard([&] {
+    REQUIRE(h && rms && n_views > 0, "invalid argument");
+    for (int v = 0; v < n_views; v++) check_rm(&rms[v]);
+    std::scoped_lock lk(h->r.read_mutex);
+    render_full(h->r, Views(rms, n_views, rgba), stats);
+  });
+}
+
+int fvc_render_cached_batch(fvc_renderer* h, const fvc_raymarch_info* rms, int32_t n_views, uint8_t* const* rgba,
+                            fvc_render_stats* stats) {
+  return guard([&] {
+    REQUIRE(h && rms && n_views > 0, "invalid argument");
+    for (int v = 0; v < n_views; v++) check_rm(&rms[v]);
+    std::scoped_lock lk(h->r.read_mutex);
+    render_cached(h->r, Views(rms, n_views, rgba), stats);
   });
 }
 
@@ -374,7 +413,7 @@ int fvc_cache_set_capacity(fvc_renderer* h, int32_t cap) {
 int fvc_get_last_result(fvc_renderer* h, float* out, int64_t n) {
   return guard([&] {
     REQUIRE(h && out, "null argument");
-    const int64_t need = (int64_t)h->r.width * h->r.height * 4;
+    const int64_t need = h->r.last_n_px * 4;
     REQUIRE(n >= need && need > 0, "buffer too small or nothing rendered");
     FVC_HIP(hipMemcpyAsync(out, h->r.results.p, need * sizeof(float), hipMemcpyDeviceToDevice, h->r.stream()));
     FVC_HIP(hipStreamSynchronize(h->r.stream()));

@@ -920,7 +920,7 @@ void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int 
   D.n_walk.ensure((size_t)n * sizeof(int));
   hipLaunchKernelGGL(k_diff_pack, dim3((N_DIFF_FRAGS * 64 + 255) / 256), dim3(256), 0, s, D.frag_index.as<int>(), base,
                      head, (int)m.n_base, D.frags.as<float>());
-  const MarchGrid mg = march_grid(r, rm);
+  const MarchGrid mg = march_grid(r, &rm, 1);
   const V3 af = {m.scene.aabb_from.x, m.scene.aabb_from.y, m.scene.aabb_from.z};
   const V3 at = {m.scene.aabb_to.x, m.scene.aabb_to.y, m.scene.aabb_to.z};
   const dim3 grid((n + 255) / 256);

@@ -121,8 +121,9 @@ void nerfacc_read_data(const std::string& dir, const NerfaccExport& ex, uint16_t
 // network launches (network.hip)
 // n_dev (optional): the sample count in device memory, n its upper bound
 void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
-                         hipStream_t s, const int* n_dev = nullptr);
-void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s);
+                         hipStream_t s, const int* n_dev = nullptr, int rows = 16);
+void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s,
+                            int rows = 16);
 void launch_network_head(const Model& m, const uint16_t* head_in16, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s);
 void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t* out32, hipStream_t s);
@@ -279,11 +280,14 @@ struct Renderer {
 
   int width = 0, height = 0;
   int64_t sample_capacity = 0;
+  int64_t ray_capacity = 0;    // pixels the ray buffers hold
+  int64_t last_n_px = 0;       // pixels of the last render (all views)
   // sub-iteration batching: samples per launch targeted, and the cap on
   // samples per ray per launch (the reference's TARGET_BATCH_SIZE and
   // MAX_N_STEPS_PER_SUBITER, performance_renderer.h:54-58)
   int64_t target_batch = 1 << 25;
   int max_steps = 32;
+  int mlp_rows = 16;     // MLP rows (samples) per wavefront and iteration: 16 or 32
   int last_iters = 0;    // sub-iterations of the previous frame (first burst length)
   // step table of the marchers (render.hip: step_table)
   DevBuf tab_t, tab_off;
@@ -321,11 +325,27 @@ struct Renderer {
 void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats);
 void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats);
 void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats);
+// autotuner of launch shapes (tune.hip)
+void autotune(Renderer& r, const fvc_raymarch_info& rm, bool cached, fvc_tune_result* out);
 // differentiable full / cached render (diff.hip)
 void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int n, const float* pos,
                   const float* base, const float* head, float* rgb, fvc_diff_stats* stats, bool cached);
 void diff_backward(Renderer& r, const float* d_rgb, float* g_pos, float* g_base, float* g_head, float* g_pool);
 void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts, float* t0s);
 void resize_renderbuffers(Renderer& r, int w, int h);
+void ensure_ray_capacity(Renderer& r, int64_t n_pixels);
+
+// a batch of views rendered together: pixel base of view v in the packed
+// pixel space, images (device, may be null)
+struct Views {
+  const fvc_raymarch_info* rms;
+  int n;
+  uint8_t* const* rgba;
+  std::vector<int64_t> base;
+  int64_t n_px;
+  Views(const fvc_raymarch_info* rms, int n, uint8_t* const* rgba);
+};
+void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats);
+void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats);
 
 }  // namespace fvc

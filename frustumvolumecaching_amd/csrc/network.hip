@@ -150,7 +150,9 @@ __device__ __forceinline__ uint2 pack_h4(_Float16 a, _Float16 b, _Float16 c, _Fl
 
 // HEAD: full inference -> half4 (r, g, b, log sigma) per sample
 // !HEAD: inferenceLatents -> 16 halves per sample
-template <bool HEAD, int P>
+// TILES: 16-sample MFMA tiles per wave and loop iteration (MLP rows per
+// wavefront = 16 TILES); the tiles' hash gathers are issued together.
+template <bool HEAD, int P, int TILES>
 __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__ pos, const float* __restrict__ dir,
                                                        int64_t n, const uint32_t* __restrict__ table,
                                                        const typename Mx<P>::T* __restrict__ frags, GridParams gp,
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
                                                        uint2* __restrict__ out16, const int* __restrict__ n_dev) {
   // n_dev: sample count written by an earlier kernel (n is then its bound)
   if (n_dev) n = *n_dev;
-  if ((int64_t)blockIdx.x * (NET_BLOCK / 64) * 16 >= n) return;
+  if ((int64_t)blockIdx.x * (NET_BLOCK / 64) * 16 * TILES >= n) return;
   __shared__ GridLevel lv[MAX_LEVELS];
   __shared__ typename Mx<P>::T wl[N_NET_FRAGS * 64];
   load_levels(gp, lv);
@@ -166,27 +168,36 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
-  const int64_t n_groups = (n + 15) / 16;
+  const int64_t n_groups = (n + 16 * TILES - 1) / (16 * TILES);
   for (int64_t grp = (int64_t)blockIdx.x * (NET_BLOCK / 64) + wave; grp < n_groups;
        grp += (int64_t)gridDim.x * (NET_BLOCK / 64)) {
     const int li = opaque(lane);
-    const int64_t s = grp * 16 + col;
-    const bool valid = s < n;
-    const int64_t sl = valid ? s : n - 1;
-    const V3 p = {pos[3 * sl], pos[3 * sl + 1], pos[3 * sl + 2]};
-    float f[8];
-    encode_quads(lv, gp, table, p, g, f);
+    int64_t s[TILES], sl[TILES];
+    bool valid[TILES];
+    float f[TILES][8];
 #pragma unroll
-    for (int e = 0; e < 8; e++) f[e] = (float)(_Float16)f[e];  // the encoding is stored as half
-    const typename Mx<P>::T x = Mx<P>::pack(f, sc.inv_act);
-    const f4 d3 = base_mlp<P>(wl, li, x, sc);
-    const _Float16 rows[4] = {(_Float16)d3[0], (_Float16)d3[1], (_Float16)d3[2], (_Float16)d3[3]};
-    if (!HEAD) {
-      if (valid) out16[s * 4 + g] = pack_h4(rows[0], rows[1], rows[2], rows[3]);
-    } else {
-      const typename Mx<P>::T hin = head_input<P>(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2], sc.inv_act);
-      const f4 o = head_mlp<P>(wl + FRAG_H1 * 64, wl + FRAG_H2 * 64, li, hin, sc);
-      if (g == 0 && valid) out_h4[s] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2], rows[0]);
+    for (int t = 0; t < TILES; t++) {
+      s[t] = (grp * TILES + t) * 16 + col;
+      valid[t] = s[t] < n;
+      sl[t] = valid[t] ? s[t] : n - 1;
+      const V3 p = {pos[3 * sl[t]], pos[3 * sl[t] + 1], pos[3 * sl[t] + 2]};
+      encode_quads(lv, gp, table, p, g, f[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < TILES; t++) {
+#pragma unroll
+      for (int e = 0; e < 8; e++) f[t][e] = (float)(_Float16)f[t][e];  // the encoding is stored as half
+      const typename Mx<P>::T x = Mx<P>::pack(f[t], sc.inv_act);
+      const f4 d3 = base_mlp<P>(wl, li, x, sc);
+      const _Float16 rows[4] = {(_Float16)d3[0], (_Float16)d3[1], (_Float16)d3[2], (_Float16)d3[3]};
+      if (!HEAD) {
+        if (valid[t]) out16[s[t] * 4 + g] = pack_h4(rows[0], rows[1], rows[2], rows[3]);
+      } else {
+        const int64_t q = sl[t];
+        const typename Mx<P>::T hin = head_input<P>(rows, g, dir[3 * q], dir[3 * q + 1], dir[3 * q + 2], sc.inv_act);
+        const f4 o = head_mlp<P>(wl + FRAG_H1 * 64, wl + FRAG_H2 * 64, li, hin, sc);
+        if (g == 0 && valid[t]) out_h4[s[t]] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2], rows[0]);
+      }
     }
   }
 }
@@ -255,24 +266,32 @@ static int grid_for(int64_t n_groups) {
   }
 
 void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
-                         hipStream_t s, const int* n_dev) {
+                         hipStream_t s, const int* n_dev, int rows) {
   if (n <= 0) return;
-#define L(P)                                                                                                 \
-  hipLaunchKernelGGL((k_network<true, P>), dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, pos, dir, n, \
-                     m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, m.mlp_scales, (uint2*)out_h4, \
-                     (uint2*)nullptr, n_dev)
+#define L2(P, TL)                                                                                                \
+  hipLaunchKernelGGL((k_network<true, P, TL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), 0, s, \
+                     pos, dir, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, m.mlp_scales,   \
+                     (uint2*)out_h4, (uint2*)nullptr, n_dev)
+#define L(P)                       \
+  if (rows == 32) { L2(P, 2); }    \
+  else { L2(P, 1); }
   FVC_BY_PREC(m, L)
 #undef L
+#undef L2
   FVC_HIP(hipGetLastError());
 }
-void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s) {
+void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s, int rows) {
   if (n <= 0) return;
-#define L(P)                                                                                                    \
-  hipLaunchKernelGGL((k_network<false, P>), dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, pos,           \
-                     (const float*)nullptr, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid,  \
+#define L2(P, TL)                                                                                                 \
+  hipLaunchKernelGGL((k_network<false, P, TL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), 0, s, \
+                     pos, (const float*)nullptr, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, \
                      m.mlp_scales, (uint2*)nullptr, (uint2*)out16, (const int*)nullptr)
+#define L(P)                       \
+  if (rows == 32) { L2(P, 2); }    \
+  else { L2(P, 1); }
   FVC_BY_PREC(m, L)
 #undef L
+#undef L2
   FVC_HIP(hipGetLastError());
 }
 void launch_network_head(const Model& m, const uint16_t* hin16, const float* dir, int64_t n, uint32_t* out_h4,

@@ -19,7 +19,9 @@ namespace fvc {
 // per pixel ray, AABB clip, start t snapped by valid_t_from_t; rays that hit
 // the AABB are appended to the alive list, every pixel result starts as
 // (0,0,0, T=1).
-__global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to,
+// Pixels of a view are numbered pix_base + y * W + x in the packed pixel
+// space of a batch of views.
+__global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, int pix_base, V3 aabb_from, V3 aabb_to,
                                                    RayState* __restrict__ rays, int* __restrict__ pix,
                                                    float4* __restrict__ rres, float4* __restrict__ results,
                                                    int* __restrict__ counter) {
@@ -28,12 +30,12 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, V3 aabb
   const bool valid = x < W && y < H;
   RayState st{};
   const bool alive = valid && init_ray_state(rm, aabb_from, aabb_to, x, y, st);
-  if (valid) results[(int64_t)y * W + x] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  if (valid) results[pix_base + y * W + x] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   int slot[1];
   block_reserve<1>({alive ? 1 : 0}, {counter}, slot);
   if (alive) {
     rays[slot[0]] = st;
-    pix[slot[0]] = y * W + x;
+    pix[slot[0]] = pix_base + y * W + x;
     rres[slot[0]] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   }
 }
@@ -148,12 +150,12 @@ __global__ __launch_bounds__(256) void k_composite(int* __restrict__ ctl, int pa
 }
 
 // writeImageBuffer_performance_kernel (performance_renderer.cu:127-149)
-__global__ void k_write_image(int W, int H, const float4* __restrict__ results, uchar4* __restrict__ rgba,
-                              int sample_index) {
+__global__ void k_write_image(int W, int H, int pix_base, const float4* __restrict__ results,
+                              uchar4* __restrict__ rgba, int sample_index) {
   const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x >= W || y >= H) return;
   const int64_t p = (int64_t)y * W + x;
-  const float4 r = results[p];
+  const float4 r = results[pix_base + p];
   const uchar4 prev = rgba[p];
   const float si = (float)sample_index, den = (float)(sample_index + 1);
   const float cr = (r.x + r.w + (float)prev.x / 255.0f * si) / den;
@@ -193,7 +195,15 @@ __global__ void k_debug_march(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, Ma
 
 void resize_renderbuffers(Renderer& r, int w, int h) {
   if (w <= 0 || h <= 0) throw Error(FVC_E_INVALID, "resolution must be positive");
-  const int64_t n = (int64_t)w * h;
+  ensure_ray_capacity(r, (int64_t)w * h);
+  r.width = w;
+  r.height = h;
+}
+
+// ray and sample buffers for n pixels (all views of a batch)
+void ensure_ray_capacity(Renderer& r, int64_t n) {
+  if (n >= ((int64_t)1 << 31) / 4) throw Error(FVC_E_INVALID, "too many pixels in one render call");
+  n = std::max(n, r.ray_capacity);  // buffers only grow
   // a launch emits at most n_alive * n_steps <= max(target, n) samples per
   // stream, and never more than n * max_steps
   const int64_t cap = std::min<int64_t>(std::max<int64_t>(r.target_batch, n), n * r.max_steps) + 64;
@@ -212,8 +222,7 @@ void resize_renderbuffers(Renderer& r, int w, int h) {
   r.s_dir.ensure(cap * 3 * sizeof(float));
   r.s_t0.ensure(cap * sizeof(float));
   r.s_out.ensure(cap * sizeof(uint2));
-  r.width = w;
-  r.height = h;
+  r.ray_capacity = n;
   r.sample_capacity = cap;
   if (!r.pinned) FVC_HIP(hipHostMalloc((void**)&r.pinned, 64 * sizeof(int), hipHostMallocDefault));
 }
@@ -260,23 +269,56 @@ void zero_counters(Renderer& r, hipStream_t s) {
   FVC_HIP(hipMemsetAsync(r.counters.p, 0, CTL_LOG * sizeof(int), s));
 }
 
-int launch_init_rays(Renderer& r, const fvc_raymarch_info& rm, hipStream_t s) {
+Views::Views(const fvc_raymarch_info* rms_, int n_, uint8_t* const* rgba_) : rms(rms_), n(n_), rgba(rgba_) {
+  if (n <= 0) throw Error(FVC_E_INVALID, "no views");
+  base.resize(n);
+  n_px = 0;
+  for (int v = 0; v < n; v++) {
+    const int W = rms[v].cam_info.res_x, H = rms[v].cam_info.res_y;
+    if (W <= 0 || H <= 0) throw Error(FVC_E_INVALID, "resolution must be positive");
+    if (std::memcmp(&rms[v].stepsize_info, &rms[0].stepsize_info, sizeof(fvc_stepsize_info)) != 0)
+      throw Error(FVC_E_INVALID, "the views of a batch must share the stepsize info");
+    base[v] = n_px;
+    n_px += (int64_t)W * H;
+  }
+}
+
+// Rays of every view appended to one alive list (one launch per view, one
+// counter); returns the number of alive rays.
+int launch_init_rays(Renderer& r, const Views& vs, hipStream_t s) {
   const Model& m = *r.model;
-  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   zero_counters(r, s);
-  hipLaunchKernelGGL(k_init_rays, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, cvt(m.scene.aabb_from),
-                     cvt(m.scene.aabb_to), r.rays[0].as<RayState>(), r.ray_pix[0].as<int>(), r.ray_res[0].as<float4>(),
-                     r.results.as<float4>(), r.counters.as<int>() + CTL_ALIVE);
-  FVC_HIP(hipGetLastError());
+  for (int v = 0; v < vs.n; v++) {
+    const fvc_raymarch_info& rm = vs.rms[v];
+    const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+    hipLaunchKernelGGL(k_init_rays, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, (int)vs.base[v],
+                       cvt(m.scene.aabb_from), cvt(m.scene.aabb_to), r.rays[0].as<RayState>(), r.ray_pix[0].as<int>(),
+                       r.ray_res[0].as<float4>(), r.results.as<float4>(), r.counters.as<int>() + CTL_ALIVE);
+    FVC_HIP(hipGetLastError());
+  }
   return read_counter(r, CTL_ALIVE, s);
 }
 
-void launch_write_image(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, hipStream_t s) {
-  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
-  if (!rgba) return;
-  hipLaunchKernelGGL(k_write_image, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, W, H,
-                     r.results.as<float4>(), (uchar4*)rgba, rm.sample_index);
-  FVC_HIP(hipGetLastError());
+void launch_write_image(Renderer& r, const Views& vs, hipStream_t s) {
+  for (int v = 0; v < vs.n; v++) {
+    const fvc_raymarch_info& rm = vs.rms[v];
+    const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+    if (!vs.rgba || !vs.rgba[v]) continue;
+    hipLaunchKernelGGL(k_write_image, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, W, H, (int)vs.base[v],
+                       r.results.as<float4>(), (uchar4*)vs.rgba[v], rm.sample_index);
+    FVC_HIP(hipGetLastError());
+  }
+}
+
+// pixels, renderbuffers and step table for a batch of views
+void prepare_views(Renderer& r, const Views& vs) {
+  r.last_n_px = vs.n_px;
+  if (vs.n == 1) {
+    const int W = vs.rms[0].cam_info.res_x, H = vs.rms[0].cam_info.res_y;
+    if (W != r.width || H != r.height) resize_renderbuffers(r, W, H);
+  } else {
+    ensure_ray_capacity(r, vs.n_px);
+  }
 }
 
 // Largest t any ray of this view can march to: the AABB exit is at most the
@@ -295,9 +337,10 @@ static float max_ray_t(const fvc_raymarch_info& rm, const fvc_scene_info& sc) {
 
 // Build (or reuse) the step table for this stepsize and view (see
 // build_step_rows); when it is disabled the marchers simply step.
-StepTableView step_table(Renderer& r, const fvc_raymarch_info& rm) {
-  const fvc_stepsize_info& st = rm.stepsize_info;
-  const float need = max_ray_t(rm, r.model->scene);
+StepTableView step_table(Renderer& r, const fvc_raymarch_info* rms, int n_views) {
+  const fvc_stepsize_info& st = rms[0].stepsize_info;
+  float need = 0.0f;
+  for (int v = 0; v < n_views; v++) need = std::max(need, max_ray_t(rms[v], r.model->scene));
   StepTableView v = step_view(st);
   if (!(r.tab_rows >= 0 && std::memcmp(&r.tab_key, &st, sizeof(st)) == 0 && need <= r.tab_limit)) {
     const float lim = need * 1.02f + 4.0f * st.dt_max;
@@ -320,11 +363,11 @@ StepTableView step_table(Renderer& r, const fvc_raymarch_info& rm) {
   return v;
 }
 
-MarchGrid march_grid(Renderer& r, const fvc_raymarch_info& rm) {
+MarchGrid march_grid(Renderer& r, const fvc_raymarch_info* rms, int n_views) {
   MarchGrid g;
   g.occ = r.model->occ;
-  g.st = rm.stepsize_info;
-  g.tab = step_table(r, rm);
+  g.st = rms[0].stepsize_info;
+  g.tab = step_table(r, rms, n_views);
   g.bf64 = r.model->bitfield.as<uint64_t>();
   return g;
 }
@@ -341,16 +384,17 @@ int steps_for(int n_alive, int64_t target, int max_steps) {
 // first burst is as long as the previous frame's loop.
 int next_burst(const Renderer& r, int it) { return it == 0 ? std::max(2, r.last_iters) : 2; }
 
-void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
+void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   const Model& m = *r.model;
-  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
-  if (W != r.width || H != r.height) resize_renderbuffers(r, W, H);
+  prepare_views(r, vs);
+  const fvc_raymarch_info& rm = vs.rms[0];
+  const double n_px = (double)vs.n_px;
   hipStream_t s = r.stream();
-  const MarchGrid mg = march_grid(r, rm);
+  const MarchGrid mg = march_grid(r, vs.rms, vs.n);
   Profiler& P = r.prof;
   hipEvent_t ev = P.begin(s);
-  const int n0 = launch_init_rays(r, rm, s);
-  P.work(P.end(ev, s, "init_rays"), (double)W * H, (double)W * H * 16 + n0 * 60.0, 0);
+  const int n0 = launch_init_rays(r, vs, s);
+  P.work(P.end(ev, s, "init_rays"), n_px, n_px * 16 + n0 * 60.0, 0);
   int* ctl = r.counters.as<int>();
   const int64_t cap = r.sample_capacity - 64;
   const int64_t target = std::min<int64_t>(r.target_batch, cap);
@@ -373,7 +417,7 @@ void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_re
       h.march = P.end(ev, s, "march");
       ev = P.begin(s);
       launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), n_bound, r.s_out.as<uint32_t>(), s,
-                          ctl + CTL_R);
+                          ctl + CTL_R, r.mlp_rows);
       h.net = P.end(ev, s, "network_full");
       ev = P.begin(s);
       hipLaunchKernelGGL(k_composite, grid, dim3(256), 0, s, ctl, cur, rm.stepsize_info, m.scene.alpha_thre,
@@ -389,8 +433,8 @@ void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_re
   }
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap, 1);
   ev = P.begin(s);
-  launch_write_image(r, rm, rgba, s);
-  P.end(ev, s, "write_image", (double)W * H, (double)W * H * 24, 0);
+  launch_write_image(r, vs, s);
+  P.end(ev, s, "write_image", n_px, n_px * 24, 0);
   const std::vector<IterLog> lg = read_iter_log(r, it, s);
   int64_t n_eval = 0;
   int n_real = 0;
@@ -408,11 +452,15 @@ void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_re
   P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
-    stats->samples_ppx = (float)n_eval / (float)((int64_t)W * H);
+    stats->samples_ppx = (float)((double)n_eval / n_px);
     stats->resample_samples_evaluated = stats->samples_ppx;
     stats->n_resamples = (uint64_t)n_eval;
     stats->n_subiters = n_real;
   }
+}
+
+void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
+  render_full(r, Views(&rm, 1, &rgba), stats);
 }
 
 void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts_host, float* t0_host) {
@@ -420,7 +468,7 @@ void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int3
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   const int64_t n = (int64_t)W * H;
   hipStream_t s = r.stream();
-  const MarchGrid mg = march_grid(r, rm);
+  const MarchGrid mg = march_grid(r, &rm, 1);
   DevBuf c, t;
   c.ensure(n * sizeof(int));
   t.ensure(std::max<int64_t>(1, n * max_per_ray) * sizeof(float));

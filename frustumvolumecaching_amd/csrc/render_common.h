@@ -137,15 +137,16 @@ FVC_HD int to_8bit(float f) {
 }
 
 // host helpers (render.hip)
-StepTableView step_table(Renderer& r, const fvc_raymarch_info& rm);
-MarchGrid march_grid(Renderer& r, const fvc_raymarch_info& rm);
+StepTableView step_table(Renderer& r, const fvc_raymarch_info* rms, int n_views);
+MarchGrid march_grid(Renderer& r, const fvc_raymarch_info* rms, int n_views);
 int read_counter(Renderer& r, int idx, hipStream_t s);
 void zero_counters(Renderer& r, hipStream_t s);
-int launch_init_rays(Renderer& r, const fvc_raymarch_info& rm, hipStream_t s);
+int launch_init_rays(Renderer& r, const Views& vs, hipStream_t s);
+void prepare_views(Renderer& r, const Views& vs);
 // per-iteration log entries {alive, n_steps, resample, cache, bins} of a frame
 struct IterLog { int alive, n_steps, n_r, n_c, bins; };
 std::vector<IterLog> read_iter_log(Renderer& r, int n_it, hipStream_t s);
-void launch_write_image(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, hipStream_t s);
+void launch_write_image(Renderer& r, const Views& vs, hipStream_t s);
 int steps_for(int n_alive, int64_t target, int max_steps);
 int next_burst(const Renderer& r, int it);
 

@@ -49,11 +49,17 @@ _SIGS = {
     "fvc_renderer_destroy": (None, [_P]),
     "fvc_renderer_set_stream": (C.c_int, [_P, _P]),
     "fvc_resize_renderbuffers": (C.c_int, [_P, C.c_int, C.c_int]),
+    "fvc_renderer_autotune": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int, C.POINTER(abi.TuneResult)]),
+    "fvc_renderer_set_mlp_rows": (C.c_int, [_P, C.c_int32]),
     "fvc_renderer_set_batching": (C.c_int, [_P, C.c_int64, C.c_int32]),
     "fvc_render_full": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.POINTER(abi.RenderStats)]),
     "fvc_cache_fill": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.POINTER(abi.InitStats)]),
     "fvc_render_cached": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.POINTER(abi.RenderStats)]),
     "fvc_cache_swap": (C.c_int, [_P]),
+    "fvc_render_full_batch": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, C.POINTER(_P),
+                                        C.POINTER(abi.RenderStats)]),
+    "fvc_render_cached_batch": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, C.POINTER(_P),
+                                          C.POINTER(abi.RenderStats)]),
     "fvc_cache_set_settings": (C.c_int, [_P, C.POINTER(abi.CacheSettings)]),
     "fvc_cache_set_capacity": (C.c_int, [_P, C.c_int32]),
     "fvc_get_last_result": (C.c_int, [_P, _P, C.c_int64]),
@@ -226,6 +232,15 @@ class Renderer:
     def set_batching(self, target_samples, max_samples_per_ray):
         check(load().fvc_renderer_set_batching(self.h, int(target_samples), int(max_samples_per_ray)))
 
+    def autotune(self, rm, cached=False):
+        """Pick MLP rows per wavefront and chunk sizes by timing renders of rm."""
+        res = abi.TuneResult()
+        check(load().fvc_renderer_autotune(self.h, C.byref(rm), int(bool(cached)), C.byref(res)))
+        return res
+
+    def set_mlp_rows(self, rows):
+        check(load().fvc_renderer_set_mlp_rows(self.h, int(rows)))
+
     def resizeRenderbuffers(self, width, height):
         check(load().fvc_resize_renderbuffers(self.h, int(width), int(height)))
 
@@ -249,6 +264,22 @@ class Renderer:
         check(load().fvc_render_cached(self.h, C.byref(rm), _addr(rgba), C.byref(st)))
         self.last_render_stats = st
         return st
+
+    def _batch(self, fn, rms, rgbas):
+        n = len(rms)
+        arr = (abi.RaymarchInfo * n)(*rms)
+        ptrs = (_P * n)(*[None if x is None else _addr(x).value for x in (rgbas or [None] * n)])
+        st = abi.RenderStats()
+        check(fn(self.h, arr, n, ptrs, C.byref(st)))
+        self.last_render_stats = st
+        return st
+
+    def render_full_batch(self, rms, rgbas=None):
+        """Render several views (any resolutions, shared stepsize) in one pass."""
+        return self._batch(load().fvc_render_full_batch, rms, rgbas)
+
+    def render_cached_batch(self, rms, rgbas=None):
+        return self._batch(load().fvc_render_cached_batch, rms, rgbas)
 
     def swapCache(self):
         check(load().fvc_cache_swap(self.h))

@@ -260,6 +260,17 @@ int fvc_render_full(fvc_renderer* renderer, const fvc_raymarch_info* rm, uint8_t
 int fvc_cache_fill(fvc_renderer* renderer, const fvc_raymarch_info* rm, fvc_init_stats* stats);
 /* CachePerformanceRenderer::render (cache_performance_renderer.cu:1059-1263). */
 int fvc_render_cached(fvc_renderer* renderer, const fvc_raymarch_info* rm, uint8_t* rgba_dev, fvc_render_stats* stats);
+/* Batched multi-view render (no counterpart in the reference, which renders
+ * one view per call): n_views views of any resolution sharing the stepsize
+ * info, rgba[v] the device uchar4 image of view v (null entries skip the
+ * write).  All views' rays form one alive list and one sample stream, so
+ * each sub-iteration is a single launch per kernel over every view's
+ * samples.  Stats are over all views; fvc_get_last_result returns the
+ * views' results concatenated. */
+int fvc_render_full_batch(fvc_renderer* renderer, const fvc_raymarch_info* rms, int32_t n_views, uint8_t* const* rgba,
+                          fvc_render_stats* stats);
+int fvc_render_cached_batch(fvc_renderer* renderer, const fvc_raymarch_info* rms, int32_t n_views,
+                            uint8_t* const* rgba, fvc_render_stats* stats);
 /* CacheRenderer::swapCache (cache_renderer.cu:1053-1057) */
 int fvc_cache_swap(fvc_renderer* renderer);
 /* CacheRenderer::setCacheSettings (cache_renderer.h:266-267) */
@@ -267,6 +278,28 @@ int fvc_cache_set_settings(fvc_renderer* renderer, const fvc_cache_settings* set
 /* Pool capacity override in bricks (0 = reference estimate n_rays/8 rounded
  * up to 16384, cache_renderer.cu:832-846 + cache.cu:143-146). */
 int fvc_cache_set_capacity(fvc_renderer* renderer, int32_t capacity_bricks);
+
+/* ---- autotuning (no counterpart in the reference) ---- */
+#define FVC_TUNE_MAX 16
+typedef struct {
+  int32_t mlp_rows;               /* chosen MLP rows per wavefront: 16 or 32 */
+  int32_t max_steps;              /* chosen samples per ray per launch */
+  int64_t target_batch;           /* chosen target samples per launch */
+  float   ms_best;                /* median frame time of the choice */
+  float   ms_default;             /* ... of the settings before tuning */
+  int32_t n_tried;
+  int32_t pad_;
+  float   ms[FVC_TUNE_MAX];       /* every candidate: time, rows, steps, target */
+  int32_t rows[FVC_TUNE_MAX];
+  int32_t steps[FVC_TUNE_MAX];
+  int64_t targets[FVC_TUNE_MAX];
+} fvc_tune_result;
+/* Times full (cached = 0) or cached renders of rm over the candidate MLP
+ * rows per wavefront and sub-iteration chunk sizes, keeps the fastest for
+ * this renderer and returns the table.  Images are not written. */
+int fvc_renderer_autotune(fvc_renderer* renderer, const fvc_raymarch_info* rm, int cached, fvc_tune_result* result);
+/* MLP rows per wavefront (16 or 32) for the network and cache-head kernels. */
+int fvc_renderer_set_mlp_rows(fvc_renderer* renderer, int32_t rows);
 
 /* ---- differentiable full render (training mode) ----
  * No counterpart entry point in the reference, whose renderers are
