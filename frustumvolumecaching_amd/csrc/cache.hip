@@ -734,8 +734,10 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   const MarchGrid mg = march_grid(r, vs.rms, vs.n);
 
   Profiler& P = r.prof;
+  // graph mode as in render_full
+  const bool graph = r.use_graph && r.graph_iters[1] > 0 && !P.enabled;
   hipEvent_t ev = P.begin(s);
-  const int n0 = launch_init_rays(r, vs, s);
+  const int n0 = graph ? (int)vs.n_px : launch_init_rays(r, vs, s);
   P.work(P.end(ev, s, "init_rays"), n_px, n_px * 16 + n0 * 60.0, 0);
   int* ctl = r.counters.as<int>();
   const int64_t target = std::min<int64_t>(r.target_batch, cap - 64);
@@ -748,10 +750,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   const int cutoff = 4096;
   struct Handles { int march, net, head, comp; };
   std::vector<Handles> hs;
-  int it = 0;
-  while (n0 > 0 && it < MAX_SUBITER) {
-    const int burst = next_burst(r, it);
-    for (int b = 0; b < burst && it < MAX_SUBITER; b++, it++) {
+  auto iteration = [&](int it) {
       const int cur = it & 1;
       hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap - 64, 0);
       Handles h;
@@ -794,8 +793,22 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       FVC_HIP(hipGetLastError());
       h.comp = P.end(ev, s, "cache_composite");
       hs.push_back(h);
-    }
-    if (read_counter(r, CTL_ALIVE + (it & 1), s) == 0) break;
+  };
+  int it = 0;
+  bool done = n0 == 0;
+  if (graph) {
+    const int k = std::min(r.graph_iters[1] + 1, MAX_SUBITER);
+    graph_run(r, 1, s, [&] {
+      launch_init_rays_async(r, vs, s);
+      for (int i = 0; i < k; i++) iteration(i);
+    });
+    it = k;
+    done = read_counter(r, CTL_ALIVE + (it & 1), s) == 0;
+  }
+  while (!done && it < MAX_SUBITER) {
+    const int burst = next_burst(r, it);
+    for (int b = 0; b < burst && it < MAX_SUBITER; b++, it++) iteration(it);
+    done = read_counter(r, CTL_ALIVE + (it & 1), s) == 0;
   }
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap - 64, 1);
   ev = P.begin(s);
@@ -818,6 +831,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
     n_hit += l.n_c;
   }
   r.last_iters = n_real;
+  r.graph_iters[1] = n_real;
   P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));

@@ -270,6 +270,14 @@ int fvc_renderer_autotune(fvc_renderer* h, const fvc_raymarch_info* rm, int cach
   });
 }
 
+int fvc_renderer_set_graph(fvc_renderer* h, int enable) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    std::scoped_lock lk(h->r.read_mutex);
+    h->r.use_graph = enable != 0;
+  });
+}
+
 int fvc_renderer_set_mlp_rows(fvc_renderer* h, int32_t rows) {
   return guard([&] {
     REQUIRE(h, "null renderer");

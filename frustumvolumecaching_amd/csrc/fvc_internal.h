@@ -8,6 +8,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "fvc_math.h"
@@ -313,7 +314,13 @@ struct Renderer {
   DevBuf f_rays, f_tiles, f_list, f_pos, f_t0, f_meta, f_out16;
   int fill_tiles = 0;
   int64_t fill_sample_capacity = 0;
+  // graph mode: executable graphs of a full (0) and a cached (1) frame
+  bool use_graph = false;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  int graph_iters[2] = {0, 0};  // sub-iterations of the last frame of each kind
   ~Renderer() {
+    for (hipGraphExec_t& g : graph_exec)
+      if (g) (void)hipGraphExecDestroy(g);
     if (pinned) (void)hipHostFree(pinned);
     if (own_stream) (void)hipStreamDestroy(own_stream);
   }
@@ -334,6 +341,7 @@ void diff_backward(Renderer& r, const float* d_rgb, float* g_pos, float* g_base,
 void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts, float* t0s);
 void resize_renderbuffers(Renderer& r, int w, int h);
 void ensure_ray_capacity(Renderer& r, int64_t n_pixels);
+void graph_run(Renderer& r, int which, hipStream_t s, const std::function<void()>& record);
 
 // a batch of views rendered together: pixel base of view v in the packed
 // pixel space, images (device, may be null)

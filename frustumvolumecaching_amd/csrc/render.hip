@@ -9,6 +9,7 @@
 // per wave), so compositing walks them in order.
 #include <algorithm>
 #include <cstring>
+#include <functional>
 
 #include "march.h"
 #include "render_common.h"
@@ -265,8 +266,52 @@ int read_counter(Renderer& r, int idx, hipStream_t s) {
   return r.pinned[idx];
 }
 
+__global__ void k_zero_ints(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// a kernel, not a memset: the frame may be recorded into a graph
 void zero_counters(Renderer& r, hipStream_t s) {
-  FVC_HIP(hipMemsetAsync(r.counters.p, 0, CTL_LOG * sizeof(int), s));
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(CTL_LOG), 0, s, r.counters.as<int>(), CTL_LOG);
+  FVC_HIP(hipGetLastError());
+}
+
+// Record the launches of `record` on stream s into a graph and run it: the
+// executable graph of slot `which` is updated in place when the topology is
+// unchanged (same sub-iteration count), else instantiated again.
+void graph_run(Renderer& r, int which, hipStream_t s, const std::function<void()>& record) {
+  FVC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    record();
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hipGraph_t g = nullptr;
+  FVC_HIP(hipStreamEndCapture(s, &g));
+  bool updated = false;
+  if (r.graph_exec[which]) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    updated = hipGraphExecUpdate(r.graph_exec[which], g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+    if (!updated) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(r.graph_exec[which]);
+      r.graph_exec[which] = nullptr;
+    }
+  }
+  if (!updated) {
+    const hipError_t e = hipGraphInstantiate(&r.graph_exec[which], g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      FVC_HIP(e);
+    }
+  }
+  (void)hipGraphDestroy(g);
+  FVC_HIP(hipGraphLaunch(r.graph_exec[which], s));
 }
 
 Views::Views(const fvc_raymarch_info* rms_, int n_, uint8_t* const* rgba_) : rms(rms_), n(n_), rgba(rgba_) {
@@ -286,6 +331,11 @@ Views::Views(const fvc_raymarch_info* rms_, int n_, uint8_t* const* rgba_) : rms
 // Rays of every view appended to one alive list (one launch per view, one
 // counter); returns the number of alive rays.
 int launch_init_rays(Renderer& r, const Views& vs, hipStream_t s) {
+  launch_init_rays_async(r, vs, s);
+  return read_counter(r, CTL_ALIVE, s);
+}
+
+void launch_init_rays_async(Renderer& r, const Views& vs, hipStream_t s) {
   const Model& m = *r.model;
   zero_counters(r, s);
   for (int v = 0; v < vs.n; v++) {
@@ -296,7 +346,6 @@ int launch_init_rays(Renderer& r, const Views& vs, hipStream_t s) {
                        r.ray_res[0].as<float4>(), r.results.as<float4>(), r.counters.as<int>() + CTL_ALIVE);
     FVC_HIP(hipGetLastError());
   }
-  return read_counter(r, CTL_ALIVE, s);
 }
 
 void launch_write_image(Renderer& r, const Views& vs, hipStream_t s) {
@@ -392,8 +441,12 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   hipStream_t s = r.stream();
   const MarchGrid mg = march_grid(r, vs.rms, vs.n);
   Profiler& P = r.prof;
+  // graph mode: the frame's launches up to the previous frame's sub-iteration
+  // count + 1 are recorded and replayed as one graph; their grids are sized
+  // for every pixel (the kernels read the live counts on the device)
+  const bool graph = r.use_graph && r.graph_iters[0] > 0 && !P.enabled;
   hipEvent_t ev = P.begin(s);
-  const int n0 = launch_init_rays(r, vs, s);
+  const int n0 = graph ? (int)vs.n_px : launch_init_rays(r, vs, s);
   P.work(P.end(ev, s, "init_rays"), n_px, n_px * 16 + n0 * 60.0, 0);
   int* ctl = r.counters.as<int>();
   const int64_t cap = r.sample_capacity - 64;
@@ -402,10 +455,7 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   const dim3 grid((n0 + 255) / 256);
   struct Handles { int march, net, comp; };
   std::vector<Handles> hs;
-  int it = 0;
-  while (n0 > 0 && it < MAX_SUBITER) {
-    const int burst = next_burst(r, it);
-    for (int b = 0; b < burst && it < MAX_SUBITER; b++, it++) {
+  auto iteration = [&](int it) {
       const int cur = it & 1;
       hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap, 0);
       Handles h;
@@ -428,8 +478,22 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       FVC_HIP(hipGetLastError());
       h.comp = P.end(ev, s, "composite");
       hs.push_back(h);
-    }
-    if (read_counter(r, CTL_ALIVE + (it & 1), s) == 0) break;
+  };
+  int it = 0;
+  bool done = n0 == 0;
+  if (graph) {
+    const int k = std::min(r.graph_iters[0] + 1, MAX_SUBITER);
+    graph_run(r, 0, s, [&] {
+      launch_init_rays_async(r, vs, s);
+      for (int i = 0; i < k; i++) iteration(i);
+    });
+    it = k;
+    done = read_counter(r, CTL_ALIVE + (it & 1), s) == 0;
+  }
+  while (!done && it < MAX_SUBITER) {
+    const int burst = next_burst(r, it);
+    for (int b = 0; b < burst && it < MAX_SUBITER; b++, it++) iteration(it);
+    done = read_counter(r, CTL_ALIVE + (it & 1), s) == 0;
   }
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap, 1);
   ev = P.begin(s);
@@ -449,6 +513,7 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
     n_eval += l.n_r;
   }
   r.last_iters = n_real;
+  r.graph_iters[0] = n_real;
   P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));

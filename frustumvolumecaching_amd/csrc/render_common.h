@@ -142,6 +142,7 @@ MarchGrid march_grid(Renderer& r, const fvc_raymarch_info* rms, int n_views);
 int read_counter(Renderer& r, int idx, hipStream_t s);
 void zero_counters(Renderer& r, hipStream_t s);
 int launch_init_rays(Renderer& r, const Views& vs, hipStream_t s);
+void launch_init_rays_async(Renderer& r, const Views& vs, hipStream_t s);
 void prepare_views(Renderer& r, const Views& vs);
 // per-iteration log entries {alive, n_steps, resample, cache, bins} of a frame
 struct IterLog { int alive, n_steps, n_r, n_c, bins; };

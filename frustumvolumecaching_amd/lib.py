@@ -51,6 +51,7 @@ _SIGS = {
     "fvc_resize_renderbuffers": (C.c_int, [_P, C.c_int, C.c_int]),
     "fvc_renderer_autotune": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int, C.POINTER(abi.TuneResult)]),
     "fvc_renderer_set_mlp_rows": (C.c_int, [_P, C.c_int32]),
+    "fvc_renderer_set_graph": (C.c_int, [_P, C.c_int]),
     "fvc_renderer_set_batching": (C.c_int, [_P, C.c_int64, C.c_int32]),
     "fvc_render_full": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.POINTER(abi.RenderStats)]),
     "fvc_cache_fill": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.POINTER(abi.InitStats)]),
@@ -237,6 +238,10 @@ class Renderer:
         res = abi.TuneResult()
         check(load().fvc_renderer_autotune(self.h, C.byref(rm), int(bool(cached)), C.byref(res)))
         return res
+
+    def set_graph(self, enable=True):
+        """Record and replay each frame as a hipGraph."""
+        check(load().fvc_renderer_set_graph(self.h, int(bool(enable))))
 
     def set_mlp_rows(self, rows):
         check(load().fvc_renderer_set_mlp_rows(self.h, int(rows)))

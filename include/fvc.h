@@ -298,6 +298,14 @@ typedef struct {
  * rows per wavefront and sub-iteration chunk sizes, keeps the fastest for
  * this renderer and returns the table.  Images are not written. */
 int fvc_renderer_autotune(fvc_renderer* renderer, const fvc_raymarch_info* rm, int cached, fvc_tune_result* result);
+/* Graph mode (no counterpart in the reference): each full or cached frame
+ * is recorded with hipStreamBeginCapture into a hipGraph (ray setup and as
+ * many sub-iterations as the previous frame needed, plus one; kernels read
+ * the live counts on the device and idle when no ray is left) and replayed
+ * with one launch; the executable graph is updated in place from frame to
+ * frame.  A frame needing more sub-iterations continues with ordinary
+ * launches.  Profiling (fvc_profile_enable) renders without graphs. */
+int fvc_renderer_set_graph(fvc_renderer* renderer, int enable);
 /* MLP rows per wavefront (16 or 32) for the network and cache-head kernels. */
 int fvc_renderer_set_mlp_rows(fvc_renderer* renderer, int32_t rows);
 

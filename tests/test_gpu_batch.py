@@ -106,3 +106,33 @@ def test_autotune_keeps_images(setup):
     after = _img(rm)
     r.render_full(rm, after)
     assert np.array_equal(before.cpu().numpy(), after.cpu().numpy())
+
+
+def test_graph_mode_replays_equal_eager(setup):
+    """Frames recorded and replayed as hipGraphs (updated in place from frame
+    to frame, eager work in between) give the eager images; a frame that
+    needs more sub-iterations than recorded finishes with ordinary launches."""
+    wl, model, r, views = setup
+    path = [wl.rm(S.yaw(wl.cam2world, 0.5 * k), 160, 120) for k in range(6)]
+    r.initCacheBlockwise(path[0])
+    eager = []
+    for rm in path:
+        a, b = _img(rm), _img(rm)
+        r.render_full(rm, a)
+        r.render_cached(rm, b)
+        eager.append((a.cpu().numpy(), b.cpu().numpy()))
+    r.set_graph(True)
+    try:
+        small = wl.rm(wl.cam2world, 16, 12)
+        r.render_full(small, _img(small))     # few sub-iterations recorded ...
+        for k, rm in enumerate(path):         # ... then a larger frame continues eagerly
+            a, b = _img(rm), _img(rm)
+            r.render_full(rm, a)
+            torch.cuda.synchronize()
+            _ = torch.ones(1024, device="cuda").sum().item()  # eager work between replays
+            r.render_cached(rm, b)
+            torch.cuda.synchronize()
+            assert np.array_equal(a.cpu().numpy(), eager[k][0]), k
+            assert np.array_equal(b.cpu().numpy(), eager[k][1]), k
+    finally:
+        r.set_graph(False)
