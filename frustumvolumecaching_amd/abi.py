@@ -104,6 +104,15 @@ class InitStats(C.Structure):
                 ("overflow", C.c_int32), ("n_subiters", C.c_int32), ("n_samples_evaluated", C.c_int64)]
 
 
+FILL_IDLE, FILL_RUNNING, FILL_DONE = 0, 1, 2
+
+
+class RefillPolicy(C.Structure):
+    """fvc_refill_policy (the viewer's m_cache_miss_limit / m_cache_update_distance)."""
+    _fields_ = [("min_hit_ratio", C.c_float), ("max_distance", C.c_float), ("enabled", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
 TUNE_MAX = 16
 
 

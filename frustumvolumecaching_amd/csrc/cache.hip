@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "march.h"
 #include "mlp_mfma.h"
@@ -319,12 +320,20 @@ static int default_capacity(int n_rays) {
   return (est_bricks + per_layer - 1) / per_layer * per_layer;
 }
 
-void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats) {
+// The fill keeps its own counters, pinned mirror and scratch, so it can run
+// on the fill stream from another thread while frames render from the
+// other cache (double-buffered renderers).
+void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats, hipStream_t s, Profiler& P) {
   const Model& m = *r.model;
   CacheStore& C = r.caches[r.fill_idx];
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
-  hipStream_t s = r.stream();
-  if (!r.counters.p || r.width != W || r.height != H) resize_renderbuffers(r, W, H);
+  r.f_counters.ensure(CTL_LOG * sizeof(int));
+  if (!r.f_pinned) FVC_HIP(hipHostMalloc((void**)&r.f_pinned, 64 * sizeof(int), hipHostMallocDefault));
+  auto read_counter = [&](Renderer&, int idx, hipStream_t st) {
+    FVC_HIP(hipMemcpyAsync(r.f_pinned + idx, r.f_counters.as<int>() + idx, sizeof(int), hipMemcpyDeviceToHost, st));
+    FVC_HIP(hipStreamSynchronize(st));
+    return r.f_pinned[idx];
+  };
 
   // resize (Cache::resize, cache.cu:44-84), sparse pool sized by the estimate
   C.rm = rm;
@@ -347,14 +356,13 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats)
   r.f_meta.ensure(f_cap * sizeof(int));
   r.f_out16.ensure(f_cap * 16 * sizeof(uint16_t));
 
-  zero_counters(r, s);
-  Profiler& P = r.prof;
+  FVC_HIP(hipMemsetAsync(r.f_counters.p, 0, CTL_LOG * sizeof(int), s));
   hipEvent_t ev = P.begin(s);
   hipLaunchKernelGGL(k_fill_init, dim3(n_tiles), dim3(64), 0, s, rm, cvt(m.scene.aabb_from), cvt(m.scene.aabb_to),
                      C.bx, r.f_rays.as<FillRay>(), r.f_tiles.as<FillTile>());
   FVC_HIP(hipGetLastError());
   P.end(ev, s, "fill_init", n_tiles * 64.0, n_tiles * (64.0 * sizeof(FillRay) + sizeof(FillTile)), 0);
-  int* cnt = r.counters.as<int>();
+  int* cnt = r.f_counters.as<int>();
   int subiter = 0;
   int64_t n_eval = 0;
   for (; subiter < 1000; subiter++) {  // CacheRenderer::MAX_SUBITER
@@ -408,6 +416,7 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats)
   const int overflow = read_counter(r, CNT_OVERFLOW, s);
   C.n_used = std::min(used, C.capacity);
   C.initialized = true;
+  FVC_HIP(hipStreamSynchronize(s));
   P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
@@ -847,6 +856,102 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
 
 void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
   render_cached(r, Views(&rm, 1, &rgba), stats);
+}
+
+void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats) {
+  cache_fill(r, rm, stats, r.stream(), r.prof);
+}
+
+// ---------------------------------------------------------------- refill
+// Background refill of a double-buffered renderer, the viewer's update
+// thread (application.cu:168-184): the fill of the back cache runs on its
+// own thread and stream (holding the write lock) while frames render from
+// the front cache; a finished fill is published by a swap.
+void fill_async(Renderer& r, const fvc_raymarch_info& rm) {
+  if (!r.double_buffered) throw Error(FVC_E_INVALID, "an asynchronous fill needs a double-buffered renderer");
+  if (r.fill_state.load() == FILL_RUNNING) throw Error(FVC_E_INVALID, "a fill is already running");
+  if (r.fill_thread.joinable()) r.fill_thread.join();
+  if (!r.fill_stream) FVC_HIP(hipStreamCreateWithFlags(&r.fill_stream, hipStreamNonBlocking));
+  r.fill_state = FILL_RUNNING;
+  r.fill_error.clear();
+  r.fill_thread = std::thread([&r, rm] {
+    try {
+      FVC_HIP(hipSetDevice(r.model->device));
+      std::scoped_lock lk(r.write_mutex);
+      cache_fill(r, rm, &r.fill_stats, r.fill_stream, r.fill_prof);
+      r.fill_pose = translation(rm.cam_info.cam2world);
+      r.fill_state = FILL_DONE;
+    } catch (const std::exception& e) {
+      r.fill_error = e.what();
+      r.fill_state = FILL_FAILED;
+    }
+  });
+}
+
+// state of the background fill; with wait, blocks until it has finished
+int fill_poll(Renderer& r, bool wait, fvc_init_stats* stats) {
+  if (wait && r.fill_thread.joinable()) r.fill_thread.join();
+  const int st = r.fill_state.load();
+  if (st == FILL_FAILED) {
+    if (r.fill_thread.joinable()) r.fill_thread.join();
+    r.fill_state = FILL_IDLE;
+    throw Error(FVC_E_HIP, "background fill failed: " + r.fill_error);
+  }
+  if (st == FILL_DONE && stats) *stats = r.fill_stats;
+  return st;
+}
+
+// publish a finished background fill (swap under both locks)
+bool fill_publish(Renderer& r) {
+  if (r.fill_state.load() != FILL_DONE) return false;
+  if (r.fill_thread.joinable()) r.fill_thread.join();
+  std::scoped_lock lk(r.read_mutex, r.write_mutex);
+  std::swap(r.render_idx, r.fill_idx);
+  r.front_pose = r.fill_pose;
+  r.have_front_pose = true;
+  r.fill_state = FILL_IDLE;
+  return true;
+}
+
+// Render from the cache and apply the refill policy of the viewer
+// (application.cu:203-213): refill when the frame's cache hit ratio drops
+// below min_hit_ratio or the camera is farther than max_distance from the
+// pose of the front cache.  The first frame fills synchronously.
+void render_cached_auto(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats,
+                        int* refill_started) {
+  if (refill_started) *refill_started = 0;
+  fill_publish(r);
+  if (!r.caches[r.render_idx].initialized) {
+    {
+      std::scoped_lock lk(r.write_mutex);
+      cache_fill(r, rm, nullptr);
+    }
+    std::scoped_lock lk(r.read_mutex, r.write_mutex);
+    if (r.double_buffered) std::swap(r.render_idx, r.fill_idx);
+    r.front_pose = translation(rm.cam_info.cam2world);
+    r.have_front_pose = true;
+  }
+  fvc_render_stats st{};
+  {
+    std::scoped_lock lk(r.read_mutex);
+    render_cached(r, rm, rgba, &st);
+  }
+  if (stats) *stats = st;
+  const fvc_refill_policy& pol = r.policy;
+  if (!pol.enabled) return;
+  const float hit = st.samples_ppx > 0.0f ? st.cache_samples_evaluated / st.samples_ppx : 1.0f;
+  const V3 d = translation(rm.cam_info.cam2world) - r.front_pose;
+  const bool moved = r.have_front_pose && std::sqrt(dot3(d, d)) > pol.max_distance;
+  if ((hit < pol.min_hit_ratio || moved) && r.fill_state.load() == FILL_IDLE) {
+    if (r.double_buffered) {
+      fill_async(r, rm);
+    } else {
+      std::scoped_lock lk(r.read_mutex, r.write_mutex);
+      cache_fill(r, rm, nullptr);
+      r.front_pose = translation(rm.cam_info.cam2world);
+    }
+    if (refill_started) *refill_started = 1;
+  }
 }
 
 }  // namespace fvc

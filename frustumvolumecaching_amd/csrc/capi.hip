@@ -396,8 +396,51 @@ int fvc_diff_cached_backward(fvc_renderer* h, const float* d_rgb, float* g_pos_e
 int fvc_cache_swap(fvc_renderer* h) {
   return guard([&] {
     REQUIRE(h, "null renderer");
+    if (fill_publish(h->r)) return;  // a finished background fill
     std::scoped_lock lk(h->r.read_mutex, h->r.write_mutex);
     if (h->r.double_buffered) std::swap(h->r.render_idx, h->r.fill_idx);
+  });
+}
+
+int fvc_cache_fill_async(fvc_renderer* h, const fvc_raymarch_info* rm) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    check_rm(rm);
+    fill_async(h->r, *rm);
+  });
+}
+
+int fvc_cache_fill_poll(fvc_renderer* h, int wait, int32_t* state, fvc_init_stats* stats) {
+  return guard([&] {
+    REQUIRE(h && state, "null argument");
+    *state = fill_poll(h->r, wait != 0, stats);
+  });
+}
+
+int fvc_cache_publish(fvc_renderer* h, int32_t* swapped) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    const bool sw = fill_publish(h->r);
+    if (swapped) *swapped = sw ? 1 : 0;
+  });
+}
+
+int fvc_renderer_set_refill_policy(fvc_renderer* h, const fvc_refill_policy* p) {
+  return guard([&] {
+    REQUIRE(h && p, "null argument");
+    REQUIRE(p->min_hit_ratio >= 0.0f && p->min_hit_ratio <= 1.0f && p->max_distance > 0.0f, "invalid policy");
+    h->r.policy = *p;
+  });
+}
+
+int fvc_render_cached_auto(fvc_renderer* h, const fvc_raymarch_info* rm, uint8_t* rgba, fvc_render_stats* stats,
+                           int32_t* refill_started) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    check_rm(rm);
+    int started = 0;
+    render_cached_auto(h->r, *rm, rgba, stats, &started);
+    if (refill_started) *refill_started = started;
   });
 }
 

@@ -10,6 +10,8 @@
 #include <string>
 #include <functional>
 #include <vector>
+#include <atomic>
+#include <thread>
 
 #include "fvc_math.h"
 
@@ -310,8 +312,19 @@ struct Renderer {
   DevBuf s_pos, s_dir, s_t0, s_out;            // resample / full stream
   DevBuf c_frox, c_ray, c_t0, c_out;            // cache-hit stream (c_ray: ray index)
   DevBuf ray_sh;                                // 16 fp16 SH coefficients per ray and launch
-  // fill scratch
-  DevBuf f_rays, f_tiles, f_list, f_pos, f_t0, f_meta, f_out16;
+  // fill scratch (own counters and pinned mirror: the fill may run on the
+  // fill stream from another thread)
+  DevBuf f_rays, f_tiles, f_list, f_pos, f_t0, f_meta, f_out16, f_counters;
+  int* f_pinned = nullptr;
+  // background refill (double-buffered renderers)
+  std::thread fill_thread;
+  std::atomic<int> fill_state{0};  // FILL_*
+  std::string fill_error;
+  fvc_init_stats fill_stats{};
+  Profiler fill_prof;
+  V3 fill_pose{0, 0, 0}, front_pose{0, 0, 0};
+  bool have_front_pose = false;
+  fvc_refill_policy policy{0.9f, 1.0e30f, 0, 0};
   int fill_tiles = 0;
   int64_t fill_sample_capacity = 0;
   // graph mode: executable graphs of a full (0) and a cached (1) frame
@@ -319,6 +332,9 @@ struct Renderer {
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   int graph_iters[2] = {0, 0};  // sub-iterations of the last frame of each kind
   ~Renderer() {
+    if (fill_thread.joinable()) fill_thread.join();
+    if (f_pinned) (void)hipHostFree(f_pinned);
+    if (fill_stream) (void)hipStreamDestroy(fill_stream);
     for (hipGraphExec_t& g : graph_exec)
       if (g) (void)hipGraphExecDestroy(g);
     if (pinned) (void)hipHostFree(pinned);
@@ -332,6 +348,13 @@ struct Renderer {
 void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats);
 void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats);
 void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats);
+enum { FILL_IDLE = 0, FILL_RUNNING = 1, FILL_DONE = 2, FILL_FAILED = 3 };
+void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats, hipStream_t s, Profiler& P);
+void fill_async(Renderer& r, const fvc_raymarch_info& rm);
+int fill_poll(Renderer& r, bool wait, fvc_init_stats* stats);
+bool fill_publish(Renderer& r);
+void render_cached_auto(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats,
+                        int* refill_started);
 // autotuner of launch shapes (tune.hip)
 void autotune(Renderer& r, const fvc_raymarch_info& rm, bool cached, fvc_tune_result* out);
 // differentiable full / cached render (diff.hip)

@@ -57,6 +57,12 @@ _SIGS = {
     "fvc_cache_fill": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.POINTER(abi.InitStats)]),
     "fvc_render_cached": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.POINTER(abi.RenderStats)]),
     "fvc_cache_swap": (C.c_int, [_P]),
+    "fvc_cache_fill_async": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo)]),
+    "fvc_cache_fill_poll": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int32), C.POINTER(abi.InitStats)]),
+    "fvc_cache_publish": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "fvc_renderer_set_refill_policy": (C.c_int, [_P, C.POINTER(abi.RefillPolicy)]),
+    "fvc_render_cached_auto": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.POINTER(abi.RenderStats),
+                                         C.POINTER(C.c_int32)]),
     "fvc_render_full_batch": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, C.POINTER(_P),
                                         C.POINTER(abi.RenderStats)]),
     "fvc_render_cached_batch": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, C.POINTER(_P),
@@ -285,6 +291,31 @@ class Renderer:
 
     def render_cached_batch(self, rms, rgbas=None):
         return self._batch(load().fvc_render_cached_batch, rms, rgbas)
+
+    # background refill (double-buffered renderers)
+    def initCacheAsync(self, rm):
+        check(load().fvc_cache_fill_async(self.h, C.byref(rm)))
+
+    def fill_poll(self, wait=False):
+        st, stats = C.c_int32(), abi.InitStats()
+        check(load().fvc_cache_fill_poll(self.h, int(bool(wait)), C.byref(st), C.byref(stats)))
+        return st.value, stats
+
+    def publish(self):
+        sw = C.c_int32()
+        check(load().fvc_cache_publish(self.h, C.byref(sw)))
+        return bool(sw.value)
+
+    def set_refill_policy(self, min_hit_ratio=0.9, max_distance=1e30, enabled=True):
+        p = abi.RefillPolicy(float(min_hit_ratio), float(max_distance), int(bool(enabled)), 0)
+        check(load().fvc_renderer_set_refill_policy(self.h, C.byref(p)))
+
+    def render_cached_auto(self, rm, rgba=None):
+        """Cached frame with the refill policy; returns (stats, refill_started)."""
+        st, started = abi.RenderStats(), C.c_int32()
+        check(load().fvc_render_cached_auto(self.h, C.byref(rm), _addr(rgba), C.byref(st), C.byref(started)))
+        self.last_render_stats = st
+        return st, bool(started.value)
 
     def swapCache(self):
         check(load().fvc_cache_swap(self.h))

@@ -271,6 +271,33 @@ int fvc_render_full_batch(fvc_renderer* renderer, const fvc_raymarch_info* rms, 
                           fvc_render_stats* stats);
 int fvc_render_cached_batch(fvc_renderer* renderer, const fvc_raymarch_info* rms, int32_t n_views,
                             uint8_t* const* rgba, fvc_render_stats* stats);
+/* ---- background refill (double-buffered renderers) ----
+ * The viewer's cache update (application.cu:168-216) as library calls: the
+ * fill of the back cache runs on its own thread and stream while frames
+ * render from the front cache. */
+#define FVC_FILL_IDLE    0
+#define FVC_FILL_RUNNING 1
+#define FVC_FILL_DONE    2   /* finished, not yet published */
+typedef struct {
+  float   min_hit_ratio;          /* refill when cache samples / samples < this (m_cache_miss_limit) */
+  float   max_distance;           /* ... or the camera moved farther than this from the cache pose */
+  int32_t enabled;
+  int32_t pad_;
+} fvc_refill_policy;
+/* start filling the back cache from rm in the background */
+int fvc_cache_fill_async(fvc_renderer* renderer, const fvc_raymarch_info* rm);
+/* state of the background fill (FVC_FILL_*); wait = 1 blocks until it ends;
+ * stats of a finished fill */
+int fvc_cache_fill_poll(fvc_renderer* renderer, int wait, int32_t* state, fvc_init_stats* stats);
+/* swap a finished background fill to the front (*swapped = 1), else no-op */
+int fvc_cache_publish(fvc_renderer* renderer, int32_t* swapped);
+int fvc_renderer_set_refill_policy(fvc_renderer* renderer, const fvc_refill_policy* policy);
+/* render_cached with the policy: publishes a finished fill first, renders,
+ * and starts a background fill at rm when the policy triggers
+ * (*refill_started = 1).  The first call fills synchronously. */
+int fvc_render_cached_auto(fvc_renderer* renderer, const fvc_raymarch_info* rm, uint8_t* rgba_dev,
+                           fvc_render_stats* stats, int32_t* refill_started);
+
 /* CacheRenderer::swapCache (cache_renderer.cu:1053-1057) */
 int fvc_cache_swap(fvc_renderer* renderer);
 /* CacheRenderer::setCacheSettings (cache_renderer.h:266-267) */
