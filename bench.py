@@ -162,6 +162,7 @@ def main():
     _, cached_ms_prof, _, _, prof_cached = cached_path(True)
     full_ms_prof, _, prof_full = full_path(True)
     r.set_graph(False)
+    async_fps = async_refill_leg(a, wl, model, rm_of, img)
 
     # ---- PSNR: cached vs full (GPU) on frames 1, 5, 9 after a fill at frame 0
     from oracle.pyoracle import psnr_8bit, to_8bit
@@ -285,6 +286,7 @@ def main():
         "kernels_cached": kernel_table(prof_cached),
         "kernels_full": kernel_table(prof_full),
         "cpu_baseline": cpu,
+        "async_refill_fps": async_fps,
         "autotune": tune,
         "graph": not a.no_graph,
         "profiled_cached_fps": round(1e3 / float(np.mean(cached_ms_prof)), 3),
@@ -292,6 +294,40 @@ def main():
         "diff": None if a.no_diff else diff_leg(a),
     }
     print(json.dumps(out), flush=True)
+
+
+def async_refill_leg(a, wl, model, rm_of, img):
+    """The same path with the refills in the background (double-buffered
+    renderer): a fill of the current pose starts every 10 frames on the fill
+    stream and is published at the first frame after it finishes; frames keep
+    rendering from the previous cache meanwhile.  Frames/s over the whole
+    path, fills included."""
+    import torch
+    from frustumvolumecaching_amd import abi, lib
+    r2 = lib.Renderer(model, double_buffered=True)
+    r2.set_graph(not a.no_graph)
+    if not a.no_tune and not (a.batch or a.max_steps):
+        r2.initCacheBlockwise(rm_of(0))
+        r2.swapCache()
+        r2.autotune(rm_of(1), cached=True)
+    r2.initCacheBlockwise(rm_of(0))
+    r2.swapCache()
+    r2.render_cached(rm_of(1), img)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(a.steps):
+        if k % 10 == 0 and k > 0:
+            r2.initCacheAsync(rm_of(k))
+        if r2.fill_poll()[0] == abi.FILL_DONE:
+            r2.publish()
+        r2.render_cached(rm_of(k), img)
+    r2.fill_poll(wait=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    r2.close()
+    fps = a.steps / dt
+    log(f"[bench] background refill: {fps:.1f} frames/s over {a.steps} frames incl. fills")
+    return round(fps, 3)
 
 
 def diff_leg(a, steps=3):
