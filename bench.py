@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default launch shapes")
     ap.add_argument("--no-graph", action="store_true", help="launch frames kernel by kernel (no hipGraph replay)")
+    ap.add_argument("--cache-latents", choices=["fp16", "fp8"], default="fp16",
+                    help="latent storage of the brick pool (fp8: E4M3 with a per-brick scale)")
     ap.add_argument("--diff-rays", type=int, default=1 << 18, help="rays per differentiable step (800x800 view)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference point for rocprofv3 "
@@ -89,6 +91,7 @@ def main():
     wl.net.precision = {"fp16": abi.PREC_FP16, "bf16": abi.PREC_BF16, "fp8": abi.PREC_FP8}[a.precision]
     model = lib.NerfNetwork.from_workload(wl)
     r = lib.Renderer(model)
+    r.set_latent_format(a.cache_latents)
     if a.batch or a.max_steps:
         r.set_batching(a.batch or (1 << 25), a.max_steps or 32)
     W, H = a.width, a.height
@@ -264,6 +267,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": a.precision,
+        "cache_latents": a.cache_latents,
         "data": "synthetic (seeded scene and random-init weights; no dataset)",
         "config": {"workload": f"unbounded_{W}x{H}: hash grid L16 F2 T2^21, 5x128^3 occupancy cascades, 64 blobs "
                                f"to r=16, cone 1/256; 30-frame path yaw 1deg + 0.02 fwd/frame, refill every 10",

@@ -139,7 +139,7 @@ class RenderStats(C.Structure):
 class CacheInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "frustum_x", "frustum_y", "frustum_z", "bricks_x", "bricks_y", "bricks_z",
-        "capacity_bricks", "n_bricks_used", "initialized", "brick_size")]
+        "capacity_bricks", "n_bricks_used", "initialized", "brick_size", "latent_format")]
 
 
 class KernelStats(C.Structure):

@@ -235,10 +235,14 @@ __global__ __launch_bounds__(1024) void k_fill_alloc(const int* __restrict__ lis
 // brick write + slab advance (accumulateNew_cache_init_second_kernel,
 // cache_renderer.cu:405-528).  Each lane owns one (x,y) column of the brick
 // and writes its 8 voxels; voxels without a sample are zero.
+// With FP8 latents (pool8 / scales) the brick's scale is the absmax of the
+// latents written into it over 448 (1 when all are zero); each latent is
+// stored as the E4M3 code (round to nearest even) of l / scale.
 __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list, int bx, int by, int latent_offset,
                                                    const uint16_t* __restrict__ out16, const int* __restrict__ f_meta,
                                                    const FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
-                                                   int* __restrict__ table, Voxel* __restrict__ pool) {
+                                                   int* __restrict__ table, Voxel* __restrict__ pool,
+                                                   Voxel8* __restrict__ pool8, float* __restrict__ scales) {
   const int t = list[blockIdx.x], lane = threadIdx.x;
   FillTile tile = tiles[t];
   if (!tile.processed) return;
@@ -247,7 +251,47 @@ __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list,
   if (tile.visible > 0) {
     overflow = tile.overflow != 0;
     if (lane == 0) table[((int64_t)tile.slab * by + t / bx) * bx + t % bx] = overflow ? -1 : tile.slot;
-    if (!overflow) {
+    if (!overflow && pool8) {
+      int amax_bits = 0;  // |latent| as float bits (non-negative floats order as integers)
+      for (int k = 0; k < r.cnt; k++) {
+        const uint16_t* o16 = out16 + ((int64_t)r.off + k) * 16;
+        for (int i = 0; i < 8; i++) {
+          const int b = __builtin_bit_cast(int, fabsf(h2f(o16[latent_offset + i])));
+          amax_bits = b > amax_bits ? b : amax_bits;
+        }
+      }
+      amax_bits = -wave_min(-amax_bits);
+      const float amax = __builtin_bit_cast(float, amax_bits);
+      const float scale = amax > 0.0f ? amax / 448.0f : 1.0f;
+      if (lane == 0) scales[tile.slot] = scale;
+      Voxel8* brick = pool8 + (int64_t)tile.slot * BRICK_VOXELS;
+      int k = 0;
+      for (int z = 0; z < BRICK; z++) {
+        Voxel8 v;
+        const bool has = k < r.cnt && f_meta[r.off + k] == z;
+        if (has) {
+          const int64_t q = (int64_t)r.off + k;
+          const uint16_t* o16 = out16 + q * 16;
+          float qv[8];
+          for (int i = 0; i < 8; i++) qv[i] = fminf(fmaxf(h2f(o16[latent_offset + i]) / scale, -448.0f), 448.0f);
+          int lo = __builtin_amdgcn_cvt_pk_fp8_f32(qv[0], qv[1], 0, false);
+          lo = __builtin_amdgcn_cvt_pk_fp8_f32(qv[2], qv[3], lo, true);
+          int hi = __builtin_amdgcn_cvt_pk_fp8_f32(qv[4], qv[5], 0, false);
+          hi = __builtin_amdgcn_cvt_pk_fp8_f32(qv[6], qv[7], hi, true);
+          v.lat[0] = (uint32_t)lo;
+          v.lat[1] = (uint32_t)hi;
+          v.density = expf(h2f(o16[0]));
+          v.isset = 1.0f;
+          k++;
+        } else {
+          v.lat[0] = v.lat[1] = 0;
+          v.density = 0.0f;
+          v.isset = 0.0f;
+        }
+        brick[z * 64 + lane] = v;
+      }
+      tile.n_bricks++;
+    } else if (!overflow) {
       Voxel* brick = pool + (int64_t)tile.slot * BRICK_VOXELS;
       int k = 0;
       for (int z = 0; z < BRICK; z++) {
@@ -343,7 +387,10 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
   const int64_t n_table = (int64_t)C.bx * C.by * C.bz;
   C.table.ensure(n_table * sizeof(int32_t));
   C.known.ensure((int64_t)W * H * sizeof(float));
-  C.pool.ensure(std::max<int64_t>(1, (int64_t)C.capacity) * BRICK_VOXELS * sizeof(Voxel));
+  C.lat8 = r.latent_format == FVC_LATENT_FP8;
+  C.pool.ensure(std::max<int64_t>(1, (int64_t)C.capacity) * BRICK_VOXELS *
+                (C.lat8 ? sizeof(Voxel8) : sizeof(Voxel)));
+  C.scales.ensure(std::max<int64_t>(1, (int64_t)C.capacity) * sizeof(float));
   FVC_HIP(hipMemsetAsync(C.table.p, 0xff, n_table * sizeof(int32_t), s));
 
   const int n_tiles = C.bx * C.by;
@@ -403,7 +450,8 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
     ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_write, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), C.bx, C.by,
                        m.net.latent_offset, r.f_out16.as<uint16_t>(), r.f_meta.as<int>(), r.f_rays.as<FillRay>(),
-                       r.f_tiles.as<FillTile>(), C.table.as<int>(), C.pool.as<Voxel>());
+                       r.f_tiles.as<FillTile>(), C.table.as<int>(), C.lat8 ? nullptr : C.pool.as<Voxel>(),
+                       C.lat8 ? C.pool.as<Voxel8>() : nullptr, C.scales.as<float>());
     FVC_HIP(hipGetLastError());
     P.end(ev, s, "fill_write", n_samples, n_samples * 36.0 + n_alive * 64.0 * 48);
   }
@@ -502,7 +550,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
         const int vx = (int)floorf(f.x), vy = (int)floorf(f.y), vz = (int)floorf(f.z);
         const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
         if (slot >= 0 &&
-            cv.pool[(int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))].density > 0.0f)
+            vox_density(cv, (int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))) > 0.0f)
           cache_mask |= 1u << k;
       }
     }
@@ -568,9 +616,17 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
 // MLP (fragments FRAG_H1C).
 __device__ Voxel k_zero_voxel;  // zero-initialised, never written
 
+// 8 E4M3 codes (latent i in byte i % 4 of word i / 4) times the brick scale
+template <int B> __device__ __forceinline__ float fp8_byte(uint32_t w) { return __builtin_amdgcn_cvt_f32_fp8((int)w, B); }
+__device__ __forceinline__ void fp8x8_decode(uint32_t w0, uint32_t w1, float s, float* out) {
+  out[0] = fp8_byte<0>(w0) * s; out[1] = fp8_byte<1>(w0) * s; out[2] = fp8_byte<2>(w0) * s; out[3] = fp8_byte<3>(w0) * s;
+  out[4] = fp8_byte<0>(w1) * s; out[5] = fp8_byte<1>(w1) * s; out[6] = fp8_byte<2>(w1) * s; out[7] = fp8_byte<3>(w1) * s;
+}
+
 // TILES: 16-sample tiles per wave and loop iteration (MLP rows per
 // wavefront = 16 TILES); all tiles' corner loads are issued before the math.
-template <int P, int TILES>
+// LAT8: the pool holds FP8 latents (Voxel8) decoded as code * brick scale.
+template <int P, int TILES, bool LAT8>
 __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_frox, const int* __restrict__ c_ray,
                                                     const uint2* __restrict__ ray_sh, int64_t n, CacheView cv,
                                                     const typename Mx<P>::T* __restrict__ frags, MlpScales sc,
@@ -593,7 +649,7 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
     const int li = opaque(lane);
     int s[TILES];
     bool valid[TILES];
-    float w[TILES][2];
+    float w[TILES][2], scl[TILES][2];
     uint4 lat[TILES][2];
     float2 di[TILES][2];
     uint2 sh[TILES];
@@ -621,11 +677,19 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
         const bool inside = in_xy && pz >= 0.0f && pz < fz;
         const int vz = inside ? (int)pz : 0;
         const int slot = cv.table[(vz >> 3) * cv.by * cv.bx + col_xy];
-        const Voxel* vp = (inside && slot >= 0)
-                              ? cv.pool + ((size_t)slot * BRICK_VOXELS + (uint32_t)((vz & 7) * 64 + loc_xy))
-                              : &k_zero_voxel;
-        lat[t][cc] = *reinterpret_cast<const uint4*>(vp->lat);
-        di[t][cc] = *reinterpret_cast<const float2*>(&vp->density);
+        const bool ok = inside && slot >= 0;
+        const size_t vi = (size_t)slot * BRICK_VOXELS + (uint32_t)((vz & 7) * 64 + loc_xy);
+        if (LAT8) {
+          const uint4 q = *reinterpret_cast<const uint4*>(ok ? (const void*)(cv.pool8 + vi) : (const void*)&k_zero_voxel);
+          lat[t][cc] = make_uint4(q.x, q.y, 0u, 0u);
+          di[t][cc] = make_float2(__builtin_bit_cast(float, q.z), __builtin_bit_cast(float, q.w));
+          scl[t][cc] = ok ? cv.scales[slot] : 0.0f;
+        } else {
+          const Voxel* vp = ok ? cv.pool + vi : &k_zero_voxel;
+          lat[t][cc] = *reinterpret_cast<const uint4*>(vp->lat);
+          di[t][cc] = *reinterpret_cast<const float2*>(&vp->density);
+          scl[t][cc] = 0.0f;
+        }
       }
     }
 #pragma unroll
@@ -636,16 +700,23 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
         const uint32_t lw[4] = {lat[t][cc].x, lat[t][cc].y, lat[t][cc].z, lat[t][cc].w};
         const float ww = w[t][cc];
         const float cw = ww * di[t][cc].y, cd = ww * di[t][cc].x;
+        float lv[8];
+        if (LAT8) {
+          fp8x8_decode(lw[0], lw[1], scl[t][cc], lv);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; k++) lv[k] = h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
+        }
         if (cc == 0) {
           wd = cw;
           dd = cd;
 #pragma unroll
-          for (int k = 0; k < 8; k++) ld[k] = ww * h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
+          for (int k = 0; k < 8; k++) ld[k] = ww * lv[k];
         } else {
           wd = wd + cw;
           dd = dd + cd;
 #pragma unroll
-          for (int k = 0; k < 8; k++) ld[k] = ld[k] + ww * h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
+          for (int k = 0; k < 8; k++) ld[k] = ld[k] + ww * lv[k];
         }
       }
       wd = xor32_sum(xor16_sum(wd));
@@ -776,10 +847,13 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
                           ctl + CTL_R, r.mlp_rows);
       h.net = P.end(ev, s, "network_resample");
       ev = P.begin(s);
-#define L2(P, TL)                                                                                                    \
-  hipLaunchKernelGGL((k_cache_head<P, TL>), dim3(head_blocks), dim3(256), 0, s, r.c_frox.as<float>(),                 \
+#define L3(P, TL, F8)                                                                                                \
+  hipLaunchKernelGGL((k_cache_head<P, TL, F8>), dim3(head_blocks), dim3(256), 0, s, r.c_frox.as<float>(),             \
                      r.c_ray.as<int>(), r.ray_sh.as<uint2>(), (int64_t)n_bound, cv, m.frags.as<typename Mx<P>::T>(),  \
                      m.mlp_scales, r.c_out.as<uint2>(), ctl + CTL_C)
+#define L2(P, TL)                      \
+  if (cv.lat8) { L3(P, TL, true); }    \
+  else { L3(P, TL, false); }
 #define L(P)                        \
   if (r.mlp_rows == 32) { L2(P, 2); } \
   else { L2(P, 1); }
@@ -790,6 +864,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       }
 #undef L
 #undef L2
+#undef L3
       FVC_HIP(hipGetLastError());
       h.head = P.end(ev, s, "cache_head");
       ev = P.begin(s);

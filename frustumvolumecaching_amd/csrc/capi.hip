@@ -454,6 +454,61 @@ int fvc_cache_set_settings(fvc_renderer* h, const fvc_cache_settings* s) {
   });
 }
 
+static CacheStore& pick(fvc_renderer* h, int which);
+
+int fvc_cache_set_latent_format(fvc_renderer* h, int format) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    REQUIRE(format == FVC_LATENT_FP16 || format == FVC_LATENT_FP8, "unknown latent format");
+    std::scoped_lock lk(h->r.write_mutex);
+    h->r.latent_format = format;
+  });
+}
+
+// OCP E4M3 code -> float (bias 7, no infinities, 0x7f / 0xff NaN)
+static float e4m3_value(uint8_t c) {
+  const int sgn = c >> 7, e = (c >> 3) & 15, m = c & 7;
+  float v;
+  if (e == 15 && m == 7) v = NAN;
+  else if (e == 0) v = std::ldexp((float)m, -9);
+  else v = std::ldexp((float)(8 + m), e - 10);
+  return sgn ? -v : v;
+}
+
+int fvc_cache_get_pool_decoded(fvc_renderer* h, int which, float* out, int64_t n) {
+  return guard([&] {
+    REQUIRE(h && out, "null argument");
+    const CacheStore& C = pick(h, which);
+    if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
+    const int64_t nv = (int64_t)C.n_used * BRICK_VOXELS;
+    REQUIRE(n >= nv * 10, "buffer too small");
+    if (C.lat8) {
+      std::vector<Voxel8> v((size_t)nv);
+      std::vector<float> sc((size_t)std::max(C.n_used, 1));
+      if (nv) FVC_HIP(hipMemcpy(v.data(), C.pool.p, nv * sizeof(Voxel8), hipMemcpyDeviceToHost));
+      if (C.n_used) FVC_HIP(hipMemcpy(sc.data(), C.scales.p, C.n_used * sizeof(float), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < nv; i++) {
+        for (int k = 0; k < 8; k++)
+          out[i * 10 + k] = e4m3_value((uint8_t)(v[i].lat[k >> 2] >> (8 * (k & 3)))) * sc[i / BRICK_VOXELS];
+        out[i * 10 + 8] = v[i].density;
+        out[i * 10 + 9] = v[i].isset;
+      }
+    } else {
+      std::vector<Voxel> v((size_t)nv);
+      if (nv) FVC_HIP(hipMemcpy(v.data(), C.pool.p, nv * sizeof(Voxel), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < nv; i++) {
+        for (int k = 0; k < 8; k++) {
+          _Float16 hv;
+          std::memcpy(&hv, &v[i].lat[k], 2);
+          out[i * 10 + k] = (float)hv;
+        }
+        out[i * 10 + 8] = v[i].density;
+        out[i * 10 + 9] = v[i].isset;
+      }
+    }
+  });
+}
+
 int fvc_cache_set_capacity(fvc_renderer* h, int32_t cap) {
   return guard([&] {
     REQUIRE(h && cap >= 0, "invalid argument");
@@ -486,6 +541,7 @@ int fvc_cache_get_info(fvc_renderer* h, int which, fvc_cache_info* info) {
     info->n_bricks_used = C.n_used;
     info->initialized = C.initialized;
     info->brick_size = BRICK;
+    info->latent_format = C.lat8 ? FVC_LATENT_FP8 : FVC_LATENT_FP16;
   });
 }
 
@@ -516,7 +572,7 @@ int fvc_cache_get_pool(fvc_renderer* h, int which, void* out, int64_t n_bytes) {
     REQUIRE(h && out, "null argument");
     const CacheStore& C = pick(h, which);
     if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
-    const int64_t have = (int64_t)C.n_used * BRICK_VOXELS * (int64_t)sizeof(Voxel);
+    const int64_t have = (int64_t)C.n_used * BRICK_VOXELS * (int64_t)(C.lat8 ? sizeof(Voxel8) : sizeof(Voxel));
     REQUIRE(n_bytes <= have, "request exceeds the used part of the pool");
     FVC_HIP(hipMemcpy(out, C.pool.p, (size_t)n_bytes, hipMemcpyDeviceToHost));
   });

@@ -898,6 +898,8 @@ void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int 
   DiffState& D = r.diff;
   hipStream_t s = r.stream();
   if (cached && !r.caches[r.render_idx].initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
+  if (cached && r.caches[r.render_idx].lat8)
+    throw Error(FVC_E_UNSUPPORTED, "the differentiable cached render needs FP16 cache latents");
   if (D.frag_index.p == nullptr) {
     const std::vector<int> t = diff_frag_index();
     D.frag_index.ensure(t.size() * sizeof(int));

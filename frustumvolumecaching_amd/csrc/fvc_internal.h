@@ -143,8 +143,17 @@ struct alignas(16) Voxel {
   float pad[2];
 };
 
+// voxel record with FP8-E4M3 latents (16 bytes): codes of l / s with the
+// brick's scale s = absmax / 448 (CacheStore::scales)
+struct alignas(16) Voxel8 {
+  uint32_t lat[2];   // 8 E4M3 codes, latent i in byte i
+  float density;
+  float isset;
+};
+
 struct CacheStore {
   bool initialized = false;
+  bool lat8 = false;  // latents as FP8-E4M3 (Voxel8) with per-brick scales
   fvc_raymarch_info rm{};
   int fx = 0, fy = 0, fz = 0;
   int bx = 0, by = 0, bz = 0;
@@ -152,7 +161,8 @@ struct CacheStore {
   int n_used = 0;
   DevBuf table;     // int32 [bz][by][bx]
   DevBuf known;     // float [fy][fx]
-  DevBuf pool;      // Voxel [capacity][512]
+  DevBuf pool;      // Voxel [capacity][512], or Voxel8 with lat8
+  DevBuf scales;    // float [capacity] (lat8)
 };
 
 // ---------------------------------------------------------------- profiling
@@ -249,8 +259,14 @@ struct CacheView {
   int fx, fy, fz, bx, by;
   const int* table;
   const float* known;
-  const Voxel* pool;
+  const Voxel* pool;     // FP16 latents
+  const Voxel8* pool8;   // FP8 latents (lat8)
+  const float* scales;   // per brick (lat8)
+  int lat8;
 };
+__device__ __forceinline__ float vox_density(const CacheView& cv, int64_t v) {
+  return cv.lat8 ? cv.pool8[v].density : cv.pool[v].density;
+}
 
 // state of the differentiable render between forward and backward (diff.hip)
 struct DiffState {
@@ -291,6 +307,7 @@ struct Renderer {
   int64_t target_batch = 1 << 25;
   int max_steps = 32;
   int mlp_rows = 16;     // MLP rows (samples) per wavefront and iteration: 16 or 32
+  int latent_format = FVC_LATENT_FP16;  // latent storage of the next fills
   int last_iters = 0;    // sub-iterations of the previous frame (first burst length)
   // step table of the marchers (render.hip: step_table)
   DevBuf tab_t, tab_off;

@@ -79,6 +79,9 @@ inline CacheView cache_view(const CacheStore& C) {
   cv.table = C.table.as<int>();
   cv.known = C.known.as<float>();
   cv.pool = C.pool.as<Voxel>();
+  cv.pool8 = C.pool.as<Voxel8>();
+  cv.scales = C.scales.as<float>();
+  cv.lat8 = C.lat8 ? 1 : 0;
   return cv;
 }
 
@@ -103,7 +106,7 @@ __device__ __forceinline__ int classify_sample(const CacheView& cv, V3 oc, V3 dc
   if (!inside || unsampled) return SAMPLE_RESAMPLE;
   const int vx = (int)floorf(f.x), vy = (int)floorf(f.y), vz = (int)floorf(f.z);
   const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
-  if (slot >= 0 && cv.pool[(int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))].density > 0.0f)
+  if (slot >= 0 && vox_density(cv, (int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))) > 0.0f)
     return SAMPLE_CACHE;
   return SAMPLE_SKIP;
 }

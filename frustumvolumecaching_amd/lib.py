@@ -69,6 +69,8 @@ _SIGS = {
                                           C.POINTER(abi.RenderStats)]),
     "fvc_cache_set_settings": (C.c_int, [_P, C.POINTER(abi.CacheSettings)]),
     "fvc_cache_set_capacity": (C.c_int, [_P, C.c_int32]),
+    "fvc_cache_set_latent_format": (C.c_int, [_P, C.c_int]),
+    "fvc_cache_get_pool_decoded": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "fvc_get_last_result": (C.c_int, [_P, _P, C.c_int64]),
     "fvc_cache_get_info": (C.c_int, [_P, C.c_int, C.POINTER(abi.CacheInfo)]),
     "fvc_cache_get_brick_table": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
@@ -322,6 +324,18 @@ class Renderer:
 
     def setCacheSettings(self, settings):
         check(load().fvc_cache_set_settings(self.h, C.byref(settings)))
+
+    def set_latent_format(self, fmt):
+        """'fp16' or 'fp8' latents for the next fills."""
+        check(load().fvc_cache_set_latent_format(self.h, {"fp16": 0, "fp8": 1}[fmt]))
+
+    def pool_decoded(self, which=0):
+        """[n_bricks, 512, 10] float32: 8 latents, density, isset."""
+        import numpy as np
+        i = self.cache_info(which)
+        out = np.zeros((max(i.n_bricks_used, 0), 512, 10), np.float32)
+        check(load().fvc_cache_get_pool_decoded(self.h, which, _addr(out), out.size))
+        return out
 
     def set_cache_capacity(self, bricks):
         check(load().fvc_cache_set_capacity(self.h, int(bricks)))

@@ -173,6 +173,7 @@ typedef struct {
   int32_t n_bricks_used;
   int32_t initialized;
   int32_t brick_size;             /* 8 */
+  int32_t latent_format;          /* FVC_LATENT_FP16 / FVC_LATENT_FP8 */
 } fvc_cache_info;
 
 /* Per-kernel timing of the renderer's launches (HIP events; the reference's
@@ -302,6 +303,16 @@ int fvc_render_cached_auto(fvc_renderer* renderer, const fvc_raymarch_info* rm, 
 int fvc_cache_swap(fvc_renderer* renderer);
 /* CacheRenderer::setCacheSettings (cache_renderer.h:266-267) */
 int fvc_cache_set_settings(fvc_renderer* renderer, const fvc_cache_settings* settings);
+/* Latent storage of the brick pool for the next fills (no counterpart in
+ * the reference, which stores half4 latents): FVC_LATENT_FP16, or
+ * FVC_LATENT_FP8 = OCP E4M3 codes of l / s with one scale s = absmax / 448
+ * per brick (a 16-byte voxel instead of 32). */
+#define FVC_LATENT_FP16 0
+#define FVC_LATENT_FP8  1
+int fvc_cache_set_latent_format(fvc_renderer* renderer, int format);
+/* The used part of a cache's pool decoded to floats, [n_bricks][512][10]:
+ * 8 latents, density, isset (either latent format). */
+int fvc_cache_get_pool_decoded(fvc_renderer* renderer, int which, float* out_host, int64_t n_floats);
 /* Pool capacity override in bricks (0 = reference estimate n_rays/8 rounded
  * up to 16384, cache_renderer.cu:832-846 + cache.cu:143-146). */
 int fvc_cache_set_capacity(fvc_renderer* renderer, int32_t capacity_bricks);

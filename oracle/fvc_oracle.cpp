@@ -608,7 +608,23 @@ template <class R> struct Cache {
   std::vector<R> density;  // [slot][512]
   std::vector<R> isset;    // [slot][512]
   fvc_init_stats stats{};
+  bool lat_fp8 = false;
 };
+
+// FP8 cache latents: per brick the scale s = absmax / 448 of its latents
+// (1 when all are zero) and each latent replaced by e4m3(l / s) * s, the
+// E4M3 code (round to nearest even) of l / s decoded and rescaled in float.
+template <class R> void quantize_latents_fp8(Cache<R>& C) {
+  const int n = std::min(C.n_used, C.capacity);
+  for (int s = 0; s < n; s++) {
+    R* l = &C.lat[(size_t)s * 512 * 8];
+    float amax = 0.0f;
+    for (int i = 0; i < 512 * 8; i++) amax = std::max(amax, std::fabs((float)l[i]));
+    const float scale = amax > 0.0f ? amax / 448.0f : 1.0f;
+    for (int i = 0; i < 512 * 8; i++) l[i] = (R)(e4m3_to_f32(f32_to_e4m3((float)l[i] / scale)) * scale);
+  }
+  C.lat_fp8 = true;
+}
 
 // Frustum depth in steps: the reference's max_step_in_scene
 // (raymarch_common.h:298-303) padded to 4 (cache_renderer.cu:841-843),
@@ -955,6 +971,7 @@ template <class R> static void fill_info(const Cache<R>& C, fvc_cache_info* info
   info->n_bricks_used = std::min(C.n_used, C.capacity);
   info->initialized = 1;
   info->brick_size = BS;
+  info->latent_format = C.lat_fp8 ? 1 : 0;
 }
 
 struct CacheHandle {
@@ -1677,6 +1694,10 @@ CacheHandle* orc_cache_fill(const Model* M, const fvc_raymarch_info* rm, int mod
   return h;
 }
 void orc_cache_destroy(CacheHandle* h) { delete h; }
+void orc_cache_quantize_fp8(CacheHandle* h) {
+  if (h->d) quantize_latents_fp8(*h->d);
+  else quantize_latents_fp8(*h->f);
+}
 
 void orc_cache_info(const CacheHandle* h, fvc_cache_info* info) {
   if (h->d) fill_info(*h->d, info); else fill_info(*h->f, info);
@@ -1737,9 +1758,9 @@ int64_t orc_diff_render(const Model* M, const fvc_raymarch_info* rm, const int32
 // Float cache from exported arrays: table [bz][by][bx], known [fy][fx],
 // pool [n_slots][512][10] (8 latents, density, isset)
 CacheHandle* orc_cache_from_arrays(const fvc_raymarch_info* rm, const fvc_cache_info* info, const int32_t* table,
-                                   const float* known, const float* pool, int64_t n_slots) {
+                                   const float* known, const float* pool, int64_t n_slots, int mode) {
   auto* h = new CacheHandle();
-  h->mode = M_F32;
+  h->mode = mode == M_EMU16 ? M_EMU16 : M_F32;
   auto* C = new Cache<float>();
   C->rm = *rm;
   C->fx = info->frustum_x; C->fy = info->frustum_y; C->fz = info->frustum_z;

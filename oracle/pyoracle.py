@@ -78,7 +78,8 @@ def lib():
             "orc_diff_render": (C.c_int64, [P, C.POINTER(abi.RaymarchInfo), P, C.c_int64, P, P, P, C.c_int, P, P,
                                             P, P, P]),
             "orc_cache_from_arrays": (P, [C.POINTER(abi.RaymarchInfo), C.POINTER(abi.CacheInfo), P, P, P,
-                                          C.c_int64]),
+                                          C.c_int64, C.c_int]),
+            "orc_cache_quantize_fp8": (None, [P]),
             "orc_diff_render_cached": (C.c_int64, [P, P, C.POINTER(abi.RaymarchInfo), P, C.c_int64, P, P, P, C.c_int,
                                                    P, P, P, P, P, P, C.POINTER(C.c_int64)]),
         }
@@ -235,15 +236,22 @@ class Cache:
         lib().orc_cache_info(self.h, C.byref(self.info))
 
     @classmethod
-    def from_arrays(cls, model, rm, info, table, known, pool):
+    def from_arrays(cls, model, rm, info, table, known, pool, mode=F32):
         """Float cache with the given contents (the export layouts of
-        brick_table, known_ranges and pool)."""
+        brick_table, known_ranges and pool); mode F32 or EMU16 (the half
+        rounding of the lookup's head inputs in render)."""
         table = np.ascontiguousarray(table, np.int32)
         known = np.ascontiguousarray(known, np.float32)
         pool = np.ascontiguousarray(pool, np.float32)
         n_slots = pool.size // (512 * 10)
-        h = lib().orc_cache_from_arrays(C.byref(rm), C.byref(info), _p(table), _p(known), _p(pool), n_slots)
-        return cls(model, rm, F32, 0, _handle=h)
+        h = lib().orc_cache_from_arrays(C.byref(rm), C.byref(info), _p(table), _p(known), _p(pool), n_slots, mode)
+        return cls(model, rm, mode, 0, _handle=h)
+
+    def quantize_fp8(self):
+        """FP8-E4M3 latents with a per-brick absmax scale (in place)."""
+        lib().orc_cache_quantize_fp8(self.h)
+        lib().orc_cache_info(self.h, C.byref(self.info))
+        return self
 
     def diff_render(self, rm, pix, params32, f64=True, d_rgb=None):
         """Differentiable cached render (float cache).  Returns (rgb [n, 3],
