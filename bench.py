@@ -42,6 +42,9 @@ def parse():
                     help="MLP arithmetic (operands; FP32 accumulation)")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default launch shapes")
+    ap.add_argument("--mlp-rows", type=int, choices=[16, 32], default=0,
+                    help="MLP rows per wavefront (with --batch / --max-steps: fixed shapes, no autotuning)")
+    ap.add_argument("--no-refill", action="store_true", help="skip the background-refill leg")
     ap.add_argument("--no-graph", action="store_true", help="launch frames kernel by kernel (no hipGraph replay)")
     ap.add_argument("--cache-latents", choices=["fp16", "fp8"], default="fp16",
                     help="latent storage of the brick pool (fp8: E4M3 with a per-brick scale)")
@@ -94,6 +97,8 @@ def main():
     r.set_latent_format(a.cache_latents)
     if a.batch or a.max_steps:
         r.set_batching(a.batch or (1 << 25), a.max_steps or 32)
+    if a.mlp_rows:
+        r.set_mlp_rows(a.mlp_rows)
     W, H = a.width, a.height
     n_px = W * H
     img = torch.zeros(n_px * 4, dtype=torch.uint8, device="cuda")
@@ -117,7 +122,7 @@ def main():
     torch.cuda.synchronize()
     log(f"[bench] warm-up done ({a.warmup} rounds)")
     tune = None
-    if not a.no_tune and not (a.batch or a.max_steps):
+    if not a.no_tune and not (a.batch or a.max_steps or a.mlp_rows):
         # autotuner: MLP rows per wavefront + chunk sizes, on a reprojected frame
         r.initCacheBlockwise(rm_of(0))
         t = r.autotune(rm_of(1), cached=True)
@@ -165,7 +170,7 @@ def main():
     _, cached_ms_prof, _, _, prof_cached = cached_path(True)
     full_ms_prof, _, prof_full = full_path(True)
     r.set_graph(False)
-    async_fps = async_refill_leg(a, wl, model, rm_of, img)
+    async_fps = None if a.no_refill else async_refill_leg(a, wl, model, rm_of, img)
 
     # ---- PSNR: cached vs full (GPU) on frames 1, 5, 9 after a fill at frame 0
     from oracle.pyoracle import psnr_8bit, to_8bit
@@ -310,7 +315,11 @@ def async_refill_leg(a, wl, model, rm_of, img):
     from frustumvolumecaching_amd import abi, lib
     r2 = lib.Renderer(model, double_buffered=True)
     r2.set_graph(not a.no_graph)
-    if not a.no_tune and not (a.batch or a.max_steps):
+    if a.batch or a.max_steps:
+        r2.set_batching(a.batch or (1 << 25), a.max_steps or 32)
+    if a.mlp_rows:
+        r2.set_mlp_rows(a.mlp_rows)
+    if not a.no_tune and not (a.batch or a.max_steps or a.mlp_rows):
         r2.initCacheBlockwise(rm_of(0))
         r2.swapCache()
         r2.autotune(rm_of(1), cached=True)
