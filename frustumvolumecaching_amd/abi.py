@@ -84,6 +84,9 @@ class NetDesc(C.Structure):
         ("head_width", C.c_int32), ("head_n_hidden_layers", C.c_int32),
         ("n_latents", C.c_int32), ("latent_offset", C.c_int32), ("sh_degree", C.c_int32),
         ("precision", C.c_int32),
+        ("split_head", C.c_int32), ("first_head_width", C.c_int32), ("first_head_n_hidden_layers", C.c_int32),
+        ("first_head_n_input", C.c_int32), ("first_latent_is_density", C.c_int32), ("first_sh_degree", C.c_int32),
+        ("freq_encoding_degree", C.c_int32), ("pad_", C.c_int32),
     ]
 
 

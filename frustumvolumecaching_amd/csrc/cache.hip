@@ -96,7 +96,8 @@ __global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list
                                                     const uint32_t* __restrict__ coarse_g, int bz,
                                                     FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
                                                     float* __restrict__ f_pos, float* __restrict__ f_t0,
-                                                    int* __restrict__ f_meta, int* __restrict__ counter) {
+                                                    int* __restrict__ f_meta, int* __restrict__ counter,
+                                                    float* __restrict__ f_dir) {
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
   load_coarse(coarse_g, occ.nlvl, coarse);
   const int t = list[blockIdx.x], lane = threadIdx.x;
@@ -148,6 +149,10 @@ __global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list
         f_pos[3 * q] = p.x; f_pos[3 * q + 1] = p.y; f_pos[3 * q + 2] = p.z;
         f_t0[q] = t0;
         f_meta[q] = j;
+        if (f_dir) {  // split models: the fill ray is the initial view direction (cache_renderer.cu:899)
+          const V3 d01 = unit_to_01(d);
+          f_dir[3 * q] = d01.x; f_dir[3 * q + 1] = d01.y; f_dir[3 * q + 2] = d01.z;
+        }
         k++;
       }
     }
@@ -164,18 +169,27 @@ __global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list
   }
 }
 
+// half in row r of a sample's 16-half network output
+__device__ __forceinline__ float out16_row(const uint2* out16, int64_t q, int r) {
+  const uint2 u = out16[q * 4 + (r >> 2)];
+  const uint32_t w = (r & 2) ? u.y : u.x;
+  return h2f((uint16_t)((r & 1) ? w >> 16 : w & 0xffffu));
+}
+
 // first loop of accumulateNew_cache_init_second_kernel (cache_renderer.cu:358-405):
 // transmittance over the slab's samples and the count of visible samples.
+// sig_row: row of log sigma in the network output (0; split models 8).
 __global__ __launch_bounds__(64) void k_fill_accum(const int* __restrict__ list, fvc_stepsize_info st, float alpha_thre,
                                                    const uint2* __restrict__ out16, const float* __restrict__ f_t0,
-                                                   FillRay* __restrict__ rays, FillTile* __restrict__ tiles) {
+                                                   FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
+                                                   int sig_row) {
   const int t = list[blockIdx.x], lane = threadIdx.x;
   if (!tiles[t].processed) return;
   FillRay r = rays[(int64_t)t * 64 + lane];
   int vis = 0;
   for (int k = 0; k < r.cnt; k++) {
     const int64_t q = (int64_t)r.off + k;
-    const float ls = h2f(out16[q * 4].x & 0xffffu);
+    const float ls = out16_row(out16, q, sig_row);
     const float t0 = f_t0[q];
     const float dt = step_dt(st, t0);
     const float density = expf(ls);
@@ -238,11 +252,15 @@ __global__ __launch_bounds__(1024) void k_fill_alloc(const int* __restrict__ lis
 // With FP8 latents (pool8 / scales) the brick's scale is the absmax of the
 // latents written into it over 448 (1 when all are zero); each latent is
 // stored as the E4M3 code (round to nearest even) of l / scale.
+// Latents come from rows latent_offset.., log sigma from row sig_row; di
+// (DensityIntermediates): latent 0 is replaced by the density as half
+// (cache_renderer.cu:481-486, 501).
 __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list, int bx, int by, int latent_offset,
                                                    const uint16_t* __restrict__ out16, const int* __restrict__ f_meta,
                                                    const FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
                                                    int* __restrict__ table, Voxel* __restrict__ pool,
-                                                   Voxel8* __restrict__ pool8, float* __restrict__ scales) {
+                                                   Voxel8* __restrict__ pool8, float* __restrict__ scales,
+                                                   int sig_row, int di) {
   const int t = list[blockIdx.x], lane = threadIdx.x;
   FillTile tile = tiles[t];
   if (!tile.processed) return;
@@ -252,11 +270,15 @@ __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list,
     overflow = tile.overflow != 0;
     if (lane == 0) table[((int64_t)tile.slab * by + t / bx) * bx + t % bx] = overflow ? -1 : tile.slot;
     if (!overflow && pool8) {
+      // latent i of a sample's output (fp16 value)
+      auto lat = [&](const uint16_t* o16, int i) {
+        return (di && i == 0) ? (float)(_Float16)expf(h2f(o16[sig_row])) : h2f(o16[latent_offset + i]);
+      };
       int amax_bits = 0;  // |latent| as float bits (non-negative floats order as integers)
       for (int k = 0; k < r.cnt; k++) {
         const uint16_t* o16 = out16 + ((int64_t)r.off + k) * 16;
         for (int i = 0; i < 8; i++) {
-          const int b = __builtin_bit_cast(int, fabsf(h2f(o16[latent_offset + i])));
+          const int b = __builtin_bit_cast(int, fabsf(lat(o16, i)));
           amax_bits = b > amax_bits ? b : amax_bits;
         }
       }
@@ -273,14 +295,14 @@ __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list,
           const int64_t q = (int64_t)r.off + k;
           const uint16_t* o16 = out16 + q * 16;
           float qv[8];
-          for (int i = 0; i < 8; i++) qv[i] = fminf(fmaxf(h2f(o16[latent_offset + i]) / scale, -448.0f), 448.0f);
+          for (int i = 0; i < 8; i++) qv[i] = fminf(fmaxf(lat(o16, i) / scale, -448.0f), 448.0f);
           int lo = __builtin_amdgcn_cvt_pk_fp8_f32(qv[0], qv[1], 0, false);
           lo = __builtin_amdgcn_cvt_pk_fp8_f32(qv[2], qv[3], lo, true);
           int hi = __builtin_amdgcn_cvt_pk_fp8_f32(qv[4], qv[5], 0, false);
           hi = __builtin_amdgcn_cvt_pk_fp8_f32(qv[6], qv[7], hi, true);
           v.lat[0] = (uint32_t)lo;
           v.lat[1] = (uint32_t)hi;
-          v.density = expf(h2f(o16[0]));
+          v.density = expf(h2f(o16[sig_row]));
           v.isset = 1.0f;
           k++;
         } else {
@@ -301,7 +323,8 @@ __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list,
           const int64_t q = (int64_t)r.off + k;
           const uint16_t* o16 = out16 + q * 16;
           for (int i = 0; i < 8; i++) v.lat[i] = o16[latent_offset + i];
-          v.density = expf(h2f(o16[0]));
+          v.density = expf(h2f(o16[sig_row]));
+          if (di) v.lat[0] = __builtin_bit_cast(uint16_t, (_Float16)v.density);
           v.isset = 1.0f;
           k++;
         } else {
@@ -402,6 +425,11 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
   r.f_t0.ensure(f_cap * sizeof(float));
   r.f_meta.ensure(f_cap * sizeof(int));
   r.f_out16.ensure(f_cap * 16 * sizeof(uint16_t));
+  const bool split = m.net.split_head != 0;
+  if (split) r.f_dir.ensure(f_cap * 3 * sizeof(float));
+  // rows of the network output: log sigma and the first stored latent
+  const int sig_row = split ? 8 : 0, lat_row = split ? 0 : m.net.latent_offset;
+  const int di = split && m.net.first_latent_is_density;
 
   FVC_HIP(hipMemsetAsync(r.f_counters.p, 0, CTL_LOG * sizeof(int), s));
   hipEvent_t ev = P.begin(s);
@@ -425,7 +453,8 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
     hipLaunchKernelGGL(k_fill_sample, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), rm.stepsize_info, m.occ,
                        m.con, m.bitfield.as<uint64_t>(), m.coarse.as<uint32_t>(), C.bz, r.f_rays.as<FillRay>(),
                        r.f_tiles.as<FillTile>(),
-                       r.f_pos.as<float>(), r.f_t0.as<float>(), r.f_meta.as<int>(), cnt + CNT_SAMPLES);
+                       r.f_pos.as<float>(), r.f_t0.as<float>(), r.f_meta.as<int>(), cnt + CNT_SAMPLES,
+                       split ? r.f_dir.as<float>() : nullptr);
     FVC_HIP(hipGetLastError());
     const int hs = P.end(ev, s, "fill_sample");
     const int n_samples = read_counter(r, CNT_SAMPLES, s);
@@ -433,13 +462,17 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
     if (n_samples <= 0) break;
     n_eval += n_samples;
     ev = P.begin(s);
-    launch_network_latents(m, r.f_pos.as<float>(), n_samples, r.f_out16.as<uint16_t>(), s, r.mlp_rows);
-    P.end(ev, s, "network_latents", n_samples, n_samples * (ALG_GATHER_B + 12.0 + 32.0),
-          n_samples * ALG_DENSITY_FLOP);
+    if (split)
+      launch_network_latents_vd(m, r.f_pos.as<float>(), r.f_dir.as<float>(), n_samples, r.f_out16.as<uint16_t>(), s,
+                                r.mlp_rows);
+    else
+      launch_network_latents(m, r.f_pos.as<float>(), n_samples, r.f_out16.as<uint16_t>(), s, r.mlp_rows);
+    P.end(ev, s, "network_latents", n_samples, n_samples * (ALG_GATHER_B + (split ? 24.0 : 12.0) + 32.0),
+          n_samples * (ALG_DENSITY_FLOP + (split ? ALG_FIRST_HEAD_FLOP : 0.0)));
     ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_accum, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), rm.stepsize_info,
                        m.scene.alpha_thre, r.f_out16.as<uint2>(), r.f_t0.as<float>(), r.f_rays.as<FillRay>(),
-                       r.f_tiles.as<FillTile>());
+                       r.f_tiles.as<FillTile>(), sig_row);
     FVC_HIP(hipGetLastError());
     P.end(ev, s, "fill_accum", n_samples, n_samples * 12.0 + n_alive * 64.0 * 56, 0);
     ev = P.begin(s);
@@ -449,9 +482,9 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
     P.end(ev, s, "fill_alloc", n_alive, n_alive * 40.0, 0);
     ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_write, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), C.bx, C.by,
-                       m.net.latent_offset, r.f_out16.as<uint16_t>(), r.f_meta.as<int>(), r.f_rays.as<FillRay>(),
+                       lat_row, r.f_out16.as<uint16_t>(), r.f_meta.as<int>(), r.f_rays.as<FillRay>(),
                        r.f_tiles.as<FillTile>(), C.table.as<int>(), C.lat8 ? nullptr : C.pool.as<Voxel>(),
-                       C.lat8 ? C.pool.as<Voxel8>() : nullptr, C.scales.as<float>());
+                       C.lat8 ? C.pool.as<Voxel8>() : nullptr, C.scales.as<float>(), sig_row, di);
     FVC_HIP(hipGetLastError());
     P.end(ev, s, "fill_write", n_samples, n_samples * 36.0 + n_alive * 64.0 * 48);
   }
@@ -623,6 +656,89 @@ __device__ __forceinline__ void fp8x8_decode(uint32_t w0, uint32_t w1, float s, 
   out[4] = fp8_byte<0>(w1) * s; out[5] = fp8_byte<1>(w1) * s; out[6] = fp8_byte<2>(w1) * s; out[7] = fp8_byte<3>(w1) * s;
 }
 
+// The two corners of a lane's lookup: (x, y) = (ox, oy), z = 0 and 1 of
+// the cell of fc = froxel - 1/2; their weights, latent words, (density,
+// isset) and the brick scale (FP8 latents).
+struct CornerPair {
+  float w[2], scl[2];
+  uint4 lat[2];
+  float2 di[2];
+};
+template <bool LAT8>
+__device__ __forceinline__ void lookup_corners(const CacheView& cv, float fcx, float fcy, float fcz, float ox, float oy,
+                                               float fx, float fy, float fz, CornerPair& c) {
+  const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
+  const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
+  const float px = ix + ox, py = iy + oy;
+  const float wx = (1.0f - ox) * (1.0f - 2.0f * lx) + lx;
+  const float wy = (1.0f - oy) * (1.0f - 2.0f * ly) + ly;
+  const float wxy = wx * wy;
+  const bool in_xy = px >= 0.0f && px < fx && py >= 0.0f && py < fy;
+  const int vx = in_xy ? (int)px : 0, vy = in_xy ? (int)py : 0;
+  const int col_xy = (vy >> 3) * cv.bx + (vx >> 3), loc_xy = (vy & 7) * 8 + (vx & 7);
+#pragma unroll
+  for (int cc = 0; cc < 2; cc++) {
+    const float pz = iz + (float)cc;
+    const float wz = cc == 0 ? (1.0f - 2.0f * lz) + lz : lz;
+    c.w[cc] = wxy * wz;
+    const bool inside = in_xy && pz >= 0.0f && pz < fz;
+    const int vz = inside ? (int)pz : 0;
+    const int slot = cv.table[(vz >> 3) * cv.by * cv.bx + col_xy];
+    const bool ok = inside && slot >= 0;
+    const size_t vi = (size_t)slot * BRICK_VOXELS + (uint32_t)((vz & 7) * 64 + loc_xy);
+    if (LAT8) {
+      const uint4 q = *reinterpret_cast<const uint4*>(ok ? (const void*)(cv.pool8 + vi) : (const void*)&k_zero_voxel);
+      c.lat[cc] = make_uint4(q.x, q.y, 0u, 0u);
+      c.di[cc] = make_float2(__builtin_bit_cast(float, q.z), __builtin_bit_cast(float, q.w));
+      c.scl[cc] = ok ? cv.scales[slot] : 0.0f;
+    } else {
+      const Voxel* vp = ok ? cv.pool + vi : &k_zero_voxel;
+      c.lat[cc] = *reinterpret_cast<const uint4*>(vp->lat);
+      c.di[cc] = *reinterpret_cast<const float2*>(&vp->density);
+      c.scl[cc] = 0.0f;
+    }
+  }
+}
+
+// Weighted sums over the 8 corners of a sample (its 4 lanes): wd = sum w
+// isset and dd = sum w sigma on every lane, n0 / n1 = the sums of latents
+// (g&1) + 4(g>>1) and that + 2 (the reduce-scatter, see k_cache_head).
+template <bool LAT8>
+__device__ __forceinline__ void lookup_sums(const CornerPair& c, float& wd, float& dd, float& n0, float& n1) {
+  float ld[8];
+#pragma unroll
+  for (int cc = 0; cc < 2; cc++) {
+    const uint32_t lw[4] = {c.lat[cc].x, c.lat[cc].y, c.lat[cc].z, c.lat[cc].w};
+    const float ww = c.w[cc];
+    const float cw = ww * c.di[cc].y, cd = ww * c.di[cc].x;
+    float lv[8];
+    if (LAT8) {
+      fp8x8_decode(lw[0], lw[1], c.scl[cc], lv);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; k++) lv[k] = h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
+    }
+    if (cc == 0) {
+      wd = cw;
+      dd = cd;
+#pragma unroll
+      for (int k = 0; k < 8; k++) ld[k] = ww * lv[k];
+    } else {
+      wd = wd + cw;
+      dd = dd + cd;
+#pragma unroll
+      for (int k = 0; k < 8; k++) ld[k] = ld[k] + ww * lv[k];
+    }
+  }
+  wd = xor32_sum(xor16_sum(wd));
+  dd = xor32_sum(xor16_sum(dd));
+  // reduce-scatter: lane^16 pairs (0,1) (2,3) (4,5) (6,7), then lane^32
+  const float m0 = swap16_sum(ld[0], ld[1]), m1 = swap16_sum(ld[2], ld[3]);
+  const float m2 = swap16_sum(ld[4], ld[5]), m3 = swap16_sum(ld[6], ld[7]);
+  n0 = swap32_sum(m0, m2);
+  n1 = swap32_sum(m1, m3);
+}
+
 // TILES: 16-sample tiles per wave and loop iteration (MLP rows per
 // wavefront = 16 TILES); all tiles' corner loads are issued before the math.
 // LAT8: the pool holds FP8 latents (Voxel8) decoded as code * brick scale.
@@ -649,82 +765,21 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
     const int li = opaque(lane);
     int s[TILES];
     bool valid[TILES];
-    float w[TILES][2], scl[TILES][2];
-    uint4 lat[TILES][2];
-    float2 di[TILES][2];
+    CornerPair cp[TILES];
     uint2 sh[TILES];
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
       s[t] = (grp * TILES + t) * 16 + col;
       valid[t] = s[t] < n32;
       const int sl = valid[t] ? s[t] : n32 - 1;
-      const float fcx = c_frox[3 * sl] - 0.5f, fcy = c_frox[3 * sl + 1] - 0.5f, fcz = c_frox[3 * sl + 2] - 0.5f;
       sh[t] = ray_sh[c_ray[sl] * 4 + g];
-      const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
-      const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
-      const float px = ix + ox, py = iy + oy;
-      const float wx = (1.0f - ox) * (1.0f - 2.0f * lx) + lx;
-      const float wy = (1.0f - oy) * (1.0f - 2.0f * ly) + ly;
-      const float wxy = wx * wy;
-      const bool in_xy = px >= 0.0f && px < fx && py >= 0.0f && py < fy;
-      const int vx = in_xy ? (int)px : 0, vy = in_xy ? (int)py : 0;
-      const int col_xy = (vy >> 3) * cv.bx + (vx >> 3), loc_xy = (vy & 7) * 8 + (vx & 7);
-#pragma unroll
-      for (int cc = 0; cc < 2; cc++) {
-        const float pz = iz + (float)cc;
-        const float wz = cc == 0 ? (1.0f - 2.0f * lz) + lz : lz;
-        w[t][cc] = wxy * wz;
-        const bool inside = in_xy && pz >= 0.0f && pz < fz;
-        const int vz = inside ? (int)pz : 0;
-        const int slot = cv.table[(vz >> 3) * cv.by * cv.bx + col_xy];
-        const bool ok = inside && slot >= 0;
-        const size_t vi = (size_t)slot * BRICK_VOXELS + (uint32_t)((vz & 7) * 64 + loc_xy);
-        if (LAT8) {
-          const uint4 q = *reinterpret_cast<const uint4*>(ok ? (const void*)(cv.pool8 + vi) : (const void*)&k_zero_voxel);
-          lat[t][cc] = make_uint4(q.x, q.y, 0u, 0u);
-          di[t][cc] = make_float2(__builtin_bit_cast(float, q.z), __builtin_bit_cast(float, q.w));
-          scl[t][cc] = ok ? cv.scales[slot] : 0.0f;
-        } else {
-          const Voxel* vp = ok ? cv.pool + vi : &k_zero_voxel;
-          lat[t][cc] = *reinterpret_cast<const uint4*>(vp->lat);
-          di[t][cc] = *reinterpret_cast<const float2*>(&vp->density);
-          scl[t][cc] = 0.0f;
-        }
-      }
+      lookup_corners<LAT8>(cv, c_frox[3 * sl] - 0.5f, c_frox[3 * sl + 1] - 0.5f, c_frox[3 * sl + 2] - 0.5f, ox, oy, fx,
+                           fy, fz, cp[t]);
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
-      float wd = 0.0f, dd = 0.0f, ld[8];
-#pragma unroll
-      for (int cc = 0; cc < 2; cc++) {
-        const uint32_t lw[4] = {lat[t][cc].x, lat[t][cc].y, lat[t][cc].z, lat[t][cc].w};
-        const float ww = w[t][cc];
-        const float cw = ww * di[t][cc].y, cd = ww * di[t][cc].x;
-        float lv[8];
-        if (LAT8) {
-          fp8x8_decode(lw[0], lw[1], scl[t][cc], lv);
-        } else {
-#pragma unroll
-          for (int k = 0; k < 8; k++) lv[k] = h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
-        }
-        if (cc == 0) {
-          wd = cw;
-          dd = cd;
-#pragma unroll
-          for (int k = 0; k < 8; k++) ld[k] = ww * lv[k];
-        } else {
-          wd = wd + cw;
-          dd = dd + cd;
-#pragma unroll
-          for (int k = 0; k < 8; k++) ld[k] = ld[k] + ww * lv[k];
-        }
-      }
-      wd = xor32_sum(xor16_sum(wd));
-      dd = xor32_sum(xor16_sum(dd));
-      // reduce-scatter: lane^16 pairs (0,1) (2,3) (4,5) (6,7), then lane^32
-      const float m0 = swap16_sum(ld[0], ld[1]), m1 = swap16_sum(ld[2], ld[3]);
-      const float m2 = swap16_sum(ld[4], ld[5]), m3 = swap16_sum(ld[6], ld[7]);
-      const float n0 = swap32_sum(m0, m2), n1 = swap32_sum(m1, m3);
+      float wd, dd, n0, n1;
+      lookup_sums<LAT8>(cp[t], wd, dd, n0, n1);
       // latents sum w*l / sum w*isset (re-weighted)
       const T x = head_input_cache<P>((_Float16)(n0 / wd), (_Float16)(n1 / wd), sh[t], sc.inv_act);
       const f4 o = head_mlp<P>(wl, wl + 4 * 64, li, x, sc);
@@ -734,6 +789,90 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
         v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
               ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[1]) << 16);
         v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[2]) |
+              ((uint32_t)__builtin_bit_cast(uint16_t, logsig) << 16);
+        c_out[s[t]] = v;
+      }
+    }
+  }
+}
+
+// Split models: the lookup (FP16 latents) with the custom colour head
+// (generateNetworkInput_cache_padded_kernel / _shfl_perf_kernel_cg +
+// inferenceHead, cache_performance_renderer.cu:323-410, 678-787).  Per
+// sample the midpoint t0 + dt/2 on its ray gives the position of the
+// frequency encoding and the initial view direction (normalize(point - cache
+// camera position), 292-312).  di (DensityIntermediates): latent 0 holds the
+// density, so latent 0 = log(sum w l_0) * W / W with W = sum w isset
+// (770-779, no clamp) and it is also the sample's log density.
+template <int TILES>
+__global__ __launch_bounds__(256) void k_cache_head_vd(const float* __restrict__ c_frox, const int* __restrict__ c_ray,
+                                                       const float* __restrict__ c_t0, const uint2* __restrict__ ray_sh,
+                                                       const RayState* __restrict__ rays, int64_t n, CacheView cv,
+                                                       fvc_stepsize_info st, ContractParams con,
+                                                       const h8* __restrict__ frags, int di, int freq_on,
+                                                       uint2* __restrict__ c_out, const int* __restrict__ n_dev) {
+  if (n_dev) n = *n_dev;
+  if ((int64_t)blockIdx.x * 4 * 16 * TILES >= n) return;
+  __shared__ h8 wl[10 * 64];  // VD_HB (4), VD_HO (2), VD_HAC (4)
+  stage_frags(frags + VD_HB * 64, wl, 10);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  const int n32 = (int)n;
+  const int n_groups = (n32 + 16 * TILES - 1) / (16 * TILES);
+  const float ox = (float)(g >> 1), oy = (float)(g & 1);
+  const float fx = (float)cv.fx, fy = (float)cv.fy, fz = (float)cv.fz;
+  const V3 cam_pos = {cv.cam.cam2world.m0.w, cv.cam.cam2world.m1.w, cv.cam.cam2world.m2.w};
+  for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
+    const int li = opaque(lane);
+    int s[TILES], ray[TILES];
+    bool valid[TILES];
+    CornerPair cp[TILES];
+    uint2 sh[TILES];
+    float t0[TILES];
+#pragma unroll
+    for (int t = 0; t < TILES; t++) {
+      s[t] = (grp * TILES + t) * 16 + col;
+      valid[t] = s[t] < n32;
+      const int sl = valid[t] ? s[t] : n32 - 1;
+      ray[t] = c_ray[sl];
+      t0[t] = c_t0[sl];
+      sh[t] = ray_sh[ray[t] * 4 + g];
+      lookup_corners<false>(cv, c_frox[3 * sl] - 0.5f, c_frox[3 * sl + 1] - 0.5f, c_frox[3 * sl + 2] - 0.5f, ox, oy,
+                            fx, fy, fz, cp[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < TILES; t++) {
+      float wd, dd, n0, n1;
+      lookup_sums<false>(cp[t], wd, dd, n0, n1);
+      float l0 = n0 / wd;
+      if (di && g == 0) l0 = (logf(n0) * wd) / wd;
+      const _Float16 lat0 = (_Float16)l0, lat1 = (_Float16)(n1 / wd);
+      // the sample's midpoint (generateNetworkInput_preparation_perf_kernel, 285-291)
+      const RayState rs = rays[ray[t]];
+      const V3 o = {rs.ox, rs.oy, rs.oz}, d = {rs.dx, rs.dy, rs.dz};
+      const float tm = t0[t] + step_dt(st, t0[t]) * 0.5f;
+      const V3 wp = o + tm * d;
+      const V3 p = contract(wp, con);
+      const V3 i01 = unit_to_01(normalize(wp - cam_pos));
+      _Float16 shi[4], dsh[4], shd[4], fa[4], fb[4];
+      sh4_quarter(i01.x, i01.y, i01.z, g, shi);
+      shd[0] = __builtin_bit_cast(_Float16, (uint16_t)(sh[t].x & 0xffffu));
+      shd[1] = __builtin_bit_cast(_Float16, (uint16_t)(sh[t].x >> 16));
+      shd[2] = __builtin_bit_cast(_Float16, (uint16_t)(sh[t].y & 0xffffu));
+      shd[3] = __builtin_bit_cast(_Float16, (uint16_t)(sh[t].y >> 16));
+#pragma unroll
+      for (int e = 0; e < 4; e++) dsh[e] = (_Float16)((float)shd[e] - (float)shi[e]);
+      freq_quad(p, 4 + (g >> 1), freq_on != 0, fa);
+      freq_quad(p, g, freq_on != 0, fb);
+      const _Float16 la[4] = {lat0, lat1, fa[2 * (g & 1)], fa[2 * (g & 1) + 1]};
+      const f4 o4 = head_vd_mlp(wl + 6 * 64, wl, wl + 4 * 64, li, h8_of(la, shd), h8_of(dsh, fb));
+      if (g == 0 && valid[t]) {
+        const _Float16 logsig = di ? lat0 : (_Float16)logf(fmaxf_(dd, 1e-6f));
+        uint2 v;
+        v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o4[0]) |
+              ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o4[1]) << 16);
+        v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o4[2]) |
               ((uint32_t)__builtin_bit_cast(uint16_t, logsig) << 16);
         c_out[s[t]] = v;
       }
@@ -857,11 +996,21 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
 #define L(P)                        \
   if (r.mlp_rows == 32) { L2(P, 2); } \
   else { L2(P, 1); }
-      switch (m.net.precision) {
-        case FVC_PREC_BF16: L(FVC_PREC_BF16); break;
-        case FVC_PREC_FP8: L(FVC_PREC_FP8); break;
-        default: L(FVC_PREC_FP16); break;
+#define LVD(TL)                                                                                                   \
+  hipLaunchKernelGGL((k_cache_head_vd<TL>), dim3(head_blocks), dim3(256), 0, s, r.c_frox.as<float>(),                 \
+                     r.c_ray.as<int>(), r.c_t0.as<float>(), r.ray_sh.as<uint2>(), r.rays[cur].as<RayState>(),         \
+                     (int64_t)n_bound, cv, rm.stepsize_info, m.con, m.frags.as<h8>(), m.net.first_latent_is_density, \
+                     m.net.freq_encoding_degree > 0, r.c_out.as<uint2>(), ctl + CTL_C)
+      if (m.net.split_head) {
+        if (r.mlp_rows == 32) { LVD(2); } else { LVD(1); }
+      } else {
+        switch (m.net.precision) {
+          case FVC_PREC_BF16: L(FVC_PREC_BF16); break;
+          case FVC_PREC_FP8: L(FVC_PREC_FP8); break;
+          default: L(FVC_PREC_FP16); break;
+        }
       }
+#undef LVD
 #undef L
 #undef L2
 #undef L3
@@ -908,8 +1057,9 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
     // per bin: occupancy byte; per classified sample: index + density probe
     // (8 B) + 28 B stream record; ray state + entry per ray
     P.work(hs[i].march, bins, bins + 36.0 * (l.n_r + l.n_c) + 84.0 * l.alive, 0);
-    P.work(hs[i].net, l.n_r, l.n_r * (ALG_GATHER_B + 24.0 + 8.0), l.n_r * (ALG_DENSITY_FLOP + ALG_COLOUR_FLOP));
-    P.work(hs[i].head, l.n_c, l.n_c * (ALG_LOOKUP_B + 24.0 + 8.0), l.n_c * ALG_COLOUR_FLOP);
+    P.work(hs[i].net, l.n_r, l.n_r * (ALG_GATHER_B + 24.0 + 8.0), l.n_r * net_flop(m.net));
+    P.work(hs[i].head, l.n_c, l.n_c * (ALG_LOOKUP_B + (m.net.split_head ? 40.0 : 20.0) + 8.0),
+           l.n_c * (m.net.split_head ? ALG_VD_COLOUR_FLOP : ALG_COLOUR_FLOP));
     P.work(hs[i].comp, l.n_r + l.n_c, 12.0 * (l.n_r + l.n_c) + 132.0 * l.alive, 0);
     n_res += l.n_r;
     n_hit += l.n_c;

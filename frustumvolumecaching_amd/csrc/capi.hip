@@ -75,12 +75,26 @@ int fvc_net_param_counts(const fvc_net_desc* net, int64_t* a, int64_t* b, int64_
   });
 }
 
-int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const uint8_t* occ,
-                     const uint16_t* p_pos, int64_t n_pos, const uint16_t* p_base, int64_t n_base,
-                     const uint16_t* p_head, int64_t n_head, fvc_model** out) {
+int fvc_net_param_counts_viewdep(const fvc_net_desc* net, int64_t* a, int64_t* b, int64_t* f, int64_t* c) {
   return guard([&] {
+    REQUIRE(net && a && b && f && c, "null argument");
+    REQUIRE(net->split_head, "not a split model: use fvc_net_param_counts");
+    GridParams g;
+    if (!net_layout(*net, g, *a, *b, *c, f)) throw Error(FVC_E_UNSUPPORTED, "unsupported network description");
+  });
+}
+
+// NerfNetwork::init + set_params (nerf_network.h:88-118, 269-308) and the
+// occupancy bitfield; p_first (split models only) is mlp_first_head
+static void create_model(const fvc_scene_info* scene, const fvc_net_desc* net, const uint8_t* occ,
+                         const uint16_t* p_pos, int64_t n_pos, const uint16_t* p_base, int64_t n_base,
+                         const uint16_t* p_first, int64_t n_first, const uint16_t* p_head, int64_t n_head,
+                         fvc_model** out) {
+  {
     REQUIRE(scene && net && occ && p_pos && p_base && p_head && out, "null argument");
     *out = nullptr;
+    if (net->split_head && !p_first) throw Error(FVC_E_INVALID, "split models need mlp_first_head: use fvc_model_create_viewdep");
+    if (!net->split_head && p_first) throw Error(FVC_E_INVALID, "mlp_first_head given for a model without split head");
     // PerformanceRenderer only supports the fixed 128^3 grid (performance_renderer.cu:162-163)
     if (scene->grid_resolution != 128) throw Error(FVC_E_UNSUPPORTED, "only grid_resolution 128 is supported");
     REQUIRE(scene->grid_nlvl >= 1 && scene->grid_nlvl <= 16, "grid_nlvl out of range");
@@ -94,9 +108,9 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
       FVC_HIP(hipGetDevice(&m.device));
       m.scene = *scene;
       m.net = *net;
-      if (!net_layout(*net, m.grid, m.n_pos_enc, m.n_base, m.n_head))
+      if (!net_layout(*net, m.grid, m.n_pos_enc, m.n_base, m.n_head, &m.n_first))
         throw Error(FVC_E_UNSUPPORTED, "unsupported network description");
-      if (n_pos != m.n_pos_enc || n_base != m.n_base || n_head != m.n_head)
+      if (n_pos != m.n_pos_enc || n_base != m.n_base || n_head != m.n_head || (p_first && n_first != m.n_first))
         throw Error(FVC_E_INVALID, "parameter counts do not match the network description");
       // Morton bitfield (occupancy_grid_to_morton_bitfield, loader.cu:20-55, 120-144)
       const int64_t vals = 128 * 128 * 128;
@@ -118,7 +132,8 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
       m.table.ensure(n_pos * sizeof(uint16_t));
       FVC_HIP(hipMemcpy(m.table.p, p_pos, n_pos * sizeof(uint16_t), hipMemcpyHostToDevice));
       std::vector<uint8_t> frags;
-      pack_fragments(p_base, p_head, net->precision, frags, m.mlp_scales);
+      if (net->split_head) pack_fragments_vd(p_base, p_first, p_head, *net, frags);
+      else pack_fragments(p_base, p_head, net->precision, frags, m.mlp_scales);
       m.frags.ensure(frags.size());
       FVC_HIP(hipMemcpy(m.frags.p, frags.data(), frags.size(), hipMemcpyHostToDevice));
       m.occ = occ_params(*scene);
@@ -139,6 +154,22 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const
       throw;
     }
     *out = h;
+  }
+}
+
+int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net, const uint8_t* occ,
+                     const uint16_t* p_pos, int64_t n_pos, const uint16_t* p_base, int64_t n_base,
+                     const uint16_t* p_head, int64_t n_head, fvc_model** out) {
+  return guard([&] { create_model(scene, net, occ, p_pos, n_pos, p_base, n_base, nullptr, 0, p_head, n_head, out); });
+}
+
+int fvc_model_create_viewdep(const fvc_scene_info* scene, const fvc_net_desc* net, const uint8_t* occ,
+                             const uint16_t* p_pos, int64_t n_pos, const uint16_t* p_base, int64_t n_base,
+                             const uint16_t* p_first, int64_t n_first, const uint16_t* p_head, int64_t n_head,
+                             fvc_model** out) {
+  return guard([&] {
+    REQUIRE(p_first, "null argument");
+    create_model(scene, net, occ, p_pos, n_pos, p_base, n_base, p_first, n_first, p_head, n_head, out);
   });
 }
 
@@ -165,7 +196,19 @@ int fvc_nerfacc_read_data(const char* dir, uint16_t* pos, uint16_t* base, uint16
     REQUIRE(dir && pos && base && head && occ, "null argument");
     NerfaccExport ex;
     nerfacc_read_config(dir, ex);
+    if (ex.net.split_head) throw Error(FVC_E_INVALID, "split model: use fvc_nerfacc_read_data_viewdep");
     nerfacc_read_data(dir, ex, pos, base, head, occ);
+  });
+}
+
+int fvc_nerfacc_read_data_viewdep(const char* dir, uint16_t* pos, uint16_t* base, uint16_t* first, uint16_t* head,
+                                  uint8_t* occ) {
+  return guard([&] {
+    REQUIRE(dir && pos && base && first && head && occ, "null argument");
+    NerfaccExport ex;
+    nerfacc_read_config(dir, ex);
+    if (!ex.net.split_head) throw Error(FVC_E_INVALID, "config.json has no mlp_first_head: use fvc_nerfacc_read_data");
+    nerfacc_read_data(dir, ex, pos, base, head, occ, first);
   });
 }
 
@@ -177,10 +220,14 @@ int fvc_model_load_nerfacc(const char* dir, fvc_scene_info* scene, fvc_stepsize_
     NerfaccExport ex;
     nerfacc_read_config(dir, ex);
     std::vector<uint16_t> pos((size_t)ex.n_pos), base((size_t)ex.n_base), head((size_t)ex.n_head);
+    std::vector<uint16_t> first((size_t)std::max<int64_t>(1, ex.n_first));
     std::vector<uint8_t> occ((size_t)ex.n_occ);
-    nerfacc_read_data(dir, ex, pos.data(), base.data(), head.data(), occ.data());
-    const int rc = fvc_model_create(&ex.scene, &ex.net, occ.data(), pos.data(), ex.n_pos, base.data(), ex.n_base,
-                                    head.data(), ex.n_head, out);
+    nerfacc_read_data(dir, ex, pos.data(), base.data(), head.data(), occ.data(), first.data());
+    const int rc = ex.net.split_head
+                       ? fvc_model_create_viewdep(&ex.scene, &ex.net, occ.data(), pos.data(), ex.n_pos, base.data(),
+                                                  ex.n_base, first.data(), ex.n_first, head.data(), ex.n_head, out)
+                       : fvc_model_create(&ex.scene, &ex.net, occ.data(), pos.data(), ex.n_pos, base.data(),
+                                          ex.n_base, head.data(), ex.n_head, out);
     if (rc != FVC_OK) throw Error(rc, fvc_last_error_message());
     if (scene) *scene = ex.scene;
     if (stepsize) *stepsize = ex.step;
@@ -214,6 +261,22 @@ int fvc_network_head(fvc_model* h, const uint16_t* hin, const float* dir, int64_
     launch_network_head(h->m, hin, dir, n, (uint32_t*)out, (hipStream_t)stream);
   });
 }
+int fvc_network_latents_viewdep(fvc_model* h, const float* pos, const float* init_dir, int64_t n, uint16_t* out,
+                                void* stream) {
+  return guard([&] {
+    REQUIRE(h && pos && init_dir && out && n >= 0, "invalid argument");
+    if (!h->m.net.split_head) throw Error(FVC_E_UNSUPPORTED, "not a split model: use fvc_network_latents");
+    launch_network_latents_vd(h->m, pos, init_dir, n, out, (hipStream_t)stream, 16);
+  });
+}
+int fvc_network_head_viewdep(fvc_model* h, const uint16_t* hin, const float* pos, const float* dir,
+                             const float* init_dir, int64_t n, uint16_t* out, void* stream) {
+  return guard([&] {
+    REQUIRE(h && hin && pos && dir && init_dir && out && n >= 0, "invalid argument");
+    if (!h->m.net.split_head) throw Error(FVC_E_UNSUPPORTED, "not a split model: use fvc_network_head");
+    launch_network_head_vd(h->m, hin, pos, dir, init_dir, n, (uint32_t*)out, (hipStream_t)stream);
+  });
+}
 int fvc_network_encode(fvc_model* h, const float* pos, int64_t n, uint16_t* out, void* stream) {
   return guard([&] {
     REQUIRE(h && pos && out && n >= 0, "invalid argument");
@@ -227,6 +290,9 @@ int fvc_renderer_create(fvc_model* model, int double_buffered, fvc_renderer** ou
     *out = nullptr;
     auto* h = new fvc_renderer();
     h->r.model = &model->m;
+    // the interpolation variant eval.cu:216 picks for the model
+    if (model->m.net.split_head && model->m.net.first_latent_is_density)
+      h->r.settings.interpol_variant = FVC_INTERPOL_DENSITY_INTERMEDIATES;
     h->r.double_buffered = double_buffered != 0;
     h->r.fill_idx = h->r.double_buffered ? 1 : 0;
     // default priority on purpose: non-default stream priorities are avoided
@@ -351,6 +417,8 @@ int fvc_diff_forward(fvc_renderer* h, const fvc_raymarch_info* rm, const int32_t
                      fvc_diff_stats* stats) {
   return guard([&] {
     REQUIRE(h, "null renderer");
+    if (h->r.model->net.split_head)
+      throw Error(FVC_E_UNSUPPORTED, "the differentiable render covers models without a split head");
     check_rm(rm);
     REQUIRE(n_rays >= 0, "negative ray count");
     REQUIRE(n_rays == 0 || (pix && pos_enc && mlp_base && mlp_head && rgb), "null argument");
@@ -374,6 +442,8 @@ int fvc_diff_cached_forward(fvc_renderer* h, const fvc_raymarch_info* rm, const 
                             fvc_diff_stats* stats) {
   return guard([&] {
     REQUIRE(h, "null renderer");
+    if (h->r.model->net.split_head)
+      throw Error(FVC_E_UNSUPPORTED, "the differentiable render covers models without a split head");
     check_rm(rm);
     REQUIRE(n_rays >= 0, "negative ray count");
     REQUIRE(n_rays == 0 || (pix && pos_enc && mlp_base && mlp_head && rgb), "null argument");
@@ -447,9 +517,15 @@ int fvc_render_cached_auto(fvc_renderer* h, const fvc_raymarch_info* rm, uint8_t
 int fvc_cache_set_settings(fvc_renderer* h, const fvc_cache_settings* s) {
   return guard([&] {
     REQUIRE(h && s, "null argument");
-    if (s->interpol_variant != FVC_INTERPOL_DENSITY || s->interpol_function != FVC_FN_LINEAR ||
-        !s->use_inter_brick_interpolation)
-      throw Error(FVC_E_UNSUPPORTED, "only the default cache settings (Density, Linear, inter-brick) are supported");
+    const fvc_net_desc& net = h->r.model->net;
+    // the variant must match what the model's latents hold (eval.cu:216)
+    const int variant = net.split_head && net.first_latent_is_density ? FVC_INTERPOL_DENSITY_INTERMEDIATES
+                                                                      : FVC_INTERPOL_DENSITY;
+    if (s->interpol_variant != variant || s->interpol_function != FVC_FN_LINEAR || !s->use_inter_brick_interpolation)
+      throw Error(FVC_E_UNSUPPORTED,
+                  variant == FVC_INTERPOL_DENSITY
+                      ? "only the default cache settings (Density, Linear, inter-brick) are supported"
+                      : "this model needs DensityIntermediates, Linear, inter-brick cache settings");
     h->r.settings = *s;
   });
 }
@@ -460,6 +536,8 @@ int fvc_cache_set_latent_format(fvc_renderer* h, int format) {
   return guard([&] {
     REQUIRE(h, "null renderer");
     REQUIRE(format == FVC_LATENT_FP16 || format == FVC_LATENT_FP8, "unknown latent format");
+    if (format == FVC_LATENT_FP8 && h->r.model->net.split_head)
+      throw Error(FVC_E_UNSUPPORTED, "FP8 cache latents are not built for split models");
     std::scoped_lock lk(h->r.write_mutex);
     h->r.latent_format = format;
   });

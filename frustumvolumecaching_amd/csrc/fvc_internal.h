@@ -82,6 +82,16 @@ static const int N_FRAGS = 24;
 // N_NET_FRAGS fragments
 static const int FRAG_W1 = 0, FRAG_W2 = 4, FRAG_W3 = 12, FRAG_H1 = 14, FRAG_H2 = 18, FRAG_H1C = 20;
 static const int N_NET_FRAGS = 20;
+// Split (view-dependent) models, FP16 only: the density MLP as above, then
+// mlp_first_head VD_F1[nb] (32 inputs), VD_F2[nb][kb] (second hidden layer,
+// if any), VD_F3[kb] (output rows = the 8 head latents in order); the
+// custom colour head's first layer over two 32-input blocks, VD_HA[nb]
+// (latents / position frequencies + SH of the view direction) and
+// VD_HB[nb] (SH differences + frequencies), its output layer VD_HO[kb], and
+// VD_HAC[nb], block A in the input order of the cache lookup.  The network
+// kernels stage the first N_VD_NET_FRAGS, the cache head VD_HB..VD_HAC+3.
+static const int VD_F1 = 14, VD_F2 = 18, VD_F3 = 26, VD_HA = 28, VD_HB = 32, VD_HO = 36, VD_HAC = 38;
+static const int N_VD_FRAGS = 42, N_VD_NET_FRAGS = 38;
 
 // per-layer accumulator scales of the MLPs (FP8: s_w[l] * s_a; otherwise 1)
 // and 1 / s_a (mlp_mfma.h)
@@ -97,7 +107,7 @@ struct Model {
   GridParams grid{};
   OccParams occ{};
   ContractParams con{};
-  int64_t n_pos_enc = 0, n_base = 0, n_head = 0;
+  int64_t n_pos_enc = 0, n_base = 0, n_head = 0, n_first = 0;
   DevBuf bitfield;   // Morton occupancy bitfield
   DevBuf coarse;     // per cascade 16^3 "any of 8^3 cells occupied" bits
   DevBuf table;      // hash grid, uint32 per entry (2 x fp16)
@@ -106,8 +116,13 @@ struct Model {
   std::vector<uint8_t> bitfield_host;
 };
 
-// layout of the parameter blocks (nerfacc export) for a description
-bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n_base, int64_t& n_head);
+// input width of a split model's custom colour head: 8 latents + 16 SH +
+// 16 SH differences + 6 * frequency degree, padded to 16
+inline int vd_head_inputs(const fvc_net_desc& d) { return (8 + 16 + 16 + 6 * d.freq_encoding_degree + 15) / 16 * 16; }
+// layout of the parameter blocks (nerfacc export) for a description; n_first
+// (split models: mlp_first_head) is 0 for plain models
+bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n_base, int64_t& n_head,
+                int64_t* n_first = nullptr);
 
 // model export of the reference (loader.hip)
 struct NerfaccExport {
@@ -115,11 +130,12 @@ struct NerfaccExport {
   fvc_stepsize_info step{};
   fvc_net_desc net{};
   bool pos_separate = true;
-  int64_t n_pos = 0, n_base = 0, n_head = 0, n_occ = 0;
+  int64_t n_pos = 0, n_base = 0, n_head = 0, n_occ = 0, n_first = 0;
 };
 void nerfacc_read_config(const std::string& dir, NerfaccExport& ex);
+// first: mlp_first_head buffer (split models; else unused)
 void nerfacc_read_data(const std::string& dir, const NerfaccExport& ex, uint16_t* pos, uint16_t* base, uint16_t* head,
-                       uint8_t* occ);
+                       uint8_t* occ, uint16_t* first = nullptr);
 
 // network launches (network.hip)
 // n_dev (optional): the sample count in device memory, n its upper bound
@@ -130,6 +146,18 @@ void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_
 void launch_network_head(const Model& m, const uint16_t* head_in16, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s);
 void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t* out32, hipStream_t s);
+// split models (viewdep.hip): full inference (the initial view direction is
+// dir), latents with the initial view direction init_dir ([8 head latents,
+// log sigma] per sample), the custom head alone
+void launch_network_full_vd(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
+                            hipStream_t s, const int* n_dev, int rows);
+void launch_network_latents_vd(const Model& m, const float* pos, const float* init_dir, int64_t n, uint16_t* out16,
+                               hipStream_t s, int rows);
+void launch_network_head_vd(const Model& m, const uint16_t* head_in16, const float* pos, const float* dir,
+                            const float* init_dir, int64_t n, uint32_t* out_h4, hipStream_t s);
+// weight fragments of a split model (FP16)
+void pack_fragments_vd(const uint16_t* base, const uint16_t* first, const uint16_t* head, const fvc_net_desc& d,
+                       std::vector<uint8_t>& out);
 
 // ---------------------------------------------------------------- cache
 static const int BRICK = 8;               // Cache::BRICK_SIZE (cache.h:17)
@@ -248,6 +276,14 @@ struct Profiler {
 //                           + 16 B latents) = 224 B
 static const double ALG_GATHER_B = 512, ALG_LOOKUP_B = 224;
 static const double ALG_DENSITY_FLOP = 13440, ALG_COLOUR_FLOP = 3456;
+// split models: first head 2 (32*64 + 64*8) (+ 2 * 64 * 64 per extra hidden
+// layer), custom colour head 2 (64*64 + 64*3)
+static const double ALG_FIRST_HEAD_FLOP = 5120, ALG_VD_COLOUR_FLOP = 8576;
+// algorithmic FLOP of one full network evaluation
+inline double net_flop(const fvc_net_desc& d) {
+  if (!d.split_head) return ALG_DENSITY_FLOP + ALG_COLOUR_FLOP;
+  return ALG_DENSITY_FLOP + ALG_FIRST_HEAD_FLOP + 8192.0 * (d.first_head_n_hidden_layers - 1) + ALG_VD_COLOUR_FLOP;
+}
 
 // ---------------------------------------------------------------- renderer
 static const int MAX_STEPS_PER_LAUNCH = 32;   // cap of fvc_renderer_set_batching
@@ -332,6 +368,7 @@ struct Renderer {
   // fill scratch (own counters and pinned mirror: the fill may run on the
   // fill stream from another thread)
   DevBuf f_rays, f_tiles, f_list, f_pos, f_t0, f_meta, f_out16, f_counters;
+  DevBuf f_dir;  // fill sample directions (split models: the initial view direction)
   int* f_pinned = nullptr;
   // background refill (double-buffered renderers)
   std::thread fill_thread;

@@ -10,6 +10,8 @@
 //                             table follows the weights in mlp_base.dat)
 //   <dir>/mlp_base.dat        fp16 density / latent MLP weights
 //   <dir>/mlp_head.dat        fp16 colour MLP weights
+//   <dir>/mlp_first_head.dat  fp16 first-head weights (split models: config
+//                             key "mlp_first_head", loader.cu:251-262)
 //   <dir>/occupancy_grid.dat  uint8 [grid_nlvl][res][res][res], > 0 occupied
 //
 // Every .dat file is raw little-endian data without a header (buffer.h:105-
@@ -67,8 +69,6 @@ void nerfacc_read_config(const std::string& dir, NerfaccExport& ex) {
     throw Error(FVC_E_INVALID, e.what());
   }
   try {
-    if (cfg.has("mlp_first_head"))
-      throw Error(FVC_E_UNSUPPORTED, "view-dependent split head models (mlp_first_head) are not supported");
     // parse(SceneInfo&) (parse_helper.h:25-54)
     const json::Value& sc = cfg.at("scene");
     fvc_scene_info& s = ex.scene;
@@ -120,18 +120,35 @@ void nerfacc_read_config(const std::string& dir, NerfaccExport& ex) {
     n.latent_offset = 1;
     n.sh_degree = sh_degree_of(head.at("encoding"));
     n.precision = FVC_PREC_FP16;
+    if (cfg.has("mlp_first_head")) {
+      // NerfNetwork::init_viewdep (nerf_network.h:120-158)
+      const json::Value& fh = cfg.at("mlp_first_head");
+      n.split_head = 1;
+      n.first_head_width = (int)fh.at("network").at("n_neurons").number();
+      n.first_head_n_hidden_layers = (int)fh.at("network").at("n_hidden_layers").number();
+      n.first_head_n_input = (int)fh.at("n_input").number();
+      n.first_latent_is_density = fh.has("first_latent_is_density") ? (fh.at("first_latent_is_density").boolean() ? 1 : 0) : 1;
+      n.first_sh_degree = sh_degree_of(fh.at("encoding"));
+      n.n_latents = (int)fh.at("n_output").number();
+      n.latent_offset = n.first_latent_is_density ? 0 : 1;
+      n.freq_encoding_degree = (int)head.number_or("freq_encoding_degree", 0);
+      // the custom head's input width (my_fully_fused_mlp.cu:1300-1308)
+      if (head.has("n_input") && (int)head.at("n_input").number() != vd_head_inputs(n))
+        throw Error(FVC_E_UNSUPPORTED, "mlp_head n_input does not match the split head's inputs");
+    }
   } catch (const Error&) {
     throw;
   } catch (const std::exception& e) {
     throw Error(FVC_E_INVALID, e.what());
   }
   GridParams g;
-  int64_t n_pos = 0, n_base = 0, n_head = 0;
-  if (!net_layout(ex.net, g, n_pos, n_base, n_head))
+  int64_t n_pos = 0, n_base = 0, n_head = 0, n_first = 0;
+  if (!net_layout(ex.net, g, n_pos, n_base, n_head, &n_first))
     throw Error(FVC_E_UNSUPPORTED, "network description in config.json is not supported");
   ex.n_pos = n_pos;
   ex.n_base = n_base;
   ex.n_head = n_head;
+  ex.n_first = n_first;
   ex.n_occ = (int64_t)ex.scene.vals_per_lvl * ex.scene.grid_nlvl;
   const int64_t b_pos = file_bytes(join(dir, "pos_enc.dat"));
   const int64_t b_base = file_bytes(join(dir, "mlp_base.dat"));
@@ -142,10 +159,16 @@ void nerfacc_read_config(const std::string& dir, NerfaccExport& ex) {
   const int64_t want_base = 2 * (ex.pos_separate ? n_base : n_base + n_pos);
   if ((ex.pos_separate && b_pos != 2 * n_pos) || b_base != want_base || b_head != 2 * n_head || b_occ != ex.n_occ)
     throw Error(FVC_E_INVALID, "parameter file sizes do not match config.json");
+  if (ex.net.split_head && file_bytes(join(dir, "mlp_first_head.dat")) != 2 * n_first)
+    throw Error(FVC_E_INVALID, "mlp_first_head.dat missing or its size does not match config.json");
 }
 
 void nerfacc_read_data(const std::string& dir, const NerfaccExport& ex, uint16_t* pos, uint16_t* base, uint16_t* head,
-                       uint8_t* occ) {
+                       uint8_t* occ, uint16_t* first) {
+  if (ex.net.split_head) {
+    if (!first) throw Error(FVC_E_INVALID, "split model: mlp_first_head.dat needs a buffer");
+    read_exact(join(dir, "mlp_first_head.dat"), first, 2 * ex.n_first);
+  }
   if (ex.pos_separate) {
     read_exact(join(dir, "pos_enc.dat"), pos, 2 * ex.n_pos);
     read_exact(join(dir, "mlp_base.dat"), base, 2 * ex.n_base);

@@ -235,4 +235,56 @@ __device__ __forceinline__ typename Mx<P>::T head_input_cache(_Float16 l0, _Floa
   }
 }
 
+// ------------------------------------------------------------ split models
+// (FP16 operands, FP32 accumulation; fragment indices VD_* in fvc_internal.h)
+typedef Mx<FVC_PREC_FP16> MxH;
+
+// mlp_first_head 32 -> 64 (FL hidden layers) -> 16 with the fragments at w
+// (the staged fragment table); returns output rows 4g..4g+3, rows 0..7 being
+// the colour head's latents 0..7
+template <int FL> __device__ __forceinline__ f4 first_head_mlp(const h8* w, int li, h8 x) {
+  f4 d1[4];
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) d1[nb] = MxH::mma(w[(VD_F1 + nb) * 64 + li], x, f4zero());
+  h8 b0 = act_pack<FVC_PREC_FP16>(d1[0], d1[1], 1.0f), b1 = act_pack<FVC_PREC_FP16>(d1[2], d1[3], 1.0f);
+  if constexpr (FL == 2) {
+    f4 d2[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++)
+      d2[nb] = MxH::mma(w[(VD_F2 + nb * 2 + 1) * 64 + li], b1, MxH::mma(w[(VD_F2 + nb * 2) * 64 + li], b0, f4zero()));
+    b0 = act_pack<FVC_PREC_FP16>(d2[0], d2[1], 1.0f);
+    b1 = act_pack<FVC_PREC_FP16>(d2[2], d2[3], 1.0f);
+  }
+  return MxH::mma(w[(VD_F3 + 1) * 64 + li], b1, MxH::mma(w[VD_F3 * 64 + li], b0, f4zero()));
+}
+
+// custom colour head (my_fully_fused_mlp.cu:677-895) on the two input
+// blocks a, b: first-layer fragments wa, wb (4 each), output fragments wo
+// (2); returns rows 4g..4g+3 (rgb in group 0)
+__device__ __forceinline__ f4 head_vd_mlp(const h8* wa, const h8* wb, const h8* wo, int li, h8 a, h8 b) {
+  f4 d[4];
+#pragma unroll
+  for (int nb = 0; nb < 4; nb++) d[nb] = MxH::mma(wa[nb * 64 + li], a, MxH::mma(wb[nb * 64 + li], b, f4zero()));
+  const h8 b0 = act_pack<FVC_PREC_FP16>(d[0], d[1], 1.0f), b1 = act_pack<FVC_PREC_FP16>(d[2], d[3], 1.0f);
+  return MxH::mma(wo[64 + li], b1, MxH::mma(wo[li], b0, f4zero()));
+}
+
+// Frequency encoding quad q (= 2 axis + cos) of the position p
+// (freqEncChunk4, my_fully_fused_mlp.cu:63-110): sin or cos of pi 2^f p_axis
+// for f = 0..3, as halves.  v_sin / v_cos take revolutions, so the argument
+// p 2^(f-1) is exact.  on = false (no frequency inputs): the padding 1.0.
+__device__ __forceinline__ void freq_quad(V3 p, int q, bool on, _Float16* o) {
+  const float x = q < 2 ? p.x : (q < 4 ? p.y : p.z);
+#pragma unroll
+  for (int f = 0; f < 4; f++) {
+    const float r = x * (0.5f * (float)(1 << f));
+    const float v = (q & 1) ? __builtin_amdgcn_cosf(r) : __builtin_amdgcn_sinf(r);
+    o[f] = on ? (_Float16)v : (_Float16)1.0f;
+  }
+}
+
+__device__ __forceinline__ h8 h8_of(const _Float16* lo, const _Float16* hi) {
+  return h8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
 }  // namespace fvc

@@ -23,12 +23,24 @@
 namespace fvc {
 
 // ------------------------------------------------------------------ layout
-bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n_base, int64_t& n_head) {
+// Split models (init_viewdep, nerf_network.h:120-158) additionally need:
+// mlp_first_head on 32 inputs (16 density-MLP outputs + SH4), 64 wide, 1 or
+// 2 hidden layers, 8 latents; the custom head's supported instance
+// (my_fully_fused_mlp.cu:1363-1421: width 64, SH degrees 4 and 4, frequency
+// degree 4 or 0); latent_offset 0 exactly when the first latent is the
+// density; FP16 arithmetic (the custom head is half only, 1271-1281).
+bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n_base, int64_t& n_head,
+                int64_t* n_first) {
   if (d.n_levels != 16 || d.n_features_per_level != 2 || d.log2_hashmap_size < 10 || d.log2_hashmap_size > 24)
     return false;
   if (d.base_width != 64 || d.base_n_hidden_layers != 2 || d.base_n_output < 9 || d.base_n_output > 16) return false;
-  if (d.head_width != 64 || d.head_n_hidden_layers != 1 || d.n_latents != 8 || d.latent_offset != 1 ||
-      d.sh_degree != 4)
+  if (d.head_width != 64 || d.head_n_hidden_layers != 1 || d.n_latents != 8 || d.sh_degree != 4) return false;
+  if (!d.split_head && d.latent_offset != 1) return false;
+  if (d.split_head &&
+      (d.first_head_width != 64 || d.first_head_n_hidden_layers < 1 || d.first_head_n_hidden_layers > 2 ||
+       d.first_head_n_input != 32 || d.first_sh_degree != 4 ||
+       (d.freq_encoding_degree != 0 && d.freq_encoding_degree != 4) ||
+       d.latent_offset != (d.first_latent_is_density ? 0 : 1) || d.precision != FVC_PREC_FP16))
     return false;
   std::memset(&g, 0, sizeof(g));
   g.n_levels = d.n_levels;
@@ -57,6 +69,11 @@ bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n
   n_pos = (int64_t)off * 2;
   n_base = 32 * 64 + 64 * 64 + 64 * 16;
   n_head = 32 * 64 + 64 * 16;
+  if (n_first) *n_first = 0;
+  if (d.split_head) {
+    n_head = (int64_t)vd_head_inputs(d) * 64 + 64 * 16;
+    if (n_first) *n_first = 32 * 64 + (d.first_head_n_hidden_layers - 1) * 64 * 64 + 64 * 16;
+  }
   return true;
 }
 
@@ -231,6 +248,117 @@ __global__ __launch_bounds__(NET_BLOCK) void k_head(const uint16_t* __restrict__
   }
 }
 
+// Split models (NerfNetwork::inference / inferenceLatents after
+// init_viewdep, nerf_network.h:160-202): hash grid and density MLP as in
+// k_network, then mlp_first_head on [the 16 density-MLP outputs, SH4 of the
+// initial view direction] and, for FULL, the custom colour head
+// (my_fully_fused_mlp.cu:677-895).  FULL: the initial view direction is the
+// view direction (performance_renderer.cu:269), so the SH differences are
+// exactly 0; output half4 (r, g, b, log sigma).  !FULL: dir is the initial
+// view direction; output [8 head latents, log sigma, 0 x 7] per sample.
+// di: the first latent is log sigma (the first head's latent row 0 is then
+// a zero-weight row replaced here).  FL: first-head hidden layers.
+template <bool FULL, int TILES, int FL>
+__global__ __launch_bounds__(NET_BLOCK) void k_network_vd(const float* __restrict__ pos, const float* __restrict__ dir,
+                                                          int64_t n, const uint32_t* __restrict__ table,
+                                                          const h8* __restrict__ frags, GridParams gp, int di,
+                                                          int freq_on, uint2* __restrict__ out,
+                                                          const int* __restrict__ n_dev) {
+  if (n_dev) n = *n_dev;
+  if ((int64_t)blockIdx.x * (NET_BLOCK / 64) * 16 * TILES >= n) return;
+  __shared__ GridLevel lv[MAX_LEVELS];
+  __shared__ h8 wl[N_VD_NET_FRAGS * 64];
+  load_levels(gp, lv);
+  stage_frags(frags, wl, FULL ? N_VD_NET_FRAGS : VD_HA);
+  __syncthreads();
+  const MlpScales sc{{1.0f, 1.0f, 1.0f, 1.0f, 1.0f}, 1.0f};
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  const int64_t n_groups = (n + 16 * TILES - 1) / (16 * TILES);
+  for (int64_t grp = (int64_t)blockIdx.x * (NET_BLOCK / 64) + wave; grp < n_groups;
+       grp += (int64_t)gridDim.x * (NET_BLOCK / 64)) {
+    const int li = opaque(lane);
+    int64_t s[TILES], sl[TILES];
+    bool valid[TILES];
+    float f[TILES][8];
+#pragma unroll
+    for (int t = 0; t < TILES; t++) {
+      s[t] = (grp * TILES + t) * 16 + col;
+      valid[t] = s[t] < n;
+      sl[t] = valid[t] ? s[t] : n - 1;
+      const V3 p = {pos[3 * sl[t]], pos[3 * sl[t] + 1], pos[3 * sl[t] + 2]};
+      encode_quads(lv, gp, table, p, g, f[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < TILES; t++) {
+#pragma unroll
+      for (int e = 0; e < 8; e++) f[t][e] = (float)(_Float16)f[t][e];
+      const f4 d3 = base_mlp<FVC_PREC_FP16>(wl, li, MxH::pack(f[t], 1.0f), sc);
+      const _Float16 bh[4] = {(_Float16)d3[0], (_Float16)d3[1], (_Float16)d3[2], (_Float16)d3[3]};
+      const int64_t q = sl[t];
+      _Float16 sh[4];
+      sh4_quarter(dir[3 * q], dir[3 * q + 1], dir[3 * q + 2], g, sh);
+      const f4 fo = first_head_mlp<FL>(wl, li, h8_of(bh, sh));
+      _Float16 lat[4] = {(_Float16)fo[0], (_Float16)fo[1], (_Float16)fo[2], (_Float16)fo[3]};
+      if (di && g == 0) lat[0] = bh[0];
+      if constexpr (!FULL) {
+        // log sigma (density-MLP row 0, group 0) into group 2
+        const int ls = __shfl_xor((int)__builtin_bit_cast(uint16_t, bh[0]), 32);
+        const _Float16 z = (_Float16)0.0f;
+        uint2 v = pack_h4(lat[0], lat[1], lat[2], lat[3]);
+        if (g == 2) v = pack_h4(__builtin_bit_cast(_Float16, (uint16_t)ls), z, z, z);
+        if (g == 3) v = make_uint2(0u, 0u);
+        if (valid[t]) out[s[t] * 4 + g] = v;
+      } else {
+        const V3 p = {pos[3 * q], pos[3 * q + 1], pos[3 * q + 2]};
+        _Float16 fa[4], fb[4];
+        freq_quad(p, g + 2, freq_on != 0, fa);
+        freq_quad(p, g, freq_on != 0, fb);
+        const _Float16 zero4[4] = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
+        const h8 a = h8_of(g < 2 ? lat : fa, sh);
+        const h8 b = h8_of(zero4, fb);
+        const f4 o = head_vd_mlp(wl + VD_HA * 64, wl + VD_HB * 64, wl + VD_HO * 64, li, a, b);
+        if (g == 0 && valid[t]) out[s[t]] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2], bh[0]);
+      }
+    }
+  }
+}
+
+// the custom colour head alone on given inputs (16 halves per sample in the
+// layout of k_network_vd<false>), with the initial view direction init
+__global__ __launch_bounds__(NET_BLOCK) void k_head_vd(const uint16_t* __restrict__ hin16, const float* __restrict__ pos,
+                                                       const float* __restrict__ dir, const float* __restrict__ init,
+                                                       int64_t n, const h8* __restrict__ frags, int di, int freq_on,
+                                                       uint2* __restrict__ out) {
+  __shared__ h8 wl[10 * 64];  // VD_HA, VD_HB, VD_HO
+  stage_frags(frags + VD_HA * 64, wl, 10);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, g = lane >> 4;
+  const int64_t n_groups = (n + 15) / 16;
+  for (int64_t grp = (int64_t)blockIdx.x * (NET_BLOCK / 64) + wave; grp < n_groups;
+       grp += (int64_t)gridDim.x * (NET_BLOCK / 64)) {
+    const int li = opaque(lane);
+    const int64_t s = grp * 16 + col;
+    const bool valid = s < n;
+    const int64_t q = valid ? s : n - 1;
+    _Float16 lat[4], sh[4], shi[4], dsh[4], fa[4], fb[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) lat[e] = __builtin_bit_cast(_Float16, hin16[q * 16 + (4 * g + e) % 8]);
+    sh4_quarter(dir[3 * q], dir[3 * q + 1], dir[3 * q + 2], g, sh);
+    sh4_quarter(init[3 * q], init[3 * q + 1], init[3 * q + 2], g, shi);
+#pragma unroll
+    for (int e = 0; e < 4; e++) dsh[e] = (_Float16)((float)sh[e] - (float)shi[e]);
+    const V3 p = {pos[3 * q], pos[3 * q + 1], pos[3 * q + 2]};
+    freq_quad(p, g + 2, freq_on != 0, fa);
+    freq_quad(p, g, freq_on != 0, fb);
+    const f4 o = head_vd_mlp(wl, wl + 4 * 64, wl + 8 * 64, li, h8_of(g < 2 ? lat : fa, sh), h8_of(dsh, fb));
+    if (g == 0 && valid)
+      out[s] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2],
+                       __builtin_bit_cast(_Float16, hin16[q * 16 + (di ? 0 : 8)]));
+  }
+}
+
 // encoding alone: 32 halves per sample (thread per sample)
 __global__ void k_encode(const float* __restrict__ pos, int64_t n, const uint32_t* __restrict__ table, GridParams gp,
                          uint16_t* __restrict__ out32) {
@@ -268,6 +396,10 @@ static int grid_for(int64_t n_groups) {
 void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s, const int* n_dev, int rows) {
   if (n <= 0) return;
+  if (m.net.split_head) {
+    launch_network_full_vd(m, pos, dir, n, out_h4, s, n_dev, rows);
+    return;
+  }
 #define L2(P, TL)                                                                                                \
   hipLaunchKernelGGL((k_network<true, P, TL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), 0, s, \
                      pos, dir, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, m.mlp_scales,   \
@@ -281,6 +413,7 @@ void launch_network_full(const Model& m, const float* pos, const float* dir, int
   FVC_HIP(hipGetLastError());
 }
 void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s, int rows) {
+  if (m.net.split_head) throw Error(FVC_E_UNSUPPORTED, "split models need the initial view direction");
   if (n <= 0) return;
 #define L2(P, TL)                                                                                                 \
   hipLaunchKernelGGL((k_network<false, P, TL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), 0, s, \
@@ -296,6 +429,7 @@ void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_
 }
 void launch_network_head(const Model& m, const uint16_t* hin16, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s) {
+  if (m.net.split_head) throw Error(FVC_E_UNSUPPORTED, "split models need the position and initial view direction");
   if (n <= 0) return;
 #define L(P)                                                                                                    \
   hipLaunchKernelGGL((k_head<P>), dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, hin16, dir, n,            \
@@ -304,6 +438,47 @@ void launch_network_head(const Model& m, const uint16_t* hin16, const float* dir
 #undef L
   FVC_HIP(hipGetLastError());
 }
+void launch_network_full_vd(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
+                            hipStream_t s, const int* n_dev, int rows) {
+  if (n <= 0) return;
+  const int di = m.net.first_latent_is_density, fq = m.net.freq_encoding_degree > 0;
+#define L(TL, FL)                                                                                              \
+  hipLaunchKernelGGL((k_network_vd<true, TL, FL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), \
+                     0, s, pos, dir, n, m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, di, fq, (uint2*)out_h4,  \
+                     n_dev)
+  if (m.net.first_head_n_hidden_layers == 2) {
+    if (rows == 32) { L(2, 2); } else { L(1, 2); }
+  } else {
+    if (rows == 32) { L(2, 1); } else { L(1, 1); }
+  }
+#undef L
+  FVC_HIP(hipGetLastError());
+}
+void launch_network_latents_vd(const Model& m, const float* pos, const float* init_dir, int64_t n, uint16_t* out16,
+                               hipStream_t s, int rows) {
+  if (n <= 0) return;
+  const int di = m.net.first_latent_is_density, fq = m.net.freq_encoding_degree > 0;
+#define L(TL, FL)                                                                                               \
+  hipLaunchKernelGGL((k_network_vd<false, TL, FL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), \
+                     0, s, pos, init_dir, n, m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, di, fq,            \
+                     (uint2*)out16, (const int*)nullptr)
+  if (m.net.first_head_n_hidden_layers == 2) {
+    if (rows == 32) { L(2, 2); } else { L(1, 2); }
+  } else {
+    if (rows == 32) { L(2, 1); } else { L(1, 1); }
+  }
+#undef L
+  FVC_HIP(hipGetLastError());
+}
+void launch_network_head_vd(const Model& m, const uint16_t* head_in16, const float* pos, const float* dir,
+                            const float* init_dir, int64_t n, uint32_t* out_h4, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_head_vd, dim3(grid_for((n + 15) / 16)), dim3(NET_BLOCK), 0, s, head_in16, pos, dir, init_dir, n,
+                     m.frags.as<h8>(), m.net.first_latent_is_density, m.net.freq_encoding_degree > 0,
+                     (uint2*)out_h4);
+  FVC_HIP(hipGetLastError());
+}
+
 void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t* out32, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_encode, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos, n, m.table.as<uint32_t>(),
@@ -410,6 +585,82 @@ void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, s
   for (int nb = 0; nb < 4; nb++) put(FRAG_H1 + nb, 3, H1, 32, nb * 16, col_head_in);
   for (int kb = 0; kb < 2; kb++) put(FRAG_H2 + kb, 4, H2, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
   for (int nb = 0; nb < 4; nb++) put(FRAG_H1C + nb, 3, H1, 32, nb * 16, col_head_cache);
+}
+
+// Split models.  Custom head input columns (my_fully_fused_mlp.cu:738-790):
+// latents 0..7, SH 8..23, SH differences 24..39, frequencies from 40 in the
+// order of freqEncChunk4 (per axis x, y, z: sin for f = 0..3, then cos);
+// quad q = 2 axis + cos holds columns 40 + 4q .. 40 + 4q + 3.  Without
+// frequencies, quads 0 and 1 carry the 8 padding inputs (value 1.0) and the
+// others multiply zero weights.
+static int col_freq(const fvc_net_desc& d, int q, int f) {
+  if (d.freq_encoding_degree == 4) return 40 + 4 * q + f;
+  return q < 2 ? 40 + 4 * q + f : -1;
+}
+// block A: groups 0, 1 latents 4g..4g+3, groups 2, 3 frequency quads 4, 5;
+// slots 4..7 SH coefficients 4g..4g+3
+static int col_vd_a(const fvc_net_desc& d, int g, int e) {
+  if (e >= 4) return 8 + 4 * g + (e - 4);
+  return g < 2 ? 4 * g + e : col_freq(d, g + 2, e);
+}
+// block B: SH differences 4g..4g+3, frequency quad g
+static int col_vd_b(const fvc_net_desc& d, int g, int e) { return e < 4 ? 24 + 4 * g + e : col_freq(d, g, e - 4); }
+// block A of the cache lookup: the two latents of the reduce-scatter, two
+// values of frequency quad 4 + g/2, SH coefficients 4g..4g+3
+static int col_vd_a_cache(const fvc_net_desc& d, int g, int e) {
+  if (e >= 4) return 8 + 4 * g + (e - 4);
+  if (e < 2) return (g & 1) + 4 * (g >> 1) + 2 * e;
+  return col_freq(d, 4 + (g >> 1), 2 * (g & 1) + (e - 2));
+}
+// mlp_first_head input: density-MLP outputs 4g..4g+3, SH of the initial
+// view direction 4g..4g+3 (nerf_network.h:186-194)
+static int col_vd_first(int g, int e) { return e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4); }
+
+void pack_fragments_vd(const uint16_t* base, const uint16_t* first, const uint16_t* head, const fvc_net_desc& d,
+                       std::vector<uint8_t>& out) {
+  out.assign((size_t)N_VD_FRAGS * 64 * 8 * 2, 0);
+  // fragment frag, lane (i, g), slot e <- W[rowf(i)][colf(g, e)] (0 where either is -1)
+  auto put = [&](int frag, const uint16_t* W, int n_in, auto rowf, auto colf) {
+    for (int lane = 0; lane < 64; lane++) {
+      const int i = lane % 16, g = lane / 16, r = rowf(i);
+      for (int e = 0; e < 8; e++) {
+        const int c = colf(g, e);
+        const uint16_t h = (r < 0 || c < 0) ? 0 : W[(size_t)r * n_in + c];
+        std::memcpy(&out[(((size_t)frag * 64 + lane) * 8 + e) * 2], &h, 2);
+      }
+    }
+  };
+  auto rows = [](int nb) { return [nb](int i) { return nb * 16 + i; }; };
+  auto chain = [](int kb) { return [kb](int g, int e) { return col_chain(kb, g, e); }; };
+  auto ident = [](int i) { return i; };
+  const uint16_t* W1 = base;
+  const uint16_t* W2 = base + 32 * 64;
+  const uint16_t* W3 = base + 32 * 64 + 64 * 64;
+  for (int nb = 0; nb < 4; nb++) put(FRAG_W1 + nb, W1, 32, rows(nb), col_first);
+  for (int nb = 0; nb < 4; nb++)
+    for (int kb = 0; kb < 2; kb++) put(FRAG_W2 + nb * 2 + kb, W2, 64, rows(nb), chain(kb));
+  for (int kb = 0; kb < 2; kb++) put(FRAG_W3 + kb, W3, 64, ident, chain(kb));
+  // mlp_first_head; output row k (k < 8) = the head's latent k: reference
+  // buffer row latent_offset + k, whose row 0 is log sigma (filled in by the
+  // kernels: zero weights here) and row r >= 1 first-head output r - 1
+  const int FL = d.first_head_n_hidden_layers;
+  for (int nb = 0; nb < 4; nb++) put(VD_F1 + nb, first, 32, rows(nb), col_vd_first);
+  if (FL == 2)
+    for (int nb = 0; nb < 4; nb++)
+      for (int kb = 0; kb < 2; kb++) put(VD_F2 + nb * 2 + kb, first + 32 * 64, 64, rows(nb), chain(kb));
+  const uint16_t* F3 = first + 32 * 64 + (FL - 1) * 64 * 64;
+  const int lo = d.latent_offset;
+  auto latent_row = [lo](int i) { return i < 8 && lo + i >= 1 ? lo + i - 1 : -1; };
+  for (int kb = 0; kb < 2; kb++) put(VD_F3 + kb, F3, 64, latent_row, chain(kb));
+  // custom colour head
+  const int HIN = vd_head_inputs(d);
+  const uint16_t* HO = head + (size_t)HIN * 64;
+  for (int nb = 0; nb < 4; nb++) {
+    put(VD_HA + nb, head, HIN, rows(nb), [&d](int g, int e) { return col_vd_a(d, g, e); });
+    put(VD_HB + nb, head, HIN, rows(nb), [&d](int g, int e) { return col_vd_b(d, g, e); });
+    put(VD_HAC + nb, head, HIN, rows(nb), [&d](int g, int e) { return col_vd_a_cache(d, g, e); });
+  }
+  for (int kb = 0; kb < 2; kb++) put(VD_HO + kb, HO, 64, ident, chain(kb));
 }
 
 }  // namespace fvc

@@ -508,7 +508,7 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
     const double bins = (double)(uint32_t)l.bins;
     // 1 B occupancy per bin, 28 B per emitted sample, ray state + entry per ray
     P.work(hs[i].march, bins, bins + 28.0 * l.n_r + 84.0 * l.alive, 0);
-    P.work(hs[i].net, l.n_r, l.n_r * (ALG_GATHER_B + 24.0 + 8.0), l.n_r * (ALG_DENSITY_FLOP + ALG_COLOUR_FLOP));
+    P.work(hs[i].net, l.n_r, l.n_r * (ALG_GATHER_B + 24.0 + 8.0), l.n_r * net_flop(m.net));
     P.work(hs[i].comp, l.n_r, 12.0 * l.n_r + 124.0 * l.alive, 0);
     n_eval += l.n_r;
   }

@@ -34,6 +34,13 @@ _SIGS = {
                                        C.POINTER(C.c_int64)]),
     "fvc_model_create": (C.c_int, [C.POINTER(abi.SceneInfo), C.POINTER(abi.NetDesc), _P, _P, C.c_int64, _P,
                                    C.c_int64, _P, C.c_int64, C.POINTER(_P)]),
+    "fvc_net_param_counts_viewdep": (C.c_int, [C.POINTER(abi.NetDesc), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                               C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "fvc_model_create_viewdep": (C.c_int, [C.POINTER(abi.SceneInfo), C.POINTER(abi.NetDesc), _P, _P, C.c_int64, _P,
+                                           C.c_int64, _P, C.c_int64, _P, C.c_int64, C.POINTER(_P)]),
+    "fvc_nerfacc_read_data_viewdep": (C.c_int, [C.c_char_p, _P, _P, _P, _P, _P]),
+    "fvc_network_latents_viewdep": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P]),
+    "fvc_network_head_viewdep": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64, _P, _P]),
     "fvc_model_destroy": (None, [_P]),
     "fvc_nerfacc_read_config": (C.c_int, [C.c_char_p, C.POINTER(abi.SceneInfo), C.POINTER(abi.StepsizeInfo),
                                           C.POINTER(abi.NetDesc), C.POINTER(C.c_int64)]),
@@ -140,7 +147,8 @@ def stepsize_setup(step, scene):
 
 def nerfacc_read(data_dir):
     """load_nerfacc_export without the GPU part (loader.cu:232-285): returns
-    (scene, stepsize, net, (pos_enc, mlp_base, mlp_head) fp16 bits, occupancy)."""
+    (scene, stepsize, net, (pos_enc, mlp_base, mlp_head) fp16 bits, occupancy);
+    split models (pos_enc, mlp_base, mlp_first_head, mlp_head)."""
     import numpy as np
     scene, step, net = abi.SceneInfo(), abi.StepsizeInfo(), abi.NetDesc()
     counts = (C.c_int64 * 4)()
@@ -148,8 +156,20 @@ def nerfacc_read(data_dir):
     check(load().fvc_nerfacc_read_config(d, C.byref(scene), C.byref(step), C.byref(net), counts))
     pos, base, head = (np.zeros(int(counts[k]), np.uint16) for k in range(3))
     occ = np.zeros(int(counts[3]), np.uint8)
+    if net.split_head:
+        first = np.zeros(net_param_counts_viewdep(net)[2], np.uint16)
+        check(load().fvc_nerfacc_read_data_viewdep(d, _addr(pos), _addr(base), _addr(first), _addr(head),
+                                                   _addr(occ)))
+        return scene, step, net, (pos, base, first, head), occ
     check(load().fvc_nerfacc_read_data(d, _addr(pos), _addr(base), _addr(head), _addr(occ)))
     return scene, step, net, (pos, base, head), occ
+
+
+def net_param_counts_viewdep(net):
+    """(pos_enc, mlp_base, mlp_first_head, mlp_head) counts of a split model."""
+    a, b, f, c = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+    check(load().fvc_net_param_counts_viewdep(C.byref(net), C.byref(a), C.byref(b), C.byref(f), C.byref(c)))
+    return a.value, b.value, f.value, c.value
 
 
 def net_param_counts(net):
@@ -163,13 +183,23 @@ class NerfNetwork:
     (nerf_network.h:84-369, loader.cu:20-55)."""
 
     def __init__(self, scene, net, occupancy_linear, params):
+        """params: (pos_enc, mlp_base, mlp_head) fp16 bits, or (pos_enc,
+        mlp_base, mlp_first_head, mlp_head) for a split model (set_params,
+        nerf_network.h:269-293)."""
         import numpy as np
-        pos, base, head = (np.ascontiguousarray(p, dtype=np.uint16) for p in params)
+        blocks = [np.ascontiguousarray(p, dtype=np.uint16) for p in params]
         occ = np.ascontiguousarray(occupancy_linear, dtype=np.uint8)
         self.scene, self.net = scene, net
         h = C.c_void_p()
-        check(load().fvc_model_create(C.byref(scene), C.byref(net), _addr(occ), _addr(pos), pos.size, _addr(base),
-                                      base.size, _addr(head), head.size, C.byref(h)))
+        if len(blocks) == 4:
+            pos, base, first, head = blocks
+            check(load().fvc_model_create_viewdep(C.byref(scene), C.byref(net), _addr(occ), _addr(pos), pos.size,
+                                                  _addr(base), base.size, _addr(first), first.size, _addr(head),
+                                                  head.size, C.byref(h)))
+        else:
+            pos, base, head = blocks
+            check(load().fvc_model_create(C.byref(scene), C.byref(net), _addr(occ), _addr(pos), pos.size,
+                                          _addr(base), base.size, _addr(head), head.size, C.byref(h)))
         self.h = h
 
     @classmethod
@@ -210,6 +240,15 @@ class NerfNetwork:
 
     def inferenceHead(self, head_in16, dir01, n, out4, stream=None):
         check(load().fvc_network_head(self.h, _addr(head_in16), _addr(dir01), n, _addr(out4), _stream(stream)))
+
+    # split models: inferenceLatents with the initial view direction, inferenceHead
+    def inferenceLatentsViewdep(self, pos, init_dir01, n, out16, stream=None):
+        check(load().fvc_network_latents_viewdep(self.h, _addr(pos), _addr(init_dir01), n, _addr(out16),
+                                                 _stream(stream)))
+
+    def inferenceHeadViewdep(self, head_in16, pos, dir01, init_dir01, n, out4, stream=None):
+        check(load().fvc_network_head_viewdep(self.h, _addr(head_in16), _addr(pos), _addr(dir01), _addr(init_dir01),
+                                              n, _addr(out4), _stream(stream)))
 
     def encode(self, pos, n, out32, stream=None):
         check(load().fvc_network_encode(self.h, _addr(pos), n, _addr(out32), _stream(stream)))

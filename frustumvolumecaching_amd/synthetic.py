@@ -58,6 +58,32 @@ def net_desc(log2_hashmap_size=19, n_levels=16, base_resolution=16, max_resoluti
         precision=precision)
 
 
+def net_desc_viewdep(log2_hashmap_size=19, first_latent_is_density=True, first_head_hidden_layers=1,
+                     freq_encoding_degree=4, **kw):
+    """Split model (NerfNetwork::init_viewdep): 16 density-MLP outputs,
+    mlp_first_head 32 -> 64 (x hidden layers) -> 8 latents, custom colour head
+    [8 latents, SH4, SH4 difference, frequency encoding] -> 64 -> 3."""
+    net = net_desc(log2_hashmap_size=log2_hashmap_size, **kw)
+    net.base_n_output = 16
+    net.latent_offset = 0 if first_latent_is_density else 1
+    net.split_head = 1
+    net.first_head_width = 64
+    net.first_head_n_hidden_layers = first_head_hidden_layers
+    net.first_head_n_input = 32
+    net.first_latent_is_density = int(bool(first_latent_is_density))
+    net.first_sh_degree = 4
+    net.freq_encoding_degree = freq_encoding_degree
+    return net
+
+
+def head_input_width(net):
+    """Input width of the colour head: 32, or for split models 8 latents +
+    16 SH + 16 SH differences + 6 * freq degree rounded up to 16."""
+    if not net.split_head:
+        return 32
+    return (8 + 16 + 16 + 6 * net.freq_encoding_degree + 15) // 16 * 16
+
+
 def grid_layout(net):
     """Per-level (scale, resolution, entries, entry offset) of the hash grid.
 
@@ -76,10 +102,17 @@ def grid_layout(net):
 
 
 def param_counts(net):
+    """(pos_enc, mlp_base, mlp_head) counts, with mlp_first_head before
+    mlp_head for split models."""
     lay = grid_layout(net)
     n_pos = (lay[-1][3] + lay[-1][2]) * net.n_features_per_level
     W, HW = net.base_width, net.head_width
-    return n_pos, 32 * W + W * W + W * 16, 32 * HW + HW * 16
+    n_head = head_input_width(net) * HW + HW * 16
+    if net.split_head:
+        FW = net.first_head_width
+        n_first = 32 * FW + (net.first_head_n_hidden_layers - 1) * FW * FW + FW * 16
+        return n_pos, 32 * W + W * W + W * 16, n_first, n_head
+    return n_pos, 32 * W + W * W + W * 16, n_head
 
 
 def xavier(seed, n_out_real, n_in, n_out_pad):
@@ -90,18 +123,27 @@ def xavier(seed, n_out_real, n_in, n_out_pad):
 
 
 def make_params(net, seed_weights=1, seed_table=2, table_range=0.1):
-    """(pos_enc, mlp_base, mlp_head) fp16 bit arrays in the nerfacc export layout."""
-    n_pos, _, _ = param_counts(net)
+    """(pos_enc, mlp_base, mlp_head) fp16 bit arrays in the nerfacc export
+    layout; split models (pos_enc, mlp_base, mlp_first_head, mlp_head)."""
+    n_pos = param_counts(net)[0]
     pos = f16_bits(uniform(seed_table, n_pos, -table_range, table_range))
     W, HW = net.base_width, net.head_width
     base = np.concatenate([
         xavier(seed_weights * 16 + 0, W, 32, W).ravel(),
         xavier(seed_weights * 16 + 1, W, W, W).ravel(),
         xavier(seed_weights * 16 + 2, net.base_n_output, W, 16).ravel()])
+    hin = head_input_width(net)
     head = np.concatenate([
-        xavier(seed_weights * 16 + 3, HW, 32, HW).ravel(),
+        xavier(seed_weights * 16 + 3, HW, hin, HW).ravel(),
         xavier(seed_weights * 16 + 4, 3, HW, 16).ravel()])
-    return pos, f16_bits(base), f16_bits(head)
+    if not net.split_head:
+        return pos, f16_bits(base), f16_bits(head)
+    FW = net.first_head_width
+    first = [xavier(seed_weights * 16 + 5, FW, 32, FW).ravel()]
+    for k in range(net.first_head_n_hidden_layers - 1):
+        first.append(xavier(seed_weights * 16 + 6 + k, FW, FW, FW).ravel())
+    first.append(xavier(seed_weights * 16 + 8, net.n_latents, FW, 16).ravel())
+    return pos, f16_bits(base), f16_bits(np.concatenate(first)), f16_bits(head)
 
 
 # ---------------------------------------------------------------------------
@@ -298,6 +340,27 @@ def small_plain(width=256, height=256, log2_hashmap_size=19, stepsize_setup=None
     return Workload("small_plain_256", scene, step, net, occ, params, look_at(eye, (0, 0, 0)), width, height)
 
 
+def small_viewdep(width=256, height=256, first_latent_is_density=True, first_head_hidden_layers=1,
+                  freq_encoding_degree=4, stepsize_setup=None):
+    """small_plain's scene and camera with a split (view-dependent) model."""
+    wl = small_plain(width, height, stepsize_setup=stepsize_setup)
+    net = net_desc_viewdep(first_latent_is_density=first_latent_is_density,
+                           first_head_hidden_layers=first_head_hidden_layers,
+                           freq_encoding_degree=freq_encoding_degree)
+    params = make_params(net, SEEDS["weights"], SEEDS["table"])
+    return Workload("small_viewdep_256", wl.scene, wl.step, net, wl.occupancy, params, wl.cam2world, width, height)
+
+
+def unbounded_1080p_viewdep(width=1920, height=1080, stepsize_setup=None):
+    """unbounded_1080p's scene and camera with a split model (DensityIntermediates,
+    one hidden first-head layer, frequency degree 4)."""
+    wl = unbounded_1080p(width, height, stepsize_setup=stepsize_setup)
+    net = net_desc_viewdep(log2_hashmap_size=wl.net.log2_hashmap_size)
+    params = make_params(net, SEEDS["weights"], SEEDS["table"])
+    return Workload("unbounded_1080p_viewdep", wl.scene, wl.step, net, wl.occupancy, params, wl.cam2world,
+                    width, height)
+
+
 def unbounded_1080p(width=1920, height=1080, log2_hashmap_size=21, stepsize_setup=None, n_blobs=64):
     """1920x1080 workload: T=2^21, 5 cascades of 128^3 over [-16,16]^3 with
     L-inf contraction of [-1,1]^3, 64 blobs out to radius 16, cone angle 1/256."""
@@ -346,7 +409,7 @@ def nerfacc_config(wl):
         c0, c1 = sc.contraction_aabb_from, sc.contraction_aabb_to
         scene["contraction"] = {"aabb": [c0.x, c0.y, c0.z, c1.x, c1.y, c1.z]}
     mlp = {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "None"}
-    return {
+    cfg = {
         "scene": scene,
         "mlp_base": {
             "encoding": {"otype": "HashGrid", "n_levels": net.n_levels, "n_features_per_level": net.n_features_per_level,
@@ -362,6 +425,19 @@ def nerfacc_config(wl):
             "network": dict(mlp, n_neurons=net.head_width, n_hidden_layers=net.head_n_hidden_layers),
         },
     }
+    if net.split_head:
+        # init_viewdep (nerf_network.h:120-158) keys
+        cfg["mlp_head"]["n_input"] = head_input_width(net)
+        cfg["mlp_head"]["freq_encoding_degree"] = net.freq_encoding_degree
+        cfg["mlp_first_head"] = {
+            "encoding": {"otype": "Composite", "nested": [
+                {"n_dims_to_encode": 3, "otype": "SphericalHarmonics", "degree": net.first_sh_degree},
+                {"otype": "Identity"}]},
+            "network": dict(mlp, n_neurons=net.first_head_width, n_hidden_layers=net.first_head_n_hidden_layers),
+            "n_input": net.first_head_n_input, "n_output": net.n_latents,
+            "first_latent_is_density": bool(net.first_latent_is_density),
+        }
+    return cfg
 
 
 def write_nerfacc_export(wl, out_dir, pos_separate=True):
@@ -372,7 +448,10 @@ def write_nerfacc_export(wl, out_dir, pos_separate=True):
     os.makedirs(out_dir, exist_ok=True)
     with open(os.path.join(out_dir, "config.json"), "w") as f:
         json.dump(nerfacc_config(wl), f, indent=1)
-    pos, base, head = (np.ascontiguousarray(p, np.uint16) for p in wl.params)
+    blocks = [np.ascontiguousarray(p, np.uint16) for p in wl.params]
+    pos, base, head = blocks[0], blocks[1], blocks[-1]
+    if len(blocks) == 4:
+        blocks[2].tofile(os.path.join(out_dir, "mlp_first_head.dat"))
     if pos_separate:
         pos.tofile(os.path.join(out_dir, "pos_enc.dat"))
         base.tofile(os.path.join(out_dir, "mlp_base.dat"))

@@ -98,7 +98,19 @@ typedef struct {
 /* Network description: the subset of the tcnn JSON config the render path
  * needs (nerf_network.h:88-118).  Supported: HashGrid L=16,F=2; density MLP
  * 32->64->64->16 (ReLU, no bias); head (n_latents + SH4(16) + pad 1.0 = 32)
- * ->64->3 (padded 16).  Anything else returns FVC_E_UNSUPPORTED. */
+ * ->64->3 (padded 16).  Anything else returns FVC_E_UNSUPPORTED.
+ *
+ * split_head = 1 describes the view-dependent split model
+ * (NerfNetwork::init_viewdep, nerf_network.h:120-158): the density MLP's 16
+ * outputs and SH4 of the initial view direction (32 inputs) feed
+ * mlp_first_head (64 wide, 1 or 2 hidden layers) whose outputs are the
+ * n_latents = 8 latents; the colour head is the reference's custom kernel
+ * (my_fully_fused_mlp.cu:677-895): inputs [8 latents, SH4(view dir),
+ * SH4(view dir) - SH4(initial view dir), frequency encoding of the position
+ * (freq_encoding_degree 4: 24 values; 0: none, 8 padding 1.0)] -> 64 -> 3.
+ * first_latent_is_density = 1: latent 0 is log sigma and the cache stores
+ * sigma in its place (DensityIntermediates, cache_renderer.cu:481-486),
+ * latent_offset = 0; else latent_offset = 1.  FP16 only. */
 enum { FVC_PREC_FP16 = 0, FVC_PREC_BF16 = 1, FVC_PREC_FP8 = 2 };
 typedef struct {
   int32_t n_levels;               /* 16 */
@@ -115,6 +127,14 @@ typedef struct {
   int32_t latent_offset;          /* 1: head input starts after log sigma (nerf_network.h:334-337) */
   int32_t sh_degree;              /* 4 */
   int32_t precision;              /* FVC_PREC_* (MLP arithmetic) */
+  int32_t split_head;             /* 1: view-dependent split model (mlp_first_head) */
+  int32_t first_head_width;       /* 64 */
+  int32_t first_head_n_hidden_layers; /* 1 or 2 */
+  int32_t first_head_n_input;     /* 32 = 16 density-MLP outputs + SH4 of the initial view direction */
+  int32_t first_latent_is_density;/* 1 (default of the reference) or 0 */
+  int32_t first_sh_degree;        /* 4 */
+  int32_t freq_encoding_degree;   /* 4 or 0 */
+  int32_t pad_;
 } fvc_net_desc;
 
 /* DensityInterpolVariant (cache_renderer.h:15-22) */
@@ -126,8 +146,11 @@ enum {
 enum { FVC_FN_LINEAR = 0, FVC_FN_SMOOTHSTEP = 1, FVC_FN_SMOOTHERSTEP = 2, FVC_FN_NEAREST = 3 };
 
 /* CacheSettings (cache_renderer.h:24-32).  Supported: interpol_variant
- * DENSITY, interpol_function LINEAR, use_inter_brick_interpolation = 1 (the
- * default_cache_settings, cache_renderer.h:55-63). */
+ * DENSITY (models whose latents exclude the density) or
+ * DENSITY_INTERMEDIATES (split models with first_latent_is_density, the
+ * variant eval.cu:216 picks for them), interpol_function LINEAR,
+ * use_inter_brick_interpolation = 1 (the default_cache_settings,
+ * cache_renderer.h:55-63).  A renderer starts with the variant of its model. */
 typedef struct {
   int32_t interpol_variant;
   int32_t interpol_function;
@@ -199,6 +222,9 @@ int fvc_stepsize_setup(fvc_stepsize_info* info, const fvc_scene_info* scene);
 /* Number of fp16 parameters of each block of the nerfacc export layout
  * (pos_enc.dat, mlp_base.dat, mlp_head.dat; loader.cu:232-285). Host only. */
 int fvc_net_param_counts(const fvc_net_desc* net, int64_t* n_pos_enc, int64_t* n_mlp_base, int64_t* n_mlp_head);
+/* The same with mlp_first_head.dat for split models (loader.cu:251-262). */
+int fvc_net_param_counts_viewdep(const fvc_net_desc* net, int64_t* n_pos_enc, int64_t* n_mlp_base,
+                                 int64_t* n_mlp_first_head, int64_t* n_mlp_head);
 
 /* ---- model: NerfNetwork<half> + occupancy bitfield (loader.cu:20-55, 287-301) ----
  * occupancy_linear_u8: host [nlvl][res][res][res] (x slowest), >0 = occupied;
@@ -209,6 +235,15 @@ int fvc_model_create(const fvc_scene_info* scene, const fvc_net_desc* net,
                      const uint16_t* params_mlp_base, int64_t n_mlp_base,
                      const uint16_t* params_mlp_head, int64_t n_mlp_head,
                      fvc_model** out_model);
+/* Split model: NerfNetwork::set_params(pos_enc, mlp_base, mlp_first_head,
+ * mlp_head) (nerf_network.h:286-293) after init_viewdep (loader.cu:254-262). */
+int fvc_model_create_viewdep(const fvc_scene_info* scene, const fvc_net_desc* net,
+                             const uint8_t* occupancy_linear_u8,
+                             const uint16_t* params_pos_enc, int64_t n_pos_enc,
+                             const uint16_t* params_mlp_base, int64_t n_mlp_base,
+                             const uint16_t* params_mlp_first_head, int64_t n_mlp_first_head,
+                             const uint16_t* params_mlp_head, int64_t n_mlp_head,
+                             fvc_model** out_model);
 void fvc_model_destroy(fvc_model* model);
 /* ---- model export: load_nerfacc_export (loader.cu:232-285) ----
  * <dir>/config.json ("scene", "mlp_base", "mlp_head"; parse_helper.h:25-63)
@@ -222,7 +257,12 @@ int fvc_nerfacc_read_config(const char* dir, fvc_scene_info* scene_out, fvc_step
 /* Read the four data blocks into caller buffers sized by read_config.  Host only. */
 int fvc_nerfacc_read_data(const char* dir, uint16_t* pos_enc, uint16_t* mlp_base, uint16_t* mlp_head,
                           uint8_t* occupancy_linear_u8);
-/* read_config + read_data + fvc_model_create; scene / stepsize / net may be NULL. */
+/* read_data for split models: mlp_first_head.dat as well (loader.cu:251),
+ * sized by fvc_net_param_counts_viewdep.  Host only. */
+int fvc_nerfacc_read_data_viewdep(const char* dir, uint16_t* pos_enc, uint16_t* mlp_base, uint16_t* mlp_first_head,
+                                  uint16_t* mlp_head, uint8_t* occupancy_linear_u8);
+/* read_config + read_data + fvc_model_create (or _viewdep when config.json
+ * has "mlp_first_head"); scene / stepsize / net may be NULL. */
 int fvc_model_load_nerfacc(const char* dir, fvc_scene_info* scene_out, fvc_stepsize_info* stepsize_out,
                            fvc_net_desc* net_out, fvc_model** out_model);
 /* Copy the Morton bitfield (nlvl*res^3/8 bytes) to host. */
@@ -237,6 +277,20 @@ int fvc_network_inference(fvc_model* model, const float* pos_dev, const float* d
 /* NerfNetwork::inferenceHead (nerf_network.h:204-266): head_in half[n][16]
  * (row 0 log sigma, rows 1..8 latents), dir01 float[n][3] -> out half[n][4]. */
 int fvc_network_head(fvc_model* model, const uint16_t* head_in_dev, const float* dir_dev, int64_t n, uint16_t* out_dev, void* stream);
+/* Split models: NerfNetwork::inferenceLatents with the initial view
+ * direction (nerf_network.h:173-196): pos, init_dir01 float[n][3] ->
+ * half[n][16] = the 8 head latents (as the colour head reads them from
+ * row latent_offset) in rows 0..7, log sigma in row 8, zeros after. */
+int fvc_network_latents_viewdep(fvc_model* model, const float* pos_dev, const float* init_dir_dev, int64_t n,
+                                uint16_t* out_dev, void* stream);
+/* Split models: inferenceHead with the custom head (nerf_network.h:215-219,
+ * my_fully_fused_mlp.cu:1363-1421): head_in half[n][16] in the layout of
+ * fvc_network_latents_viewdep, pos (frequency encoding), dir01, init_dir01
+ * float[n][3] -> half[n][4] = (r, g, b raw, density row: latent 0 when
+ * first_latent_is_density, else row 8). */
+int fvc_network_head_viewdep(fvc_model* model, const uint16_t* head_in_dev, const float* pos_dev,
+                             const float* dir_dev, const float* init_dir_dev, int64_t n, uint16_t* out_dev,
+                             void* stream);
 /* Multiresolution hash-grid encoding alone: pos float[n][3] -> half[n][32]. */
 int fvc_network_encode(fvc_model* model, const float* pos_dev, int64_t n, uint16_t* out_dev, void* stream);
 

@@ -35,6 +35,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -326,10 +327,14 @@ struct Net {
   std::vector<uint32_t> res, size, offset;  // per level: resolution, entries, entry offset
   std::vector<float> table;                 // fp16 values
   std::vector<float> W1, W2, W3, H1, H2;    // fp16 values, row-major [out][in]
-  int n_pos_enc = 0, n_base = 0, n_head = 0;
+  std::vector<float> F1, F2, F3;            // mlp_first_head of split models
+  int n_pos_enc = 0, n_base = 0, n_head = 0, n_first = 0;
   int head_in = 32;
+  // view-dependent split model (NerfNetwork::init_viewdep, nerf_network.h:120-158)
+  bool split = false, di = false;
+  int freq = 0;
   // per-layer absmax (FP8 mode)
-  float amax_w[5] = {0, 0, 0, 0, 0};
+  float amax_w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 // HashGrid geometry shared with fvc_net_param_counts (tcnn conventions).
@@ -359,6 +364,20 @@ bool net_layout(const fvc_net_desc& d, Net& n) {
   n.n_pos_enc = (int)off * n.F;
   n.n_base = 32 * d.base_width + d.base_width * d.base_width + d.base_width * 16;
   n.n_head = 32 * d.head_width + d.head_width * 16;
+  n.split = d.split_head != 0;
+  if (n.split) {
+    // custom head input: 8 latents + 16 SH + 16 SH differences + 6 * freq, padded to 16
+    if (d.n_latents != 8 || d.first_head_n_input != 32 || d.first_head_n_hidden_layers < 1 ||
+        d.first_head_n_hidden_layers > 2 || (d.freq_encoding_degree != 0 && d.freq_encoding_degree != 4) ||
+        d.latent_offset != (d.first_latent_is_density ? 0 : 1))
+      return false;
+    n.di = d.first_latent_is_density != 0;
+    n.freq = d.freq_encoding_degree;
+    n.head_in = (8 + 16 + 16 + 6 * n.freq + 15) / 16 * 16;
+    const int FW = d.first_head_width;
+    n.n_first = 32 * FW + (d.first_head_n_hidden_layers - 1) * FW * FW + FW * 16;
+    n.n_head = n.head_in * d.head_width + d.head_width * 16;
+  }
   return true;
 }
 
@@ -491,12 +510,94 @@ template <class R> void head(const Net& n, const R* lat, V3<R> dir01, Mode m, R*
 }
 
 // inference = latents + head + extract_density (nerf_network.h:160-171, 259-265)
+template <class R> void head_vd(const Net& n, const R* lat8, V3<R> pos, V3<R> dir01, V3<R> init01, Mode m, R* rgb);
+template <class R> void latents_vd(const Net& n, V3<R> p, V3<R> init01, Mode m, R* out16);
 template <class R> void inference(const Net& n, V3<R> p, V3<R> dir01, Mode m, R* out4) {
   R l16[16];
+  if (n.split) {
+    // the initial view direction is the view direction (performance_renderer.cu:269)
+    latents_vd(n, p, dir01, m, l16);
+    R rgb[3];
+    head_vd(n, l16, p, dir01, dir01, m, rgb);
+    out4[0] = rgb[0]; out4[1] = rgb[1]; out4[2] = rgb[2]; out4[3] = l16[8];
+    return;
+  }
   latents(n, p, m, l16);
   R rgb[3];
   head(n, l16 + n.d.latent_offset, dir01, m, rgb);
   out4[0] = rgb[0]; out4[1] = rgb[1]; out4[2] = rgb[2]; out4[3] = l16[0];
+}
+
+// ---------------------------------------------------------------------------
+// View-dependent split model
+// ---------------------------------------------------------------------------
+// inferenceLatents of a split model (nerf_network.h:183-196): mlp_base, then
+// mlp_first_head on [the 16 density-MLP outputs, SH4(initial view dir)]; its
+// outputs land in rows 1.. of the reference's buffer behind log sigma (row
+// 0), and the colour head reads 8 latents from row latent_offset.
+// out16 = [those 8 latents, log sigma, 0 ...].
+template <class R> void latents_vd(const Net& n, V3<R> p, V3<R> init01, Mode m, R* out16) {
+  R base16[16];
+  latents(n, p, m, base16);
+  R x[32], sh[16];
+  for (int k = 0; k < 16; k++) x[k] = base16[k];
+  sh4(init01, sh);
+  for (int k = 0; k < 16; k++) x[16 + k] = q16(sh[k], m);
+  const int FW = n.d.first_head_width;
+  R h1[128], h2[128], fo[16];
+  layer(n, 5, n.F1, FW, 32, x, h1, true, m, false);
+  const R* last = h1;
+  if (n.d.first_head_n_hidden_layers == 2) {
+    layer(n, 6, n.F2, FW, FW, h1, h2, true, m, false);
+    last = h2;
+  }
+  layer(n, 7, n.F3, 16, FW, last, fo, false, m, true);
+  // reference buffer row r: r = 0 log sigma, r >= 1 first-head output r - 1
+  for (int k = 0; k < 8; k++) {
+    const int row = n.d.latent_offset + k;
+    out16[k] = row == 0 ? base16[0] : fo[row - 1];
+  }
+  out16[8] = base16[0];
+  for (int k = 9; k < 16; k++) out16[k] = 0;
+}
+
+// freqEncChunk4 (my_fully_fused_mlp.cu:63-110): sin / cos of (2^f pi_f) * x,
+// the argument rounded to float as the reference forms it (F64: exact)
+template <class R> R freq_value(R x, int f, bool cosine) {
+  if (std::is_same<R, double>::value) {
+    const double a = std::ldexp(M_PI, f) * (double)x;
+    return (R)(cosine ? std::cos(a) : std::sin(a));
+  }
+  const float a = std::ldexp((float)M_PI, f) * (float)x;
+  return (R)(float)(cosine ? std::cos((double)a) : std::sin((double)a));
+}
+
+// custom colour head of split models (kernel_my_mlp_fused_wmma_fast,
+// my_fully_fused_mlp.cu:677-895; input assembly 738-790): [8 latents,
+// SH4(dir), SH4(dir) - SH4(init) (difference of the half coefficients,
+// rounded to half: shCoeffDiffChunk4, 193-203), frequency encoding of pos
+// (per axis x, y, z: sin(pi 2^f x) for f = 0..3, then cos), padding 1.0]
+// -> head_width (ReLU) -> 3.
+template <class R> void head_vd(const Net& n, const R* lat8, V3<R> pos, V3<R> dir01, V3<R> init01, Mode m, R* rgb) {
+  R x[64], a[16], b[16], h[128], o[16];
+  for (int k = 0; k < 8; k++) x[k] = lat8[k];
+  sh4(dir01, a);
+  sh4(init01, b);
+  for (int k = 0; k < 16; k++) {
+    const R ha = q16(a[k], m), hb = q16(b[k], m);
+    x[8 + k] = ha;
+    x[24 + k] = q16(ha - hb, m);
+  }
+  const R pc[3] = {pos.x, pos.y, pos.z};
+  int k = 40;
+  if (n.freq == 4)
+    for (int d = 0; d < 3; d++)
+      for (int c = 0; c < 2; c++)
+        for (int f = 0; f < 4; f++) x[k++] = q16(freq_value(pc[d], f, c == 1), m);
+  for (; k < n.head_in; k++) x[k] = R(1);
+  layer(n, 3, n.H1, n.d.head_width, n.head_in, x, h, true, m, false);
+  layer(n, 4, n.H2, 16, n.d.head_width, h, o, false, m, true);
+  rgb[0] = o[0]; rgb[1] = o[1]; rgb[2] = o[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -676,6 +777,8 @@ Cache<R>* cache_fill(const Model& M, const fvc_raymarch_info& rm, Mode mode, int
   C->rm = rm;
   const fvc_scene_info& sc = M.scene;
   const Step<R> st(rm.stepsize_info);
+  // rows of log sigma and of the first latent in a sample's network output
+  const int sig_row = M.net.split ? 8 : 0, lat_row = M.net.split ? 0 : M.net.d.latent_offset;
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   C->fx = W; C->fy = H; C->fz = frustum_depth(rm, sc);
   C->bx = (W + BS - 1) / BS; C->by = (H + BS - 1) / BS; C->bz = (C->fz + BS - 1) / BS;
@@ -768,13 +871,17 @@ Cache<R>* cache_fill(const Model& M, const fvc_raymarch_info& rm, Mode mode, int
         const FillRay<R>& r = T.rays[s.lane];
         const R dt = st.dt(s.t0);
         const R tm = (s.t0 + (s.t0 + dt)) * R(0.5);
-        latents(M.net, contract(r.ray.o + tm * r.ray.d, sc), mode, s.out);
+        // split models: the initial view direction is the fill ray's (cache_renderer.cu:899)
+        if (M.net.split)
+          latents_vd(M.net, contract(r.ray.o + tm * r.ray.d, sc), unit_to_01(r.ray.d), mode, s.out);
+        else
+          latents(M.net, contract(r.ray.o + tm * r.ray.d, sc), mode, s.out);
       }
       // transmittance walk in step order per ray (samples are stored per ray in step order)
       for (auto& s : T.samples) {
         FillRay<R>& r = T.rays[s.lane];
         const R dt = st.dt(s.t0);
-        const R density = std::exp(s.out[0]);
+        const R density = std::exp(s.out[sig_row]);
         R alpha = R(1) - std::exp(-density * dt);
         alpha = std::min(std::max(alpha, R(0)), R(1));
         if (alpha > (R)sc.alpha_thre) { r.T = r.T * (R(1) - alpha); T.visible++; }
@@ -817,9 +924,11 @@ Cache<R>* cache_fill(const Model& M, const fvc_raymarch_info& rm, Mode mode, int
           for (auto& s : T.samples) {
             const int vx = s.lane % BS, vy = s.lane / BS, vz = s.step;
             const size_t v = (size_t)slot * 512 + (size_t)(vz * 64 + vy * 8 + vx);
-            C->density[v] = std::exp(s.out[0]);
+            C->density[v] = std::exp(s.out[sig_row]);
             C->isset[v] = R(1);
-            for (int i = 0; i < 8; i++) C->lat[v * 8 + i] = s.out[M.net.d.latent_offset + i];
+            for (int i = 0; i < 8; i++) C->lat[v * 8 + i] = s.out[lat_row + i];
+            // DensityIntermediates: latent 0 holds the density (cache_renderer.cu:481-486, 501)
+            if (M.net.di) C->lat[v * 8] = q16(C->density[v], mode);
           }
         }
       }
@@ -855,9 +964,13 @@ Cache<R>* cache_fill(const Model& M, const fvc_raymarch_info& rm, Mode mode, int
 }
 
 // Trilinear cache lookup (generateNetworkInput_cache_shfl_perf_kernel_cg,
-// cache_performance_renderer.cu:678-787, FIRST_LATENT_IS_DENSITY = false):
-// head input row 0 = log(max(sum w*sigma, 1e-6)), rows 1..8 = sum w*l / sum w*isset.
-template <class R> void cache_lookup(const Cache<R>& C, V3<R> f, Mode mode, R* head_in9) {
+// cache_performance_renderer.cu:678-787).  FIRST_LATENT_IS_DENSITY = false
+// (di = false): head input row 0 = log(max(sum w*sigma, 1e-6)), rows 1..8 =
+// latents sum w*l / sum w*isset.  di = true (DensityIntermediates): latent 0
+// holds the density, so latent 0 = log(sum w*l_0) * W / W (W = sum w*isset,
+// no clamp, 770-779) and it is also the density row: head_in9 = [latent 0,
+// latents 0..7].
+template <class R> void cache_lookup(const Cache<R>& C, V3<R> f, Mode mode, R* head_in9, bool di = false) {
   const R fc[3] = {f.x - R(0.5), f.y - R(0.5), f.z - R(0.5)};
   R i0[3], lt[3];
   for (int k = 0; k < 3; k++) { i0[k] = std::floor(fc[k]); lt[k] = fc[k] - i0[k]; }
@@ -890,7 +1003,12 @@ template <class R> void cache_lookup(const Cache<R>& C, V3<R> f, Mode mode, R* h
   for (int i = 0; i < 8; i++) {
     R a[8];
     for (int c = 0; c < 8; c++) a[c] = ld[c][i];
-    head_in9[1 + i] = q16(tree(a) / wsum, mode);
+    if (di && i == 0) {
+      R v = std::log(tree(a)) * wsum;
+      head_in9[1] = head_in9[0] = q16(v / wsum, mode);
+    } else {
+      head_in9[1 + i] = q16(tree(a) / wsum, mode);
+    }
   }
 }
 
@@ -903,6 +1021,7 @@ void render_cached(const Model& M, const Cache<R>& C, const fvc_raymarch_info& r
   const Step<R> st(rm.stepsize_info);
   const Step<R> cst(C.rm.stepsize_info);
   const Mat34<R> w2c(C.rm.cam_info.world2cam);
+  const V3<R> cam_pos = Mat34<R>(C.rm.cam_info.cam2world).translation();
   const fvc_scene_info& sc = M.scene;
   int64_t tr = 0, tc = 0;
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : tr, tc)
@@ -948,9 +1067,17 @@ void render_cached(const Model& M, const Cache<R>& C, const fvc_raymarch_info& r
           R tf2;
           const V3<R> f2 = cam2froxel(oc + tm2 * dc, C.rm.cam_info, cst, tf2);
           R hin[9];
-          cache_lookup(C, f2, mode, hin);
+          cache_lookup(C, f2, mode, hin, M.net.di);
           R rgb[3];
-          head(M.net, hin + 1, dir01, mode, rgb);
+          if (M.net.split) {
+            // generateNetworkInput_cache_padded_kernel (323-410): position of
+            // the midpoint, initial view direction from the cache camera
+            const V3<R> wp2 = ri.ray.o + tm2 * ri.ray.d;
+            const V3<R> init01 = unit_to_01(normalize(wp2 - cam_pos));
+            head_vd(M.net, hin + 1, contract(wp2, sc), dir01, init01, mode, rgb);
+          } else {
+            head(M.net, hin + 1, dir01, mode, rgb);
+          }
           o4[0] = rgb[0]; o4[1] = rgb[1]; o4[2] = rgb[2]; o4[3] = hin[0];
           tc++;
         }
@@ -1565,13 +1692,22 @@ int orc_net_param_counts(const fvc_net_desc* d, int64_t* a, int64_t* b, int64_t*
   return 0;
 }
 
-Model* orc_model_create(const fvc_scene_info* sc, const fvc_net_desc* d, const uint8_t* occ_linear,
-                        const uint16_t* p_pos, int64_t n_pos, const uint16_t* p_base, int64_t n_base,
-                        const uint16_t* p_head, int64_t n_head) {
+int orc_net_param_counts_viewdep(const fvc_net_desc* d, int64_t* a, int64_t* b, int64_t* f, int64_t* c) {
+  Net n;
+  if (!net_layout(*d, n) || !n.split) return -1;
+  *a = n.n_pos_enc; *b = n.n_base; *f = n.n_first; *c = n.n_head;
+  return 0;
+}
+
+// split models pass mlp_first_head (p_first, n_first), others nullptr / 0
+Model* orc_model_create_viewdep(const fvc_scene_info* sc, const fvc_net_desc* d, const uint8_t* occ_linear,
+                                const uint16_t* p_pos, int64_t n_pos, const uint16_t* p_base, int64_t n_base,
+                                const uint16_t* p_first, int64_t n_first, const uint16_t* p_head, int64_t n_head) {
   auto* M = new Model();
   M->scene = *sc;
   if (!net_layout(*d, M->net) || n_pos != M->net.n_pos_enc || n_base != M->net.n_base || n_head != M->net.n_head ||
-      sc->grid_resolution != 128) {
+      sc->grid_resolution != 128 || (M->net.split && (!p_first || n_first != M->net.n_first)) ||
+      (!M->net.split && p_first)) {
     delete M;
     return nullptr;
   }
@@ -1589,15 +1725,30 @@ Model* orc_model_create(const fvc_scene_info* sc, const fvc_net_desc* d, const u
   take(p_base, 0, 32 * W, n.W1);
   take(p_base, 32 * W, (int64_t)W * W, n.W2);
   take(p_base, 32 * W + (int64_t)W * W, 16 * W, n.W3);
-  take(p_head, 0, 32 * HW, n.H1);
-  take(p_head, 32 * HW, 16 * HW, n.H2);
-  const std::vector<float>* Ws[5] = {&n.W1, &n.W2, &n.W3, &n.H1, &n.H2};
-  for (int l = 0; l < 5; l++) {
+  take(p_head, 0, (int64_t)n.head_in * HW, n.H1);
+  take(p_head, (int64_t)n.head_in * HW, 16 * HW, n.H2);
+  if (n.split) {
+    const int FW = d->first_head_width;
+    take(p_first, 0, 32 * FW, n.F1);
+    int64_t off = 32 * FW;
+    if (d->first_head_n_hidden_layers == 2) {
+      take(p_first, off, (int64_t)FW * FW, n.F2);
+      off += (int64_t)FW * FW;
+    }
+    take(p_first, off, 16 * FW, n.F3);
+  }
+  const std::vector<float>* Ws[8] = {&n.W1, &n.W2, &n.W3, &n.H1, &n.H2, &n.F1, &n.F2, &n.F3};
+  for (int l = 0; l < 8; l++) {
     float a = 0;
     for (float v : *Ws[l]) a = std::max(a, std::fabs(v));
     n.amax_w[l] = a;
   }
   return M;
+}
+Model* orc_model_create(const fvc_scene_info* sc, const fvc_net_desc* d, const uint8_t* occ_linear,
+                        const uint16_t* p_pos, int64_t n_pos, const uint16_t* p_base, int64_t n_base,
+                        const uint16_t* p_head, int64_t n_head) {
+  return orc_model_create_viewdep(sc, d, occ_linear, p_pos, n_pos, p_base, n_base, nullptr, 0, p_head, n_head);
 }
 void orc_model_destroy(Model* M) { delete M; }
 void orc_model_bitfield(const Model* M, uint8_t* out) { std::memcpy(out, M->bitfield.data(), M->bitfield.size()); }
@@ -1629,6 +1780,51 @@ void orc_latents(const Model* M, const float* pos, int64_t n, int mode, double* 
       latents(M->net, V3<float>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, (Mode)mode, o);
       for (int k = 0; k < 16; k++) out16[16 * i + k] = o[k];
     }
+  }
+}
+
+// split models: latents_vd (layout [8 head latents, log sigma, 0 ...])
+void orc_latents_viewdep(const Model* M, const float* pos, const float* init_dir, int64_t n, int mode,
+                         double* out16) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    if (mode == M_F64) {
+      double o[16];
+      latents_vd(M->net, V3<double>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]},
+                 V3<double>{init_dir[3 * i], init_dir[3 * i + 1], init_dir[3 * i + 2]}, (Mode)mode, o);
+      for (int k = 0; k < 16; k++) out16[16 * i + k] = o[k];
+    } else {
+      float o[16];
+      latents_vd(M->net, V3<float>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]},
+                 V3<float>{init_dir[3 * i], init_dir[3 * i + 1], init_dir[3 * i + 2]}, (Mode)mode, o);
+      for (int k = 0; k < 16; k++) out16[16 * i + k] = o[k];
+    }
+  }
+}
+
+// split models: head_vd on head_in16 in the layout of orc_latents_viewdep;
+// out4[3] = the density row (latent 0 with first_latent_is_density, else row 8)
+void orc_head_viewdep(const Model* M, const float* head_in16, const float* pos, const float* dir,
+                      const float* init_dir, int64_t n, int mode, double* out4) {
+  const int drow = M->net.di ? 0 : 8;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    if (mode == M_F64) {
+      double lat[8], rgb[3];
+      for (int k = 0; k < 8; k++) lat[k] = head_in16[16 * i + k];
+      head_vd(M->net, lat, V3<double>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]},
+              V3<double>{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]},
+              V3<double>{init_dir[3 * i], init_dir[3 * i + 1], init_dir[3 * i + 2]}, (Mode)mode, rgb);
+      for (int k = 0; k < 3; k++) out4[4 * i + k] = rgb[k];
+    } else {
+      float lat[8], rgb[3];
+      for (int k = 0; k < 8; k++) lat[k] = head_in16[16 * i + k];
+      head_vd(M->net, lat, V3<float>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]},
+              V3<float>{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]},
+              V3<float>{init_dir[3 * i], init_dir[3 * i + 1], init_dir[3 * i + 2]}, (Mode)mode, rgb);
+      for (int k = 0; k < 3; k++) out4[4 * i + k] = rgb[k];
+    }
+    out4[4 * i + 3] = head_in16[16 * i + drow];
   }
 }
 
@@ -1730,15 +1926,17 @@ void orc_render_cached(const Model* M, const CacheHandle* h, const fvc_raymarch_
 }
 
 // trilinear lookup alone at given froxel points (head inputs rows 0..8)
-void orc_cache_lookup(const CacheHandle* h, const float* froxel, int64_t n, double* out9) {
+void orc_cache_lookup(const CacheHandle* h, const float* froxel, int64_t n, int di, double* out9) {
   for (int64_t i = 0; i < n; i++) {
     if (h->d) {
       double o[9];
-      cache_lookup(*h->d, V3<double>{froxel[3 * i], froxel[3 * i + 1], froxel[3 * i + 2]}, (Mode)h->mode, o);
+      cache_lookup(*h->d, V3<double>{froxel[3 * i], froxel[3 * i + 1], froxel[3 * i + 2]}, (Mode)h->mode, o,
+                   di != 0);
       for (int k = 0; k < 9; k++) out9[9 * i + k] = o[k];
     } else {
       float o[9];
-      cache_lookup(*h->f, V3<float>{froxel[3 * i], froxel[3 * i + 1], froxel[3 * i + 2]}, (Mode)h->mode, o);
+      cache_lookup(*h->f, V3<float>{froxel[3 * i], froxel[3 * i + 1], froxel[3 * i + 2]}, (Mode)h->mode, o,
+                   di != 0);
       for (int k = 0; k < 9; k++) out9[9 * i + k] = o[k];
     }
   }
@@ -1751,6 +1949,7 @@ void orc_cache_lookup(const CacheHandle* h, const float* froxel, int64_t n, doub
 int64_t orc_diff_render(const Model* M, const fvc_raymarch_info* rm, const int32_t* pix, int64_t n, const float* pos,
                         const float* base, const float* head, int f64, double* rgb, const double* d_rgb,
                         double* g_pos, double* g_base, double* g_head) {
+  if (M->net.split) return -1;  // the adjoint covers the plain model
   if (f64) return diff_render<double>(*M, *rm, pix, n, pos, base, head, rgb, d_rgb, g_pos, g_base, g_head);
   return diff_render<float>(*M, *rm, pix, n, pos, base, head, rgb, d_rgb, g_pos, g_base, g_head);
 }
@@ -1787,7 +1986,7 @@ int64_t orc_diff_render_cached(const Model* M, const CacheHandle* h, const fvc_r
                                int64_t n, const float* pos, const float* base, const float* head, int f64, double* rgb,
                                const double* d_rgb, double* g_pos, double* g_base, double* g_head, double* g_pool,
                                int64_t* n_cache) {
-  if (!h->f) return -1;
+  if (!h->f || M->net.split) return -1;
   if (f64)
     return diff_render_cached<double>(*M, *h->f, *rm, pix, n, pos, base, head, rgb, d_rgb, g_pos, g_base, g_head,
                                       g_pool, n_cache);

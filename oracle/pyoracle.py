@@ -57,6 +57,13 @@ def lib():
                                                C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
             "orc_model_create": (P, [C.POINTER(abi.SceneInfo), C.POINTER(abi.NetDesc), P, P, C.c_int64, P,
                                      C.c_int64, P, C.c_int64]),
+            "orc_net_param_counts_viewdep": (C.c_int, [C.POINTER(abi.NetDesc), C.POINTER(C.c_int64),
+                                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                                       C.POINTER(C.c_int64)]),
+            "orc_model_create_viewdep": (P, [C.POINTER(abi.SceneInfo), C.POINTER(abi.NetDesc), P, P, C.c_int64, P,
+                                             C.c_int64, P, C.c_int64, P, C.c_int64]),
+            "orc_latents_viewdep": (None, [P, P, P, C.c_int64, C.c_int, P]),
+            "orc_head_viewdep": (None, [P, P, P, P, P, C.c_int64, C.c_int, P]),
             "orc_model_destroy": (None, [P]),
             "orc_model_bitfield": (None, [P, P]),
             "orc_encode": (None, [P, P, C.c_int64, C.c_int, P]),
@@ -74,7 +81,7 @@ def lib():
             "orc_cache_pool": (None, [P, C.c_int64, P]),
             "orc_render_cached": (None, [P, P, C.POINTER(abi.RaymarchInfo), C.c_int, C.c_int, C.c_int, C.c_int,
                                          P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
-            "orc_cache_lookup": (None, [P, P, C.c_int64, P]),
+            "orc_cache_lookup": (None, [P, P, C.c_int64, C.c_int, P]),
             "orc_diff_render": (C.c_int64, [P, C.POINTER(abi.RaymarchInfo), P, C.c_int64, P, P, P, C.c_int, P, P,
                                             P, P, P]),
             "orc_cache_from_arrays": (P, [C.POINTER(abi.RaymarchInfo), C.POINTER(abi.CacheInfo), P, P, P,
@@ -122,6 +129,12 @@ def net_param_counts(net):
     return a.value, b.value, c.value
 
 
+def net_param_counts_viewdep(net):
+    a, b, f, c = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+    assert lib().orc_net_param_counts_viewdep(C.byref(net), C.byref(a), C.byref(b), C.byref(f), C.byref(c)) == 0
+    return a.value, b.value, f.value, c.value
+
+
 def _f3(a):
     return np.ascontiguousarray(a, dtype=np.float32).reshape(-1, 3)
 
@@ -130,12 +143,20 @@ class Model:
     """Oracle model: scene + network + occupancy (mirrors fvc_model_create)."""
 
     def __init__(self, scene, net, occupancy_linear, params):
-        pos, base, head = (np.ascontiguousarray(p, dtype=np.uint16) for p in params)
+        """params: (pos_enc, mlp_base, mlp_head), or (pos_enc, mlp_base,
+        mlp_first_head, mlp_head) for a split model."""
+        blocks = [np.ascontiguousarray(p, dtype=np.uint16) for p in params]
         occ = np.ascontiguousarray(occupancy_linear, dtype=np.uint8)
         self.scene, self.net = scene, net
-        self._keep = (pos, base, head, occ)
-        self.h = lib().orc_model_create(C.byref(scene), C.byref(net), _p(occ), _p(pos), pos.size, _p(base),
-                                        base.size, _p(head), head.size)
+        self._keep = (*blocks, occ)
+        if len(blocks) == 4:
+            pos, base, first, head = blocks
+            self.h = lib().orc_model_create_viewdep(C.byref(scene), C.byref(net), _p(occ), _p(pos), pos.size,
+                                                    _p(base), base.size, _p(first), first.size, _p(head), head.size)
+        else:
+            pos, base, head = blocks
+            self.h = lib().orc_model_create(C.byref(scene), C.byref(net), _p(occ), _p(pos), pos.size, _p(base),
+                                            base.size, _p(head), head.size)
         if not self.h:
             raise ValueError("oracle rejected the model description")
 
@@ -183,6 +204,22 @@ class Model:
         dir01 = _f3(dir01)
         out = np.zeros((len(hin), 4), np.float64)
         lib().orc_head(self.h, _p(hin), _p(dir01), len(hin), mode, _p(out))
+        return out
+
+    def latents_viewdep(self, pos, init_dir01, mode=EMU16):
+        """Split models: [8 head latents, log sigma, 0...] per sample."""
+        pos, init = _f3(pos), _f3(init_dir01)
+        out = np.zeros((len(pos), 16), np.float64)
+        lib().orc_latents_viewdep(self.h, _p(pos), _p(init), len(pos), mode, _p(out))
+        return out
+
+    def head_viewdep(self, head_in16, pos, dir01, init_dir01, mode=EMU16):
+        """Split models: the custom colour head on [8 latents, log sigma]
+        rows; returns (r, g, b raw, density row)."""
+        hin = np.ascontiguousarray(head_in16, dtype=np.float32).reshape(-1, 16)
+        pos, dir01, init = _f3(pos), _f3(dir01), _f3(init_dir01)
+        out = np.zeros((len(hin), 4), np.float64)
+        lib().orc_head_viewdep(self.h, _p(hin), _p(pos), _p(dir01), _p(init), len(hin), mode, _p(out))
         return out
 
     def march(self, rm, max_per_ray=64):
@@ -297,10 +334,14 @@ class Cache:
         lib().orc_cache_pool(self.h, n, _p(out))
         return out
 
-    def lookup(self, froxel):
+    def lookup(self, froxel, di=None):
+        """Head inputs [density row, latents 0..7] at froxel points; di
+        (DensityIntermediates) defaults to the model's first_latent_is_density."""
+        if di is None:
+            di = bool(self.model.net.split_head and self.model.net.first_latent_is_density)
         f = _f3(froxel)
         out = np.zeros((len(f), 9), np.float64)
-        lib().orc_cache_lookup(self.h, _p(f), len(f), _p(out))
+        lib().orc_cache_lookup(self.h, _p(f), len(f), int(bool(di)), _p(out))
         return out
 
     def render(self, rm, crop=None):

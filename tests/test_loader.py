@@ -63,7 +63,13 @@ def test_export_errors(tmp_path, wl):
     _expect_error(d, lib.E_INVALID)
     d = S.write_nerfacc_export(wl, str(tmp_path / "split"))
     cfg = json.load(open(os.path.join(d, "config.json")))
-    cfg["mlp_first_head"] = {"n_output": 8}
+    cfg["mlp_first_head"] = {"n_output": 8}  # no "network": malformed
+    json.dump(cfg, open(os.path.join(d, "config.json"), "w"))
+    _expect_error(d, lib.E_INVALID)
+    # a split head outside the built instance (first head 128 wide)
+    cfg["mlp_first_head"] = {"n_output": 8, "n_input": 32, "encoding": {"otype": "SphericalHarmonics", "degree": 4},
+                             "network": {"n_neurons": 128, "n_hidden_layers": 1}}
+    cfg["mlp_head"]["freq_encoding_degree"] = 4
     json.dump(cfg, open(os.path.join(d, "config.json"), "w"))
     _expect_error(d, lib.E_UNSUPPORTED)
     with open(os.path.join(d, "config.json"), "w") as f:
