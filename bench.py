@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--cache-latents", choices=["fp16", "fp8"], default="fp16",
                     help="latent storage of the brick pool (fp8: E4M3 with a per-brick scale)")
     ap.add_argument("--diff-rays", type=int, default=1 << 18, help="rays per differentiable step (800x800 view)")
+    ap.add_argument("--viewdep", action="store_true",
+                    help="the same scene with a view-dependent split model (first head + custom colour head, "
+                         "DensityIntermediates cache)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference point for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -87,7 +90,13 @@ def main():
     from frustumvolumecaching_amd import lib, synthetic as S
 
     t0 = time.time()
-    wl = S.unbounded_1080p(width=a.width, height=a.height, stepsize_setup=lib.stepsize_setup)
+    if a.viewdep:
+        wl = S.unbounded_1080p_viewdep(width=a.width, height=a.height, stepsize_setup=lib.stepsize_setup)
+        if a.precision != "fp16" or a.cache_latents != "fp16":
+            sys.exit("bench.py: split models are FP16 only (MLP and cache latents)")
+        a.no_diff = True  # the differentiable render covers models without a split head
+    else:
+        wl = S.unbounded_1080p(width=a.width, height=a.height, stepsize_setup=lib.stepsize_setup)
     path = S.camera_path_1080p(wl, 30)
     log(f"[bench] workload built in {time.time() - t0:.1f}s")
     from frustumvolumecaching_amd import abi
@@ -240,7 +249,7 @@ def main():
     # FETCH_SIZE / WRITE_SIZE passes (calibrated against a plain copy)
     traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "cache_march_1080p_traffic.json")
-    if os.path.exists(tpath) and (W, H) == (1920, 1080):
+    if os.path.exists(tpath) and (W, H) == (1920, 1080) and not a.viewdep:
         with open(tpath) as fh:
             tj = json.load(fh)
         if tj.get("kernel") == dom:
@@ -276,7 +285,10 @@ def main():
         "data": "synthetic (seeded scene and random-init weights; no dataset)",
         "config": {"workload": f"unbounded_{W}x{H}: hash grid L16 F2 T2^21, 5x128^3 occupancy cascades, 64 blobs "
                                f"to r=16, cone 1/256; 30-frame path yaw 1deg + 0.02 fwd/frame, refill every 10",
-                   "resolution": [W, H], "frames": a.steps, "refill_every": 10},
+                   "resolution": [W, H], "frames": a.steps, "refill_every": 10,
+                   "model": "split head: first head 32-64-8, colour head 64-64-3 with SH differences and position "
+                            "frequencies, DensityIntermediates cache" if a.viewdep else
+                            "density MLP 32-64-64-16, colour MLP 32-64-3"},
         "cached_fps": round(1e3 / cached_mean, 3),
         "full_render_fps": round(1e3 / full_mean, 3),
         "speedup_cached_vs_full": round(full_mean / cached_mean, 3),
