@@ -130,20 +130,33 @@ def main():
         r.render_full(rm_of(1), img)
     torch.cuda.synchronize()
     log(f"[bench] warm-up done ({a.warmup} rounds)")
-    tune = None
+    tune = tune_full = None
+    shapes = {"cached": None, "full": None}
     if not a.no_tune and not (a.batch or a.max_steps or a.mlp_rows):
-        # autotuner: MLP rows per wavefront + chunk sizes, on a reprojected frame
+        # autotuner: MLP rows per wavefront + chunk sizes, per path on frame 1
         r.initCacheBlockwise(rm_of(0))
-        t = r.autotune(rm_of(1), cached=True)
-        tune = {"mlp_rows": t.mlp_rows, "target_batch": int(t.target_batch), "max_steps_per_ray": t.max_steps,
-                "ms_best": round(t.ms_best, 3), "ms_default": round(t.ms_default, 3)}
-        log(f"[bench] autotune: {tune}")
+        for kind in ("cached", "full"):
+            t = r.autotune(rm_of(1), cached=kind == "cached")
+            shapes[kind] = (t.mlp_rows, int(t.target_batch), t.max_steps)
+            d = {"mlp_rows": t.mlp_rows, "target_batch": int(t.target_batch), "max_steps_per_ray": t.max_steps,
+                 "ms_best": round(t.ms_best, 3), "ms_default": round(t.ms_default, 3)}
+            if kind == "cached":
+                tune = d
+            else:
+                tune_full = d
+            log(f"[bench] autotune {kind}: {d}")
+
+    def use_shapes(kind):
+        if shapes[kind]:
+            r.set_mlp_rows(shapes[kind][0])
+            r.set_batching(shapes[kind][1], shapes[kind][2])
 
     r.set_graph(not a.no_graph)
 
     def cached_path(profiled):
         """The 30-frame path: a fill every 10 frames, a reprojected frame each."""
         fill_ms, cached_ms, fill_stats, cached_stats = [], [], [], []
+        use_shapes("cached")
         r.profile_enable(profiled)
         for k in range(a.steps):
             if k % 10 == 0:
@@ -159,6 +172,7 @@ def main():
 
     def full_path(profiled):
         full_ms, full_stats = [], []
+        use_shapes("full")
         r.profile_enable(profiled)
         for k in range(a.steps):
             st, ms = timed(lambda: r.render_full(rm_of(k), img))
@@ -309,6 +323,7 @@ def main():
         "cpu_baseline": cpu,
         "async_refill_fps": async_fps,
         "autotune": tune,
+        "autotune_full": tune_full,
         "graph": not a.no_graph,
         "profiled_cached_fps": round(1e3 / float(np.mean(cached_ms_prof)), 3),
         "profiled_full_render_fps": round(1e3 / float(np.mean(full_ms_prof)), 3),

@@ -521,6 +521,7 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
 // reference stops a launch when either stream reaches n_steps samples; here
 // the launch holds n_steps samples in total.  That only moves sub-iteration
 // boundaries: every ray still composites the same samples in the same order.
+static constexpr int CLS_EMIT = 4;  // samples whose t0 the emission loads together
 __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int par, int trip_cutoff, MarchGrid mg,
                                                      ContractParams con, const uint32_t* __restrict__ coarse_g,
                                                      CacheView cv, RayState* __restrict__ rays, int4* __restrict__ entry,
@@ -564,27 +565,45 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
     // classification (sample_cache_performance_kernel, cache_performance_renderer.cu:96-160)
     oc = transform(cv.cam.world2cam, mr.o.x, mr.o.y, mr.o.z, 1.0f);
     dc = transform(cv.cam.world2cam, mr.d.x, mr.d.y, mr.d.z, 0.0f);
-    for (int k = 0; k < cnt; k++) {
-      const float t0 = t_scr[(int64_t)k * n_alive + i];
-      const float t1 = t0 + step_dt(mg.st, t0);
-      const float tm = (t0 + t1) * 0.5f;
-      float tf;
-      const V3 f = cam2froxel(oc + tm * dc, cv.cam, cv.st, tf);
-      const bool in_fov = f.x >= 0.0f && f.x < (float)cv.fx && f.y >= 0.0f && f.y < (float)cv.fy;
-      const bool inside = in_fov && f.z >= 0.0f && f.z < (float)cv.fz;
-      bool unsampled = true;
-      if (in_fov) {
-        const float known = cv.known[(int64_t)(int)f.y * cv.fx + (int)f.x];
-        unsampled = tf < cv.st.near_ || tf > known;
+    // CLS samples per step; each step issues all its loads (t0, known range,
+    // brick slot, then voxel density) before using them, from clamped
+    // addresses whose results are selected away where they do not apply
+    constexpr int CLS = 4;
+    const float* dens = cv.lat8 ? &cv.pool8[0].density : &cv.pool[0].density;
+    const int vstride = cv.lat8 ? (int)(sizeof(Voxel8) / 4) : (int)(sizeof(Voxel) / 4);
+    for (int k0 = 0; k0 < cnt; k0 += CLS) {
+      float t0v[CLS], tf[CLS], known[CLS], dv[CLS];
+      V3 f[CLS];
+      bool in_fov[CLS], inside[CLS];
+      int slot[CLS], loc[CLS];
+#pragma unroll
+      for (int u = 0; u < CLS; u++) t0v[u] = t_scr[(int64_t)min(k0 + u, cnt - 1) * n_alive + i];
+#pragma unroll
+      for (int u = 0; u < CLS; u++) {
+        const float t0 = t0v[u];
+        const float t1 = t0 + step_dt(mg.st, t0);
+        const float tm = (t0 + t1) * 0.5f;
+        f[u] = cam2froxel(oc + tm * dc, cv.cam, cv.st, tf[u]);
+        in_fov[u] = f[u].x >= 0.0f && f[u].x < (float)cv.fx && f[u].y >= 0.0f && f[u].y < (float)cv.fy;
+        inside[u] = in_fov[u] && f[u].z >= 0.0f && f[u].z < (float)cv.fz;
+        const int vx = inside[u] ? (int)floorf(f[u].x) : 0, vy = inside[u] ? (int)floorf(f[u].y) : 0;
+        const int vz = inside[u] ? (int)floorf(f[u].z) : 0;
+        const int64_t kidx = in_fov[u] ? (int64_t)(int)f[u].y * cv.fx + (int)f[u].x : 0;
+        known[u] = cv.known[kidx];
+        slot[u] = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
+        loc[u] = (vz & 7) * 64 + (vy & 7) * 8 + (vx & 7);
       }
-      if (!inside || unsampled) {
-        res_mask |= 1u << k;
-      } else {
-        const int vx = (int)floorf(f.x), vy = (int)floorf(f.y), vz = (int)floorf(f.z);
-        const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
-        if (slot >= 0 &&
-            vox_density(cv, (int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))) > 0.0f)
-          cache_mask |= 1u << k;
+#pragma unroll
+      for (int u = 0; u < CLS; u++) {
+        const int64_t v = slot[u] >= 0 ? (int64_t)slot[u] * BRICK_VOXELS + loc[u] : 0;
+        dv[u] = dens[v * vstride];
+      }
+#pragma unroll
+      for (int u = 0; u < CLS; u++) {
+        if (k0 + u >= cnt) break;
+        const bool unsampled = !in_fov[u] || tf[u] < cv.st.near_ || tf[u] > known[u];
+        if (!inside[u] || unsampled) res_mask |= 1u << (k0 + u);
+        else if (slot[u] >= 0 && dv[u] > 0.0f) cache_mask |= 1u << (k0 + u);
       }
     }
   }
@@ -597,30 +616,40 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
     const V3 d01 = unit_to_01(d);
     uint64_t order = 0;  // bit q set: the q-th kept sample is a cache sample
     int kr = 0, kc = 0;
-    for (int k = 0; k < cnt; k++) {
-      const bool is_r = (res_mask >> k) & 1u, is_c = (cache_mask >> k) & 1u;
-      if (!is_r && !is_c) continue;
-      const float t0 = t_scr[(int64_t)k * n_alive + i];
-      const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;
-      if (is_r) {
-        // generateNetworkInput_resample_perf_kernel (198-240): t_mid = t0 + dt/2
-        const V3 p = contract(o + tm2 * d, con);
-        const int64_t q = (int64_t)off_r + kr;
-        s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
-        s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
-        s_t0[q] = t0;
-      } else {
-        // generateNetworkInput_preparation_perf_kernel (242-313)
-        float tf;
-        const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
-        const int64_t q = (int64_t)off_c + kc;
-        c_frox[3 * q] = f.x; c_frox[3 * q + 1] = f.y; c_frox[3 * q + 2] = f.z;
-        c_ray[q] = i;
-        c_t0[q] = t0;
-        order |= 1ull << (kr + kc);
+    const uint32_t kept = res_mask | cache_mask;
+    for (int k0 = 0; k0 < cnt; k0 += CLS_EMIT) {
+      if (((kept >> k0) & ((1u << CLS_EMIT) - 1u)) == 0u) continue;
+      float t0v[CLS_EMIT];
+#pragma unroll
+      for (int u = 0; u < CLS_EMIT; u++) t0v[u] = t_scr[(int64_t)min(k0 + u, cnt - 1) * n_alive + i];
+#pragma unroll
+      for (int u = 0; u < CLS_EMIT; u++) {
+        const int k = k0 + u;
+        if (k >= cnt) break;
+        const bool is_r = (res_mask >> k) & 1u, is_c = (cache_mask >> k) & 1u;
+        if (!is_r && !is_c) continue;
+        const float t0 = t0v[u];
+        const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;
+        if (is_r) {
+          // generateNetworkInput_resample_perf_kernel (198-240): t_mid = t0 + dt/2
+          const V3 p = contract(o + tm2 * d, con);
+          const int64_t q = (int64_t)off_r + kr;
+          s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
+          s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
+          s_t0[q] = t0;
+        } else {
+          // generateNetworkInput_preparation_perf_kernel (242-313)
+          float tf;
+          const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
+          const int64_t q = (int64_t)off_c + kc;
+          c_frox[3 * q] = f.x; c_frox[3 * q + 1] = f.y; c_frox[3 * q + 2] = f.z;
+          c_ray[q] = i;
+          c_t0[q] = t0;
+          order |= 1ull << (kr + kc);
+        }
+        kr += is_r ? 1 : 0;
+        kc += is_c ? 1 : 0;
       }
-      kr += is_r ? 1 : 0;
-      kc += is_c ? 1 : 0;
     }
     if (nc > 0) {
       // the colour MLP's SH input depends on the ray only

@@ -85,15 +85,24 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
   if (valid) {
     const V3 o = {rs.ox, rs.oy, rs.oz}, d = {rs.dx, rs.dy, rs.dz};
     const V3 d01 = unit_to_01(d);
-    for (int k = 0; k < cnt; k++) {
-      const float t0 = t_scr[(int64_t)k * n_alive + i];
-      const float t1 = t0 + step_dt(mg.st, t0);
-      const float tm = (t0 + t1) * 0.5f;
-      const V3 p = contract(o + tm * d, con);
-      const int64_t q = (int64_t)off + k;
-      s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
-      s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
-      s_t0[q] = t0;
+    // the t0 of 4 samples are loaded together before they are used
+    for (int k0 = 0; k0 < cnt; k0 += 4) {
+      float t0v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) t0v[u] = t_scr[(int64_t)min(k0 + u, cnt - 1) * n_alive + i];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int k = k0 + u;
+        if (k >= cnt) break;
+        const float t0 = t0v[u];
+        const float t1 = t0 + step_dt(mg.st, t0);
+        const float tm = (t0 + t1) * 0.5f;
+        const V3 p = contract(o + tm * d, con);
+        const int64_t q = (int64_t)off + k;
+        s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
+        s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
+        s_t0[q] = t0;
+      }
     }
     rays[i].t = rs.t;
     rays[i].j = rs.j;
