@@ -527,8 +527,8 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
                                                      CacheView cv, RayState* __restrict__ rays, int4* __restrict__ entry,
                                                      int* __restrict__ alive_flag, float* __restrict__ t_scr,
                                                      float* __restrict__ s_pos, float* __restrict__ s_dir,
-                                                     float* __restrict__ s_t0, float* __restrict__ c_frox,
-                                                     int* __restrict__ c_ray, float* __restrict__ c_t0,
+                                                     float* __restrict__ s_t0, float4* __restrict__ c_rec,
+                                                     float* __restrict__ c_t0,
                                                      uint4* __restrict__ ray_sh, uint64_t* __restrict__ order_out) {
   const int n_alive = ctl[CTL_ALIVE + par];
   if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;  // whole block past the alive rays
@@ -642,8 +642,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
           float tf;
           const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
           const int64_t q = (int64_t)off_c + kc;
-          c_frox[3 * q] = f.x; c_frox[3 * q + 1] = f.y; c_frox[3 * q + 2] = f.z;
-          c_ray[q] = i;
+          c_rec[q] = make_float4(f.x, f.y, f.z, __builtin_bit_cast(float, i));
           c_t0[q] = t0;
           order |= 1ull << (kr + kc);
         }
@@ -772,8 +771,8 @@ __device__ __forceinline__ void lookup_sums(const CornerPair& c, float& wd, floa
 // wavefront = 16 TILES); all tiles' corner loads are issued before the math.
 // LAT8: the pool holds FP8 latents (Voxel8) decoded as code * brick scale.
 template <int P, int TILES, bool LAT8>
-__global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_frox, const int* __restrict__ c_ray,
-                                                    const uint2* __restrict__ ray_sh, int64_t n, CacheView cv,
+__global__ __launch_bounds__(256) void k_cache_head(const float4* __restrict__ c_rec, const uint2* __restrict__ ray_sh,
+                                                    int64_t n, CacheView cv,
                                                     const typename Mx<P>::T* __restrict__ frags, MlpScales sc,
                                                     uint2* __restrict__ c_out, const int* __restrict__ n_dev) {
   // n_dev: sample count written by the marcher (n is then its bound)
@@ -801,9 +800,9 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
       s[t] = (grp * TILES + t) * 16 + col;
       valid[t] = s[t] < n32;
       const int sl = valid[t] ? s[t] : n32 - 1;
-      sh[t] = ray_sh[c_ray[sl] * 4 + g];
-      lookup_corners<LAT8>(cv, c_frox[3 * sl] - 0.5f, c_frox[3 * sl + 1] - 0.5f, c_frox[3 * sl + 2] - 0.5f, ox, oy, fx,
-                           fy, fz, cp[t]);
+      const float4 rec = c_rec[sl];
+      sh[t] = ray_sh[__builtin_bit_cast(int, rec.w) * 4 + g];
+      lookup_corners<LAT8>(cv, rec.x - 0.5f, rec.y - 0.5f, rec.z - 0.5f, ox, oy, fx, fy, fz, cp[t]);
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
@@ -834,8 +833,8 @@ __global__ __launch_bounds__(256) void k_cache_head(const float* __restrict__ c_
 // density, so latent 0 = log(sum w l_0) * W / W with W = sum w isset
 // (770-779, no clamp) and it is also the sample's log density.
 template <int TILES>
-__global__ __launch_bounds__(256) void k_cache_head_vd(const float* __restrict__ c_frox, const int* __restrict__ c_ray,
-                                                       const float* __restrict__ c_t0, const uint2* __restrict__ ray_sh,
+__global__ __launch_bounds__(256) void k_cache_head_vd(const float4* __restrict__ c_rec, const float* __restrict__ c_t0,
+                                                       const uint2* __restrict__ ray_sh,
                                                        const RayState* __restrict__ rays, int64_t n, CacheView cv,
                                                        fvc_stepsize_info st, ContractParams con,
                                                        const h8* __restrict__ frags, int di, int freq_on,
@@ -864,11 +863,11 @@ __global__ __launch_bounds__(256) void k_cache_head_vd(const float* __restrict__
       s[t] = (grp * TILES + t) * 16 + col;
       valid[t] = s[t] < n32;
       const int sl = valid[t] ? s[t] : n32 - 1;
-      ray[t] = c_ray[sl];
+      const float4 rec = c_rec[sl];
+      ray[t] = __builtin_bit_cast(int, rec.w);
       t0[t] = c_t0[sl];
       sh[t] = ray_sh[ray[t] * 4 + g];
-      lookup_corners<false>(cv, c_frox[3 * sl] - 0.5f, c_frox[3 * sl + 1] - 0.5f, c_frox[3 * sl + 2] - 0.5f, ox, oy,
-                            fx, fy, fz, cp[t]);
+      lookup_corners<false>(cv, rec.x - 0.5f, rec.y - 0.5f, rec.z - 0.5f, ox, oy, fx, fy, fz, cp[t]);
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
@@ -972,8 +971,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   const fvc_raymarch_info& rm = vs.rms[0];
   const double n_px = (double)vs.n_px;
   const int64_t cap = r.sample_capacity;
-  r.c_frox.ensure(cap * 3 * sizeof(float));
-  r.c_ray.ensure(cap * sizeof(int));
+  r.c_rec.ensure(cap * sizeof(float4));
   r.ray_sh.ensure(vs.n_px * 32);
   r.c_t0.ensure(cap * sizeof(float));
   r.c_out.ensure(cap * sizeof(uint2));
@@ -1006,7 +1004,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       hipLaunchKernelGGL(k_cache_march, grid, dim3(256), 0, s, ctl, cur, cutoff, mg, m.con, m.coarse.as<uint32_t>(), cv,
                          r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
                          r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
-                         r.c_frox.as<float>(), r.c_ray.as<int>(), r.c_t0.as<float>(), r.ray_sh.as<uint4>(),
+                         r.c_rec.as<float4>(), r.c_t0.as<float>(), r.ray_sh.as<uint4>(),
                          r.ray_order.as<uint64_t>());
       FVC_HIP(hipGetLastError());
       h.march = P.end(ev, s, "cache_march");
@@ -1016,8 +1014,8 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       h.net = P.end(ev, s, "network_resample");
       ev = P.begin(s);
 #define L3(P, TL, F8)                                                                                                \
-  hipLaunchKernelGGL((k_cache_head<P, TL, F8>), dim3(head_blocks), dim3(256), 0, s, r.c_frox.as<float>(),             \
-                     r.c_ray.as<int>(), r.ray_sh.as<uint2>(), (int64_t)n_bound, cv, m.frags.as<typename Mx<P>::T>(),  \
+  hipLaunchKernelGGL((k_cache_head<P, TL, F8>), dim3(head_blocks), dim3(256), 0, s, r.c_rec.as<float4>(),            \
+                     r.ray_sh.as<uint2>(), (int64_t)n_bound, cv, m.frags.as<typename Mx<P>::T>(),                     \
                      m.mlp_scales, r.c_out.as<uint2>(), ctl + CTL_C)
 #define L2(P, TL)                      \
   if (cv.lat8) { L3(P, TL, true); }    \
@@ -1026,8 +1024,8 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   if (r.mlp_rows == 32) { L2(P, 2); } \
   else { L2(P, 1); }
 #define LVD(TL)                                                                                                   \
-  hipLaunchKernelGGL((k_cache_head_vd<TL>), dim3(head_blocks), dim3(256), 0, s, r.c_frox.as<float>(),                 \
-                     r.c_ray.as<int>(), r.c_t0.as<float>(), r.ray_sh.as<uint2>(), r.rays[cur].as<RayState>(),         \
+  hipLaunchKernelGGL((k_cache_head_vd<TL>), dim3(head_blocks), dim3(256), 0, s, r.c_rec.as<float4>(),                \
+                     r.c_t0.as<float>(), r.ray_sh.as<uint2>(), r.rays[cur].as<RayState>(),                           \
                      (int64_t)n_bound, cv, rm.stepsize_info, m.con, m.frags.as<h8>(), m.net.first_latent_is_density, \
                      m.net.freq_encoding_degree > 0, r.c_out.as<uint2>(), ctl + CTL_C)
       if (m.net.split_head) {

@@ -363,7 +363,7 @@ struct Renderer {
   int* pinned = nullptr; // pinned host mirror of the counters
   // sample streams
   DevBuf s_pos, s_dir, s_t0, s_out;            // resample / full stream
-  DevBuf c_frox, c_ray, c_t0, c_out;            // cache-hit stream (c_ray: ray index)
+  DevBuf c_rec, c_t0, c_out;  // cache-hit stream: c_rec = (froxel x, y, z, ray index bits) per sample
   DevBuf ray_sh;                                // 16 fp16 SH coefficients per ray and launch
   // fill scratch (own counters and pinned mirror: the fill may run on the
   // fill stream from another thread)
