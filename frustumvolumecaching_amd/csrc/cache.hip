@@ -528,8 +528,9 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
                                                      int* __restrict__ alive_flag, float* __restrict__ t_scr,
                                                      float* __restrict__ s_pos, float* __restrict__ s_dir,
                                                      float* __restrict__ s_t0, float4* __restrict__ c_rec,
-                                                     float* __restrict__ c_t0,
-                                                     uint4* __restrict__ ray_sh, uint64_t* __restrict__ order_out) {
+                                                     float* __restrict__ c_t0, const int* __restrict__ pix_in,
+                                                     uint4* __restrict__ ray_sh, float4* __restrict__ ray_od,
+                                                     uint64_t* __restrict__ order_out) {
   const int n_alive = ctl[CTL_ALIVE + par];
   if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;  // whole block past the alive rays
   const int n_steps = ctl[CTL_NSTEPS];
@@ -545,8 +546,23 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
   bool alive = false;
   RayState r{};
   V3 oc = {0, 0, 0}, dc = {0, 0, 0};
+  int px = 0;
   if (valid) {
     r = rays[i];
+    px = pix_in[i];
+    if (r.j == 0) {
+      // first launch of the ray: its per-pixel tables (the colour MLP's SH
+      // input depends on the direction only; split models also read the ray)
+      uint4 lo, hi;
+      const V3 d01 = unit_to_01(V3{r.dx, r.dy, r.dz});
+      sh4_all(d01.x, d01.y, d01.z, lo, hi);
+      ray_sh[2 * (int64_t)px] = lo;
+      ray_sh[2 * (int64_t)px + 1] = hi;
+      if (ray_od) {
+        ray_od[2 * (int64_t)px] = make_float4(r.ox, r.oy, r.oz, 0.0f);
+        ray_od[2 * (int64_t)px + 1] = make_float4(r.dx, r.dy, r.dz, 0.0f);
+      }
+    }
     alive = true;
     MarchRay mr = march_ray(r);
     int trips = 0;
@@ -642,20 +658,13 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
           float tf;
           const V3 f = cam2froxel(oc + tm2 * dc, cv.cam, cv.st, tf);
           const int64_t q = (int64_t)off_c + kc;
-          c_rec[q] = make_float4(f.x, f.y, f.z, __builtin_bit_cast(float, i));
+          c_rec[q] = make_float4(f.x, f.y, f.z, __builtin_bit_cast(float, px));
           c_t0[q] = t0;
           order |= 1ull << (kr + kc);
         }
         kr += is_r ? 1 : 0;
         kc += is_c ? 1 : 0;
       }
-    }
-    if (nc > 0) {
-      // the colour MLP's SH input depends on the ray only
-      uint4 lo, hi;
-      sh4_all(d01.x, d01.y, d01.z, lo, hi);
-      ray_sh[2 * i] = lo;
-      ray_sh[2 * i + 1] = hi;
     }
     rays[i].t = r.t;
     rays[i].j = r.j;
@@ -835,7 +844,7 @@ __global__ __launch_bounds__(256) void k_cache_head(const float4* __restrict__ c
 template <int TILES>
 __global__ __launch_bounds__(256) void k_cache_head_vd(const float4* __restrict__ c_rec, const float* __restrict__ c_t0,
                                                        const uint2* __restrict__ ray_sh,
-                                                       const RayState* __restrict__ rays, int64_t n, CacheView cv,
+                                                       const float4* __restrict__ ray_od, int64_t n, CacheView cv,
                                                        fvc_stepsize_info st, ContractParams con,
                                                        const h8* __restrict__ frags, int di, int freq_on,
                                                        uint2* __restrict__ c_out, const int* __restrict__ n_dev) {
@@ -877,8 +886,8 @@ __global__ __launch_bounds__(256) void k_cache_head_vd(const float4* __restrict_
       if (di && g == 0) l0 = (logf(n0) * wd) / wd;
       const _Float16 lat0 = (_Float16)l0, lat1 = (_Float16)(n1 / wd);
       // the sample's midpoint (generateNetworkInput_preparation_perf_kernel, 285-291)
-      const RayState rs = rays[ray[t]];
-      const V3 o = {rs.ox, rs.oy, rs.oz}, d = {rs.dx, rs.dy, rs.dz};
+      const float4 ro = ray_od[2 * (int64_t)ray[t]], rd = ray_od[2 * (int64_t)ray[t] + 1];
+      const V3 o = {ro.x, ro.y, ro.z}, d = {rd.x, rd.y, rd.z};
       const float tm = t0[t] + step_dt(st, t0[t]) * 0.5f;
       const V3 wp = o + tm * d;
       const V3 p = contract(wp, con);
@@ -973,6 +982,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   const int64_t cap = r.sample_capacity;
   r.c_rec.ensure(cap * sizeof(float4));
   r.ray_sh.ensure(vs.n_px * 32);
+  if (m.net.split_head) r.ray_od.ensure(vs.n_px * 32);
   r.c_t0.ensure(cap * sizeof(float));
   r.c_out.ensure(cap * sizeof(uint2));
   hipStream_t s = r.stream();
@@ -1004,7 +1014,8 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       hipLaunchKernelGGL(k_cache_march, grid, dim3(256), 0, s, ctl, cur, cutoff, mg, m.con, m.coarse.as<uint32_t>(), cv,
                          r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
                          r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
-                         r.c_rec.as<float4>(), r.c_t0.as<float>(), r.ray_sh.as<uint4>(),
+                         r.c_rec.as<float4>(), r.c_t0.as<float>(), r.ray_pix[cur].as<int>(), r.ray_sh.as<uint4>(),
+                         m.net.split_head ? r.ray_od.as<float4>() : nullptr,
                          r.ray_order.as<uint64_t>());
       FVC_HIP(hipGetLastError());
       h.march = P.end(ev, s, "cache_march");
@@ -1025,7 +1036,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   else { L2(P, 1); }
 #define LVD(TL)                                                                                                   \
   hipLaunchKernelGGL((k_cache_head_vd<TL>), dim3(head_blocks), dim3(256), 0, s, r.c_rec.as<float4>(),                \
-                     r.c_t0.as<float>(), r.ray_sh.as<uint2>(), r.rays[cur].as<RayState>(),                           \
+                     r.c_t0.as<float>(), r.ray_sh.as<uint2>(), r.ray_od.as<float4>(),                                 \
                      (int64_t)n_bound, cv, rm.stepsize_info, m.con, m.frags.as<h8>(), m.net.first_latent_is_density, \
                      m.net.freq_encoding_degree > 0, r.c_out.as<uint2>(), ctl + CTL_C)
       if (m.net.split_head) {

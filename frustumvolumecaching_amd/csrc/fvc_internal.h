@@ -363,8 +363,9 @@ struct Renderer {
   int* pinned = nullptr; // pinned host mirror of the counters
   // sample streams
   DevBuf s_pos, s_dir, s_t0, s_out;            // resample / full stream
-  DevBuf c_rec, c_t0, c_out;  // cache-hit stream: c_rec = (froxel x, y, z, ray index bits) per sample
-  DevBuf ray_sh;                                // 16 fp16 SH coefficients per ray and launch
+  DevBuf c_rec, c_t0, c_out;  // cache-hit stream: c_rec = (froxel x, y, z, pixel index bits) per sample
+  DevBuf ray_od;              // split models: ray origin, direction per pixel (2 float4)
+  DevBuf ray_sh;                                // 16 fp16 SH coefficients per pixel (first launch of its ray)
   // fill scratch (own counters and pinned mirror: the fill may run on the
   // fill stream from another thread)
   DevBuf f_rays, f_tiles, f_list, f_pos, f_t0, f_meta, f_out16, f_counters;
