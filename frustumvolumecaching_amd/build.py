@@ -5,9 +5,10 @@
 Every translation unit is compiled with -ffp-contract=off and correctly
 rounded fp32 division / square root, which the bit-exact marcher relies on,
 and with packed-FP32 instruction selection disabled.  In the differentiable
-path MFMA results go to VGPRs (-amdgpu-mfma-vgpr-form): its FP32 layers chain
-through registers with no accumulator-register reads (measured: forward 28 %
-faster; the render kernels gained nothing from it).
+path and the cache kernels MFMA results go to VGPRs (-amdgpu-mfma-vgpr-form):
+the layers chain through registers with no accumulator-register reads
+(measured: differentiable forward 28 % faster, cache head 4 %; the network
+kernels gained nothing from it).
 """
 import argparse
 import concurrent.futures as cf
@@ -28,7 +29,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-EXTRA = {"diff.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+EXTRA = {"diff.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"], "cache.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 SOURCES = ["network.hip", "render.hip", "cache.hip", "diff.hip", "tune.hip", "loader.hip", "capi.hip"]
 

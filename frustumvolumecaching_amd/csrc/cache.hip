@@ -776,60 +776,140 @@ __device__ __forceinline__ void lookup_sums(const CornerPair& c, float& wd, floa
   n1 = swap32_sum(m1, m3);
 }
 
-// TILES: 16-sample tiles per wave and loop iteration (MLP rows per
-// wavefront = 16 TILES); all tiles' corner loads are issued before the math.
-// LAT8: the pool holds FP8 latents (Voxel8) decoded as code * brick scale.
-template <int P, int TILES, bool LAT8>
-__global__ __launch_bounds__(256) void k_cache_head(const float4* __restrict__ c_rec, const uint2* __restrict__ ray_sh,
+// One lane per sample: a wave takes 64 cache samples.  Each lane fetches
+// its sample's 8 corners (corner c = 4x + 2y + z, outside the frustum or in
+// unset bricks the zero voxel), sums them in the reference's order
+// ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) (cache_performance_renderer.cu:736-
+// 787), forms the colour-MLP input row -- latents sum w*l / sum w*isset, SH of
+// its pixel's ray, the constant 1.0 padding (nerf_network.h:248-253) -- and
+// writes it to LDS in the MFMA operand order of FRAG_H1 (lane group g of a
+// tile reads the 8 halves of its K slots); the wave then evaluates its four
+// 16-sample tiles.  The corner arithmetic is done once per sample instead of
+// once per lane of a 4-lane group.  LAT8: the pool holds FP8 latents (Voxel8)
+// decoded as code * brick scale.
+static const int HEAD_REC = 5;  // uint4 per sample record in LDS: 4 operand groups + log sigma (80 bytes)
+
+template <int P, bool LAT8>
+__global__ __launch_bounds__(256) void k_cache_head(const float4* __restrict__ c_rec, const uint4* __restrict__ ray_sh,
                                                     int64_t n, CacheView cv,
                                                     const typename Mx<P>::T* __restrict__ frags, MlpScales sc,
                                                     uint2* __restrict__ c_out, const int* __restrict__ n_dev) {
   // n_dev: sample count written by the marcher (n is then its bound)
   if (n_dev) n = *n_dev;
-  if ((int64_t)blockIdx.x * 4 * 16 * TILES >= n) return;
+  if ((int64_t)blockIdx.x * 256 >= n) return;
   typedef typename Mx<P>::T T;
-  __shared__ T wl[6 * 64];  // FRAG_H1C (4), FRAG_H2 (2)
-  stage_frags(frags + FRAG_H1C * 64, wl, 4);
-  stage_frags(frags + FRAG_H2 * 64, wl + 4 * 64, 2);
+  __shared__ T wl[6 * 64];  // FRAG_H1 (4), FRAG_H2 (2)
+  __shared__ uint4 rec_lds[4][64 * HEAD_REC];
+  stage_frags(frags + FRAG_H1 * 64, wl, 6);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
-  const int n32 = (int)n;  // the stream holds fewer than 2^31 / 3 samples
-  const int n_groups = (n32 + 16 * TILES - 1) / (16 * TILES);
-  const float ox = (float)(g >> 1), oy = (float)(g & 1);
-  const float fx = (float)cv.fx, fy = (float)cv.fy, fz = (float)cv.fz;
+  const int n32 = (int)n;  // the stream holds fewer than 2^31 / 4 samples
+  const int n_groups = (n32 + 63) / 64;
+  uint4* recw = rec_lds[wave];
+  const uint32_t one2 = 0x3c003c00u;  // two fp16 1.0
   for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
     const int li = opaque(lane);
-    int s[TILES];
-    bool valid[TILES];
-    CornerPair cp[TILES];
-    uint2 sh[TILES];
+    const int s = grp * 64 + lane;
+    const int sl = s < n32 ? s : n32 - 1;
+    const float4 rec = c_rec[sl];
+    const int px = __builtin_bit_cast(int, rec.w);
+    const uint4 sh_lo = ray_sh[2 * (int64_t)px], sh_hi = ray_sh[2 * (int64_t)px + 1];
+    const float fcx = rec.x - 0.5f, fcy = rec.y - 0.5f, fcz = rec.z - 0.5f;
+    const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
+    const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
+    // weight factors (1 - off)(1 - 2l) + l of offsets 0 and 1
+    const float wxs[2] = {(1.0f - 2.0f * lx) + lx, lx}, wys[2] = {(1.0f - 2.0f * ly) + ly, ly};
+    const float wzs[2] = {(1.0f - 2.0f * lz) + lz, lz};
+    uint4 lat[8];
+    float2 dis[8];
+    float scl[8];
+    // integer cell of the corner (0, 0, 0); corner coordinates are exact
+    // small integers, so the frustum test is an unsigned compare
+    const int ixi = (int)ix, iyi = (int)iy, izi = (int)iz;
 #pragma unroll
-    for (int t = 0; t < TILES; t++) {
-      s[t] = (grp * TILES + t) * 16 + col;
-      valid[t] = s[t] < n32;
-      const int sl = valid[t] ? s[t] : n32 - 1;
-      const float4 rec = c_rec[sl];
-      sh[t] = ray_sh[__builtin_bit_cast(int, rec.w) * 4 + g];
-      lookup_corners<LAT8>(cv, rec.x - 0.5f, rec.y - 0.5f, rec.z - 0.5f, ox, oy, fx, fy, fz, cp[t]);
+    for (int c = 0; c < 8; c++) {
+      const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
+      const uint32_t cx = (uint32_t)(ixi + ox), cy = (uint32_t)(iyi + oy), cz = (uint32_t)(izi + oz);
+      const bool inside = cx < (uint32_t)cv.fx && cy < (uint32_t)cv.fy && cz < (uint32_t)cv.fz;
+      const uint32_t vx = inside ? cx : 0u, vy = inside ? cy : 0u, vz = inside ? cz : 0u;
+      const int slot = cv.table[((vz >> 3) * (uint32_t)cv.by + (vy >> 3)) * (uint32_t)cv.bx + (vx >> 3)];
+      const bool ok = inside && slot >= 0;
+      const size_t vi = (size_t)(uint32_t)slot * BRICK_VOXELS + ((vz & 7u) * 64u + (vy & 7u) * 8u + (vx & 7u));
+      if (LAT8) {
+        const uint4 q = *reinterpret_cast<const uint4*>(ok ? (const void*)(cv.pool8 + vi) : (const void*)&k_zero_voxel);
+        lat[c] = make_uint4(q.x, q.y, 0u, 0u);
+        dis[c] = make_float2(__builtin_bit_cast(float, q.z), __builtin_bit_cast(float, q.w));
+        scl[c] = ok ? cv.scales[(uint32_t)slot] : 0.0f;
+      } else {
+        const Voxel* vp = ok ? cv.pool + vi : &k_zero_voxel;
+        lat[c] = *reinterpret_cast<const uint4*>(vp->lat);
+        dis[c] = *reinterpret_cast<const float2*>(&vp->density);
+        scl[c] = 0.0f;
+      }
     }
+    // per-corner products, then the reference's summation tree
+    float cw[8], cd[8], cl[8][8];
 #pragma unroll
-    for (int t = 0; t < TILES; t++) {
-      float wd, dd, n0, n1;
-      lookup_sums<LAT8>(cp[t], wd, dd, n0, n1);
-      // latents sum w*l / sum w*isset (re-weighted)
-      const T x = head_input_cache<P>((_Float16)(n0 / wd), (_Float16)(n1 / wd), sh[t], sc.inv_act);
+    for (int c = 0; c < 8; c++) {
+      const float w = (wxs[(c >> 2) & 1] * wys[(c >> 1) & 1]) * wzs[c & 1];
+      cw[c] = w * dis[c].y;
+      cd[c] = w * dis[c].x;
+      float lv[8];
+      if (LAT8) {
+        fp8x8_decode(lat[c].x, lat[c].y, scl[c], lv);
+      } else {
+        const uint32_t lw[4] = {lat[c].x, lat[c].y, lat[c].z, lat[c].w};
+#pragma unroll
+        for (int k = 0; k < 8; k++) lv[k] = h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) cl[k][c] = w * lv[k];
+    }
+    auto tree = [](const float* v) { return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])); };
+    const float wd = tree(cw), dd = tree(cd);
+    _Float16 L[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) L[k] = (_Float16)(tree(cl[k]) / wd);
+    const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
+    auto pk = [](_Float16 a, _Float16 b) {
+      return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+    };
+    const _Float16 h1 = (_Float16)1.0f;
+    // operand groups: base-output rows 4g..4g+3 (rows 1..8 = latents, the rest
+    // the padding 1.0) and SH coefficients 4g..4g+3
+    uint4* r5 = recw + lane * HEAD_REC;
+    r5[0] = make_uint4(pk(h1, L[0]), pk(L[1], L[2]), sh_lo.x, sh_lo.y);
+    r5[1] = make_uint4(pk(L[3], L[4]), pk(L[5], L[6]), sh_lo.z, sh_lo.w);
+    r5[2] = make_uint4(pk(L[7], h1), one2, sh_hi.x, sh_hi.y);
+    r5[3] = make_uint4(one2, one2, sh_hi.z, sh_hi.w);
+    r5[4] = make_uint4((uint32_t)__builtin_bit_cast(uint16_t, logsig), 0u, 0u, 0u);
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const uint4* rt = recw + (16 * t + col) * HEAD_REC;
+      const uint4 b = rt[g];
+      T x;
+      if constexpr (P == FVC_PREC_FP16) {
+        x = __builtin_bit_cast(T, b);
+      } else {
+        const uint32_t w4[4] = {b.x, b.y, b.z, b.w};
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = h2f((uint16_t)(w4[k >> 1] >> (16 * (k & 1))));
+        x = Mx<P>::pack(v, sc.inv_act);
+      }
       const f4 o = head_mlp<P>(wl, wl + 4 * 64, li, x, sc);
-      if (g == 0 && valid[t]) {
-        const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
+      const int st = grp * 64 + 16 * t + col;
+      if (g == 0 && st < n32) {
         uint2 v;
         v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[0]) |
               ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[1]) << 16);
-        v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[2]) |
-              ((uint32_t)__builtin_bit_cast(uint16_t, logsig) << 16);
-        c_out[s[t]] = v;
+        v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o[2]) | (rt[4].x << 16);
+        c_out[st] = v;
       }
     }
+    wave_sync();
   }
 }
 
@@ -1024,16 +1104,13 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
                           ctl + CTL_R, r.mlp_rows);
       h.net = P.end(ev, s, "network_resample");
       ev = P.begin(s);
-#define L3(P, TL, F8)                                                                                                \
-  hipLaunchKernelGGL((k_cache_head<P, TL, F8>), dim3(head_blocks), dim3(256), 0, s, r.c_rec.as<float4>(),            \
-                     r.ray_sh.as<uint2>(), (int64_t)n_bound, cv, m.frags.as<typename Mx<P>::T>(),                     \
-                     m.mlp_scales, r.c_out.as<uint2>(), ctl + CTL_C)
-#define L2(P, TL)                      \
-  if (cv.lat8) { L3(P, TL, true); }    \
-  else { L3(P, TL, false); }
-#define L(P)                        \
-  if (r.mlp_rows == 32) { L2(P, 2); } \
-  else { L2(P, 1); }
+#define L2(P, F8)                                                                                                    \
+  hipLaunchKernelGGL((k_cache_head<P, F8>), dim3(head_blocks), dim3(256), 0, s, r.c_rec.as<float4>(),                 \
+                     r.ray_sh.as<uint4>(), (int64_t)n_bound, cv, m.frags.as<typename Mx<P>::T>(), m.mlp_scales,       \
+                     r.c_out.as<uint2>(), ctl + CTL_C)
+#define L(P)                     \
+  if (cv.lat8) { L2(P, true); }  \
+  else { L2(P, false); }
 #define LVD(TL)                                                                                                   \
   hipLaunchKernelGGL((k_cache_head_vd<TL>), dim3(head_blocks), dim3(256), 0, s, r.c_rec.as<float4>(),                \
                      r.c_t0.as<float>(), r.ray_sh.as<uint2>(), r.ray_od.as<float4>(),                                 \
@@ -1051,7 +1128,6 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
 #undef LVD
 #undef L
 #undef L2
-#undef L3
       FVC_HIP(hipGetLastError());
       h.head = P.end(ev, s, "cache_head");
       ev = P.begin(s);

@@ -428,11 +428,6 @@ enum {
 };
 static const int STG = 20;  // LDS row stride of the transposed staging (floats)
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // dW[n_out][n_in] += sum over the tile's 16 samples of dY[row][s] X[col][s]:
 // dY as NBY D blocks (rows 16 nb + 4 g + e), X as MX slots with column

@@ -112,4 +112,11 @@ __device__ __forceinline__ int wave_min(int v) {
   return v;
 }
 
+// LDS written by lanes of a wave, read by other lanes of the same wave
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 }  // namespace fvc
