@@ -674,16 +674,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
   }
 }
 
-// Trilinear lookup (generateNetworkInput_cache_shfl_perf_kernel_cg,
-// cache_performance_renderer.cu:678-787) fused with the colour MLP.  Lane
-// (sample l%16, group g) fetches the corners (x, y) = (g >> 1, g & 1), z = 0
-// and 1 (corners 2g and 2g+1 with corner bits x=4, y=2, z=1).  Corners
-// outside the frustum or in unset bricks read a zero voxel.  The latent sums
-// are reduce-scattered over the 4 lanes of a sample (row swaps over lane^16,
-// then lane^32), which reproduces the reference's
-// ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) summation order and leaves lane g with
-// latents (g&1) + 4(g>>1) and that + 2, the inputs it feeds to the colour
-// MLP (fragments FRAG_H1C).
+// Corners outside the frustum or in unset bricks read this zero voxel.
 __device__ Voxel k_zero_voxel;  // zero-initialised, never written
 
 // 8 E4M3 codes (latent i in byte i % 4 of word i / 4) times the brick scale
@@ -693,87 +684,70 @@ __device__ __forceinline__ void fp8x8_decode(uint32_t w0, uint32_t w1, float s, 
   out[4] = fp8_byte<0>(w1) * s; out[5] = fp8_byte<1>(w1) * s; out[6] = fp8_byte<2>(w1) * s; out[7] = fp8_byte<3>(w1) * s;
 }
 
-// The two corners of a lane's lookup: (x, y) = (ox, oy), z = 0 and 1 of
-// the cell of fc = froxel - 1/2; their weights, latent words, (density,
-// isset) and the brick scale (FP8 latents).
-struct CornerPair {
-  float w[2], scl[2];
-  uint4 lat[2];
-  float2 di[2];
-};
+// The trilinear lookup of one cache sample by one lane: its 8 corners
+// (corner c = 4x + 2y + z; outside the frustum or in unset bricks the zero
+// voxel) summed in the reference's order ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7))
+// (cache_performance_renderer.cu:736-787): wd = sum w*isset, dd = sum
+// w*sigma, sums[k] = sum w*l_k.  rec = (froxel x, y, z, -).
 template <bool LAT8>
-__device__ __forceinline__ void lookup_corners(const CacheView& cv, float fcx, float fcy, float fcz, float ox, float oy,
-                                               float fx, float fy, float fz, CornerPair& c) {
+__device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 rec, float& wd, float& dd,
+                                              float* sums) {
+  const float fcx = rec.x - 0.5f, fcy = rec.y - 0.5f, fcz = rec.z - 0.5f;
   const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
   const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
-  const float px = ix + ox, py = iy + oy;
-  const float wx = (1.0f - ox) * (1.0f - 2.0f * lx) + lx;
-  const float wy = (1.0f - oy) * (1.0f - 2.0f * ly) + ly;
-  const float wxy = wx * wy;
-  const bool in_xy = px >= 0.0f && px < fx && py >= 0.0f && py < fy;
-  const int vx = in_xy ? (int)px : 0, vy = in_xy ? (int)py : 0;
-  const int col_xy = (vy >> 3) * cv.bx + (vx >> 3), loc_xy = (vy & 7) * 8 + (vx & 7);
+  // weight factors (1 - off)(1 - 2l) + l of offsets 0 and 1
+  const float wxs[2] = {(1.0f - 2.0f * lx) + lx, lx}, wys[2] = {(1.0f - 2.0f * ly) + ly, ly};
+  const float wzs[2] = {(1.0f - 2.0f * lz) + lz, lz};
+  uint4 lat[8];
+  float2 dis[8];
+  float scl[8];
+  // integer cell of the corner (0, 0, 0); corner coordinates are exact
+  // small integers, so the frustum test is an unsigned compare
+  const int ixi = (int)ix, iyi = (int)iy, izi = (int)iz;
 #pragma unroll
-  for (int cc = 0; cc < 2; cc++) {
-    const float pz = iz + (float)cc;
-    const float wz = cc == 0 ? (1.0f - 2.0f * lz) + lz : lz;
-    c.w[cc] = wxy * wz;
-    const bool inside = in_xy && pz >= 0.0f && pz < fz;
-    const int vz = inside ? (int)pz : 0;
-    const int slot = cv.table[(vz >> 3) * cv.by * cv.bx + col_xy];
+  for (int c = 0; c < 8; c++) {
+    const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
+    const uint32_t cx = (uint32_t)(ixi + ox), cy = (uint32_t)(iyi + oy), cz = (uint32_t)(izi + oz);
+    const bool inside = cx < (uint32_t)cv.fx && cy < (uint32_t)cv.fy && cz < (uint32_t)cv.fz;
+    const uint32_t vx = inside ? cx : 0u, vy = inside ? cy : 0u, vz = inside ? cz : 0u;
+    const int slot = cv.table[((vz >> 3) * (uint32_t)cv.by + (vy >> 3)) * (uint32_t)cv.bx + (vx >> 3)];
     const bool ok = inside && slot >= 0;
-    const size_t vi = (size_t)slot * BRICK_VOXELS + (uint32_t)((vz & 7) * 64 + loc_xy);
+    const size_t vi = (size_t)(uint32_t)slot * BRICK_VOXELS + ((vz & 7u) * 64u + (vy & 7u) * 8u + (vx & 7u));
     if (LAT8) {
       const uint4 q = *reinterpret_cast<const uint4*>(ok ? (const void*)(cv.pool8 + vi) : (const void*)&k_zero_voxel);
-      c.lat[cc] = make_uint4(q.x, q.y, 0u, 0u);
-      c.di[cc] = make_float2(__builtin_bit_cast(float, q.z), __builtin_bit_cast(float, q.w));
-      c.scl[cc] = ok ? cv.scales[slot] : 0.0f;
+      lat[c] = make_uint4(q.x, q.y, 0u, 0u);
+      dis[c] = make_float2(__builtin_bit_cast(float, q.z), __builtin_bit_cast(float, q.w));
+      scl[c] = ok ? cv.scales[(uint32_t)slot] : 0.0f;
     } else {
       const Voxel* vp = ok ? cv.pool + vi : &k_zero_voxel;
-      c.lat[cc] = *reinterpret_cast<const uint4*>(vp->lat);
-      c.di[cc] = *reinterpret_cast<const float2*>(&vp->density);
-      c.scl[cc] = 0.0f;
+      lat[c] = *reinterpret_cast<const uint4*>(vp->lat);
+      dis[c] = *reinterpret_cast<const float2*>(&vp->density);
+      scl[c] = 0.0f;
     }
   }
-}
-
-// Weighted sums over the 8 corners of a sample (its 4 lanes): wd = sum w
-// isset and dd = sum w sigma on every lane, n0 / n1 = the sums of latents
-// (g&1) + 4(g>>1) and that + 2 (the reduce-scatter, see k_cache_head).
-template <bool LAT8>
-__device__ __forceinline__ void lookup_sums(const CornerPair& c, float& wd, float& dd, float& n0, float& n1) {
-  float ld[8];
+  // per-corner products, then the reference's summation tree
+  float cw[8], cd[8], cl[8][8];
 #pragma unroll
-  for (int cc = 0; cc < 2; cc++) {
-    const uint32_t lw[4] = {c.lat[cc].x, c.lat[cc].y, c.lat[cc].z, c.lat[cc].w};
-    const float ww = c.w[cc];
-    const float cw = ww * c.di[cc].y, cd = ww * c.di[cc].x;
+  for (int c = 0; c < 8; c++) {
+    const float w = (wxs[(c >> 2) & 1] * wys[(c >> 1) & 1]) * wzs[c & 1];
+    cw[c] = w * dis[c].y;
+    cd[c] = w * dis[c].x;
     float lv[8];
     if (LAT8) {
-      fp8x8_decode(lw[0], lw[1], c.scl[cc], lv);
+      fp8x8_decode(lat[c].x, lat[c].y, scl[c], lv);
     } else {
+      const uint32_t lw[4] = {lat[c].x, lat[c].y, lat[c].z, lat[c].w};
 #pragma unroll
       for (int k = 0; k < 8; k++) lv[k] = h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
     }
-    if (cc == 0) {
-      wd = cw;
-      dd = cd;
 #pragma unroll
-      for (int k = 0; k < 8; k++) ld[k] = ww * lv[k];
-    } else {
-      wd = wd + cw;
-      dd = dd + cd;
-#pragma unroll
-      for (int k = 0; k < 8; k++) ld[k] = ld[k] + ww * lv[k];
-    }
+    for (int k = 0; k < 8; k++) cl[k][c] = w * lv[k];
   }
-  wd = xor32_sum(xor16_sum(wd));
-  dd = xor32_sum(xor16_sum(dd));
-  // reduce-scatter: lane^16 pairs (0,1) (2,3) (4,5) (6,7), then lane^32
-  const float m0 = swap16_sum(ld[0], ld[1]), m1 = swap16_sum(ld[2], ld[3]);
-  const float m2 = swap16_sum(ld[4], ld[5]), m3 = swap16_sum(ld[6], ld[7]);
-  n0 = swap32_sum(m0, m2);
-  n1 = swap32_sum(m1, m3);
+  auto tree = [](const float* v) { return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])); };
+  wd = tree(cw);
+  dd = tree(cd);
+#pragma unroll
+  for (int k = 0; k < 8; k++) sums[k] = tree(cl[k]);
 }
 
 // One lane per sample: a wave takes 64 cache samples.  Each lane fetches
@@ -815,62 +789,11 @@ __global__ __launch_bounds__(256) void k_cache_head(const float4* __restrict__ c
     const float4 rec = c_rec[sl];
     const int px = __builtin_bit_cast(int, rec.w);
     const uint4 sh_lo = ray_sh[2 * (int64_t)px], sh_hi = ray_sh[2 * (int64_t)px + 1];
-    const float fcx = rec.x - 0.5f, fcy = rec.y - 0.5f, fcz = rec.z - 0.5f;
-    const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
-    const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
-    // weight factors (1 - off)(1 - 2l) + l of offsets 0 and 1
-    const float wxs[2] = {(1.0f - 2.0f * lx) + lx, lx}, wys[2] = {(1.0f - 2.0f * ly) + ly, ly};
-    const float wzs[2] = {(1.0f - 2.0f * lz) + lz, lz};
-    uint4 lat[8];
-    float2 dis[8];
-    float scl[8];
-    // integer cell of the corner (0, 0, 0); corner coordinates are exact
-    // small integers, so the frustum test is an unsigned compare
-    const int ixi = (int)ix, iyi = (int)iy, izi = (int)iz;
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
-      const uint32_t cx = (uint32_t)(ixi + ox), cy = (uint32_t)(iyi + oy), cz = (uint32_t)(izi + oz);
-      const bool inside = cx < (uint32_t)cv.fx && cy < (uint32_t)cv.fy && cz < (uint32_t)cv.fz;
-      const uint32_t vx = inside ? cx : 0u, vy = inside ? cy : 0u, vz = inside ? cz : 0u;
-      const int slot = cv.table[((vz >> 3) * (uint32_t)cv.by + (vy >> 3)) * (uint32_t)cv.bx + (vx >> 3)];
-      const bool ok = inside && slot >= 0;
-      const size_t vi = (size_t)(uint32_t)slot * BRICK_VOXELS + ((vz & 7u) * 64u + (vy & 7u) * 8u + (vx & 7u));
-      if (LAT8) {
-        const uint4 q = *reinterpret_cast<const uint4*>(ok ? (const void*)(cv.pool8 + vi) : (const void*)&k_zero_voxel);
-        lat[c] = make_uint4(q.x, q.y, 0u, 0u);
-        dis[c] = make_float2(__builtin_bit_cast(float, q.z), __builtin_bit_cast(float, q.w));
-        scl[c] = ok ? cv.scales[(uint32_t)slot] : 0.0f;
-      } else {
-        const Voxel* vp = ok ? cv.pool + vi : &k_zero_voxel;
-        lat[c] = *reinterpret_cast<const uint4*>(vp->lat);
-        dis[c] = *reinterpret_cast<const float2*>(&vp->density);
-        scl[c] = 0.0f;
-      }
-    }
-    // per-corner products, then the reference's summation tree
-    float cw[8], cd[8], cl[8][8];
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const float w = (wxs[(c >> 2) & 1] * wys[(c >> 1) & 1]) * wzs[c & 1];
-      cw[c] = w * dis[c].y;
-      cd[c] = w * dis[c].x;
-      float lv[8];
-      if (LAT8) {
-        fp8x8_decode(lat[c].x, lat[c].y, scl[c], lv);
-      } else {
-        const uint32_t lw[4] = {lat[c].x, lat[c].y, lat[c].z, lat[c].w};
-#pragma unroll
-        for (int k = 0; k < 8; k++) lv[k] = h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
-      }
-#pragma unroll
-      for (int k = 0; k < 8; k++) cl[k][c] = w * lv[k];
-    }
-    auto tree = [](const float* v) { return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])); };
-    const float wd = tree(cw), dd = tree(cd);
+    float wd, dd, sums[8];
+    lookup_sample<LAT8>(cv, rec, wd, dd, sums);
     _Float16 L[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) L[k] = (_Float16)(tree(cl[k]) / wd);
+    for (int k = 0; k < 8; k++) L[k] = (_Float16)(sums[k] / wd);
     const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
     auto pk = [](_Float16 a, _Float16 b) {
       return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
@@ -913,87 +836,101 @@ __global__ __launch_bounds__(256) void k_cache_head(const float4* __restrict__ c
   }
 }
 
-// Split models: the lookup (FP16 latents) with the custom colour head
-// (generateNetworkInput_cache_padded_kernel / _shfl_perf_kernel_cg +
-// inferenceHead, cache_performance_renderer.cu:323-410, 678-787).  Per
-// sample the midpoint t0 + dt/2 on its ray gives the position of the
-// frequency encoding and the initial view direction (normalize(point - cache
-// camera position), 292-312).  di (DensityIntermediates): latent 0 holds the
-// density, so latent 0 = log(sum w l_0) * W / W with W = sum w isset
-// (770-779, no clamp) and it is also the sample's log density.
-template <int TILES>
+// Split models: the lookup of k_cache_head (FP16 latents, one lane per
+// sample) with the custom colour head (generateNetworkInput_cache_padded_kernel
+// / _shfl_perf_kernel_cg + inferenceHead, cache_performance_renderer.cu:
+// 323-410, 678-787).  Per sample the midpoint t0 + dt/2 on its pixel's ray
+// gives the position of the frequency encoding and the initial view
+// direction (normalize(point - cache camera position), 292-312).  di
+// (DensityIntermediates): latent 0 holds the density, so latent 0 =
+// log(sum w l_0) * W / W with W = sum w isset (770-779, no clamp) and it is
+// also the sample's log density.  The 64 head inputs go to LDS as the two
+// operand blocks (VD_HA, VD_HB order) of each lane group.
+static const int HEAD_REC_VD = 9;  // uint4 per sample record: block A (4 groups), block B (4 groups), log sigma
+
 __global__ __launch_bounds__(256) void k_cache_head_vd(const float4* __restrict__ c_rec, const float* __restrict__ c_t0,
-                                                       const uint2* __restrict__ ray_sh,
+                                                       const uint4* __restrict__ ray_sh,
                                                        const float4* __restrict__ ray_od, int64_t n, CacheView cv,
                                                        fvc_stepsize_info st, ContractParams con,
                                                        const h8* __restrict__ frags, int di, int freq_on,
                                                        uint2* __restrict__ c_out, const int* __restrict__ n_dev) {
   if (n_dev) n = *n_dev;
-  if ((int64_t)blockIdx.x * 4 * 16 * TILES >= n) return;
-  __shared__ h8 wl[10 * 64];  // VD_HB (4), VD_HO (2), VD_HAC (4)
-  stage_frags(frags + VD_HB * 64, wl, 10);
+  if ((int64_t)blockIdx.x * 256 >= n) return;
+  __shared__ h8 wl[10 * 64];  // VD_HA (4), VD_HB (4), VD_HO (2)
+  __shared__ uint4 rec_lds[4][64 * HEAD_REC_VD];
+  stage_frags(frags + VD_HA * 64, wl, 10);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
   const int n32 = (int)n;
-  const int n_groups = (n32 + 16 * TILES - 1) / (16 * TILES);
-  const float ox = (float)(g >> 1), oy = (float)(g & 1);
-  const float fx = (float)cv.fx, fy = (float)cv.fy, fz = (float)cv.fz;
+  const int n_groups = (n32 + 63) / 64;
   const V3 cam_pos = {cv.cam.cam2world.m0.w, cv.cam.cam2world.m1.w, cv.cam.cam2world.m2.w};
+  uint4* recw = rec_lds[wave];
+  typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  auto pk = [](_Float16 a, _Float16 b) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+  };
   for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
     const int li = opaque(lane);
-    int s[TILES], ray[TILES];
-    bool valid[TILES];
-    CornerPair cp[TILES];
-    uint2 sh[TILES];
-    float t0[TILES];
+    const int s = grp * 64 + lane;
+    const int sl = s < n32 ? s : n32 - 1;
+    const float4 rec = c_rec[sl];
+    const int px = __builtin_bit_cast(int, rec.w);
+    const float t0 = c_t0[sl];
+    const uint4 sh_lo = ray_sh[2 * (int64_t)px], sh_hi = ray_sh[2 * (int64_t)px + 1];
+    const float4 ro = ray_od[2 * (int64_t)px], rd = ray_od[2 * (int64_t)px + 1];
+    float wd, dd, sums[8];
+    lookup_sample<false>(cv, rec, wd, dd, sums);
+    _Float16 L[8];
 #pragma unroll
-    for (int t = 0; t < TILES; t++) {
-      s[t] = (grp * TILES + t) * 16 + col;
-      valid[t] = s[t] < n32;
-      const int sl = valid[t] ? s[t] : n32 - 1;
-      const float4 rec = c_rec[sl];
-      ray[t] = __builtin_bit_cast(int, rec.w);
-      t0[t] = c_t0[sl];
-      sh[t] = ray_sh[ray[t] * 4 + g];
-      lookup_corners<false>(cv, rec.x - 0.5f, rec.y - 0.5f, rec.z - 0.5f, ox, oy, fx, fy, fz, cp[t]);
-    }
+    for (int k = 0; k < 8; k++) L[k] = (_Float16)(sums[k] / wd);
+    if (di) L[0] = (_Float16)((logf(sums[0]) * wd) / wd);
+    const _Float16 logsig = di ? L[0] : (_Float16)logf(fmaxf_(dd, 1e-6f));
+    // the sample's midpoint (generateNetworkInput_preparation_perf_kernel, 285-291)
+    const V3 o = {ro.x, ro.y, ro.z}, d = {rd.x, rd.y, rd.z};
+    const float tm = t0 + step_dt(st, t0) * 0.5f;
+    const V3 wp = o + tm * d;
+    const V3 p = contract(wp, con);
+    const V3 i01 = unit_to_01(normalize(wp - cam_pos));
+    uint4 ilo, ihi;
+    sh4_all(i01.x, i01.y, i01.z, ilo, ihi);
+    // SH differences: the half coefficients subtracted in half (shCoeffDiffChunk4)
+    const uint32_t dw[8] = {sh_lo.x, sh_lo.y, sh_lo.z, sh_lo.w, sh_hi.x, sh_hi.y, sh_hi.z, sh_hi.w};
+    const uint32_t iw[8] = {ilo.x, ilo.y, ilo.z, ilo.w, ihi.x, ihi.y, ihi.z, ihi.w};
+    uint32_t df[8];
 #pragma unroll
-    for (int t = 0; t < TILES; t++) {
-      float wd, dd, n0, n1;
-      lookup_sums<false>(cp[t], wd, dd, n0, n1);
-      float l0 = n0 / wd;
-      if (di && g == 0) l0 = (logf(n0) * wd) / wd;
-      const _Float16 lat0 = (_Float16)l0, lat1 = (_Float16)(n1 / wd);
-      // the sample's midpoint (generateNetworkInput_preparation_perf_kernel, 285-291)
-      const float4 ro = ray_od[2 * (int64_t)ray[t]], rd = ray_od[2 * (int64_t)ray[t] + 1];
-      const V3 o = {ro.x, ro.y, ro.z}, d = {rd.x, rd.y, rd.z};
-      const float tm = t0[t] + step_dt(st, t0[t]) * 0.5f;
-      const V3 wp = o + tm * d;
-      const V3 p = contract(wp, con);
-      const V3 i01 = unit_to_01(normalize(wp - cam_pos));
-      _Float16 shi[4], dsh[4], shd[4], fa[4], fb[4];
-      sh4_quarter(i01.x, i01.y, i01.z, g, shi);
-      shd[0] = __builtin_bit_cast(_Float16, (uint16_t)(sh[t].x & 0xffffu));
-      shd[1] = __builtin_bit_cast(_Float16, (uint16_t)(sh[t].x >> 16));
-      shd[2] = __builtin_bit_cast(_Float16, (uint16_t)(sh[t].y & 0xffffu));
-      shd[3] = __builtin_bit_cast(_Float16, (uint16_t)(sh[t].y >> 16));
+    for (int k = 0; k < 8; k++)
+      df[k] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(h2, dw[k]) - __builtin_bit_cast(h2, iw[k]));
+    _Float16 fq[6][4];
 #pragma unroll
-      for (int e = 0; e < 4; e++) dsh[e] = (_Float16)((float)shd[e] - (float)shi[e]);
-      freq_quad(p, 4 + (g >> 1), freq_on != 0, fa);
-      freq_quad(p, g, freq_on != 0, fb);
-      const _Float16 la[4] = {lat0, lat1, fa[2 * (g & 1)], fa[2 * (g & 1) + 1]};
-      const f4 o4 = head_vd_mlp(wl + 6 * 64, wl, wl + 4 * 64, li, h8_of(la, shd), h8_of(dsh, fb));
-      if (g == 0 && valid[t]) {
-        const _Float16 logsig = di ? lat0 : (_Float16)logf(fmaxf_(dd, 1e-6f));
+    for (int q = 0; q < 6; q++) freq_quad(p, q, freq_on != 0, fq[q]);
+    uint4* r9 = recw + lane * HEAD_REC_VD;
+    // block A: latents 4g..4g+3 (g < 2) or frequency quad g + 2 (g >= 2); SH 4g..4g+3
+    r9[0] = make_uint4(pk(L[0], L[1]), pk(L[2], L[3]), dw[0], dw[1]);
+    r9[1] = make_uint4(pk(L[4], L[5]), pk(L[6], L[7]), dw[2], dw[3]);
+    r9[2] = make_uint4(pk(fq[4][0], fq[4][1]), pk(fq[4][2], fq[4][3]), dw[4], dw[5]);
+    r9[3] = make_uint4(pk(fq[5][0], fq[5][1]), pk(fq[5][2], fq[5][3]), dw[6], dw[7]);
+    // block B: SH differences 4g..4g+3, frequency quad g
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      r9[4 + q] = make_uint4(df[2 * q], df[2 * q + 1], pk(fq[q][0], fq[q][1]), pk(fq[q][2], fq[q][3]));
+    r9[8] = make_uint4((uint32_t)__builtin_bit_cast(uint16_t, logsig), 0u, 0u, 0u);
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const uint4* rt = recw + (16 * t + col) * HEAD_REC_VD;
+      const h8 xa = __builtin_bit_cast(h8, rt[g]), xb = __builtin_bit_cast(h8, rt[4 + g]);
+      const f4 o4 = head_vd_mlp(wl, wl + 4 * 64, wl + 8 * 64, li, xa, xb);
+      const int st2 = grp * 64 + 16 * t + col;
+      if (g == 0 && st2 < n32) {
         uint2 v;
         v.x = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o4[0]) |
               ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o4[1]) << 16);
-        v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o4[2]) |
-              ((uint32_t)__builtin_bit_cast(uint16_t, logsig) << 16);
-        c_out[s[t]] = v;
+        v.y = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)o4[2]) | (rt[8].x << 16);
+        c_out[st2] = v;
       }
     }
+    wave_sync();
   }
 }
 
@@ -1111,13 +1048,11 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
 #define L(P)                     \
   if (cv.lat8) { L2(P, true); }  \
   else { L2(P, false); }
-#define LVD(TL)                                                                                                   \
-  hipLaunchKernelGGL((k_cache_head_vd<TL>), dim3(head_blocks), dim3(256), 0, s, r.c_rec.as<float4>(),                \
-                     r.c_t0.as<float>(), r.ray_sh.as<uint2>(), r.ray_od.as<float4>(),                                 \
-                     (int64_t)n_bound, cv, rm.stepsize_info, m.con, m.frags.as<h8>(), m.net.first_latent_is_density, \
-                     m.net.freq_encoding_degree > 0, r.c_out.as<uint2>(), ctl + CTL_C)
       if (m.net.split_head) {
-        if (r.mlp_rows == 32) { LVD(2); } else { LVD(1); }
+        hipLaunchKernelGGL(k_cache_head_vd, dim3(head_blocks), dim3(256), 0, s, r.c_rec.as<float4>(),
+                           r.c_t0.as<float>(), r.ray_sh.as<uint4>(), r.ray_od.as<float4>(), (int64_t)n_bound, cv,
+                           rm.stepsize_info, m.con, m.frags.as<h8>(), m.net.first_latent_is_density,
+                           m.net.freq_encoding_degree > 0, r.c_out.as<uint2>(), ctl + CTL_C);
       } else {
         switch (m.net.precision) {
           case FVC_PREC_BF16: L(FVC_PREC_BF16); break;
@@ -1125,7 +1060,6 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
           default: L(FVC_PREC_FP16); break;
         }
       }
-#undef LVD
 #undef L
 #undef L2
       FVC_HIP(hipGetLastError());

@@ -76,22 +76,18 @@ struct GridParams {
 
 // Weight fragments: 20 MFMA A-operands of 64 lanes x 8 halves, in the order
 // W1[nb=0..3], W2[nb=0..3][kb=0..1], W3[kb=0..1], H1[nb=0..3], H2[kb=0..1].
-static const int N_FRAGS = 24;
-// FRAG_H1C: the colour MLP's first layer again, in the input order of the
-// cache lookup (head_input_cache); the network kernels stage the first
-// N_NET_FRAGS fragments
-static const int FRAG_W1 = 0, FRAG_W2 = 4, FRAG_W3 = 12, FRAG_H1 = 14, FRAG_H2 = 18, FRAG_H1C = 20;
+static const int N_FRAGS = 20;
+static const int FRAG_W1 = 0, FRAG_W2 = 4, FRAG_W3 = 12, FRAG_H1 = 14, FRAG_H2 = 18;
 static const int N_NET_FRAGS = 20;
 // Split (view-dependent) models, FP16 only: the density MLP as above, then
 // mlp_first_head VD_F1[nb] (32 inputs), VD_F2[nb][kb] (second hidden layer,
 // if any), VD_F3[kb] (output rows = the 8 head latents in order); the
 // custom colour head's first layer over two 32-input blocks, VD_HA[nb]
 // (latents / position frequencies + SH of the view direction) and
-// VD_HB[nb] (SH differences + frequencies), its output layer VD_HO[kb], and
-// VD_HAC[nb], block A in the input order of the cache lookup.  The network
-// kernels stage the first N_VD_NET_FRAGS, the cache head VD_HB..VD_HAC+3.
-static const int VD_F1 = 14, VD_F2 = 18, VD_F3 = 26, VD_HA = 28, VD_HB = 32, VD_HO = 36, VD_HAC = 38;
-static const int N_VD_FRAGS = 42, N_VD_NET_FRAGS = 38;
+// VD_HB[nb] (SH differences + frequencies), and its output layer VD_HO[kb].
+// The network kernels stage all of them, the head kernels VD_HA..VD_HO+1.
+static const int VD_F1 = 14, VD_F2 = 18, VD_F3 = 26, VD_HA = 28, VD_HB = 32, VD_HO = 36;
+static const int N_VD_FRAGS = 38, N_VD_NET_FRAGS = 38;
 
 // per-layer accumulator scales of the MLPs (FP8: s_w[l] * s_a; otherwise 1)
 // and 1 / s_a (mlp_mfma.h)

@@ -211,30 +211,6 @@ __device__ __forceinline__ typename Mx<P>::T head_input(const _Float16* rows4, i
   return Mx<P>::pack(v, inv_act);
 }
 
-// Head input fragment of the cache lookup (fragments FRAG_H1C): slots 0, 1
-// carry latents (g & 1) + 4 (g >> 1) and that + 2, slots 2, 3 two of the
-// constant 1.0 padding inputs, slots 4..7 SH coefficients 4g..4g+3 (packed
-// halves in sh).
-template <int P>
-__device__ __forceinline__ typename Mx<P>::T head_input_cache(_Float16 l0, _Float16 l1, uint2 sh, float inv_act) {
-  if constexpr (P == FVC_PREC_FP16) {
-    const uint32_t w0 = (uint32_t)__builtin_bit_cast(uint16_t, l0) | ((uint32_t)__builtin_bit_cast(uint16_t, l1) << 16);
-    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
-    const u4v w = {w0, 0x3c003c00u, sh.x, sh.y};
-    return __builtin_bit_cast(h8, w);
-  } else {
-    const float v[8] = {(float)l0,
-                        (float)l1,
-                        1.0f,
-                        1.0f,
-                        (float)__builtin_bit_cast(_Float16, (uint16_t)(sh.x & 0xffffu)),
-                        (float)__builtin_bit_cast(_Float16, (uint16_t)(sh.x >> 16)),
-                        (float)__builtin_bit_cast(_Float16, (uint16_t)(sh.y & 0xffffu)),
-                        (float)__builtin_bit_cast(_Float16, (uint16_t)(sh.y >> 16))};
-    return Mx<P>::pack(v, inv_act);
-  }
-}
-
 // ------------------------------------------------------------ split models
 // (FP16 operands, FP32 accumulation; fragment indices VD_* in fvc_internal.h)
 typedef Mx<FVC_PREC_FP16> MxH;

@@ -498,11 +498,6 @@ static int col_head_in(int g, int e) {                                          
   if (r >= 1 && r <= 8) return r - 1;      // latent r - 1
   return r == 0 ? 24 : r + 16;             // padding columns 24..31
 }
-static int col_head_cache(int g, int e) {                                                      // H1C
-  if (e >= 4) return 8 + 4 * g + (e - 4);         // SH coefficient 4g + e - 4
-  if (e < 2) return (g & 1) + 4 * (g >> 1) + 2 * e;  // latents of the reduce-scatter
-  return 24 + 2 * g + (e - 2);                    // padding columns 24..31
-}
 
 // host conversions for the weight fragments
 static float f16_value(uint16_t h) {
@@ -584,7 +579,6 @@ void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, s
   for (int kb = 0; kb < 2; kb++) put(FRAG_W3 + kb, 2, W3, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
   for (int nb = 0; nb < 4; nb++) put(FRAG_H1 + nb, 3, H1, 32, nb * 16, col_head_in);
   for (int kb = 0; kb < 2; kb++) put(FRAG_H2 + kb, 4, H2, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
-  for (int nb = 0; nb < 4; nb++) put(FRAG_H1C + nb, 3, H1, 32, nb * 16, col_head_cache);
 }
 
 // Split models.  Custom head input columns (my_fully_fused_mlp.cu:738-790):
@@ -605,13 +599,6 @@ static int col_vd_a(const fvc_net_desc& d, int g, int e) {
 }
 // block B: SH differences 4g..4g+3, frequency quad g
 static int col_vd_b(const fvc_net_desc& d, int g, int e) { return e < 4 ? 24 + 4 * g + e : col_freq(d, g, e - 4); }
-// block A of the cache lookup: the two latents of the reduce-scatter, two
-// values of frequency quad 4 + g/2, SH coefficients 4g..4g+3
-static int col_vd_a_cache(const fvc_net_desc& d, int g, int e) {
-  if (e >= 4) return 8 + 4 * g + (e - 4);
-  if (e < 2) return (g & 1) + 4 * (g >> 1) + 2 * e;
-  return col_freq(d, 4 + (g >> 1), 2 * (g & 1) + (e - 2));
-}
 // mlp_first_head input: density-MLP outputs 4g..4g+3, SH of the initial
 // view direction 4g..4g+3 (nerf_network.h:186-194)
 static int col_vd_first(int g, int e) { return e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4); }
@@ -658,7 +645,6 @@ void pack_fragments_vd(const uint16_t* base, const uint16_t* first, const uint16
   for (int nb = 0; nb < 4; nb++) {
     put(VD_HA + nb, head, HIN, rows(nb), [&d](int g, int e) { return col_vd_a(d, g, e); });
     put(VD_HB + nb, head, HIN, rows(nb), [&d](int g, int e) { return col_vd_b(d, g, e); });
-    put(VD_HAC + nb, head, HIN, rows(nb), [&d](int g, int e) { return col_vd_a_cache(d, g, e); });
   }
   for (int kb = 0; kb < 2; kb++) put(VD_HO + kb, HO, 64, ident, chain(kb));
 }

@@ -81,23 +81,6 @@ __device__ __forceinline__ float xor32_sum(float v) {
   return __builtin_bit_cast(float, (unsigned)p[0]) + __builtin_bit_cast(float, (unsigned)p[1]);
 }
 
-// Two pair sums with one row swap: swap16_sum(a, b) is a + a(lane ^ 16) on
-// even rows (lane / 16 even) and b + b(lane ^ 16) on odd rows; swap32_sum(a,
-// b) is a + a(lane ^ 32) on lanes 0..31 and b + b(lane ^ 32) on lanes 32..63.
-// (v_permlane16_swap: odd rows of the first operand trade places with even
-// rows of the second; v_permlane32_swap: upper half of the first with the
-// lower half of the second.)
-__device__ __forceinline__ float swap16_sum(float a, float b) {
-  const auto p = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
-                                                  false, false);
-  return __builtin_bit_cast(float, (unsigned)p[0]) + __builtin_bit_cast(float, (unsigned)p[1]);
-}
-__device__ __forceinline__ float swap32_sum(float a, float b) {
-  const auto p = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
-                                                  false, false);
-  return __builtin_bit_cast(float, (unsigned)p[0]) + __builtin_bit_cast(float, (unsigned)p[1]);
-}
-
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
