@@ -116,7 +116,7 @@ class RefillPolicy(C.Structure):
                 ("pad_", C.c_int32)]
 
 
-TUNE_MAX = 16
+TUNE_MAX = 24
 
 
 class TuneResult(C.Structure):

@@ -542,7 +542,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   int cnt = 0, bins = 0;
-  uint32_t res_mask = 0, cache_mask = 0;  // class of march sample k
+  uint64_t res_mask = 0, cache_mask = 0;  // class of march sample k
   bool alive = false;
   RayState r{};
   V3 oc = {0, 0, 0}, dc = {0, 0, 0};
@@ -618,12 +618,12 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
       for (int u = 0; u < CLS; u++) {
         if (k0 + u >= cnt) break;
         const bool unsampled = !in_fov[u] || tf[u] < cv.st.near_ || tf[u] > known[u];
-        if (!inside[u] || unsampled) res_mask |= 1u << (k0 + u);
-        else if (slot[u] >= 0 && dv[u] > 0.0f) cache_mask |= 1u << (k0 + u);
+        if (!inside[u] || unsampled) res_mask |= 1ull << (k0 + u);
+        else if (slot[u] >= 0 && dv[u] > 0.0f) cache_mask |= 1ull << (k0 + u);
       }
     }
   }
-  const int nr = __popc(res_mask), nc = __popc(cache_mask);
+  const int nr = __popcll(res_mask), nc = __popcll(cache_mask);
   int slot[3];
   block_reserve<3>({nr, nc, bins}, {cnt_r, cnt_c, step_counter}, slot);
   const int off_r = slot[0], off_c = slot[1];
@@ -632,9 +632,9 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
     const V3 d01 = unit_to_01(d);
     uint64_t order = 0;  // bit q set: the q-th kept sample is a cache sample
     int kr = 0, kc = 0;
-    const uint32_t kept = res_mask | cache_mask;
+    const uint64_t kept = res_mask | cache_mask;
     for (int k0 = 0; k0 < cnt; k0 += CLS_EMIT) {
-      if (((kept >> k0) & ((1u << CLS_EMIT) - 1u)) == 0u) continue;
+      if (((kept >> k0) & ((1ull << CLS_EMIT) - 1ull)) == 0ull) continue;
       float t0v[CLS_EMIT];
 #pragma unroll
       for (int u = 0; u < CLS_EMIT; u++) t0v[u] = t_scr[(int64_t)min(k0 + u, cnt - 1) * n_alive + i];
@@ -642,7 +642,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
       for (int u = 0; u < CLS_EMIT; u++) {
         const int k = k0 + u;
         if (k >= cnt) break;
-        const bool is_r = (res_mask >> k) & 1u, is_c = (cache_mask >> k) & 1u;
+        const bool is_r = (res_mask >> k) & 1ull, is_c = (cache_mask >> k) & 1ull;
         if (!is_r && !is_c) continue;
         const float t0 = t0v[u];
         const float tm2 = t0 + step_dt(mg.st, t0) * 0.5f;

@@ -318,7 +318,7 @@ int fvc_renderer_set_batching(fvc_renderer* h, int64_t target_samples, int32_t m
     REQUIRE(h, "null renderer");
     REQUIRE(target_samples >= 4096 && target_samples <= (int64_t)1 << 28, "target_samples out of range");
     REQUIRE(max_samples_per_ray >= 1 && max_samples_per_ray <= MAX_STEPS_PER_LAUNCH,
-            "max_samples_per_ray must be in [1, 32]");
+            "max_samples_per_ray must be in [1, 64]");
     h->r.target_batch = target_samples;
     h->r.max_steps = max_samples_per_ray;
     if (h->r.width > 0) resize_renderbuffers(h->r, h->r.width, h->r.height);

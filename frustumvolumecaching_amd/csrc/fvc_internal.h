@@ -282,7 +282,7 @@ inline double net_flop(const fvc_net_desc& d) {
 }
 
 // ---------------------------------------------------------------- renderer
-static const int MAX_STEPS_PER_LAUNCH = 32;   // cap of fvc_renderer_set_batching
+static const int MAX_STEPS_PER_LAUNCH = 64;   // cap of fvc_renderer_set_batching (64-bit sample masks)
 
 // the render cache as the kernels see it
 struct CacheView {

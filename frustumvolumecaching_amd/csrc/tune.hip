@@ -22,7 +22,7 @@ void autotune(Renderer& r, const fvc_raymarch_info& rm, bool cached, fvc_tune_re
   if (cached && !r.caches[r.render_idx].initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
   const int rows_c[2] = {16, 32};
   const int64_t target_c[3] = {(int64_t)1 << 23, (int64_t)1 << 24, (int64_t)1 << 25};
-  const int steps_c[2] = {16, 32};
+  const int steps_c[3] = {16, 32, 64};
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   const int rows0 = r.mlp_rows, steps0 = r.max_steps;
   const int64_t target0 = r.target_batch;

@@ -304,7 +304,7 @@ int fvc_renderer_set_stream(fvc_renderer* renderer, void* stream);
 /* Sub-iteration batching (the reference's TARGET_BATCH_SIZE = 2^21 and
  * MAX_N_STEPS_PER_SUBITER = 8, performance_renderer.h:54-58; here 2^25 and 32
  * by default): samples targeted per launch, and the cap on samples per ray
- * per launch (1..32).  Changes launch sizes only, never the image. */
+ * per launch (1..64).  Changes launch sizes only, never the image. */
 int fvc_renderer_set_batching(fvc_renderer* renderer, int64_t target_samples, int32_t max_samples_per_ray);
 /* Renderer::resizeRenderbuffers (renderer.h:112) */
 int fvc_resize_renderbuffers(fvc_renderer* renderer, int width, int height);
@@ -372,7 +372,7 @@ int fvc_cache_get_pool_decoded(fvc_renderer* renderer, int which, float* out_hos
 int fvc_cache_set_capacity(fvc_renderer* renderer, int32_t capacity_bricks);
 
 /* ---- autotuning (no counterpart in the reference) ---- */
-#define FVC_TUNE_MAX 16
+#define FVC_TUNE_MAX 24
 typedef struct {
   int32_t mlp_rows;               /* chosen MLP rows per wavefront: 16 or 32 */
   int32_t max_steps;              /* chosen samples per ray per launch */

@@ -101,7 +101,7 @@ def test_autotune_keeps_images(setup):
     res = r.autotune(rm, cached=False)
     print(f"[tune] rows {res.mlp_rows} target 2^{int(np.log2(res.target_batch))} steps {res.max_steps}: "
           f"{res.ms_best:.3f} ms (default {res.ms_default:.3f} ms, {res.n_tried} tried)", flush=True)
-    assert res.n_tried == 12 and res.mlp_rows in (16, 32) and res.max_steps in (16, 32)
+    assert res.n_tried == 18 and res.mlp_rows in (16, 32) and res.max_steps in (16, 32, 64)
     assert res.ms_best <= min(res.ms[:res.n_tried]) + 1e-6
     after = _img(rm)
     r.render_full(rm, after)
