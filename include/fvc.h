@@ -377,7 +377,7 @@ typedef struct {
   int32_t mlp_rows;               /* chosen MLP rows per wavefront: 16 or 32 */
   int32_t max_steps;              /* chosen samples per ray per launch */
   int64_t target_batch;           /* chosen target samples per launch */
-  float   ms_best;                /* median frame time of the choice */
+  float   ms_best;                /* frame time of the choice (fastest of five renders) */
   float   ms_default;             /* ... of the settings before tuning */
   int32_t n_tried;
   int32_t pad_;
