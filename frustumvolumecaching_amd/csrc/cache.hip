@@ -704,13 +704,27 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
   // integer cell of the corner (0, 0, 0); corner coordinates are exact
   // small integers, so the frustum test is an unsigned compare
   const int ixi = (int)ix, iyi = (int)iy, izi = (int)iz;
+  // brick slots: the corners share the brick of corner (0, 0, 0) unless an
+  // offset crosses a brick face, so only lanes whose corner lies in another
+  // brick load its slot (masked lanes cost no cache access)
+  const uint32_t bx0 = (uint32_t)ixi >> 3, by0 = (uint32_t)iyi >> 3, bz0 = (uint32_t)izi >> 3;
+  const bool crx = (ixi & 7) == 7, cry = (iyi & 7) == 7, crz = (izi & 7) == 7;
+  const uint32_t b0 = (bz0 * (uint32_t)cv.by + by0) * (uint32_t)cv.bx + bx0;
+  // corner (0, 0, 0) lies in the frustum whenever any corner does; its
+  // brick index is only read then (cell -1 gives a huge unsigned index)
+  const bool in0 = (uint32_t)ixi < (uint32_t)cv.fx && (uint32_t)iyi < (uint32_t)cv.fy &&
+                   (uint32_t)izi < (uint32_t)cv.fz;
+  const int slot0 = cv.table[in0 ? b0 : 0u];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
     const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
     const uint32_t cx = (uint32_t)(ixi + ox), cy = (uint32_t)(iyi + oy), cz = (uint32_t)(izi + oz);
     const bool inside = cx < (uint32_t)cv.fx && cy < (uint32_t)cv.fy && cz < (uint32_t)cv.fz;
     const uint32_t vx = inside ? cx : 0u, vy = inside ? cy : 0u, vz = inside ? cz : 0u;
-    const int slot = cv.table[((vz >> 3) * (uint32_t)cv.by + (vy >> 3)) * (uint32_t)cv.bx + (vx >> 3)];
+    const bool other = (ox && crx) || (oy && cry) || (oz && crz) || !in0;
+    int slot = slot0;
+    if (c != 0 && other && inside)
+      slot = cv.table[((vz >> 3) * (uint32_t)cv.by + (vy >> 3)) * (uint32_t)cv.bx + (vx >> 3)];
     const bool ok = inside && slot >= 0;
     const size_t vi = (size_t)(uint32_t)slot * BRICK_VOXELS + ((vz & 7u) * 64u + (vy & 7u) * 8u + (vx & 7u));
     if (LAT8) {
@@ -725,23 +739,27 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
       scl[c] = 0.0f;
     }
   }
-  // per-corner products, then the reference's summation tree
+  // per-corner products, then the reference's summation tree.  FP16
+  // latents: w * l as fma(l, w, -0), the exact product (signed zeros
+  // included), which takes the half operand directly (v_fma_mix_f32); the
+  // -0 is opaque so that the fma is not folded back into a multiply.
+  const float nz = opaque_f(-0.0f);
   float cw[8], cd[8], cl[8][8];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
     const float w = (wxs[(c >> 2) & 1] * wys[(c >> 1) & 1]) * wzs[c & 1];
     cw[c] = w * dis[c].y;
     cd[c] = w * dis[c].x;
-    float lv[8];
     if (LAT8) {
+      float lv[8];
       fp8x8_decode(lat[c].x, lat[c].y, scl[c], lv);
+#pragma unroll
+      for (int k = 0; k < 8; k++) cl[k][c] = w * lv[k];
     } else {
       const uint32_t lw[4] = {lat[c].x, lat[c].y, lat[c].z, lat[c].w};
 #pragma unroll
-      for (int k = 0; k < 8; k++) lv[k] = h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1))));
+      for (int k = 0; k < 8; k++) cl[k][c] = __builtin_fmaf(h2f((uint16_t)(lw[k >> 1] >> (16 * (k & 1)))), w, nz);
     }
-#pragma unroll
-    for (int k = 0; k < 8; k++) cl[k][c] = w * lv[k];
   }
   auto tree = [](const float* v) { return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])); };
   wd = tree(cw);
@@ -782,18 +800,25 @@ __global__ __launch_bounds__(256) void k_cache_head(const float4* __restrict__ c
   const int n_groups = (n32 + 63) / 64;
   uint4* recw = rec_lds[wave];
   const uint32_t one2 = 0x3c003c00u;  // two fp16 1.0
-  for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
+  const int gstride = gridDim.x * 4;
+  // the record of the wave's next group is loaded one iteration ahead, so
+  // the corner loads wait for one load less
+  int grp = blockIdx.x * 4 + wave;
+  float4 rec_next = c_rec[min(grp * 64 + lane, n32 - 1)];
+  for (; grp < n_groups; grp += gstride) {
     const int li = opaque(lane);
-    const int s = grp * 64 + lane;
-    const int sl = s < n32 ? s : n32 - 1;
-    const float4 rec = c_rec[sl];
+    const float4 rec = rec_next;
+    rec_next = c_rec[min((grp + gstride) * 64 + lane, n32 - 1)];
     const int px = __builtin_bit_cast(int, rec.w);
     const uint4 sh_lo = ray_sh[2 * (int64_t)px], sh_hi = ray_sh[2 * (int64_t)px + 1];
     float wd, dd, sums[8];
     lookup_sample<LAT8>(cv, rec, wd, dd, sums);
     _Float16 L[8];
+    // sums[k] / wd (wd >= 1/8: the voxel holding the sample is set and
+    // weighs at least 1/2 per axis)
+    const float rw = 1.0f / wd;
 #pragma unroll
-    for (int k = 0; k < 8; k++) L[k] = (_Float16)(sums[k] / wd);
+    for (int k = 0; k < 8; k++) L[k] = (_Float16)div_recip(sums[k], wd, rw);
     const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
     auto pk = [](_Float16 a, _Float16 b) {
       return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
@@ -870,21 +895,28 @@ __global__ __launch_bounds__(256) void k_cache_head_vd(const float4* __restrict_
   auto pk = [](_Float16 a, _Float16 b) {
     return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
   };
-  for (int grp = blockIdx.x * 4 + wave; grp < n_groups; grp += gridDim.x * 4) {
+  const int gstride = gridDim.x * 4;
+  // record and t0 of the wave's next group loaded one iteration ahead
+  int grp = blockIdx.x * 4 + wave;
+  float4 rec_next = c_rec[min(grp * 64 + lane, n32 - 1)];
+  float t0_next = c_t0[min(grp * 64 + lane, n32 - 1)];
+  for (; grp < n_groups; grp += gstride) {
     const int li = opaque(lane);
-    const int s = grp * 64 + lane;
-    const int sl = s < n32 ? s : n32 - 1;
-    const float4 rec = c_rec[sl];
+    const float4 rec = rec_next;
+    const float t0 = t0_next;
+    const int sn = min((grp + gstride) * 64 + lane, n32 - 1);
+    rec_next = c_rec[sn];
+    t0_next = c_t0[sn];
     const int px = __builtin_bit_cast(int, rec.w);
-    const float t0 = c_t0[sl];
     const uint4 sh_lo = ray_sh[2 * (int64_t)px], sh_hi = ray_sh[2 * (int64_t)px + 1];
     const float4 ro = ray_od[2 * (int64_t)px], rd = ray_od[2 * (int64_t)px + 1];
     float wd, dd, sums[8];
     lookup_sample<false>(cv, rec, wd, dd, sums);
     _Float16 L[8];
+    const float rw = 1.0f / wd;  // as in k_cache_head
 #pragma unroll
-    for (int k = 0; k < 8; k++) L[k] = (_Float16)(sums[k] / wd);
-    if (di) L[0] = (_Float16)((logf(sums[0]) * wd) / wd);
+    for (int k = 0; k < 8; k++) L[k] = (_Float16)div_recip(sums[k], wd, rw);
+    if (di) L[0] = (_Float16)div_recip(logf(sums[0]) * wd, wd, rw);
     const _Float16 logsig = di ? L[0] : (_Float16)logf(fmaxf_(dd, 1e-6f));
     // the sample's midpoint (generateNetworkInput_preparation_perf_kernel, 285-291)
     const V3 o = {ro.x, ro.y, ro.z}, d = {rd.x, rd.y, rd.z};

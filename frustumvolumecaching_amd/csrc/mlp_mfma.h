@@ -146,6 +146,10 @@ __device__ __forceinline__ int opaque(int v) {
   asm volatile("" : "+v"(v));
   return v;
 }
+__device__ __forceinline__ float opaque_f(float v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
 
 // SH degree 4 coefficients 4g..4g+3 of dir01*2-1 (my_fully_fused_mlp.cu:206-234),
 // rounded to fp16.

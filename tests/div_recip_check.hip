@@ -1,0 +1,32 @@
+// Host check of fvc::div_recip (fvc_math.h) against IEEE a / b: every b in
+// [2^-8, 2] with 8 random numerators each (both signs, magnitudes 2^-40 ..
+// 2^8), plus signed zeros.  Prints the counts; exits 1 on any mismatch.
+#include <cstdio>
+#include <cstring>
+
+#include "../frustumvolumecaching_amd/csrc/fvc_math.h"
+
+static float f_of(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+static uint32_t u_of(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+
+int main() {
+  uint64_t s = 88172645463325252ull, n = 0, bad = 0;
+  auto rnd = [&] { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return (uint32_t)(s >> 16); };
+  for (uint32_t bu = u_of(0.00390625f); bu <= u_of(2.0f); bu++) {
+    const float b = f_of(bu);
+    volatile float one = 1.0f;
+    const float y = one / b;
+    for (int k = 0; k < 8; k++) {
+      const uint32_t e = 87 + rnd() % 48, m = rnd() & 0x7fffffu, sg = rnd() & 1u;
+      const float a = f_of((sg << 31) | (e << 23) | m);
+      n++;
+      if (u_of(a / b) != u_of(fvc::div_recip(a, b, y))) bad++;
+    }
+    for (float z : {0.0f, -0.0f}) {
+      n++;
+      if (u_of(z / b) != u_of(fvc::div_recip(z, b, y))) bad++;
+    }
+  }
+  std::printf("%llu cases, %llu mismatches\n", (unsigned long long)n, (unsigned long long)bad);
+  return bad != 0;
+}
