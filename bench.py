@@ -42,7 +42,7 @@ def parse():
                     help="MLP arithmetic (operands; FP32 accumulation)")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default launch shapes")
-    ap.add_argument("--mlp-rows", type=int, choices=[16, 32], default=0,
+    ap.add_argument("--mlp-rows", type=int, choices=[16, 32, 64], default=0,
                     help="MLP rows per wavefront (with --batch / --max-steps: fixed shapes, no autotuning)")
     ap.add_argument("--no-refill", action="store_true", help="skip the background-refill leg")
     ap.add_argument("--no-graph", action="store_true", help="launch frames kernel by kernel (no hipGraph replay)")

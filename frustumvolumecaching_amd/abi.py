@@ -116,7 +116,7 @@ class RefillPolicy(C.Structure):
                 ("pad_", C.c_int32)]
 
 
-TUNE_MAX = 24
+TUNE_MAX = 32
 
 
 class TuneResult(C.Structure):

@@ -347,7 +347,7 @@ int fvc_renderer_set_graph(fvc_renderer* h, int enable) {
 int fvc_renderer_set_mlp_rows(fvc_renderer* h, int32_t rows) {
   return guard([&] {
     REQUIRE(h, "null renderer");
-    REQUIRE(rows == 16 || rows == 32, "rows must be 16 or 32");
+    REQUIRE(rows == 16 || rows == 32 || rows == 64, "rows must be 16, 32 or 64");
     h->r.mlp_rows = rows;
   });
 }

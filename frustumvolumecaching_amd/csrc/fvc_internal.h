@@ -338,7 +338,7 @@ struct Renderer {
   // MAX_N_STEPS_PER_SUBITER, performance_renderer.h:54-58)
   int64_t target_batch = 1 << 25;
   int max_steps = 32;
-  int mlp_rows = 16;     // MLP rows (samples) per wavefront and iteration: 16 or 32
+  int mlp_rows = 16;     // MLP rows (samples) per wavefront and iteration: 16, 32 or 64
   int latent_format = FVC_LATENT_FP16;  // latent storage of the next fills
   int last_iters = 0;    // sub-iterations of the previous frame (first burst length)
   // step table of the marchers (render.hip: step_table)

@@ -405,7 +405,8 @@ void launch_network_full(const Model& m, const float* pos, const float* dir, int
                      pos, dir, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, m.mlp_scales,   \
                      (uint2*)out_h4, (uint2*)nullptr, n_dev)
 #define L(P)                       \
-  if (rows == 32) { L2(P, 2); }    \
+  if (rows == 64) { L2(P, 4); }    \
+  else if (rows == 32) { L2(P, 2); } \
   else { L2(P, 1); }
   FVC_BY_PREC(m, L)
 #undef L
@@ -420,7 +421,8 @@ void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_
                      pos, (const float*)nullptr, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, \
                      m.mlp_scales, (uint2*)nullptr, (uint2*)out16, (const int*)nullptr)
 #define L(P)                       \
-  if (rows == 32) { L2(P, 2); }    \
+  if (rows == 64) { L2(P, 4); }    \
+  else if (rows == 32) { L2(P, 2); } \
   else { L2(P, 1); }
   FVC_BY_PREC(m, L)
 #undef L
@@ -447,9 +449,9 @@ void launch_network_full_vd(const Model& m, const float* pos, const float* dir, 
                      0, s, pos, dir, n, m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, di, fq, (uint2*)out_h4,  \
                      n_dev)
   if (m.net.first_head_n_hidden_layers == 2) {
-    if (rows == 32) { L(2, 2); } else { L(1, 2); }
+    if (rows == 64) { L(4, 2); } else if (rows == 32) { L(2, 2); } else { L(1, 2); }
   } else {
-    if (rows == 32) { L(2, 1); } else { L(1, 1); }
+    if (rows == 64) { L(4, 1); } else if (rows == 32) { L(2, 1); } else { L(1, 1); }
   }
 #undef L
   FVC_HIP(hipGetLastError());
@@ -463,9 +465,9 @@ void launch_network_latents_vd(const Model& m, const float* pos, const float* in
                      0, s, pos, init_dir, n, m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, di, fq,            \
                      (uint2*)out16, (const int*)nullptr)
   if (m.net.first_head_n_hidden_layers == 2) {
-    if (rows == 32) { L(2, 2); } else { L(1, 2); }
+    if (rows == 64) { L(4, 2); } else if (rows == 32) { L(2, 2); } else { L(1, 2); }
   } else {
-    if (rows == 32) { L(2, 1); } else { L(1, 1); }
+    if (rows == 64) { L(4, 1); } else if (rows == 32) { L(2, 1); } else { L(1, 1); }
   }
 #undef L
   FVC_HIP(hipGetLastError());

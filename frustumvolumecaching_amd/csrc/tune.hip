@@ -1,8 +1,8 @@
 // tune.hip -- autotuner of the renderer's launch shapes.
 //
-// Candidates: MLP rows per wavefront and loop iteration (16 or 32 samples:
-// one or two 16-sample MFMA tiles per wave, whose gathers are issued
-// together) for the network kernels, and the sub-iteration chunk sizes
+// Candidates: MLP rows per wavefront and loop iteration (16, 32 or 64
+// samples: one, two or four 16-sample MFMA tiles per wave, whose gathers are
+// issued together) for the network kernels, and the sub-iteration chunk sizes
 // (target samples per launch 2^23 / 2^24 / 2^25, samples per ray per launch
 // 16 / 32 / 64: the reference's TARGET_BATCH_SIZE and
 // MAX_N_STEPS_PER_SUBITER, performance_renderer.h:54-58).  Each candidate
@@ -20,7 +20,7 @@ namespace fvc {
 
 void autotune(Renderer& r, const fvc_raymarch_info& rm, bool cached, fvc_tune_result* out) {
   if (cached && !r.caches[r.render_idx].initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
-  const int rows_c[2] = {16, 32};
+  const int rows_c[3] = {16, 32, 64};
   const int64_t target_c[3] = {(int64_t)1 << 23, (int64_t)1 << 24, (int64_t)1 << 25};
   const int steps_c[3] = {16, 32, 64};
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;

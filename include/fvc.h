@@ -372,9 +372,9 @@ int fvc_cache_get_pool_decoded(fvc_renderer* renderer, int which, float* out_hos
 int fvc_cache_set_capacity(fvc_renderer* renderer, int32_t capacity_bricks);
 
 /* ---- autotuning (no counterpart in the reference) ---- */
-#define FVC_TUNE_MAX 24
+#define FVC_TUNE_MAX 32
 typedef struct {
-  int32_t mlp_rows;               /* chosen MLP rows per wavefront: 16 or 32 */
+  int32_t mlp_rows;               /* chosen MLP rows per wavefront: 16, 32 or 64 */
   int32_t max_steps;              /* chosen samples per ray per launch */
   int64_t target_batch;           /* chosen target samples per launch */
   float   ms_best;                /* frame time of the choice (fastest of five renders) */
@@ -398,7 +398,8 @@ int fvc_renderer_autotune(fvc_renderer* renderer, const fvc_raymarch_info* rm, i
  * frame.  A frame needing more sub-iterations continues with ordinary
  * launches.  Profiling (fvc_profile_enable) renders without graphs. */
 int fvc_renderer_set_graph(fvc_renderer* renderer, int enable);
-/* MLP rows per wavefront (16 or 32) for the network and cache-head kernels. */
+/* MLP rows per wavefront (16, 32 or 64) for the network kernels (the cache
+ * heads always take 64 samples per wave, one per lane). */
 int fvc_renderer_set_mlp_rows(fvc_renderer* renderer, int32_t rows);
 
 /* ---- differentiable full render (training mode) ----

@@ -77,20 +77,21 @@ def test_batch_rejects_mixed_stepsize(setup):
         r.render_full_batch([views[0], rm2])
 
 
-def test_mlp_rows_32_equals_16(setup):
+def test_mlp_rows_32_and_64_equal_16(setup):
     wl, model, r, views = setup
     rm = views[2]
     out = {}
     r.initCacheBlockwise(views[0])
-    for rows in (16, 32):
+    for rows in (16, 32, 64):
         r.set_mlp_rows(rows)
         a, b = _img(rm), _img(rm)
         r.render_full(rm, a)
         r.render_cached(rm, b)
         out[rows] = (a.cpu().numpy(), b.cpu().numpy())
     r.set_mlp_rows(16)
-    assert np.array_equal(out[16][0], out[32][0])
-    assert np.array_equal(out[16][1], out[32][1])
+    for rows in (32, 64):
+        assert np.array_equal(out[16][0], out[rows][0]), rows
+        assert np.array_equal(out[16][1], out[rows][1]), rows
 
 
 def test_autotune_keeps_images(setup):
@@ -101,7 +102,7 @@ def test_autotune_keeps_images(setup):
     res = r.autotune(rm, cached=False)
     print(f"[tune] rows {res.mlp_rows} target 2^{int(np.log2(res.target_batch))} steps {res.max_steps}: "
           f"{res.ms_best:.3f} ms (default {res.ms_default:.3f} ms, {res.n_tried} tried)", flush=True)
-    assert res.n_tried == 18 and res.mlp_rows in (16, 32) and res.max_steps in (16, 32, 64)
+    assert res.n_tried == 27 and res.mlp_rows in (16, 32, 64) and res.max_steps in (16, 32, 64)
     assert res.ms_best <= min(res.ms[:res.n_tried]) + 1e-6
     after = _img(rm)
     r.render_full(rm, after)

@@ -168,7 +168,8 @@ def test_cached_at_fill_pose_matches_full_render(vd, filled, oracle_mod):
 
 
 def test_launch_shapes_and_graph_give_identical_images(vd, filled):
-    """32 MLP rows per wavefront and graph replay give the same images."""
+    """32 and 64 MLP rows per wavefront and graph replay give the same
+    images."""
     wl, gm, om, r = vd
     rm = wl.rm(S.yaw(wl.cam2world, 1.0), width=RES, height=RES)
     base_full = _render(r, rm, False)[0]
@@ -181,6 +182,10 @@ def test_launch_shapes_and_graph_give_identical_images(vd, filled):
         for _ in range(2):
             assert np.array_equal(_render(r, rm, False)[0], base_full)
             assert np.array_equal(_render(r, rm, True)[0], base_cached)
+        r.set_graph(False)
+        r.set_mlp_rows(64)
+        assert np.array_equal(_render(r, rm, False)[0], base_full)
+        assert np.array_equal(_render(r, rm, True)[0], base_cached)
     finally:
         r.set_graph(False)
         r.set_mlp_rows(16)
