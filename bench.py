@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--mlp-rows", type=int, choices=[16, 32, 64], default=0,
                     help="MLP rows per wavefront (with --batch / --max-steps: fixed shapes, no autotuning)")
     ap.add_argument("--no-refill", action="store_true", help="skip the background-refill leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the multi-view batch leg")
     ap.add_argument("--no-graph", action="store_true", help="launch frames kernel by kernel (no hipGraph replay)")
     ap.add_argument("--cache-latents", choices=["fp16", "fp8"], default="fp16",
                     help="latent storage of the brick pool (fp8: E4M3 with a per-brick scale)")
@@ -194,6 +195,7 @@ def main():
     full_ms_prof, _, prof_full = full_path(True)
     r.set_graph(False)
     async_fps = None if a.no_refill else async_refill_leg(a, wl, model, rm_of, img)
+    batching = None if a.no_batch else batch_leg(wl, r, path, use_shapes)
 
     # ---- PSNR: cached vs full (GPU) on frames 1, 5, 9 after a fill at frame 0
     from oracle.pyoracle import psnr_8bit, to_8bit
@@ -322,6 +324,7 @@ def main():
         "kernels_full": kernel_table(prof_full),
         "cpu_baseline": cpu,
         "async_refill_fps": async_fps,
+        "batching": batching,
         "autotune": tune,
         "autotune_full": tune_full,
         "graph": not a.no_graph,
@@ -368,6 +371,44 @@ def async_refill_leg(a, wl, model, rm_of, img):
     fps = a.steps / dt
     log(f"[bench] background refill: {fps:.1f} frames/s over {a.steps} frames incl. fills")
     return round(fps, 3)
+
+
+def batch_leg(wl, r, path, use_shapes, reps=3):
+    """Sixteen views of resolutions 320x180 .. 1920x1080 (poses 1..8 of the
+    camera path) rendered one at a time and as one multi-view batch (one
+    alive list and one sample stream for every view), full and cached (cache
+    filled at the path's first pose, 1920x1080).  Eager launches; the best of
+    reps rounds each."""
+    import torch
+    res = [(320, 180), (640, 360), (960, 540), (1280, 720), (1920, 1080)]
+    wh = [res[k % len(res)] for k in range(16)]
+    views = [wl.rm(path[1 + k % 8], w, h) for k, (w, h) in enumerate(wh)]
+    imgs = [torch.zeros(w * h * 4, dtype=torch.uint8, device="cuda") for (w, h) in wh]
+    r.initCacheBlockwise(wl.rm(path[0], 1920, 1080))
+    out = {"views": len(views), "resolutions": [f"{w}x{h}" for (w, h) in res],
+           "pixels": sum(w * h for (w, h) in wh)}
+    for kind in ("full", "cached"):
+        use_shapes(kind)
+        one, many = getattr(r, f"render_{kind}"), getattr(r, f"render_{kind}_batch")
+        best_s = best_b = 1e30
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for v, im in zip(views, imgs):
+                one(v, im)
+            torch.cuda.synchronize()
+            best_s = min(best_s, time.perf_counter() - t)
+            t = time.perf_counter()
+            st = many(views, imgs)
+            torch.cuda.synchronize()
+            best_b = min(best_b, time.perf_counter() - t)
+        n = int(st.n_resamples + st.n_cache_hits)
+        out[kind] = {"single_ms": round(best_s * 1e3, 3), "batch_ms": round(best_b * 1e3, 3),
+                     "views_per_s_batch": round(len(views) / best_b, 1),
+                     "msamples_per_s_batch": round(n / best_b / 1e6, 1),
+                     "speedup_batch_vs_single": round(best_s / best_b, 3)}
+    log(f"[bench] multi-view batch: {out}")
+    return out
 
 
 def diff_leg(a, steps=3):

@@ -1045,7 +1045,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   const int n0 = graph ? (int)vs.n_px : launch_init_rays(r, vs, s);
   P.work(P.end(ev, s, "init_rays"), n_px, n_px * 16 + n0 * 60.0, 0);
   int* ctl = r.counters.as<int>();
-  const int64_t target = std::min<int64_t>(r.target_batch, cap - 64);
+  const int64_t target = std::min<int64_t>(vs.target(r.target_batch), cap - 64);
   const int64_t n_bound = std::min<int64_t>(cap - 64, (int64_t)n0 * r.max_steps);  // samples per launch
   const dim3 grid((n0 + 255) / 256);
   const int head_blocks = (int)std::min<int64_t>((n_bound + 255) / 256, 256 * 16);

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -414,7 +415,7 @@ void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int 
 void diff_backward(Renderer& r, const float* d_rgb, float* g_pos, float* g_base, float* g_head, float* g_pool);
 void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts, float* t0s);
 void resize_renderbuffers(Renderer& r, int w, int h);
-void ensure_ray_capacity(Renderer& r, int64_t n_pixels);
+void ensure_ray_capacity(Renderer& r, int64_t n_pixels, int64_t target);
 void graph_run(Renderer& r, int which, hipStream_t s, const std::function<void()>& record);
 
 // a batch of views rendered together: pixel base of view v in the packed
@@ -425,7 +426,15 @@ struct Views {
   uint8_t* const* rgba;
   std::vector<int64_t> base;
   int64_t n_px;
+  int64_t max_px;  // pixels of the largest view
   Views(const fvc_raymarch_info* rms, int n, uint8_t* const* rgba);
+  // samples per launch for a target set per view: a batch scales it by its
+  // pixels over its largest view's, so every view keeps the per-ray budget
+  // of a single render (at most 2^28)
+  int64_t target(int64_t per_view) const {
+    if (n_px <= max_px) return per_view;
+    return (int64_t)std::min(268435456.0, (double)per_view * (double)n_px / (double)max_px);
+  }
 };
 void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats);
 void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats);

@@ -205,18 +205,19 @@ __global__ void k_debug_march(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, Ma
 
 void resize_renderbuffers(Renderer& r, int w, int h) {
   if (w <= 0 || h <= 0) throw Error(FVC_E_INVALID, "resolution must be positive");
-  ensure_ray_capacity(r, (int64_t)w * h);
+  ensure_ray_capacity(r, (int64_t)w * h, r.target_batch);
   r.width = w;
   r.height = h;
 }
 
-// ray and sample buffers for n pixels (all views of a batch)
-void ensure_ray_capacity(Renderer& r, int64_t n) {
+// ray and sample buffers for n pixels (all views of a batch) and a launch
+// target of `target` samples
+void ensure_ray_capacity(Renderer& r, int64_t n, int64_t target) {
   if (n >= ((int64_t)1 << 31) / 4) throw Error(FVC_E_INVALID, "too many pixels in one render call");
   n = std::max(n, r.ray_capacity);  // buffers only grow
   // a launch emits at most n_alive * n_steps <= max(target, n) samples per
   // stream, and never more than n * max_steps
-  const int64_t cap = std::min<int64_t>(std::max<int64_t>(r.target_batch, n), n * r.max_steps) + 64;
+  const int64_t cap = std::min<int64_t>(std::max<int64_t>(target, n), n * r.max_steps) + 64;
   for (int b = 0; b < 2; b++) {
     r.rays[b].ensure(n * sizeof(RayState));
     r.ray_pix[b].ensure(n * sizeof(int));
@@ -327,6 +328,7 @@ Views::Views(const fvc_raymarch_info* rms_, int n_, uint8_t* const* rgba_) : rms
   if (n <= 0) throw Error(FVC_E_INVALID, "no views");
   base.resize(n);
   n_px = 0;
+  max_px = 0;
   for (int v = 0; v < n; v++) {
     const int W = rms[v].cam_info.res_x, H = rms[v].cam_info.res_y;
     if (W <= 0 || H <= 0) throw Error(FVC_E_INVALID, "resolution must be positive");
@@ -334,6 +336,7 @@ Views::Views(const fvc_raymarch_info* rms_, int n_, uint8_t* const* rgba_) : rms
       throw Error(FVC_E_INVALID, "the views of a batch must share the stepsize info");
     base[v] = n_px;
     n_px += (int64_t)W * H;
+    max_px = std::max<int64_t>(max_px, (int64_t)W * H);
   }
 }
 
@@ -375,7 +378,7 @@ void prepare_views(Renderer& r, const Views& vs) {
     const int W = vs.rms[0].cam_info.res_x, H = vs.rms[0].cam_info.res_y;
     if (W != r.width || H != r.height) resize_renderbuffers(r, W, H);
   } else {
-    ensure_ray_capacity(r, vs.n_px);
+    ensure_ray_capacity(r, vs.n_px, vs.target(r.target_batch));
   }
 }
 
@@ -459,7 +462,7 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   P.work(P.end(ev, s, "init_rays"), n_px, n_px * 16 + n0 * 60.0, 0);
   int* ctl = r.counters.as<int>();
   const int64_t cap = r.sample_capacity - 64;
-  const int64_t target = std::min<int64_t>(r.target_batch, cap);
+  const int64_t target = std::min<int64_t>(vs.target(r.target_batch), cap);
   const int64_t n_bound = std::min<int64_t>(cap, (int64_t)n0 * r.max_steps);  // samples per launch
   const dim3 grid((n0 + 255) / 256);
   struct Handles { int march, net, comp; };
