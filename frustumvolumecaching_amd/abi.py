@@ -152,7 +152,7 @@ class KernelStats(C.Structure):
 
 # voxel record of the brick pool: 8 fp16 latents, density, isset, 2 pad floats
 VOXEL_DTYPE = np.dtype([("lat", np.uint16, 8), ("density", np.float32), ("isset", np.float32),
-                        ("pad", np.float32, 2)])
+                        ("xnext", np.float32, 2)])
 assert VOXEL_DTYPE.itemsize == 32
 
 

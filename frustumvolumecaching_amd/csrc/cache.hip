@@ -332,7 +332,11 @@ __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list,
           v.density = 0.0f;
           v.isset = 0.0f;
         }
-        v.pad[0] = v.pad[1] = 0.0f;
+        // lane = 8 y + x: lane + 1 holds the +x neighbour of the brick row
+        const float nd = __shfl_down(v.density, 1, 64), ns = __shfl_down(v.isset, 1, 64);
+        const bool row_end = (lane & 7) == 7;
+        v.xnext[0] = row_end ? 0.0f : nd;
+        v.xnext[1] = row_end ? 0.0f : ns;
         brick[z * 64 + lane] = v;
       }
       tile.n_bricks++;
@@ -701,6 +705,8 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
   uint4 lat[8];
   float2 dis[8];
   float scl[8];
+  const Voxel* vp[8];  // FP16 pool: corner voxels (the zero voxel outside)
+  bool vok[8];
   // integer cell of the corner (0, 0, 0); corner coordinates are exact
   // small integers, so the frustum test is an unsigned compare
   const int ixi = (int)ix, iyi = (int)iy, izi = (int)iz;
@@ -715,6 +721,12 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
   const bool in0 = (uint32_t)ixi < (uint32_t)cv.fx && (uint32_t)iyi < (uint32_t)cv.fy &&
                    (uint32_t)izi < (uint32_t)cv.fz;
   const int slot0 = cv.table[in0 ? b0 : 0u];
+  // every slot load is issued before any voxel load waits for one
+  int slots[8];
+  bool ins[8], own[8];  // own: the corner's brick is not corner 0's
+  uint32_t locs[8];
+  // FP8 latents: the brick scale of corner 0, shared like its slot
+  const float scale0 = LAT8 ? cv.scales[in0 && slot0 >= 0 ? (uint32_t)slot0 : 0u] : 0.0f;
 #pragma unroll
   for (int c = 0; c < 8; c++) {
     const int ox = (c >> 2) & 1, oy = (c >> 1) & 1, oz = c & 1;
@@ -722,21 +734,55 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
     const bool inside = cx < (uint32_t)cv.fx && cy < (uint32_t)cv.fy && cz < (uint32_t)cv.fz;
     const uint32_t vx = inside ? cx : 0u, vy = inside ? cy : 0u, vz = inside ? cz : 0u;
     const bool other = (ox && crx) || (oy && cry) || (oz && crz) || !in0;
-    int slot = slot0;
-    if (c != 0 && other && inside)
-      slot = cv.table[((vz >> 3) * (uint32_t)cv.by + (vy >> 3)) * (uint32_t)cv.bx + (vx >> 3)];
-    const bool ok = inside && slot >= 0;
-    const size_t vi = (size_t)(uint32_t)slot * BRICK_VOXELS + ((vz & 7u) * 64u + (vy & 7u) * 8u + (vx & 7u));
+    slots[c] = slot0;
+    own[c] = c != 0 && other && inside;
+    if (own[c]) slots[c] = cv.table[((vz >> 3) * (uint32_t)cv.by + (vy >> 3)) * (uint32_t)cv.bx + (vx >> 3)];
+    ins[c] = inside;
+    locs[c] = (vz & 7u) * 64u + (vy & 7u) * 8u + (vx & 7u);
+  }
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int slot = slots[c];
+    const bool ok = ins[c] && slot >= 0;
+    const size_t vi = (size_t)(uint32_t)slot * BRICK_VOXELS + locs[c];
     if (LAT8) {
       const uint4 q = *reinterpret_cast<const uint4*>(ok ? (const void*)(cv.pool8 + vi) : (const void*)&k_zero_voxel);
       lat[c] = make_uint4(q.x, q.y, 0u, 0u);
       dis[c] = make_float2(__builtin_bit_cast(float, q.z), __builtin_bit_cast(float, q.w));
-      scl[c] = ok ? cv.scales[(uint32_t)slot] : 0.0f;
+      float sc = scale0;
+      if (own[c] && ok) sc = cv.scales[(uint32_t)slot];
+      scl[c] = ok ? sc : 0.0f;
     } else {
-      const Voxel* vp = ok ? cv.pool + vi : &k_zero_voxel;
-      lat[c] = *reinterpret_cast<const uint4*>(vp->lat);
-      dis[c] = *reinterpret_cast<const float2*>(&vp->density);
+      vok[c] = ok;
+      vp[c] = ok ? cv.pool + vi : &k_zero_voxel;
+      lat[c] = *reinterpret_cast<const uint4*>(vp[c]->lat);
       scl[c] = 0.0f;
+    }
+  }
+  if (!LAT8) {
+    // density and isset: corner c (x offset 0) with its xnext, which is
+    // corner c + 4 when both are set voxels of one brick row; otherwise
+    // corner c + 4 loads its own (only those lanes access memory).  The
+    // 16-byte loads pass through an empty asm once every load is issued, so
+    // that they stay single loads and the masked loads stay masked.
+    f4 d4[4];
+    float2 d1[4];
+    bool shared[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      d4[c] = *reinterpret_cast<const f4*>(&vp[c]->density);
+      shared[c] = vok[c] && vok[c + 4] && !crx;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      d1[c] = make_float2(0.0f, 0.0f);
+      if (!shared[c]) d1[c] = *reinterpret_cast<const float2*>(&vp[c + 4]->density);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      asm volatile("" : "+v"(d4[c]));
+      dis[c] = make_float2(d4[c][0], d4[c][1]);
+      dis[c + 4] = shared[c] ? make_float2(d4[c][2], d4[c][3]) : d1[c];
     }
   }
   // per-corner products, then the reference's summation tree.  FP16

@@ -160,12 +160,14 @@ void pack_fragments_vd(const uint16_t* base, const uint16_t* first, const uint16
 static const int BRICK = 8;               // Cache::BRICK_SIZE (cache.h:17)
 static const int BRICK_VOXELS = 512;
 
-// voxel record of the brick pool (32 bytes)
+// voxel record of the brick pool (32 bytes).  xnext repeats density and
+// isset of the +x neighbour in the same brick row (zeros at x = 7), so that
+// one 16-byte load reads both corners of an x pair of a trilinear lookup.
 struct alignas(16) Voxel {
   uint16_t lat[8];   // fp16 latents
   float density;
   float isset;
-  float pad[2];
+  float xnext[2];
 };
 
 // voxel record with FP8-E4M3 latents (16 bytes): codes of l / s with the
