@@ -1028,10 +1028,14 @@ __global__ __launch_bounds__(256) void k_cache_composite(
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   bool keep = false;
+  int px = 0;
   float4 res = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   if (valid) {
     res = res_in[i];
     const int4 e = entry[i];
+    // loaded with the entry: they are only needed after the samples
+    const int ray_alive = alive_flag[i];
+    px = pix_in[i];
     const uint64_t ord = order[i];
     const int n = e.y + e.w;
     bool visible = true;  // transmittance still above the early-stop threshold
@@ -1055,14 +1059,14 @@ __global__ __launch_bounds__(256) void k_cache_composite(
       for (int u = 0; u < COMPOSITE_BATCH; u++)
         if (visible && k0 + u < n) visible = composite_sample(res, v[u], t[u], st, alpha_thre);
     }
-    keep = visible && alive_flag[i] != 0;
-    if (!keep) results[pix_in[i]] = res;
+    keep = visible && ray_alive != 0;
+    if (!keep) results[px] = res;
   }
   int slot[1];
   block_reserve<1>({keep ? 1 : 0}, {counter}, slot);
   if (keep) {
     rays_out[slot[0]] = rays_in[i];
-    pix_out[slot[0]] = pix_in[i];
+    pix_out[slot[0]] = px;
     res_out[slot[0]] = res;
   }
 }

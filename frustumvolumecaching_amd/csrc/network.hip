@@ -192,13 +192,23 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
     int64_t s[TILES], sl[TILES];
     bool valid[TILES];
     float f[TILES][8];
+    V3 pv[TILES], dv[TILES];  // positions; view directions (used after the density MLP)
+    // every tile's position (and direction) is loaded before any tile's
+    // gathers, which wait for them (the wait counter is in order)
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
       s[t] = (grp * TILES + t) * 16 + col;
       valid[t] = s[t] < n;
       sl[t] = valid[t] ? s[t] : n - 1;
-      const V3 p = {pos[3 * sl[t]], pos[3 * sl[t] + 1], pos[3 * sl[t] + 2]};
-      encode_quads(lv, gp, table, p, g, f[t]);
+      pv[t] = V3{pos[3 * sl[t]], pos[3 * sl[t] + 1], pos[3 * sl[t] + 2]};
+      if (HEAD) dv[t] = V3{dir[3 * sl[t]], dir[3 * sl[t] + 1], dir[3 * sl[t] + 2]};
+    }
+#pragma unroll
+    for (int t = 0; t < TILES; t++) {
+      encode_quads(lv, gp, table, pv[t], g, f[t]);
+      // pinned here (after the gathers are issued): otherwise the load sinks
+      // to its use after the density MLP and waits there on its own
+      if (HEAD) asm volatile("" : "+v"(dv[t].x), "+v"(dv[t].y), "+v"(dv[t].z));
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
@@ -210,8 +220,7 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
       if (!HEAD) {
         if (valid[t]) out16[s[t] * 4 + g] = pack_h4(rows[0], rows[1], rows[2], rows[3]);
       } else {
-        const int64_t q = sl[t];
-        const typename Mx<P>::T hin = head_input<P>(rows, g, dir[3 * q], dir[3 * q + 1], dir[3 * q + 2], sc.inv_act);
+        const typename Mx<P>::T hin = head_input<P>(rows, g, dv[t].x, dv[t].y, dv[t].z, sc.inv_act);
         const f4 o = head_mlp<P>(wl + FRAG_H1 * 64, wl + FRAG_H2 * 64, li, hin, sc);
         if (g == 0 && valid[t]) out_h4[s[t]] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2], rows[0]);
       }
@@ -281,13 +290,20 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network_vd(const float* __restric
     int64_t s[TILES], sl[TILES];
     bool valid[TILES];
     float f[TILES][8];
+    V3 pv[TILES], dv[TILES];  // positions and directions, loaded before the gathers
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
       s[t] = (grp * TILES + t) * 16 + col;
       valid[t] = s[t] < n;
       sl[t] = valid[t] ? s[t] : n - 1;
-      const V3 p = {pos[3 * sl[t]], pos[3 * sl[t] + 1], pos[3 * sl[t] + 2]};
-      encode_quads(lv, gp, table, p, g, f[t]);
+      pv[t] = V3{pos[3 * sl[t]], pos[3 * sl[t] + 1], pos[3 * sl[t] + 2]};
+      dv[t] = V3{dir[3 * sl[t]], dir[3 * sl[t] + 1], dir[3 * sl[t] + 2]};
+    }
+#pragma unroll
+    for (int t = 0; t < TILES; t++) {
+      encode_quads(lv, gp, table, pv[t], g, f[t]);
+      // pinned here (after the gathers are issued), as in k_network
+      asm volatile("" : "+v"(dv[t].x), "+v"(dv[t].y), "+v"(dv[t].z));
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
@@ -295,9 +311,8 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network_vd(const float* __restric
       for (int e = 0; e < 8; e++) f[t][e] = (float)(_Float16)f[t][e];
       const f4 d3 = base_mlp<FVC_PREC_FP16>(wl, li, MxH::pack(f[t], 1.0f), sc);
       const _Float16 bh[4] = {(_Float16)d3[0], (_Float16)d3[1], (_Float16)d3[2], (_Float16)d3[3]};
-      const int64_t q = sl[t];
       _Float16 sh[4];
-      sh4_quarter(dir[3 * q], dir[3 * q + 1], dir[3 * q + 2], g, sh);
+      sh4_quarter(dv[t].x, dv[t].y, dv[t].z, g, sh);
       const f4 fo = first_head_mlp<FL>(wl, li, h8_of(bh, sh));
       _Float16 lat[4] = {(_Float16)fo[0], (_Float16)fo[1], (_Float16)fo[2], (_Float16)fo[3]};
       if (di && g == 0) lat[0] = bh[0];
@@ -310,10 +325,9 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network_vd(const float* __restric
         if (g == 3) v = make_uint2(0u, 0u);
         if (valid[t]) out[s[t] * 4 + g] = v;
       } else {
-        const V3 p = {pos[3 * q], pos[3 * q + 1], pos[3 * q + 2]};
         _Float16 fa[4], fb[4];
-        freq_quad(p, g + 2, freq_on != 0, fa);
-        freq_quad(p, g, freq_on != 0, fb);
+        freq_quad(pv[t], g + 2, freq_on != 0, fa);
+        freq_quad(pv[t], g, freq_on != 0, fb);
         const _Float16 zero4[4] = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
         const h8 a = h8_of(g < 2 ? lat : fa, sh);
         const h8 b = h8_of(zero4, fb);

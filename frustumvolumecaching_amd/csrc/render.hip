@@ -128,10 +128,14 @@ __global__ __launch_bounds__(256) void k_composite(int* __restrict__ ctl, int pa
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   bool keep = false;
+  int px = 0;
   float4 res = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   if (valid) {
     res = res_in[i];
     const int4 e = entry[i];
+    // loaded with the entry: they are only needed after the samples
+    const int ray_alive = alive_flag[i];
+    px = pix_in[i];
     bool visible = true;  // transmittance still above the early-stop threshold
     for (int k0 = 0; k0 < e.y && visible; k0 += COMPOSITE_BATCH) {
       uint2 v[COMPOSITE_BATCH];
@@ -147,14 +151,14 @@ __global__ __launch_bounds__(256) void k_composite(int* __restrict__ ctl, int pa
       for (int u = 0; u < COMPOSITE_BATCH; u++)
         if (visible && k0 + u < e.y) visible = composite_sample(res, v[u], t[u], st, alpha_thre);
     }
-    keep = visible && alive_flag[i] != 0;
-    if (!keep) results[pix_in[i]] = res;
+    keep = visible && ray_alive != 0;
+    if (!keep) results[px] = res;
   }
   int slot[1];
   block_reserve<1>({keep ? 1 : 0}, {counter}, slot);
   if (keep) {
     rays_out[slot[0]] = rays_in[i];
-    pix_out[slot[0]] = pix_in[i];
+    pix_out[slot[0]] = px;
     res_out[slot[0]] = res;
   }
 }
