@@ -568,7 +568,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
       }
     }
     alive = true;
-    MarchRay mr = march_ray(r);
+    MarchRay mr = march_ray(r, mg.tab);
     int trips = 0;
     while (cnt < n_steps && trips < trip_cutoff) {
       trips++;

@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_diff_count(fvc_raymarch_info rm, V3 aab
   rays[i] = r;
   int cnt = 0;
   if (hit) {
-    MarchRay mr = march_ray(r);
+    MarchRay mr = march_ray(r, mg.tab);
     int bins = 0;
     while (true) {
       float t0;
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void k_diff_emit(int n, MarchGrid mg, Contract
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || counts[i] == 0) return;
   const RayState r = rays[i];
-  MarchRay mr = march_ray(r);
+  MarchRay mr = march_ray(r, mg.tab);
   const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
   int q = (int)offsets[i];  // the batch holds fewer than 2^31 samples
   const int end = q + counts[i];
@@ -655,7 +655,7 @@ __global__ __launch_bounds__(256) void k_diffc_count(fvc_raymarch_info rm, V3 aa
   rays[i] = r;
   int nr = 0, nc = 0;
   if (hit) {
-    MarchRay mr = march_ray(r);
+    MarchRay mr = march_ray(r, mg.tab);
     const V3 oc = transform(cv.cam.world2cam, mr.o.x, mr.o.y, mr.o.z, 1.0f);
     const V3 dc = transform(cv.cam.world2cam, mr.d.x, mr.d.y, mr.d.z, 0.0f);
     int bins = 0;
@@ -692,7 +692,7 @@ __global__ __launch_bounds__(256) void k_diffc_emit(int n, MarchGrid mg, Contrac
   offsets[i] = q;
   if (nr + nc == 0) return;
   const RayState r = rays[i];
-  MarchRay mr = march_ray(r);
+  MarchRay mr = march_ray(r, mg.tab);
   const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
   const V3 oc = transform(cv.cam.world2cam, o.x, o.y, o.z, 1.0f);
   const V3 dc = transform(cv.cam.world2cam, d.x, d.y, d.z, 0.0f);

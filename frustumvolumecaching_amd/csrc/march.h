@@ -216,6 +216,16 @@ FVC_HD float step_estimate(const StepTableView& v, float t) {
   return fast_log2(t) * v.log_a + v.log_b;
 }
 
+// Where row m lies in the table: base = its first entry, last = the largest
+// k whose bin end is in the row (base = -1: no row).  A ray reads it once
+// per launch (MarchRay), not at every jump.
+struct StepRow { int base, last; };
+FVC_HD StepRow step_row(const StepTableView& tab, int m) {
+  if (m < 0 || m >= tab.n_rows) return StepRow{-1, -1};
+  const int base = tab.off[m];
+  return StepRow{base, tab.off[m + 1] - base - 2};
+}
+
 // The bin before j (midpoint in [lo, hi]) is skipped; advance to the first
 // bin k >= j whose midpoint exceeds hi.  Every bin in between has its
 // midpoint in [that midpoint, hi], inside the same empty box.  t1 and j keep
@@ -225,12 +235,11 @@ FVC_HD float step_estimate(const StepTableView& v, float t) {
 // float recurrence drifts by a small fraction of a bin over a row), so one
 // round of four independent loads around the estimate decides it; a
 // bisection or a forward walk covers the rare misses.
-FVC_HD int table_jump(const StepTableView& tab, int m, int& j, float& t1, float hi) {
-  if (m < 0 || m >= tab.n_rows) return 0;
-  const int base = tab.off[m];
-  const int last = tab.off[m + 1] - base - 2;  // largest k whose bin end is in the row
+FVC_HD int table_jump(const StepTableView& tab, StepRow rw, int m, int& j, float& t1, float hi) {
+  if (rw.base < 0) return 0;
+  const int last = rw.last;
   if (j >= last) return 0;
-  const float* row = tab.t + base;
+  const float* row = tab.t + rw.base;
   // first k with midpoint(k) > hi: k + m + near_i + 1/2 > step(hi)
   int k = (int)floorf(step_estimate(tab, hi) - tab.near_i - (float)m - 0.5f) + 1;
   k = k < j ? j : (k > last ? last : k);
@@ -324,6 +333,7 @@ struct MarchRay {
   int m, j;
   float t1;
   Skip sk;
+  StepRow row;  // step_row(tab, m)
 };
 
 enum { TRIP_EMPTY = 0, TRIP_SAMPLE = 1, TRIP_END = 2 };
@@ -345,7 +355,7 @@ FVC_HD int march_trip(const MarchGrid& g, const uint32_t* coarse, MarchRay& r, f
     if (occ_query(r.o + tm * r.d, g.occ, g.bf64, coarse, r.slab, r.sk)) return TRIP_SAMPLE;
     if (!in_skip(r.sk, tm)) return TRIP_EMPTY;
   }
-  bins += table_jump(g.tab, r.m, r.j, r.t1, r.sk.hi);
+  bins += table_jump(g.tab, r.row, r.m, r.j, r.t1, r.sk.hi);
   return TRIP_EMPTY;
 }
 

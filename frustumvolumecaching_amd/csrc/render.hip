@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
   if (valid) {
     rs = rays[i];
     alive = true;
-    MarchRay r = march_ray(rs);
+    MarchRay r = march_ray(rs, mg.tab);
     while (cnt < n_steps) {
       float t0;
       const int res = march_trip(mg, coarse, r, t0, bins);
@@ -191,7 +191,7 @@ __global__ void k_debug_march(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, Ma
   RayState rs;
   int cnt = 0;
   if (init_ray_state(rm, aabb_from, aabb_to, x, y, rs)) {
-    MarchRay r = march_ray(rs);
+    MarchRay r = march_ray(rs, mg.tab);
     int bins = 0;
     while (true) {
       float t0;

@@ -37,8 +37,9 @@ __device__ __forceinline__ bool init_ray_state(const fvc_raymarch_info& rm, V3 a
   return hit;
 }
 
-__device__ __forceinline__ MarchRay march_ray(const RayState& s) {
+__device__ __forceinline__ MarchRay march_ray(const RayState& s, const StepTableView& tab) {
   MarchRay r;
+  r.row = step_row(tab, s.m);
   r.o = V3{s.ox, s.oy, s.oz};
   r.d = V3{s.dx, s.dy, s.dz};
   r.slab = ray_slab(r.o, r.d);
