@@ -383,12 +383,12 @@ static int frustum_depth(const fvc_raymarch_info& rm, const fvc_scene_info& sc) 
 }
 
 // CacheRenderer::resizeCache estimate (cache_renderer.cu:832-846 + cache.cu:143-146)
-static int default_capacity(int n_rays) {
-  const int est_samples = n_rays * 32;
-  const int per_brick = (int)(0.5f * BRICK * BRICK * BRICK);
-  const int est_bricks = est_samples / per_brick;
-  const int per_layer = 128 * 128;
-  return (est_bricks + per_layer - 1) / per_layer * per_layer;
+static int default_capacity(int64_t n_rays) {
+  const int64_t est_samples = n_rays * 32;
+  const int64_t per_brick = (int64_t)(0.5f * BRICK * BRICK * BRICK);
+  const int64_t est_bricks = est_samples / per_brick;
+  const int64_t per_layer = 128 * 128;
+  return (int)std::min<int64_t>(MAX_BRICKS, (est_bricks + per_layer - 1) / per_layer * per_layer);
 }
 
 // The fill keeps its own counters, pinned mirror and scratch, so it can run
@@ -410,7 +410,7 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
   C.rm = rm;
   C.fx = W; C.fy = H; C.fz = frustum_depth(rm, m.scene);
   C.bx = (W + BRICK - 1) / BRICK; C.by = (H + BRICK - 1) / BRICK; C.bz = (C.fz + BRICK - 1) / BRICK;
-  C.capacity = r.capacity_override > 0 ? r.capacity_override : default_capacity(W * H);
+  C.capacity = std::min(MAX_BRICKS, r.capacity_override > 0 ? r.capacity_override : default_capacity((int64_t)W * H));
   const int64_t n_table = (int64_t)C.bx * C.by * C.bz;
   C.table.ensure(n_table * sizeof(int32_t));
   C.known.ensure((int64_t)W * H * sizeof(float));
@@ -688,6 +688,15 @@ __device__ __forceinline__ void fp8x8_decode(uint32_t w0, uint32_t w1, float s, 
   out[4] = fp8_byte<0>(w1) * s; out[5] = fp8_byte<1>(w1) * s; out[6] = fp8_byte<2>(w1) * s; out[7] = fp8_byte<3>(w1) * s;
 }
 
+// The value of the other lane of the pair (2k, 2k + 1) (DPP quad_perm
+// [1, 0, 3, 2]; every lane of the wave active).
+__device__ __forceinline__ uint32_t swap_pair(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint4 swap_pair(uint4 v) {
+  return make_uint4(swap_pair(v.x), swap_pair(v.y), swap_pair(v.z), swap_pair(v.w));
+}
+
 // The trilinear lookup of one cache sample by one lane: its 8 corners
 // (corner c = 4x + 2y + z; outside the frustum or in unset bricks the zero
 // voxel) summed in the reference's order ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7))
@@ -706,6 +715,7 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
   float2 dis[8];
   float scl[8];
   const Voxel* vp[8];  // FP16 pool: corner voxels (the zero voxel outside)
+  uint32_t vidx[8];    // ... their pool indices (slot * 512 + voxel)
   bool vok[8];
   // integer cell of the corner (0, 0, 0); corner coordinates are exact
   // small integers, so the frustum test is an unsigned compare
@@ -754,8 +764,9 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
       scl[c] = ok ? sc : 0.0f;
     } else {
       vok[c] = ok;
+      vidx[c] = (uint32_t)vi;
       vp[c] = ok ? cv.pool + vi : &k_zero_voxel;
-      lat[c] = *reinterpret_cast<const uint4*>(vp[c]->lat);
+      if (c >= 4) lat[c] = *reinterpret_cast<const uint4*>(vp[c]->lat);  // c < 4: pair loads below
       scl[c] = 0.0f;
     }
   }
@@ -765,12 +776,29 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
     // corner c + 4 loads its own (only those lanes access memory).  The
     // 16-byte loads pass through an empty asm once every load is issued, so
     // that they stay single loads and the masked loads stay masked.
-    f4 d4[4];
+    //
+    // Lane pairs (2k, 2k + 1) fetch the x-offset-0 voxels of their two
+    // samples together: one instruction reads both halves of the even
+    // lane's voxel (latents to the even lane, density half to the odd one),
+    // the next both halves of the odd lane's, so the two lanes of each load
+    // hit one cache line; one pair swap then hands each lane the half it
+    // needs from its partner.
+    const bool odd = (threadIdx.x & 1) != 0;
+    uint4 r1[4], r2[4];
     float2 d1[4];
     bool shared[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      d4[c] = *reinterpret_cast<const f4*>(&vp[c]->density);
+      // the partner's voxel, from its index (a pointer passed between lanes
+      // would lose its address space)
+      const uint32_t pvi = swap_pair(vidx[c]);
+      const bool pok = swap_pair((uint32_t)vok[c]) != 0u;
+      const Voxel* pv = pok ? cv.pool + pvi : &k_zero_voxel;
+      const char* a1 = reinterpret_cast<const char*>(odd ? pv : vp[c]);  // the even lane's voxel
+      const char* a2 = reinterpret_cast<const char*>(odd ? vp[c] : pv);  // the odd lane's voxel
+      const int half = odd ? 16 : 0;
+      r1[c] = *reinterpret_cast<const uint4*>(a1 + half);
+      r2[c] = *reinterpret_cast<const uint4*>(a2 + half);
       shared[c] = vok[c] && vok[c + 4] && !crx;
     }
 #pragma unroll
@@ -780,9 +808,13 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
     }
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      asm volatile("" : "+v"(d4[c]));
-      dis[c] = make_float2(d4[c][0], d4[c][1]);
-      dis[c + 4] = shared[c] ? make_float2(d4[c][2], d4[c][3]) : d1[c];
+      // even lane: r1 = its latents, r2 = the odd lane's latents (sent);
+      // odd lane: r1 = the even lane's density half (sent), r2 = its own
+      const uint4 recv = swap_pair(odd ? r1[c] : r2[c]);
+      lat[c] = odd ? recv : r1[c];
+      const uint4 dh = odd ? r2[c] : recv;  // density, isset, xnext of the own voxel
+      dis[c] = make_float2(__builtin_bit_cast(float, dh.x), __builtin_bit_cast(float, dh.y));
+      dis[c + 4] = shared[c] ? make_float2(__builtin_bit_cast(float, dh.z), __builtin_bit_cast(float, dh.w)) : d1[c];
     }
   }
   // per-corner products, then the reference's summation tree.  FP16

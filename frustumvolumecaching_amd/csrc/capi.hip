@@ -590,6 +590,7 @@ int fvc_cache_get_pool_decoded(fvc_renderer* h, int which, float* out, int64_t n
 int fvc_cache_set_capacity(fvc_renderer* h, int32_t cap) {
   return guard([&] {
     REQUIRE(h && cap >= 0, "invalid argument");
+    REQUIRE(cap <= MAX_BRICKS, "capacity above 2^23 - 1 bricks");
     h->r.capacity_override = cap;
   });
 }

@@ -159,6 +159,8 @@ void pack_fragments_vd(const uint16_t* base, const uint16_t* first, const uint16
 // ---------------------------------------------------------------- cache
 static const int BRICK = 8;               // Cache::BRICK_SIZE (cache.h:17)
 static const int BRICK_VOXELS = 512;
+// pool capacity limit: voxel indices (slot * 512 + voxel) stay below 2^32
+static const int MAX_BRICKS = (1 << 23) - 1;
 
 // voxel record of the brick pool (32 bytes).  xnext repeats density and
 // isset of the +x neighbour in the same brick row (zeros at x = 7), so that

@@ -368,7 +368,8 @@ int fvc_cache_set_latent_format(fvc_renderer* renderer, int format);
  * 8 latents, density, isset (either latent format). */
 int fvc_cache_get_pool_decoded(fvc_renderer* renderer, int which, float* out_host, int64_t n_floats);
 /* Pool capacity override in bricks (0 = reference estimate n_rays/8 rounded
- * up to 16384, cache_renderer.cu:832-846 + cache.cu:143-146). */
+ * up to 16384, cache_renderer.cu:832-846 + cache.cu:143-146); at most
+ * 2^23 - 1 bricks (FVC_E_INVALID above; the estimate is clamped to it). */
 int fvc_cache_set_capacity(fvc_renderer* renderer, int32_t capacity_bricks);
 
 /* ---- autotuning (no counterpart in the reference) ---- */
