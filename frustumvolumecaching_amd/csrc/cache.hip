@@ -525,7 +525,7 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
 // reference stops a launch when either stream reaches n_steps samples; here
 // the launch holds n_steps samples in total.  That only moves sub-iteration
 // boundaries: every ray still composites the same samples in the same order.
-static constexpr int CLS_EMIT = 4;  // samples whose t0 the emission loads together
+static constexpr int CLS_EMIT = 16;  // samples whose t0 the emission loads together
 __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int par, int trip_cutoff, MarchGrid mg,
                                                      ContractParams con, const uint32_t* __restrict__ coarse_g,
                                                      CacheView cv, RayState* __restrict__ rays, int4* __restrict__ entry,

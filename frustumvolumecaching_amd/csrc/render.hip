@@ -46,6 +46,8 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, int pix
 // unit_to_01(dir) and t0 of every occupied sample into the compacted stream.
 // The t0 of a ray's k-th sample of the launch goes to t_scr[k * n_alive + i]
 // (coalesced across the wave when read back for the stream).
+static constexpr int EMIT = 16;  // samples whose t0 the full marcher's emission loads together
+
 __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, MarchGrid mg, ContractParams con,
                                                const uint32_t* __restrict__ coarse_g, RayState* __restrict__ rays,
                                                int4* __restrict__ entry, int* __restrict__ alive_flag,
@@ -85,13 +87,13 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
   if (valid) {
     const V3 o = {rs.ox, rs.oy, rs.oz}, d = {rs.dx, rs.dy, rs.dz};
     const V3 d01 = unit_to_01(d);
-    // the t0 of 4 samples are loaded together before they are used
-    for (int k0 = 0; k0 < cnt; k0 += 4) {
-      float t0v[4];
+    // the t0 of EMIT samples are loaded together before they are used
+    for (int k0 = 0; k0 < cnt; k0 += EMIT) {
+      float t0v[EMIT];
 #pragma unroll
-      for (int u = 0; u < 4; u++) t0v[u] = t_scr[(int64_t)min(k0 + u, cnt - 1) * n_alive + i];
+      for (int u = 0; u < EMIT; u++) t0v[u] = t_scr[(int64_t)min(k0 + u, cnt - 1) * n_alive + i];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < EMIT; u++) {
         const int k = k0 + u;
         if (k >= cnt) break;
         const float t0 = t0v[u];
