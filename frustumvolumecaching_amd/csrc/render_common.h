@@ -52,7 +52,7 @@ __device__ __forceinline__ MarchRay march_ray(const RayState& s, const StepTable
 }
 
 // samples a compositing lane loads ahead
-static const int COMPOSITE_BATCH = 8;
+static const int COMPOSITE_BATCH = 16;
 
 // slots of Renderer::counters
 enum { CNT_ALIVE = 0, CNT_SAMPLES = 1, CNT_CACHE = 2, CNT_STEPS = 3, CNT_TILES = 4, CNT_SLOTS = 5, CNT_OVERFLOW = 6 };
