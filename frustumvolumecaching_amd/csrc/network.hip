@@ -393,7 +393,10 @@ __global__ void k_encode(const float* __restrict__ pos, int64_t n, const uint32_
 static int grid_for(int64_t n_groups) {
   const int64_t waves = n_groups;
   int64_t blocks = (waves + NET_BLOCK / 64 - 1) / (NET_BLOCK / 64);
-  const int64_t cap = 256 * 16;  // 256 CUs x 16 blocks: grid-stride beyond
+  // 16384 blocks (64 per CU), grid-stride beyond: measured on the 1080p path
+  // against 4096 - 65536 (more blocks help the full render; the resample
+  // launches, sized by a bound, pay for blocks that find no work)
+  const int64_t cap = 16384;
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
   return (int)blocks;
