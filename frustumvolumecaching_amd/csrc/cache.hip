@@ -528,13 +528,14 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
 static constexpr int CLS_EMIT = 16;  // samples whose t0 the emission loads together
 __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int par, int trip_cutoff, MarchGrid mg,
                                                      ContractParams con, const uint32_t* __restrict__ coarse_g,
-                                                     CacheView cv, RayState* __restrict__ rays, int4* __restrict__ entry,
+                                                     CacheView cv, RayDyn* __restrict__ rays, int4* __restrict__ entry,
                                                      int* __restrict__ alive_flag, float* __restrict__ t_scr,
                                                      float* __restrict__ s_pos, float* __restrict__ s_dir,
                                                      float* __restrict__ s_t0, float4* __restrict__ c_rec,
                                                      float* __restrict__ c_t0, const int* __restrict__ pix_in,
                                                      uint4* __restrict__ ray_sh, float4* __restrict__ ray_od,
-                                                     uint64_t* __restrict__ order_out) {
+                                                     uint64_t* __restrict__ order_out,
+                                                     const RayState* __restrict__ rconst) {
   const int n_alive = ctl[CTL_ALIVE + par];
   if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;  // whole block past the alive rays
   const int n_steps = ctl[CTL_NSTEPS];
@@ -552,8 +553,11 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
   V3 oc = {0, 0, 0}, dc = {0, 0, 0};
   int px = 0;
   if (valid) {
-    r = rays[i];
     px = pix_in[i];
+    const RayDyn dy = rays[i];
+    r = rconst[px];
+    r.t = dy.t;
+    r.j = dy.j;
     if (r.j == 0) {
       // first launch of the ray: its per-pixel tables (the colour MLP's SH
       // input depends on the direction only; split models also read the ray)
@@ -670,8 +674,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
         kc += is_c ? 1 : 0;
       }
     }
-    rays[i].t = r.t;
-    rays[i].j = r.j;
+    rays[i] = RayDyn{r.t, r.j};
     entry[i] = make_int4(off_r, nr, off_c, nc);
     order_out[i] = order;
     alive_flag[i] = alive;
@@ -1048,11 +1051,11 @@ __global__ __launch_bounds__(256) void k_cache_head_vd(const float4* __restrict_
 // the resample and cache streams of a ray by t0 (resample first iff t0_r <
 // t0_c) and composite; then compact the alive rays.
 __global__ __launch_bounds__(256) void k_cache_composite(
-    int* __restrict__ ctl, int par, fvc_stepsize_info st, float alpha_thre, const RayState* __restrict__ rays_in,
+    int* __restrict__ ctl, int par, fvc_stepsize_info st, float alpha_thre, const RayDyn* __restrict__ rays_in,
     const int* __restrict__ pix_in, const float4* __restrict__ res_in, const int4* __restrict__ entry,
     const uint64_t* __restrict__ order, const int* __restrict__ alive_flag, const uint2* __restrict__ s_out,
     const float* __restrict__ s_t0, const uint2* __restrict__ c_out, const float* __restrict__ c_t0,
-    RayState* __restrict__ rays_out, int* __restrict__ pix_out, float4* __restrict__ res_out,
+    RayDyn* __restrict__ rays_out, int* __restrict__ pix_out, float4* __restrict__ res_out,
     float4* __restrict__ results) {
   const int n_alive = ctl[CTL_ALIVE + par];
   if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;
@@ -1143,11 +1146,11 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       Handles h;
       ev = P.begin(s);
       hipLaunchKernelGGL(k_cache_march, grid, dim3(256), 0, s, ctl, cur, cutoff, mg, m.con, m.coarse.as<uint32_t>(), cv,
-                         r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
+                         r.rays[cur].as<RayDyn>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
                          r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
                          r.c_rec.as<float4>(), r.c_t0.as<float>(), r.ray_pix[cur].as<int>(), r.ray_sh.as<uint4>(),
                          m.net.split_head ? r.ray_od.as<float4>() : nullptr,
-                         r.ray_order.as<uint64_t>());
+                         r.ray_order.as<uint64_t>(), r.ray_const.as<RayState>());
       FVC_HIP(hipGetLastError());
       h.march = P.end(ev, s, "cache_march");
       ev = P.begin(s);
@@ -1180,10 +1183,10 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       h.head = P.end(ev, s, "cache_head");
       ev = P.begin(s);
       hipLaunchKernelGGL(k_cache_composite, grid, dim3(256), 0, s, ctl, cur, rm.stepsize_info, m.scene.alpha_thre,
-                         r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
+                         r.rays[cur].as<RayDyn>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
                          r.ray_entry.as<int4>(), r.ray_order.as<uint64_t>(), r.ray_alive.as<int>(),
                          r.s_out.as<uint2>(), r.s_t0.as<float>(), r.c_out.as<uint2>(), r.c_t0.as<float>(),
-                         r.rays[cur ^ 1].as<RayState>(), r.ray_pix[cur ^ 1].as<int>(),
+                         r.rays[cur ^ 1].as<RayDyn>(), r.ray_pix[cur ^ 1].as<int>(),
                          r.ray_res[cur ^ 1].as<float4>(), r.results.as<float4>());
       FVC_HIP(hipGetLastError());
       h.comp = P.end(ev, s, "cache_composite");

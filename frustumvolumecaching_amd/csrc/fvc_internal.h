@@ -352,7 +352,8 @@ struct Renderer {
   float tab_limit = 0.0f;
   int tab_rows = -1;   // -1: not built; 0: disabled for this stepsize
   // ray state (double buffered for compaction)
-  DevBuf rays[2];        // RayState
+  DevBuf rays[2];        // RayDyn of the alive list
+  DevBuf ray_const;      // RayState per pixel (frame start)
   DevBuf ray_pix[2];     // int32 pixel index
   DevBuf ray_res[2];     // float4 rgb + T
   DevBuf results;        // float4 [H*W] final

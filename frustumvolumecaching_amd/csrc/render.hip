@@ -23,9 +23,9 @@ namespace fvc {
 // Pixels of a view are numbered pix_base + y * W + x in the packed pixel
 // space of a batch of views.
 __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, int pix_base, V3 aabb_from, V3 aabb_to,
-                                                   RayState* __restrict__ rays, int* __restrict__ pix,
-                                                   float4* __restrict__ rres, float4* __restrict__ results,
-                                                   int* __restrict__ counter) {
+                                                   RayDyn* __restrict__ rays, RayState* __restrict__ rconst,
+                                                   int* __restrict__ pix, float4* __restrict__ rres,
+                                                   float4* __restrict__ results, int* __restrict__ counter) {
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
   const bool valid = x < W && y < H;
@@ -35,7 +35,8 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, int pix
   int slot[1];
   block_reserve<1>({alive ? 1 : 0}, {counter}, slot);
   if (alive) {
-    rays[slot[0]] = st;
+    rconst[pix_base + y * W + x] = st;
+    rays[slot[0]] = RayDyn{st.t, st.j};
     pix[slot[0]] = pix_base + y * W + x;
     rres[slot[0]] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   }
@@ -49,7 +50,8 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, int pix
 static constexpr int EMIT = 16;  // samples whose t0 the full marcher's emission loads together
 
 __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, MarchGrid mg, ContractParams con,
-                                               const uint32_t* __restrict__ coarse_g, RayState* __restrict__ rays,
+                                               const uint32_t* __restrict__ coarse_g, RayDyn* __restrict__ rays,
+                                               const RayState* __restrict__ rconst, const int* __restrict__ pix_in,
                                                int4* __restrict__ entry, int* __restrict__ alive_flag,
                                                float* __restrict__ t_scr, float* __restrict__ s_pos,
                                                float* __restrict__ s_dir, float* __restrict__ s_t0) {
@@ -66,7 +68,10 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
   bool alive = false;
   RayState rs{};
   if (valid) {
-    rs = rays[i];
+    const RayDyn dy = rays[i];
+    rs = rconst[pix_in[i]];
+    rs.t = dy.t;
+    rs.j = dy.j;
     alive = true;
     MarchRay r = march_ray(rs, mg.tab);
     while (cnt < n_steps) {
@@ -106,8 +111,7 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
         s_t0[q] = t0;
       }
     }
-    rays[i].t = rs.t;
-    rays[i].j = rs.j;
+    rays[i] = RayDyn{rs.t, rs.j};
     entry[i] = make_int4(off, cnt, 0, 0);
     alive_flag[i] = alive;
   }
@@ -118,11 +122,11 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
 // of a ray are read in batches of COMPOSITE_BATCH independent loads ahead of
 // the sequential compositing.
 __global__ __launch_bounds__(256) void k_composite(int* __restrict__ ctl, int par, fvc_stepsize_info st,
-                                                   float alpha_thre, const RayState* __restrict__ rays_in,
+                                                   float alpha_thre, const RayDyn* __restrict__ rays_in,
                                                    const int* __restrict__ pix_in, const float4* __restrict__ res_in,
                                                    const int4* __restrict__ entry, const int* __restrict__ alive_flag,
                                                    const uint2* __restrict__ s_out, const float* __restrict__ s_t0,
-                                                   RayState* __restrict__ rays_out, int* __restrict__ pix_out,
+                                                   RayDyn* __restrict__ rays_out, int* __restrict__ pix_out,
                                                    float4* __restrict__ res_out, float4* __restrict__ results) {
   const int n_alive = ctl[CTL_ALIVE + par];
   if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;
@@ -225,11 +229,12 @@ void ensure_ray_capacity(Renderer& r, int64_t n, int64_t target) {
   // stream, and never more than n * max_steps
   const int64_t cap = std::min<int64_t>(std::max<int64_t>(target, n), n * r.max_steps) + 64;
   for (int b = 0; b < 2; b++) {
-    r.rays[b].ensure(n * sizeof(RayState));
+    r.rays[b].ensure(n * sizeof(RayDyn));
     r.ray_pix[b].ensure(n * sizeof(int));
     r.ray_res[b].ensure(n * sizeof(float4));
   }
   r.results.ensure(n * sizeof(float4));
+  r.ray_const.ensure(n * sizeof(RayState));
   r.ray_entry.ensure(n * sizeof(int4));
   r.ray_order.ensure(n * sizeof(uint64_t));
   r.t_scratch.ensure(n * 2 * MAX_STEPS_PER_LAUNCH * sizeof(float));
@@ -360,8 +365,9 @@ void launch_init_rays_async(Renderer& r, const Views& vs, hipStream_t s) {
     const fvc_raymarch_info& rm = vs.rms[v];
     const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
     hipLaunchKernelGGL(k_init_rays, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, (int)vs.base[v],
-                       cvt(m.scene.aabb_from), cvt(m.scene.aabb_to), r.rays[0].as<RayState>(), r.ray_pix[0].as<int>(),
-                       r.ray_res[0].as<float4>(), r.results.as<float4>(), r.counters.as<int>() + CTL_ALIVE);
+                       cvt(m.scene.aabb_from), cvt(m.scene.aabb_to), r.rays[0].as<RayDyn>(),
+                       r.ray_const.as<RayState>(), r.ray_pix[0].as<int>(), r.ray_res[0].as<float4>(),
+                       r.results.as<float4>(), r.counters.as<int>() + CTL_ALIVE);
     FVC_HIP(hipGetLastError());
   }
 }
@@ -479,8 +485,9 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       Handles h;
       ev = P.begin(s);
       hipLaunchKernelGGL(k_march, grid, dim3(256), 0, s, ctl, cur, mg, m.con, m.coarse.as<uint32_t>(),
-                         r.rays[cur].as<RayState>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
-                         r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>());
+                         r.rays[cur].as<RayDyn>(), r.ray_const.as<RayState>(), r.ray_pix[cur].as<int>(),
+                         r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.t_scratch.as<float>(), r.s_pos.as<float>(),
+                         r.s_dir.as<float>(), r.s_t0.as<float>());
       FVC_HIP(hipGetLastError());
       h.march = P.end(ev, s, "march");
       ev = P.begin(s);
@@ -489,9 +496,9 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       h.net = P.end(ev, s, "network_full");
       ev = P.begin(s);
       hipLaunchKernelGGL(k_composite, grid, dim3(256), 0, s, ctl, cur, rm.stepsize_info, m.scene.alpha_thre,
-                         r.rays[cur].as<RayState>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
+                         r.rays[cur].as<RayDyn>(), r.ray_pix[cur].as<int>(), r.ray_res[cur].as<float4>(),
                          r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.s_out.as<uint2>(), r.s_t0.as<float>(),
-                         r.rays[cur ^ 1].as<RayState>(), r.ray_pix[cur ^ 1].as<int>(),
+                         r.rays[cur ^ 1].as<RayDyn>(), r.ray_pix[cur ^ 1].as<int>(),
                          r.ray_res[cur ^ 1].as<float4>(), r.results.as<float4>());
       FVC_HIP(hipGetLastError());
       h.comp = P.end(ev, s, "composite");

@@ -15,6 +15,13 @@ struct RayState {
   float ox, oy, oz, dx, dy, dz, far_, t;
   int m, j, pad0, pad1;
 };
+// Where a ray stands between sub-iterations: next bin start t == T[m][j].
+// The alive list compacts this and the pixel index; the rest of the ray
+// (RayState, written once per frame at its pixel) stays in place.
+struct RayDyn {
+  float t;
+  int j;
+};
 
 // Ray of pixel (x, y) clipped to the scene AABB, its start snapped by
 // valid_t_from_t with the bin index kept: t = t_from_step(m + near_i)
