@@ -109,6 +109,13 @@ class InitStats(C.Structure):
 
 FILL_IDLE, FILL_RUNNING, FILL_DONE = 0, 1, 2
 
+# DensityInterpolVariant (cache_renderer.h:15-22), InterpolFunction
+# (common.h:221-227), CachePerformanceRenderer::RenderMode
+# (cache_performance_renderer.h:23-28)
+INTERPOL_DENSITY, INTERPOL_DENSITY_INTERMEDIATES = 0, 3
+FN_LINEAR, FN_SMOOTHSTEP, FN_SMOOTHERSTEP, FN_NEAREST = 0, 1, 2, 3
+RENDER_BOTH, RENDER_ONLY_CACHE, RENDER_ONLY_RESAMPLED = 0, 1, 2
+
 
 class RefillPolicy(C.Structure):
     """fvc_refill_policy (the viewer's m_cache_miss_limit / m_cache_update_distance)."""

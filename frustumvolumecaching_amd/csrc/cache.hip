@@ -360,15 +360,19 @@ __global__ __launch_bounds__(64) void k_fill_known(int W, int H, int bx, fvc_ste
 
 // ------------------------------------------------------------------ geometry
 // Frustum depth in steps: the reference's max_step_in_scene
-// (raymarch_common.h:298-303, cache_renderer.cu:841-843) padded to 4,
-// extended to the farthest AABB corner seen from the fill camera (plus one
-// slab) so a camera outside the AABB keeps every brick inside the frustum.
+// (raymarch_common.h:298-303, cache_renderer.cu:841-843), ceil(step_from_t(
+// |AABB diagonal| + near)) padded to 4.  A camera inside the AABB sees no
+// point farther than the diagonal, so this is the reference's depth exactly.
+// A fill camera outside the AABB can see points up to its farthest AABB
+// corner; the reference fill has no depth guard there (its slab walk,
+// cache_renderer.cu:130-178, would address bricks past the frustum), so the
+// diagonal is replaced by that corner's distance (at most far) whenever it is
+// longer, and every brick stays inside the frustum.
 static int frustum_depth(const fvc_raymarch_info& rm, const fvc_scene_info& sc) {
   const fvc_stepsize_info& st = rm.stepsize_info;
   const V3 from = cvt(sc.aabb_from), to = cvt(sc.aabb_to);
   const V3 dg = to - from;
   const float diag = std::sqrt(dot3(dg, dg));
-  int steps = (int)std::ceil(step_from_t(st, diag + st.near_));
   const V3 o = translation(rm.cam_info.cam2world);
   float tf = 0.0f;
   for (int c = 0; c < 8; c++) {
@@ -377,8 +381,8 @@ static int frustum_depth(const fvc_raymarch_info& rm, const fvc_scene_info& sc) 
     tf = std::max(tf, std::sqrt(dot3(v, v)));
   }
   tf = std::min(tf, st.far_);
-  const int steps_cam = (int)std::ceil(step_from_t(st, tf)) + BRICK;
-  steps = std::max(steps, steps_cam);
+  const float reach = tf > diag ? tf : diag;
+  const int steps = (int)std::ceil(step_from_t(st, reach + st.near_));
   return (steps + 3) / 4 * 4;
 }
 
@@ -394,8 +398,17 @@ static int default_capacity(int64_t n_rays) {
 // The fill keeps its own counters, pinned mirror and scratch, so it can run
 // on the fill stream from another thread while frames render from the
 // other cache (double-buffered renderers).
+// CachePerformanceRenderer's constructor check (cache_performance_renderer.cu:
+// 964-967): the cache holds 8 latents (2 half4 arrays)
+static void check_cache_latents(const Model& m) {
+  if (m.net.n_latents > 8)
+    throw Error(FVC_E_UNSUPPORTED,
+                "Latent width of the model is larger than the (hard-coded) cache size of 8 latents");
+}
+
 void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats, hipStream_t s, Profiler& P) {
   const Model& m = *r.model;
+  check_cache_latents(m);
   CacheStore& C = r.caches[r.fill_idx];
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
   r.f_counters.ensure(CTL_LOG * sizeof(int));
@@ -433,7 +446,7 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
   if (split) r.f_dir.ensure(f_cap * 3 * sizeof(float));
   // rows of the network output: log sigma and the first stored latent
   const int sig_row = split ? 8 : 0, lat_row = split ? 0 : m.net.latent_offset;
-  const int di = split && m.net.first_latent_is_density;
+  const int di = m.net.first_latent_is_density ? 1 : 0;
 
   FVC_HIP(hipMemsetAsync(r.f_counters.p, 0, CTL_LOG * sizeof(int), s));
   hipEvent_t ev = P.begin(s);
@@ -525,6 +538,11 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
 // reference stops a launch when either stream reaches n_steps samples; here
 // the launch holds n_steps samples in total.  That only moves sub-iteration
 // boundaries: every ray still composites the same samples in the same order.
+// render_mode ONLY_CACHE / ONLY_RESAMPLED drops the samples of the other
+// class (cache_performance_renderer.cu:117-118, 125-126).
+// dbg_codes (optional, inspection only): the class of every occupied sample
+// of the ray in march order (0 skip, 1 resample, 2 cache), the first dbg_k
+// at dbg_codes[pixel * dbg_k ..], dbg_n[pixel] counting them across launches.
 static constexpr int CLS_EMIT = 16;  // samples whose t0 the emission loads together
 __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int par, int trip_cutoff, MarchGrid mg,
                                                      ContractParams con, const uint32_t* __restrict__ coarse_g,
@@ -535,7 +553,9 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
                                                      float* __restrict__ c_t0, const int* __restrict__ pix_in,
                                                      uint4* __restrict__ ray_sh, float4* __restrict__ ray_od,
                                                      uint64_t* __restrict__ order_out,
-                                                     const RayState* __restrict__ rconst) {
+                                                     const RayState* __restrict__ rconst, int render_mode,
+                                                     int8_t* __restrict__ dbg_codes, int dbg_k,
+                                                     int* __restrict__ dbg_n) {
   const int n_alive = ctl[CTL_ALIVE + par];
   if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;  // whole block past the alive rays
   const int n_steps = ctl[CTL_NSTEPS];
@@ -626,8 +646,11 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
       for (int u = 0; u < CLS; u++) {
         if (k0 + u >= cnt) break;
         const bool unsampled = !in_fov[u] || tf[u] < cv.st.near_ || tf[u] > known[u];
-        if (!inside[u] || unsampled) res_mask |= 1ull << (k0 + u);
-        else if (slot[u] >= 0 && dv[u] > 0.0f) cache_mask |= 1ull << (k0 + u);
+        if (!inside[u] || unsampled) {
+          if (render_mode != FVC_RENDER_ONLY_CACHE) res_mask |= 1ull << (k0 + u);
+        } else if (render_mode != FVC_RENDER_ONLY_RESAMPLED && slot[u] >= 0 && dv[u] > 0.0f) {
+          cache_mask |= 1ull << (k0 + u);
+        }
       }
     }
   }
@@ -678,6 +701,14 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
     entry[i] = make_int4(off_r, nr, off_c, nc);
     order_out[i] = order;
     alive_flag[i] = alive;
+    if (dbg_codes) {
+      const int base = dbg_n[px];
+      for (int k = 0; k < cnt; k++)
+        if (base + k < dbg_k)
+          dbg_codes[(int64_t)px * dbg_k + base + k] =
+              (int8_t)(((res_mask >> k) & 1ull) ? 1 : (((cache_mask >> k) & 1ull) ? 2 : 0));
+      dbg_n[px] = base + cnt;
+    }
   }
 }
 
@@ -705,12 +736,39 @@ __device__ __forceinline__ uint4 swap_pair(uint4 v) {
 // voxel) summed in the reference's order ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7))
 // (cache_performance_renderer.cu:736-787): wd = sum w*isset, dd = sum
 // w*sigma, sums[k] = sum w*l_k.  rec = (froxel x, y, z, -).
+// applyInterpolationFunction (raymarch_common.h:393-402) of one fraction:
+// smoothstep x*x*(3 - 2x), smootherstep x*x*x*(x*(6x - 15) + 10)
+// (helper_math_extension.h:146-170, the same evaluation order), nearest =
+// round half to even.
+__device__ __forceinline__ float interp_weight(float t, int fn) {
+  if (fn == FVC_FN_SMOOTHSTEP) return (t * t) * (3.0f - 2.0f * t);
+  if (fn == FVC_FN_SMOOTHERSTEP) return ((t * t) * t) * (t * (6.0f * t - 15.0f) + 10.0f);
+  if (fn == FVC_FN_NEAREST) return __builtin_rintf(t);
+  return t;
+}
+// use_inter_brick_interpolation = false: the coordinate clamped into
+// [8b + 1/2, 8b + 7.5] of its brick b (froxel2data, raymarch_common.h:341-357)
+__device__ __forceinline__ float clamp_to_brick(float x) {
+  const float b = floorf(x * 0.125f) * 8.0f;
+  return b + fminf_(fmaxf_(x - b, 0.5f), 7.5f);
+}
+
 template <bool LAT8>
-__device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 rec, float& wd, float& dd,
+__device__ __forceinline__ void lookup_sample(const CacheView& cv, float4 rec, float& wd, float& dd,
                                               float* sums) {
+  if (!cv.inter_brick) {
+    rec.x = clamp_to_brick(rec.x);
+    rec.y = clamp_to_brick(rec.y);
+    rec.z = clamp_to_brick(rec.z);
+  }
   const float fcx = rec.x - 0.5f, fcy = rec.y - 0.5f, fcz = rec.z - 0.5f;
   const float ix = floorf(fcx), iy = floorf(fcy), iz = floorf(fcz);
-  const float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
+  float lx = fcx - ix, ly = fcy - iy, lz = fcz - iz;
+  if (cv.interp_fn != FVC_FN_LINEAR) {
+    lx = interp_weight(lx, cv.interp_fn);
+    ly = interp_weight(ly, cv.interp_fn);
+    lz = interp_weight(lz, cv.interp_fn);
+  }
   // weight factors (1 - off)(1 - 2l) + l of offsets 0 and 1
   const float wxs[2] = {(1.0f - 2.0f * lx) + lx, lx}, wys[2] = {(1.0f - 2.0f * ly) + ly, ly};
   const float wzs[2] = {(1.0f - 2.0f * lz) + lz, lz};
@@ -849,6 +907,34 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
   for (int k = 0; k < 8; k++) sums[k] = tree(cl[k]);
 }
 
+// Head inputs of one cache sample from its lookup: the 8 latents
+// Sum w*l / Sum w*isset and the density row log(max(Sum w*sigma, 1e-6)),
+// rounded to half (generateNetworkInput_cache_shfl_perf_kernel_cg,
+// cache_performance_renderer.cu:736-787).  di (DensityIntermediates): latent
+// 0 holds the density, latent 0 = log(Sum w*l0) * W / W with W = Sum w*isset
+// (770-779, no clamp), and it is also the density row.  Sum w*isset >= 1/8
+// for every interpolation function but nearest (the voxel holding the sample
+// is set and weighs at least 1/2 per axis), so one correctly rounded
+// reciprocal serves the 8 divisions; nearest weights are 0 or 1 and divide
+// as IEEE division does (0/0 included).
+template <bool LAT8>
+__device__ __forceinline__ void cache_head_inputs(const CacheView& cv, float4 rec, int di, _Float16* L,
+                                                  _Float16& logsig) {
+  float wd, dd, sums[8];
+  lookup_sample<LAT8>(cv, rec, wd, dd, sums);
+  if (cv.interp_fn == FVC_FN_NEAREST) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) L[k] = (_Float16)(sums[k] / wd);
+    if (di) L[0] = (_Float16)((logf(sums[0]) * wd) / wd);
+  } else {
+    const float rw = 1.0f / wd;
+#pragma unroll
+    for (int k = 0; k < 8; k++) L[k] = (_Float16)div_recip(sums[k], wd, rw);
+    if (di) L[0] = (_Float16)div_recip(logf(sums[0]) * wd, wd, rw);
+  }
+  logsig = di ? L[0] : (_Float16)logf(fmaxf_(dd, 1e-6f));
+}
+
 // One lane per sample: a wave takes 64 cache samples.  Each lane fetches
 // its sample's 8 corners (corner c = 4x + 2y + z, outside the frustum or in
 // unset bricks the zero voxel), sums them in the reference's order
@@ -862,11 +948,15 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, const float4 
 // decoded as code * brick scale.
 static const int HEAD_REC = 5;  // uint4 per sample record in LDS: 4 operand groups + log sigma (80 bytes)
 
+// The colour head's first 16 inputs are the density-MLP output rows in the
+// layout of the model (sc.lat_lo, sc.lat_hi): latents in rows [lo, lo + n),
+// the constant 1.0 elsewhere; di: DensityIntermediates (the reference's plain
+// models, lo = 0).
 template <int P, bool LAT8>
 __global__ __launch_bounds__(256) void k_cache_head(const float4* __restrict__ c_rec, const uint4* __restrict__ ray_sh,
                                                     int64_t n, CacheView cv,
                                                     const typename Mx<P>::T* __restrict__ frags, MlpScales sc,
-                                                    uint2* __restrict__ c_out, const int* __restrict__ n_dev) {
+                                                    int di, uint2* __restrict__ c_out, const int* __restrict__ n_dev) {
   // n_dev: sample count written by the marcher (n is then its bound)
   if (n_dev) n = *n_dev;
   if ((int64_t)blockIdx.x * 256 >= n) return;
@@ -892,25 +982,25 @@ __global__ __launch_bounds__(256) void k_cache_head(const float4* __restrict__ c
     rec_next = c_rec[min((grp + gstride) * 64 + lane, n32 - 1)];
     const int px = __builtin_bit_cast(int, rec.w);
     const uint4 sh_lo = ray_sh[2 * (int64_t)px], sh_hi = ray_sh[2 * (int64_t)px + 1];
-    float wd, dd, sums[8];
-    lookup_sample<LAT8>(cv, rec, wd, dd, sums);
-    _Float16 L[8];
-    // sums[k] / wd (wd >= 1/8: the voxel holding the sample is set and
-    // weighs at least 1/2 per axis)
-    const float rw = 1.0f / wd;
-#pragma unroll
-    for (int k = 0; k < 8; k++) L[k] = (_Float16)div_recip(sums[k], wd, rw);
-    const _Float16 logsig = (_Float16)logf(fmaxf_(dd, 1e-6f));
+    _Float16 L[8], logsig;
+    cache_head_inputs<LAT8>(cv, rec, di, L, logsig);
     auto pk = [](_Float16 a, _Float16 b) {
       return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
     };
     const _Float16 h1 = (_Float16)1.0f;
-    // operand groups: base-output rows 4g..4g+3 (rows 1..8 = latents, the rest
-    // the padding 1.0) and SH coefficients 4g..4g+3
+    // operand groups: base-output rows 4g..4g+3 (latents in rows [lo, lo + n),
+    // the rest the padding 1.0; lo = 1 with n = 8, or lo = 0 with n <= 8) and
+    // SH coefficients 4g..4g+3
+    const int lo = sc.lat_lo, hi = sc.lat_hi;
+    _Float16 R[9];
+    R[0] = lo ? h1 : L[0];
+#pragma unroll
+    for (int r = 1; r < 8; r++) R[r] = r < hi ? (lo ? L[r - 1] : L[r]) : h1;
+    R[8] = hi > 8 ? L[7] : h1;
     uint4* r5 = recw + lane * HEAD_REC;
-    r5[0] = make_uint4(pk(h1, L[0]), pk(L[1], L[2]), sh_lo.x, sh_lo.y);
-    r5[1] = make_uint4(pk(L[3], L[4]), pk(L[5], L[6]), sh_lo.z, sh_lo.w);
-    r5[2] = make_uint4(pk(L[7], h1), one2, sh_hi.x, sh_hi.y);
+    r5[0] = make_uint4(pk(R[0], R[1]), pk(R[2], R[3]), sh_lo.x, sh_lo.y);
+    r5[1] = make_uint4(pk(R[4], R[5]), pk(R[6], R[7]), sh_lo.z, sh_lo.w);
+    r5[2] = make_uint4(pk(R[8], h1), one2, sh_hi.x, sh_hi.y);
     r5[3] = make_uint4(one2, one2, sh_hi.z, sh_hi.w);
     r5[4] = make_uint4((uint32_t)__builtin_bit_cast(uint16_t, logsig), 0u, 0u, 0u);
     wave_sync();
@@ -991,14 +1081,8 @@ __global__ __launch_bounds__(256) void k_cache_head_vd(const float4* __restrict_
     const int px = __builtin_bit_cast(int, rec.w);
     const uint4 sh_lo = ray_sh[2 * (int64_t)px], sh_hi = ray_sh[2 * (int64_t)px + 1];
     const float4 ro = ray_od[2 * (int64_t)px], rd = ray_od[2 * (int64_t)px + 1];
-    float wd, dd, sums[8];
-    lookup_sample<false>(cv, rec, wd, dd, sums);
-    _Float16 L[8];
-    const float rw = 1.0f / wd;  // as in k_cache_head
-#pragma unroll
-    for (int k = 0; k < 8; k++) L[k] = (_Float16)div_recip(sums[k], wd, rw);
-    if (di) L[0] = (_Float16)div_recip(logf(sums[0]) * wd, wd, rw);
-    const _Float16 logsig = di ? L[0] : (_Float16)logf(fmaxf_(dd, 1e-6f));
+    _Float16 L[8], logsig;
+    cache_head_inputs<false>(cv, rec, di, L, logsig);
     // the sample's midpoint (generateNetworkInput_preparation_perf_kernel, 285-291)
     const V3 o = {ro.x, ro.y, ro.z}, d = {rd.x, rd.y, rd.z};
     const float tm = t0 + step_dt(st, t0) * 0.5f;
@@ -1056,7 +1140,9 @@ __global__ __launch_bounds__(256) void k_cache_composite(
     const uint64_t* __restrict__ order, const int* __restrict__ alive_flag, const uint2* __restrict__ s_out,
     const float* __restrict__ s_t0, const uint2* __restrict__ c_out, const float* __restrict__ c_t0,
     RayDyn* __restrict__ rays_out, int* __restrict__ pix_out, float4* __restrict__ res_out,
-    float4* __restrict__ results) {
+    float4* __restrict__ results, int2* __restrict__ dbg_comp) {
+  // dbg_comp (optional, inspection only): per pixel the resampled and cached
+  // samples composited, up to and including the one that ends the ray
   const int n_alive = ctl[CTL_ALIVE + par];
   if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;
   int* counter = ctl + CTL_ALIVE + (par ^ 1);
@@ -1075,6 +1161,7 @@ __global__ __launch_bounds__(256) void k_cache_composite(
     const int n = e.y + e.w;
     bool visible = true;  // transmittance still above the early-stop threshold
     int kr = 0, kc = 0;
+    int n_done = 0;       // samples composited
     // the two streams interleaved in march (= t0) order, batch by batch
     for (int k0 = 0; k0 < n && visible; k0 += COMPOSITE_BATCH) {
       uint2 v[COMPOSITE_BATCH];
@@ -1092,10 +1179,19 @@ __global__ __launch_bounds__(256) void k_cache_composite(
       }
 #pragma unroll
       for (int u = 0; u < COMPOSITE_BATCH; u++)
-        if (visible && k0 + u < n) visible = composite_sample(res, v[u], t[u], st, alpha_thre);
+        if (visible && k0 + u < n) {
+          visible = composite_sample(res, v[u], t[u], st, alpha_thre);
+          n_done++;
+        }
     }
     keep = visible && ray_alive != 0;
     if (!keep) results[px] = res;
+    if (dbg_comp) {
+      int cc = 0;
+      for (int k = 0; k < n_done; k++) cc += (int)((ord >> k) & 1u);
+      const int2 d = dbg_comp[px];
+      dbg_comp[px] = make_int2(d.x + n_done - cc, d.y + cc);
+    }
   }
   int slot[1];
   block_reserve<1>({keep ? 1 : 0}, {counter}, slot);
@@ -1108,6 +1204,7 @@ __global__ __launch_bounds__(256) void k_cache_composite(
 
 void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   const Model& m = *r.model;
+  check_cache_latents(m);
   const CacheStore& C = r.caches[r.render_idx];
   if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
   prepare_views(r, vs);
@@ -1120,12 +1217,16 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   r.c_t0.ensure(cap * sizeof(float));
   r.c_out.ensure(cap * sizeof(uint2));
   hipStream_t s = r.stream();
-  const CacheView cv = cache_view(C);
+  CacheView cv = cache_view(C);
+  cv.interp_fn = r.settings.interpol_function;
+  cv.inter_brick = r.settings.use_inter_brick_interpolation ? 1 : 0;
+  // DensityIntermediates for the reference's plain models (latent 0 = density)
+  const int di_plain = !m.net.split_head && m.net.first_latent_is_density ? 1 : 0;
   const MarchGrid mg = march_grid(r, vs.rms, vs.n);
 
   Profiler& P = r.prof;
   // graph mode as in render_full
-  const bool graph = r.use_graph && r.graph_iters[1] > 0 && !P.enabled;
+  const bool graph = r.use_graph && r.graph_iters[1] > 0 && !P.enabled && !r.dbg_codes;
   hipEvent_t ev = P.begin(s);
   const int n0 = graph ? (int)vs.n_px : launch_init_rays(r, vs, s);
   P.work(P.end(ev, s, "init_rays"), n_px, n_px * 16 + n0 * 60.0, 0);
@@ -1150,7 +1251,8 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
                          r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
                          r.c_rec.as<float4>(), r.c_t0.as<float>(), r.ray_pix[cur].as<int>(), r.ray_sh.as<uint4>(),
                          m.net.split_head ? r.ray_od.as<float4>() : nullptr,
-                         r.ray_order.as<uint64_t>(), r.ray_const.as<RayState>());
+                         r.ray_order.as<uint64_t>(), r.ray_const.as<RayState>(), r.render_mode, r.dbg_codes, r.dbg_k,
+                         r.dbg_n);
       FVC_HIP(hipGetLastError());
       h.march = P.end(ev, s, "cache_march");
       ev = P.begin(s);
@@ -1161,7 +1263,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
 #define L2(P, F8)                                                                                                    \
   hipLaunchKernelGGL((k_cache_head<P, F8>), dim3(head_blocks), dim3(256), 0, s, r.c_rec.as<float4>(),                 \
                      r.ray_sh.as<uint4>(), (int64_t)n_bound, cv, m.frags.as<typename Mx<P>::T>(), m.mlp_scales,       \
-                     r.c_out.as<uint2>(), ctl + CTL_C)
+                     di_plain, r.c_out.as<uint2>(), ctl + CTL_C)
 #define L(P)                     \
   if (cv.lat8) { L2(P, true); }  \
   else { L2(P, false); }
@@ -1187,7 +1289,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
                          r.ray_entry.as<int4>(), r.ray_order.as<uint64_t>(), r.ray_alive.as<int>(),
                          r.s_out.as<uint2>(), r.s_t0.as<float>(), r.c_out.as<uint2>(), r.c_t0.as<float>(),
                          r.rays[cur ^ 1].as<RayDyn>(), r.ray_pix[cur ^ 1].as<int>(),
-                         r.ray_res[cur ^ 1].as<float4>(), r.results.as<float4>());
+                         r.ray_res[cur ^ 1].as<float4>(), r.results.as<float4>(), r.dbg_comp);
       FVC_HIP(hipGetLastError());
       h.comp = P.end(ev, s, "cache_composite");
       hs.push_back(h);
@@ -1250,6 +1352,82 @@ void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_
 
 void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats) {
   cache_fill(r, rm, stats, r.stream(), r.prof);
+}
+
+// ---------------------------------------------------------------- inspection
+// The cache head's lookup alone at given froxels (one lane per froxel, full
+// waves: the lookup exchanges halves between lane pairs): [density row,
+// latents 0..7] per froxel, as the head receives them.
+template <bool LAT8>
+__global__ __launch_bounds__(256) void k_debug_lookup(const float* __restrict__ fr, int n, CacheView cv, int di,
+                                                      float* __restrict__ out9) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = i < n ? i : n - 1;
+  const float4 rec = make_float4(fr[3 * j], fr[3 * j + 1], fr[3 * j + 2], 0.0f);
+  _Float16 L[8], logsig;
+  cache_head_inputs<LAT8>(cv, rec, di, L, logsig);
+  if (i < n) {
+    out9[9 * (int64_t)i] = (float)logsig;
+#pragma unroll
+    for (int k = 0; k < 8; k++) out9[9 * (int64_t)i + 1 + k] = (float)L[k];
+  }
+}
+
+void debug_lookup(Renderer& r, int which, const float* fr_host, int64_t n, float* out_host) {
+  const Model& m = *r.model;
+  const CacheStore& C = r.caches[which == 0 ? r.render_idx : r.fill_idx];
+  if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
+  if (n <= 0) return;
+  if (n > (1 << 24)) throw Error(FVC_E_INVALID, "at most 2^24 froxels per call");
+  hipStream_t s = r.stream();
+  CacheView cv = cache_view(C);
+  cv.interp_fn = r.settings.interpol_function;
+  cv.inter_brick = r.settings.use_inter_brick_interpolation ? 1 : 0;
+  DevBuf f, o;
+  f.ensure(n * 3 * sizeof(float));
+  o.ensure(n * 9 * sizeof(float));
+  FVC_HIP(hipMemcpyAsync(f.p, fr_host, n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  const int di = m.net.first_latent_is_density ? 1 : 0;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (cv.lat8)
+    hipLaunchKernelGGL(k_debug_lookup<true>, grid, dim3(256), 0, s, f.as<float>(), (int)n, cv, di, o.as<float>());
+  else
+    hipLaunchKernelGGL(k_debug_lookup<false>, grid, dim3(256), 0, s, f.as<float>(), (int)n, cv, di, o.as<float>());
+  FVC_HIP(hipGetLastError());
+  FVC_HIP(hipMemcpyAsync(out_host, o.p, n * 9 * sizeof(float), hipMemcpyDeviceToHost, s));
+  FVC_HIP(hipStreamSynchronize(s));
+}
+
+// A cached render with the per-pixel inspection records (see k_cache_march
+// and k_cache_composite): codes [pixels][k] (-1 past the ray's classified
+// samples), classified [pixels], composited [pixels][2].
+void debug_render_cached(Renderer& r, const fvc_raymarch_info& rm, int k, int8_t* codes_host, int32_t* classified_host,
+                         int32_t* composited_host, fvc_render_stats* stats) {
+  const int64_t n = (int64_t)rm.cam_info.res_x * rm.cam_info.res_y;
+  if (k < 0 || (int64_t)k * n > ((int64_t)1 << 30)) throw Error(FVC_E_INVALID, "record too large");
+  hipStream_t s = r.stream();
+  DevBuf codes, cls, comp;
+  codes.ensure(std::max<int64_t>(1, n * k));
+  cls.ensure(n * sizeof(int));
+  comp.ensure(n * sizeof(int2));
+  FVC_HIP(hipMemsetAsync(codes.p, 0xff, std::max<int64_t>(1, n * k), s));
+  FVC_HIP(hipMemsetAsync(cls.p, 0, n * sizeof(int), s));
+  FVC_HIP(hipMemsetAsync(comp.p, 0, n * sizeof(int2), s));
+  r.dbg_codes = codes.as<int8_t>();
+  r.dbg_k = k;
+  r.dbg_n = cls.as<int>();
+  r.dbg_comp = comp.as<int2>();
+  try {
+    render_cached(r, rm, nullptr, stats);
+  } catch (...) {
+    r.dbg_codes = nullptr; r.dbg_n = nullptr; r.dbg_comp = nullptr; r.dbg_k = 0;
+    throw;
+  }
+  r.dbg_codes = nullptr; r.dbg_n = nullptr; r.dbg_comp = nullptr; r.dbg_k = 0;
+  if (codes_host && k > 0) FVC_HIP(hipMemcpyAsync(codes_host, codes.p, n * k, hipMemcpyDeviceToHost, s));
+  if (classified_host) FVC_HIP(hipMemcpyAsync(classified_host, cls.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (composited_host) FVC_HIP(hipMemcpyAsync(composited_host, comp.p, n * sizeof(int2), hipMemcpyDeviceToHost, s));
+  FVC_HIP(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------- refill

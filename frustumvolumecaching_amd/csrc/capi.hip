@@ -11,8 +11,8 @@
 namespace fvc {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& m) { g_last_error = m; }
-void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, std::vector<uint8_t>& out,
-                    MlpScales& sc);
+void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, int lat_lo, int n_lat,
+                    std::vector<uint8_t>& out, MlpScales& sc);
 }  // namespace fvc
 
 using namespace fvc;
@@ -133,7 +133,7 @@ static void create_model(const fvc_scene_info* scene, const fvc_net_desc* net, c
       FVC_HIP(hipMemcpy(m.table.p, p_pos, n_pos * sizeof(uint16_t), hipMemcpyHostToDevice));
       std::vector<uint8_t> frags;
       if (net->split_head) pack_fragments_vd(p_base, p_first, p_head, *net, frags);
-      else pack_fragments(p_base, p_head, net->precision, frags, m.mlp_scales);
+      else pack_fragments(p_base, p_head, net->precision, net->latent_offset, net->n_latents, frags, m.mlp_scales);
       m.frags.ensure(frags.size());
       FVC_HIP(hipMemcpy(m.frags.p, frags.data(), frags.size(), hipMemcpyHostToDevice));
       m.occ = occ_params(*scene);
@@ -291,7 +291,7 @@ int fvc_renderer_create(fvc_model* model, int double_buffered, fvc_renderer** ou
     auto* h = new fvc_renderer();
     h->r.model = &model->m;
     // the interpolation variant eval.cu:216 picks for the model
-    if (model->m.net.split_head && model->m.net.first_latent_is_density)
+    if (model->m.net.first_latent_is_density)
       h->r.settings.interpol_variant = FVC_INTERPOL_DENSITY_INTERMEDIATES;
     h->r.double_buffered = double_buffered != 0;
     h->r.fill_idx = h->r.double_buffered ? 1 : 0;
@@ -370,6 +370,8 @@ int fvc_render_full(fvc_renderer* h, const fvc_raymarch_info* rm, uint8_t* rgba,
   return guard([&] {
     REQUIRE(h, "null renderer");
     check_rm(rm);
+    // the full render shares the ray and sample buffers of cached renders
+    std::scoped_lock lk(h->r.read_mutex);
     render_full(h->r, *rm, rgba, stats);
   });
 }
@@ -412,13 +414,25 @@ int fvc_render_cached_batch(fvc_renderer* h, const fvc_raymarch_info* rms, int32
   });
 }
 
+// The differentiable render covers the separate-density layout (log sigma
+// in row 0, 8 latents after it, FP16 model parameters) with the default
+// cache lookup (linear, inter-brick, both sample classes).
+static void check_diff_model(const Renderer& r, bool cached) {
+  const fvc_net_desc& n = r.model->net;
+  if (n.split_head || n.latent_offset != 1)
+    throw Error(FVC_E_UNSUPPORTED,
+                "the differentiable render covers models with a separate density row (latent_offset 1), no split head");
+  if (cached && (r.settings.interpol_function != FVC_FN_LINEAR || !r.settings.use_inter_brick_interpolation ||
+                 r.render_mode != FVC_RENDER_BOTH))
+    throw Error(FVC_E_UNSUPPORTED, "the differentiable cached render uses the default cache settings and render mode");
+}
+
 int fvc_diff_forward(fvc_renderer* h, const fvc_raymarch_info* rm, const int32_t* pix, int32_t n_rays,
                      const float* pos_enc, const float* mlp_base, const float* mlp_head, float* rgb,
                      fvc_diff_stats* stats) {
   return guard([&] {
     REQUIRE(h, "null renderer");
-    if (h->r.model->net.split_head)
-      throw Error(FVC_E_UNSUPPORTED, "the differentiable render covers models without a split head");
+    check_diff_model(h->r, false);
     check_rm(rm);
     REQUIRE(n_rays >= 0, "negative ray count");
     REQUIRE(n_rays == 0 || (pix && pos_enc && mlp_base && mlp_head && rgb), "null argument");
@@ -442,8 +456,7 @@ int fvc_diff_cached_forward(fvc_renderer* h, const fvc_raymarch_info* rm, const 
                             fvc_diff_stats* stats) {
   return guard([&] {
     REQUIRE(h, "null renderer");
-    if (h->r.model->net.split_head)
-      throw Error(FVC_E_UNSUPPORTED, "the differentiable render covers models without a split head");
+    check_diff_model(h->r, true);
     check_rm(rm);
     REQUIRE(n_rays >= 0, "negative ray count");
     REQUIRE(n_rays == 0 || (pix && pos_enc && mlp_base && mlp_head && rgb), "null argument");
@@ -466,7 +479,11 @@ int fvc_diff_cached_backward(fvc_renderer* h, const float* d_rgb, float* g_pos_e
 int fvc_cache_swap(fvc_renderer* h) {
   return guard([&] {
     REQUIRE(h, "null renderer");
-    if (fill_publish(h->r)) return;  // a finished background fill
+    // a background fill (running or finished) is joined and published: that
+    // swap is the one swapCache asks for (a second swap here would bring the
+    // stale cache back to the front)
+    if (h->r.fill_state.load() == FILL_RUNNING) fill_poll(h->r, true, nullptr);
+    if (fill_publish(h->r)) return;
     std::scoped_lock lk(h->r.read_mutex, h->r.write_mutex);
     if (h->r.double_buffered) std::swap(h->r.render_idx, h->r.fill_idx);
   });
@@ -519,14 +536,46 @@ int fvc_cache_set_settings(fvc_renderer* h, const fvc_cache_settings* s) {
     REQUIRE(h && s, "null argument");
     const fvc_net_desc& net = h->r.model->net;
     // the variant must match what the model's latents hold (eval.cu:216)
-    const int variant = net.split_head && net.first_latent_is_density ? FVC_INTERPOL_DENSITY_INTERMEDIATES
-                                                                      : FVC_INTERPOL_DENSITY;
-    if (s->interpol_variant != variant || s->interpol_function != FVC_FN_LINEAR || !s->use_inter_brick_interpolation)
-      throw Error(FVC_E_UNSUPPORTED,
-                  variant == FVC_INTERPOL_DENSITY
-                      ? "only the default cache settings (Density, Linear, inter-brick) are supported"
-                      : "this model needs DensityIntermediates, Linear, inter-brick cache settings");
+    const int variant = net.first_latent_is_density ? FVC_INTERPOL_DENSITY_INTERMEDIATES : FVC_INTERPOL_DENSITY;
+    if (s->interpol_variant != variant)
+      throw Error(FVC_E_UNSUPPORTED, variant == FVC_INTERPOL_DENSITY
+                                         ? "this model's cache needs the Density interpolation variant"
+                                         : "this model's cache needs the DensityIntermediates interpolation variant");
+    REQUIRE(s->interpol_function >= FVC_FN_LINEAR && s->interpol_function <= FVC_FN_NEAREST,
+            "unknown interpolation function");
+    std::scoped_lock lk(h->r.read_mutex);
     h->r.settings = *s;
+    h->r.settings.use_inter_brick_interpolation = s->use_inter_brick_interpolation != 0;
+  });
+}
+
+int fvc_cache_set_render_mode(fvc_renderer* h, int mode) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    REQUIRE(mode == FVC_RENDER_BOTH || mode == FVC_RENDER_ONLY_CACHE || mode == FVC_RENDER_ONLY_RESAMPLED,
+            "unknown render mode");
+    std::scoped_lock lk(h->r.read_mutex);
+    h->r.render_mode = mode;
+  });
+}
+
+int fvc_debug_cache_lookup(fvc_renderer* h, int which, const float* froxel_host, int64_t n, float* out_host) {
+  return guard([&] {
+    REQUIRE(h && (n == 0 || (froxel_host && out_host)) && n >= 0, "invalid argument");
+    REQUIRE(which == 0 || which == 1, "which must be 0 or 1");
+    std::scoped_lock lk(h->r.read_mutex);
+    debug_lookup(h->r, which, froxel_host, n, out_host);
+  });
+}
+
+int fvc_debug_render_cached(fvc_renderer* h, const fvc_raymarch_info* rm, int32_t max_codes, int8_t* codes_host,
+                            int32_t* classified_host, int32_t* composited_host, fvc_render_stats* stats) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    check_rm(rm);
+    REQUIRE(max_codes >= 0 && (max_codes == 0 || codes_host), "invalid argument");
+    std::scoped_lock lk(h->r.read_mutex);
+    debug_render_cached(h->r, *rm, max_codes, codes_host, classified_host, composited_host, stats);
   });
 }
 

@@ -91,10 +91,14 @@ static const int VD_F1 = 14, VD_F2 = 18, VD_F3 = 26, VD_HA = 28, VD_HB = 32, VD_
 static const int N_VD_FRAGS = 38, N_VD_NET_FRAGS = 38;
 
 // per-layer accumulator scales of the MLPs (FP8: s_w[l] * s_a; otherwise 1)
-// and 1 / s_a (mlp_mfma.h)
+// and 1 / s_a (mlp_mfma.h), and the density-MLP output rows [lat_lo, lat_hi)
+// that the colour head reads as latents (the other rows of its first 16
+// inputs are the constant 1.0 padding): rows 1..8 for models with a separate
+// density row, rows 0..n_output-1 for the reference's plain models
 struct MlpScales {
   float layer[5];   // W1, W2, W3, H1, H2
   float inv_act;
+  int lat_lo, lat_hi;
 };
 
 struct Model {
@@ -300,6 +304,10 @@ struct CacheView {
   const Voxel8* pool8;   // FP8 latents (lat8)
   const float* scales;   // per brick (lat8)
   int lat8;
+  // CacheSettings (cache_renderer.h:24-32): interpol_function (FVC_FN_*) and
+  // use_inter_brick_interpolation
+  int interp_fn;
+  int inter_brick;
 };
 __device__ __forceinline__ float vox_density(const CacheView& cv, int64_t v) {
   return cv.lat8 ? cv.pool8[v].density : cv.pool[v].density;
@@ -330,6 +338,14 @@ struct Renderer {
   CacheStore caches[2];
   int render_idx = 0, fill_idx = 0;
   fvc_cache_settings settings{0, 0, 1, 1, 1, 1};
+  int render_mode = FVC_RENDER_BOTH;  // CachePerformanceRenderer::_render_mode
+  // inspection records of a cached render (device; set only by
+  // fvc_debug_render_cached): per-pixel sample classes, their count, and
+  // the composited resample / cache samples
+  int8_t* dbg_codes = nullptr;
+  int dbg_k = 0;
+  int* dbg_n = nullptr;
+  int2* dbg_comp = nullptr;
   int capacity_override = 0;
   std::mutex read_mutex, write_mutex;
   Profiler prof;
@@ -419,6 +435,9 @@ void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int 
                   const float* base, const float* head, float* rgb, fvc_diff_stats* stats, bool cached);
 void diff_backward(Renderer& r, const float* d_rgb, float* g_pos, float* g_base, float* g_head, float* g_pool);
 void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts, float* t0s);
+void debug_lookup(Renderer& r, int which, const float* fr_host, int64_t n, float* out_host);
+void debug_render_cached(Renderer& r, const fvc_raymarch_info& rm, int k, int8_t* codes_host, int32_t* classified_host,
+                         int32_t* composited_host, fvc_render_stats* stats);
 void resize_renderbuffers(Renderer& r, int w, int h);
 void ensure_ray_capacity(Renderer& r, int64_t n_pixels, int64_t target);
 void graph_run(Renderer& r, int which, hipStream_t s, const std::function<void()>& record);

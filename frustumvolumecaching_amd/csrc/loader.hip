@@ -114,10 +114,29 @@ void nerfacc_read_config(const std::string& dir, NerfaccExport& ex) {
     n.base_n_output = (int)base.at("n_output").number();
     n.head_width = (int)head.at("network").at("n_neurons").number();
     n.head_n_hidden_layers = (int)head.at("network").at("n_hidden_layers").number();
-    // separate log sigma in row 0, latents after it (latent_offset,
-    // nerf_network.h:334-337, first latent is not the density)
-    n.n_latents = n.base_n_output - 1;
-    n.latent_offset = 1;
+    if (head.has("latent_offset") && (int)head.at("latent_offset").number() == 1) {
+      // the synthetic separate-density layout ("latent_offset": 1 is this
+      // library's key, not the reference's): log sigma in row 0, the 8
+      // latents after it, the density kept apart in the cache
+      n.latent_offset = 1;
+      n.n_latents = 8;
+      n.first_latent_is_density = 0;
+    } else {
+      // NerfNetwork::init (nerf_network.h:88-118): m_n_latents = n_output
+      // and m_first_latent_is_density stays true, so latent_offset() = 0
+      // (nerf_network.h:93, 334-342): the head reads rows 0..n_output-1,
+      // log sigma first; eval picks DensityIntermediates (eval.cu:216)
+      n.latent_offset = 0;
+      n.n_latents = n.base_n_output;
+      n.first_latent_is_density = 1;
+    }
+    // head input: the latents, SH, then padding with 1.0 up to the head's
+    // 32 inputs (nerf_network.h:108-109, 248-253); without "n_input" the
+    // reference leaves those rows as the buffer holds them, here they are
+    // the padding
+    if (head.has("n_input") && !cfg.has("mlp_first_head") &&
+        (int)head.at("n_input").number() != n.n_latents + 16)
+      throw Error(FVC_E_UNSUPPORTED, "mlp_head n_input must be the latents + 16 SH inputs");
     n.sh_degree = sh_degree_of(head.at("encoding"));
     n.precision = FVC_PREC_FP16;
     if (cfg.has("mlp_first_head")) {

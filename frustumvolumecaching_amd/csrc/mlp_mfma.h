@@ -194,19 +194,19 @@ __device__ __forceinline__ void sh4_all(float dx01, float dy01, float dz01, uint
   hi = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
-// Head input fragment.  Slots e=0..3 carry base-output rows 4g+e: rows 1..8
-// are the latents, rows 0 and 9..15 are replaced by the constant 1.0 padding
-// of the reference's head input (nerf_network.h:248-253); slots e=4..7 carry
-// SH coefficients 4g..4g+3.  Values are the fp16 ones the reference stores,
-// then rounded to the MLP's operand precision.
+// Head input fragment.  Slots e=0..3 carry base-output rows 4g+e: rows
+// [lo, hi) are the latents, the other rows are replaced by the constant 1.0
+// padding of the reference's head input (nerf_network.h:248-253); slots
+// e=4..7 carry SH coefficients 4g..4g+3.  Values are the fp16 ones the
+// reference stores, then rounded to the MLP's operand precision.
 template <int P>
 __device__ __forceinline__ typename Mx<P>::T head_input(const _Float16* rows4, int g, float dx01, float dy01,
-                                                         float dz01, float inv_act) {
+                                                         float dz01, float inv_act, int lo, int hi) {
   float v[8];
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     const int r = 4 * g + e;
-    v[e] = (r >= 1 && r <= 8) ? (float)rows4[e] : 1.0f;
+    v[e] = (r >= lo && r < hi) ? (float)rows4[e] : 1.0f;
   }
   _Float16 sh[4];
   sh4_quarter(dx01, dy01, dz01, g, sh);

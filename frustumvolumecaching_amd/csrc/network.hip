@@ -33,9 +33,21 @@ bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n
                 int64_t* n_first) {
   if (d.n_levels != 16 || d.n_features_per_level != 2 || d.log2_hashmap_size < 10 || d.log2_hashmap_size > 24)
     return false;
-  if (d.base_width != 64 || d.base_n_hidden_layers != 2 || d.base_n_output < 9 || d.base_n_output > 16) return false;
-  if (d.head_width != 64 || d.head_n_hidden_layers != 1 || d.n_latents != 8 || d.sh_degree != 4) return false;
-  if (!d.split_head && d.latent_offset != 1) return false;
+  if (d.base_width != 64 || d.base_n_hidden_layers != 2 || d.base_n_output < 1 || d.base_n_output > 16) return false;
+  if (d.head_width != 64 || d.head_n_hidden_layers != 1 || d.sh_degree != 4) return false;
+  // Plain models (NerfNetwork::init): the reference keeps
+  // m_first_latent_is_density = true, so latent_offset() = 0 and the head
+  // reads the first n_output rows, log sigma first (nerf_network.h:93, 214,
+  // 334-342); the cache stores the density in latent 0 (DensityIntermediates,
+  // eval.cu:216).  latent_offset 1 (log sigma in row 0, 8 latents after it,
+  // the density in its own cache array) is the synthetic workload layout.
+  if (!d.split_head) {
+    const bool ref = d.latent_offset == 0 && d.first_latent_is_density && d.n_latents == d.base_n_output;
+    const bool sep = d.latent_offset == 1 && !d.first_latent_is_density && d.n_latents == 8 && d.base_n_output >= 9;
+    if (!ref && !sep) return false;
+  } else if (d.n_latents != 8 || d.base_n_output < 9) {
+    return false;
+  }
   if (d.split_head &&
       (d.first_head_width != 64 || d.first_head_n_hidden_layers < 1 || d.first_head_n_hidden_layers > 2 ||
        d.first_head_n_input != 32 || d.first_sh_degree != 4 ||
@@ -220,7 +232,8 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
       if (!HEAD) {
         if (valid[t]) out16[s[t] * 4 + g] = pack_h4(rows[0], rows[1], rows[2], rows[3]);
       } else {
-        const typename Mx<P>::T hin = head_input<P>(rows, g, dv[t].x, dv[t].y, dv[t].z, sc.inv_act);
+        const typename Mx<P>::T hin = head_input<P>(rows, g, dv[t].x, dv[t].y, dv[t].z, sc.inv_act, sc.lat_lo,
+                                                    sc.lat_hi);
         const f4 o = head_mlp<P>(wl + FRAG_H1 * 64, wl + FRAG_H2 * 64, li, hin, sc);
         if (g == 0 && valid[t]) out_h4[s[t]] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2], rows[0]);
       }
@@ -228,8 +241,8 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
   }
 }
 
-// inferenceHead on given head inputs (16 halves per sample: row 0 log sigma,
-// rows 1..8 latents)
+// inferenceHead on given head inputs (16 halves per sample in the layout of
+// the density-MLP output: row 0 log sigma, latents in rows [lat_lo, lat_hi))
 template <int P>
 __global__ __launch_bounds__(NET_BLOCK) void k_head(const uint16_t* __restrict__ hin16, const float* __restrict__ dir,
                                                     int64_t n, const typename Mx<P>::T* __restrict__ frags,
@@ -249,7 +262,8 @@ __global__ __launch_bounds__(NET_BLOCK) void k_head(const uint16_t* __restrict__
     _Float16 rows[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) rows[e] = __builtin_bit_cast(_Float16, hin16[sl * 16 + 4 * g + e]);
-    const typename Mx<P>::T x = head_input<P>(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2], sc.inv_act);
+    const typename Mx<P>::T x = head_input<P>(rows, g, dir[3 * sl], dir[3 * sl + 1], dir[3 * sl + 2], sc.inv_act,
+                                              sc.lat_lo, sc.lat_hi);
     const f4 o = head_mlp<P>(wl + FRAG_H1 * 64, wl + FRAG_H2 * 64, li, x, sc);
     if (g == 0 && valid)
       out_h4[s] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2],
@@ -511,11 +525,14 @@ void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t
 // W1: slot pair j of group g holds level 4j+g (encode_quads)
 static int col_first(int g, int e) { return 2 * (4 * (e >> 1) + g) + (e & 1); }
 static int col_chain(int kb, int g, int e) { return e < 4 ? 32 * kb + 4 * g + e : 32 * kb + 16 + 4 * g + e - 4; }
-static int col_head_in(int g, int e) {                                                         // H1
-  if (e >= 4) return 8 + 4 * g + (e - 4);  // SH coefficient 4g + e - 4
+// H1: the head's inputs are [n latents, 16 SH, 16 - n padding]; slot e < 4
+// of group g carries base-output row r = 4g + e, a latent when r is in
+// [lo, lo + n), else one of the padding inputs (in row order)
+static int col_head_in(int g, int e, int lo, int n) {
+  if (e >= 4) return n + 4 * g + (e - 4);  // SH coefficient 4g + e - 4
   const int r = 4 * g + e;
-  if (r >= 1 && r <= 8) return r - 1;      // latent r - 1
-  return r == 0 ? 24 : r + 16;             // padding columns 24..31
+  if (r >= lo && r < lo + n) return r - lo;  // latent r - lo
+  return n + 16 + (r < lo ? r : r - n);      // padding
 }
 
 // host conversions for the weight fragments
@@ -557,8 +574,10 @@ static uint8_t e4m3_code(float f) {
 // Fragments in the operand format of the precision: 2 bytes per value for
 // FP16 / BF16, 1 byte for FP8 (E4M3 codes of w / s_w, per-layer
 // s_w = absmax / 448 over the whole fp16 layer).  Fills the scales.
-void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, std::vector<uint8_t>& out,
-                    MlpScales& sc) {
+void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, int lat_lo, int n_lat,
+                    std::vector<uint8_t>& out, MlpScales& sc) {
+  sc.lat_lo = lat_lo;
+  sc.lat_hi = lat_lo + n_lat;
   const int bytes = precision == FVC_PREC_FP8 ? 1 : 2;
   out.assign((size_t)N_FRAGS * 64 * 8 * bytes, 0);
   const uint16_t* W1 = base;
@@ -596,7 +615,8 @@ void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, s
     for (int kb = 0; kb < 2; kb++)
       put(FRAG_W2 + nb * 2 + kb, 1, W2, 64, nb * 16, [kb](int g, int e) { return col_chain(kb, g, e); });
   for (int kb = 0; kb < 2; kb++) put(FRAG_W3 + kb, 2, W3, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
-  for (int nb = 0; nb < 4; nb++) put(FRAG_H1 + nb, 3, H1, 32, nb * 16, col_head_in);
+  for (int nb = 0; nb < 4; nb++)
+    put(FRAG_H1 + nb, 3, H1, 32, nb * 16, [=](int g, int e) { return col_head_in(g, e, lat_lo, n_lat); });
   for (int kb = 0; kb < 2; kb++) put(FRAG_H2 + kb, 4, H2, 64, 0, [kb](int g, int e) { return col_chain(kb, g, e); });
 }
 

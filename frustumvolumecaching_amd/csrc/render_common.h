@@ -90,6 +90,8 @@ inline CacheView cache_view(const CacheStore& C) {
   cv.pool8 = C.pool.as<Voxel8>();
   cv.scales = C.scales.as<float>();
   cv.lat8 = C.lat8 ? 1 : 0;
+  cv.interp_fn = FVC_FN_LINEAR;
+  cv.inter_brick = 1;
   return cv;
 }
 

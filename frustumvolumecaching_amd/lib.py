@@ -84,6 +84,10 @@ _SIGS = {
     "fvc_cache_get_known_ranges": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "fvc_cache_get_pool": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "fvc_debug_march": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, _P, _P]),
+    "fvc_cache_set_render_mode": (C.c_int, [_P, C.c_int]),
+    "fvc_debug_cache_lookup": (C.c_int, [_P, C.c_int, _P, C.c_int64, _P]),
+    "fvc_debug_render_cached": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, _P, _P, _P,
+                                          C.POINTER(abi.RenderStats)]),
     "fvc_diff_forward": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, C.c_int32, _P, _P, _P, _P,
                                    C.POINTER(abi.DiffStats)]),
     "fvc_diff_backward": (C.c_int, [_P, _P, _P, _P, _P]),
@@ -129,6 +133,20 @@ def _addr(x):
     if hasattr(x, "ctypes"):
         return C.c_void_p(x.ctypes.data)
     raise TypeError(f"cannot take the address of {type(x)}")
+
+
+def _after_caller(*tensors):
+    """Finish the work the caller queued on torch's current stream before the
+    renderer reads or writes these tensors.  The renderer launches on its
+    own non-blocking stream, which does not wait for torch's stream: without
+    this a zero fill of a gradient block, a parameter update or the loss
+    gradient queued there could still be running when the library's kernels
+    read or accumulate into the same memory."""
+    for t in tensors:
+        if getattr(t, "is_cuda", False):
+            import torch
+            torch.cuda.current_stream(t.device).synchronize()
+            return
 
 
 def _stream(s):
@@ -299,6 +317,7 @@ class Renderer:
     # PerformanceRenderer::render
     def render_full(self, rm, rgba=None):
         st = abi.RenderStats()
+        _after_caller(rgba)
         check(load().fvc_render_full(self.h, C.byref(rm), _addr(rgba), C.byref(st)))
         self.last_render_stats = st
         return st
@@ -313,6 +332,7 @@ class Renderer:
     # CachePerformanceRenderer::render
     def render_cached(self, rm, rgba=None):
         st = abi.RenderStats()
+        _after_caller(rgba)
         check(load().fvc_render_cached(self.h, C.byref(rm), _addr(rgba), C.byref(st)))
         self.last_render_stats = st
         return st
@@ -322,6 +342,7 @@ class Renderer:
         arr = (abi.RaymarchInfo * n)(*rms)
         ptrs = (_P * n)(*[None if x is None else _addr(x).value for x in (rgbas or [None] * n)])
         st = abi.RenderStats()
+        _after_caller(*(rgbas or []))
         check(fn(self.h, arr, n, ptrs, C.byref(st)))
         self.last_render_stats = st
         return st
@@ -364,6 +385,46 @@ class Renderer:
     def setCacheSettings(self, settings):
         check(load().fvc_cache_set_settings(self.h, C.byref(settings)))
 
+    def cache_settings(self, interpol_function=abi.FN_LINEAR, use_inter_brick_interpolation=True):
+        """setCacheSettings with the model's interpolation variant."""
+        s = abi.CacheSettings()
+        s.interpol_variant = (abi.INTERPOL_DENSITY_INTERMEDIATES if self.model.net.first_latent_is_density
+                              else abi.INTERPOL_DENSITY)
+        s.interpol_function = int(interpol_function)
+        s.use_hw_interpol = 1
+        s.reweight_intermediates = 1
+        s.disable_reweighting_first_intermediate = 1
+        s.use_inter_brick_interpolation = int(bool(use_inter_brick_interpolation))
+        self.setCacheSettings(s)
+
+    def set_render_mode(self, mode):
+        """CachePerformanceRenderer::_render_mode: abi.RENDER_BOTH,
+        RENDER_ONLY_CACHE or RENDER_ONLY_RESAMPLED."""
+        check(load().fvc_cache_set_render_mode(self.h, int(mode)))
+
+    def debug_lookup(self, froxel, which=0):
+        """The cache head's lookup at froxel points [n, 3]: [n, 9] = (density
+        row, latents 0..7) as half values."""
+        import numpy as np
+        f = np.ascontiguousarray(froxel, np.float32).reshape(-1, 3)
+        out = np.zeros((len(f), 9), np.float32)
+        check(load().fvc_debug_cache_lookup(self.h, int(which), _addr(f), len(f), _addr(out)))
+        return out
+
+    def debug_render_cached(self, rm, k=64):
+        """render_cached with the per-pixel records: codes [H, W, k] (0 skip,
+        1 resample, 2 cache, -1 past the classified samples), classified
+        [H, W], composited [H, W, 2] (resampled, cached)."""
+        import numpy as np
+        H, W = rm.cam_info.res_y, rm.cam_info.res_x
+        codes = np.full((H, W, k), -1, np.int8)
+        cls = np.zeros((H, W), np.int32)
+        comp = np.zeros((H, W, 2), np.int32)
+        st = abi.RenderStats()
+        check(load().fvc_debug_render_cached(self.h, C.byref(rm), int(k), _addr(codes), _addr(cls), _addr(comp),
+                                             C.byref(st)))
+        return codes, cls, comp, st
+
     def set_latent_format(self, fmt):
         """'fp16' or 'fp8' latents for the next fills."""
         check(load().fvc_cache_set_latent_format(self.h, {"fp16": 0, "fp8": 1}[fmt]))
@@ -380,6 +441,7 @@ class Renderer:
         check(load().fvc_cache_set_capacity(self.h, int(bricks)))
 
     def last_result(self, out_dev, n_floats):
+        _after_caller(out_dev)
         check(load().fvc_get_last_result(self.h, _addr(out_dev), int(n_floats)))
 
     def cache_info(self, which=0):
@@ -428,6 +490,7 @@ class Renderer:
         with FP32 parameter blocks params = (pos_enc, mlp_base, mlp_head)."""
         st = abi.DiffStats()
         pos, base, head = params
+        _after_caller(pix, pos, base, head, rgb)
         check(load().fvc_diff_forward(self.h, C.byref(rm), _addr(pix), int(pix.numel()), _addr(pos), _addr(base),
                                       _addr(head), _addr(rgb), C.byref(st)))
         return st
@@ -435,12 +498,14 @@ class Renderer:
     def diff_backward(self, d_rgb, grads):
         """Adds d(sum(d_rgb * rgb))/d(params) of the last diff_forward to grads."""
         g_pos, g_base, g_head = grads
+        _after_caller(d_rgb, g_pos, g_base, g_head)
         check(load().fvc_diff_backward(self.h, _addr(d_rgb), _addr(g_pos), _addr(g_base), _addr(g_head)))
 
     def diff_cached_forward(self, rm, pix, params, rgb):
         """diff_forward against the render cache (resample / cache samples)."""
         st = abi.DiffStats()
         pos, base, head = params
+        _after_caller(pix, pos, base, head, rgb)
         check(load().fvc_diff_cached_forward(self.h, C.byref(rm), _addr(pix), int(pix.numel()), _addr(pos),
                                              _addr(base), _addr(head), _addr(rgb), C.byref(st)))
         return st
@@ -449,6 +514,7 @@ class Renderer:
         """diff_backward of the last cached forward; g_pool [bricks, 512, 10]
         receives the voxel latent / density gradients."""
         g_pos, g_base, g_head = grads
+        _after_caller(d_rgb, g_pos, g_base, g_head, g_pool)
         check(load().fvc_diff_cached_backward(self.h, _addr(d_rgb), _addr(g_pos), _addr(g_base), _addr(g_head),
                                               _addr(g_pool)))
 

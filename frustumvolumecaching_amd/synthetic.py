@@ -48,14 +48,27 @@ def f16_bits(a):
 # ---------------------------------------------------------------------------
 # network
 # ---------------------------------------------------------------------------
-def net_desc(log2_hashmap_size=19, n_levels=16, base_resolution=16, max_resolution=2048, precision=abi.PREC_FP16):
+def net_desc(log2_hashmap_size=19, n_levels=16, base_resolution=16, max_resolution=2048, precision=abi.PREC_FP16,
+             layout="separate", n_output=None):
+    """layout "separate": log sigma in output row 0 and 8 latents after it
+    (the density kept apart in the cache); "reference": the reference's plain
+    models, the head reads output rows 0..n_output-1 with log sigma first
+    (latent_offset 0, DensityIntermediates cache; nerf_network.h:88-118,
+    334-342)."""
     pls = math.exp(math.log(max_resolution / base_resolution) / (n_levels - 1))
+    if layout == "separate":
+        n_out, n_lat, off, fld = n_output or 9, 8, 1, 0
+    elif layout == "reference":
+        n_out = n_output or 8
+        n_lat, off, fld = n_out, 0, 1
+    else:
+        raise ValueError(layout)
     return abi.NetDesc(
         n_levels=n_levels, n_features_per_level=2, log2_hashmap_size=log2_hashmap_size,
         base_resolution=base_resolution, per_level_scale=float(np.float32(pls)),
-        base_width=64, base_n_hidden_layers=2, base_n_output=9,
-        head_width=64, head_n_hidden_layers=1, n_latents=8, latent_offset=1, sh_degree=4,
-        precision=precision)
+        base_width=64, base_n_hidden_layers=2, base_n_output=n_out,
+        head_width=64, head_n_hidden_layers=1, n_latents=n_lat, latent_offset=off, sh_degree=4,
+        precision=precision, first_latent_is_density=fld)
 
 
 def net_desc_viewdep(log2_hashmap_size=19, first_latent_is_density=True, first_head_hidden_layers=1,
@@ -340,6 +353,17 @@ def small_plain(width=256, height=256, log2_hashmap_size=19, stepsize_setup=None
     return Workload("small_plain_256", scene, step, net, occ, params, look_at(eye, (0, 0, 0)), width, height)
 
 
+def small_plain_reference(width=256, height=256, n_output=8, stepsize_setup=None):
+    """The small scene with the reference's plain-model layout: the density
+    MLP's n_output rows (log sigma first) are the head's latents, and the
+    cache stores the density in latent 0 (DensityIntermediates)."""
+    wl = small_plain(width, height, stepsize_setup=stepsize_setup)
+    net = net_desc(layout="reference", n_output=n_output)
+    params = make_params(net, SEEDS["weights"], SEEDS["table"])
+    return Workload("small_plain_reference_256", wl.scene, wl.step, net, wl.occupancy, params, wl.cam2world, width,
+                    height)
+
+
 def small_viewdep(width=256, height=256, first_latent_is_density=True, first_head_hidden_layers=1,
                   freq_encoding_degree=4, stepsize_setup=None):
     """small_plain's scene and camera with a split (view-dependent) model."""
@@ -425,6 +449,12 @@ def nerfacc_config(wl):
             "network": dict(mlp, n_neurons=net.head_width, n_hidden_layers=net.head_n_hidden_layers),
         },
     }
+    if not net.split_head:
+        cfg["mlp_head"]["n_input"] = net.n_latents + 16
+        if net.latent_offset == 1:
+            # this library's key for the separate-density layout (reference
+            # plain exports read the first n_output rows as latents)
+            cfg["mlp_head"]["latent_offset"] = 1
     if net.split_head:
         # init_viewdep (nerf_network.h:120-158) keys
         cfg["mlp_head"]["n_input"] = head_input_width(net)

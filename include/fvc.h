@@ -13,7 +13,7 @@
  *   PerformanceRenderer::{render, resizeRenderbuffers}     src/performance_renderer.cu:186-286
  *   CacheRenderer::{resizeCache, initCacheBlockwise,
  *                   swapCache, setCacheSettings}           src/cache_renderer.cu:832-1057,
- *                                                           include/cache_renderer.h:243-267
+ *                                                           include/cache_renderer.h:42-66
  *   CachePerformanceRenderer::render                       src/cache_performance_renderer.cu:1059-1263
  *   Cache (brick indirection, pool, known_ranges_to)       include/cache.h:12-74
  *
@@ -100,6 +100,17 @@ typedef struct {
  * 32->64->64->16 (ReLU, no bias); head (n_latents + SH4(16) + pad 1.0 = 32)
  * ->64->3 (padded 16).  Anything else returns FVC_E_UNSUPPORTED.
  *
+ * Plain models have one of two latent layouts:
+ *  - the reference's (NerfNetwork::init, nerf_network.h:88-118, 334-342):
+ *    latent_offset = 0, first_latent_is_density = 1, n_latents =
+ *    base_n_output (1..16): the head reads output rows 0..n-1, log sigma
+ *    first, and the cache stores the density in latent 0
+ *    (DensityIntermediates); cache rendering needs n_latents <= 8
+ *    (cache_performance_renderer.cu:964-967);
+ *  - separate density (the synthetic workloads): latent_offset = 1,
+ *    first_latent_is_density = 0, n_latents = 8, base_n_output >= 9: row 0
+ *    is log sigma, rows 1..8 the latents, the cache keeps the density apart.
+ *
  * split_head = 1 describes the view-dependent split model
  * (NerfNetwork::init_viewdep, nerf_network.h:120-158): the density MLP's 16
  * outputs and SH4 of the initial view direction (32 inputs) feed
@@ -123,8 +134,8 @@ typedef struct {
   int32_t base_n_output;          /* 1 (log sigma) + n_latents, <= 16 */
   int32_t head_width;             /* 64 */
   int32_t head_n_hidden_layers;   /* 1 */
-  int32_t n_latents;              /* 8 */
-  int32_t latent_offset;          /* 1: head input starts after log sigma (nerf_network.h:334-337) */
+  int32_t n_latents;              /* rows of the density-MLP output the head reads as latents */
+  int32_t latent_offset;          /* first such row (nerf_network.h:334-337): 0 = log sigma is latent 0 */
   int32_t sh_degree;              /* 4 */
   int32_t precision;              /* FVC_PREC_* (MLP arithmetic) */
   int32_t split_head;             /* 1: view-dependent split model (mlp_first_head) */
@@ -145,12 +156,20 @@ enum {
 /* InterpolFunction (common.h:221-227) */
 enum { FVC_FN_LINEAR = 0, FVC_FN_SMOOTHSTEP = 1, FVC_FN_SMOOTHERSTEP = 2, FVC_FN_NEAREST = 3 };
 
-/* CacheSettings (cache_renderer.h:24-32).  Supported: interpol_variant
- * DENSITY (models whose latents exclude the density) or
- * DENSITY_INTERMEDIATES (split models with first_latent_is_density, the
- * variant eval.cu:216 picks for them), interpol_function LINEAR,
- * use_inter_brick_interpolation = 1 (the default_cache_settings,
- * cache_renderer.h:55-63).  A renderer starts with the variant of its model. */
+/* CacheSettings (cache_renderer.h:24-32).  interpol_variant must match the
+ * model: DENSITY for models whose latents exclude the density (separate
+ * density row), DENSITY_INTERMEDIATES for models whose first latent is the
+ * density (the reference's plain models and split models with
+ * first_latent_is_density; the variant eval.cu:216 picks).  interpol_function
+ * LINEAR, SMOOTHSTEP, SMOOTHERSTEP or NEAREST weights the 8 lookup corners
+ * (applyInterpolationFunction, raymarch_common.h:393-402).
+ * use_inter_brick_interpolation = 0 clamps the lookup point into its brick
+ * (froxel2data, raymarch_common.h:341-357, as the reference's basic cache
+ * renderer applies it; its performance renderer ignores the flag).
+ * use_hw_interpol, reweight_intermediates and
+ * disable_reweighting_first_intermediate are ignored, as by the reference's
+ * performance renderer.  A renderer starts with the variant of its model,
+ * linear weights and inter-brick interpolation (cache_renderer.h:55-63). */
 typedef struct {
   int32_t interpol_variant;
   int32_t interpol_function;
@@ -353,10 +372,19 @@ int fvc_renderer_set_refill_policy(fvc_renderer* renderer, const fvc_refill_poli
 int fvc_render_cached_auto(fvc_renderer* renderer, const fvc_raymarch_info* rm, uint8_t* rgba_dev,
                            fvc_render_stats* stats, int32_t* refill_started);
 
-/* CacheRenderer::swapCache (cache_renderer.cu:1053-1057) */
+/* CacheRenderer::swapCache (cache_renderer.cu:1053-1057); a background fill
+ * still running is waited for and published instead. */
 int fvc_cache_swap(fvc_renderer* renderer);
-/* CacheRenderer::setCacheSettings (cache_renderer.h:266-267) */
+/* CacheRenderer::setCacheSettings (cache_renderer.h:65) */
 int fvc_cache_set_settings(fvc_renderer* renderer, const fvc_cache_settings* settings);
+/* CachePerformanceRenderer::RenderMode (cache_performance_renderer.h:23-28),
+ * the public _render_mode member (:121): ONLY_CACHE drops the samples that
+ * would be resampled, ONLY_RESAMPLED the samples that would come from the
+ * cache (cache_performance_renderer.cu:117-118, 125-126). */
+#define FVC_RENDER_BOTH           0
+#define FVC_RENDER_ONLY_CACHE     1
+#define FVC_RENDER_ONLY_RESAMPLED 2
+int fvc_cache_set_render_mode(fvc_renderer* renderer, int mode);
 /* Latent storage of the brick pool for the next fills (no counterpart in
  * the reference, which stores half4 latents): FVC_LATENT_FP16, or
  * FVC_LATENT_FP8 = OCP E4M3 codes of l / s with one scale s = absmax / 448
@@ -458,6 +486,21 @@ int fvc_cache_get_pool(fvc_renderer* renderer, int which, void* out_host, int64_
  * first max_per_ray sample start times t0; host outputs. */
 int fvc_debug_march(fvc_renderer* renderer, const fvc_raymarch_info* rm, int32_t max_per_ray,
                     int32_t* counts_host, float* t0_host);
+/* The cache head's trilinear lookup alone (generateNetworkInput_cache_shfl_
+ * perf_kernel_cg, cache_performance_renderer.cu:678-787) at n froxel points
+ * (host float[n][3]) of cache `which`, with the renderer's cache settings:
+ * host out float[n][9] = the head inputs as half values, [density row,
+ * latents 0..7]. */
+int fvc_debug_cache_lookup(fvc_renderer* renderer, int which, const float* froxel_host, int64_t n, float* out_host);
+/* CachePerformanceRenderer::render with inspection records (host outputs):
+ * codes int8[H*W][max_codes] = the class of each occupied sample of the
+ * pixel's ray in march order as sample_cache_performance_kernel decides it
+ * (cache_performance_renderer.cu:96-160; 0 skip, 1 resample, 2 cache, -1
+ * past the samples classified), classified int32[H*W] = occupied samples
+ * classified, composited int32[H*W][2] = resampled and cached samples
+ * composited up to and including the one that ends the ray. */
+int fvc_debug_render_cached(fvc_renderer* renderer, const fvc_raymarch_info* rm, int32_t max_codes, int8_t* codes_host,
+                            int32_t* classified_host, int32_t* composited_host, fvc_render_stats* stats);
 
 #ifdef __cplusplus
 }

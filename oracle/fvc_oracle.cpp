@@ -365,6 +365,24 @@ bool net_layout(const fvc_net_desc& d, Net& n) {
   n.n_base = 32 * d.base_width + d.base_width * d.base_width + d.base_width * 16;
   n.n_head = 32 * d.head_width + d.head_width * 16;
   n.split = d.split_head != 0;
+  if (!n.split) {
+    // plain models (NerfNetwork::init, nerf_network.h:88-118): the reference
+    // keeps m_first_latent_is_density = true, so latent_offset() = 0 and the
+    // head reads the first n_output rows, log sigma first (nerf_network.h:93,
+    // 214, 334-342); the cache then stores the density in latent 0
+    // (DensityIntermediates, eval.cu:216).  The separate-density layout
+    // (latent_offset 1: log sigma in row 0 and 8 latents after it, density
+    // in its own cache array) is this build's synthetic workload layout.
+    if (d.latent_offset == 0) {
+      if (!d.first_latent_is_density || d.n_latents != d.base_n_output || d.n_latents < 1 || d.n_latents > 16)
+        return false;
+      n.di = true;
+    } else if (d.latent_offset == 1) {
+      if (d.first_latent_is_density || d.n_latents != 8 || d.base_n_output < 9 || d.base_n_output > 16) return false;
+    } else {
+      return false;
+    }
+  }
   if (n.split) {
     // custom head input: 8 latents + 16 SH + 16 SH differences + 6 * freq, padded to 16
     if (d.n_latents != 8 || d.first_head_n_input != 32 || d.first_head_n_hidden_layers < 1 ||
@@ -710,6 +728,12 @@ template <class R> struct Cache {
   std::vector<R> isset;    // [slot][512]
   fvc_init_stats stats{};
   bool lat_fp8 = false;
+  // render options (CacheSettings::interpol_function and
+  // use_inter_brick_interpolation, cache_renderer.h:24-32;
+  // CachePerformanceRenderer::RenderMode, cache_performance_renderer.h:23-28)
+  int interp_fn = 0;        // 0 linear, 1 smoothstep, 2 smootherstep, 3 nearest
+  bool inter_brick = true;  // false: the lookup point is clamped into its brick
+  int render_mode = 0;      // 0 both, 1 only cache, 2 only resampled
 };
 
 // FP8 cache latents: per brick the scale s = absmax / 448 of its latents
@@ -728,15 +752,16 @@ template <class R> void quantize_latents_fp8(Cache<R>& C) {
 }
 
 // Frustum depth in steps: the reference's max_step_in_scene
-// (raymarch_common.h:298-303) padded to 4 (cache_renderer.cu:841-843),
-// extended to reach the farthest AABB corner seen from the fill camera so
-// that a camera outside the AABB keeps every brick inside the frustum.
+// (raymarch_common.h:298-303) padded to 4 (cache_renderer.cu:841-843):
+// ceil(step_from_t(|AABB diagonal| + near)).  For a fill camera outside the
+// AABB whose farthest AABB corner (at most far) lies beyond the diagonal,
+// that distance replaces the diagonal, so every brick stays in the frustum
+// (the reference's fill has no depth guard, cache_renderer.cu:130-178).
 int frustum_depth(const fvc_raymarch_info& rm, const fvc_scene_info& sc) {
   const Step<float> st(rm.stepsize_info);
   const V3<float> from = cvt<float>(sc.aabb_from), to = cvt<float>(sc.aabb_to);
   const V3<float> dg = to - from;
   const float diag = std::sqrt(dot3(dg, dg));
-  int steps = (int)std::ceil(st.step_from_t(diag + st.near_));
   const Mat34<float> c2w(rm.cam_info.cam2world);
   const V3<float> o = c2w.translation();
   float tf = 0.0f;
@@ -746,8 +771,8 @@ int frustum_depth(const fvc_raymarch_info& rm, const fvc_scene_info& sc) {
     tf = std::max(tf, std::sqrt(dot3(v, v)));
   }
   tf = std::min(tf, st.far_);
-  const int steps_cam = (int)std::ceil(st.step_from_t(tf)) + BS;
-  steps = std::max(steps, steps_cam);
+  const float reach = tf > diag ? tf : diag;
+  const int steps = (int)std::ceil(st.step_from_t(reach + st.near_));
   return (steps + 3) / 4 * 4;
 }
 
@@ -970,10 +995,34 @@ Cache<R>* cache_fill(const Model& M, const fvc_raymarch_info& rm, Mode mode, int
 // holds the density, so latent 0 = log(sum w*l_0) * W / W (W = sum w*isset,
 // no clamp, 770-779) and it is also the density row: head_in9 = [latent 0,
 // latents 0..7].
+// applyInterpolationFunction (raymarch_common.h:393-402) on one fraction:
+// smoothstep x*x*(3 - 2x), smootherstep x*x*x*(x*(6x - 15) + 10)
+// (helper_math_extension.h:146-170, the same evaluation order), nearest
+// = round half to even (nearbyintf).
+template <class R> R interp_fn(R t, int fn) {
+  switch (fn) {
+    case 1: return (t * t) * (R(3) - R(2) * t);
+    case 2: return ((t * t) * t) * (t * (R(6) * t - R(15)) + R(10));
+    case 3: return std::nearbyint(t);
+    default: return t;
+  }
+}
+
+// use_inter_brick_interpolation = false: the point is clamped into
+// [8b + 1/2, 8b + 7.5] of its brick b per axis (froxel2data,
+// raymarch_common.h:341-357, as the reference's basic cache renderer applies
+// it), so all 8 corners lie in one brick.
+template <class R> R clamp_to_brick(R x) {
+  const R b = std::floor(x / R(BS)) * R(BS);
+  const R in = std::min(std::max(x - b, R(0.5)), R(BS) - R(0.5));
+  return b + in;
+}
+
 template <class R> void cache_lookup(const Cache<R>& C, V3<R> f, Mode mode, R* head_in9, bool di = false) {
+  if (!C.inter_brick) f = V3<R>{clamp_to_brick(f.x), clamp_to_brick(f.y), clamp_to_brick(f.z)};
   const R fc[3] = {f.x - R(0.5), f.y - R(0.5), f.z - R(0.5)};
   R i0[3], lt[3];
-  for (int k = 0; k < 3; k++) { i0[k] = std::floor(fc[k]); lt[k] = fc[k] - i0[k]; }
+  for (int k = 0; k < 3; k++) { i0[k] = std::floor(fc[k]); lt[k] = interp_fn(fc[k] - i0[k], C.interp_fn); }
   R wd[8], dd[8], ld[8][8];
   for (int c = 0; c < 8; c++) {
     const int off[3] = {(c >> 2) & 1, (c >> 1) & 1, c & 1};
@@ -1014,10 +1063,14 @@ template <class R> void cache_lookup(const Cache<R>& C, V3<R> f, Mode mode, R* h
 
 // Cached render of a pixel crop (CachePerformanceRenderer::render,
 // cache_performance_renderer.cu:22-177 classification, 198-313 network
-// inputs, 789-908 merged compositing).
+// inputs, 789-908 merged compositing).  Optional record per pixel of the
+// crop: the class of each occupied sample in march order up to the ray's
+// end (0 skip, 1 resample, 2 cache; the first K in codes[i * K ..]) and
+// counts3[i] = {occupied samples classified, resampled, cached}.
 template <class R>
 void render_cached(const Model& M, const Cache<R>& C, const fvc_raymarch_info& rm, Mode mode, int x0, int y0, int w,
-                   int h, float* out, int64_t* n_res, int64_t* n_cache) {
+                   int h, float* out, int64_t* n_res, int64_t* n_cache, int K = 0, int8_t* codes = nullptr,
+                   int32_t* counts3 = nullptr) {
   const Step<R> st(rm.stepsize_info);
   const Step<R> cst(C.rm.stepsize_info);
   const Mat34<R> w2c(C.rm.cam_info.world2cam);
@@ -1029,6 +1082,11 @@ void render_cached(const Model& M, const Cache<R>& C, const fvc_raymarch_info& r
     const int px = x0 + (int)(i % w), py = y0 + (int)(i / w);
     const RayInit<R> ri = init_ray<R>(rm, sc, px, py);
     Accum<R> acc;
+    int n_cls = 0, n_r = 0, n_c = 0;
+    auto record = [&](int code) {
+      if (codes && n_cls < K) codes[i * K + n_cls] = (int8_t)code;
+      n_cls++;
+    };
     if (ri.alive) {
       const V3<R> dir01 = unit_to_01(ri.ray.d);
       // transformRay (raymarch_common.h:253-262)
@@ -1055,15 +1113,22 @@ void render_cached(const Model& M, const Cache<R>& C, const fvc_raymarch_info& r
         const R tm2 = t0 + dt * R(0.5);
         R o4[4];
         if (!inside || unsampled) {
+          // ONLY_CACHE skips the samples to resample (cache_performance_renderer.cu:117-118)
+          if (C.render_mode == 1) { record(0); continue; }
+          record(1);
           // resample: full network (generateNetworkInput_resample_perf_kernel, 198-240)
           inference(M.net, contract(ri.ray.o + tm2 * ri.ray.d, sc), dir01, mode, o4);
           tr++;
+          n_r++;
         } else {
+          // ONLY_RESAMPLED skips the cache side (cache_performance_renderer.cu:125-126)
+          if (C.render_mode == 2) { record(0); continue; }
           const int vx = (int)std::floor(f.x), vy = (int)std::floor(f.y), vz = (int)std::floor(f.z);
           const int slot = C.table[((size_t)(vz / BS) * C.by + vy / BS) * C.bx + vx / BS];
-          if (slot < 0) continue;
+          if (slot < 0) { record(0); continue; }
           const size_t v = (size_t)slot * 512 + (size_t)((vz % BS) * 64 + (vy % BS) * 8 + vx % BS);
-          if (!(C.density[v] > R(0))) continue;
+          if (!(C.density[v] > R(0))) { record(0); continue; }
+          record(2);
           R tf2;
           const V3<R> f2 = cam2froxel(oc + tm2 * dc, C.rm.cam_info, cst, tf2);
           R hin[9];
@@ -1080,9 +1145,15 @@ void render_cached(const Model& M, const Cache<R>& C, const fvc_raymarch_info& r
           }
           o4[0] = rgb[0]; o4[1] = rgb[1]; o4[2] = rgb[2]; o4[3] = hin[0];
           tc++;
+          n_c++;
         }
         if (!acc.add(o4, dt, (R)sc.alpha_thre)) break;
       }
+    }
+    if (counts3) {
+      counts3[3 * i] = n_cls;
+      counts3[3 * i + 1] = n_r;
+      counts3[3 * i + 2] = n_c;
     }
     float* o = out + i * 4;
     o[0] = (float)acc.rgb[0]; o[1] = (float)acc.rgb[1]; o[2] = (float)acc.rgb[2]; o[3] = (float)acc.T;
@@ -1942,6 +2013,27 @@ void orc_cache_lookup(const CacheHandle* h, const float* froxel, int64_t n, int 
   }
 }
 
+// CacheSettings::interpol_function, use_inter_brick_interpolation and the
+// RenderMode of later renders / lookups of this cache
+void orc_cache_set_options(CacheHandle* h, int interp_fn, int inter_brick, int render_mode) {
+  auto set = [&](auto& C) {
+    C.interp_fn = interp_fn;
+    C.inter_brick = inter_brick != 0;
+    C.render_mode = render_mode;
+  };
+  if (h->d) set(*h->d);
+  else set(*h->f);
+}
+
+// render_cached with the per-pixel classification record (codes [n][K],
+// counts3 [n][3])
+void orc_render_cached_record(const Model* M, const CacheHandle* h, const fvc_raymarch_info* rm, int x0, int y0,
+                              int w, int hh, float* out, int64_t* n_res, int64_t* n_cache, int K, int8_t* codes,
+                              int32_t* counts3) {
+  if (h->d) render_cached<double>(*M, *h->d, *rm, (Mode)h->mode, x0, y0, w, hh, out, n_res, n_cache, K, codes, counts3);
+  else render_cached<float>(*M, *h->f, *rm, (Mode)h->mode, x0, y0, w, hh, out, n_res, n_cache, K, codes, counts3);
+}
+
 // Differentiable full render of the pixels pix[n] (y * W + x), see
 // diff_render; f64 selects double arithmetic (else float).  rgb[n][3];
 // with d_rgb[n][3] the parameter gradients are added to g_pos, g_base,
@@ -1959,24 +2051,33 @@ int64_t orc_diff_render(const Model* M, const fvc_raymarch_info* rm, const int32
 CacheHandle* orc_cache_from_arrays(const fvc_raymarch_info* rm, const fvc_cache_info* info, const int32_t* table,
                                    const float* known, const float* pool, int64_t n_slots, int mode) {
   auto* h = new CacheHandle();
-  h->mode = mode == M_EMU16 ? M_EMU16 : M_F32;
-  auto* C = new Cache<float>();
-  C->rm = *rm;
-  C->fx = info->frustum_x; C->fy = info->frustum_y; C->fz = info->frustum_z;
-  C->bx = info->bricks_x; C->by = info->bricks_y; C->bz = info->bricks_z;
-  C->capacity = (int)n_slots;
-  C->n_used = (int)n_slots;
-  C->table.assign(table, table + (size_t)C->bx * C->by * C->bz);
-  C->known_to.assign(known, known + (size_t)C->fx * C->fy);
-  C->lat.resize((size_t)n_slots * 512 * 8);
-  C->density.resize((size_t)n_slots * 512);
-  C->isset.resize((size_t)n_slots * 512);
-  for (int64_t v = 0; v < n_slots * 512; v++) {
-    for (int i = 0; i < 8; i++) C->lat[v * 8 + i] = pool[v * 10 + i];
-    C->density[v] = pool[v * 10 + 8];
-    C->isset[v] = pool[v * 10 + 9];
+  h->mode = mode == M_EMU16 ? M_EMU16 : (mode == M_F64 ? M_F64 : M_F32);
+  auto fill = [&](auto* C) {
+    C->rm = *rm;
+    C->fx = info->frustum_x; C->fy = info->frustum_y; C->fz = info->frustum_z;
+    C->bx = info->bricks_x; C->by = info->bricks_y; C->bz = info->bricks_z;
+    C->capacity = (int)n_slots;
+    C->n_used = (int)n_slots;
+    C->table.assign(table, table + (size_t)C->bx * C->by * C->bz);
+    C->known_to.assign(known, known + (size_t)C->fx * C->fy);
+    C->lat.resize((size_t)n_slots * 512 * 8);
+    C->density.resize((size_t)n_slots * 512);
+    C->isset.resize((size_t)n_slots * 512);
+    for (int64_t v = 0; v < n_slots * 512; v++) {
+      for (int i = 0; i < 8; i++) C->lat[v * 8 + i] = pool[v * 10 + i];
+      C->density[v] = pool[v * 10 + 8];
+      C->isset[v] = pool[v * 10 + 9];
+    }
+  };
+  if (h->mode == M_F64) {
+    auto* C = new Cache<double>();
+    fill(C);
+    h->d.reset(C);
+  } else {
+    auto* C = new Cache<float>();
+    fill(C);
+    h->f.reset(C);
   }
-  h->f.reset(C);
   return h;
 }
 

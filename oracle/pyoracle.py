@@ -87,6 +87,10 @@ def lib():
             "orc_cache_from_arrays": (P, [C.POINTER(abi.RaymarchInfo), C.POINTER(abi.CacheInfo), P, P, P,
                                           C.c_int64, C.c_int]),
             "orc_cache_quantize_fp8": (None, [P]),
+            "orc_cache_set_options": (None, [P, C.c_int, C.c_int, C.c_int]),
+            "orc_render_cached_record": (None, [P, P, C.POINTER(abi.RaymarchInfo), C.c_int, C.c_int, C.c_int,
+                                                C.c_int, P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int,
+                                                P, P]),
             "orc_diff_render_cached": (C.c_int64, [P, P, C.POINTER(abi.RaymarchInfo), P, C.c_int64, P, P, P, C.c_int,
                                                    P, P, P, P, P, P, C.POINTER(C.c_int64)]),
         }
@@ -334,11 +338,19 @@ class Cache:
         lib().orc_cache_pool(self.h, n, _p(out))
         return out
 
+    def set_options(self, interp_fn=0, inter_brick=True, render_mode=0):
+        """CacheSettings::interpol_function (0 linear, 1 smoothstep,
+        2 smootherstep, 3 nearest), use_inter_brick_interpolation and the
+        RenderMode (0 both, 1 only cache, 2 only resampled) of later renders
+        and lookups."""
+        lib().orc_cache_set_options(self.h, int(interp_fn), int(bool(inter_brick)), int(render_mode))
+        return self
+
     def lookup(self, froxel, di=None):
         """Head inputs [density row, latents 0..7] at froxel points; di
         (DensityIntermediates) defaults to the model's first_latent_is_density."""
         if di is None:
-            di = bool(self.model.net.split_head and self.model.net.first_latent_is_density)
+            di = bool(self.model.net.first_latent_is_density)
         f = _f3(froxel)
         out = np.zeros((len(f), 9), np.float64)
         lib().orc_cache_lookup(self.h, _p(f), len(f), int(bool(di)), _p(out))
@@ -350,6 +362,20 @@ class Cache:
         nr, nc = C.c_int64(), C.c_int64()
         lib().orc_render_cached(self.model.h, self.h, C.byref(rm), x0, y0, w, h, _p(out), C.byref(nr), C.byref(nc))
         return out, nr.value, nc.value
+
+    def render_record(self, rm, k=64, crop=None):
+        """render() with the per-pixel classification record: codes [h, w, k]
+        (class of each occupied sample in march order until the ray ends: 0
+        skip, 1 resample, 2 cache; -1 past the end) and counts [h, w, 3] =
+        (samples classified, resampled, cached)."""
+        x0, y0, w, h = crop or (0, 0, rm.cam_info.res_x, rm.cam_info.res_y)
+        out = np.zeros((h, w, 4), np.float32)
+        codes = np.full((h, w, k), -1, np.int8)
+        counts = np.zeros((h, w, 3), np.int32)
+        nr, nc = C.c_int64(), C.c_int64()
+        lib().orc_render_cached_record(self.model.h, self.h, C.byref(rm), x0, y0, w, h, _p(out), C.byref(nr),
+                                       C.byref(nc), k, _p(codes), _p(counts))
+        return out, codes, counts
 
 
 def to_8bit(rgbt):

@@ -81,6 +81,31 @@ def test_autograd_function(small):
         assert np.linalg.norm(gp - g) <= 1e-5 * np.linalg.norm(g)
 
 
+def test_backward_waits_for_torch_stream(small):
+    """The renderer runs on its own stream: gradient blocks zeroed on torch's
+    stream right before the backward (behind a long queue of other torch
+    work) must be zero before the adjoint kernels accumulate into them."""
+    r, rm = small["r"], small["rm"]
+    p_t = tuple(_dev(p) for p in small["params"])
+    rgb = torch.empty(len(small["pix"]), 3, dtype=torch.float32, device="cuda")
+    r.diff_forward(rm, _dev(small["pix"]), p_t, rgb)
+    d = _dev(2.0 * (rgb.cpu().numpy().astype(np.float64) - small["target"]), torch.float32)
+    grads = tuple(torch.full_like(p, float("nan")) for p in p_t)
+    torch.cuda.synchronize()
+    a = torch.randn(4096, 4096, device="cuda")
+    for _ in range(6):
+        a = (a @ a) * 1e-3
+    for g in grads:
+        g.zero_()
+    r.diff_backward(d, grads)
+    torch.cuda.synchronize()
+    assert grads[0].numel() >= 1 << 22
+    for g, ref in zip(grads, small["grads"]):
+        gh = g.cpu().numpy().astype(np.float64)
+        assert np.all(np.isfinite(gh))
+        assert np.linalg.norm(gh - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
 def test_gradient_steps_reduce_loss(small):
     p_t = [_dev(p).requires_grad_(True) for p in small["params"]]
     pix_t = _dev(small["pix"])

@@ -98,3 +98,44 @@ def test_loaded_model_renders_like_the_direct_one(tmp_path, wl):
         r.close()
     assert np.array_equal(imgs[0], imgs[1])
     assert np.array_equal(direct.bitfield(), loaded.bitfield())
+
+
+def test_reference_plain_export_layout(tmp_path):
+    """A plain export as the reference writes it (no "latent_offset" key):
+    NerfNetwork::init makes the n_output rows, log sigma first, the head's
+    latents (nerf_network.h:93, 334-342) -- latent_offset 0,
+    first_latent_is_density 1, n_latents = n_output."""
+    ref = S.small_plain_reference(width=32, height=32, n_output=8)
+    d = S.write_nerfacc_export(ref, str(tmp_path / "r"))
+    cfg = json.load(open(os.path.join(d, "config.json")))
+    assert "latent_offset" not in cfg["mlp_head"] and cfg["mlp_head"]["n_input"] == 24
+    _, _, net, params, _ = lib.nerfacc_read(d)
+    assert (net.latent_offset, net.n_latents, net.first_latent_is_density, net.base_n_output) == (0, 8, 1, 8)
+    for got, exp in zip(params, ref.params):
+        assert np.array_equal(got, np.asarray(exp, np.uint16))
+    # the same file without n_input (its default): still the reference layout
+    del cfg["mlp_head"]["n_input"]
+    cfg["mlp_base"]["n_output"] = 16
+    json.dump(cfg, open(os.path.join(d, "config.json"), "w"))
+    _, _, net, _, _ = lib.nerfacc_read(d)
+    assert (net.latent_offset, net.n_latents, net.first_latent_is_density) == (0, 16, 1)
+    # the synthetic separate-density layout carries its own key
+    sep = S.small_plain(width=32, height=32)
+    d2 = S.write_nerfacc_export(sep, str(tmp_path / "s"))
+    _, _, net2, _, _ = lib.nerfacc_read(d2)
+    assert (net2.latent_offset, net2.n_latents, net2.first_latent_is_density, net2.base_n_output) == (1, 8, 0, 9)
+    # a head n_input that is not latents + SH is refused
+    cfg2 = json.load(open(os.path.join(d2, "config.json")))
+    cfg2["mlp_head"]["n_input"] = 40
+    json.dump(cfg2, open(os.path.join(d2, "config.json"), "w"))
+    _expect_error(d2, lib.E_UNSUPPORTED)
+
+
+def test_param_counts_of_reference_layouts(oracle_mod):
+    for n_out in (1, 8, 16):
+        net = S.net_desc(layout="reference", n_output=n_out)
+        assert lib.net_param_counts(net) == S.param_counts(net) == oracle_mod.net_param_counts(net)
+    bad = S.net_desc(layout="reference", n_output=8)
+    bad.n_latents = 7
+    with pytest.raises(lib.FvcError):
+        lib.net_param_counts(bad)
