@@ -927,10 +927,12 @@ __device__ __forceinline__ void cache_head_inputs(const CacheView& cv, float4 re
     for (int k = 0; k < 8; k++) L[k] = (_Float16)(sums[k] / wd);
     if (di) L[0] = (_Float16)((logf(sums[0]) * wd) / wd);
   } else {
+    // the quotient is rounded to float, then to half, as the reference's
+    // __float2half(a / b) does (f16_of: not fused with div_recip's last fma)
     const float rw = 1.0f / wd;
 #pragma unroll
-    for (int k = 0; k < 8; k++) L[k] = (_Float16)div_recip(sums[k], wd, rw);
-    if (di) L[0] = (_Float16)div_recip(logf(sums[0]) * wd, wd, rw);
+    for (int k = 0; k < 8; k++) L[k] = f16_of(div_recip(sums[k], wd, rw));
+    if (di) L[0] = f16_of(div_recip(logf(sums[0]) * wd, wd, rw));
   }
   logsig = di ? L[0] : (_Float16)logf(fmaxf_(dd, 1e-6f));
 }

@@ -150,6 +150,16 @@ __device__ __forceinline__ float opaque_f(float v) {
   asm volatile("" : "+v"(v));
   return v;
 }
+// float -> half, round to nearest even, of the float value as computed.
+// Written as the instruction itself: the compiler would otherwise fuse a
+// product or fma that produces the float with the conversion
+// (v_fma_mix{lo,hi}_f16, one rounding instead of the reference's two, which
+// differs where the float is a half midpoint).
+__device__ __forceinline__ _Float16 f16_of(float v) {
+  uint32_t h;
+  asm volatile("v_cvt_f16_f32 %0, %1" : "=v"(h) : "v"(v));
+  return __builtin_bit_cast(_Float16, (uint16_t)h);
+}
 
 // SH degree 4 coefficients 4g..4g+3 of dir01*2-1 (my_fully_fused_mlp.cu:206-234),
 // rounded to fp16.
@@ -178,7 +188,9 @@ __device__ __forceinline__ void sh4_quarter(float dx01, float dy01, float dz01, 
     c2 = 1.4453057213202769f * z * (x2 - y2);
     c3 = 0.59004358992664352f * x * (-x2 + 3.0f * y2);
   }
-  o[0] = (_Float16)c0; o[1] = (_Float16)c1; o[2] = (_Float16)c2; o[3] = (_Float16)c3;
+  // rounded to float, then to half (the reference's __float2half of the
+  // float coefficient)
+  o[0] = f16_of(c0); o[1] = f16_of(c1); o[2] = f16_of(c2); o[3] = f16_of(c3);
 }
 
 // all 16 coefficients as 4 dwords of packed halves per 8 coefficients

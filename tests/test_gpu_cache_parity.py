@@ -63,17 +63,31 @@ def _gpu_cache_in_oracle(O, d, mode=None):
                                mode=O.EMU16 if mode is None else mode)
 
 
-def _froxels(info, rng, n=6000):
-    """Random froxels over the frustum, plus points on brick faces and on
-    the frustum's faces (corners outside)."""
-    dims = np.array([info.frustum_x, info.frustum_y, info.frustum_z], np.float64)
-    p = rng.random((n, 3)) * dims
-    k = n // 3
+def _froxels(r, rng, n=6000):
+    """Froxels around set voxels (offsets up to one voxel from the centre, so
+    corners fall in unset voxels, other bricks or outside the frustum), a
+    third of them next to a brick face, plus points on the frustum's faces."""
+    info = r.cache_info(0)
+    tab = r.brick_table(0)
+    isset = r.pool(0)["isset"].reshape(-1, 8, 8, 8)
+    bz, by, bx = np.nonzero(tab >= 0)
+    k = rng.integers(0, len(bz), n)
+    slot = tab[bz[k], by[k], bx[k]]
+    vox = np.zeros((n, 3), np.int64)
+    for i in range(n):
+        zz, yy, xx = np.nonzero(isset[slot[i]] == 1)
+        j = rng.integers(len(zz))
+        vox[i] = (bx[k[i]] * 8 + xx[j], by[k[i]] * 8 + yy[j], bz[k[i]] * 8 + zz[j])
+    p = vox + 0.5 + rng.uniform(-1.0, 1.0, (n, 3))
+    # next to a brick face: the voxel's x (y, z) at 7 or 8 within its brick
+    m = n // 3
     for a in range(3):
-        b = rng.integers(0, max(1, int(dims[a]) // 8 - 1), k)
-        p[:k, a] = np.where(rng.random(k) < 0.5, 8 * b + 7.5 + rng.random(k), p[:k, a])
-    p[k:k + 200, 0] = rng.random(200) * 0.5
-    p[k + 200:k + 400, 1] = dims[1] - rng.random(200) * 0.5
+        face = (vox[:m, a] // 8) * 8 + 7
+        p[:m, a] = np.where(rng.random(m) < 0.5, face + 0.5 + rng.random(m), p[:m, a])
+    dims = np.array([info.frustum_x, info.frustum_y, info.frustum_z], np.float64)
+    p = np.clip(p, 0.0, dims - 1e-3)
+    p[m:m + 100, 0] = rng.random(100) * 0.5
+    p[m + 100:m + 200, 1] = dims[1] - rng.random(100) * 0.5
     return p.astype(np.float32)
 
 
@@ -81,7 +95,7 @@ def _compare_lookup(got, ref):
     """Keep froxels whose lookup weight is positive (NaN otherwise on both
     sides); latents bit-exact, the density row within one fp16 ulp."""
     ok = np.all(np.isfinite(ref), axis=1)
-    assert ok.sum() > 500
+    assert ok.sum() > 0.5 * len(ok)
     assert np.all(np.isfinite(got[ok]))
     assert np.array_equal(got[ok, 1:].astype(np.float16).view(np.uint16),
                           ref[ok, 1:].astype(np.float16).view(np.uint16))
@@ -92,7 +106,7 @@ def _compare_lookup(got, ref):
 def test_lookup_matches_oracle_off_centre(which, small, unbounded, oracle_mod):
     d = small if which == "small" else unbounded
     cache = _gpu_cache_in_oracle(oracle_mod, d)
-    p = _froxels(d["r"].cache_info(0), np.random.default_rng(11))
+    p = _froxels(d["r"], np.random.default_rng(11))
     got = d["r"].debug_lookup(p).astype(np.float64)
     ref = cache.lookup(p, di=False)
     _compare_lookup(got, ref)
@@ -103,7 +117,7 @@ def test_lookup_matches_oracle_off_centre(which, small, unbounded, oracle_mod):
 def test_lookup_settings_match_oracle(fn, inter, small, oracle_mod):
     r = small["r"]
     cache = _gpu_cache_in_oracle(oracle_mod, small).set_options(interp_fn=fn, inter_brick=inter)
-    p = _froxels(r.cache_info(0), np.random.default_rng(12 + fn))
+    p = _froxels(r, np.random.default_rng(12 + fn))
     r.cache_settings(fn, inter)
     try:
         got = r.debug_lookup(p).astype(np.float64)
@@ -148,7 +162,7 @@ def test_classification_bit_exact(which, small, unbounded, oracle_mod):
     nr, nc = int(comp[..., 0].sum()), int(comp[..., 1].sum())
     print(f"[classify] {which}: {nr} resampled, {nc} cached samples composited; "
           f"{int((o_cnt[..., 0] > K).sum())} rays with more than {K} samples", flush=True)
-    assert nc > 0 and nr > 0
+    assert nc > 0 and (nr > 0 or which == "small")   # the small scene's 2 degree yaw stays inside the cache
 
 
 @pytest.mark.parametrize("mode", [abi.RENDER_ONLY_CACHE, abi.RENDER_ONLY_RESAMPLED])
