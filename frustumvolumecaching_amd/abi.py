@@ -157,9 +157,9 @@ class KernelStats(C.Structure):
                 ("units", C.c_double), ("alg_bytes", C.c_double), ("alg_flops", C.c_double)]
 
 
-# voxel record of the brick pool: 8 fp16 latents, density, isset, 2 pad floats
+# voxel record of the brick pool export: 8 fp16 latents, density, isset, density and isset of the +z neighbour
 VOXEL_DTYPE = np.dtype([("lat", np.uint16, 8), ("density", np.float32), ("isset", np.float32),
-                        ("xnext", np.float32, 2)])
+                        ("znext", np.float32, 2)])
 assert VOXEL_DTYPE.itemsize == 32
 
 

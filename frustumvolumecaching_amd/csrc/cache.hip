@@ -286,7 +286,7 @@ __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list,
       const float amax = __builtin_bit_cast(float, amax_bits);
       const float scale = amax > 0.0f ? amax / 448.0f : 1.0f;
       if (lane == 0) scales[tile.slot] = scale;
-      Voxel8* brick = pool8 + (int64_t)tile.slot * BRICK_VOXELS;
+      Voxel8* brick = pool8 + (int64_t)tile.slot * BRICK_VOXELS + brick_voxel(lane & 7, lane >> 3, 0);
       int k = 0;
       for (int z = 0; z < BRICK; z++) {
         Voxel8 v;
@@ -310,15 +310,19 @@ __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list,
           v.density = 0.0f;
           v.isset = 0.0f;
         }
-        brick[z * 64 + lane] = v;
+        brick[z] = v;
       }
       tile.n_bricks++;
     } else if (!overflow) {
-      Voxel* brick = pool + (int64_t)tile.slot * BRICK_VOXELS;
-      int k = 0;
-      for (int z = 0; z < BRICK; z++) {
+      // the lane's brick column (x, y) = (lane & 7, lane >> 3): 8 consecutive
+      // records, written from z = 7 down so each record carries the density
+      // and isset of the one above it (znext)
+      Voxel* brick = pool + (int64_t)tile.slot * BRICK_VOXELS + brick_voxel(lane & 7, lane >> 3, 0);
+      int k = r.cnt - 1;
+      float up_d = 0.0f, up_s = 0.0f;
+      for (int z = BRICK - 1; z >= 0; z--) {
         Voxel v;
-        const bool has = k < r.cnt && f_meta[r.off + k] == z;
+        const bool has = k >= 0 && f_meta[r.off + k] == z;
         if (has) {
           const int64_t q = (int64_t)r.off + k;
           const uint16_t* o16 = out16 + q * 16;
@@ -326,18 +330,17 @@ __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list,
           v.density = expf(h2f(o16[sig_row]));
           if (di) v.lat[0] = __builtin_bit_cast(uint16_t, (_Float16)v.density);
           v.isset = 1.0f;
-          k++;
+          k--;
         } else {
           for (int i = 0; i < 8; i++) v.lat[i] = 0;
           v.density = 0.0f;
           v.isset = 0.0f;
         }
-        // lane = 8 y + x: lane + 1 holds the +x neighbour of the brick row
-        const float nd = __shfl_down(v.density, 1, 64), ns = __shfl_down(v.isset, 1, 64);
-        const bool row_end = (lane & 7) == 7;
-        v.xnext[0] = row_end ? 0.0f : nd;
-        v.xnext[1] = row_end ? 0.0f : ns;
-        brick[z * 64 + lane] = v;
+        v.znext[0] = up_d;
+        v.znext[1] = up_s;
+        up_d = v.density;
+        up_s = v.isset;
+        brick[z] = v;
       }
       tile.n_bricks++;
     }
@@ -635,7 +638,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
         const int64_t kidx = in_fov[u] ? (int64_t)(int)f[u].y * cv.fx + (int)f[u].x : 0;
         known[u] = cv.known[kidx];
         slot[u] = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
-        loc[u] = (vz & 7) * 64 + (vy & 7) * 8 + (vx & 7);
+        loc[u] = brick_voxel(vx, vy, vz);
       }
 #pragma unroll
       for (int u = 0; u < CLS; u++) {
@@ -722,15 +725,6 @@ __device__ __forceinline__ void fp8x8_decode(uint32_t w0, uint32_t w1, float s, 
   out[4] = fp8_byte<0>(w1) * s; out[5] = fp8_byte<1>(w1) * s; out[6] = fp8_byte<2>(w1) * s; out[7] = fp8_byte<3>(w1) * s;
 }
 
-// The value of the other lane of the pair (2k, 2k + 1) (DPP quad_perm
-// [1, 0, 3, 2]; every lane of the wave active).
-__device__ __forceinline__ uint32_t swap_pair(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
-}
-__device__ __forceinline__ uint4 swap_pair(uint4 v) {
-  return make_uint4(swap_pair(v.x), swap_pair(v.y), swap_pair(v.z), swap_pair(v.w));
-}
-
 // The trilinear lookup of one cache sample by one lane: its 8 corners
 // (corner c = 4x + 2y + z; outside the frustum or in unset bricks the zero
 // voxel) summed in the reference's order ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7))
@@ -776,7 +770,6 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, float4 rec, f
   float2 dis[8];
   float scl[8];
   const Voxel* vp[8];  // FP16 pool: corner voxels (the zero voxel outside)
-  uint32_t vidx[8];    // ... their pool indices (slot * 512 + voxel)
   bool vok[8];
   // integer cell of the corner (0, 0, 0); corner coordinates are exact
   // small integers, so the frustum test is an unsigned compare
@@ -809,7 +802,7 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, float4 rec, f
     own[c] = c != 0 && other && inside;
     if (own[c]) slots[c] = cv.table[((vz >> 3) * (uint32_t)cv.by + (vy >> 3)) * (uint32_t)cv.bx + (vx >> 3)];
     ins[c] = inside;
-    locs[c] = (vz & 7u) * 64u + (vy & 7u) * 8u + (vx & 7u);
+    locs[c] = (uint32_t)brick_voxel((int)vx, (int)vy, (int)vz);
   }
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -825,57 +818,33 @@ __device__ __forceinline__ void lookup_sample(const CacheView& cv, float4 rec, f
       scl[c] = ok ? sc : 0.0f;
     } else {
       vok[c] = ok;
-      vidx[c] = (uint32_t)vi;
       vp[c] = ok ? cv.pool + vi : &k_zero_voxel;
-      if (c >= 4) lat[c] = *reinterpret_cast<const uint4*>(vp[c]->lat);  // c < 4: pair loads below
+      lat[c] = *reinterpret_cast<const uint4*>(vp[c]->lat);
       scl[c] = 0.0f;
     }
   }
   if (!LAT8) {
-    // density and isset: corner c (x offset 0) with its xnext, which is
-    // corner c + 4 when both are set voxels of one brick row; otherwise
-    // corner c + 4 loads its own (only those lanes access memory).  The
-    // 16-byte loads pass through an empty asm once every load is issued, so
-    // that they stay single loads and the masked loads stay masked.
-    //
-    // Lane pairs (2k, 2k + 1) fetch the x-offset-0 voxels of their two
-    // samples together: one instruction reads both halves of the even
-    // lane's voxel (latents to the even lane, density half to the odd one),
-    // the next both halves of the odd lane's, so the two lanes of each load
-    // hit one cache line; one pair swap then hands each lane the half it
-    // needs from its partner.
-    const bool odd = (threadIdx.x & 1) != 0;
-    uint4 r1[4], r2[4];
+    // density and isset: corner c (z offset 0) with its znext, which is
+    // corner c + 1 when both are voxels of one brick column; otherwise
+    // corner c + 1 loads its own (only those lanes access memory)
+    float4 dh[4];
     float2 d1[4];
     bool shared[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      // the partner's voxel, from its index (a pointer passed between lanes
-      // would lose its address space)
-      const uint32_t pvi = swap_pair(vidx[c]);
-      const bool pok = swap_pair((uint32_t)vok[c]) != 0u;
-      const Voxel* pv = pok ? cv.pool + pvi : &k_zero_voxel;
-      const char* a1 = reinterpret_cast<const char*>(odd ? pv : vp[c]);  // the even lane's voxel
-      const char* a2 = reinterpret_cast<const char*>(odd ? vp[c] : pv);  // the odd lane's voxel
-      const int half = odd ? 16 : 0;
-      r1[c] = *reinterpret_cast<const uint4*>(a1 + half);
-      r2[c] = *reinterpret_cast<const uint4*>(a2 + half);
-      shared[c] = vok[c] && vok[c + 4] && !crx;
+    for (int p = 0; p < 4; p++) {
+      const int c = 2 * p;
+      dh[p] = *reinterpret_cast<const float4*>(&vp[c]->density);
+      shared[p] = vok[c] && vok[c + 1] && !crz;
     }
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      d1[c] = make_float2(0.0f, 0.0f);
-      if (!shared[c]) d1[c] = *reinterpret_cast<const float2*>(&vp[c + 4]->density);
+    for (int p = 0; p < 4; p++) {
+      d1[p] = make_float2(0.0f, 0.0f);
+      if (!shared[p]) d1[p] = *reinterpret_cast<const float2*>(&vp[2 * p + 1]->density);
     }
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      // even lane: r1 = its latents, r2 = the odd lane's latents (sent);
-      // odd lane: r1 = the even lane's density half (sent), r2 = its own
-      const uint4 recv = swap_pair(odd ? r1[c] : r2[c]);
-      lat[c] = odd ? recv : r1[c];
-      const uint4 dh = odd ? r2[c] : recv;  // density, isset, xnext of the own voxel
-      dis[c] = make_float2(__builtin_bit_cast(float, dh.x), __builtin_bit_cast(float, dh.y));
-      dis[c + 4] = shared[c] ? make_float2(__builtin_bit_cast(float, dh.z), __builtin_bit_cast(float, dh.w)) : d1[c];
+    for (int p = 0; p < 4; p++) {
+      dis[2 * p] = make_float2(dh[p].x, dh[p].y);
+      dis[2 * p + 1] = shared[p] ? make_float2(dh[p].z, dh[p].w) : d1[p];
     }
   }
   // per-corner products, then the reference's summation tree.  FP16
@@ -1357,9 +1326,8 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats)
 }
 
 // ---------------------------------------------------------------- inspection
-// The cache head's lookup alone at given froxels (one lane per froxel, full
-// waves: the lookup exchanges halves between lane pairs): [density row,
-// latents 0..7] per froxel, as the head receives them.
+// The cache head's lookup alone at given froxels (one lane per froxel):
+// [density row, latents 0..7] per froxel, as the head receives them.
 template <bool LAT8>
 __global__ __launch_bounds__(256) void k_debug_lookup(const float* __restrict__ fr, int n, CacheView cv, int di,
                                                       float* __restrict__ out9) {

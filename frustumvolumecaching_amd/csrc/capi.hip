@@ -609,28 +609,35 @@ int fvc_cache_get_pool_decoded(fvc_renderer* h, int which, float* out, int64_t n
     if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
     const int64_t nv = (int64_t)C.n_used * BRICK_VOXELS;
     REQUIRE(n >= nv * 10, "buffer too small");
+    // voxel i of the export (reference order) is record src(i) of the pool
+    auto src = [](int64_t i) {
+      const int64_t b = i / BRICK_VOXELS, c = i % BRICK_VOXELS;
+      return b * BRICK_VOXELS + brick_voxel((int)(c & 7), (int)((c >> 3) & 7), (int)(c >> 6));
+    };
     if (C.lat8) {
       std::vector<Voxel8> v((size_t)nv);
       std::vector<float> sc((size_t)std::max(C.n_used, 1));
       if (nv) FVC_HIP(hipMemcpy(v.data(), C.pool.p, nv * sizeof(Voxel8), hipMemcpyDeviceToHost));
       if (C.n_used) FVC_HIP(hipMemcpy(sc.data(), C.scales.p, C.n_used * sizeof(float), hipMemcpyDeviceToHost));
       for (int64_t i = 0; i < nv; i++) {
+        const Voxel8& x = v[src(i)];
         for (int k = 0; k < 8; k++)
-          out[i * 10 + k] = e4m3_value((uint8_t)(v[i].lat[k >> 2] >> (8 * (k & 3)))) * sc[i / BRICK_VOXELS];
-        out[i * 10 + 8] = v[i].density;
-        out[i * 10 + 9] = v[i].isset;
+          out[i * 10 + k] = e4m3_value((uint8_t)(x.lat[k >> 2] >> (8 * (k & 3)))) * sc[i / BRICK_VOXELS];
+        out[i * 10 + 8] = x.density;
+        out[i * 10 + 9] = x.isset;
       }
     } else {
       std::vector<Voxel> v((size_t)nv);
       if (nv) FVC_HIP(hipMemcpy(v.data(), C.pool.p, nv * sizeof(Voxel), hipMemcpyDeviceToHost));
       for (int64_t i = 0; i < nv; i++) {
+        const Voxel& x = v[src(i)];
         for (int k = 0; k < 8; k++) {
           _Float16 hv;
-          std::memcpy(&hv, &v[i].lat[k], 2);
+          std::memcpy(&hv, &x.lat[k], 2);
           out[i * 10 + k] = (float)hv;
         }
-        out[i * 10 + 8] = v[i].density;
-        out[i * 10 + 9] = v[i].isset;
+        out[i * 10 + 8] = x.density;
+        out[i * 10 + 9] = x.isset;
       }
     }
   });
@@ -700,9 +707,18 @@ int fvc_cache_get_pool(fvc_renderer* h, int which, void* out, int64_t n_bytes) {
     REQUIRE(h && out, "null argument");
     const CacheStore& C = pick(h, which);
     if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
-    const int64_t have = (int64_t)C.n_used * BRICK_VOXELS * (int64_t)(C.lat8 ? sizeof(Voxel8) : sizeof(Voxel));
-    REQUIRE(n_bytes <= have, "request exceeds the used part of the pool");
-    FVC_HIP(hipMemcpy(out, C.pool.p, (size_t)n_bytes, hipMemcpyDeviceToHost));
+    const size_t rec = C.lat8 ? sizeof(Voxel8) : sizeof(Voxel);
+    const int64_t nv = (int64_t)C.n_used * BRICK_VOXELS;
+    REQUIRE(n_bytes >= 0 && n_bytes <= nv * (int64_t)rec, "request exceeds the used part of the pool");
+    // records in the reference's voxel order (x fastest inside a brick)
+    std::vector<uint8_t> raw((size_t)(nv * rec));
+    if (nv) FVC_HIP(hipMemcpy(raw.data(), C.pool.p, raw.size(), hipMemcpyDeviceToHost));
+    uint8_t* o = static_cast<uint8_t*>(out);
+    for (int64_t i = 0; i * (int64_t)rec < n_bytes; i++) {
+      const int64_t b = i / BRICK_VOXELS, c = i % BRICK_VOXELS;
+      const int64_t j = b * BRICK_VOXELS + brick_voxel((int)(c & 7), (int)((c >> 3) & 7), (int)(c >> 6));
+      std::memcpy(o + i * rec, raw.data() + j * rec, (size_t)std::min<int64_t>((int64_t)rec, n_bytes - i * (int64_t)rec));
+    }
   });
 }
 

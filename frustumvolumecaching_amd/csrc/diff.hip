@@ -729,7 +729,7 @@ __global__ __launch_bounds__(256) void k_diffc_emit(int n, MarchGrid mg, Contrac
 // corners (x, y) = (g >> 1, g & 1), z = 0 / 1; every lane ends with the sums
 struct LookupLane {
   float w[2];
-  int64_t v[2];  // voxel index, -1: outside or unset brick
+  int64_t v[2];  // voxel index in the reference's order (g_pool), -1: outside or unset brick
   float wsum, dsum, lat[8];
 };
 __device__ __forceinline__ void lookup_f32(const CacheView& cv, V3 f, int g, LookupLane& L) {
@@ -755,9 +755,8 @@ __device__ __forceinline__ void lookup_f32(const CacheView& cv, V3 f, int g, Loo
       const int vx = (int)px, vy = (int)py, vz = (int)pz;
       const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
       if (slot >= 0) {
-        const int64_t v = (int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7));
-        L.v[cc] = v;
-        const Voxel vox = cv.pool[v];
+        L.v[cc] = (int64_t)slot * BRICK_VOXELS + canon_voxel(vx, vy, vz);
+        const Voxel vox = cv.pool[(int64_t)slot * BRICK_VOXELS + brick_voxel(vx, vy, vz)];
         cw = w * vox.isset;
         cd = w * vox.density;
 #pragma unroll

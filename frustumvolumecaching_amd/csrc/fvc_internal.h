@@ -166,14 +166,22 @@ static const int BRICK_VOXELS = 512;
 // pool capacity limit: voxel indices (slot * 512 + voxel) stay below 2^32
 static const int MAX_BRICKS = (1 << 23) - 1;
 
-// voxel record of the brick pool (32 bytes).  xnext repeats density and
-// isset of the +x neighbour in the same brick row (zeros at x = 7), so that
-// one 16-byte load reads both corners of an x pair of a trilinear lookup.
+// Voxel order inside a brick of the pool: z fastest, then y, then x.  A
+// ray's consecutive samples are one froxel apart in z (the froxel z is the
+// step index), so the trilinear corners of neighbouring samples -- the lanes
+// of a wave in the cache head -- share cache lines.  Exports and voxel
+// gradients use the reference's order (x fastest, z slowest: canon_voxel).
+FVC_HD int brick_voxel(int x, int y, int z) { return (x & 7) * 64 + (y & 7) * 8 + (z & 7); }
+FVC_HD int canon_voxel(int x, int y, int z) { return (z & 7) * 64 + (y & 7) * 8 + (x & 7); }
+
+// voxel record of the brick pool (32 bytes).  znext repeats density and
+// isset of the +z neighbour in the same brick column (zeros at z = 7), so
+// that one 16-byte load reads both corners of a z pair of a trilinear lookup.
 struct alignas(16) Voxel {
   uint16_t lat[8];   // fp16 latents
   float density;
   float isset;
-  float xnext[2];
+  float znext[2];
 };
 
 // voxel record with FP8-E4M3 latents (16 bytes): codes of l / s with the

@@ -116,7 +116,7 @@ __device__ __forceinline__ int classify_sample(const CacheView& cv, V3 oc, V3 dc
   if (!inside || unsampled) return SAMPLE_RESAMPLE;
   const int vx = (int)floorf(f.x), vy = (int)floorf(f.y), vz = (int)floorf(f.z);
   const int slot = cv.table[((int64_t)(vz >> 3) * cv.by + (vy >> 3)) * cv.bx + (vx >> 3)];
-  if (slot >= 0 && vox_density(cv, (int64_t)slot * BRICK_VOXELS + ((vz & 7) * 64 + (vy & 7) * 8 + (vx & 7))) > 0.0f)
+  if (slot >= 0 && vox_density(cv, (int64_t)slot * BRICK_VOXELS + brick_voxel(vx, vy, vz)) > 0.0f)
     return SAMPLE_CACHE;
   return SAMPLE_SKIP;
 }

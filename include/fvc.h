@@ -476,9 +476,10 @@ int fvc_cache_get_brick_table(fvc_renderer* renderer, int which, int32_t* out_ho
 /* known_ranges_to float[H][W] (cache.h:36) */
 int fvc_cache_get_known_ranges(fvc_renderer* renderer, int which, float* out_host, int64_t n);
 /* Voxel records of the first n_bricks pool slots: per voxel
- * {half latents[8]; float density; float isset; float xnext[2]} = 32 bytes,
- * voxel order z,y,x inside a brick; xnext repeats density and isset of the
- * +x neighbour in the brick row (zeros at x = 7). */
+ * {half latents[8]; float density; float isset; float znext[2]} = 32 bytes,
+ * voxel order z,y,x inside a brick (x fastest, as the reference's bricks;
+ * the pool itself keeps z fastest); znext repeats density and isset of the
+ * +z neighbour in the brick column (zeros at z = 7). */
 int fvc_cache_get_pool(fvc_renderer* renderer, int which, void* out_host, int64_t n_bytes);
 
 /* Ray marcher alone (sample_performance_kernel, performance_renderer.cu:15-69,

@@ -184,11 +184,12 @@ def test_fill_pool_matches_oracle(renderer, filled):
     assert_fp16_close(lat, op[..., :8])
     dens_rel = np.abs(gp["density"] - op[..., 8]) / np.maximum(op[..., 8], 1e-30)
     assert dens_rel.max() < 5e-3
-    # xnext: density and isset of the +x neighbour in the brick row, zeros at x = 7
-    xn = gp["xnext"]
-    assert np.array_equal(xn[..., :7, 0].view(np.uint32), gp["density"][..., 1:].view(np.uint32))
-    assert np.array_equal(xn[..., :7, 1].view(np.uint32), gp["isset"][..., 1:].view(np.uint32))
-    assert not np.any(xn[..., 7, :])
+    # znext: density and isset of the +z neighbour in the brick column, zeros
+    # at z = 7 (export order [brick][z][y][x])
+    zn = gp["znext"]
+    assert np.array_equal(zn[:, :7, ..., 0].view(np.uint32), gp["density"][:, 1:].view(np.uint32))
+    assert np.array_equal(zn[:, :7, ..., 1].view(np.uint32), gp["isset"][:, 1:].view(np.uint32))
+    assert not np.any(zn[:, 7])
 
 
 def test_cached_render_matches_oracle(renderer, filled, omodel, wl, oracle_mod):
