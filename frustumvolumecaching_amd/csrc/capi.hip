@@ -376,6 +376,16 @@ int fvc_render_full(fvc_renderer* h, const fvc_raymarch_info* rm, uint8_t* rgba,
   });
 }
 
+int fvc_render_full_noalloc(fvc_renderer* h, const fvc_raymarch_info* rm, uint8_t* rgba, float* result,
+                            fvc_render_stats* stats) {
+  return guard([&] {
+    REQUIRE(h, "null renderer");
+    check_rm(rm);
+    std::scoped_lock lk(h->r.read_mutex);
+    render_full_noalloc(h->r, *rm, rgba, result, stats);
+  });
+}
+
 int fvc_cache_fill(fvc_renderer* h, const fvc_raymarch_info* rm, fvc_init_stats* stats) {
   return guard([&] {
     REQUIRE(h, "null renderer");

@@ -444,6 +444,10 @@ void diff_forward(Renderer& r, const fvc_raymarch_info& rm, const int* pix, int 
 void diff_backward(Renderer& r, const float* d_rgb, float* g_pos, float* g_base, float* g_head, float* g_pool);
 void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts, float* t0s);
 void debug_lookup(Renderer& r, int which, const float* fr_host, int64_t n, float* out_host);
+// full render with its per-frame buffers in the library's static scratch
+void render_full_noalloc(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, float* result,
+                         fvc_render_stats* stats);
+size_t noalloc_scratch_bytes();
 void debug_render_cached(Renderer& r, const fvc_raymarch_info& rm, int k, int8_t* codes_host, int32_t* classified_host,
                          int32_t* composited_host, fvc_render_stats* stats);
 void resize_renderbuffers(Renderer& r, int w, int h);

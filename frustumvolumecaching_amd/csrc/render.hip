@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstring>
 #include <functional>
+#include <mutex>
 
 #include "march.h"
 #include "render_common.h"
@@ -551,6 +552,151 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
 
 void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
   render_full(r, Views(&rm, 1, &rgba), stats);
+}
+
+// ---------------------------------------------------------------- no-allocation render
+// The full render of one view with every per-frame device buffer (ray state,
+// sample streams, counters) inside `scratch`, a file-scope device array of
+// NOALLOC_SCRATCH_BYTES: for callers that cannot allocate device memory.
+// Views of at most NOALLOC_MAX_PIXELS pixels; at most NOALLOC_STEPS samples
+// per ray and NOALLOC_SAMPLES per launch; rays step every bin (no step
+// table: the samples are the same, bit for bit).  Each GPU holds its own
+// copy of the array; calls that overlap on one GPU race on it.
+static constexpr int64_t NOALLOC_MAX_PIXELS = 256 * 256;
+static constexpr int NOALLOC_STEPS = 16;
+static constexpr int64_t NOALLOC_SAMPLES = (1 << 19) + 64;
+static constexpr int NOALLOC_MAX_ITERS = 256;
+static constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+static constexpr size_t NOALLOC_CTL = align256(CTL_WORDS * sizeof(int));
+static constexpr size_t NOALLOC_PER_PIXEL = 2 * sizeof(RayDyn) + sizeof(RayState) + 2 * sizeof(int) +
+                                            2 * sizeof(float4) + sizeof(float4) + sizeof(int4) + sizeof(int) +
+                                            NOALLOC_STEPS * sizeof(float);
+static constexpr size_t NOALLOC_PER_SAMPLE = 3 * sizeof(float) + 3 * sizeof(float) + sizeof(float) + sizeof(uint2);
+static constexpr size_t NOALLOC_SCRATCH_BYTES =
+    NOALLOC_CTL + align256(NOALLOC_MAX_PIXELS * NOALLOC_PER_PIXEL) + 16 * 256 +
+    align256(NOALLOC_SAMPLES * NOALLOC_PER_SAMPLE);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// the device address of `scratch` on each device, taken once per device
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[64] = {};
+
+static unsigned char* scratch_on(int device) {
+  if (device < 0 || device >= 64) throw Error(FVC_E_INVALID, "device index out of range");
+  std::scoped_lock lk(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    FVC_HIP(hipGetDevice(&cur));
+    if (cur != device) FVC_HIP(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) FVC_HIP(hipSetDevice(cur));
+    FVC_HIP(e);
+    g_scratch_ptr[device] = ptr;
+  }
+  return static_cast<unsigned char*>(g_scratch_ptr[device]);
+}
+
+size_t noalloc_scratch_bytes() { return NOALLOC_SCRATCH_BYTES; }
+
+void render_full_noalloc(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, float* result, fvc_render_stats* stats) {
+  const Model& m = *r.model;
+  const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
+  const int64_t n_px = (int64_t)W * H;
+  if (n_px > NOALLOC_MAX_PIXELS)
+    throw Error(FVC_E_INVALID, "the no-allocation render takes views of at most 65536 pixels");
+  hipStream_t s = r.stream();
+  unsigned char* base = scratch_on(m.device);
+  // carve the scratch: counters, per-pixel ray state, sample streams
+  int* ctl = reinterpret_cast<int*>(base);
+  unsigned char* p = base + NOALLOC_CTL;
+  auto take = [&](size_t bytes) {
+    unsigned char* q = p;
+    p += align256(bytes);
+    return q;
+  };
+  const int64_t P = n_px;
+  RayDyn* rays[2] = {reinterpret_cast<RayDyn*>(take(P * sizeof(RayDyn))),
+                     reinterpret_cast<RayDyn*>(take(P * sizeof(RayDyn)))};
+  RayState* rconst = reinterpret_cast<RayState*>(take(P * sizeof(RayState)));
+  int* pix[2] = {reinterpret_cast<int*>(take(P * sizeof(int))), reinterpret_cast<int*>(take(P * sizeof(int)))};
+  float4* rres[2] = {reinterpret_cast<float4*>(take(P * sizeof(float4))),
+                     reinterpret_cast<float4*>(take(P * sizeof(float4)))};
+  float4* results = reinterpret_cast<float4*>(take(P * sizeof(float4)));
+  int4* entry = reinterpret_cast<int4*>(take(P * sizeof(int4)));
+  int* alive = reinterpret_cast<int*>(take(P * sizeof(int)));
+  float* t_scr = reinterpret_cast<float*>(take(P * NOALLOC_STEPS * sizeof(float)));
+  const int64_t S = NOALLOC_SAMPLES;
+  float* s_pos = reinterpret_cast<float*>(take(S * 3 * sizeof(float)));
+  float* s_dir = reinterpret_cast<float*>(take(S * 3 * sizeof(float)));
+  float* s_t0 = reinterpret_cast<float*>(take(S * sizeof(float)));
+  uint2* s_out = reinterpret_cast<uint2*>(take(S * sizeof(uint2)));
+  if ((size_t)(p - base) > NOALLOC_SCRATCH_BYTES) throw Error(FVC_E_INVALID, "scratch layout exceeds its array");
+
+  MarchGrid mg;
+  mg.occ = m.occ;
+  mg.st = rm.stepsize_info;
+  mg.tab = step_view(rm.stepsize_info);  // no table rows: every bin is stepped
+  mg.tab.t = nullptr;
+  mg.tab.off = nullptr;
+  mg.tab.n_rows = 0;
+  mg.bf64 = m.bitfield.as<uint64_t>();
+
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(CTL_LOG), 0, s, ctl, CTL_LOG);
+  hipLaunchKernelGGL(k_init_rays, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, 0, cvt(m.scene.aabb_from),
+                     cvt(m.scene.aabb_to), rays[0], rconst, pix[0], rres[0], results, ctl + CTL_ALIVE);
+  FVC_HIP(hipGetLastError());
+  const int64_t cap = S - 64;
+  const int64_t n_bound = std::min<int64_t>(cap, P * NOALLOC_STEPS);
+  const dim3 grid((unsigned)((P + 255) / 256));
+  auto iteration = [&](int it) {
+    const int cur = it & 1;
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, cap, NOALLOC_STEPS, cap, 0);
+    hipLaunchKernelGGL(k_march, grid, dim3(256), 0, s, ctl, cur, mg, m.con, m.coarse.as<uint32_t>(), rays[cur], rconst,
+                       pix[cur], entry, alive, t_scr, s_pos, s_dir, s_t0);
+    FVC_HIP(hipGetLastError());
+    launch_network_full(m, s_pos, s_dir, n_bound, reinterpret_cast<uint32_t*>(s_out), s, ctl + CTL_R, r.mlp_rows);
+    hipLaunchKernelGGL(k_composite, grid, dim3(256), 0, s, ctl, cur, rm.stepsize_info, m.scene.alpha_thre, rays[cur],
+                       pix[cur], rres[cur], entry, alive, s_out, s_t0, rays[cur ^ 1], pix[cur ^ 1], rres[cur ^ 1],
+                       results);
+    FVC_HIP(hipGetLastError());
+  };
+  // bursts of sub-iterations planned on the device; the host reads the
+  // alive count once per burst (a copy to a host variable)
+  int it = 0;
+  bool done = false;
+  while (!done) {
+    if (it >= NOALLOC_MAX_ITERS)
+      throw Error(FVC_E_INVALID, "the no-allocation render needs more than 256 sub-iterations for this view");
+    for (int b = 0; b < 4 && it < NOALLOC_MAX_ITERS; b++, it++) iteration(it);
+    int n_alive = 0;
+    FVC_HIP(hipMemcpyAsync(&n_alive, ctl + CTL_ALIVE + (it & 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    FVC_HIP(hipStreamSynchronize(s));
+    done = n_alive == 0;
+  }
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, cap, NOALLOC_STEPS, cap, 1);
+  if (rgba) {
+    hipLaunchKernelGGL(k_write_image, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, W, H, 0, results,
+                       (uchar4*)rgba, rm.sample_index);
+    FVC_HIP(hipGetLastError());
+  }
+  if (result) FVC_HIP(hipMemcpyAsync(result, results, P * sizeof(float4), hipMemcpyDeviceToDevice, s));
+  IterLog lg[NOALLOC_MAX_ITERS];
+  FVC_HIP(hipMemcpyAsync(lg, ctl + CTL_LOG, sizeof(IterLog) * it, hipMemcpyDeviceToHost, s));
+  FVC_HIP(hipStreamSynchronize(s));
+  if (stats) {
+    int64_t n_eval = 0;
+    int n_real = 0;
+    for (int i = 0; i < it; i++) {
+      n_eval += lg[i].n_r;
+      if (lg[i].alive > 0) n_real = i + 1;
+    }
+    std::memset(stats, 0, sizeof(*stats));
+    stats->samples_ppx = (float)((double)n_eval / (double)P);
+    stats->resample_samples_evaluated = stats->samples_ppx;
+    stats->n_resamples = (uint64_t)n_eval;
+    stats->n_subiters = n_real;
+  }
 }
 
 void debug_march(Renderer& r, const fvc_raymarch_info& rm, int max_per_ray, int32_t* counts_host, float* t0_host) {

@@ -85,6 +85,7 @@ _SIGS = {
     "fvc_cache_get_pool": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "fvc_debug_march": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, _P, _P]),
     "fvc_cache_set_render_mode": (C.c_int, [_P, C.c_int]),
+    "fvc_render_full_noalloc": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), _P, _P, C.POINTER(abi.RenderStats)]),
     "fvc_debug_cache_lookup": (C.c_int, [_P, C.c_int, _P, C.c_int64, _P]),
     "fvc_debug_render_cached": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, _P, _P, _P,
                                           C.POINTER(abi.RenderStats)]),
@@ -319,6 +320,15 @@ class Renderer:
         st = abi.RenderStats()
         _after_caller(rgba)
         check(load().fvc_render_full(self.h, C.byref(rm), _addr(rgba), C.byref(st)))
+        self.last_render_stats = st
+        return st
+
+    def render_full_noalloc(self, rm, rgba=None, result=None):
+        """render_full with the per-frame buffers in the library's static
+        scratch (views up to 256 x 256; no device allocation)."""
+        st = abi.RenderStats()
+        _after_caller(rgba, result)
+        check(load().fvc_render_full_noalloc(self.h, C.byref(rm), _addr(rgba), _addr(result), C.byref(st)))
         self.last_render_stats = st
         return st
 

@@ -330,6 +330,18 @@ int fvc_resize_renderbuffers(fvc_renderer* renderer, int width, int height);
 /* PerformanceRenderer::render (performance_renderer.cu:214-286):
  * rgba_dev uchar4[H][W] (running mean over sample_index, as the reference). */
 int fvc_render_full(fvc_renderer* renderer, const fvc_raymarch_info* rm, uint8_t* rgba_dev, fvc_render_stats* stats);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): fvc_render_full for views of at most 256 x 256 pixels, with
+ * every per-frame device buffer (ray state, sample streams, counters) in one
+ * static device array of the library (about 33 MB, held by each GPU the
+ * library runs on until the process ends) instead of the renderer's
+ * buffers; no device or pinned memory is allocated.  rgba_dev (uchar4, may
+ * be NULL) as fvc_render_full; result_dev (float4 rgb + T per pixel, may be
+ * NULL) receives the float result.  Larger views return FVC_E_INVALID.
+ * Calls that overlap on one GPU race on the array (one at a time per GPU). */
+int fvc_render_full_noalloc(fvc_renderer* renderer, const fvc_raymarch_info* rm, uint8_t* rgba_dev, float* result_dev,
+                            fvc_render_stats* stats);
 /* CacheRenderer::initCacheBlockwise (cache_renderer.cu:931-1051) into the fill target. */
 int fvc_cache_fill(fvc_renderer* renderer, const fvc_raymarch_info* rm, fvc_init_stats* stats);
 /* CachePerformanceRenderer::render (cache_performance_renderer.cu:1059-1263). */
