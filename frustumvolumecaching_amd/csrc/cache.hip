@@ -65,107 +65,177 @@ __global__ __launch_bounds__(64) void k_fill_init(fvc_raymarch_info rm, V3 aabb_
   }
 }
 
-// alive tiles in tile order (deterministic compaction), single block
-__global__ __launch_bounds__(1024) void k_fill_list(int n_tiles, const FillTile* __restrict__ tiles,
-                                                    int* __restrict__ list, int* __restrict__ count) {
-  __shared__ int part[1024];
-  const int tid = threadIdx.x;
-  const int per = (n_tiles + 1023) / 1024;
-  const int b = tid * per, e = min(n_tiles, b + per);
-  int c = 0;
-  for (int i = b; i < e; i++) c += tiles[i].alive != 0;
-  part[tid] = c;
+// Fill counters (Renderer::f_counters): CNT_* slots, then per sub-iteration
+// {alive tiles, samples} from FILL_LOG on.
+static const int FILL_LOG = 64;
+static const int FILL_MAX_ITERS = 1000;  // CacheRenderer::MAX_SUBITER
+static const int FILL_COUNTER_WORDS = FILL_LOG + 2 * (FILL_MAX_ITERS + 1);
+
+__global__ void k_set_ints(int* __restrict__ p, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// alive tiles in tile order (deterministic compaction); one block, each
+// thread takes a run of consecutive tiles (flags in a bit mask), one block
+// prefix count over the runs
+static const int SCAN_RUN = 32;  // tiles per thread: 32768 tiles per pass of 1024 threads
+__device__ __forceinline__ int block_excl_sum(int v, int& total) {
+  __shared__ int s_w[16];
+  const int wave = (int)(threadIdx.x >> 6), n_waves = (int)(blockDim.x >> 6);
+  int wt;
+  const int pre = wave_excl_scan(v, wt);
+  if (lane_id() == 0) s_w[wave] = wt;
   __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const int v = tid >= d ? part[tid - d] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
+  int off = 0, tot = 0;
+  for (int w = 0; w < n_waves; w++) {
+    const int c = s_w[w];
+    off += w < wave ? c : 0;
+    tot += c;
   }
-  int o = part[tid] - c;
-  for (int i = b; i < e; i++)
-    if (tiles[i].alive) list[o++] = i;
-  if (tid == 1023) *count = part[1023];
+  __syncthreads();
+  total = tot;
+  return off + pre;
+}
+
+__global__ __launch_bounds__(1024) void k_fill_list(int n_tiles, const FillTile* __restrict__ tiles,
+                                                    int* __restrict__ list, int* __restrict__ cnt, int it) {
+  __shared__ uint8_t flags[1024 * SCAN_RUN];
+  int base = 0;
+  for (int c0 = 0; c0 < n_tiles; c0 += 1024 * SCAN_RUN) {
+    // coalesced loads (independent, all in flight), then runs from LDS
+#pragma unroll
+    for (int k = 0; k < SCAN_RUN; k++) {
+      const int i = c0 + k * 1024 + (int)threadIdx.x;
+      flags[k * 1024 + threadIdx.x] = i < n_tiles ? (uint8_t)(tiles[i].alive != 0) : (uint8_t)0;
+    }
+    __syncthreads();
+    uint32_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_RUN; k++) bits |= (uint32_t)flags[threadIdx.x * SCAN_RUN + k] << k;
+    const int b = c0 + (int)threadIdx.x * SCAN_RUN;
+    int tot;
+    int o = base + block_excl_sum(__popc(bits), tot);
+    while (bits) {
+      const int k = __ffs(bits) - 1;
+      bits &= bits - 1;
+      list[o++] = b + k;
+    }
+    base += tot;
+  }
+  if (threadIdx.x == 0) {
+    cnt[CNT_TILES] = base;
+    cnt[CNT_SAMPLES] = 0;
+    if (it <= FILL_MAX_ITERS) cnt[FILL_LOG + 2 * it] = base;
+  }
 }
 
 // sample_cache_init_first_kernel + sample_cache_init_second_kernel
 // (cache_renderer.cu:83-287): skip empty slabs, emit the occupied samples of
-// the first non-empty one as a compacted stream.
-__global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list, fvc_stepsize_info st, OccParams occ,
-                                                    ContractParams con, const uint64_t* __restrict__ bf,
+// the first non-empty one as a compacted stream.  One wave per tile of the
+// alive list (grid-stride).  After an empty slab the wave jumps over the
+// slabs that every live lane's empty box covers: a lane's first bin that is
+// neither in its skip interval nor past its far bound lies at or after
+// step_from_t(min(hi, far)) - near_i - 2 (two bins of margin for rounding),
+// so no slab before the wave minimum of that bound (in slabs) holds a
+// sample or ends a ray, and the tile ends at the same slab as a walk.
+__global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list, const int* __restrict__ cnt,
+                                                    fvc_stepsize_info st, OccParams occ, ContractParams con,
+                                                    const uint64_t* __restrict__ bf,
                                                     const uint32_t* __restrict__ coarse_g, int bz,
                                                     FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
                                                     float* __restrict__ f_pos, float* __restrict__ f_t0,
                                                     int* __restrict__ f_meta, int* __restrict__ counter,
                                                     float* __restrict__ f_dir) {
+  const int n_list = cnt[CNT_TILES];
+  if ((int)blockIdx.x >= n_list) return;
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
   load_coarse(coarse_g, occ.nlvl, coarse);
-  const int t = list[blockIdx.x], lane = threadIdx.x;
-  FillRay r = rays[(int64_t)t * 64 + lane];
-  FillTile tile = tiles[t];
-  const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
-  const RaySlab rslab = ray_slab(o, d);
-  Skip sk = no_skip();
-  int slab = tile.slab;
-  unsigned mask = 0;
-  int total = 0;
-  while (true) {
-    if (wave_sum(r.alive) <= 0) break;
-    if (slab >= bz) {  // beyond the frustum: the column ends
-      r.alive = 0;
-      break;
-    }
-    float t1 = t_from_step(st, (float)(slab * BRICK) + st.near_i);
-    mask = 0;
-    for (int j = 0; j < BRICK; j++) {
-      const float t0 = t1;
-      const float dt = step_dt(st, t0);
-      t1 = t0 + dt;
-      const float tm = (t0 + t1) * 0.5f;
-      if (tm > r.far_) { r.alive = 0; continue; }
-      if (in_skip(sk, tm)) continue;
-      if (occ_query(o + tm * d, occ, bf, coarse, rslab, sk)) mask |= 1u << j;
-    }
-    total = wave_sum(__popc(mask));
-    if (total > 0) break;
-    slab++;
-  }
-  const bool tile_alive = wave_sum(r.alive) > 0;
-  const bool processed = total > 0 && tile_alive;
-  r.cnt = 0;
-  if (processed) {
-    const int n = __popc(mask);
-    const int off = wave_reserve(n, counter);
-    float t1 = t_from_step(st, (float)(slab * BRICK) + st.near_i);
-    int k = 0;
-    for (int j = 0; j < BRICK; j++) {
-      const float t0 = t1;
-      const float dt = step_dt(st, t0);
-      t1 = t0 + dt;
-      if (mask & (1u << j)) {
-        const float tm = (t0 + t1) * 0.5f;
-        const V3 p = contract(o + tm * d, con);
-        const int64_t q = (int64_t)off + k;
-        f_pos[3 * q] = p.x; f_pos[3 * q + 1] = p.y; f_pos[3 * q + 2] = p.z;
-        f_t0[q] = t0;
-        f_meta[q] = j;
-        if (f_dir) {  // split models: the fill ray is the initial view direction (cache_renderer.cu:899)
-          const V3 d01 = unit_to_01(d);
-          f_dir[3 * q] = d01.x; f_dir[3 * q + 1] = d01.y; f_dir[3 * q + 2] = d01.z;
+  const int lane = threadIdx.x;
+  for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
+    const int t = list[li];
+    FillRay r = rays[(int64_t)t * 64 + lane];
+    FillTile tile = tiles[t];
+    const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
+    const RaySlab rslab = ray_slab(o, d);
+    Skip sk = no_skip();
+    int slab = tile.slab;
+    unsigned mask = 0;
+    int total = 0;
+    while (true) {
+      if (wave_sum(r.alive) <= 0) break;
+      if (slab >= bz) {  // beyond the frustum: the column ends
+        r.alive = 0;
+        break;
+      }
+      float t1 = t_from_step(st, (float)(slab * BRICK) + st.near_i);
+      float tms[BRICK];
+#pragma unroll
+      for (int j = 0; j < BRICK; j++) {
+        const float t0 = t1;
+        t1 = t0 + step_dt(st, t0);
+        tms[j] = (t0 + t1) * 0.5f;
+      }
+      // bins inside the skip interval of an earlier empty bin need no query
+      mask = 0;
+#pragma unroll
+      for (int j = 0; j < BRICK; j++) {
+        const float tm = tms[j];
+        if (tm > r.far_) { r.alive = 0; continue; }
+        if (in_skip(sk, tm)) continue;
+        if (occ_query(o + tm * d, occ, bf, coarse, rslab, sk)) mask |= 1u << j;
+      }
+      total = wave_sum(__popc(mask));
+      if (total > 0) break;
+      slab++;
+      if (wave_sum(r.alive) > 0) {
+        int cand = 0x7fffffff;
+        if (r.alive) {
+          const float hi = fminf_(sk.hi, r.far_);
+          const float sl = fminf_((step_from_t(st, hi) - st.near_i - 2.0f) * 0.125f, 1.0e9f);
+          cand = sk.hi >= sk.lo ? (int)floorf(sl) : slab;
         }
-        k++;
+        const int nxt = wave_min(cand);
+        if (nxt > slab) slab = nxt < bz ? nxt : bz;
       }
     }
-    r.cnt = n;
-    r.off = off;
-  }
-  rays[(int64_t)t * 64 + lane] = r;
-  if (lane == 0) {
-    tile.slab = slab;
-    tile.alive = tile_alive;
-    tile.processed = processed;
-    tile.visible = 0;
-    tiles[t] = tile;
+    const bool tile_alive = wave_sum(r.alive) > 0;
+    const bool processed = total > 0 && tile_alive;
+    r.cnt = 0;
+    if (processed) {
+      const int n = __popc(mask);
+      const int off = wave_reserve(n, counter);
+      float t1 = t_from_step(st, (float)(slab * BRICK) + st.near_i);
+      int k = 0;
+      for (int j = 0; j < BRICK; j++) {
+        const float t0 = t1;
+        const float dt = step_dt(st, t0);
+        t1 = t0 + dt;
+        if (mask & (1u << j)) {
+          const float tm = (t0 + t1) * 0.5f;
+          const V3 p = contract(o + tm * d, con);
+          const int64_t q = (int64_t)off + k;
+          f_pos[3 * q] = p.x; f_pos[3 * q + 1] = p.y; f_pos[3 * q + 2] = p.z;
+          f_t0[q] = t0;
+          f_meta[q] = j;
+          if (f_dir) {  // split models: the fill ray is the initial view direction (cache_renderer.cu:899)
+            const V3 d01 = unit_to_01(d);
+            f_dir[3 * q] = d01.x; f_dir[3 * q + 1] = d01.y; f_dir[3 * q + 2] = d01.z;
+          }
+          k++;
+        }
+      }
+      r.cnt = n;
+      r.off = off;
+    }
+    rays[(int64_t)t * 64 + lane] = r;
+    if (lane == 0) {
+      tile.slab = slab;
+      tile.alive = tile_alive;
+      tile.processed = processed;
+      tile.visible = 0;
+      tiles[t] = tile;
+    }
   }
 }
 
@@ -179,177 +249,235 @@ __device__ __forceinline__ float out16_row(const uint2* out16, int64_t q, int r)
 // first loop of accumulateNew_cache_init_second_kernel (cache_renderer.cu:358-405):
 // transmittance over the slab's samples and the count of visible samples.
 // sig_row: row of log sigma in the network output (0; split models 8).
-__global__ __launch_bounds__(64) void k_fill_accum(const int* __restrict__ list, fvc_stepsize_info st, float alpha_thre,
+// Logs the sub-iteration's sample count.
+__global__ __launch_bounds__(64) void k_fill_accum(const int* __restrict__ list, int* __restrict__ cnt, int it,
+                                                   fvc_stepsize_info st, float alpha_thre,
                                                    const uint2* __restrict__ out16, const float* __restrict__ f_t0,
                                                    FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
                                                    int sig_row) {
-  const int t = list[blockIdx.x], lane = threadIdx.x;
-  if (!tiles[t].processed) return;
-  FillRay r = rays[(int64_t)t * 64 + lane];
-  int vis = 0;
-  for (int k = 0; k < r.cnt; k++) {
-    const int64_t q = (int64_t)r.off + k;
-    const float ls = out16_row(out16, q, sig_row);
-    const float t0 = f_t0[q];
-    const float dt = step_dt(st, t0);
-    const float density = expf(ls);
-    float alpha = 1.0f - expf(-density * dt);
-    alpha = fminf_(fmaxf_(alpha, 0.0f), 1.0f);
-    if (alpha > alpha_thre) {
-      r.T = r.T * (1.0f - alpha);
-      vis++;
+  const int n_list = cnt[CNT_TILES];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && it <= FILL_MAX_ITERS) cnt[FILL_LOG + 2 * it + 1] = cnt[CNT_SAMPLES];
+  const int lane = threadIdx.x;
+  for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
+    const int t = list[li];
+    if (!tiles[t].processed) continue;
+    FillRay r = rays[(int64_t)t * 64 + lane];
+    int vis = 0;
+    // a lane has at most 8 samples (one slab): all loaded before the
+    // sequential transmittance product
+    float lsv[BRICK], t0v[BRICK];
+#pragma unroll
+    for (int k = 0; k < BRICK; k++) {
+      // loads from a valid index (0 past the lane's samples)
+      const int64_t q = k < r.cnt ? (int64_t)r.off + k : 0;
+      lsv[k] = out16_row(out16, q, sig_row);
+      t0v[k] = f_t0[q];
     }
-    if (r.T < 1e-4f) r.alive = 0;
+#pragma unroll
+    for (int k = 0; k < BRICK; k++) {
+      if (k >= r.cnt) break;
+      const float ls = lsv[k];
+      const float t0 = t0v[k];
+      const float dt = step_dt(st, t0);
+      const float density = expf(ls);
+      float alpha = 1.0f - expf(-density * dt);
+      alpha = fminf_(fmaxf_(alpha, 0.0f), 1.0f);
+      if (alpha > alpha_thre) {
+        r.T = r.T * (1.0f - alpha);
+        vis++;
+      }
+      if (r.T < 1e-4f) r.alive = 0;
+    }
+    rays[(int64_t)t * 64 + lane].T = r.T;
+    rays[(int64_t)t * 64 + lane].alive = r.alive;
+    const int v = wave_sum(vis);
+    if (lane == 0) tiles[t].visible = v;
   }
-  rays[(int64_t)t * 64 + lane].T = r.T;
-  rays[(int64_t)t * 64 + lane].alive = r.alive;
-  const int v = wave_sum(vis);
-  if (lane == 0) tiles[t].visible = v;
 }
 
-// brick slot allocation in tile order; single block
-__global__ __launch_bounds__(1024) void k_fill_alloc(const int* __restrict__ list, const int* __restrict__ count,
-                                                     int capacity, FillTile* __restrict__ tiles,
-                                                     int* __restrict__ n_used, int* __restrict__ overflow_flag) {
-  __shared__ int part[1024];
-  const int n = *count;
-  const int tid = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int b = tid * per, e = min(n, b + per);
-  int c = 0;
-  for (int i = b; i < e; i++) {
-    const FillTile& ft = tiles[list[i]];
-    c += (ft.processed && ft.visible > 0);
-  }
-  part[tid] = c;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const int v = tid >= d ? part[tid - d] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
-  }
-  const int base = *n_used;
-  int o = base + part[tid] - c;
-  for (int i = b; i < e; i++) {
-    FillTile& ft = tiles[list[i]];
-    if (ft.processed && ft.visible > 0) {
-      ft.slot = o;
-      ft.overflow = o >= capacity;
-      if (ft.overflow) *overflow_flag = 1;
-      o++;
-    } else {
-      ft.overflow = 0;
+// brick slot allocation in list (= tile) order; one block, runs of
+// consecutive list entries per thread, one block prefix count per pass
+__global__ __launch_bounds__(1024) void k_fill_alloc(const int* __restrict__ list, int* __restrict__ cnt,
+                                                     int capacity, FillTile* __restrict__ tiles) {
+  __shared__ uint8_t flags[1024 * SCAN_RUN];
+  const int n = cnt[CNT_TILES];
+  int base = cnt[CNT_SLOTS];
+  bool any_overflow = false;
+  for (int c0 = 0; c0 < n; c0 += 1024 * SCAN_RUN) {
+#pragma unroll
+    for (int k = 0; k < SCAN_RUN; k++) {
+      const int i = c0 + k * 1024 + (int)threadIdx.x;
+      uint8_t f = 0;
+      if (i < n) {
+        const FillTile& ft = tiles[list[i]];
+        f = ft.processed && ft.visible > 0;
+      }
+      flags[k * 1024 + threadIdx.x] = f;
     }
+    __syncthreads();
+    uint32_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_RUN; k++) bits |= (uint32_t)flags[threadIdx.x * SCAN_RUN + k] << k;
+    const int b = c0 + (int)threadIdx.x * SCAN_RUN;
+    int tot;
+    int o = base + block_excl_sum(__popc(bits), tot);
+    while (bits) {
+      const int k = __ffs(bits) - 1;
+      bits &= bits - 1;
+      const int t = list[b + k];
+      tiles[t].slot = o;
+      tiles[t].overflow = o >= capacity;
+      any_overflow |= o >= capacity;
+      o++;
+    }
+    base += tot;
   }
+  if (any_overflow) cnt[CNT_OVERFLOW] = 1;
   __syncthreads();
-  if (tid == 1023) *n_used = base + part[1023];
+  if (threadIdx.x == 0) cnt[CNT_SLOTS] = base;
 }
 
 // brick write + slab advance (accumulateNew_cache_init_second_kernel,
 // cache_renderer.cu:405-528).  Each lane owns one (x,y) column of the brick
-// and writes its 8 voxels; voxels without a sample are zero.
+// and produces its 8 voxels (zeros where it has no sample); the brick is
+// assembled in LDS and stored as contiguous 16-byte chunks.
 // With FP8 latents (pool8 / scales) the brick's scale is the absmax of the
 // latents written into it over 448 (1 when all are zero); each latent is
 // stored as the E4M3 code (round to nearest even) of l / scale.
 // Latents come from rows latent_offset.., log sigma from row sig_row; di
 // (DensityIntermediates): latent 0 is replaced by the density as half
 // (cache_renderer.cu:481-486, 501).
-__global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list, int bx, int by, int latent_offset,
-                                                   const uint16_t* __restrict__ out16, const int* __restrict__ f_meta,
-                                                   const FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
-                                                   int* __restrict__ table, Voxel* __restrict__ pool,
-                                                   Voxel8* __restrict__ pool8, float* __restrict__ scales,
-                                                   int sig_row, int di) {
-  const int t = list[blockIdx.x], lane = threadIdx.x;
-  FillTile tile = tiles[t];
-  if (!tile.processed) return;
-  const FillRay r = rays[(int64_t)t * 64 + lane];
-  bool overflow = false;
-  if (tile.visible > 0) {
-    overflow = tile.overflow != 0;
-    if (lane == 0) table[((int64_t)tile.slab * by + t / bx) * bx + t % bx] = overflow ? -1 : tile.slot;
-    if (!overflow && pool8) {
-      // latent i of a sample's output (fp16 value)
-      auto lat = [&](const uint16_t* o16, int i) {
-        return (di && i == 0) ? (float)(_Float16)expf(h2f(o16[sig_row])) : h2f(o16[latent_offset + i]);
-      };
-      int amax_bits = 0;  // |latent| as float bits (non-negative floats order as integers)
-      for (int k = 0; k < r.cnt; k++) {
-        const uint16_t* o16 = out16 + ((int64_t)r.off + k) * 16;
-        for (int i = 0; i < 8; i++) {
-          const int b = __builtin_bit_cast(int, fabsf(lat(o16, i)));
-          amax_bits = b > amax_bits ? b : amax_bits;
+__global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list, const int* __restrict__ cnt, int bx,
+                                                   int by, int latent_offset, const uint16_t* __restrict__ out16,
+                                                   const int* __restrict__ f_meta, const FillRay* __restrict__ rays,
+                                                   FillTile* __restrict__ tiles, int* __restrict__ table,
+                                                   Voxel* __restrict__ pool, Voxel8* __restrict__ pool8,
+                                                   float* __restrict__ scales, int sig_row, int di) {
+  __shared__ uint4 brick_lds[BRICK_VOXELS * sizeof(Voxel) / 16];
+  const int n_list = cnt[CNT_TILES];
+  const int lane = threadIdx.x;
+  for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
+    const int t = list[li];
+    FillTile tile = tiles[t];
+    if (!tile.processed) continue;
+    const FillRay r = rays[(int64_t)t * 64 + lane];
+    bool overflow = false;
+    if (tile.visible > 0) {
+      overflow = tile.overflow != 0;
+      if (lane == 0) table[((int64_t)tile.slab * by + t / bx) * bx + t % bx] = overflow ? -1 : tile.slot;
+      // the lane's brick column (x, y) = (lane & 7, lane >> 3) is 8
+      // consecutive records starting at brick_voxel(x, y, 0)
+      const int col = brick_voxel(lane & 7, lane >> 3, 0);
+      if (!overflow && pool8) {
+        // latent i of a sample's output (fp16 value)
+        auto lat = [&](const uint16_t* o16, int i) {
+          return (di && i == 0) ? (float)(_Float16)expf(h2f(o16[sig_row])) : h2f(o16[latent_offset + i]);
+        };
+        int amax_bits = 0;  // |latent| as float bits (non-negative floats order as integers)
+        for (int k = 0; k < r.cnt; k++) {
+          const uint16_t* o16 = out16 + ((int64_t)r.off + k) * 16;
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            const int b = __builtin_bit_cast(int, fabsf(lat(o16, i)));
+            amax_bits = b > amax_bits ? b : amax_bits;
+          }
         }
-      }
-      amax_bits = -wave_min(-amax_bits);
-      const float amax = __builtin_bit_cast(float, amax_bits);
-      const float scale = amax > 0.0f ? amax / 448.0f : 1.0f;
-      if (lane == 0) scales[tile.slot] = scale;
-      Voxel8* brick = pool8 + (int64_t)tile.slot * BRICK_VOXELS + brick_voxel(lane & 7, lane >> 3, 0);
-      int k = 0;
-      for (int z = 0; z < BRICK; z++) {
-        Voxel8 v;
-        const bool has = k < r.cnt && f_meta[r.off + k] == z;
-        if (has) {
-          const int64_t q = (int64_t)r.off + k;
-          const uint16_t* o16 = out16 + q * 16;
-          float qv[8];
-          for (int i = 0; i < 8; i++) qv[i] = fminf(fmaxf(lat(o16, i) / scale, -448.0f), 448.0f);
-          int lo = __builtin_amdgcn_cvt_pk_fp8_f32(qv[0], qv[1], 0, false);
-          lo = __builtin_amdgcn_cvt_pk_fp8_f32(qv[2], qv[3], lo, true);
-          int hi = __builtin_amdgcn_cvt_pk_fp8_f32(qv[4], qv[5], 0, false);
-          hi = __builtin_amdgcn_cvt_pk_fp8_f32(qv[6], qv[7], hi, true);
-          v.lat[0] = (uint32_t)lo;
-          v.lat[1] = (uint32_t)hi;
-          v.density = expf(h2f(o16[sig_row]));
-          v.isset = 1.0f;
-          k++;
-        } else {
-          v.lat[0] = v.lat[1] = 0;
-          v.density = 0.0f;
-          v.isset = 0.0f;
+        amax_bits = -wave_min(-amax_bits);
+        const float amax = __builtin_bit_cast(float, amax_bits);
+        const float scale = amax > 0.0f ? amax / 448.0f : 1.0f;
+        if (lane == 0) scales[tile.slot] = scale;
+        Voxel8* lds8 = reinterpret_cast<Voxel8*>(brick_lds);
+        int k = 0;
+        for (int z = 0; z < BRICK; z++) {
+          Voxel8 v;
+          const bool has = k < r.cnt && f_meta[r.off + k] == z;
+          if (has) {
+            const int64_t q = (int64_t)r.off + k;
+            const uint16_t* o16 = out16 + q * 16;
+            float qv[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) qv[i] = fminf(fmaxf(lat(o16, i) / scale, -448.0f), 448.0f);
+            int lo = __builtin_amdgcn_cvt_pk_fp8_f32(qv[0], qv[1], 0, false);
+            lo = __builtin_amdgcn_cvt_pk_fp8_f32(qv[2], qv[3], lo, true);
+            int hi = __builtin_amdgcn_cvt_pk_fp8_f32(qv[4], qv[5], 0, false);
+            hi = __builtin_amdgcn_cvt_pk_fp8_f32(qv[6], qv[7], hi, true);
+            v.lat[0] = (uint32_t)lo;
+            v.lat[1] = (uint32_t)hi;
+            v.density = expf(h2f(o16[sig_row]));
+            v.isset = 1.0f;
+            k++;
+          } else {
+            v.lat[0] = v.lat[1] = 0;
+            v.density = 0.0f;
+            v.isset = 0.0f;
+          }
+          lds8[col + z] = v;
         }
-        brick[z] = v;
-      }
-      tile.n_bricks++;
-    } else if (!overflow) {
-      // the lane's brick column (x, y) = (lane & 7, lane >> 3): 8 consecutive
-      // records, written from z = 7 down so each record carries the density
-      // and isset of the one above it (znext)
-      Voxel* brick = pool + (int64_t)tile.slot * BRICK_VOXELS + brick_voxel(lane & 7, lane >> 3, 0);
-      int k = r.cnt - 1;
-      float up_d = 0.0f, up_s = 0.0f;
-      for (int z = BRICK - 1; z >= 0; z--) {
-        Voxel v;
-        const bool has = k >= 0 && f_meta[r.off + k] == z;
-        if (has) {
-          const int64_t q = (int64_t)r.off + k;
-          const uint16_t* o16 = out16 + q * 16;
-          for (int i = 0; i < 8; i++) v.lat[i] = o16[latent_offset + i];
-          v.density = expf(h2f(o16[sig_row]));
+        wave_sync();
+        uint4* dst = reinterpret_cast<uint4*>(pool8 + (int64_t)tile.slot * BRICK_VOXELS);
+        for (int i = lane; i < BRICK_VOXELS * (int)sizeof(Voxel8) / 16; i += 64) dst[i] = brick_lds[i];
+        wave_sync();
+        tile.n_bricks++;
+      } else if (!overflow) {
+        // the lane's column in LDS: zeros, then a record at the z of each
+        // sample (the lane's at most 8 samples loaded together), then each
+        // record's znext from the one above it
+        Voxel* lds = reinterpret_cast<Voxel*>(brick_lds);
+        const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int z = 0; z < BRICK; z++) {
+          reinterpret_cast<uint4*>(lds + col + z)[0] = zero;
+          reinterpret_cast<uint4*>(lds + col + z)[1] = zero;
+        }
+        int meta[BRICK];
+        uint4 row[BRICK][2];
+#pragma unroll
+        for (int k = 0; k < BRICK; k++) {
+          // loads from a valid index (0 past the lane's samples), selected after
+          const int64_t q = k < r.cnt ? (int64_t)r.off + k : 0;
+          const int mt = f_meta[q];
+          const uint4 r0 = reinterpret_cast<const uint4*>(out16 + q * 16)[0];
+          const uint4 r1 = reinterpret_cast<const uint4*>(out16 + q * 16)[1];
+          meta[k] = k < r.cnt ? mt : -1;
+          row[k][0] = r0;
+          row[k][1] = r1;
+        }
+#pragma unroll
+        for (int k = 0; k < BRICK; k++) {
+          if (meta[k] < 0) break;
+          // halves 0..8 of the output row (latents start at row 0 or 1, log
+          // sigma is row 0 or 8), taken at fixed positions
+          const uint32_t w[5] = {row[k][0].x, row[k][0].y, row[k][0].z, row[k][0].w, row[k][1].x};
+          uint16_t e[10];
+#pragma unroll
+          for (int m = 0; m < 10; m++) e[m] = (uint16_t)(w[m >> 1] >> (16 * (m & 1)));
+          Voxel v;
+#pragma unroll
+          for (int i = 0; i < 8; i++) v.lat[i] = latent_offset ? e[i + 1] : e[i];
+          v.density = expf(h2f(sig_row ? e[8] : e[0]));
           if (di) v.lat[0] = __builtin_bit_cast(uint16_t, (_Float16)v.density);
           v.isset = 1.0f;
-          k--;
-        } else {
-          for (int i = 0; i < 8; i++) v.lat[i] = 0;
-          v.density = 0.0f;
-          v.isset = 0.0f;
+          v.znext[0] = v.znext[1] = 0.0f;
+          lds[col + meta[k]] = v;
         }
-        v.znext[0] = up_d;
-        v.znext[1] = up_s;
-        up_d = v.density;
-        up_s = v.isset;
-        brick[z] = v;
+#pragma unroll
+        for (int z = 0; z < BRICK - 1; z++) {
+          lds[col + z].znext[0] = lds[col + z + 1].density;
+          lds[col + z].znext[1] = lds[col + z + 1].isset;
+        }
+        wave_sync();
+        uint4* dst = reinterpret_cast<uint4*>(pool + (int64_t)tile.slot * BRICK_VOXELS);
+        for (int i = lane; i < BRICK_VOXELS * (int)sizeof(Voxel) / 16; i += 64) dst[i] = brick_lds[i];
+        wave_sync();
+        tile.n_bricks++;
       }
-      tile.n_bricks++;
     }
-  }
-  const int n_alive = wave_sum(r.alive);
-  if (lane == 0) {
-    if (!overflow) tile.slab++;
-    tile.alive = !overflow && n_alive > 0;
-    tiles[t] = tile;
+    const int n_alive = wave_sum(r.alive);
+    if (lane == 0) {
+      if (!overflow) tile.slab++;
+      tile.alive = !overflow && n_alive > 0;
+      tiles[t] = tile;
+    }
   }
 }
 
@@ -398,9 +526,6 @@ static int default_capacity(int64_t n_rays) {
   return (int)std::min<int64_t>(MAX_BRICKS, (est_bricks + per_layer - 1) / per_layer * per_layer);
 }
 
-// The fill keeps its own counters, pinned mirror and scratch, so it can run
-// on the fill stream from another thread while frames render from the
-// other cache (double-buffered renderers).
 // CachePerformanceRenderer's constructor check (cache_performance_renderer.cu:
 // 964-967): the cache holds 8 latents (2 half4 arrays)
 static void check_cache_latents(const Model& m) {
@@ -409,18 +534,25 @@ static void check_cache_latents(const Model& m) {
                 "Latent width of the model is larger than the (hard-coded) cache size of 8 latents");
 }
 
+// The fill keeps its own counters, pinned mirror and scratch, so it can run
+// on the fill stream from another thread while frames render from the
+// other cache (double-buffered renderers).
+//
+// Sub-iterations are planned on the device: every kernel reads the alive
+// tile count and the sample count that the previous kernels left in the
+// fill counters, and idles when nothing is left, so the host issues bursts
+// of sub-iterations and reads the per-iteration log {alive tiles, samples}
+// once per burst (the reference reads two counts per sub-iteration,
+// cache_renderer.cu:986-991, 1003-1006).  The first burst is as long as the
+// previous fill's loop, plus one; with graph mode it is replayed as a graph.
 void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats, hipStream_t s, Profiler& P) {
   const Model& m = *r.model;
   check_cache_latents(m);
   CacheStore& C = r.caches[r.fill_idx];
   const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
-  r.f_counters.ensure(CTL_LOG * sizeof(int));
-  if (!r.f_pinned) FVC_HIP(hipHostMalloc((void**)&r.f_pinned, 64 * sizeof(int), hipHostMallocDefault));
-  auto read_counter = [&](Renderer&, int idx, hipStream_t st) {
-    FVC_HIP(hipMemcpyAsync(r.f_pinned + idx, r.f_counters.as<int>() + idx, sizeof(int), hipMemcpyDeviceToHost, st));
-    FVC_HIP(hipStreamSynchronize(st));
-    return r.f_pinned[idx];
-  };
+  r.f_counters.ensure(FILL_COUNTER_WORDS * sizeof(int));
+  if (!r.f_pinned)
+    FVC_HIP(hipHostMalloc((void**)&r.f_pinned, FILL_COUNTER_WORDS * sizeof(int), hipHostMallocDefault));
 
   // resize (Cache::resize, cache.cu:44-84), sparse pool sized by the estimate
   C.rm = rm;
@@ -434,7 +566,6 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
   C.pool.ensure(std::max<int64_t>(1, (int64_t)C.capacity) * BRICK_VOXELS *
                 (C.lat8 ? sizeof(Voxel8) : sizeof(Voxel)));
   C.scales.ensure(std::max<int64_t>(1, (int64_t)C.capacity) * sizeof(float));
-  FVC_HIP(hipMemsetAsync(C.table.p, 0xff, n_table * sizeof(int32_t), s));
 
   const int n_tiles = C.bx * C.by;
   const int64_t f_cap = (int64_t)n_tiles * 512;
@@ -450,74 +581,124 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
   // rows of the network output: log sigma and the first stored latent
   const int sig_row = split ? 8 : 0, lat_row = split ? 0 : m.net.latent_offset;
   const int di = m.net.first_latent_is_density ? 1 : 0;
-
-  FVC_HIP(hipMemsetAsync(r.f_counters.p, 0, CTL_LOG * sizeof(int), s));
-  hipEvent_t ev = P.begin(s);
-  hipLaunchKernelGGL(k_fill_init, dim3(n_tiles), dim3(64), 0, s, rm, cvt(m.scene.aabb_from), cvt(m.scene.aabb_to),
-                     C.bx, r.f_rays.as<FillRay>(), r.f_tiles.as<FillTile>());
-  FVC_HIP(hipGetLastError());
-  P.end(ev, s, "fill_init", n_tiles * 64.0, n_tiles * (64.0 * sizeof(FillRay) + sizeof(FillTile)), 0);
   int* cnt = r.f_counters.as<int>();
-  int subiter = 0;
-  int64_t n_eval = 0;
-  for (; subiter < 1000; subiter++) {  // CacheRenderer::MAX_SUBITER
-    ev = P.begin(s);
+  // one block (wave) per tile of the alive list; blocks past the list's
+  // length return at once (grid-stride beyond 65536 tiles)
+  const unsigned grid_t = (unsigned)std::min(n_tiles, 65536);
+
+  // clears are kernels, not memsets: the fill may be recorded into a graph
+  auto setup = [&] {
+    hipLaunchKernelGGL(k_set_ints, dim3((unsigned)((n_table + 255) / 256)), dim3(256), 0, s, C.table.as<int>(),
+                       n_table, -1);
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(FILL_LOG), 0, s, cnt, (int64_t)FILL_LOG, 0);
+    hipLaunchKernelGGL(k_fill_init, dim3(n_tiles), dim3(64), 0, s, rm, cvt(m.scene.aabb_from),
+                       cvt(m.scene.aabb_to), C.bx, r.f_rays.as<FillRay>(), r.f_tiles.as<FillTile>());
+    FVC_HIP(hipGetLastError());
+  };
+  struct Handles { int list, sample, net, accum, alloc, write; };
+  std::vector<Handles> hs;
+  auto iteration = [&](int it) {
+    Handles h;
+    hipEvent_t ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_list, dim3(1), dim3(1024), 0, s, n_tiles, r.f_tiles.as<FillTile>(), r.f_list.as<int>(),
-                       cnt + CNT_TILES);
+                       cnt, it);
     FVC_HIP(hipGetLastError());
-    P.end(ev, s, "fill_list", n_tiles, n_tiles * 36.0, 0);
-    const int n_alive = read_counter(r, CNT_TILES, s);
-    if (n_alive == 0) break;
-    FVC_HIP(hipMemsetAsync(cnt + CNT_SAMPLES, 0, sizeof(int), s));
+    h.list = P.end(ev, s, "fill_list");
     ev = P.begin(s);
-    hipLaunchKernelGGL(k_fill_sample, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), rm.stepsize_info, m.occ,
+    hipLaunchKernelGGL(k_fill_sample, dim3(grid_t), dim3(64), 0, s, r.f_list.as<int>(), cnt, rm.stepsize_info, m.occ,
                        m.con, m.bitfield.as<uint64_t>(), m.coarse.as<uint32_t>(), C.bz, r.f_rays.as<FillRay>(),
-                       r.f_tiles.as<FillTile>(),
-                       r.f_pos.as<float>(), r.f_t0.as<float>(), r.f_meta.as<int>(), cnt + CNT_SAMPLES,
-                       split ? r.f_dir.as<float>() : nullptr);
+                       r.f_tiles.as<FillTile>(), r.f_pos.as<float>(), r.f_t0.as<float>(), r.f_meta.as<int>(),
+                       cnt + CNT_SAMPLES, split ? r.f_dir.as<float>() : nullptr);
     FVC_HIP(hipGetLastError());
-    const int hs = P.end(ev, s, "fill_sample");
-    const int n_samples = read_counter(r, CNT_SAMPLES, s);
-    P.work(hs, n_samples, n_alive * 64.0 * (2 * sizeof(FillRay) + 8) + 20.0 * n_samples, 0);
-    if (n_samples <= 0) break;
-    n_eval += n_samples;
+    h.sample = P.end(ev, s, "fill_sample");
     ev = P.begin(s);
     if (split)
-      launch_network_latents_vd(m, r.f_pos.as<float>(), r.f_dir.as<float>(), n_samples, r.f_out16.as<uint16_t>(), s,
-                                r.mlp_rows);
+      launch_network_latents_vd(m, r.f_pos.as<float>(), r.f_dir.as<float>(), f_cap, r.f_out16.as<uint16_t>(), s,
+                                r.mlp_rows, cnt + CNT_SAMPLES);
     else
-      launch_network_latents(m, r.f_pos.as<float>(), n_samples, r.f_out16.as<uint16_t>(), s, r.mlp_rows);
-    P.end(ev, s, "network_latents", n_samples, n_samples * (ALG_GATHER_B + (split ? 24.0 : 12.0) + 32.0),
-          n_samples * (ALG_DENSITY_FLOP + (split ? ALG_FIRST_HEAD_FLOP : 0.0)));
+      launch_network_latents(m, r.f_pos.as<float>(), f_cap, r.f_out16.as<uint16_t>(), s, r.mlp_rows,
+                             cnt + CNT_SAMPLES);
+    h.net = P.end(ev, s, "network_latents");
     ev = P.begin(s);
-    hipLaunchKernelGGL(k_fill_accum, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), rm.stepsize_info,
+    hipLaunchKernelGGL(k_fill_accum, dim3(grid_t), dim3(64), 0, s, r.f_list.as<int>(), cnt, it, rm.stepsize_info,
                        m.scene.alpha_thre, r.f_out16.as<uint2>(), r.f_t0.as<float>(), r.f_rays.as<FillRay>(),
                        r.f_tiles.as<FillTile>(), sig_row);
     FVC_HIP(hipGetLastError());
-    P.end(ev, s, "fill_accum", n_samples, n_samples * 12.0 + n_alive * 64.0 * 56, 0);
+    h.accum = P.end(ev, s, "fill_accum");
     ev = P.begin(s);
-    hipLaunchKernelGGL(k_fill_alloc, dim3(1), dim3(1024), 0, s, r.f_list.as<int>(), cnt + CNT_TILES, C.capacity,
-                       r.f_tiles.as<FillTile>(), cnt + CNT_SLOTS, cnt + CNT_OVERFLOW);
+    hipLaunchKernelGGL(k_fill_alloc, dim3(1), dim3(1024), 0, s, r.f_list.as<int>(), cnt, C.capacity,
+                       r.f_tiles.as<FillTile>());
     FVC_HIP(hipGetLastError());
-    P.end(ev, s, "fill_alloc", n_alive, n_alive * 40.0, 0);
+    h.alloc = P.end(ev, s, "fill_alloc");
     ev = P.begin(s);
-    hipLaunchKernelGGL(k_fill_write, dim3(n_alive), dim3(64), 0, s, r.f_list.as<int>(), C.bx, C.by,
-                       lat_row, r.f_out16.as<uint16_t>(), r.f_meta.as<int>(), r.f_rays.as<FillRay>(),
+    hipLaunchKernelGGL(k_fill_write, dim3(grid_t), dim3(64), 0, s, r.f_list.as<int>(), cnt, C.bx, C.by, lat_row,
+                       r.f_out16.as<uint16_t>(), r.f_meta.as<int>(), r.f_rays.as<FillRay>(),
                        r.f_tiles.as<FillTile>(), C.table.as<int>(), C.lat8 ? nullptr : C.pool.as<Voxel>(),
                        C.lat8 ? C.pool.as<Voxel8>() : nullptr, C.scales.as<float>(), sig_row, di);
     FVC_HIP(hipGetLastError());
-    P.end(ev, s, "fill_write", n_samples, n_samples * 36.0 + n_alive * 64.0 * 48);
+    h.write = P.end(ev, s, "fill_write");
+    hs.push_back(h);
+  };
+  // the log of sub-iterations [a, b): true when the loop has ended
+  auto read_log = [&](int a, int b) {
+    FVC_HIP(hipMemcpyAsync(r.f_pinned + FILL_LOG + 2 * a, cnt + FILL_LOG + 2 * a, 2 * (b - a) * sizeof(int),
+                           hipMemcpyDeviceToHost, s));
+    FVC_HIP(hipStreamSynchronize(s));
+    for (int i = a; i < b; i++)
+      if (r.f_pinned[FILL_LOG + 2 * i] == 0 || r.f_pinned[FILL_LOG + 2 * i + 1] == 0) return true;
+    return false;
+  };
+  int it = 0;
+  bool done = false;
+  const int first = std::min(FILL_MAX_ITERS, std::max(2, r.last_fill_iters + 1));
+  const bool graph = r.use_graph && r.last_fill_iters > 0 && !P.enabled;
+  if (graph) {
+    graph_run(r, 2, s, [&] {
+      setup();
+      for (int i = 0; i < first; i++) iteration(i);
+    });
+  } else {
+    setup();
+    for (; it < first; it++) iteration(it);
   }
-  ev = P.begin(s);
+  it = first;
+  done = read_log(0, it);
+  while (!done && it < FILL_MAX_ITERS) {
+    const int a = it;
+    for (int b = 0; b < 4 && it < FILL_MAX_ITERS; b++, it++) iteration(it);
+    done = read_log(a, it);
+  }
+  hipEvent_t ev = P.begin(s);
   hipLaunchKernelGGL(k_fill_known, dim3(n_tiles), dim3(64), 0, s, W, H, C.bx, rm.stepsize_info,
                      r.f_tiles.as<FillTile>(), C.known.as<float>());
   FVC_HIP(hipGetLastError());
   P.end(ev, s, "fill_known", (double)W * H, (double)W * H * 4 + n_tiles * 32.0, 0);
-  const int used = read_counter(r, CNT_SLOTS, s);
-  const int overflow = read_counter(r, CNT_OVERFLOW, s);
+  FVC_HIP(hipMemcpyAsync(r.f_pinned, cnt, FILL_LOG * sizeof(int), hipMemcpyDeviceToHost, s));
+  FVC_HIP(hipStreamSynchronize(s));
+  const int used = r.f_pinned[CNT_SLOTS];
+  const int overflow = r.f_pinned[CNT_OVERFLOW];
+  // sub-iterations with samples (the reference's loop ends at the first
+  // round without alive tiles or samples, cache_renderer.cu:1017-1018)
+  int subiter = 0;
+  int64_t n_eval = 0;
+  while (subiter < it && r.f_pinned[FILL_LOG + 2 * subiter] > 0 && r.f_pinned[FILL_LOG + 2 * subiter + 1] > 0) {
+    n_eval += r.f_pinned[FILL_LOG + 2 * subiter + 1];
+    subiter++;
+  }
+  for (size_t i = 0; i < hs.size(); i++) {
+    const double nt = i < (size_t)subiter ? r.f_pinned[FILL_LOG + 2 * i] : 0.0;
+    const double ns = i < (size_t)subiter ? r.f_pinned[FILL_LOG + 2 * i + 1] : 0.0;
+    P.work(hs[i].list, n_tiles, n_tiles * 36.0, 0);
+    P.work(hs[i].sample, ns, nt * 64.0 * (2 * sizeof(FillRay) + 8) + 20.0 * ns, 0);
+    P.work(hs[i].net, ns, ns * (ALG_GATHER_B + (split ? 24.0 : 12.0) + 32.0),
+           ns * (ALG_DENSITY_FLOP + (split ? ALG_FIRST_HEAD_FLOP : 0.0)));
+    P.work(hs[i].accum, ns, ns * 12.0 + nt * 64.0 * 56, 0);
+    P.work(hs[i].alloc, nt, nt * 40.0, 0);
+    P.work(hs[i].write, ns, ns * 36.0 + nt * 64.0 * 48, 0);
+  }
+  r.last_fill_iters = subiter;
   C.n_used = std::min(used, C.capacity);
   C.initialized = true;
-  FVC_HIP(hipStreamSynchronize(s));
   P.collect();
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));

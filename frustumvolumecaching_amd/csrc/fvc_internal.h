@@ -143,7 +143,7 @@ void nerfacc_read_data(const std::string& dir, const NerfaccExport& ex, uint16_t
 void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s, const int* n_dev = nullptr, int rows = 16);
 void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s,
-                            int rows = 16);
+                            int rows = 16, const int* n_dev = nullptr);
 void launch_network_head(const Model& m, const uint16_t* head_in16, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s);
 void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t* out32, hipStream_t s);
@@ -153,7 +153,7 @@ void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t
 void launch_network_full_vd(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
                             hipStream_t s, const int* n_dev, int rows);
 void launch_network_latents_vd(const Model& m, const float* pos, const float* init_dir, int64_t n, uint16_t* out16,
-                               hipStream_t s, int rows);
+                               hipStream_t s, int rows, const int* n_dev = nullptr);
 void launch_network_head_vd(const Model& m, const uint16_t* head_in16, const float* pos, const float* dir,
                             const float* init_dir, int64_t n, uint32_t* out_h4, hipStream_t s);
 // weight fragments of a split model (FP16)
@@ -370,6 +370,7 @@ struct Renderer {
   int mlp_rows = 16;     // MLP rows (samples) per wavefront and iteration: 16, 32 or 64
   int latent_format = FVC_LATENT_FP16;  // latent storage of the next fills
   int last_iters = 0;    // sub-iterations of the previous frame (first burst length)
+  int last_fill_iters = 0;  // ... of the previous fill
   // step table of the marchers (render.hip: step_table)
   DevBuf tab_t, tab_off;
   fvc_stepsize_info tab_key{};
@@ -408,9 +409,10 @@ struct Renderer {
   fvc_refill_policy policy{0.9f, 1.0e30f, 0, 0};
   int fill_tiles = 0;
   int64_t fill_sample_capacity = 0;
-  // graph mode: executable graphs of a full (0) and a cached (1) frame
+  // graph mode: executable graphs of a full (0) and a cached (1) frame and
+  // of a fill (2)
   bool use_graph = false;
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
   int graph_iters[2] = {0, 0};  // sub-iterations of the last frame of each kind
   ~Renderer() {
     if (fill_thread.joinable()) fill_thread.join();

@@ -444,13 +444,14 @@ void launch_network_full(const Model& m, const float* pos, const float* dir, int
 #undef L2
   FVC_HIP(hipGetLastError());
 }
-void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s, int rows) {
+void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s, int rows,
+                            const int* n_dev) {
   if (m.net.split_head) throw Error(FVC_E_UNSUPPORTED, "split models need the initial view direction");
   if (n <= 0) return;
 #define L2(P, TL)                                                                                                 \
   hipLaunchKernelGGL((k_network<false, P, TL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), 0, s, \
                      pos, (const float*)nullptr, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, \
-                     m.mlp_scales, (uint2*)nullptr, (uint2*)out16, (const int*)nullptr)
+                     m.mlp_scales, (uint2*)nullptr, (uint2*)out16, n_dev)
 #define L(P)                       \
   if (rows == 64) { L2(P, 4); }    \
   else if (rows == 32) { L2(P, 2); } \
@@ -488,13 +489,13 @@ void launch_network_full_vd(const Model& m, const float* pos, const float* dir, 
   FVC_HIP(hipGetLastError());
 }
 void launch_network_latents_vd(const Model& m, const float* pos, const float* init_dir, int64_t n, uint16_t* out16,
-                               hipStream_t s, int rows) {
+                               hipStream_t s, int rows, const int* n_dev) {
   if (n <= 0) return;
   const int di = m.net.first_latent_is_density, fq = m.net.freq_encoding_degree > 0;
 #define L(TL, FL)                                                                                               \
   hipLaunchKernelGGL((k_network_vd<false, TL, FL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), \
                      0, s, pos, init_dir, n, m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, di, fq,            \
-                     (uint2*)out16, (const int*)nullptr)
+                     (uint2*)out16, n_dev)
   if (m.net.first_head_n_hidden_layers == 2) {
     if (rows == 64) { L(4, 2); } else if (rows == 32) { L(2, 2); } else { L(1, 2); }
   } else {
