@@ -1359,6 +1359,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   check_cache_latents(m);
   const CacheStore& C = r.caches[r.render_idx];
   if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
+  if (vs.scenes.size() > 1) throw Error(FVC_E_UNSUPPORTED, "a cached batch renders the scene of the renderer's cache");
   prepare_views(r, vs);
   const fvc_raymarch_info& rm = vs.rms[0];
   const double n_px = (double)vs.n_px;
@@ -1499,7 +1500,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
 }
 
 void render_cached(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
-  render_cached(r, Views(&rm, 1, &rgba), stats);
+  render_cached(r, Views(&rm, 1, &rgba, r.model), stats);
 }
 
 void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats) {

@@ -86,6 +86,25 @@ int fvc_net_param_counts_viewdep(const fvc_net_desc* net, int64_t* a, int64_t* b
 
 // NerfNetwork::init + set_params (nerf_network.h:88-118, 269-308) and the
 // occupancy bitfield; p_first (split models only) is mlp_first_head
+// 64-bit hash of a byte range (8-byte words mixed with the splitmix64
+// finaliser, then the tail bytes), chained through h
+static uint64_t hash_bytes(const void* p, size_t n, uint64_t h) {
+  auto mix = [](uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  };
+  const uint8_t* b = static_cast<const uint8_t*>(p);
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, b + i, 8);
+    h = mix(h ^ w) + 0x9e3779b97f4a7c15ull;
+  }
+  for (; i < n; i++) h = mix(h ^ b[i]);
+  return mix(h ^ n);
+}
+
 static void create_model(const fvc_scene_info* scene, const fvc_net_desc* net, const uint8_t* occ,
                          const uint16_t* p_pos, int64_t n_pos, const uint16_t* p_base, int64_t n_base,
                          const uint16_t* p_first, int64_t n_first, const uint16_t* p_head, int64_t n_head,
@@ -136,6 +155,10 @@ static void create_model(const fvc_scene_info* scene, const fvc_net_desc* net, c
       else pack_fragments(p_base, p_head, net->precision, net->latent_offset, net->n_latents, frags, m.mlp_scales);
       m.frags.ensure(frags.size());
       FVC_HIP(hipMemcpy(m.frags.p, frags.data(), frags.size(), hipMemcpyHostToDevice));
+      uint64_t hsh = hash_bytes(net, sizeof(*net), 0x9e3779b97f4a7c15ull);
+      hsh = hash_bytes(p_pos, n_pos * sizeof(uint16_t), hsh);
+      hsh = hash_bytes(frags.data(), frags.size(), hsh);
+      m.net_hash = hsh;
       m.occ = occ_params(*scene);
       const V3 from = cvt(scene->aabb_from), to = cvt(scene->aabb_to);
       // apply_contraction constants (raymarch_common.h:610-653)
@@ -410,7 +433,19 @@ int fvc_render_full_batch(fvc_renderer* h, const fvc_raymarch_info* rms, int32_t
     REQUIRE(h && rms && n_views > 0, "invalid argument");
     for (int v = 0; v < n_views; v++) check_rm(&rms[v]);
     std::scoped_lock lk(h->r.read_mutex);
-    render_full(h->r, Views(rms, n_views, rgba), stats);
+    render_full(h->r, Views(rms, n_views, rgba, h->r.model), stats);
+  });
+}
+
+int fvc_render_full_batch_scenes(fvc_renderer* h, fvc_model* const* models, const fvc_raymarch_info* rms,
+                                 int32_t n_views, uint8_t* const* rgba, fvc_render_stats* stats) {
+  return guard([&] {
+    REQUIRE(h && models && rms && n_views > 0, "invalid argument");
+    for (int v = 0; v < n_views; v++) check_rm(&rms[v]);
+    std::vector<const Model*> ms((size_t)n_views);
+    for (int v = 0; v < n_views; v++) ms[v] = models[v] ? &models[v]->m : nullptr;
+    std::scoped_lock lk(h->r.read_mutex);
+    render_full(h->r, Views(rms, n_views, rgba, h->r.model, ms.data()), stats);
   });
 }
 
@@ -420,7 +455,7 @@ int fvc_render_cached_batch(fvc_renderer* h, const fvc_raymarch_info* rms, int32
     REQUIRE(h && rms && n_views > 0, "invalid argument");
     for (int v = 0; v < n_views; v++) check_rm(&rms[v]);
     std::scoped_lock lk(h->r.read_mutex);
-    render_cached(h->r, Views(rms, n_views, rgba), stats);
+    render_cached(h->r, Views(rms, n_views, rgba, h->r.model), stats);
   });
 }
 

@@ -115,6 +115,17 @@ struct Model {
   DevBuf frags;      // packed weight fragments in the MLP precision
   MlpScales mlp_scales{};
   std::vector<uint8_t> bitfield_host;
+  // hash of the network description and parameters: models with equal
+  // hashes share the network (views of a mixed-scene batch must)
+  uint64_t net_hash = 0;
+};
+
+// The occupancy of one scene of a mixed-scene batch, as the marcher reads it
+// (Morton bitfield, coarse empty-space table); the scenes of one batch share
+// the scene box, so their OccParams are equal.
+struct SceneRef {
+  const uint64_t* bf64;
+  const uint32_t* coarse;
 };
 
 // input width of a split model's custom colour head: 8 latents + 16 SH +
@@ -300,6 +311,7 @@ inline double net_flop(const fvc_net_desc& d) {
 
 // ---------------------------------------------------------------- renderer
 static const int MAX_STEPS_PER_LAUNCH = 64;   // cap of fvc_renderer_set_batching (64-bit sample masks)
+static const int MAX_SCENES = 64;             // distinct models in one mixed-scene batch
 
 // the render cache as the kernels see it
 struct CacheView {
@@ -393,6 +405,9 @@ struct Renderer {
   DevBuf c_rec, c_t0, c_out;  // cache-hit stream: c_rec = (froxel x, y, z, pixel index bits) per sample
   DevBuf ray_od;              // split models: ray origin, direction per pixel (2 float4)
   DevBuf ray_sh;                                // 16 fp16 SH coefficients per pixel (first launch of its ray)
+  // mixed-scene batches: occupancy of each scene (SceneRef per scene)
+  DevBuf scene_refs;
+  std::vector<SceneRef> scene_refs_host;
   // fill scratch (own counters and pinned mirror: the fill may run on the
   // fill stream from another thread)
   DevBuf f_rays, f_tiles, f_list, f_pos, f_t0, f_meta, f_out16, f_counters;
@@ -465,7 +480,15 @@ struct Views {
   std::vector<int64_t> base;
   int64_t n_px;
   int64_t max_px;  // pixels of the largest view
-  Views(const fvc_raymarch_info* rms, int n, uint8_t* const* rgba);
+  // scenes (mixed-scene batches): the distinct models of the views, the
+  // renderer's own model first when no per-view models are given; scene[v]
+  // indexes them
+  std::vector<const Model*> scenes;
+  std::vector<int> scene;
+  // models (optional): the model of each view (null: the renderer's); they
+  // must share the renderer's network and scene box (checked)
+  Views(const fvc_raymarch_info* rms, int n, uint8_t* const* rgba, const Model* own,
+        const Model* const* models = nullptr);
   // samples per launch for a target set per view: a batch scales it by its
   // pixels over its largest view's, so every view keeps the per-ray budget
   // of a single render (at most 2^28)

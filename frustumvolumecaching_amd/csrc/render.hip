@@ -23,7 +23,8 @@ namespace fvc {
 // (0,0,0, T=1).
 // Pixels of a view are numbered pix_base + y * W + x in the packed pixel
 // space of a batch of views.
-__global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, int pix_base, V3 aabb_from, V3 aabb_to,
+__global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, int pix_base, int scene, V3 aabb_from,
+                                                   V3 aabb_to,
                                                    RayDyn* __restrict__ rays, RayState* __restrict__ rconst,
                                                    int* __restrict__ pix, float4* __restrict__ rres,
                                                    float4* __restrict__ results, int* __restrict__ counter) {
@@ -36,6 +37,7 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, int pix
   int slot[1];
   block_reserve<1>({alive ? 1 : 0}, {counter}, slot);
   if (alive) {
+    st.scene = scene;
     rconst[pix_base + y * W + x] = st;
     rays[slot[0]] = RayDyn{st.t, st.j};
     pix[slot[0]] = pix_base + y * W + x;
@@ -50,8 +52,13 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, int pix
 // (coalesced across the wave when read back for the stream).
 static constexpr int EMIT = 16;  // samples whose t0 the full marcher's emission loads together
 
+// MULTI (mixed-scene batches): each ray reads the occupancy of its view's
+// scene (scenes[scene]) with the coarse table from global memory (L2
+// resident) instead of the block's LDS copy.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, MarchGrid mg, ContractParams con,
-                                               const uint32_t* __restrict__ coarse_g, RayDyn* __restrict__ rays,
+                                               const uint32_t* __restrict__ coarse_g,
+                                               const SceneRef* __restrict__ scenes, RayDyn* __restrict__ rays,
                                                const RayState* __restrict__ rconst, const int* __restrict__ pix_in,
                                                int4* __restrict__ entry, int* __restrict__ alive_flag,
                                                float* __restrict__ t_scr, float* __restrict__ s_pos,
@@ -62,7 +69,7 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
   int* s_counter = ctl + CTL_R;
   int* step_counter = ctl + CTL_BINS;
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
-  load_coarse(coarse_g, mg.occ.nlvl, coarse);
+  if constexpr (!MULTI) load_coarse(coarse_g, mg.occ.nlvl, coarse);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_alive;
   int cnt = 0, bins = 0;
@@ -74,10 +81,19 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
     rs.t = dy.t;
     rs.j = dy.j;
     alive = true;
-    MarchRay r = march_ray(rs, mg.tab);
+    MarchGrid g = mg;
+    const uint32_t* cz;
+    if constexpr (MULTI) {
+      const SceneRef sr = scenes[rs.scene];
+      g.bf64 = sr.bf64;
+      cz = sr.coarse;
+    } else {
+      cz = coarse;
+    }
+    MarchRay r = march_ray(rs, g.tab);
     while (cnt < n_steps) {
       float t0;
-      const int res = march_trip(mg, coarse, r, t0, bins);
+      const int res = march_trip(g, cz, r, t0, bins);
       if (res == TRIP_END) { alive = false; break; }
       if (res == TRIP_SAMPLE) {
         t_scr[(int64_t)cnt * n_alive + i] = t0;
@@ -336,9 +352,26 @@ void graph_run(Renderer& r, int which, hipStream_t s, const std::function<void()
   FVC_HIP(hipGraphLaunch(r.graph_exec[which], s));
 }
 
-Views::Views(const fvc_raymarch_info* rms_, int n_, uint8_t* const* rgba_) : rms(rms_), n(n_), rgba(rgba_) {
+Views::Views(const fvc_raymarch_info* rms_, int n_, uint8_t* const* rgba_, const Model* own,
+             const Model* const* models)
+    : rms(rms_), n(n_), rgba(rgba_) {
   if (n <= 0) throw Error(FVC_E_INVALID, "no views");
   base.resize(n);
+  scenes.assign(1, own);
+  scene.assign(n, 0);
+  for (int v = 0; models && v < n; v++) {
+    const Model* mv = models[v] ? models[v] : own;
+    if (mv == own) continue;
+    if (mv->device != own->device) throw Error(FVC_E_INVALID, "the models of a batch must live on one device");
+    if (mv->net_hash != own->net_hash || std::memcmp(&mv->net, &own->net, sizeof(fvc_net_desc)) != 0)
+      throw Error(FVC_E_UNSUPPORTED, "the models of a batch must share the renderer's network");
+    if (std::memcmp(&mv->scene, &own->scene, sizeof(fvc_scene_info)) != 0)
+      throw Error(FVC_E_UNSUPPORTED, "the models of a batch must share the scene box and cascades");
+    const auto it = std::find(scenes.begin(), scenes.end(), mv);
+    scene[v] = (int)(it - scenes.begin());
+    if (it == scenes.end()) scenes.push_back(mv);
+  }
+  if ((int)scenes.size() > MAX_SCENES) throw Error(FVC_E_UNSUPPORTED, "too many scenes in one batch");
   n_px = 0;
   max_px = 0;
   for (int v = 0; v < n; v++) {
@@ -366,7 +399,7 @@ void launch_init_rays_async(Renderer& r, const Views& vs, hipStream_t s) {
     const fvc_raymarch_info& rm = vs.rms[v];
     const int W = rm.cam_info.res_x, H = rm.cam_info.res_y;
     hipLaunchKernelGGL(k_init_rays, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, (int)vs.base[v],
-                       cvt(m.scene.aabb_from), cvt(m.scene.aabb_to), r.rays[0].as<RayDyn>(),
+                       vs.scene[v], cvt(m.scene.aabb_from), cvt(m.scene.aabb_to), r.rays[0].as<RayDyn>(),
                        r.ray_const.as<RayState>(), r.ray_pix[0].as<int>(), r.ray_res[0].as<float4>(),
                        r.results.as<float4>(), r.counters.as<int>() + CTL_ALIVE);
     FVC_HIP(hipGetLastError());
@@ -446,6 +479,19 @@ MarchGrid march_grid(Renderer& r, const fvc_raymarch_info* rms, int n_views) {
   return g;
 }
 
+// the occupancy table of a mixed-scene batch's scenes (uploaded when it changes)
+void upload_scene_refs(Renderer& r, const Views& vs) {
+  std::vector<SceneRef> refs;
+  for (const Model* m : vs.scenes) refs.push_back(SceneRef{m->bitfield.as<uint64_t>(), m->coarse.as<uint32_t>()});
+  if (refs.size() == r.scene_refs_host.size() &&
+      std::memcmp(refs.data(), r.scene_refs_host.data(), refs.size() * sizeof(SceneRef)) == 0)
+    return;
+  r.scene_refs.ensure(MAX_SCENES * sizeof(SceneRef));
+  FVC_HIP(hipStreamSynchronize(r.stream()));
+  FVC_HIP(hipMemcpy(r.scene_refs.p, refs.data(), refs.size() * sizeof(SceneRef), hipMemcpyHostToDevice));
+  r.scene_refs_host = refs;
+}
+
 int steps_for(int n_alive, int64_t target, int max_steps) {
   int64_t k = target / std::max(n_alive, 1);
   if (k < 1) k = 1;
@@ -465,6 +511,8 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
   const double n_px = (double)vs.n_px;
   hipStream_t s = r.stream();
   const MarchGrid mg = march_grid(r, vs.rms, vs.n);
+  const bool multi = vs.scenes.size() > 1;
+  if (multi) upload_scene_refs(r, vs);
   Profiler& P = r.prof;
   // graph mode: the frame's launches up to the previous frame's sub-iteration
   // count + 1 are recorded and replayed as one graph; their grids are sized
@@ -485,8 +533,8 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, target, r.max_steps, cap, 0);
       Handles h;
       ev = P.begin(s);
-      hipLaunchKernelGGL(k_march, grid, dim3(256), 0, s, ctl, cur, mg, m.con, m.coarse.as<uint32_t>(),
-                         r.rays[cur].as<RayDyn>(), r.ray_const.as<RayState>(), r.ray_pix[cur].as<int>(),
+      hipLaunchKernelGGL(multi ? k_march<true> : k_march<false>, grid, dim3(256), 0, s, ctl, cur, mg, m.con,
+                         m.coarse.as<uint32_t>(), r.scene_refs.as<SceneRef>(), r.rays[cur].as<RayDyn>(), r.ray_const.as<RayState>(), r.ray_pix[cur].as<int>(),
                          r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.t_scratch.as<float>(), r.s_pos.as<float>(),
                          r.s_dir.as<float>(), r.s_t0.as<float>());
       FVC_HIP(hipGetLastError());
@@ -551,7 +599,7 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
 }
 
 void render_full(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba, fvc_render_stats* stats) {
-  render_full(r, Views(&rm, 1, &rgba), stats);
+  render_full(r, Views(&rm, 1, &rgba, r.model), stats);
 }
 
 // ---------------------------------------------------------------- no-allocation render
@@ -643,8 +691,8 @@ void render_full_noalloc(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba
   mg.bf64 = m.bitfield.as<uint64_t>();
 
   hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(CTL_LOG), 0, s, ctl, CTL_LOG);
-  hipLaunchKernelGGL(k_init_rays, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, 0, cvt(m.scene.aabb_from),
-                     cvt(m.scene.aabb_to), rays[0], rconst, pix[0], rres[0], results, ctl + CTL_ALIVE);
+  hipLaunchKernelGGL(k_init_rays, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, rm, 0, 0,
+                     cvt(m.scene.aabb_from), cvt(m.scene.aabb_to), rays[0], rconst, pix[0], rres[0], results, ctl + CTL_ALIVE);
   FVC_HIP(hipGetLastError());
   const int64_t cap = S - 64;
   const int64_t n_bound = std::min<int64_t>(cap, P * NOALLOC_STEPS);
@@ -652,7 +700,8 @@ void render_full_noalloc(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba
   auto iteration = [&](int it) {
     const int cur = it & 1;
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, cap, NOALLOC_STEPS, cap, 0);
-    hipLaunchKernelGGL(k_march, grid, dim3(256), 0, s, ctl, cur, mg, m.con, m.coarse.as<uint32_t>(), rays[cur], rconst,
+    hipLaunchKernelGGL(k_march<false>, grid, dim3(256), 0, s, ctl, cur, mg, m.con, m.coarse.as<uint32_t>(), nullptr,
+                       rays[cur], rconst,
                        pix[cur], entry, alive, t_scr, s_pos, s_dir, s_t0);
     FVC_HIP(hipGetLastError());
     launch_network_full(m, s_pos, s_dir, n_bound, reinterpret_cast<uint32_t*>(s_out), s, ctl + CTL_R, r.mlp_rows);

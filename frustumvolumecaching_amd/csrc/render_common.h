@@ -13,7 +13,9 @@ namespace fvc {
 // bins advanced j, so t == T[m][j]).
 struct RayState {
   float ox, oy, oz, dx, dy, dz, far_, t;
-  int m, j, pad0, pad1;
+  int m, j;
+  int scene;  // scene of the ray's view (mixed-scene batches; else 0)
+  int pad;
 };
 // Where a ray stands between sub-iterations: next bin start t == T[m][j].
 // The alive list compacts this and the pixel index; the rest of the ray
@@ -40,7 +42,7 @@ __device__ __forceinline__ bool init_ray_state(const fvc_raymarch_info& rm, V3 a
   st.t = hit ? t_from_step(s, mf + s.near_i) : s.near_;
   st.m = (int)mf;
   st.j = 0;
-  st.pad0 = st.pad1 = 0;
+  st.scene = st.pad = 0;
   return hit;
 }
 
@@ -152,6 +154,7 @@ FVC_HD int to_8bit(float f) {
 // host helpers (render.hip)
 StepTableView step_table(Renderer& r, const fvc_raymarch_info* rms, int n_views);
 MarchGrid march_grid(Renderer& r, const fvc_raymarch_info* rms, int n_views);
+void upload_scene_refs(Renderer& r, const Views& vs);
 int read_counter(Renderer& r, int idx, hipStream_t s);
 void zero_counters(Renderer& r, hipStream_t s);
 int launch_init_rays(Renderer& r, const Views& vs, hipStream_t s);

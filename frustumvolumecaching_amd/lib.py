@@ -74,6 +74,8 @@ _SIGS = {
                                         C.POINTER(abi.RenderStats)]),
     "fvc_render_cached_batch": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int32, C.POINTER(_P),
                                           C.POINTER(abi.RenderStats)]),
+    "fvc_render_full_batch_scenes": (C.c_int, [_P, C.POINTER(_P), C.POINTER(abi.RaymarchInfo), C.c_int32,
+                                               C.POINTER(_P), C.POINTER(abi.RenderStats)]),
     "fvc_cache_set_settings": (C.c_int, [_P, C.POINTER(abi.CacheSettings)]),
     "fvc_cache_set_capacity": (C.c_int, [_P, C.c_int32]),
     "fvc_cache_set_latent_format": (C.c_int, [_P, C.c_int]),
@@ -357,9 +359,19 @@ class Renderer:
         self.last_render_stats = st
         return st
 
-    def render_full_batch(self, rms, rgbas=None):
-        """Render several views (any resolutions, shared stepsize) in one pass."""
-        return self._batch(load().fvc_render_full_batch, rms, rgbas)
+    def render_full_batch(self, rms, rgbas=None, models=None):
+        """Render several views (any resolutions, shared stepsize) in one pass.
+        models (optional): the NerfNetwork of each view (None: the renderer's),
+        sharing the renderer's network and scene box; their occupancy grids
+        may differ."""
+        if models is None:
+            return self._batch(load().fvc_render_full_batch, rms, rgbas)
+        n = len(rms)
+        if len(models) != n:
+            raise ValueError("one model per view")
+        mp = (_P * n)(*[None if m is None else m.h for m in models])
+        return self._batch(lambda h, arr, k, ptrs, st: load().fvc_render_full_batch_scenes(h, mp, arr, k, ptrs, st),
+                           rms, rgbas)
 
     def render_cached_batch(self, rms, rgbas=None):
         return self._batch(load().fvc_render_cached_batch, rms, rgbas)

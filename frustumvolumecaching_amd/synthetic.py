@@ -410,6 +410,16 @@ def camera_path_1080p(wl, n_frames=30, yaw_deg=1.0, forward=0.02):
     return mats
 
 
+BATCH_RESOLUTIONS = ((320, 180), (640, 360), (1280, 720), (1920, 1080))
+BATCH_BLOB_COUNTS = (2, 4, 8, 16, 32, 64)
+
+
+def batch_views(wl, path, n_views=16, resolutions=BATCH_RESOLUTIONS):
+    """Multi-view batch: view k at pose 1 + k % 8 of the camera path with
+    resolution resolutions[k % len(resolutions)]."""
+    return [wl.rm(path[1 + k % 8], *resolutions[k % len(resolutions)]) for k in range(n_views)]
+
+
 def _default_setup(step, scene):
     """StepsizeInfo::setAdditionalInfo through the product library (host code)."""
     from . import lib

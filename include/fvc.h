@@ -357,6 +357,14 @@ int fvc_render_full_batch(fvc_renderer* renderer, const fvc_raymarch_info* rms, 
                           fvc_render_stats* stats);
 int fvc_render_cached_batch(fvc_renderer* renderer, const fvc_raymarch_info* rms, int32_t n_views,
                             uint8_t* const* rgba, fvc_render_stats* stats);
+/* Mixed-scene full-render batch (no counterpart in the reference): view v
+ * renders the scene of models[v] (null: the renderer's model).  The models
+ * must share the renderer's network (description and parameters equal) and
+ * scene info (box, cascades, contraction); their occupancy grids may differ,
+ * so rays of one packed stream march different grids.  FVC_E_UNSUPPORTED
+ * otherwise, or for more than 64 distinct models. */
+int fvc_render_full_batch_scenes(fvc_renderer* renderer, fvc_model* const* models, const fvc_raymarch_info* rms,
+                                 int32_t n_views, uint8_t* const* rgba, fvc_render_stats* stats);
 /* ---- background refill (double-buffered renderers) ----
  * The viewer's cache update (application.cu:168-216) as library calls: the
  * fill of the back cache runs on its own thread and stream while frames
