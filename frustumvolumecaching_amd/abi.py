@@ -123,7 +123,7 @@ class RefillPolicy(C.Structure):
                 ("pad_", C.c_int32)]
 
 
-TUNE_MAX = 32
+TUNE_MAX = 64
 
 
 class TuneResult(C.Structure):
@@ -132,6 +132,24 @@ class TuneResult(C.Structure):
                 ("ms_best", C.c_float), ("ms_default", C.c_float), ("n_tried", C.c_int32), ("pad_", C.c_int32),
                 ("ms", C.c_float * TUNE_MAX), ("rows", C.c_int32 * TUNE_MAX), ("steps", C.c_int32 * TUNE_MAX),
                 ("targets", C.c_int64 * TUNE_MAX)]
+
+
+class TuneSpace(C.Structure):
+    """fvc_tune_space (autotuner candidate sets)."""
+    _fields_ = [("n_rows", C.c_int32), ("rows", C.c_int32 * 3), ("n_steps", C.c_int32), ("steps", C.c_int32 * 3),
+                ("n_targets", C.c_int32), ("pad_", C.c_int32), ("targets", C.c_int64 * 8)]
+
+    @classmethod
+    def of(cls, rows, targets, steps):
+        sp = cls()
+        sp.n_rows, sp.n_steps, sp.n_targets = len(rows), len(steps), len(targets)
+        for i, v in enumerate(rows):
+            sp.rows[i] = v
+        for i, v in enumerate(steps):
+            sp.steps[i] = v
+        for i, v in enumerate(targets):
+            sp.targets[i] = v
+        return sp
 
 
 class DiffStats(C.Structure):

@@ -359,6 +359,18 @@ int fvc_renderer_autotune(fvc_renderer* h, const fvc_raymarch_info* rm, int cach
   });
 }
 
+int fvc_renderer_autotune_batch(fvc_renderer* h, fvc_model* const* models, const fvc_raymarch_info* rms,
+                                int32_t n_views, int cached, const fvc_tune_space* space, fvc_tune_result* result) {
+  return guard([&] {
+    REQUIRE(h && rms && n_views > 0, "invalid argument");
+    for (int v = 0; v < n_views; v++) check_rm(&rms[v]);
+    std::vector<const Model*> ms((size_t)n_views, nullptr);
+    for (int v = 0; models && v < n_views; v++) ms[v] = models[v] ? &models[v]->m : nullptr;
+    std::scoped_lock lk(h->r.read_mutex);
+    autotune(h->r, Views(rms, n_views, nullptr, h->r.model, ms.data()), cached != 0, space, result);
+  });
+}
+
 int fvc_renderer_set_graph(fvc_renderer* h, int enable) {
   return guard([&] {
     REQUIRE(h, "null renderer");

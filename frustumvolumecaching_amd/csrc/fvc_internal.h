@@ -499,5 +499,7 @@ struct Views {
 };
 void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats);
 void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats);
+// space: candidate sets (null: the default sweep)
+void autotune(Renderer& r, const Views& vs, bool cached, const fvc_tune_space* space, fvc_tune_result* out);
 
 }  // namespace fvc

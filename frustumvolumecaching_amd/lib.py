@@ -57,6 +57,8 @@ _SIGS = {
     "fvc_renderer_set_stream": (C.c_int, [_P, _P]),
     "fvc_resize_renderbuffers": (C.c_int, [_P, C.c_int, C.c_int]),
     "fvc_renderer_autotune": (C.c_int, [_P, C.POINTER(abi.RaymarchInfo), C.c_int, C.POINTER(abi.TuneResult)]),
+    "fvc_renderer_autotune_batch": (C.c_int, [_P, C.POINTER(_P), C.POINTER(abi.RaymarchInfo), C.c_int32, C.c_int,
+                                              C.POINTER(abi.TuneSpace), C.POINTER(abi.TuneResult)]),
     "fvc_renderer_set_mlp_rows": (C.c_int, [_P, C.c_int32]),
     "fvc_renderer_set_graph": (C.c_int, [_P, C.c_int]),
     "fvc_renderer_set_batching": (C.c_int, [_P, C.c_int64, C.c_int32]),
@@ -305,6 +307,20 @@ class Renderer:
         """Pick MLP rows per wavefront and chunk sizes by timing renders of rm."""
         res = abi.TuneResult()
         check(load().fvc_renderer_autotune(self.h, C.byref(rm), int(bool(cached)), C.byref(res)))
+        return res
+
+    def autotune_batch(self, rms, cached=False, models=None, rows=None, targets=None, steps=None):
+        """autotune over a batch of views (and per-view models, full only);
+        rows / targets / steps: candidate sets (None: the defaults)."""
+        res = abi.TuneResult()
+        n = len(rms)
+        arr = (abi.RaymarchInfo * n)(*rms)
+        mp = None if models is None else (_P * n)(*[None if m is None else m.h for m in models])
+        sp = None
+        if rows or targets or steps:
+            sp = abi.TuneSpace.of(rows or (16, 32, 64), targets or (1 << 23, 1 << 24, 1 << 25), steps or (16, 32, 64))
+        check(load().fvc_renderer_autotune_batch(self.h, mp, arr, n, int(bool(cached)),
+                                                 None if sp is None else C.byref(sp), C.byref(res)))
         return res
 
     def set_graph(self, enable=True):

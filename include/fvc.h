@@ -421,7 +421,7 @@ int fvc_cache_get_pool_decoded(fvc_renderer* renderer, int which, float* out_hos
 int fvc_cache_set_capacity(fvc_renderer* renderer, int32_t capacity_bricks);
 
 /* ---- autotuning (no counterpart in the reference) ---- */
-#define FVC_TUNE_MAX 32
+#define FVC_TUNE_MAX 64
 typedef struct {
   int32_t mlp_rows;               /* chosen MLP rows per wavefront: 16, 32 or 64 */
   int32_t max_steps;              /* chosen samples per ray per launch */
@@ -439,6 +439,22 @@ typedef struct {
  * rows per wavefront and sub-iteration chunk sizes, keeps the fastest for
  * this renderer and returns the table.  Images are not written. */
 int fvc_renderer_autotune(fvc_renderer* renderer, const fvc_raymarch_info* rm, int cached, fvc_tune_result* result);
+/* Candidate sets of a sweep: MLP rows per wavefront (16, 32, 64), samples
+ * per ray per launch (1..64) and target samples per launch (2^10..2^28; a
+ * batch scales it by its pixels over its largest view's); at most
+ * FVC_TUNE_MAX combinations. */
+typedef struct {
+  int32_t n_rows, rows[3];
+  int32_t n_steps, steps[3];
+  int32_t n_targets, pad_;
+  int64_t targets[8];
+} fvc_tune_space;
+/* fvc_renderer_autotune over a batch of views (full: the views may name
+ * models as in fvc_render_full_batch_scenes, models may be NULL) and a
+ * candidate space (NULL: rows {16, 32, 64} x targets {2^23, 2^24, 2^25} x
+ * steps {16, 32, 64}). */
+int fvc_renderer_autotune_batch(fvc_renderer* renderer, fvc_model* const* models, const fvc_raymarch_info* rms,
+                                int32_t n_views, int cached, const fvc_tune_space* space, fvc_tune_result* result);
 /* Graph mode (no counterpart in the reference): each full or cached frame
  * is recorded with hipStreamBeginCapture into a hipGraph (ray setup and as
  * many sub-iterations as the previous frame needed, plus one; kernels read

@@ -171,3 +171,27 @@ def test_mixed_scene_batch_rejects_other_networks(setup):
             r.render_full_batch(views[:2], models=[None, m2])
     finally:
         m2.close()
+
+
+def test_autotune_batch_over_chunk_sizes(setup):
+    """The tuner over a batch of one view per resolution, MLP rows {16, 64}
+    and chunk sizes 2^16 .. 2^20 samples per launch: every candidate is
+    timed, the choice is the fastest, and the images do not change."""
+    wl, model, r, path, views = setup
+    vs = views[:4]
+    before = [_img(rm) for rm in vs]
+    r.render_full_batch(vs, before)
+    targets = tuple(1 << k for k in range(16, 21))
+    res = r.autotune_batch(vs, rows=(16, 64), targets=targets, steps=(32,))
+    print(f"[tune] batch of {len(vs)}: rows {res.mlp_rows} target 2^{int(np.log2(res.target_batch))} "
+          f"steps {res.max_steps}: {res.ms_best:.2f} ms (default {res.ms_default:.2f} ms)", flush=True)
+    assert res.n_tried == 10
+    assert sorted(set(res.targets[:res.n_tried])) == list(targets)
+    assert res.ms_best <= min(res.ms[:res.n_tried]) + 1e-6
+    assert res.mlp_rows in (16, 64) and res.target_batch in targets and res.max_steps == 32
+    after = [_img(rm) for rm in vs]
+    r.render_full_batch(vs, after)
+    for a, b in zip(before, after):
+        assert torch.equal(a, b)
+    r.set_batching(1 << 25, 32)
+    r.set_mlp_rows(16)
