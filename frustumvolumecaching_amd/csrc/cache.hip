@@ -732,8 +732,8 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
                                                      ContractParams con, const uint32_t* __restrict__ coarse_g,
                                                      CacheView cv, RayDyn* __restrict__ rays, int4* __restrict__ entry,
                                                      int* __restrict__ alive_flag, float* __restrict__ t_scr,
-                                                     float* __restrict__ s_pos, float* __restrict__ s_dir,
-                                                     float* __restrict__ s_t0, float4* __restrict__ c_rec,
+                                                     float4* __restrict__ s_pos, float* __restrict__ s_t0,
+                                                     float4* __restrict__ c_rec,
                                                      float* __restrict__ c_t0, const int* __restrict__ pix_in,
                                                      uint4* __restrict__ ray_sh, float4* __restrict__ ray_od,
                                                      uint64_t* __restrict__ order_out,
@@ -844,7 +844,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
   const int off_r = slot[0], off_c = slot[1];
   if (valid) {
     const V3 o = {r.ox, r.oy, r.oz}, d = {r.dx, r.dy, r.dz};
-    const V3 d01 = unit_to_01(d);
+    const float pxf = __builtin_bit_cast(float, px);
     uint64_t order = 0;  // bit q set: the q-th kept sample is a cache sample
     int kr = 0, kc = 0;
     const uint64_t kept = res_mask | cache_mask;
@@ -865,8 +865,7 @@ __global__ __launch_bounds__(256) void k_cache_march(int* __restrict__ ctl, int 
           // generateNetworkInput_resample_perf_kernel (198-240): t_mid = t0 + dt/2
           const V3 p = contract(o + tm2 * d, con);
           const int64_t q = (int64_t)off_r + kr;
-          s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
-          s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
+          s_pos[q] = make_float4(p.x, p.y, p.z, pxf);
           s_t0[q] = t0;
         } else {
           // generateNetworkInput_preparation_perf_kernel (242-313)
@@ -1401,16 +1400,15 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       ev = P.begin(s);
       hipLaunchKernelGGL(k_cache_march, grid, dim3(256), 0, s, ctl, cur, cutoff, mg, m.con, m.coarse.as<uint32_t>(), cv,
                          r.rays[cur].as<RayDyn>(), r.ray_entry.as<int4>(), r.ray_alive.as<int>(),
-                         r.t_scratch.as<float>(), r.s_pos.as<float>(), r.s_dir.as<float>(), r.s_t0.as<float>(),
-                         r.c_rec.as<float4>(), r.c_t0.as<float>(), r.ray_pix[cur].as<int>(), r.ray_sh.as<uint4>(),
+                         r.t_scratch.as<float>(), r.s_pos.as<float4>(), r.s_t0.as<float>(), r.c_rec.as<float4>(), r.c_t0.as<float>(), r.ray_pix[cur].as<int>(), r.ray_sh.as<uint4>(),
                          m.net.split_head ? r.ray_od.as<float4>() : nullptr,
                          r.ray_order.as<uint64_t>(), r.ray_const.as<RayState>(), r.render_mode, r.dbg_codes, r.dbg_k,
                          r.dbg_n);
       FVC_HIP(hipGetLastError());
       h.march = P.end(ev, s, "cache_march");
       ev = P.begin(s);
-      launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), n_bound, r.s_out.as<uint32_t>(), s,
-                          ctl + CTL_R, r.mlp_rows);
+      launch_network_full_pix(m, r.s_pos.as<float4>(), r.ray_sh.as<uint4>(), n_bound, r.s_out.as<uint32_t>(), s,
+                              ctl + CTL_R, r.mlp_rows);
       h.net = P.end(ev, s, "network_resample");
       ev = P.begin(s);
 #define L2(P, F8)                                                                                                    \
@@ -1476,8 +1474,11 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
     const double bins = (double)(uint32_t)l.bins;
     // per bin: occupancy byte; per classified sample: index + density probe
     // (8 B) + 28 B stream record; ray state + entry per ray
-    P.work(hs[i].march, bins, bins + 36.0 * (l.n_r + l.n_c) + 84.0 * l.alive, 0);
-    P.work(hs[i].net, l.n_r, l.n_r * (ALG_GATHER_B + 24.0 + 8.0), l.n_r * net_flop(m.net));
+    // march: 1 B occupancy per bin; per emitted sample its 20 B record
+    // (position or froxel + pixel, t0) and the 12 B its classification reads
+    // (known range, brick slot, voxel density); ray state + entry per ray
+    P.work(hs[i].march, bins, bins + 32.0 * (l.n_r + l.n_c) + 84.0 * l.alive, 0);
+    P.work(hs[i].net, l.n_r, l.n_r * (ALG_GATHER_B + 16.0 + 8.0), l.n_r * net_flop(m.net));
     P.work(hs[i].head, l.n_c, l.n_c * (ALG_LOOKUP_B + (m.net.split_head ? 40.0 : 20.0) + 8.0),
            l.n_c * (m.net.split_head ? ALG_VD_COLOUR_FLOP : ALG_COLOUR_FLOP));
     P.work(hs[i].comp, l.n_r + l.n_c, 12.0 * (l.n_r + l.n_c) + 132.0 * l.alive, 0);

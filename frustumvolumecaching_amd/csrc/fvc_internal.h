@@ -153,6 +153,12 @@ void nerfacc_read_data(const std::string& dir, const NerfaccExport& ex, uint16_t
 // n_dev (optional): the sample count in device memory, n its upper bound
 void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s, const int* n_dev = nullptr, int rows = 16);
+// the renderer's sample stream: pos4 = (x, y, z, pixel index bits) per
+// sample, ray_sh the per-pixel SH table (sh4_all)
+void launch_network_full_pix(const Model& m, const float4* pos4, const uint4* ray_sh, int64_t n, uint32_t* out_h4,
+                             hipStream_t s, const int* n_dev, int rows);
+void launch_network_full_ex(const Model& m, const float* pos, const float* dir, bool pix, int64_t n,
+                            uint32_t* out_h4, hipStream_t s, const int* n_dev, int rows);
 void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s,
                             int rows = 16, const int* n_dev = nullptr);
 void launch_network_head(const Model& m, const uint16_t* head_in16, const float* dir, int64_t n, uint32_t* out_h4,
@@ -161,8 +167,8 @@ void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t
 // split models (viewdep.hip): full inference (the initial view direction is
 // dir), latents with the initial view direction init_dir ([8 head latents,
 // log sigma] per sample), the custom head alone
-void launch_network_full_vd(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
-                            hipStream_t s, const int* n_dev, int rows);
+void launch_network_full_vd(const Model& m, const float* pos, const float* dir, bool pix, int64_t n,
+                            uint32_t* out_h4, hipStream_t s, const int* n_dev, int rows);
 void launch_network_latents_vd(const Model& m, const float* pos, const float* init_dir, int64_t n, uint16_t* out16,
                                hipStream_t s, int rows, const int* n_dev = nullptr);
 void launch_network_head_vd(const Model& m, const uint16_t* head_in16, const float* pos, const float* dir,
@@ -401,7 +407,7 @@ struct Renderer {
   DevBuf counters;       // int32 device counters
   int* pinned = nullptr; // pinned host mirror of the counters
   // sample streams
-  DevBuf s_pos, s_dir, s_t0, s_out;            // resample / full stream
+  DevBuf s_pos, s_t0, s_out;  // resample / full stream: float4 (position, pixel index bits), t0, half4 output
   DevBuf c_rec, c_t0, c_out;  // cache-hit stream: c_rec = (froxel x, y, z, pixel index bits) per sample
   DevBuf ray_od;              // split models: ray origin, direction per pixel (2 float4)
   DevBuf ray_sh;                                // 16 fp16 SH coefficients per pixel (first launch of its ray)

@@ -212,19 +212,33 @@ __device__ __forceinline__ void sh4_all(float dx01, float dy01, float dz01, uint
 // e=4..7 carry SH coefficients 4g..4g+3.  Values are the fp16 ones the
 // reference stores, then rounded to the MLP's operand precision.
 template <int P>
-__device__ __forceinline__ typename Mx<P>::T head_input(const _Float16* rows4, int g, float dx01, float dy01,
-                                                         float dz01, float inv_act, int lo, int hi) {
+__device__ __forceinline__ typename Mx<P>::T head_input_sh(const _Float16* rows4, int g, const _Float16* sh,
+                                                            float inv_act, int lo, int hi) {
   float v[8];
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     const int r = 4 * g + e;
     v[e] = (r >= lo && r < hi) ? (float)rows4[e] : 1.0f;
   }
-  _Float16 sh[4];
-  sh4_quarter(dx01, dy01, dz01, g, sh);
 #pragma unroll
   for (int e = 0; e < 4; e++) v[4 + e] = (float)sh[e];
   return Mx<P>::pack(v, inv_act);
+}
+template <int P>
+__device__ __forceinline__ typename Mx<P>::T head_input(const _Float16* rows4, int g, float dx01, float dy01,
+                                                         float dz01, float inv_act, int lo, int hi) {
+  _Float16 sh[4];
+  sh4_quarter(dx01, dy01, dz01, g, sh);
+  return head_input_sh<P>(rows4, g, sh, inv_act, lo, hi);
+}
+
+// SH coefficients 4g..4g+3 from a pixel's table entry (sh4_all's 16 halves,
+// read as four uint2)
+__device__ __forceinline__ void sh4_unpack(uint2 q, _Float16* sh) {
+  sh[0] = __builtin_bit_cast(_Float16, (uint16_t)(q.x & 0xffffu));
+  sh[1] = __builtin_bit_cast(_Float16, (uint16_t)(q.x >> 16));
+  sh[2] = __builtin_bit_cast(_Float16, (uint16_t)(q.y & 0xffffu));
+  sh[3] = __builtin_bit_cast(_Float16, (uint16_t)(q.y >> 16));
 }
 
 // ------------------------------------------------------------ split models

@@ -181,7 +181,11 @@ __device__ __forceinline__ uint2 pack_h4(_Float16 a, _Float16 b, _Float16 c, _Fl
 // !HEAD: inferenceLatents -> 16 halves per sample
 // TILES: 16-sample MFMA tiles per wave and loop iteration (MLP rows per
 // wavefront = 16 TILES); the tiles' hash gathers are issued together.
-template <bool HEAD, int P, int TILES>
+// PIX (renderer sample streams): pos holds float4 (x, y, z, pixel index
+// bits) per sample and dir is the per-pixel SH table (sh4_all, 32 bytes per
+// pixel, written by the marcher at the ray's first launch) instead of a
+// direction per sample.
+template <bool HEAD, int P, int TILES, bool PIX = false>
 __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__ pos, const float* __restrict__ dir,
                                                        int64_t n, const uint32_t* __restrict__ table,
                                                        const typename Mx<P>::T* __restrict__ frags, GridParams gp,
@@ -205,6 +209,8 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
     bool valid[TILES];
     float f[TILES][8];
     V3 pv[TILES], dv[TILES];  // positions; view directions (used after the density MLP)
+    uint2 shq[TILES];         // PIX: the lane group's SH coefficients
+    int px[TILES];
     // every tile's position (and direction) is loaded before any tile's
     // gathers, which wait for them (the wait counter is in order)
 #pragma unroll
@@ -212,15 +218,26 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
       s[t] = (grp * TILES + t) * 16 + col;
       valid[t] = s[t] < n;
       sl[t] = valid[t] ? s[t] : n - 1;
-      pv[t] = V3{pos[3 * sl[t]], pos[3 * sl[t] + 1], pos[3 * sl[t] + 2]};
-      if (HEAD) dv[t] = V3{dir[3 * sl[t]], dir[3 * sl[t] + 1], dir[3 * sl[t] + 2]};
+      if constexpr (PIX) {
+        const float4 q = reinterpret_cast<const float4*>(pos)[sl[t]];
+        pv[t] = V3{q.x, q.y, q.z};
+        px[t] = __builtin_bit_cast(int, q.w);
+      } else {
+        pv[t] = V3{pos[3 * sl[t]], pos[3 * sl[t] + 1], pos[3 * sl[t] + 2]};
+        if (HEAD) dv[t] = V3{dir[3 * sl[t]], dir[3 * sl[t] + 1], dir[3 * sl[t] + 2]};
+      }
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
       encode_quads(lv, gp, table, pv[t], g, f[t]);
       // pinned here (after the gathers are issued): otherwise the load sinks
       // to its use after the density MLP and waits there on its own
-      if (HEAD) asm volatile("" : "+v"(dv[t].x), "+v"(dv[t].y), "+v"(dv[t].z));
+      if constexpr (HEAD && PIX) {
+        shq[t] = reinterpret_cast<const uint2*>(dir)[(int64_t)px[t] * 4 + g];
+        asm volatile("" : "+v"(shq[t].x), "+v"(shq[t].y));
+      } else if (HEAD) {
+        asm volatile("" : "+v"(dv[t].x), "+v"(dv[t].y), "+v"(dv[t].z));
+      }
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
@@ -232,8 +249,14 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
       if (!HEAD) {
         if (valid[t]) out16[s[t] * 4 + g] = pack_h4(rows[0], rows[1], rows[2], rows[3]);
       } else {
-        const typename Mx<P>::T hin = head_input<P>(rows, g, dv[t].x, dv[t].y, dv[t].z, sc.inv_act, sc.lat_lo,
-                                                    sc.lat_hi);
+        typename Mx<P>::T hin;
+        if constexpr (PIX) {
+          _Float16 sh[4];
+          sh4_unpack(shq[t], sh);
+          hin = head_input_sh<P>(rows, g, sh, sc.inv_act, sc.lat_lo, sc.lat_hi);
+        } else {
+          hin = head_input<P>(rows, g, dv[t].x, dv[t].y, dv[t].z, sc.inv_act, sc.lat_lo, sc.lat_hi);
+        }
         const f4 o = head_mlp<P>(wl + FRAG_H1 * 64, wl + FRAG_H2 * 64, li, hin, sc);
         if (g == 0 && valid[t]) out_h4[s[t]] = pack_h4((_Float16)o[0], (_Float16)o[1], (_Float16)o[2], rows[0]);
       }
@@ -281,7 +304,8 @@ __global__ __launch_bounds__(NET_BLOCK) void k_head(const uint16_t* __restrict__
 // view direction; output [8 head latents, log sigma, 0 x 7] per sample.
 // di: the first latent is log sigma (the first head's latent row 0 is then
 // a zero-weight row replaced here).  FL: first-head hidden layers.
-template <bool FULL, int TILES, int FL>
+// PIX (FULL only): the renderer's sample stream as in k_network.
+template <bool FULL, int TILES, int FL, bool PIX = false>
 __global__ __launch_bounds__(NET_BLOCK) void k_network_vd(const float* __restrict__ pos, const float* __restrict__ dir,
                                                           int64_t n, const uint32_t* __restrict__ table,
                                                           const h8* __restrict__ frags, GridParams gp, int di,
@@ -305,19 +329,32 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network_vd(const float* __restric
     bool valid[TILES];
     float f[TILES][8];
     V3 pv[TILES], dv[TILES];  // positions and directions, loaded before the gathers
+    uint2 shq[TILES];         // PIX: the lane group's SH coefficients
+    int px[TILES];
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
       s[t] = (grp * TILES + t) * 16 + col;
       valid[t] = s[t] < n;
       sl[t] = valid[t] ? s[t] : n - 1;
-      pv[t] = V3{pos[3 * sl[t]], pos[3 * sl[t] + 1], pos[3 * sl[t] + 2]};
-      dv[t] = V3{dir[3 * sl[t]], dir[3 * sl[t] + 1], dir[3 * sl[t] + 2]};
+      if constexpr (PIX) {
+        const float4 q = reinterpret_cast<const float4*>(pos)[sl[t]];
+        pv[t] = V3{q.x, q.y, q.z};
+        px[t] = __builtin_bit_cast(int, q.w);
+      } else {
+        pv[t] = V3{pos[3 * sl[t]], pos[3 * sl[t] + 1], pos[3 * sl[t] + 2]};
+        dv[t] = V3{dir[3 * sl[t]], dir[3 * sl[t] + 1], dir[3 * sl[t] + 2]};
+      }
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
       encode_quads(lv, gp, table, pv[t], g, f[t]);
       // pinned here (after the gathers are issued), as in k_network
-      asm volatile("" : "+v"(dv[t].x), "+v"(dv[t].y), "+v"(dv[t].z));
+      if constexpr (PIX) {
+        shq[t] = reinterpret_cast<const uint2*>(dir)[(int64_t)px[t] * 4 + g];
+        asm volatile("" : "+v"(shq[t].x), "+v"(shq[t].y));
+      } else {
+        asm volatile("" : "+v"(dv[t].x), "+v"(dv[t].y), "+v"(dv[t].z));
+      }
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
@@ -326,7 +363,8 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network_vd(const float* __restric
       const f4 d3 = base_mlp<FVC_PREC_FP16>(wl, li, MxH::pack(f[t], 1.0f), sc);
       const _Float16 bh[4] = {(_Float16)d3[0], (_Float16)d3[1], (_Float16)d3[2], (_Float16)d3[3]};
       _Float16 sh[4];
-      sh4_quarter(dv[t].x, dv[t].y, dv[t].z, g, sh);
+      if constexpr (PIX) sh4_unpack(shq[t], sh);
+      else sh4_quarter(dv[t].x, dv[t].y, dv[t].z, g, sh);
       const f4 fo = first_head_mlp<FL>(wl, li, h8_of(bh, sh));
       _Float16 lat[4] = {(_Float16)fo[0], (_Float16)fo[1], (_Float16)fo[2], (_Float16)fo[3]};
       if (di && g == 0) lat[0] = bh[0];
@@ -426,15 +464,27 @@ static int grid_for(int64_t n_groups) {
 
 void launch_network_full(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s, const int* n_dev, int rows) {
+  launch_network_full_ex(m, pos, dir, false, n, out_h4, s, n_dev, rows);
+}
+void launch_network_full_pix(const Model& m, const float4* pos4, const uint4* ray_sh, int64_t n, uint32_t* out_h4,
+                             hipStream_t s, const int* n_dev, int rows) {
+  launch_network_full_ex(m, reinterpret_cast<const float*>(pos4), reinterpret_cast<const float*>(ray_sh), true, n,
+                         out_h4, s, n_dev, rows);
+}
+void launch_network_full_ex(const Model& m, const float* pos, const float* dir, bool pix, int64_t n, uint32_t* out_h4,
+                            hipStream_t s, const int* n_dev, int rows) {
   if (n <= 0) return;
   if (m.net.split_head) {
-    launch_network_full_vd(m, pos, dir, n, out_h4, s, n_dev, rows);
+    launch_network_full_vd(m, pos, dir, pix, n, out_h4, s, n_dev, rows);
     return;
   }
-#define L2(P, TL)                                                                                                \
-  hipLaunchKernelGGL((k_network<true, P, TL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), 0, s, \
-                     pos, dir, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, m.mlp_scales,   \
+#define L3(P, TL, PX)                                                                                              \
+  hipLaunchKernelGGL((k_network<true, P, TL, PX>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), 0, \
+                     s, pos, dir, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, m.mlp_scales,  \
                      (uint2*)out_h4, (uint2*)nullptr, n_dev)
+#define L2(P, TL)          \
+  if (pix) { L3(P, TL, true); } \
+  else { L3(P, TL, false); }
 #define L(P)                       \
   if (rows == 64) { L2(P, 4); }    \
   else if (rows == 32) { L2(P, 2); } \
@@ -442,6 +492,7 @@ void launch_network_full(const Model& m, const float* pos, const float* dir, int
   FVC_BY_PREC(m, L)
 #undef L
 #undef L2
+#undef L3
   FVC_HIP(hipGetLastError());
 }
 void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s, int rows,
@@ -472,20 +523,24 @@ void launch_network_head(const Model& m, const uint16_t* hin16, const float* dir
 #undef L
   FVC_HIP(hipGetLastError());
 }
-void launch_network_full_vd(const Model& m, const float* pos, const float* dir, int64_t n, uint32_t* out_h4,
-                            hipStream_t s, const int* n_dev, int rows) {
+void launch_network_full_vd(const Model& m, const float* pos, const float* dir, bool pix, int64_t n,
+                            uint32_t* out_h4, hipStream_t s, const int* n_dev, int rows) {
   if (n <= 0) return;
   const int di = m.net.first_latent_is_density, fq = m.net.freq_encoding_degree > 0;
-#define L(TL, FL)                                                                                              \
-  hipLaunchKernelGGL((k_network_vd<true, TL, FL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), \
-                     0, s, pos, dir, n, m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, di, fq, (uint2*)out_h4,  \
-                     n_dev)
+#define L2(TL, FL, PX)                                                                                            \
+  hipLaunchKernelGGL((k_network_vd<true, TL, FL, PX>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))),             \
+                     dim3(NET_BLOCK), 0, s, pos, dir, n, m.table.as<uint32_t>(), m.frags.as<h8>(), m.grid, di, fq,  \
+                     (uint2*)out_h4, n_dev)
+#define L(TL, FL)                \
+  if (pix) { L2(TL, FL, true); } \
+  else { L2(TL, FL, false); }
   if (m.net.first_head_n_hidden_layers == 2) {
     if (rows == 64) { L(4, 2); } else if (rows == 32) { L(2, 2); } else { L(1, 2); }
   } else {
     if (rows == 64) { L(4, 1); } else if (rows == 32) { L(2, 1); } else { L(1, 1); }
   }
 #undef L
+#undef L2
   FVC_HIP(hipGetLastError());
 }
 void launch_network_latents_vd(const Model& m, const float* pos, const float* init_dir, int64_t n, uint16_t* out16,

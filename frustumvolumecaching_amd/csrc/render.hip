@@ -13,6 +13,7 @@
 #include <mutex>
 
 #include "march.h"
+#include "mlp_mfma.h"
 #include "render_common.h"
 
 namespace fvc {
@@ -46,10 +47,14 @@ __global__ __launch_bounds__(256) void k_init_rays(fvc_raymarch_info rm, int pix
 }
 
 // sample_performance_kernel (performance_renderer.cu:15-69): march until
-// n_steps occupied bin midpoints or t_mid > far; emit contracted position,
-// unit_to_01(dir) and t0 of every occupied sample into the compacted stream.
-// The t0 of a ray's k-th sample of the launch goes to t_scr[k * n_alive + i]
-// (coalesced across the wave when read back for the stream).
+// n_steps occupied bin midpoints or t_mid > far; emit the contracted
+// position with the pixel index (float4) and t0 of every occupied sample
+// into the compacted stream.  The view direction is per ray: at its first
+// launch a ray writes the SH coefficients of unit_to_01(dir) to its pixel's
+// entry of ray_sh, which the network reads instead of a direction per
+// sample.  The t0 of a ray's k-th sample of the launch goes to
+// t_scr[k * n_alive + i] (coalesced across the wave when read back for the
+// stream).
 static constexpr int EMIT = 16;  // samples whose t0 the full marcher's emission loads together
 
 // MULTI (mixed-scene batches): each ray reads the occupancy of its view's
@@ -61,8 +66,8 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
                                                const SceneRef* __restrict__ scenes, RayDyn* __restrict__ rays,
                                                const RayState* __restrict__ rconst, const int* __restrict__ pix_in,
                                                int4* __restrict__ entry, int* __restrict__ alive_flag,
-                                               float* __restrict__ t_scr, float* __restrict__ s_pos,
-                                               float* __restrict__ s_dir, float* __restrict__ s_t0) {
+                                               float* __restrict__ t_scr, float4* __restrict__ s_pos,
+                                               uint4* __restrict__ ray_sh, float* __restrict__ s_t0) {
   const int n_alive = ctl[CTL_ALIVE + par];
   if ((int)(blockIdx.x * blockDim.x) >= n_alive) return;  // whole block past the alive rays
   const int n_steps = ctl[CTL_NSTEPS];
@@ -75,11 +80,21 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
   int cnt = 0, bins = 0;
   bool alive = false;
   RayState rs{};
+  int px = 0;
   if (valid) {
     const RayDyn dy = rays[i];
-    rs = rconst[pix_in[i]];
+    px = pix_in[i];
+    rs = rconst[px];
     rs.t = dy.t;
     rs.j = dy.j;
+    if (rs.j == 0) {
+      // first launch of the ray: the colour MLP's SH input of its direction
+      uint4 lo, hi;
+      const V3 d01 = unit_to_01(V3{rs.dx, rs.dy, rs.dz});
+      sh4_all(d01.x, d01.y, d01.z, lo, hi);
+      ray_sh[2 * (int64_t)px] = lo;
+      ray_sh[2 * (int64_t)px + 1] = hi;
+    }
     alive = true;
     MarchGrid g = mg;
     const uint32_t* cz;
@@ -108,7 +123,7 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
   const int off = slot[0];
   if (valid) {
     const V3 o = {rs.ox, rs.oy, rs.oz}, d = {rs.dx, rs.dy, rs.dz};
-    const V3 d01 = unit_to_01(d);
+    const float pxf = __builtin_bit_cast(float, px);
     // the t0 of EMIT samples are loaded together before they are used
     for (int k0 = 0; k0 < cnt; k0 += EMIT) {
       float t0v[EMIT];
@@ -123,8 +138,7 @@ __global__ __launch_bounds__(256) void k_march(int* __restrict__ ctl, int par, M
         const float tm = (t0 + t1) * 0.5f;
         const V3 p = contract(o + tm * d, con);
         const int64_t q = (int64_t)off + k;
-        s_pos[3 * q] = p.x; s_pos[3 * q + 1] = p.y; s_pos[3 * q + 2] = p.z;
-        s_dir[3 * q] = d01.x; s_dir[3 * q + 1] = d01.y; s_dir[3 * q + 2] = d01.z;
+        s_pos[q] = make_float4(p.x, p.y, p.z, pxf);
         s_t0[q] = t0;
       }
     }
@@ -257,9 +271,9 @@ void ensure_ray_capacity(Renderer& r, int64_t n, int64_t target) {
   r.t_scratch.ensure(n * 2 * MAX_STEPS_PER_LAUNCH * sizeof(float));
   r.ray_alive.ensure(n * sizeof(int));
   r.counters.ensure(CTL_WORDS * sizeof(int));
-  r.s_pos.ensure(cap * 3 * sizeof(float));
-  r.s_dir.ensure(cap * 3 * sizeof(float));
+  r.s_pos.ensure(cap * sizeof(float4));
   r.s_t0.ensure(cap * sizeof(float));
+  r.ray_sh.ensure(n * 32);
   r.s_out.ensure(cap * sizeof(uint2));
   r.ray_capacity = n;
   r.sample_capacity = cap;
@@ -535,13 +549,13 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
       ev = P.begin(s);
       hipLaunchKernelGGL(multi ? k_march<true> : k_march<false>, grid, dim3(256), 0, s, ctl, cur, mg, m.con,
                          m.coarse.as<uint32_t>(), r.scene_refs.as<SceneRef>(), r.rays[cur].as<RayDyn>(), r.ray_const.as<RayState>(), r.ray_pix[cur].as<int>(),
-                         r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.t_scratch.as<float>(), r.s_pos.as<float>(),
-                         r.s_dir.as<float>(), r.s_t0.as<float>());
+                         r.ray_entry.as<int4>(), r.ray_alive.as<int>(), r.t_scratch.as<float>(),
+                         r.s_pos.as<float4>(), r.ray_sh.as<uint4>(), r.s_t0.as<float>());
       FVC_HIP(hipGetLastError());
       h.march = P.end(ev, s, "march");
       ev = P.begin(s);
-      launch_network_full(m, r.s_pos.as<float>(), r.s_dir.as<float>(), n_bound, r.s_out.as<uint32_t>(), s,
-                          ctl + CTL_R, r.mlp_rows);
+      launch_network_full_pix(m, r.s_pos.as<float4>(), r.ray_sh.as<uint4>(), n_bound, r.s_out.as<uint32_t>(), s,
+                              ctl + CTL_R, r.mlp_rows);
       h.net = P.end(ev, s, "network_full");
       ev = P.begin(s);
       hipLaunchKernelGGL(k_composite, grid, dim3(256), 0, s, ctl, cur, rm.stepsize_info, m.scene.alpha_thre,
@@ -580,9 +594,10 @@ void render_full(Renderer& r, const Views& vs, fvc_render_stats* stats) {
     const IterLog& l = lg[i];
     if (l.alive > 0) n_real = i + 1;
     const double bins = (double)(uint32_t)l.bins;
-    // 1 B occupancy per bin, 28 B per emitted sample, ray state + entry per ray
-    P.work(hs[i].march, bins, bins + 28.0 * l.n_r + 84.0 * l.alive, 0);
-    P.work(hs[i].net, l.n_r, l.n_r * (ALG_GATHER_B + 24.0 + 8.0), l.n_r * net_flop(m.net));
+    // 1 B occupancy per bin, 20 B per emitted sample (position + pixel, t0),
+    // ray state + entry per ray
+    P.work(hs[i].march, bins, bins + 20.0 * l.n_r + 84.0 * l.alive, 0);
+    P.work(hs[i].net, l.n_r, l.n_r * (ALG_GATHER_B + 16.0 + 8.0), l.n_r * net_flop(m.net));
     P.work(hs[i].comp, l.n_r, 12.0 * l.n_r + 124.0 * l.alive, 0);
     n_eval += l.n_r;
   }
@@ -618,8 +633,8 @@ static constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 static constexpr size_t NOALLOC_CTL = align256(CTL_WORDS * sizeof(int));
 static constexpr size_t NOALLOC_PER_PIXEL = 2 * sizeof(RayDyn) + sizeof(RayState) + 2 * sizeof(int) +
                                             2 * sizeof(float4) + sizeof(float4) + sizeof(int4) + sizeof(int) +
-                                            NOALLOC_STEPS * sizeof(float);
-static constexpr size_t NOALLOC_PER_SAMPLE = 3 * sizeof(float) + 3 * sizeof(float) + sizeof(float) + sizeof(uint2);
+                                            NOALLOC_STEPS * sizeof(float) + 2 * sizeof(uint4);
+static constexpr size_t NOALLOC_PER_SAMPLE = sizeof(float4) + sizeof(float) + sizeof(uint2);
 static constexpr size_t NOALLOC_SCRATCH_BYTES =
     NOALLOC_CTL + align256(NOALLOC_MAX_PIXELS * NOALLOC_PER_PIXEL) + 16 * 256 +
     align256(NOALLOC_SAMPLES * NOALLOC_PER_SAMPLE);
@@ -674,9 +689,9 @@ void render_full_noalloc(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba
   int4* entry = reinterpret_cast<int4*>(take(P * sizeof(int4)));
   int* alive = reinterpret_cast<int*>(take(P * sizeof(int)));
   float* t_scr = reinterpret_cast<float*>(take(P * NOALLOC_STEPS * sizeof(float)));
+  uint4* ray_sh = reinterpret_cast<uint4*>(take(P * 2 * sizeof(uint4)));
   const int64_t S = NOALLOC_SAMPLES;
-  float* s_pos = reinterpret_cast<float*>(take(S * 3 * sizeof(float)));
-  float* s_dir = reinterpret_cast<float*>(take(S * 3 * sizeof(float)));
+  float4* s_pos = reinterpret_cast<float4*>(take(S * sizeof(float4)));
   float* s_t0 = reinterpret_cast<float*>(take(S * sizeof(float)));
   uint2* s_out = reinterpret_cast<uint2*>(take(S * sizeof(uint2)));
   if ((size_t)(p - base) > NOALLOC_SCRATCH_BYTES) throw Error(FVC_E_INVALID, "scratch layout exceeds its array");
@@ -702,9 +717,10 @@ void render_full_noalloc(Renderer& r, const fvc_raymarch_info& rm, uint8_t* rgba
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, s, ctl, it, cap, NOALLOC_STEPS, cap, 0);
     hipLaunchKernelGGL(k_march<false>, grid, dim3(256), 0, s, ctl, cur, mg, m.con, m.coarse.as<uint32_t>(), nullptr,
                        rays[cur], rconst,
-                       pix[cur], entry, alive, t_scr, s_pos, s_dir, s_t0);
+                       pix[cur], entry, alive, t_scr, s_pos, ray_sh, s_t0);
     FVC_HIP(hipGetLastError());
-    launch_network_full(m, s_pos, s_dir, n_bound, reinterpret_cast<uint32_t*>(s_out), s, ctl + CTL_R, r.mlp_rows);
+    launch_network_full_pix(m, s_pos, ray_sh, n_bound, reinterpret_cast<uint32_t*>(s_out), s, ctl + CTL_R,
+                            r.mlp_rows);
     hipLaunchKernelGGL(k_composite, grid, dim3(256), 0, s, ctl, cur, rm.stepsize_info, m.scene.alpha_thre, rays[cur],
                        pix[cur], rres[cur], entry, alive, s_out, s_t0, rays[cur ^ 1], pix[cur ^ 1], rres[cur ^ 1],
                        results);
