@@ -12,6 +12,7 @@ reprojected frame.  Every frame is also rendered in full for the comparison.
 `value` = reprojected frames/s.  Prints one JSON line.
 """
 import argparse
+import platform
 import json
 import math
 import os
@@ -38,7 +39,7 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=16, help="rows of the CPU baseline band")
     ap.add_argument("--batch", type=int, default=0, help="target samples per launch (0 = library default)")
     ap.add_argument("--max-steps", type=int, default=0, help="cap on samples per ray per launch")
-    ap.add_argument("--precision", choices=["fp16", "bf16", "fp8"], default="fp16",
+    ap.add_argument("--precision", choices=["fp16", "bf16", "bf16x3", "fp8"], default="fp16",
                     help="MLP arithmetic (operands; FP32 accumulation)")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default launch shapes")
@@ -53,6 +54,11 @@ def parse():
     ap.add_argument("--viewdep", action="store_true",
                     help="the same scene with a view-dependent split model (first head + custom colour head, "
                          "DensityIntermediates cache)")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the precision report (small scene, every MLP / latent format vs the FP64 oracle)")
+    ap.add_argument("--precision-res", type=int, default=256,
+                    help="resolution of the precision report's small-scene frames (the scene's own is 256)")
+    ap.add_argument("--precision-frames", type=int, default=30, help="frames of the precision report's path")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference point for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -74,6 +80,36 @@ def copy_calibration(mib):
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def to_8bit(rgbt):
+    """White background + uchar conversion of a float rgb + T image
+    (applyWhiteBackground + color_to_uchar4, raymarch_common.h:547-566)."""
+    import numpy as np
+    rgb = rgbt[..., :3] + rgbt[..., 3:4]
+    return np.clip((rgb * np.float32(256.0)).astype(np.int32), 0, 255).astype(np.uint8)
+
+
+def psnr_8bit(a, b):
+    """PSNR of two uint8 RGB images (ImageBuffer::computePSNR,
+    image_buffer.cu:43-52, 184-200); inf when they are equal."""
+    import numpy as np
+    d = (a.astype(np.float64) - b.astype(np.float64)) / 255.0
+    mse = np.mean(d * d)
+    return float("inf") if mse == 0 else float(-10.0 * np.log10(mse))
+
+
+def db(v):
+    """A PSNR for the JSON line: None when the images are equal (inf)."""
+    import math
+    return round(v, 2) if math.isfinite(v) else None
+
+
+def mean_db(vals):
+    """Mean PSNR over the frames whose images differ (None if none do)."""
+    import math
+    f = [v for v in vals if math.isfinite(v)]
+    return round(sum(f) / len(f), 2) if f else None
 
 
 def main():
@@ -101,7 +137,8 @@ def main():
     path = S.camera_path_1080p(wl, 30)
     log(f"[bench] workload built in {time.time() - t0:.1f}s")
     from frustumvolumecaching_amd import abi
-    wl.net.precision = {"fp16": abi.PREC_FP16, "bf16": abi.PREC_BF16, "fp8": abi.PREC_FP8}[a.precision]
+    wl.net.precision = {"fp16": abi.PREC_FP16, "bf16": abi.PREC_BF16, "bf16x3": abi.PREC_BF16X3,
+                        "fp8": abi.PREC_FP8}[a.precision]
     model = lib.NerfNetwork.from_workload(wl)
     r = lib.Renderer(model)
     r.set_latent_format(a.cache_latents)
@@ -213,7 +250,6 @@ def main():
     batching = None if a.no_batch else batch_leg(wl, r, path, use_shapes)
 
     # ---- PSNR: cached vs full (GPU) on frames 1, 5, 9 after a fill at frame 0
-    from oracle.pyoracle import psnr_8bit, to_8bit
     psnr_cf = []
     r.initCacheBlockwise(rm_of(0))
     for k in (1, 5, 9):
@@ -225,9 +261,14 @@ def main():
         cache_img = to_8bit(res_buf.cpu().numpy().reshape(H, W, 4))
         psnr_cf.append(psnr_8bit(cache_img, full_img))
 
-    # ---- PSNR vs the FP64 oracle on a window of frame 1 (full render)
-    psnr_fp64 = None
+    # ---- CPU leg: the oracle on the host cores.  (a) the CPU baseline: the
+    # oracle (FP32) on a band of rows of the same workload, timed and
+    # extrapolated to the frame by the samples the GPU evaluated; (b) the
+    # FP64 oracle images the GPU's are checked against (a crop of frame 1
+    # here, the small-scene path of the precision report below).
+    psnr_fp64, cpu = None, None
     import oracle
+    om = None
     if not a.no_psnr:
         om = oracle.Model.from_workload(wl)
         crop = (W // 2 - 48, H // 2 - 32, 96, 64)
@@ -236,14 +277,16 @@ def main():
         g = res_buf.cpu().numpy().reshape(H, W, 4)[crop[1]:crop[1] + crop[3], crop[0]:crop[0] + crop[2]]
         ref64, _ = om.render_full(rm_of(1), oracle.F64, crop=crop)
         psnr_fp64 = psnr_8bit(to_8bit(g), to_8bit(ref64))
-
-    # ---- CPU baseline: the oracle (FP32) on a band of rows, same workload
-    cpu = None
     if not a.no_cpu:
-        om = oracle.Model.from_workload(wl)
+        om = om or oracle.Model.from_workload(wl)
         rows = a.cpu_rows
         band = (0, H // 2 - rows // 2, W, rows)
         threads = oracle.num_threads()
+        # the GPU's sample counts of the same frames (fill at frame 0, frame 1)
+        g_fill = r.initCacheBlockwise(rm_of(0)).n_samples_evaluated
+        stc = r.render_cached(rm_of(1))
+        g_cached = int(stc.n_resamples + stc.n_cache_hits)
+        g_full = int(r.render_full(rm_of(1)).n_resamples)
         t = time.perf_counter()
         oc = om.cache_fill(rm_of(0), oracle.F32, crop=band)
         t_fill = time.perf_counter() - t
@@ -253,15 +296,21 @@ def main():
         t = time.perf_counter()
         _, nf = om.render_full(rm_of(1), oracle.F32, crop=band)
         t_full = time.perf_counter() - t
-        scale = H / rows
-        cpu = {"value": round(1.0 / (t_cached * scale), 6), "unit": "frames/s", "cores": threads, "kind": "port",
+        # CPU time per frame = band time x (frame samples / band samples)
+        s_cached = g_cached / max(1, nr + nc)
+        s_full = g_full / max(1, nf)
+        s_fill = g_fill / max(1, oc.stats.n_samples_evaluated)
+        cpu = {"value": round(1.0 / (t_cached * s_cached), 6), "unit": "frames/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(),
                "sample": f"oracle FP32 fill of frame 0 + reprojected frame 1 on a {W}x{rows} band through the image "
-                         f"centre ({nr + nc} samples), extrapolated x{scale:.1f} to {W}x{H}",
-               "full_render_fps": round(1.0 / (t_full * scale), 6),
-               "fill_ms": round(t_fill * scale * 1e3, 1),
+                         f"centre ({nr + nc} samples of the frame's {g_cached}), extrapolated by samples "
+                         f"(x{s_cached:.1f}) to {W}x{H}",
+               "full_render_fps": round(1.0 / (t_full * s_full), 6),
+               "fill_ms": round(t_fill * s_fill * 1e3, 1),
                "msamples_per_s_cached": round((nr + nc) / t_cached / 1e6, 3),
                "msamples_per_s_full": round(nf / t_full / 1e6, 3)}
         log(f"[bench] cpu baseline: {cpu}")
+    precision = None if a.no_precision else precision_leg(a)
 
     # ---- numbers
     cached_mean = float(np.mean(cached_ms))
@@ -298,7 +347,7 @@ def main():
         for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"]):
             t = v["ms"] * 1e-3
             out[k] = {"ms": round(v["ms"], 3), "launches": v["launches"],
-                      "GB_s": round(v["bytes"] / t / 1e9, 1) if t > 0 else 0.0,
+                      "alg_GB_s": round(v["bytes"] / t / 1e9, 1) if t > 0 else 0.0,
                       "TFLOP_s": round(v["flops"] / t / 1e12, 2) if t > 0 else 0.0}
         return out
 
@@ -334,12 +383,13 @@ def main():
         "cache_hit_fraction": round(hit_frac, 4),
         "fill_bricks": int(fill_stats[0].n_bricks_set) if fill_stats else 0,
         "fill_overflow": int(max((s.overflow for s in fill_stats), default=0)),
-        "psnr_cached_vs_full_db": [round(p, 2) for p in psnr_cf],
-        "psnr_full_vs_fp64_oracle_db": None if psnr_fp64 is None else round(psnr_fp64, 2),
+        "psnr_cached_vs_full_db": [db(p) for p in psnr_cf],
+        "psnr_full_vs_fp64_oracle_db": None if psnr_fp64 is None else db(psnr_fp64),
         "roofline": roofline,
         "kernels_cached": kernel_table(prof_cached),
         "kernels_full": kernel_table(prof_full),
         "cpu_baseline": cpu,
+        "precision": precision,
         "async_refill_fps": async_fps,
         "batching": batching,
         "autotune": tune,
@@ -350,6 +400,103 @@ def main():
         "diff": None if a.no_diff else diff_leg(a),
     }
     print(json.dumps(out), flush=True)
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def precision_leg(a):
+    """The precision report: the small scene (hash grid T=2^19, 8 blobs) at
+    a.precision_res^2 along a a.precision_frames-frame path (yaw 1 deg + 0.02
+    forward per frame, cache refilled every 10 frames), every frame rendered
+    full and reprojected on the GPU with each MLP / latent format, against
+    the FP64 oracle's full render and its cached render (FP64 fill at the
+    same poses).  Per frame: PSNR of the 8-bit images and the max-abs error
+    of the composited colour (rgb + T).  The FP64 frames are the CPU leg's:
+    the oracle on the host cores, timed as such."""
+    import copy
+    import numpy as np
+    import torch
+    import oracle
+    from frustumvolumecaching_amd import abi, lib, synthetic as S
+    R, F = a.precision_res, a.precision_frames
+    wl = S.small_plain(R, R, stepsize_setup=lib.stepsize_setup)
+    path = S.camera_path_1080p(wl, F)
+    rms = [wl.rm(path[k], R, R) for k in range(F)]
+    fills = [k for k in range(F) if k % 10 == 0]
+    # FP64 oracle frames on the host cores
+    om = oracle.Model.from_workload(wl)
+    t = time.perf_counter()
+    ref_full = [om.render_full(rm, oracle.F64)[0] for rm in rms]
+    ref_cached = []
+    for k, rm in enumerate(rms):
+        if k in fills:
+            oc = om.cache_fill(rm, oracle.F64)
+        ref_cached.append(oc.render(rm)[0])
+    t_cpu = time.perf_counter() - t
+    log(f"[bench] precision: FP64 oracle frames in {t_cpu:.1f}s")
+    res = torch.empty(R * R * 4, dtype=torch.float32, device="cuda")
+
+    def frame(r, rm, cached):
+        (r.render_cached if cached else r.render_full)(rm)
+        r.last_result(res, R * R * 4)
+        return res.cpu().numpy().reshape(R, R, 4)
+
+    def err(g, ref):
+        fg, fr = g[..., :3] + g[..., 3:4], ref[..., :3] + ref[..., 3:4]
+        return psnr_8bit(to_8bit(g), to_8bit(ref)), float(np.abs(fg - fr).max())
+
+    variants = [("fp16", abi.PREC_FP16, "fp16"), ("bf16", abi.PREC_BF16, "fp16"), ("fp8", abi.PREC_FP8, "fp16"),
+                ("fp16_fp8latents", abi.PREC_FP16, "fp8")]
+    if hasattr(abi, "PREC_BF16X3"):
+        variants.insert(2, ("bf16x3", abi.PREC_BF16X3, "fp16"))
+    out = {"workload": f"small_plain {R}x{R}: hash grid L16 F2 T2^19, 8 blobs, {F}-frame path yaw 1deg + 0.02 "
+                       f"fwd/frame, refill every 10",
+           "reference": "FP64 oracle (full render; cached render of an FP64 fill at the same poses)",
+           "cpu_fp64_s": round(t_cpu, 1), "variants": {}}
+    for name, prec, lat in variants:
+        net = copy.copy(wl.net)
+        net.precision = prec
+        m = lib.NerfNetwork(wl.scene, net, wl.occupancy, wl.params)
+        r = lib.Renderer(m)
+        r.set_latent_format(lat)
+        pf, mf, pc, mc = [], [], [], []
+        try:
+            for k, rm in enumerate(rms):
+                p, e = err(frame(r, rm, False), ref_full[k])
+                pf.append(p); mf.append(e)
+                if k in fills:
+                    r.initCacheBlockwise(rm)
+                p, e = err(frame(r, rm, True), ref_cached[k])
+                pc.append(p); mc.append(e)
+        finally:
+            r.close()
+            m.close()
+        out["variants"][name] = {"mlp": name.split("_")[0], "latents": lat,
+                                 "full_psnr_db": [db(x) for x in pf], "cached_psnr_db": [db(x) for x in pc],
+                                 "full_maxabs": [round(x, 5) for x in mf], "cached_maxabs": [round(x, 5) for x in mc],
+                                 "full_psnr_mean_db": mean_db(pf), "cached_psnr_mean_db": mean_db(pc),
+                                 "full_maxabs_max": round(max(mf), 5), "cached_maxabs_max": round(max(mc), 5)}
+    base = out["variants"]["fp16"]
+    for name, v in out["variants"].items():
+        ok = True
+        for kind in ("full", "cached"):
+            x, b = v[f"{kind}_psnr_mean_db"], base[f"{kind}_psnr_mean_db"]
+            v[f"{kind}_db_vs_fp16"] = None if x is None or b is None else round(x - b, 2)
+            ok = ok and (v[f"{kind}_db_vs_fp16"] is None or v[f"{kind}_db_vs_fp16"] >= -0.5)
+        v["within_0_5db_of_fp16"] = ok
+    log("[bench] precision: " + json.dumps({k: (v["full_psnr_mean_db"], v["cached_psnr_mean_db"])
+                                             for k, v in out["variants"].items()}))
+    return out
 
 
 def async_refill_leg(a, wl, model, rm_of, img):

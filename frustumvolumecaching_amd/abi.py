@@ -72,7 +72,7 @@ class RaymarchInfo(C.Structure):
     ]
 
 
-PREC_FP16, PREC_BF16, PREC_FP8 = 0, 1, 2
+PREC_FP16, PREC_BF16, PREC_FP8, PREC_BF16X3 = 0, 1, 2, 3
 
 
 class NetDesc(C.Structure):

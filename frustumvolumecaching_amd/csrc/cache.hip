@@ -1427,6 +1427,7 @@ void render_cached(Renderer& r, const Views& vs, fvc_render_stats* stats) {
         switch (m.net.precision) {
           case FVC_PREC_BF16: L(FVC_PREC_BF16); break;
           case FVC_PREC_FP8: L(FVC_PREC_FP8); break;
+          case FVC_PREC_BF16X3: L(FVC_PREC_BF16X3); break;
           default: L(FVC_PREC_FP16); break;
         }
       }

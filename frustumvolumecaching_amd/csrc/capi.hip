@@ -117,7 +117,8 @@ static void create_model(const fvc_scene_info* scene, const fvc_net_desc* net, c
     // PerformanceRenderer only supports the fixed 128^3 grid (performance_renderer.cu:162-163)
     if (scene->grid_resolution != 128) throw Error(FVC_E_UNSUPPORTED, "only grid_resolution 128 is supported");
     REQUIRE(scene->grid_nlvl >= 1 && scene->grid_nlvl <= 16, "grid_nlvl out of range");
-    if (net->precision != FVC_PREC_FP16 && net->precision != FVC_PREC_BF16 && net->precision != FVC_PREC_FP8)
+    if (net->precision != FVC_PREC_FP16 && net->precision != FVC_PREC_BF16 && net->precision != FVC_PREC_FP8 &&
+        net->precision != FVC_PREC_BF16X3)
       throw Error(FVC_E_UNSUPPORTED, "unknown MLP precision");
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) throw Error(FVC_E_NO_DEVICE, "no HIP device");

@@ -1,5 +1,6 @@
 // mlp_mfma.h -- MLP tiles on the 16x16x32 MFMA instructions of gfx950, in
-// FP16, BF16 or FP8-E4M3 operands with FP32 accumulation.
+// FP16, BF16, two-term BF16 (BF16X3) or FP8-E4M3 operands with FP32
+// accumulation.
 //
 // One wave64 evaluates 16 samples per step.  Operands use the transposed
 // formulation D[neuron][sample] = W[neuron][k] * X[k][sample]:
@@ -17,7 +18,12 @@
 // bfloat16 (RNE).  In FP8 mode weights are E4M3 codes of w / s_w with the
 // per-layer scale s_w = absmax(W) / 448, activations codes of a / s_a with
 // the fixed s_a = 64 / 448 (saturating at +-448); each layer's accumulator is
-// rescaled by s_w * s_a.  The weight fragments are staged in LDS per block.
+// rescaled by s_w * s_a.  In BF16X3 mode every operand is the pair (hi, lo)
+// of bfloat16s with hi = bf16(v), lo = bf16(v - hi), and a product is
+// hi*hi + hi*lo + lo*hi on three MFMAs (about 16 significant bits per
+// operand: the fp16 weights and inputs are exact, hidden activations keep
+// more bits than the reference's fp16).  The weight fragments are staged in
+// LDS per block.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -57,6 +63,26 @@ template <> struct Mx<FVC_PREC_BF16> {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
 };
+// two-term bfloat16 operand: v ~ hi + lo
+struct B3 {
+  b8 hi, lo;
+};
+template <> struct Mx<FVC_PREC_BF16X3> {
+  typedef B3 T;
+  static __device__ __forceinline__ T pack(const float* v, float) {
+    typedef float f8v __attribute__((ext_vector_type(8)));
+    const f8v w = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+    const b8 hi = __builtin_convertvector(w, b8);
+    const f8v r = w - __builtin_convertvector(hi, f8v);
+    return T{hi, __builtin_convertvector(r, b8)};
+  }
+  // the small terms first: hi*lo and lo*hi, then hi*hi
+  static __device__ __forceinline__ f4 mma(T a, T b, f4 c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
+  }
+};
 template <> struct Mx<FVC_PREC_FP8> {
   typedef long T;
   // E4M3 codes of v * inv_scale, clamped to the finite range (the hardware
@@ -94,7 +120,8 @@ typedef short s8v __attribute__((ext_vector_type(8)));
 
 // B fragment from two 16-neuron D blocks: ReLU, then operand rounding
 template <int P> __device__ __forceinline__ typename Mx<P>::T act_pack(f4 lo, f4 hi, float inv_act) {
-  if constexpr (P == FVC_PREC_FP8) {
+  if constexpr (P == FVC_PREC_FP8 || P == FVC_PREC_BF16X3) {
+    // ReLU on the float values, then the operand format
     const float v[8] = {relu_bits(lo[0]), relu_bits(lo[1]), relu_bits(lo[2]), relu_bits(lo[3]),
                         relu_bits(hi[0]), relu_bits(hi[1]), relu_bits(hi[2]), relu_bits(hi[3])};
     return Mx<P>::pack(v, inv_act);

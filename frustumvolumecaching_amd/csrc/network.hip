@@ -459,6 +459,7 @@ static int grid_for(int64_t n_groups) {
   switch ((m).net.precision) {                                               \
     case FVC_PREC_BF16: LAUNCH(FVC_PREC_BF16); break;                        \
     case FVC_PREC_FP8: LAUNCH(FVC_PREC_FP8); break;                          \
+    case FVC_PREC_BF16X3: LAUNCH(FVC_PREC_BF16X3); break;                    \
     default: LAUNCH(FVC_PREC_FP16); break;                                   \
   }
 
@@ -606,6 +607,12 @@ static uint16_t bf16_bits(float f) {  // round to nearest even
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
 }
+static float bf16_value(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
 // OCP E4M3 (bias 7, max 448), round to nearest even, saturating
 static uint8_t e4m3_code(float f) {
   const uint8_t sign = std::signbit(f) ? 0x80 : 0x00;
@@ -634,8 +641,9 @@ void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, i
                     std::vector<uint8_t>& out, MlpScales& sc) {
   sc.lat_lo = lat_lo;
   sc.lat_hi = lat_lo + n_lat;
-  const int bytes = precision == FVC_PREC_FP8 ? 1 : 2;
-  out.assign((size_t)N_FRAGS * 64 * 8 * bytes, 0);
+  // bytes per lane and fragment: 8 operands, two terms of 8 in BF16X3
+  const int lane_bytes = precision == FVC_PREC_FP8 ? 8 : (precision == FVC_PREC_BF16X3 ? 32 : 16);
+  out.assign((size_t)N_FRAGS * 64 * lane_bytes, 0);
   const uint16_t* W1 = base;
   const uint16_t* W2 = base + 32 * 64;
   const uint16_t* W3 = base + 32 * 64 + 64 * 64;
@@ -656,12 +664,19 @@ void pack_fragments(const uint16_t* base, const uint16_t* head, int precision, i
       const int i = lane % 16, g = lane / 16;
       for (int e = 0; e < 8; e++) {
         const uint16_t h = W[(size_t)(row0 + i) * n_in + colf(g, e)];
-        const size_t at = ((size_t)frag * 64 + lane) * 8 + e;
+        const size_t base = ((size_t)frag * 64 + lane) * lane_bytes;
         if (precision == FVC_PREC_FP8) {
-          out[at] = e4m3_code(f16_value(h) / wscale[layer]);
+          out[base + e] = e4m3_code(f16_value(h) / wscale[layer]);
+        } else if (precision == FVC_PREC_BF16X3) {
+          // hi = bf16(w), lo = bf16(w - hi): exact for fp16 weights
+          const float w = f16_value(h);
+          const uint16_t hi = bf16_bits(w);
+          const uint16_t lo = bf16_bits(w - bf16_value(hi));
+          std::memcpy(&out[base + 2 * e], &hi, 2);
+          std::memcpy(&out[base + 16 + 2 * e], &lo, 2);
         } else {
           const uint16_t v = precision == FVC_PREC_BF16 ? bf16_bits(f16_value(h)) : h;
-          std::memcpy(&out[at * 2], &v, 2);
+          std::memcpy(&out[base + 2 * e], &v, 2);
         }
       }
     }

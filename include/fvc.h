@@ -122,7 +122,12 @@ typedef struct {
  * first_latent_is_density = 1: latent 0 is log sigma and the cache stores
  * sigma in its place (DensityIntermediates, cache_renderer.cu:481-486),
  * latent_offset = 0; else latent_offset = 1.  FP16 only. */
-enum { FVC_PREC_FP16 = 0, FVC_PREC_BF16 = 1, FVC_PREC_FP8 = 2 };
+/* MLP operand formats (no counterpart in the reference, which runs FP16):
+ * FP16; BF16; FP8-E4M3 with per-layer scales; BF16X3: every operand split
+ * into two bfloat16 terms (hi = bf16(v), lo = bf16(v - hi)) and each product
+ * taken as hi*hi + hi*lo + lo*hi on three BF16 MFMAs (the lo*lo term
+ * dropped; Henry, Tang, Heinecke 2019), FP32 accumulation throughout. */
+enum { FVC_PREC_FP16 = 0, FVC_PREC_BF16 = 1, FVC_PREC_FP8 = 2, FVC_PREC_BF16X3 = 3 };
 typedef struct {
   int32_t n_levels;               /* 16 */
   int32_t n_features_per_level;   /* 2 */

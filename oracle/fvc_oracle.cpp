@@ -13,6 +13,10 @@
 //   mode 3  EMU_BF16 as EMU16, MLP activations/weights rounded to bfloat16
 //   mode 4  EMU_FP8  as EMU16, MLP weights + activations in FP8-E4M3 with a
 //                    per-layer absmax scale (FP32 accumulate)
+//   mode 5  EMU_BF16X3 as EMU16 for stored values; MLP weights and
+//                    activations as two bfloat16 terms hi = bf16(v),
+//                    lo = bf16(v - hi), each product hi*hi + hi*lo + lo*hi
+//                    (FP32 accumulate), hidden activations kept in float
 // Every function cites the reference file:line it follows (paths relative
 // to the reference repository root).
 //
@@ -43,7 +47,7 @@
 
 namespace orc {
 
-enum Mode { M_F64 = 0, M_F32 = 1, M_EMU16 = 2, M_EMU_BF16 = 3, M_EMU_FP8 = 4 };
+enum Mode { M_F64 = 0, M_F32 = 1, M_EMU16 = 2, M_EMU_BF16 = 3, M_EMU_FP8 = 4, M_EMU_BF16X3 = 5 };
 
 // ---------------------------------------------------------------------------
 // Deterministic float log / exp2 (range reduction + series, no FMA).
@@ -466,6 +470,15 @@ void layer(const Net& n, int li, const std::vector<float>& W, int n_out, int n_i
   for (int j = 0; j < n_out; j++) {
     R acc = 0;
     const float* w = &W[(size_t)j * n_in];
+    if (m == M_EMU_BF16X3) {
+      // two-term bfloat16 operands; the lo*lo product is dropped
+      for (int k = 0; k < n_in; k++) {
+        const float wh = round_bf16(w[k]), wl = round_bf16(w[k] - wh);
+        const float xf = (float)x[k];
+        const float xh = round_bf16(xf), xl = round_bf16(xf - xh);
+        acc += (R)(wh * xl + wl * xh) + (R)(wh * xh);
+      }
+    } else
     for (int k = 0; k < n_in; k++) {
       R wv = (R)w[k];
       if (m == M_EMU_BF16) wv = (R)round_bf16((float)wv);

@@ -10,7 +10,7 @@ from frustumvolumecaching_amd import abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-F64, F32, EMU16, EMU_BF16, EMU_FP8 = 0, 1, 2, 3, 4
+F64, F32, EMU16, EMU_BF16, EMU_FP8, EMU_BF16X3 = 0, 1, 2, 3, 4, 5
 
 _lib = None
 

@@ -1,12 +1,16 @@
-"""BF16 and FP8-E4M3 MLP variants against the oracle's emulated BF16 / FP8
-modes (same operand rounding and per-layer scales, FP32 accumulation in a
-different order), through the C ABI.
+"""BF16, two-term BF16 (BF16X3) and FP8-E4M3 MLP variants against the
+oracle's emulation of the same scheme (same operand rounding / splitting
+and per-layer scales, FP32 accumulation in a different order), through the
+C ABI.
 
 Bars (measured error x ~5-10 headroom):
-* network outputs: BF16 max 2e-3, 99.9th percentile 2e-4; FP8 max 5e-2,
-  99.9th percentile 1e-2 (absolute, outputs of order 1);
+* network outputs: BF16 and BF16X3 max 2e-3, 99.9th percentile 2e-4; FP8
+  max 5e-2, 99.9th percentile 1e-2 (absolute, outputs of order 1);
 * images within 2/255 of the same-mode oracle, full and cached renders;
-* PSNR vs the FP64 oracle stated per precision (FP16 >= BF16 >= FP8).
+* PSNR vs the FP64 oracle: FP16 above 60 dB; BF16X3 within 0.5 dB of FP16
+  (the precision target); BF16 above 55 dB and FP8 above 45 dB, their
+  measured floors with margin: the 0.5 dB target is not met by them (8 and
+  4 mantissa bits against FP16's 11; the xfail test states it).
 """
 import copy
 
@@ -21,7 +25,8 @@ from .gpu_util import dev, empty, host_f16, to_8bit
 pytestmark = pytest.mark.gpu
 
 RES = 96
-PRECS = {"bf16": (abi.PREC_BF16, 2e-3, 2e-4), "fp8": (abi.PREC_FP8, 5e-2, 1e-2)}
+PRECS = {"bf16": (abi.PREC_BF16, 2e-3, 2e-4), "bf16x3": (abi.PREC_BF16X3, 2e-3, 2e-4),
+         "fp8": (abi.PREC_FP8, 5e-2, 1e-2)}
 
 
 @pytest.fixture(scope="module")
@@ -41,7 +46,7 @@ def _model(wl, prec):
 
 
 def _mode(oracle_mod, name):
-    return {"bf16": oracle_mod.EMU_BF16, "fp8": oracle_mod.EMU_FP8}[name]
+    return {"bf16": oracle_mod.EMU_BF16, "bf16x3": oracle_mod.EMU_BF16X3, "fp8": oracle_mod.EMU_FP8}[name]
 
 
 def _close(got, ref, tmax, tq):
@@ -51,7 +56,7 @@ def _close(got, ref, tmax, tq):
     assert np.quantile(err, 0.999) <= tq * scale, np.quantile(err, 0.999)
 
 
-@pytest.mark.parametrize("name", ["bf16", "fp8"])
+@pytest.mark.parametrize("name", ["bf16", "bf16x3", "fp8"])
 def test_network_outputs_match_oracle(wl, omodel, oracle_mod, name):
     prec, tmax, tq = PRECS[name]
     g = _model(wl, prec)
@@ -85,7 +90,7 @@ def _render(r, rm, cached=False):
     return res.cpu().numpy().reshape(rm.cam_info.res_y, rm.cam_info.res_x, 4)
 
 
-@pytest.mark.parametrize("name", ["bf16", "fp8"])
+@pytest.mark.parametrize("name", ["bf16", "bf16x3", "fp8"])
 def test_renders_match_oracle(wl, omodel, oracle_mod, name):
     prec = PRECS[name][0]
     mode = _mode(oracle_mod, name)
@@ -106,16 +111,29 @@ def test_renders_match_oracle(wl, omodel, oracle_mod, name):
     g.close()
 
 
-def test_psnr_vs_fp64_per_precision(wl, omodel, oracle_mod):
+@pytest.fixture(scope="module")
+def psnr(wl, omodel, oracle_mod):
     rm = wl.rm(width=RES, height=RES)
     ref64, _ = omodel.render_full(rm, oracle_mod.F64)
-    psnr = {}
-    for name, prec in [("fp16", abi.PREC_FP16), ("bf16", abi.PREC_BF16), ("fp8", abi.PREC_FP8)]:
+    out = {}
+    for name, prec in [("fp16", abi.PREC_FP16), ("bf16", abi.PREC_BF16), ("bf16x3", abi.PREC_BF16X3),
+                       ("fp8", abi.PREC_FP8)]:
         g = _model(wl, prec)
         r = lib.Renderer(g)
-        psnr[name] = oracle_mod.psnr_8bit(to_8bit(_render(r, rm)), to_8bit(ref64))
+        out[name] = oracle_mod.psnr_8bit(to_8bit(_render(r, rm)), to_8bit(ref64))
         r.close()
         g.close()
-    print("PSNR vs FP64 oracle (dB):", {k: round(v, 2) for k, v in psnr.items()})
-    assert psnr["fp16"] > 60 and psnr["bf16"] > 55 and psnr["fp8"] > 45
-    assert psnr["fp16"] >= psnr["bf16"] - 0.5 >= psnr["fp8"] - 1.0
+    print("PSNR vs FP64 oracle (dB):", {k: round(v, 2) for k, v in out.items()})
+    return out
+
+
+def test_psnr_vs_fp64_per_precision(psnr):
+    assert psnr["fp16"] > 60
+    assert psnr["bf16x3"] >= psnr["fp16"] - 0.5
+    assert psnr["bf16"] > 55 and psnr["fp8"] > 45
+
+
+@pytest.mark.xfail(strict=True, reason="plain BF16 (8 mantissa bits) and FP8-E4M3 (4) MLP operands lose more than "
+                                       "0.5 dB against FP16 (11); the two-term BF16X3 split meets the target")
+def test_bf16_and_fp8_within_half_db_of_fp16(psnr):
+    assert psnr["bf16"] >= psnr["fp16"] - 0.5 and psnr["fp8"] >= psnr["fp16"] - 0.5

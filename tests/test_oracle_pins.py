@@ -505,3 +505,43 @@ def test_cached_render_at_fill_pose_matches_full_render(O, filled):
     cached, nr, nc = cache.render(rm, crop=inner)
     assert nc > 10 * max(nr, 1)
     assert O.psnr_8bit(O.to_8bit(cached), O.to_8bit(full)) > 40.0
+
+
+def _bf16(a):
+    """Round to bfloat16 (nearest even) from the float32 value."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7fff + ((u >> 16) & 1)) & 0xffff0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _split_layer(W, X):
+    """Two-term bfloat16 product of W [out, in] and X [in, n]: hi*hi + hi*lo +
+    lo*hi with hi = bf16(v), lo = bf16(v - hi) (float64 sums)."""
+    wh = _bf16(W)
+    wl = _bf16(W - wh)
+    xh = _bf16(X)
+    xl = _bf16(X.astype(np.float32).astype(np.float64) - xh)
+    return wh @ xh + wh @ xl + wl @ xh
+
+
+def test_bf16x3_mode_is_the_two_term_split(O):
+    """EMU_BF16X3: each MLP product is the two-term bfloat16 split (lo*lo
+    dropped), hidden activations unrounded, outputs stored as fp16.  The
+    oracle's float accumulation agrees with the float64 split to one fp16 ulp
+    of the output, and its error against F64 is far below plain BF16's (a
+    split reduced to hi*hi is plain BF16)."""
+    wl = S.small_plain()
+    model = O.Model.from_workload(wl)
+    W1, W2, W3, _, _ = _weights(wl)
+    pts = np.random.default_rng(18).random((256, 3)).astype(np.float32)
+    enc = model.encode(pts, mode=O.EMU16)
+    h1 = np.maximum(_split_layer(W1, enc.T), 0)
+    h2 = np.maximum(_split_layer(W2, h1), 0)
+    ref = _split_layer(W3, h2).T.astype(np.float16).astype(np.float64)
+    got = model.latents(pts, mode=O.EMU_BF16X3)
+    ulp = 2.0 ** (np.floor(np.log2(np.maximum(np.abs(ref), 2.0 ** -14))) - 10)
+    assert (np.abs(got - ref) / ulp).max() <= 1.0
+    f64 = model.latents(pts, mode=O.F64)
+    e_x3 = np.abs(got - f64).mean()
+    e_bf16 = np.abs(model.latents(pts, mode=O.EMU_BF16) - f64).mean()
+    assert e_x3 < 0.1 * e_bf16, (e_x3, e_bf16)
