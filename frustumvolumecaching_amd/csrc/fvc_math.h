@@ -98,7 +98,8 @@ FVC_HD float step_from_t(const fvc_stepsize_info& s, float t) {
 }
 // calculate_stepsize (raymarch_common.h:274-284)
 FVC_HD float step_dt(const fvc_stepsize_info& s, float t) {
-  return fminf_(fmaxf_(t * s.cone_angle, s.dt_min), s.dt_max);
+  // the operands are positive and finite: the native minimum / maximum
+  return fminf(fmaxf(t * s.cone_angle, s.dt_min), s.dt_max);
 }
 // valid_t_from_t (raymarch_common.h:286-296)
 FVC_HD float valid_t_from_t(const fvc_stepsize_info& s, float t) {
@@ -124,6 +125,8 @@ struct OccParams {
   V3 unit_inv;    // 1 / unit_ext, exact when pow2
   V3 box_c;       // world centre of the cascades (unit coordinate 0.5)
   V3 box_e;       // world extent of cascade 0 (unit_ext, or 1 when normalized)
+  float be_max;   // largest box_e component
+  float bc_max;   // largest |box_c| component
   int nlvl;
   int normalized;
   int pow2;       // every unit_ext component is a power of two
@@ -237,6 +240,8 @@ inline OccParams occ_params(const fvc_scene_info& sc) {
   o.normalized = sc.normalized;
   o.box_e = sc.normalized ? V3{1.0f, 1.0f, 1.0f} : o.unit_ext;
   o.box_c = sc.normalized ? V3{0.5f, 0.5f, 0.5f} : o.unit_off + 0.5f * o.unit_ext;
+  o.be_max = std::max(std::max(o.box_e.x, o.box_e.y), o.box_e.z);
+  o.bc_max = std::max(std::max(std::fabs(o.box_c.x), std::fabs(o.box_c.y)), std::fabs(o.box_c.z));
   return o;
 }
 

@@ -80,27 +80,24 @@ FVC_HD V3 occ_unit(V3 p, const OccParams& o) {
 }
 
 // Per-ray terms of box_skip: 1/d with +-1e30 for (near) axis-parallel
-// directions (the slab test then needs no special case), o/d, |1/d|, the
-// sign of d and the largest |o| component.
+// directions (the slab test then needs no special case), o/d, |1/d| and the
+// largest |o| component.
 struct RaySlab {
   V3 inv, oinv, ainv;
   float omax;
-  int neg;   // bit a set: d_a < 0
 };
 FVC_HD RaySlab ray_slab(V3 o, V3 d) {
   RaySlab r;
   const float dd[3] = {d.x, d.y, d.z}, oo[3] = {o.x, o.y, o.z};
   float iv[3];
-  r.neg = 0;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     iv[a] = fabsf(dd[a]) < 1e-30f ? (dd[a] < 0.0f ? -1e30f : 1e30f) : 1.0f / dd[a];
-    r.neg |= (iv[a] < 0.0f ? 1 : 0) << a;
   }
   r.inv = V3{iv[0], iv[1], iv[2]};
   r.oinv = V3{oo[0] * iv[0], oo[1] * iv[1], oo[2] * iv[2]};
   r.ainv = V3{fabsf(iv[0]), fabsf(iv[1]), fabsf(iv[2])};
-  r.omax = fmaxf_(fmaxf_(fabsf(oo[0]), fabsf(oo[1])), fabsf(oo[2]));
+  r.omax = fmaxf(fmaxf(fabsf(oo[0]), fabsf(oo[1])), fabsf(oo[2]));
   return r;
 }
 
@@ -125,25 +122,26 @@ FVC_HD Skip box_skip(int cx, int cy, int cz, int g, int mip, const OccParams& o,
   const float iv[3] = {rs.inv.x, rs.inv.y, rs.inv.z}, oi[3] = {rs.oinv.x, rs.oinv.y, rs.oinv.z};
   const float ai[3] = {rs.ainv.x, rs.ainv.y, rs.ainv.z};
   float tn[3], tf[3];
-  float cmag = 0.0f;
+  float tout0 = 3.0e38f;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     const float w = be[a] * s;
     const float lo = fmaf((float)(ci[a] - 64), w, bc[a]);
     const float hi = fmaf(gs, w, lo);
     const float ta = fmaf(lo, iv[a], -oi[a]), tb = fmaf(hi, iv[a], -oi[a]);
-    const bool neg = (rs.neg >> a) & 1;
-    tn[a] = neg ? tb : ta;
-    tf[a] = neg ? ta : tb;
-    cmag = fmaxf_(cmag, fmaf(64.0f, w, fabsf(bc[a])));
+    // entry and exit along the axis (the values are finite: |1/d| <= 1e30)
+    tn[a] = fminf(ta, tb);
+    tf[a] = fmaxf(ta, tb);
+    tout0 = fminf(tout0, tf[a]);
   }
-  const float tout0 = fminf_(fminf_(tf[0], tf[1]), tf[2]);
+  // the cascade's largest coordinate, bounded by 64 w_max + |box_c|_max
+  const float cmag = fmaf(64.0f * s, o.be_max, o.bc_max);
   const float m = 4e-5f * (rs.omax + fabsf(tout0) + cmag);
   float tin = -3.0e38f, tout = 3.0e38f;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    tin = fmaxf_(tin, fmaf(m, ai[a], tn[a]));
-    tout = fminf_(tout, fmaf(-m, ai[a], tf[a]));
+    tin = fmaxf(tin, fmaf(m, ai[a], tn[a]));
+    tout = fminf(tout, fmaf(-m, ai[a], tf[a]));
   }
   return Skip{tin, tout};
 }
@@ -155,7 +153,7 @@ FVC_HD bool occ_query(V3 p, const OccParams& o, const uint64_t* __restrict__ bf6
                       const RaySlab& rs, Skip& sk) {
   const V3 u = occ_unit(p, o);
   const float ax = fabsf(u.x - 0.5f), ay = fabsf(u.y - 0.5f), az = fabsf(u.z - 0.5f);
-  const float mx = fmaxf_(fmaxf_(ax, ay), az);
+  const float mx = fmaxf(fmaxf(ax, ay), az);
   int e;
   frexpf(mx, &e);
   const int mip = e + 1 > 0 ? e + 1 : 0;
