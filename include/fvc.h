@@ -339,7 +339,7 @@ int fvc_render_full(fvc_renderer* renderer, const fvc_raymarch_info* rm, uint8_t
  * memory (code under HIP graph capture, serving loops, harnesses that forbid
  * hipMalloc): fvc_render_full for views of at most 256 x 256 pixels, with
  * every per-frame device buffer (ray state, sample streams, counters) in one
- * static device array of the library (about 33 MB, held by each GPU the
+ * static device array of the library (about 30 MB, held by each GPU the
  * library runs on until the process ends) instead of the renderer's
  * buffers; no device or pinned memory is allocated.  rgba_dev (uchar4, may
  * be NULL) as fvc_render_full; result_dev (float4 rgb + T per pixel, may be
