@@ -31,15 +31,15 @@ FVC_HD float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 FVC_HD V3 cvt(fvc_float3 f) { return {f.x, f.y, f.z}; }
 // helper_math normalize (helper_math.h:1326-1330), invLen = 1/sqrt(.)
 FVC_HD V3 normalize(V3 v) { const float inv = 1.0f / sqrtf(dot3(v, v)); return v * inv; }
-// a / b correctly rounded for b > 0, given y = 1 / b correctly rounded (one
-// IEEE division shared by every numerator over the same b): q = a*y lies
-// within an ulp of a/b, r = a - b*q is exact, and q + r*y rounds to a/b
+// a / b correctly rounded, given y = 1 / b correctly rounded (one IEEE
+// division shared by every numerator over the same b): q = a*y lies within
+// an ulp of a/b, r = a - b*q is exact, and q + r*y rounds to a/b
 // (Markstein's theorem), for normal b and quotients that neither overflow nor
-// underflow; a zero quotient keeps a's sign, as a / b does.
+// underflow; a zero quotient takes the sign of a*y, as a / b does.
 FVC_HD float div_recip(float a, float b, float y) {
   const float q = a * y;
   const float r = fmaf(-q, b, a);
-  return copysignf(fmaf(r, y, q), a);
+  return copysignf(fmaf(r, y, q), q);
 }
 FVC_HD float fminf_(float a, float b) { return a < b ? a : b; }
 FVC_HD float fmaxf_(float a, float b) { return a > b ? a : b; }

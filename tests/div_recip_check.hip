@@ -1,6 +1,9 @@
 // Host check of fvc::div_recip (fvc_math.h) against IEEE a / b: every b in
-// [2^-8, 2] with 8 random numerators each (both signs, magnitudes 2^-40 ..
-// 2^8), plus signed zeros.  Prints the counts; exits 1 on any mismatch.
+// +-[2^-8, 2] with 8 random numerators each (both signs, magnitudes 2^-40 ..
+// 2^8), plus signed zeros; then 4 M random divisors of either sign with
+// magnitudes 2^-60 .. 2^60 and numerators within 2^-30 .. 2^30 of them (the
+// reprojection divides camera coordinates by their depth and by the
+// stepsize constants).  Prints the counts; exits 1 on any mismatch.
 #include <cstdio>
 #include <cstring>
 
@@ -12,20 +15,29 @@ static uint32_t u_of(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
 int main() {
   uint64_t s = 88172645463325252ull, n = 0, bad = 0;
   auto rnd = [&] { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return (uint32_t)(s >> 16); };
-  for (uint32_t bu = u_of(0.00390625f); bu <= u_of(2.0f); bu++) {
-    const float b = f_of(bu);
-    volatile float one = 1.0f;
+  volatile float one = 1.0f;
+  for (uint32_t bsign = 0; bsign < 2; bsign++)
+    for (uint32_t bu = u_of(0.00390625f); bu <= u_of(2.0f); bu++) {
+      const float b = f_of(bu | (bsign << 31));
+      const float y = one / b;
+      for (int k = 0; k < 8; k++) {
+        const uint32_t e = 87 + rnd() % 48, m = rnd() & 0x7fffffu, sg = rnd() & 1u;
+        const float a = f_of((sg << 31) | (e << 23) | m);
+        n++;
+        if (u_of(a / b) != u_of(fvc::div_recip(a, b, y))) bad++;
+      }
+      for (float z : {0.0f, -0.0f}) {
+        n++;
+        if (u_of(z / b) != u_of(fvc::div_recip(z, b, y))) bad++;
+      }
+    }
+  for (int k = 0; k < (1 << 22); k++) {
+    const uint32_t eb = 67 + rnd() % 121, ea = eb - 30 + rnd() % 61;
+    const float b = f_of(((rnd() & 1u) << 31) | (eb << 23) | (rnd() & 0x7fffffu));
+    const float a = f_of(((rnd() & 1u) << 31) | (ea << 23) | (rnd() & 0x7fffffu));
     const float y = one / b;
-    for (int k = 0; k < 8; k++) {
-      const uint32_t e = 87 + rnd() % 48, m = rnd() & 0x7fffffu, sg = rnd() & 1u;
-      const float a = f_of((sg << 31) | (e << 23) | m);
-      n++;
-      if (u_of(a / b) != u_of(fvc::div_recip(a, b, y))) bad++;
-    }
-    for (float z : {0.0f, -0.0f}) {
-      n++;
-      if (u_of(z / b) != u_of(fvc::div_recip(z, b, y))) bad++;
-    }
+    n++;
+    if (u_of(a / b) != u_of(fvc::div_recip(a, b, y))) bad++;
   }
   std::printf("%llu cases, %llu mismatches\n", (unsigned long long)n, (unsigned long long)bad);
   return bad != 0;

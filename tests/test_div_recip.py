@@ -1,9 +1,12 @@
 """The cache heads divide the 8 trilinear latent sums by one weight sum with
 fvc::div_recip (one IEEE reciprocal, then Markstein's correction per
 numerator) instead of 8 IEEE divisions (cache_performance_renderer.cu:770-787
-divides each sum).  The product header is built for the host and checked
-against a / b bit for bit over every divisor in [2^-8, 2] (the weight sums
-lie in [1/8, 1]) with random numerators and signed zeros."""
+divides each sum), and the reprojection divides camera coordinates by their
+depth and by the stepsize constants the same way.  The product header is
+built for the host and checked against a / b bit for bit over every divisor
+in +-[2^-8, 2] (the weight sums lie in [1/8, 1]) with random numerators and
+signed zeros, and over 4 M random divisors of either sign in 2^-60 .. 2^60
+with quotients in 2^-30 .. 2^30."""
 import os
 import shutil
 import subprocess
