@@ -204,7 +204,8 @@ def main():
             prof_shapes = (int(sh[1]), int(sh[3]), int(sh[5]))
 
     def cached_path(profiled):
-        """The 30-frame path: a fill every 10 frames, a reprojected frame each."""
+        """The first a.steps frames of the path: a fill every 10 frames, a
+        reprojected frame each."""
         fill_ms, cached_ms, fill_stats, cached_stats = [], [], [], []
         use_shapes("cached")
         if profiled and prof_shapes:
@@ -366,7 +367,8 @@ def main():
         "cache_latents": a.cache_latents,
         "data": "synthetic (seeded scene and random-init weights; no dataset)",
         "config": {"workload": f"unbounded_{W}x{H}: hash grid L16 F2 T2^21, 5x128^3 occupancy cascades, 64 blobs "
-                               f"to r=16, cone 1/256; 30-frame path yaw 1deg + 0.02 fwd/frame, refill every 10",
+                               f"to r=16, cone 1/256; first {a.steps} frames of the 30-frame path (yaw 1deg + 0.02 fwd "
+                               f"per frame), refill every 10",
                    "resolution": [W, H], "frames": a.steps, "refill_every": 10,
                    "model": "split head: first head 32-64-8, colour head 64-64-3 with SH differences and position "
                             "frequencies, DensityIntermediates cache" if a.viewdep else

@@ -400,7 +400,7 @@ int fvc_render_cached_auto(fvc_renderer* renderer, const fvc_raymarch_info* rm, 
 /* CacheRenderer::swapCache (cache_renderer.cu:1053-1057); a background fill
  * still running is waited for and published instead. */
 int fvc_cache_swap(fvc_renderer* renderer);
-/* CacheRenderer::setCacheSettings (cache_renderer.h:65) */
+/* CacheRenderer::setCacheSettings (cache_renderer.h:66) */
 int fvc_cache_set_settings(fvc_renderer* renderer, const fvc_cache_settings* settings);
 /* CachePerformanceRenderer::RenderMode (cache_performance_renderer.h:23-28),
  * the public _render_mode member (:121): ONLY_CACHE drops the samples that
