@@ -49,6 +49,7 @@ static void check_space(const fvc_tune_space& sp) {
 
 void autotune(Renderer& r, const Views& vs, bool cached, const fvc_tune_space* space, fvc_tune_result* out) {
   if (cached && !r.caches[r.render_idx].initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
+  if (cached && vs.scenes.size() > 1) throw Error(FVC_E_UNSUPPORTED, "a cached batch renders the scene of the renderer's cache");
   const fvc_tune_space sp = space ? *space : default_space();
   check_space(sp);
   const int rows0 = r.mlp_rows, steps0 = r.max_steps;
@@ -84,27 +85,35 @@ void autotune(Renderer& r, const Views& vs, bool cached, const fvc_tune_space* s
     }
     return (float)best_ms;
   };
-  res.ms_default = time_one(rows0, target0, steps0);
   float best = 3.0e38f;
   int k = 0;
-  for (int a = 0; a < sp.n_rows; a++)
-    for (int b = 0; b < sp.n_targets; b++)
-      for (int c = 0; c < sp.n_steps; c++) {
-        const int rows = sp.rows[a], steps = sp.steps[c];
-        const int64_t target = sp.targets[b];
-        const float ms = time_one(rows, target, steps);
-        res.ms[k] = ms;
-        res.rows[k] = rows;
-        res.steps[k] = steps;
-        res.targets[k] = target;
-        k++;
-        if (ms < best) {
-          best = ms;
-          res.mlp_rows = rows;
-          res.target_batch = target;
-          res.max_steps = steps;
+  try {
+    res.ms_default = time_one(rows0, target0, steps0);
+    for (int a = 0; a < sp.n_rows; a++)
+      for (int b = 0; b < sp.n_targets; b++)
+        for (int c = 0; c < sp.n_steps; c++) {
+          const int rows = sp.rows[a], steps = sp.steps[c];
+          const int64_t target = sp.targets[b];
+          const float ms = time_one(rows, target, steps);
+          res.ms[k] = ms;
+          res.rows[k] = rows;
+          res.steps[k] = steps;
+          res.targets[k] = target;
+          k++;
+          if (ms < best) {
+            best = ms;
+            res.mlp_rows = rows;
+            res.target_batch = target;
+            res.max_steps = steps;
+          }
         }
-      }
+  } catch (...) {
+    // the renderer keeps the shapes it had before the sweep
+    r.mlp_rows = rows0;
+    r.target_batch = target0;
+    r.max_steps = steps0;
+    throw;
+  }
   res.n_tried = k;
   res.ms_best = best;
   shape(res.mlp_rows, res.target_batch, res.max_steps);
