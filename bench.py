@@ -12,6 +12,7 @@ reprojected frame.  Every frame is also rendered in full for the comparison.
 `value` = reprojected frames/s.  Prints one JSON line.
 """
 import argparse
+import gc
 import platform
 import json
 import math
@@ -168,6 +169,12 @@ def main():
         r.render_full(rm_of(1), img)
     torch.cuda.synchronize()
     log(f"[bench] warm-up done ({a.warmup} rounds)")
+    # a full collector pass over torch's object graph takes tens of ms: one
+    # landing inside a timed frame would be counted as that frame's time, so
+    # collect once here and keep the collector off for the timed legs
+    gc.collect()
+    gc.freeze()
+    gc.disable()
     tune = tune_full = None
     shapes = {"cached": None, "full": None}
     if not a.no_tune and not (a.batch or a.max_steps or a.mlp_rows):
@@ -240,7 +247,8 @@ def main():
     t_region = time.perf_counter()
     fill_ms, cached_ms, fill_stats, cached_stats, _ = cached_path(False)
     region_s = time.perf_counter() - t_region
-    log(f"[bench] cached path: {a.steps} frames, {len(fill_ms)} fills, {region_s:.2f}s")
+    log(f"[bench] cached path: {a.steps} frames, {len(fill_ms)} fills, {region_s:.2f}s; frame ms "
+        f"{[round(x, 1) for x in cached_ms]}; fill ms {[round(x, 1) for x in fill_ms]}")
     full_ms, full_stats, _ = full_path(False)
     log(f"[bench] full renders done")
     # ---- the same frames again with per-kernel HIP events (kernel tables, roofline)
@@ -528,16 +536,19 @@ def async_refill_leg(a, wl, model, rm_of, img):
     r2.swapCache()
     r2.render_cached(rm_of(1), img)
     torch.cuda.synchronize()
-    frame_ms = []
+    frame_ms, events = [], []
     t = time.perf_counter()
     for k in range(a.steps):
         tf = time.perf_counter()
         if k % 10 == 0 and k > 0:
             r2.initCacheAsync(rm_of(k))
+            events.append(f"fill@{k}")
         if r2.fill_poll()[0] == abi.FILL_DONE:
             r2.publish()
-        r2.render_cached(rm_of(k), img)
+            events.append(f"publish@{k}:{(time.perf_counter() - tf) * 1e3:.1f}ms")
+        st2 = r2.render_cached(rm_of(k), img)
         frame_ms.append((time.perf_counter() - tf) * 1e3)
+        events.append(st2.n_subiters)
     r2.fill_poll(wait=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t
@@ -545,7 +556,7 @@ def async_refill_leg(a, wl, model, rm_of, img):
     fps = a.steps / dt
     out = {"fps": round(fps, 3), "frame_ms_median": round(sorted(frame_ms)[len(frame_ms) // 2], 3),
            "frame_ms_max": round(max(frame_ms), 3)}
-    log(f"[bench] background refill: {out}")
+    log(f"[bench] background refill: {out}; frame ms {[round(x, 1) for x in frame_ms]}; {events}")
     return out
 
 

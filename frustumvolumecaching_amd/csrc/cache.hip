@@ -566,6 +566,20 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
   C.pool.ensure(std::max<int64_t>(1, (int64_t)C.capacity) * BRICK_VOXELS *
                 (C.lat8 ? sizeof(Voxel8) : sizeof(Voxel)));
   C.scales.ensure(std::max<int64_t>(1, (int64_t)C.capacity) * sizeof(float));
+  // a double-buffered renderer's other cache, while it holds nothing, gets
+  // buffers of this cache's sizes now: the first background fill into it
+  // (a view of about this size) then makes no hipMalloc, which would stall
+  // the frames rendering meanwhile
+  if (r.double_buffered) {
+    CacheStore& O = r.caches[r.fill_idx ^ 1];
+    if (&O != &C && !O.initialized) {
+      O.table.ensure(n_table * sizeof(int32_t));
+      O.known.ensure((int64_t)W * H * sizeof(float));
+      O.pool.ensure(std::max<int64_t>(1, (int64_t)C.capacity) * BRICK_VOXELS *
+                    (C.lat8 ? sizeof(Voxel8) : sizeof(Voxel)));
+      O.scales.ensure(std::max<int64_t>(1, (int64_t)C.capacity) * sizeof(float));
+    }
+  }
 
   const int n_tiles = C.bx * C.by;
   const int64_t f_cap = (int64_t)n_tiles * 512;
