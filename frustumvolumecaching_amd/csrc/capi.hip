@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "fvc_internal.h"
 #include "march.h"
@@ -333,6 +334,7 @@ void fvc_renderer_destroy(fvc_renderer* r) { delete r; }
 int fvc_renderer_set_stream(fvc_renderer* h, void* stream) {
   return guard([&] {
     REQUIRE(h, "null argument");
+    std::scoped_lock lk(h->r.read_mutex);
     h->r.user_stream = (hipStream_t)stream;
   });
 }
@@ -343,6 +345,8 @@ int fvc_renderer_set_batching(fvc_renderer* h, int64_t target_samples, int32_t m
     REQUIRE(target_samples >= 4096 && target_samples <= (int64_t)1 << 28, "target_samples out of range");
     REQUIRE(max_samples_per_ray >= 1 && max_samples_per_ray <= MAX_STEPS_PER_LAUNCH,
             "max_samples_per_ray must be in [1, 64]");
+    // the render buffers may grow: not while a render uses them
+    std::scoped_lock lk(h->r.read_mutex);
     h->r.target_batch = target_samples;
     h->r.max_steps = max_samples_per_ray;
     if (h->r.width > 0) resize_renderbuffers(h->r, h->r.width, h->r.height);
@@ -384,6 +388,7 @@ int fvc_renderer_set_mlp_rows(fvc_renderer* h, int32_t rows) {
   return guard([&] {
     REQUIRE(h, "null renderer");
     REQUIRE(rows == 16 || rows == 32 || rows == 64, "rows must be 16, 32 or 64");
+    std::scoped_lock lk(h->r.read_mutex);
     h->r.mlp_rows = rows;
   });
 }
@@ -391,6 +396,7 @@ int fvc_renderer_set_mlp_rows(fvc_renderer* h, int32_t rows) {
 int fvc_resize_renderbuffers(fvc_renderer* h, int w, int hh) {
   return guard([&] {
     REQUIRE(h, "null argument");
+    std::scoped_lock lk(h->r.read_mutex);
     resize_renderbuffers(h->r, w, hh);
   });
 }
@@ -638,6 +644,7 @@ int fvc_debug_render_cached(fvc_renderer* h, const fvc_raymarch_info* rm, int32_
 }
 
 static CacheStore& pick(fvc_renderer* h, int which);
+static std::unique_lock<std::mutex> cache_lock(fvc_renderer* h, int which);
 
 int fvc_cache_set_latent_format(fvc_renderer* h, int format) {
   return guard([&] {
@@ -663,6 +670,7 @@ static float e4m3_value(uint8_t c) {
 int fvc_cache_get_pool_decoded(fvc_renderer* h, int which, float* out, int64_t n) {
   return guard([&] {
     REQUIRE(h && out, "null argument");
+    const auto lk = cache_lock(h, which);
     const CacheStore& C = pick(h, which);
     if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
     const int64_t nv = (int64_t)C.n_used * BRICK_VOXELS;
@@ -705,6 +713,7 @@ int fvc_cache_set_capacity(fvc_renderer* h, int32_t cap) {
   return guard([&] {
     REQUIRE(h && cap >= 0, "invalid argument");
     REQUIRE(cap <= MAX_BRICKS, "capacity above 2^23 - 1 bricks");
+    std::scoped_lock lk(h->r.write_mutex);
     h->r.capacity_override = cap;
   });
 }
@@ -712,6 +721,7 @@ int fvc_cache_set_capacity(fvc_renderer* h, int32_t cap) {
 int fvc_get_last_result(fvc_renderer* h, float* out, int64_t n) {
   return guard([&] {
     REQUIRE(h && out, "null argument");
+    std::scoped_lock lk(h->r.read_mutex);
     const int64_t need = h->r.last_n_px * 4;
     REQUIRE(n >= need && need > 0, "buffer too small or nothing rendered");
     FVC_HIP(hipMemcpyAsync(out, h->r.results.p, need * sizeof(float), hipMemcpyDeviceToDevice, h->r.stream()));
@@ -724,9 +734,28 @@ static CacheStore& pick(fvc_renderer* h, int which) {
   return h->r.caches[which == 0 ? h->r.render_idx : h->r.fill_idx];
 }
 
+// The lock that keeps cache `which` still while it is read: renders hold
+// read_mutex and never change the front cache; fills write the fill cache
+// under write_mutex (a background fill on its own thread); a publish swaps
+// the two under both.  A background fill is marked running before its
+// thread takes write_mutex, so a reader of the fill cache that gets the lock
+// first lets it go and waits for the fill: the contents it reads are the
+// finished fill's, as for a reader that comes later.
+static std::unique_lock<std::mutex> cache_lock(fvc_renderer* h, int which) {
+  REQUIRE(which == 0 || which == 1, "which must be 0 or 1");
+  if (which == 0) return std::unique_lock<std::mutex>(h->r.read_mutex);
+  for (;;) {
+    std::unique_lock<std::mutex> lk(h->r.write_mutex);
+    if (h->r.fill_state.load() != FILL_RUNNING) return lk;
+    lk.unlock();
+    std::this_thread::yield();
+  }
+}
+
 int fvc_cache_get_info(fvc_renderer* h, int which, fvc_cache_info* info) {
   return guard([&] {
     REQUIRE(h && info, "null argument");
+    const auto lk = cache_lock(h, which);
     const CacheStore& C = pick(h, which);
     info->frustum_x = C.fx; info->frustum_y = C.fy; info->frustum_z = C.fz;
     info->bricks_x = C.bx; info->bricks_y = C.by; info->bricks_z = C.bz;
@@ -741,6 +770,7 @@ int fvc_cache_get_info(fvc_renderer* h, int which, fvc_cache_info* info) {
 int fvc_cache_get_brick_table(fvc_renderer* h, int which, int32_t* out, int64_t n) {
   return guard([&] {
     REQUIRE(h && out, "null argument");
+    const auto lk = cache_lock(h, which);
     const CacheStore& C = pick(h, which);
     if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
     const int64_t need = (int64_t)C.bx * C.by * C.bz;
@@ -752,6 +782,7 @@ int fvc_cache_get_brick_table(fvc_renderer* h, int which, int32_t* out, int64_t 
 int fvc_cache_get_known_ranges(fvc_renderer* h, int which, float* out, int64_t n) {
   return guard([&] {
     REQUIRE(h && out, "null argument");
+    const auto lk = cache_lock(h, which);
     const CacheStore& C = pick(h, which);
     if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
     const int64_t need = (int64_t)C.fx * C.fy;
@@ -763,6 +794,7 @@ int fvc_cache_get_known_ranges(fvc_renderer* h, int which, float* out, int64_t n
 int fvc_cache_get_pool(fvc_renderer* h, int which, void* out, int64_t n_bytes) {
   return guard([&] {
     REQUIRE(h && out, "null argument");
+    const auto lk = cache_lock(h, which);
     const CacheStore& C = pick(h, which);
     if (!C.initialized) throw Error(FVC_E_NOT_INITIALIZED, "Cache is not initialized!");
     const size_t rec = C.lat8 ? sizeof(Voxel8) : sizeof(Voxel);
@@ -783,6 +815,7 @@ int fvc_cache_get_pool(fvc_renderer* h, int which, void* out, int64_t n_bytes) {
 int fvc_profile_enable(fvc_renderer* h, int enable) {
   return guard([&] {
     REQUIRE(h, "null renderer");
+    std::scoped_lock lk(h->r.read_mutex);
     h->r.prof.table.clear();
     h->r.prof.enabled = enable != 0;
   });
@@ -791,6 +824,7 @@ int fvc_profile_enable(fvc_renderer* h, int enable) {
 int fvc_profile_get(fvc_renderer* h, fvc_kernel_stats* out, int32_t max_entries, int32_t* n_entries) {
   return guard([&] {
     REQUIRE(h && n_entries && (out || max_entries == 0), "null argument");
+    std::scoped_lock lk(h->r.read_mutex);
     int i = 0;
     for (const auto& kv : h->r.prof.table) {
       if (i < max_entries) {
@@ -813,6 +847,7 @@ int fvc_debug_march(fvc_renderer* h, const fvc_raymarch_info* rm, int32_t max_pe
   return guard([&] {
     REQUIRE(h && counts && max_per_ray >= 0, "invalid argument");
     check_rm(rm);
+    std::scoped_lock lk(h->r.read_mutex);
     debug_march(h->r, *rm, max_per_ray, counts, t0s);
   });
 }

@@ -511,6 +511,9 @@ int fvc_profile_get(fvc_renderer* renderer, fvc_kernel_stats* out, int32_t max_e
 /* Float result of the last render before the 8-bit write: rgb + transmittance
  * per pixel, float[H][W][4] (BaseRayResult, renderer_common.h:74-78). */
 int fvc_get_last_result(fvc_renderer* renderer, float* out_dev, int64_t n_floats);
+/* Cache getters: which 0 = the render (front) cache, 1 = the fill target.
+ * Reading the fill target while a background fill runs waits for that fill
+ * and returns its result; the render cache can be read meanwhile. */
 int fvc_cache_get_info(fvc_renderer* renderer, int which /*0 = render cache, 1 = fill target*/, fvc_cache_info* info);
 /* Indirection table int32[bz][by][bx]: pool slot or -1 (cache.h:31-33). */
 int fvc_cache_get_brick_table(fvc_renderer* renderer, int which, int32_t* out_host, int64_t n);

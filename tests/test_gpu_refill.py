@@ -83,3 +83,25 @@ def test_async_fill_needs_double_buffering(setup):
     with pytest.raises(lib.FvcError):
         r.initCacheAsync(path[0])
     r.close()
+
+
+def test_cache_getters_wait_for_a_running_fill(setup):
+    """The back cache's contents are read under the fill lock: a getter called
+    while a background fill runs returns the finished fill's table, equal to
+    a synchronous fill's; the front cache stays readable meanwhile."""
+    wl, model, path = setup
+    r = lib.Renderer(model, double_buffered=True)
+    r.initCacheBlockwise(path[0])
+    r.swapCache()
+    front = r.brick_table(0)
+    r.initCacheAsync(path[3])
+    assert np.array_equal(r.brick_table(0), front)
+    back = r.brick_table(1)
+    state, stats = r.fill_poll(wait=True)
+    assert state == abi.FILL_DONE
+    assert r.cache_info(1).n_bricks_used == stats.n_bricks_set
+    ref = lib.Renderer(model)
+    ref.initCacheBlockwise(path[3])
+    assert np.array_equal(back, ref.brick_table(0))
+    ref.close()
+    r.close()
