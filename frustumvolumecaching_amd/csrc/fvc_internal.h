@@ -66,6 +66,7 @@ struct GridLevel {
   uint32_t res2;     // res * res
   uint32_t size;     // entries in the level
   uint32_t offset;   // first entry of the level
+  uint32_t first;    // the value of that entry (set by the kernels' level copy)
 };
 struct GridParams {
   GridLevel lv[MAX_LEVELS];

@@ -90,13 +90,59 @@ bool net_layout(const fvc_net_desc& d, GridParams& g, int64_t& n_pos, int64_t& n
 }
 
 // ------------------------------------------------------------------ encode
+// Entries of corners x and x + 1 of one (y, z) row of a level, with one
+// 8-byte load (dword aligned) for both where the table allows.  ky, kz: the
+// row's y / z index terms (products for dense levels, hash terms otherwise).
+//  * dense: entries a and a + 1 (a = the row's x index reduced to the
+//    level); only when a is the level's last entry does a + 1 wrap to entry
+//    0: then the pair (a - 1, a) is read and corner x + 1 takes L.first.
+//  * hashed (the level's size is a power of two): for even x, x and x + 1
+//    differ in bit 0 only, so their entries i0 and i0 ^ 1 form the aligned
+//    pair holding i0; for odd x corner x + 1 is a load of its own, made by
+//    those lanes only.
+// No branch waits on a load, so every gather of a level is in flight before
+// the first wait.
+__device__ __forceinline__ void row_pair(const uint32_t* __restrict__ tl, const GridLevel& L, bool dense, uint32_t gx,
+                                         uint32_t ky, uint32_t kz, uint32_t& e0, uint32_t& e1) {
+  uint32_t lo, i1 = 0u;
+  bool sel, use_first = false, use_extra = false;
+  if (dense) {
+    const uint32_t t = gx + ky + kz;  // t + 1 < 2 * size
+    const uint32_t a = min(t, t - L.size);
+    const bool pair = a + 1u < L.size;
+    lo = pair ? a : a - 1u;
+    sel = !pair;
+    use_first = !pair;
+  } else {
+    const uint32_t k = ky ^ kz;
+    const uint32_t i0 = (gx ^ k) & (L.size - 1u);
+    lo = i0 & ~1u;
+    sel = i0 & 1u;
+    use_extra = gx & 1u;
+    i1 = ((gx + 1u) ^ k) & (L.size - 1u);
+  }
+  uint2 v;
+  __builtin_memcpy(&v, tl + lo, sizeof(v));
+  uint32_t extra = 0u;
+  if (use_extra) extra = tl[i1];
+  e0 = sel ? v.y : v.x;
+  e1 = use_first ? L.first : (use_extra ? extra : (sel ? v.x : v.y));
+}
+
 // Features of levels g, g+4, g+8, g+12 at p (clamped to [0,1]^3) -> f[8]
 // (f[2j], f[2j+1] = level 4j+g).  With this assignment the four lane groups
 // of a wave work on levels 4j..4j+3 together, which are either all dense,
 // all hashed, or (one quad at most) mixed: the first two take wave-uniform
 // branches.  Per level the y / z index terms of both corners are formed once
-// (24-bit products for dense levels, which stay below 2^24); each corner then
-// costs one 3-input add or xor and the reduction to the level's entries.
+// (24-bit products for dense levels, which stay below 2^24); each row of
+// corners x, x + 1 then costs one add or xor, the reduction to the level's
+// entries and (mostly) one 8-byte load (row_pair).
+//
+// HASHED_PAIRS: hashed levels read their rows with row_pair as well (the
+// render kernels); otherwise they, and the mixed quad, read every corner
+// with a load of its own (the fill's latents kernel, whose spatially
+// coherent samples run faster that way).
+template <bool HASHED_PAIRS>
 __device__ __forceinline__ void encode_quads(const GridLevel* __restrict__ lv, const GridParams& gp,
                                             const uint32_t* __restrict__ table, V3 p, int g, float* f) {
   const float px = fminf_(fmaxf_(p.x, 0.0f), 1.0f);
@@ -113,35 +159,29 @@ __device__ __forceinline__ void encode_quads(const GridLevel* __restrict__ lv, c
     const uint32_t gx = (uint32_t)(int)fx, gy = (uint32_t)(int)fy, gz = (uint32_t)(int)fz;
     wx[j] = ax - fx; wy[j] = ay - fy; wz[j] = az - fz;
     const uint32_t* tl = table + L.offset;
-    uint32_t idx[8];
-    if ((gp.quad_dense >> j) & 1u) {
-      const uint32_t y0 = __umul24(gy, L.res), z0 = __umul24(gz, L.res2);
-      const uint32_t yy[2] = {y0, y0 + L.res}, zz[2] = {z0, z0 + L.res2};
-#pragma unroll
-      for (int c = 0; c < 8; c++) {
-        const uint32_t t = gx + (c & 1) + yy[(c >> 1) & 1] + zz[c >> 2];
-        idx[c] = min(t, t - L.size);  // t < 2 * size
-      }
-    } else if ((gp.quad_hash >> j) & 1u) {
-      const uint32_t y0 = gy * 2654435761u, z0 = gz * 805459861u;
-      const uint32_t yy[2] = {y0, y0 + 2654435761u}, zz[2] = {z0, z0 + 805459861u};
-#pragma unroll
-      for (int c = 0; c < 8; c++) idx[c] = ((gx + (c & 1)) ^ yy[(c >> 1) & 1] ^ zz[c >> 2]) & (L.size - 1u);
-    } else {
-      const bool dense = (gp.dense_mask >> l) & 1u;
+    // the four (y, z) rows of the cell, corners x and x + 1 each
+    auto rows = [&](bool dense, bool pairs) {
       const uint32_t y0 = dense ? __umul24(gy, L.res) : gy * 2654435761u;
       const uint32_t z0 = dense ? __umul24(gz, L.res2) : gz * 805459861u;
       const uint32_t yy[2] = {y0, y0 + (dense ? L.res : 2654435761u)};
       const uint32_t zz[2] = {z0, z0 + (dense ? L.res2 : 805459861u)};
 #pragma unroll
-      for (int c = 0; c < 8; c++) {
-        const uint32_t x = gx + (c & 1), y = yy[(c >> 1) & 1], z = zz[c >> 2];
-        const uint32_t ta = x + y + z, th = (x ^ y ^ z) & (L.size - 1u);
-        idx[c] = dense ? min(ta, ta - L.size) : th;
-      }
-    }
+      for (int c = 0; c < 8; c += 2) {
+        const uint32_t ky = yy[(c >> 1) & 1], kz = zz[c >> 2];
+        if (pairs) {
+          row_pair(tl, L, dense, gx, ky, kz, ent[j][c], ent[j][c + 1]);
+        } else {
 #pragma unroll
-    for (int c = 0; c < 8; c++) ent[j][c] = tl[idx[c]];
+          for (int o = 0; o < 2; o++) {
+            const uint32_t x = gx + o, ta = x + ky + kz, th = (x ^ ky ^ kz) & (L.size - 1u);
+            ent[j][c + o] = tl[dense ? min(ta, ta - L.size) : th];  // ta < 2 * size
+          }
+        }
+      }
+    };
+    if ((gp.quad_dense >> j) & 1u) rows(true, true);
+    else if ((gp.quad_hash >> j) & 1u) rows(false, HASHED_PAIRS);
+    else rows((gp.dense_mask >> l) & 1u, HASHED_PAIRS);
   }
 #pragma unroll
   for (int j = 0; j < 4; j++) {
@@ -162,8 +202,14 @@ __device__ __forceinline__ void encode_quads(const GridLevel* __restrict__ lv, c
   }
 }
 
-__device__ __forceinline__ void load_levels(const GridParams& gp, GridLevel* lv) {
-  if (threadIdx.x < MAX_LEVELS) lv[threadIdx.x] = gp.lv[threadIdx.x];
+// the block's copy of the level table, with each level's entry 0
+__device__ __forceinline__ void load_levels(const GridParams& gp, const uint32_t* __restrict__ table,
+                                            GridLevel* lv) {
+  if (threadIdx.x < MAX_LEVELS) {
+    GridLevel L = gp.lv[threadIdx.x];
+    if ((int)threadIdx.x < gp.n_levels) L.first = table[L.offset];
+    lv[threadIdx.x] = L;
+  }
   __syncthreads();
 }
 
@@ -196,7 +242,7 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
   if ((int64_t)blockIdx.x * (NET_BLOCK / 64) * 16 * TILES >= n) return;
   __shared__ GridLevel lv[MAX_LEVELS];
   __shared__ typename Mx<P>::T wl[N_NET_FRAGS * 64];
-  load_levels(gp, lv);
+  load_levels(gp, table, lv);
   stage_frags(frags, wl, N_NET_FRAGS);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -229,7 +275,7 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
-      encode_quads(lv, gp, table, pv[t], g, f[t]);
+      encode_quads<HEAD>(lv, gp, table, pv[t], g, f[t]);
       // pinned here (after the gathers are issued): otherwise the load sinks
       // to its use after the density MLP and waits there on its own
       if constexpr (HEAD && PIX) {
@@ -315,7 +361,7 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network_vd(const float* __restric
   if ((int64_t)blockIdx.x * (NET_BLOCK / 64) * 16 * TILES >= n) return;
   __shared__ GridLevel lv[MAX_LEVELS];
   __shared__ h8 wl[N_VD_NET_FRAGS * 64];
-  load_levels(gp, lv);
+  load_levels(gp, table, lv);
   stage_frags(frags, wl, FULL ? N_VD_NET_FRAGS : VD_HA);
   __syncthreads();
   const MlpScales sc{{1.0f, 1.0f, 1.0f, 1.0f, 1.0f}, 1.0f};
@@ -347,7 +393,7 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network_vd(const float* __restric
     }
 #pragma unroll
     for (int t = 0; t < TILES; t++) {
-      encode_quads(lv, gp, table, pv[t], g, f[t]);
+      encode_quads<FULL>(lv, gp, table, pv[t], g, f[t]);
       // pinned here (after the gathers are issued), as in k_network
       if constexpr (PIX) {
         shq[t] = reinterpret_cast<const uint2*>(dir)[(int64_t)px[t] * 4 + g];
@@ -429,13 +475,13 @@ __global__ __launch_bounds__(NET_BLOCK) void k_head_vd(const uint16_t* __restric
 __global__ void k_encode(const float* __restrict__ pos, int64_t n, const uint32_t* __restrict__ table, GridParams gp,
                          uint16_t* __restrict__ out32) {
   __shared__ GridLevel lv[MAX_LEVELS];
-  load_levels(gp, lv);
+  load_levels(gp, table, lv);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const V3 p = {pos[3 * s], pos[3 * s + 1], pos[3 * s + 2]};
   for (int q = 0; q < 4; q++) {
     float f[8];
-    encode_quads(lv, gp, table, p, q, f);
+    encode_quads<true>(lv, gp, table, p, q, f);
     for (int j = 0; j < 4; j++)
       for (int k = 0; k < 2; k++)
         out32[s * 32 + 2 * (4 * j + q) + k] = __builtin_bit_cast(uint16_t, (_Float16)f[2 * j + k]);

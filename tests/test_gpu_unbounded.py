@@ -32,6 +32,23 @@ def _result(r, rm):
     return res.cpu().numpy().reshape(H, W, 4)
 
 
+def test_hashgrid_encoding_bit_exact(setup, oracle_mod):
+    """The T=2^21 grid's encoding at random points and on the faces, edges and
+    corners of the unit cube, where the dense levels' corner x + 1 wraps to
+    the level's first entry."""
+    from .gpu_util import dev, empty, host_f16
+    wl, model, r, path, om = setup
+    rng = np.random.default_rng(5)
+    p = rng.random((8192, 3)).astype(np.float32)
+    edge = np.array([0.0, 1e-7, 0.5, 0.9999999, 1.0], np.float32)
+    g = np.stack(np.meshgrid(edge, edge, edge, indexing="ij"), -1).reshape(-1, 3)
+    p[:len(g)] = g
+    out = empty((len(p), 32), torch.int16)
+    model.encode(dev(p), len(p), out)
+    torch.cuda.synchronize()
+    assert np.array_equal(host_f16(out), om.encode(p, oracle_mod.EMU16))
+
+
 def test_full_render_matches_oracle(setup, oracle_mod):
     wl, model, r, path, om = setup
     for k in (0, 7):
