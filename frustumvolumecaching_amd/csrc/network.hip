@@ -194,8 +194,10 @@ __device__ __forceinline__ void encode_quads(const GridLevel* __restrict__ lv, c
       const uint32_t e = ent[j][c];
       const float v0 = (float)__builtin_bit_cast(_Float16, (uint16_t)(e & 0xffffu));
       const float v1 = (float)__builtin_bit_cast(_Float16, (uint16_t)(e >> 16));
-      a0 = a0 + w * v0;
-      a1 = a1 + w * v1;
+      // fused: one v_fma_mix_f32 per corner and feature (the half operand
+      // is read directly)
+      a0 = __builtin_fmaf(v0, w, a0);
+      a1 = __builtin_fmaf(v1, w, a1);
     }
     f[2 * j] = a0;
     f[2 * j + 1] = a1;

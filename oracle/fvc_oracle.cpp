@@ -449,8 +449,10 @@ template <class R> void encode(const Net& n, V3<R> p, Mode m, R* out) {
       }
       const uint32_t idx = grid_index(n, l, q[0], q[1], q[2]);
       const float* e = &n.table[((size_t)n.offset[l] + idx) * 2];
-      acc[0] += w * (R)e[0];
-      acc[1] += w * (R)e[1];
+      // fused multiply-add per corner, as the product's encoder does (tcnn
+      // itself is compiled with contraction on; its rounding is not pinned)
+      acc[0] = std::fma(w, (R)e[0], acc[0]);
+      acc[1] = std::fma(w, (R)e[1], acc[1]);
     }
     out[2 * l + 0] = q16(acc[0], m);
     out[2 * l + 1] = q16(acc[1], m);
