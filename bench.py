@@ -48,6 +48,8 @@ def parse():
                     help="MLP rows per wavefront (with --batch / --max-steps: fixed shapes, no autotuning)")
     ap.add_argument("--no-refill", action="store_true", help="skip the background-refill leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-view batch leg")
+    ap.add_argument("--no-latent-leg", action="store_true",
+                    help="skip the cached path with FP8 cache latents (run when --cache-latents is fp16)")
     ap.add_argument("--no-graph", action="store_true", help="launch frames kernel by kernel (no hipGraph replay)")
     ap.add_argument("--cache-latents", choices=["fp16", "fp8"], default="fp16",
                     help="latent storage of the brick pool (fp8: E4M3 with a per-brick scale)")
@@ -256,6 +258,9 @@ def main():
     full_ms_prof, _, prof_full = full_path(True)
     r.set_graph(False)
     async_fps = None if a.no_refill else async_refill_leg(a, wl, model, rm_of, img)
+    fp8_latents = None
+    if not a.no_latent_leg and a.cache_latents == "fp16" and not a.viewdep:
+        fp8_latents = fp8_latent_leg(a, model, r, rm_of, img, res_buf, shapes["cached"])
     batching = None if a.no_batch else batch_leg(wl, r, path, use_shapes)
 
     # ---- PSNR: cached vs full (GPU) on frames 1, 5, 9 after a fill at frame 0
@@ -400,6 +405,7 @@ def main():
         "kernels_full": kernel_table(prof_full),
         "cpu_baseline": cpu,
         "precision": precision,
+        "fp8_latents": fp8_latents,
         "async_refill_fps": None if async_fps is None else async_fps["fps"],
         "async_refill": async_fps,
         "sync_refill_frame_ms_max": round(max(c + (fill_ms[k // 10] if k % 10 == 0 and k // 10 < len(fill_ms) else 0.0)
@@ -557,6 +563,62 @@ def async_refill_leg(a, wl, model, rm_of, img):
     out = {"fps": round(fps, 3), "frame_ms_median": round(sorted(frame_ms)[len(frame_ms) // 2], 3),
            "frame_ms_max": round(max(frame_ms), 3)}
     log(f"[bench] background refill: {out}; frame ms {[round(x, 1) for x in frame_ms]}; {events}")
+    return out
+
+
+def fp8_latent_leg(a, model, r, rm_of, img, res_buf, shape):
+    """The cached path with the brick pool's latents stored as FP8-E4M3 with a
+    per-brick scale (the reference stores half): frames/s over the same
+    frames and fills as the headline, and the 8-bit PSNR of frames 1, 5, 9
+    against the FP16-latent renderer r (both filled at frame 0).  Image
+    quality against the FP64 oracle per latent format is in the precision
+    report."""
+    import numpy as np
+    import torch
+    from frustumvolumecaching_amd import lib
+    r8 = lib.Renderer(model)
+    r8.set_latent_format("fp8")
+    r8.set_graph(not a.no_graph)
+    if shape:
+        r8.set_mlp_rows(shape[0])
+        r8.set_batching(shape[1], shape[2])
+    elif a.batch or a.max_steps:
+        r8.set_batching(a.batch or (1 << 25), a.max_steps or 32)
+    if a.mlp_rows:
+        r8.set_mlp_rows(a.mlp_rows)
+    r8.initCacheBlockwise(rm_of(0))
+    r8.render_cached(rm_of(1), img)
+    torch.cuda.synchronize()
+    frame_ms, fill_ms = [], []
+    for k in range(a.steps):
+        if k % 10 == 0:
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r8.initCacheBlockwise(rm_of(k))
+            torch.cuda.synchronize()
+            fill_ms.append((time.perf_counter() - t) * 1e3)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r8.render_cached(rm_of(k), img)
+        torch.cuda.synchronize()
+        frame_ms.append((time.perf_counter() - t) * 1e3)
+    W, H = rm_of(0).cam_info.res_x, rm_of(0).cam_info.res_y
+    r.set_graph(False)
+    r8.set_graph(False)
+    r.initCacheBlockwise(rm_of(0))
+    r8.initCacheBlockwise(rm_of(0))
+    psnr = []
+    for k in (1, 5, 9):
+        imgs = []
+        for rr in (r, r8):
+            rr.render_cached(rm_of(k))
+            rr.last_result(res_buf, W * H * 4)
+            imgs.append(to_8bit(res_buf.cpu().numpy().reshape(H, W, 4)))
+        psnr.append(db(psnr_8bit(imgs[1], imgs[0])))
+    r8.close()
+    out = {"cached_fps": round(1e3 / float(np.mean(frame_ms)), 3), "fill_ms": round(float(np.mean(fill_ms)), 3),
+           "psnr_vs_fp16_latents_db": psnr}
+    log(f"[bench] fp8 cache latents: {out}")
     return out
 
 
