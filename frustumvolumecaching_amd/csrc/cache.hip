@@ -38,7 +38,8 @@ struct FillTile {
 // ------------------------------------------------------------------ fill
 // initPayloads_cache_init_kernel (cache_renderer.cu:18-81)
 __global__ __launch_bounds__(64) void k_fill_init(fvc_raymarch_info rm, V3 aabb_from, V3 aabb_to, int bx,
-                                                  FillRay* __restrict__ rays, FillTile* __restrict__ tiles) {
+                                                  FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
+                                                  uint8_t* __restrict__ alive8) {
   const int t = blockIdx.x, lane = threadIdx.x;
   const int px = (t % bx) * BRICK + (lane & 7), py = (t / bx) * BRICK + (lane >> 3);
   const fvc_stepsize_info& s = rm.stepsize_info;
@@ -62,6 +63,7 @@ __global__ __launch_bounds__(64) void k_fill_init(fvc_raymarch_info rm, V3 aabb_
     ft.slab = mn / BRICK;
     ft.alive = n_alive > 0;
     tiles[t] = ft;
+    alive8[t] = (uint8_t)ft.alive;
   }
 }
 
@@ -78,7 +80,8 @@ __global__ void k_set_ints(int* __restrict__ p, int64_t n, int v) {
 
 // alive tiles in tile order (deterministic compaction); one block, each
 // thread takes a run of consecutive tiles (flags in a bit mask), one block
-// prefix count over the runs
+// prefix count over the runs.  The flags are the tiles' alive bytes (alive8,
+// kept beside the tile records), read coalesced.
 static const int SCAN_RUN = 32;  // tiles per thread: 32768 tiles per pass of 1024 threads
 __device__ __forceinline__ int block_excl_sum(int v, int& total) {
   __shared__ int s_w[16];
@@ -98,7 +101,7 @@ __device__ __forceinline__ int block_excl_sum(int v, int& total) {
   return off + pre;
 }
 
-__global__ __launch_bounds__(1024) void k_fill_list(int n_tiles, const FillTile* __restrict__ tiles,
+__global__ __launch_bounds__(1024) void k_fill_list(int n_tiles, const uint8_t* __restrict__ alive8,
                                                     int* __restrict__ list, int* __restrict__ cnt, int it) {
   __shared__ uint8_t flags[1024 * SCAN_RUN];
   int base = 0;
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(1024) void k_fill_list(int n_tiles, const FillTile*
 #pragma unroll
     for (int k = 0; k < SCAN_RUN; k++) {
       const int i = c0 + k * 1024 + (int)threadIdx.x;
-      flags[k * 1024 + threadIdx.x] = i < n_tiles ? (uint8_t)(tiles[i].alive != 0) : (uint8_t)0;
+      flags[k * 1024 + threadIdx.x] = i < n_tiles ? alive8[i] : (uint8_t)0;
     }
     __syncthreads();
     uint32_t bits = 0;
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list
                                                     FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
                                                     float* __restrict__ f_pos, float* __restrict__ f_t0,
                                                     int* __restrict__ f_meta, int* __restrict__ counter,
-                                                    float* __restrict__ f_dir) {
+                                                    float* __restrict__ f_dir, uint8_t* __restrict__ alive8) {
   const int n_list = cnt[CNT_TILES];
   if ((int)blockIdx.x >= n_list) return;
   __shared__ uint32_t coarse[16 * COARSE_WORDS];
@@ -235,6 +238,7 @@ __global__ __launch_bounds__(64) void k_fill_sample(const int* __restrict__ list
       tile.processed = processed;
       tile.visible = 0;
       tiles[t] = tile;
+      alive8[t] = (uint8_t)tile_alive;
     }
   }
 }
@@ -254,13 +258,16 @@ __global__ __launch_bounds__(64) void k_fill_accum(const int* __restrict__ list,
                                                    fvc_stepsize_info st, float alpha_thre,
                                                    const uint2* __restrict__ out16, const float* __restrict__ f_t0,
                                                    FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
-                                                   int sig_row) {
+                                                   int sig_row, uint8_t* __restrict__ alloc8) {
   const int n_list = cnt[CNT_TILES];
   if (blockIdx.x == 0 && threadIdx.x == 0 && it <= FILL_MAX_ITERS) cnt[FILL_LOG + 2 * it + 1] = cnt[CNT_SAMPLES];
   const int lane = threadIdx.x;
   for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
     const int t = list[li];
-    if (!tiles[t].processed) continue;
+    if (!tiles[t].processed) {
+      if (lane == 0) alloc8[li] = 0;
+      continue;
+    }
     FillRay r = rays[(int64_t)t * 64 + lane];
     int vis = 0;
     // a lane has at most 8 samples (one slab): all loaded before the
@@ -291,14 +298,18 @@ __global__ __launch_bounds__(64) void k_fill_accum(const int* __restrict__ list,
     rays[(int64_t)t * 64 + lane].T = r.T;
     rays[(int64_t)t * 64 + lane].alive = r.alive;
     const int v = wave_sum(vis);
-    if (lane == 0) tiles[t].visible = v;
+    if (lane == 0) {
+      tiles[t].visible = v;
+      alloc8[li] = (uint8_t)(v > 0);  // the tile takes a brick this sub-iteration
+    }
   }
 }
 
 // brick slot allocation in list (= tile) order; one block, runs of
 // consecutive list entries per thread, one block prefix count per pass
 __global__ __launch_bounds__(1024) void k_fill_alloc(const int* __restrict__ list, int* __restrict__ cnt,
-                                                     int capacity, FillTile* __restrict__ tiles) {
+                                                     int capacity, FillTile* __restrict__ tiles,
+                                                     const uint8_t* __restrict__ alloc8) {
   __shared__ uint8_t flags[1024 * SCAN_RUN];
   const int n = cnt[CNT_TILES];
   int base = cnt[CNT_SLOTS];
@@ -307,12 +318,8 @@ __global__ __launch_bounds__(1024) void k_fill_alloc(const int* __restrict__ lis
 #pragma unroll
     for (int k = 0; k < SCAN_RUN; k++) {
       const int i = c0 + k * 1024 + (int)threadIdx.x;
-      uint8_t f = 0;
-      if (i < n) {
-        const FillTile& ft = tiles[list[i]];
-        f = ft.processed && ft.visible > 0;
-      }
-      flags[k * 1024 + threadIdx.x] = f;
+      // processed with visible samples (k_fill_accum's byte per list entry)
+      flags[k * 1024 + threadIdx.x] = i < n ? alloc8[i] : (uint8_t)0;
     }
     __syncthreads();
     uint32_t bits = 0;
@@ -352,7 +359,8 @@ __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list,
                                                    const int* __restrict__ f_meta, const FillRay* __restrict__ rays,
                                                    FillTile* __restrict__ tiles, int* __restrict__ table,
                                                    Voxel* __restrict__ pool, Voxel8* __restrict__ pool8,
-                                                   float* __restrict__ scales, int sig_row, int di) {
+                                                   float* __restrict__ scales, int sig_row, int di,
+                                                   uint8_t* __restrict__ alive8) {
   __shared__ uint4 brick_lds[BRICK_VOXELS * sizeof(Voxel) / 16];
   const int n_list = cnt[CNT_TILES];
   const int lane = threadIdx.x;
@@ -477,6 +485,7 @@ __global__ __launch_bounds__(64) void k_fill_write(const int* __restrict__ list,
       if (!overflow) tile.slab++;
       tile.alive = !overflow && n_alive > 0;
       tiles[t] = tile;
+      alive8[t] = (uint8_t)tile.alive;
     }
   }
 }
@@ -586,6 +595,8 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
   r.f_rays.ensure((int64_t)n_tiles * 64 * sizeof(FillRay));
   r.f_tiles.ensure((int64_t)n_tiles * sizeof(FillTile));
   r.f_list.ensure((int64_t)n_tiles * sizeof(int));
+  r.f_alive8.ensure(n_tiles);
+  r.f_alloc8.ensure(n_tiles);
   r.f_pos.ensure(f_cap * 3 * sizeof(float));
   r.f_t0.ensure(f_cap * sizeof(float));
   r.f_meta.ensure(f_cap * sizeof(int));
@@ -606,7 +617,8 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
                        n_table, -1);
     hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(FILL_LOG), 0, s, cnt, (int64_t)FILL_LOG, 0);
     hipLaunchKernelGGL(k_fill_init, dim3(n_tiles), dim3(64), 0, s, rm, cvt(m.scene.aabb_from),
-                       cvt(m.scene.aabb_to), C.bx, r.f_rays.as<FillRay>(), r.f_tiles.as<FillTile>());
+                       cvt(m.scene.aabb_to), C.bx, r.f_rays.as<FillRay>(), r.f_tiles.as<FillTile>(),
+                       r.f_alive8.as<uint8_t>());
     FVC_HIP(hipGetLastError());
   };
   struct Handles { int list, sample, net, accum, alloc, write; };
@@ -614,15 +626,15 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
   auto iteration = [&](int it) {
     Handles h;
     hipEvent_t ev = P.begin(s);
-    hipLaunchKernelGGL(k_fill_list, dim3(1), dim3(1024), 0, s, n_tiles, r.f_tiles.as<FillTile>(), r.f_list.as<int>(),
-                       cnt, it);
+    hipLaunchKernelGGL(k_fill_list, dim3(1), dim3(1024), 0, s, n_tiles, r.f_alive8.as<uint8_t>(),
+                       r.f_list.as<int>(), cnt, it);
     FVC_HIP(hipGetLastError());
     h.list = P.end(ev, s, "fill_list");
     ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_sample, dim3(grid_t), dim3(64), 0, s, r.f_list.as<int>(), cnt, rm.stepsize_info, m.occ,
                        m.con, m.bitfield.as<uint64_t>(), m.coarse.as<uint32_t>(), C.bz, r.f_rays.as<FillRay>(),
                        r.f_tiles.as<FillTile>(), r.f_pos.as<float>(), r.f_t0.as<float>(), r.f_meta.as<int>(),
-                       cnt + CNT_SAMPLES, split ? r.f_dir.as<float>() : nullptr);
+                       cnt + CNT_SAMPLES, split ? r.f_dir.as<float>() : nullptr, r.f_alive8.as<uint8_t>());
     FVC_HIP(hipGetLastError());
     h.sample = P.end(ev, s, "fill_sample");
     ev = P.begin(s);
@@ -636,19 +648,20 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
     ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_accum, dim3(grid_t), dim3(64), 0, s, r.f_list.as<int>(), cnt, it, rm.stepsize_info,
                        m.scene.alpha_thre, r.f_out16.as<uint2>(), r.f_t0.as<float>(), r.f_rays.as<FillRay>(),
-                       r.f_tiles.as<FillTile>(), sig_row);
+                       r.f_tiles.as<FillTile>(), sig_row, r.f_alloc8.as<uint8_t>());
     FVC_HIP(hipGetLastError());
     h.accum = P.end(ev, s, "fill_accum");
     ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_alloc, dim3(1), dim3(1024), 0, s, r.f_list.as<int>(), cnt, C.capacity,
-                       r.f_tiles.as<FillTile>());
+                       r.f_tiles.as<FillTile>(), r.f_alloc8.as<uint8_t>());
     FVC_HIP(hipGetLastError());
     h.alloc = P.end(ev, s, "fill_alloc");
     ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_write, dim3(grid_t), dim3(64), 0, s, r.f_list.as<int>(), cnt, C.bx, C.by, lat_row,
                        r.f_out16.as<uint16_t>(), r.f_meta.as<int>(), r.f_rays.as<FillRay>(),
                        r.f_tiles.as<FillTile>(), C.table.as<int>(), C.lat8 ? nullptr : C.pool.as<Voxel>(),
-                       C.lat8 ? C.pool.as<Voxel8>() : nullptr, C.scales.as<float>(), sig_row, di);
+                       C.lat8 ? C.pool.as<Voxel8>() : nullptr, C.scales.as<float>(), sig_row, di,
+                       r.f_alive8.as<uint8_t>());
     FVC_HIP(hipGetLastError());
     h.write = P.end(ev, s, "fill_write");
     hs.push_back(h);

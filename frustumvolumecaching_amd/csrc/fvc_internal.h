@@ -418,6 +418,7 @@ struct Renderer {
   // fill scratch (own counters and pinned mirror: the fill may run on the
   // fill stream from another thread)
   DevBuf f_rays, f_tiles, f_list, f_pos, f_t0, f_meta, f_out16, f_counters;
+  DevBuf f_alive8, f_alloc8;  // fill: alive byte per tile, brick-request byte per list entry
   DevBuf f_dir;  // fill sample directions (split models: the initial view direction)
   int* f_pinned = nullptr;
   // background refill (double-buffered renderers)
