@@ -252,13 +252,15 @@ __device__ __forceinline__ float out16_row(const uint2* out16, int64_t q, int r)
 
 // first loop of accumulateNew_cache_init_second_kernel (cache_renderer.cu:358-405):
 // transmittance over the slab's samples and the count of visible samples.
-// sig_row: row of log sigma in the network output (0; split models 8).
+// sig_row: row of log sigma in the network output (0; split models 8);
+// lsig (plain models): that row as a compact half array, read instead.
 // Logs the sub-iteration's sample count.
 __global__ __launch_bounds__(64) void k_fill_accum(const int* __restrict__ list, int* __restrict__ cnt, int it,
                                                    fvc_stepsize_info st, float alpha_thre,
                                                    const uint2* __restrict__ out16, const float* __restrict__ f_t0,
                                                    FillRay* __restrict__ rays, FillTile* __restrict__ tiles,
-                                                   int sig_row, uint8_t* __restrict__ alloc8) {
+                                                   int sig_row, uint8_t* __restrict__ alloc8,
+                                                   const uint16_t* __restrict__ lsig) {
   const int n_list = cnt[CNT_TILES];
   if (blockIdx.x == 0 && threadIdx.x == 0 && it <= FILL_MAX_ITERS) cnt[FILL_LOG + 2 * it + 1] = cnt[CNT_SAMPLES];
   const int lane = threadIdx.x;
@@ -277,7 +279,7 @@ __global__ __launch_bounds__(64) void k_fill_accum(const int* __restrict__ list,
     for (int k = 0; k < BRICK; k++) {
       // loads from a valid index (0 past the lane's samples)
       const int64_t q = k < r.cnt ? (int64_t)r.off + k : 0;
-      lsv[k] = out16_row(out16, q, sig_row);
+      lsv[k] = lsig ? h2f(lsig[q]) : out16_row(out16, q, sig_row);
       t0v[k] = f_t0[q];
     }
 #pragma unroll
@@ -601,6 +603,7 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
   r.f_t0.ensure(f_cap * sizeof(float));
   r.f_meta.ensure(f_cap * sizeof(int));
   r.f_out16.ensure(f_cap * 16 * sizeof(uint16_t));
+  r.f_lsig.ensure(f_cap * sizeof(uint16_t));
   const bool split = m.net.split_head != 0;
   if (split) r.f_dir.ensure(f_cap * 3 * sizeof(float));
   // rows of the network output: log sigma and the first stored latent
@@ -643,12 +646,13 @@ void cache_fill(Renderer& r, const fvc_raymarch_info& rm, fvc_init_stats* stats,
                                 r.mlp_rows, cnt + CNT_SAMPLES);
     else
       launch_network_latents(m, r.f_pos.as<float>(), f_cap, r.f_out16.as<uint16_t>(), s, r.mlp_rows,
-                             cnt + CNT_SAMPLES);
+                             cnt + CNT_SAMPLES, r.f_lsig.as<uint16_t>());
     h.net = P.end(ev, s, "network_latents");
     ev = P.begin(s);
     hipLaunchKernelGGL(k_fill_accum, dim3(grid_t), dim3(64), 0, s, r.f_list.as<int>(), cnt, it, rm.stepsize_info,
                        m.scene.alpha_thre, r.f_out16.as<uint2>(), r.f_t0.as<float>(), r.f_rays.as<FillRay>(),
-                       r.f_tiles.as<FillTile>(), sig_row, r.f_alloc8.as<uint8_t>());
+                       r.f_tiles.as<FillTile>(), sig_row, r.f_alloc8.as<uint8_t>(),
+                       split ? nullptr : r.f_lsig.as<uint16_t>());
     FVC_HIP(hipGetLastError());
     h.accum = P.end(ev, s, "fill_accum");
     ev = P.begin(s);

@@ -160,8 +160,9 @@ void launch_network_full_pix(const Model& m, const float4* pos4, const uint4* ra
                              hipStream_t s, const int* n_dev, int rows);
 void launch_network_full_ex(const Model& m, const float* pos, const float* dir, bool pix, int64_t n,
                             uint32_t* out_h4, hipStream_t s, const int* n_dev, int rows);
+// log_sigma (optional): row 0 of each sample once more, as a compact half array
 void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s,
-                            int rows = 16, const int* n_dev = nullptr);
+                            int rows = 16, const int* n_dev = nullptr, uint16_t* log_sigma = nullptr);
 void launch_network_head(const Model& m, const uint16_t* head_in16, const float* dir, int64_t n, uint32_t* out_h4,
                          hipStream_t s);
 void launch_network_encode(const Model& m, const float* pos, int64_t n, uint16_t* out32, hipStream_t s);
@@ -419,6 +420,7 @@ struct Renderer {
   // fill stream from another thread)
   DevBuf f_rays, f_tiles, f_list, f_pos, f_t0, f_meta, f_out16, f_counters;
   DevBuf f_alive8, f_alloc8;  // fill: alive byte per tile, brick-request byte per list entry
+  DevBuf f_lsig;              // fill: log sigma per sample (half), plain models
   DevBuf f_dir;  // fill sample directions (split models: the initial view direction)
   int* f_pinned = nullptr;
   // background refill (double-buffered renderers)

@@ -296,6 +296,10 @@ __global__ __launch_bounds__(NET_BLOCK) void k_network(const float* __restrict__
       const _Float16 rows[4] = {(_Float16)d3[0], (_Float16)d3[1], (_Float16)d3[2], (_Float16)d3[3]};
       if (!HEAD) {
         if (valid[t]) out16[s[t] * 4 + g] = pack_h4(rows[0], rows[1], rows[2], rows[3]);
+        // out_h4, when given: row 0 (log sigma) of each sample once more, as
+        // a compact half array (the fill's transmittance pass reads only it)
+        if (out_h4 && g == 0 && valid[t])
+          reinterpret_cast<uint16_t*>(out_h4)[s[t]] = __builtin_bit_cast(uint16_t, rows[0]);
       } else {
         typename Mx<P>::T hin;
         if constexpr (PIX) {
@@ -545,13 +549,13 @@ void launch_network_full_ex(const Model& m, const float* pos, const float* dir, 
   FVC_HIP(hipGetLastError());
 }
 void launch_network_latents(const Model& m, const float* pos, int64_t n, uint16_t* out16, hipStream_t s, int rows,
-                            const int* n_dev) {
+                            const int* n_dev, uint16_t* log_sigma) {
   if (m.net.split_head) throw Error(FVC_E_UNSUPPORTED, "split models need the initial view direction");
   if (n <= 0) return;
 #define L2(P, TL)                                                                                                 \
   hipLaunchKernelGGL((k_network<false, P, TL>), dim3(grid_for((n + 16 * TL - 1) / (16 * TL))), dim3(NET_BLOCK), 0, s, \
                      pos, (const float*)nullptr, n, m.table.as<uint32_t>(), m.frags.as<typename Mx<P>::T>(), m.grid, \
-                     m.mlp_scales, (uint2*)nullptr, (uint2*)out16, n_dev)
+                     m.mlp_scales, (uint2*)log_sigma, (uint2*)out16, n_dev)
 #define L(P)                       \
   if (rows == 64) { L2(P, 4); }    \
   else if (rows == 32) { L2(P, 2); } \
