@@ -1627,18 +1627,24 @@ void fill_async(Renderer& r, const fvc_raymarch_info& rm) {
   if (!r.fill_stream) FVC_HIP(hipStreamCreateWithFlags(&r.fill_stream, hipStreamNonBlocking));
   r.fill_state = FILL_RUNNING;
   r.fill_error.clear();
-  r.fill_thread = std::thread([&r, rm] {
-    try {
-      FVC_HIP(hipSetDevice(r.model->device));
-      std::scoped_lock lk(r.write_mutex);
-      cache_fill(r, rm, &r.fill_stats, r.fill_stream, r.fill_prof);
-      r.fill_pose = translation(rm.cam_info.cam2world);
-      r.fill_state = FILL_DONE;
-    } catch (const std::exception& e) {
-      r.fill_error = e.what();
-      r.fill_state = FILL_FAILED;
-    }
-  });
+  try {
+    r.fill_thread = std::thread([&r, rm] {
+      try {
+        FVC_HIP(hipSetDevice(r.model->device));
+        std::scoped_lock lk(r.write_mutex);
+        cache_fill(r, rm, &r.fill_stats, r.fill_stream, r.fill_prof);
+        r.fill_pose = translation(rm.cam_info.cam2world);
+        r.fill_state = FILL_DONE;
+      } catch (const std::exception& e) {
+        r.fill_error = e.what();
+        r.fill_state = FILL_FAILED;
+      }
+    });
+  } catch (...) {
+    // no thread: nothing runs, and readers of the fill cache must not wait
+    r.fill_state = FILL_IDLE;
+    throw;
+  }
 }
 
 // state of the background fill; with wait, blocks until it has finished
