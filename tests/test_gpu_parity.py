@@ -230,3 +230,23 @@ def test_cached_at_fill_pose_matches_full_render(renderer, filled, wl, oracle_mo
     torch.cuda.synchronize()
     res = res.cpu().numpy().reshape(RES, RES, 4)
     assert oracle_mod.psnr_8bit(to_8bit(res).astype(np.uint8), to_8bit(full).astype(np.uint8)) >= 35.0
+
+
+@pytest.mark.parametrize("capacity", [1, 37, 300])
+def test_fill_capacity_overflow_bit_exact(gmodel, omodel, wl, oracle_mod, capacity):
+    """A pool too small for the view: brick requests past the capacity stop
+    their columns (cache_renderer.cu:422-426, a stat, not an error); the
+    table, the known ranges, the used-brick count and the overflow flag
+    equal the oracle's fill with the same capacity."""
+    r = lib.Renderer(gmodel)
+    r.set_cache_capacity(capacity)
+    rm = wl.rm(None, RES, RES)
+    st = r.initCacheBlockwise(rm)
+    oc = omodel.cache_fill(rm, oracle_mod.EMU16, capacity=capacity)
+    assert st.overflow == 1 and oc.stats.overflow == 1
+    assert st.n_bricks_set == oc.stats.n_bricks_set <= capacity
+    assert r.cache_info().capacity_bricks == oc.info.capacity_bricks == capacity
+    assert np.array_equal(r.brick_table(), oc.brick_table())
+    assert np.array_equal(r.known_ranges().view(np.uint32), oc.known_ranges().view(np.uint32))
+    assert st.n_subiters == oc.stats.n_subiters
+    r.close()
