@@ -1,8 +1,8 @@
 """Edge cases of the render path against the oracle: views that miss the
 scene (no ray enters the box: an all-background image, no samples, an empty
 cache), resolutions that are not multiples of the 16x16 ray blocks or the
-8x8 brick tiles (1x1, 7x3, 17x9, 65x33), and the cached render of a view
-whose cache holds nothing."""
+8x8 brick tiles (1x1, 7x3, 17x9, 65x33), a reprojected view entirely
+outside the cache frustum, and refills at changing view sizes."""
 import numpy as np
 import pytest
 import torch
@@ -91,4 +91,44 @@ def test_ragged_resolutions(setup, oracle_mod, wh):
     ref2, nr, nc = oc.render(rm2)
     assert st2.n_resamples + st2.n_cache_hits >= nr + nc
     assert np.abs(to_8bit(res2) - to_8bit(ref2)).max() <= 2
+    r.close()
+
+
+def test_cached_render_outside_the_cache(setup, oracle_mod):
+    """After a 90 degree turn no sample falls inside the cache frustum: every
+    occupied sample is resampled, and the image equals the full render's."""
+    wl, model, om = setup
+    rm0 = wl.rm(None, 40, 32)
+    rm1 = wl.rm(S.yaw(wl.cam2world, 90.0), 40, 32)
+    r = lib.Renderer(model)
+    r.initCacheBlockwise(rm0)
+    img = _image(rm1)
+    st = r.render_cached(rm1, img)
+    res = _result(r, rm1)
+    assert st.n_cache_hits == 0
+    oc = om.cache_fill(rm0, oracle_mod.EMU16)
+    ref, nr, nc = oc.render(rm1)
+    assert nc == 0
+    assert np.abs(to_8bit(res) - to_8bit(ref)).max() <= 2
+    stf = r.render_full(rm1, img)
+    assert st.n_resamples == stf.n_resamples
+    assert np.array_equal(_result(r, rm1), res)
+    r.close()
+
+
+def test_refill_at_another_resolution(setup, oracle_mod):
+    """A cache filled at one view size and filled again at another: the
+    brick grid follows the new view and equals the oracle's fill of it."""
+    wl, model, om = setup
+    r = lib.Renderer(model)
+    for wh in ((48, 40), (65, 33), (16, 16)):
+        rm = wl.rm(None, *wh)
+        fs = r.initCacheBlockwise(rm)
+        oc = om.cache_fill(rm, oracle_mod.EMU16)
+        gi, oi = r.cache_info(), oc.info
+        assert (gi.frustum_x, gi.frustum_y, gi.bricks_x, gi.bricks_y) == (oi.frustum_x, oi.frustum_y, oi.bricks_x,
+                                                                          oi.bricks_y)
+        assert fs.n_bricks_set == oc.stats.n_bricks_set
+        assert np.array_equal(r.brick_table(), oc.brick_table())
+        assert np.array_equal(r.known_ranges().view(np.uint32), oc.known_ranges().view(np.uint32))
     r.close()
