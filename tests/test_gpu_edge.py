@@ -132,3 +132,23 @@ def test_refill_at_another_resolution(setup, oracle_mod):
         assert np.array_equal(r.brick_table(), oc.brick_table())
         assert np.array_equal(r.known_ranges().view(np.uint32), oc.known_ranges().view(np.uint32))
     r.close()
+
+
+def test_batches_with_degenerate_views(setup):
+    """A batch holding a 1x1 view, a view that misses the scene and a ragged
+    one: every view's image equals its single-view render, full and cached."""
+    wl, model, om = setup
+    views = [wl.rm(None, 1, 1), wl.rm(S.yaw(wl.cam2world, 180.0), 16, 16), wl.rm(S.yaw(wl.cam2world, 1.0), 65, 33)]
+    r = lib.Renderer(model)
+    r.initCacheBlockwise(wl.rm(None, 64, 64))
+    for kind in ("full", "cached"):
+        single = []
+        for rm in views:
+            img = _image(rm)
+            getattr(r, f"render_{kind}")(rm, img)
+            single.append(img.cpu().numpy())
+        imgs = [_image(rm) for rm in views]
+        getattr(r, f"render_{kind}_batch")(views, imgs)
+        for a, b in zip(single, imgs):
+            assert np.array_equal(a, b.cpu().numpy()), kind
+    r.close()
